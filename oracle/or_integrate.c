@@ -1,0 +1,335 @@
+/*
+ * Oracle: pulse integration over a hot-region mesh exploiting azimuthal
+ * invariance.  TEST INFRASTRUCTURE ONLY.
+ * Restates xpsi/cellmesh/integrator_for_azimuthal_invariance.pyx:70-665 with
+ * one thread (the reference's recommended setting, Likelihood.py:43-50),
+ * the blackbody atmosphere xpsi/surface_radiation_field/hot_BB.pyx:54-98 and
+ * the 4-D cubic Lagrange table hot_Num4D.pyx:66-460 (its base-node search
+ * keeps state from one call to the next, replayed here in call order).
+ * Image deflection: rays.pyx:53-57.  Disk blocking (R_in < 1e6) is not used.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#include "xpsi_oracle.h"
+
+static const double C_SI = 2.99792458e8;
+static const double KEV = 1.60217662e-16;
+static const double K_B = 1.38064852e-23;
+static const double H_KEV = 4.135667662e-18;
+static int eq12(double x, double y) { return fabs(x - y) < 1.0e-12; }
+
+/* rays.pyx:53-57 */
+static double image_deflection(int order, double psi)
+{
+    if (order % 2 != 0) return (double)(order + 1) * M_PI + pow(-1.0, (double)order) * psi;
+    return (double)order * M_PI + pow(-1.0, (double)order) * psi;
+}
+
+/* hot_BB.pyx:54-82, norm :84-98 */
+static double bb_I(double E, const double *vec)
+{
+    double kT = K_B / KEV * pow(10.0, vec[0]);
+    return E * E * E / (exp(E / kT) - 1.0);
+}
+static const double BB_NORM = 1.0e-7 * 5.040366110812353e22;
+static const double NUM4D_NORM = 1.0e-7 / 4.135667662e-18;
+
+/* per-call state of the 4-D table interpolator (hot_Num4D.pyx:109-199) */
+typedef struct {
+    const or_atmosphere *a;
+    size_t BN[4];
+    double node[8];
+    double vcache[4];
+    double diff[16];
+    double space[16];
+} n4_state;
+
+static void n4_space(const double *x, size_t bn, double *s)
+{
+    s[0] = 1.0 / (x[bn] - x[bn + 1]); s[0] /= x[bn] - x[bn + 2]; s[0] /= x[bn] - x[bn + 3];
+    s[1] = 1.0 / (x[bn + 1] - x[bn]); s[1] /= x[bn + 1] - x[bn + 2]; s[1] /= x[bn + 1] - x[bn + 3];
+    s[2] = 1.0 / (x[bn + 2] - x[bn]); s[2] /= x[bn + 2] - x[bn + 1]; s[2] /= x[bn + 2] - x[bn + 3];
+    s[3] = 1.0 / (x[bn + 3] - x[bn]); s[3] /= x[bn + 3] - x[bn + 1]; s[3] /= x[bn + 3] - x[bn + 2];
+}
+
+static void n4_diff(const double *x, size_t bn, double v, double *d)
+{
+    d[0] = v - x[bn + 1]; d[0] *= v - x[bn + 2]; d[0] *= v - x[bn + 3];
+    d[1] = v - x[bn]; d[1] *= v - x[bn + 2]; d[1] *= v - x[bn + 3];
+    d[2] = v - x[bn]; d[2] *= v - x[bn + 1]; d[2] *= v - x[bn + 3];
+    d[3] = v - x[bn]; d[3] *= v - x[bn + 1]; d[3] *= v - x[bn + 2];
+}
+
+static void n4_init(n4_state *s, const or_atmosphere *a)
+{
+    s->a = a;
+    for (int i = 0; i < 4; i++) {
+        const double *x = a->params[i];
+        s->BN[i] = 0;
+        s->vcache[i] = x[1];
+        s->node[2 * i] = x[1];
+        s->node[2 * i + 1] = x[2];
+        n4_space(x, 0, s->space + 4 * i);
+        n4_diff(x, 0, x[1], s->diff + 4 * i);
+    }
+}
+
+/* hot_Num4D.pyx:248-438 */
+static double n4_I(n4_state *s, double E, double mu, const double *VEC)
+{
+    const or_atmosphere *a = s->a;
+    double v[4];
+    v[0] = VEC[0]; v[1] = VEC[1]; v[2] = mu;
+    v[3] = log10(E / (K_B / KEV * pow(10.0, VEC[0])));
+    for (int i = 0; i < 4; i++) {
+        const double *x = a->params[i];
+        size_t N = a->N[i];
+        int upd = 0;
+        if (v[i] < s->node[2 * i] && s->BN[i] != 0) {
+            upd = 1;
+            while (v[i] < x[s->BN[i] + 1]) {
+                if (s->BN[i] > 0) s->BN[i]--;
+                else if (v[i] <= x[0]) { v[i] = x[0]; break; }
+                else break;
+            }
+            s->node[2 * i] = x[s->BN[i] + 1];
+            s->node[2 * i + 1] = x[s->BN[i] + 2];
+        } else if (v[i] > s->node[2 * i + 1] && s->BN[i] != N - 4) {
+            upd = 1;
+            while (v[i] > x[s->BN[i] + 2]) {
+                if (s->BN[i] < N - 4) s->BN[i]++;
+                else if (v[i] >= x[N - 1]) { v[i] = x[N - 1]; break; }
+                else break;
+            }
+            s->node[2 * i] = x[s->BN[i] + 1];
+            s->node[2 * i + 1] = x[s->BN[i] + 2];
+        }
+        if (s->vcache[i] != v[i] || upd) {
+            n4_diff(x, s->BN[i], v[i], s->diff + 4 * i);
+            s->vcache[i] = v[i];
+        }
+        if (upd) n4_space(x, s->BN[i], s->space + 4 * i);
+    }
+    size_t S0 = a->N[1] * a->N[2] * a->N[3], S1 = a->N[2] * a->N[3], S2 = a->N[3];
+    double I = 0.0;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 4; k++)
+                for (int l = 0; l < 4; l++) {
+                    double w = s->diff[i] * s->diff[4 + j] * s->diff[8 + k] * s->diff[12 + l];
+                    w *= s->space[i] * s->space[4 + j] * s->space[8 + k] * s->space[12 + l];
+                    size_t at = (s->BN[0] + i) * S0 + (s->BN[1] + j) * S1 +
+                                (s->BN[2] + k) * S2 + s->BN[3] + l;
+                    I += w * a->intensity[at];
+                }
+    if (I < 0.0) return 0.0;
+    return I * pow(10.0, 3.0 * v[0]);
+}
+
+int or_integrate(double R, double omega, double r_s, double inclination,
+                 size_t S_rows, size_t S_cols, const double *cellArea,
+                 const double *radial, const double *r_s_over_r,
+                 const double *theta, const double *phi,
+                 const double *cellParams, const int *radiates, size_t N_R,
+                 const double *deflection, const double *cos_alpha,
+                 const double *lag, const double *maxDeflection,
+                 const double *cos_gammaArray, size_t N_E,
+                 const double *energies, size_t N_L, const double *leaves,
+                 size_t N_P, const double *phases, int atm_ext,
+                 const or_atmosphere *atm, int image_order_limit,
+                 double *flux)
+{
+    (void)R; (void)r_s;
+    const double hpi = 0.5 * M_PI, two_pi = 2.0 * M_PI;
+    double sin_i = sin(inclination), cos_i = cos(inclination);
+    size_t leaf_lim = (N_L % 2 == 0) ? N_L / 2 : (N_L + 1) / 2;
+    int image_order = image_order_limit > 0 ? image_order_limit : 0;
+    double *PHASE = (double *)malloc(N_L * sizeof(double));
+    double *PROF = (double *)calloc(N_E * N_L, sizeof(double));
+    double *cdefl = (double *)malloc(N_R * sizeof(double));
+    double *calt = (double *)malloc(N_R * sizeof(double));
+    double *sa = (double *)malloc(4 * N_R * sizeof(double));   /* psi -> cos a */
+    double *sl = (double *)malloc(4 * N_R * sizeof(double));   /* psi -> lag */
+    double *st = (double *)malloc(4 * N_R * sizeof(double));   /* cos psi -> cos a */
+    double *ab = (double *)malloc(3 * N_L * sizeof(double));
+    double *mb = (double *)malloc((N_L + 3) * sizeof(double));
+    memset(flux, 0, N_E * N_P * sizeof(double));
+    double norm = (atm_ext == 2) ? NUM4D_NORM : BB_NORM;
+    n4_state n4;
+    if (atm_ext == 2) n4_init(&n4, atm);
+    int terminate = 0;
+    for (size_t i = 0; i < S_rows && !terminate; i++) {
+        size_t J = 0, j;
+        for (j = 0; j < S_cols; j++) if (radiates[i * S_cols + j] == 1) { J = j; break; }
+        if (j == S_cols) continue;
+        const double *D = deflection + i * N_R, *A = cos_alpha + i * N_R, *L = lag + i * N_R;
+        size_t jn = 0;
+        while (D[jn] <= hpi) jn++;
+        /* reversed tables over cos psi: increasing cos(deflection) */
+        for (size_t k = 0; k < N_R; k++) {
+            cdefl[k] = cos(D[N_R - k - 1]);
+            calt[k] = A[N_R - k - 1];
+        }
+        const double *xalt = cdefl + (N_R - jn - 1), *yalt = calt + (N_R - jn - 1);
+        size_t nalt = jn + 1;
+        or_steffen_init(xalt, yalt, nalt, st, st + N_R, st + 2 * N_R, st + 3 * N_R);
+        or_steffen_init(D, A, N_R, sa, sa + N_R, sa + 2 * N_R, sa + 3 * N_R);
+        or_steffen_init(D, L, N_R, sl, sl + N_R, sl + 2 * N_R, sl + 3 * N_R);
+        double radius = radial[i];
+        double Grav_z = sqrt(1.0 - r_s_over_r[i]);
+        double cos_gamma = cos_gammaArray[i];
+        double sin_gamma = sqrt(1.0 - cos_gamma * cos_gamma);
+        double th_i = theta[i * S_cols];
+        double cos_th = cos(th_i), sin_th = sin(th_i), th_over_pi = th_i / M_PI;
+        double beta = radius * omega * sin_th / (C_SI * Grav_z);
+        double Lorentz = sqrt(1.0 - beta * beta);
+        double c_alpha = -1.0, deriv = -1.0;
+        const double *VEC = cellParams + (i * S_cols + J) * 2;
+        int IO = image_order == 0 ? (int)ceil(maxDeflection[i] / M_PI) : image_order;
+        for (int I = 0; I < IO; I++) {
+            int invis = 2;
+            size_t invis_phase = 0;
+            for (size_t k = 0; k < leaf_lim; k++) {
+                double cos_psi = cos_i * cos_th + sin_i * sin_th * cos(leaves[k]);
+                double psi = image_deflection(I, acos(cos_psi));
+                double sin_psi = sin(psi);
+                if (!eq12(psi, 0.0) && eq12(sin_psi, 0.0)) {
+                    double ii = (cos_i >= 0.0) ? inclination + inclination * 1.0e-6
+                                               : inclination - inclination * 1.0e-6;
+                    cos_psi = cos(ii) * cos_th + sin(ii) * sin_th * cos(leaves[k]);
+                    psi = image_deflection(I, acos(cos_psi));
+                    sin_psi = sin(psi);
+                }
+                int calc = 0;
+                double mu = 0.0, sin_alpha = 0.0;
+                if (psi <= maxDeflection[i]) {
+                    if (psi < D[0] || psi > D[N_R - 1]) {
+                        printf("Interpolation error: deflection = %.16e\n", psi);
+                        terminate = 1;
+                        break;
+                    }
+                    if (psi <= hpi && cos_psi >= xalt[0])
+                        c_alpha = or_steffen_eval(xalt, nalt, st, st + N_R, st + 2 * N_R, st + 3 * N_R, cos_psi);
+                    else
+                        c_alpha = or_steffen_eval(D, N_R, sa, sa + N_R, sa + 2 * N_R, sa + 3 * N_R, psi);
+                    sin_alpha = sqrt(1.0 - c_alpha * c_alpha);
+                    mu = c_alpha * cos_gamma;
+                    if (!eq12(psi, 0.0)) {
+                        double cos_delta = (cos_i - cos_th * cos_psi) / (sin_th * sin_psi);
+                        if (th_over_pi < 0.5) mu = mu + sin_alpha * sin_gamma * cos_delta;
+                        else mu = mu - sin_alpha * sin_gamma * cos_delta;
+                    }
+                    calc = (mu > 0.0) ? 1 : 0;
+                }
+                if (calc) {
+                    if (psi <= hpi && cos_psi >= xalt[0]) {
+                        deriv = or_steffen_deriv(xalt, nalt, st, st + N_R, st + 2 * N_R, cos_psi);
+                    } else {
+                        deriv = or_steffen_deriv(D, N_R, sa, sa + N_R, sa + 2 * N_R, psi);
+                        deriv = exp(log(fabs(deriv)) - log(fabs(sin_psi)));
+                    }
+                    if (psi < D[0] || psi > D[N_R - 1]) {
+                        printf("Interpolation error: deflection = %.16e\n", psi);
+                        terminate = 1;
+                        break;
+                    }
+                    double plag = or_steffen_eval(D, N_R, sl, sl + N_R, sl + 2 * N_R, sl + 3 * N_R, psi);
+                    for (int ks = 0; ks < 2; ks++) {
+                        int take = (0 < k && k < leaf_lim - 1) || (k == 0 && ks == 0) ||
+                                   (k == leaf_lim - 1 && N_L % 2 == 1 && ks == 0) ||
+                                   (k == leaf_lim - 1 && N_L % 2 == 0);
+                        if (!take) continue;
+                        size_t kd = ks == 0 ? k : N_L - 1 - k;
+                        double cos_xi, superlum, eta;
+                        if (!eq12(psi, 0.0)) {
+                            cos_xi = sin_alpha * sin_i * sin(leaves[kd]) / sin_psi;
+                            superlum = 1.0 + beta * cos_xi;
+                            eta = Lorentz / superlum;
+                        } else {
+                            superlum = 1.0;
+                            eta = Lorentz;
+                        }
+                        double Z = eta * Grav_z;
+                        double ABB = mu * eta;
+                        double GEOM = mu * fabs(deriv) * Grav_z * eta * eta * eta / superlum;
+                        PHASE[kd] = leaves[kd] + plag;
+                        for (size_t p = 0; p < N_E; p++) {
+                            double Ep = energies[p] / Z;
+                            double IE = (atm_ext == 2) ? n4_I(&n4, Ep, ABB, VEC) : bb_I(Ep, VEC);
+                            PROF[p * N_L + kd] = (IE * norm - 0.0) * GEOM;
+                        }
+                    }
+                    if (k == 0) {
+                        PHASE[N_L - 1] = PHASE[0] + two_pi;
+                        for (size_t p = 0; p < N_E; p++) PROF[p * N_L + N_L - 1] = PROF[p * N_L];
+                    } else if (invis == 2) {
+                        double step = leaves[k] / (double)k;
+                        for (size_t m = N_L - k; m < N_L; m++) PHASE[m] = PHASE[m - 1] + step;
+                        for (size_t p = 0; p < N_E; p++)
+                            for (size_t m = N_L - k; m < N_L; m++) PROF[p * N_L + m] = 0.0;
+                        PHASE[0] = PHASE[N_L - 1] - two_pi;
+                        for (size_t m = 1; m < k; m++) PHASE[m] = PHASE[m - 1] + step;
+                        for (size_t p = 0; p < N_E; p++) {
+                            PROF[p * N_L] = 0.0;
+                            for (size_t m = 1; m < k; m++) PROF[p * N_L + m] = 0.0;
+                        }
+                    } else if (invis == 1) {
+                        double step = (PHASE[k] - PHASE[invis_phase - 1]) / (double)(k - invis_phase + 1);
+                        for (size_t m = invis_phase; m < k; m++) PHASE[m] = PHASE[m - 1] + step;
+                        step = (PHASE[N_L - invis_phase] - PHASE[N_L - 1 - k]) / (double)(k - invis_phase + 1);
+                        for (size_t m = N_L - k; m < N_L - invis_phase; m++) PHASE[m] = PHASE[m - 1] + step;
+                    }
+                    invis = 0;
+                } else {
+                    if (invis == 0) {
+                        double step = (PHASE[N_L - k] - PHASE[k - 1]) / (double)(N_L - 2 * k + 1);
+                        for (size_t m = k; m < N_L - k; m++) PHASE[m] = PHASE[m - 1] + step;
+                        for (size_t p = 0; p < N_E; p++)
+                            for (size_t m = k; m < N_L - k; m++) PROF[p * N_L + m] = 0.0;
+                        invis = 1;
+                        invis_phase = k;
+                    }
+                }
+            }
+            if (terminate) break;
+            if (invis == 2) break;
+            for (size_t m = 1; m < N_L; m++) {
+                if (PHASE[m] <= PHASE[m - 1]) {
+                    printf("Interpolation error: phases are not strictly increasing.\n");
+                    terminate = 1;
+                    break;
+                }
+            }
+            if (terminate) break;
+            for (size_t p = 0; p < N_E && !terminate; p++) {
+                const double *yp = PROF + p * N_L;
+                or_akima_init(PHASE, yp, N_L, 1, ab, ab + N_L, ab + 2 * N_L, mb);
+                for (size_t jj = 0; jj < S_cols && !terminate; jj++) {
+                    if (radiates[i * S_cols + jj] != 1) continue;
+                    double shift = phi[i * S_cols + jj];
+                    double area = cellArea[i * S_cols + jj];
+                    for (size_t k = 0; k < N_P; k++) {
+                        double x = phases[k] + shift;
+                        if (x > PHASE[N_L - 1]) { while (x > PHASE[N_L - 1]) x -= two_pi; }
+                        else if (x < PHASE[0]) { while (x < PHASE[0]) x += two_pi; }
+                        if (x < PHASE[0] || x > PHASE[N_L - 1]) {
+                            printf("Interpolation error: phase = %.16e\n", x);
+                            terminate = 1;
+                            break;
+                        }
+                        double v = or_akima_eval(PHASE, yp, N_L, ab, ab + N_L, ab + 2 * N_L, x);
+                        if (v > 0.0) flux[p * N_P + k] += area * v;
+                    }
+                }
+            }
+            if (terminate) break;
+        }
+    }
+    for (size_t p = 0; p < N_E; p++)
+        for (size_t k = 0; k < N_P; k++) flux[p * N_P + k] /= energies[p] * KEV;
+    free(PHASE); free(PROF); free(cdefl); free(calt); free(sa); free(sl);
+    free(st); free(ab); free(mb);
+    return terminate;
+}

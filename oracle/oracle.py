@@ -1,0 +1,397 @@
+"""CPU oracle: ctypes bindings to oracle/_build/liboracle.so and a
+restatement of the reference model layer's orchestration.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by the smoke check and by the
+benchmark's CPU-baseline leg, always as the checker.
+
+The orchestration follows
+  xpsi/Spacetime.py:111-190 (derived quantities),
+  xpsi/HotRegion.py:774-1017 (mesh, rays, lag calibration, cell parameters),
+  xpsi/HotRegion.py:1115-1241 (integration per super/cede member),
+  xpsi/Likelihood.py:297-432 (divide by d^2, register, signal call),
+  xpsi/Signal.py:379-453, 1092-1197 (energy grid, energy integral, fold),
+  examples/examples_fast/Modules/CustomSignal.py:52-67 (marginal likelihood).
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+# xpsi/global_imports.py:67-88
+c_light = 2.99792458e8
+kpc = 3.08567758e19
+keV = 1.60217662e-16
+GM_sun = 0.5 * 2.95325024e3
+G_SI = 6.6740831e-11
+km = 1.0e3
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "_build", "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        d, sz, i = C.c_double, C.c_size_t, C.c_int
+        L.or_interval.restype = sz
+        L.or_interval.argtypes = [_dp, sz, d]
+        L.or_steffen_init.argtypes = [_dp, _dp, sz, _dp, _dp, _dp, _dp]
+        L.or_steffen_eval.restype = d
+        L.or_steffen_eval.argtypes = [_dp, sz, _dp, _dp, _dp, _dp, d]
+        L.or_steffen_deriv.restype = d
+        L.or_steffen_deriv.argtypes = [_dp, sz, _dp, _dp, _dp, d]
+        L.or_akima_init.argtypes = [_dp, _dp, sz, i, _dp, _dp, _dp, _dp]
+        L.or_akima_eval.restype = d
+        L.or_akima_eval.argtypes = [_dp, _dp, sz, _dp, _dp, _dp, d]
+        L.or_akima_integ.restype = d
+        L.or_akima_integ.argtypes = [_dp, _dp, sz, _dp, _dp, _dp, d, d]
+        L.or_effective_gravity.restype = d
+        L.or_effective_gravity.argtypes = [d, d, d, d, i]
+        L.or_allocate_cells.argtypes = [sz, sz, sz] + [d] * 6 + [i] + [d] * 8 + [_dp]
+        L.or_construct_cell_mesh.restype = i
+        L.or_construct_cell_mesh.argtypes = ([i, sz, sz] + [d] * 5 + [i] + [d] * 5 + [_dp] * 7)
+        L.or_compute_rays.restype = i
+        L.or_compute_rays.argtypes = [sz, d, _dp, _dp, sz, _dp, _dp, _dp, _dp]
+        L.or_integrate.restype = i
+        L.or_integrate.argtypes = ([d] * 4 + [sz, sz] + [_dp] * 6 + [_ip, sz] +
+                                   [_dp] * 5 + [sz, _dp, sz, _dp, sz, _dp, i,
+                                                C.c_void_p, i, _dp])
+        L.or_energy_integrator.argtypes = [sz, sz, _dp, _dp, sz, _dp, _dp]
+        L.or_precomputation.argtypes = [sz, sz, _ip, _dp]
+        L.or_eval_marginal_likelihood.restype = i
+        L.or_eval_marginal_likelihood.argtypes = (
+            [d, sz, _dp, sz, _dp, sz, C.c_void_p, C.c_void_p, C.c_void_p, _dp,
+             _dp, _dp, d, d, d, d, d, C.c_void_p, d, C.c_void_p,
+             C.POINTER(C.c_double), _dp, _dp, _dp])
+        _LIB = L
+    return _LIB
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ----------------------------------------------------------------- numerics
+def steffen(x, y):
+    x, y = _c(x), _c(y)
+    n = len(x)
+    co = [np.zeros(n) for _ in range(4)]
+    lib().or_steffen_init(x, y, n, *co)
+    return x, co
+
+
+def steffen_eval(sp, xq, deriv=False):
+    x, co = sp
+    if deriv:
+        return lib().or_steffen_deriv(x, len(x), co[0], co[1], co[2], float(xq))
+    return lib().or_steffen_eval(x, len(x), *co, float(xq))
+
+
+def akima(x, y, periodic=True):
+    x, y = _c(x), _c(y)
+    n = len(x)
+    co = [np.zeros(n) for _ in range(3)]
+    lib().or_akima_init(x, y, n, int(periodic), *co, np.zeros(n + 3))
+    return x, y, co
+
+
+def akima_eval(sp, xq):
+    x, y, co = sp
+    return lib().or_akima_eval(x, y, len(x), *co, float(xq))
+
+
+def akima_integ(sp, a, b):
+    x, y, co = sp
+    return lib().or_akima_integ(x, y, len(x), *co, float(a), float(b))
+
+
+# ------------------------------------------------------------ cell mesh/rays
+def allocate_cells(num_cells, min_sqrt, max_sqrt, mass, r_s, R_eq, Omega,
+                   zeta, epsilon, shape, superRadius, cedeRadius,
+                   superColatitude, cedeColatitude, superAzimuth,
+                   holeColatitude, holeRadius, holeAzimuth):
+    out = np.zeros(4)
+    lib().or_allocate_cells(int(num_cells), int(min_sqrt), int(max_sqrt),
+                            mass, r_s, R_eq, Omega, zeta, epsilon, int(shape),
+                            superRadius, cedeRadius, superColatitude,
+                            cedeColatitude, superAzimuth, holeColatitude,
+                            holeRadius, holeAzimuth, out)
+    return tuple(out)
+
+
+def construct_cell_mesh(polar, numCell, S, mass, r_s, R_eq, zeta, epsilon,
+                        shape, cedeRadius, cedeColatitude, superRadius,
+                        superColatitude, superAzimuth):
+    S = int(S)
+    theta, phi, area = (np.zeros((S, S)) for _ in range(3))
+    r, maxA, cg, eg = (np.zeros(S) for _ in range(4))
+    lib().or_construct_cell_mesh(int(polar), int(numCell), S, mass, r_s, R_eq,
+                                 zeta, epsilon, int(shape), cedeRadius,
+                                 cedeColatitude, superRadius, superColatitude,
+                                 superAzimuth, theta, phi, r, area, maxA, cg, eg)
+    return theta, phi, r, area, maxA, cg, eg
+
+
+def compute_rays(r_s, r_s_over_r, maxAlpha, N_R):
+    u = _c(r_s_over_r)
+    N_u = len(u)
+    defl, cosa, lag = (np.zeros((N_u, N_R)) for _ in range(3))
+    maxD = np.zeros(N_u)
+    t = lib().or_compute_rays(N_u, r_s, u, maxAlpha, N_R, defl, cosa, lag, maxD)
+    return t, defl, cosa, lag, maxD
+
+
+class _Atm(C.Structure):
+    _fields_ = [("ndims", C.c_size_t), ("params", C.c_void_p * 4),
+                ("N", C.c_size_t * 4), ("intensity", C.c_void_p)]
+
+
+def integrate(omega, r_s, inclination, cellArea, radial, r_s_over_r, theta,
+              phi, cellParams, radiates, deflection, cos_alpha, lag,
+              maxDeflection, cos_gamma, energies, leaves, phases, atm_ext=1,
+              atmosphere=None, image_order_limit=None, R=0.0):
+    S0, S1 = cellArea.shape
+    N_R = deflection.shape[1]
+    flux = np.zeros((len(energies), len(phases)))
+    keep = []
+    atm_ptr = None
+    if atm_ext == 2:
+        arrs = [_c(a) for a in atmosphere]
+        keep.append(arrs)
+        st = _Atm()
+        st.ndims = 4
+        for k in range(4):
+            st.params[k] = arrs[k].ctypes.data
+            st.N[k] = len(arrs[k])
+        st.intensity = arrs[4].ctypes.data
+        keep.append(st)
+        atm_ptr = C.addressof(st)
+    t = lib().or_integrate(R, omega, r_s, inclination, S0, S1, _c(cellArea),
+                           _c(radial), _c(r_s_over_r), _c(theta), _c(phi),
+                           _c(cellParams), _c(radiates, np.int32), N_R,
+                           _c(deflection), _c(cos_alpha), _c(lag),
+                           _c(maxDeflection), _c(cos_gamma), len(energies),
+                           _c(energies), len(leaves), _c(leaves), len(phases),
+                           _c(phases), int(atm_ext), atm_ptr,
+                           int(image_order_limit or 0), flux)
+    return t, flux
+
+
+# ------------------------------------------------------------ signal / lnL
+def energy_integrator(signal, log10_energies, log10_edges):
+    signal = _c(signal)
+    N_E, N_P = signal.shape
+    out = np.zeros((len(log10_edges) - 1, N_P))
+    lib().or_energy_integrator(N_E, N_P, signal, _c(log10_energies),
+                               len(log10_edges), _c(log10_edges), out)
+    return out
+
+
+def precomputation(counts):
+    counts = _c(counts, np.int32)
+    out = np.zeros(counts.shape[0])
+    lib().or_precomputation(counts.shape[0], counts.shape[1], counts, out)
+    return out
+
+
+def eval_marginal_likelihood(exposure_time, phases, counts, components,
+                             component_phases, phase_shifts, precomp, support,
+                             epsabs=0.0, epsrel=1.0e-8, epsilon=1.0e-3,
+                             sigmas=10.0, llzero=-1.0e90, allow_negative=False,
+                             slim=20.0, background=None):
+    comps = [_c(cmp) for cmp in components]
+    cph = [_c(p) for p in component_phases]
+    nc = len(comps)
+    N_ch, N_b = counts.shape
+    cp = (C.c_void_p * nc)(*[cmp.ctypes.data for cmp in comps])
+    pp = (C.c_void_p * nc)(*[p.ctypes.data for p in cph])
+    nps = (C.c_size_t * nc)(*[cmp.shape[1] for cmp in comps])
+    if isinstance(allow_negative, bool):
+        an = np.full(nc, int(allow_negative), dtype=np.uint8)
+    else:
+        an = np.asarray(allow_negative, dtype=np.uint8)
+    bg = None if background is None else _c(background)
+    STAR = np.zeros((N_ch, N_b))
+    bml, bmls = np.zeros(N_ch), np.zeros(N_ch)
+    lnl = C.c_double(0.0)
+    lib().or_eval_marginal_likelihood(
+        float(exposure_time), N_b, _c(phases), N_ch, _c(counts), nc, cp, nps,
+        pp, _c(phase_shifts), _c(precomp), _c(support), epsabs, epsrel,
+        epsilon, sigmas, llzero, an.ctypes.data, slim,
+        None if bg is None else bg.ctypes.data, C.byref(lnl), STAR, bml, bmls)
+    return lnl.value, STAR, bml, bmls
+
+
+# ---------------------------------------------------------- model pipeline
+def spacetime(mass, radius, distance, cos_inclination, frequency,
+              star_shape="AGM_14"):
+    """Derived quantities, xpsi/Spacetime.py:111-190."""
+    r_g = mass * GM_sun
+    M = mass * GM_sun * c_light ** 2 / G_SI
+    R = radius * km
+    Omega = 2.0 * math.pi * frequency
+    return dict(mass=mass, M=M, r_g=r_g, r_s=2.0 * r_g, R=R, Omega=Omega,
+                f=frequency, i=math.acos(cos_inclination),
+                d=distance * kpc, d_sq=(distance * kpc) ** 2, zeta=r_g / R,
+                epsilon=Omega ** 2 * R ** 3 / (G_SI * M),
+                shape={"AGM_14": 0, "sphere": 1}[star_shape])
+
+
+def construct_energy_array(num_energies, edges):
+    """Signal.py:1092-1197 for one signal (no coverage gaps)."""
+    L, U = edges[0], edges[-1]
+    e = np.logspace(np.log10(L), np.log10(U), int(math.ceil(num_energies)),
+                    base=10.0)[::-1]
+    return np.ascontiguousarray(e[::-1])
+
+
+def hot_region_embed(st, hr, mode_frequency):
+    """HotRegion.py:774-1017: mesh, rays, calibrated lag, cell parameters.
+
+    `hr` holds the region parameters and resolution settings:
+    super_colatitude, super_radius, super_temperature, phase_shift, and
+    optionally cede_colatitude/cede_radius/cede_azimuth/cede_temperature,
+    omit_*; sqrt_num_cells, min/max_sqrt_num_cells, num_rays, num_leaves,
+    num_phases, is_antiphased, image_order_limit, atm_ext.
+    Returns (members, error) with members a list of dicts.
+    """
+    sc, sr = hr["super_colatitude"], hr["super_radius"]
+    cede = hr.get("cede_radius", 0.0) > 0.0 or hr.get("cede", False)
+    cr = hr.get("cede_radius", 0.0) if cede else 0.0
+    cc = hr.get("cede_colatitude", sc) if cede else sc
+    ca = hr.get("cede_azimuth", 0.0) if cede else 0.0
+    oc = hr.get("omit_colatitude", sc)
+    orad = hr.get("omit_radius", 0.0)
+    oa = hr.get("omit_azimuth", 0.0)
+    nc = int(hr.get("sqrt_num_cells", 32)) ** 2
+    alloc = allocate_cells(nc, hr.get("min_sqrt_num_cells", 10),
+                           hr.get("max_sqrt_num_cells", 80), st["M"],
+                           st["r_s"], st["R"], st["Omega"], st["zeta"],
+                           st["epsilon"], st["shape"], sr, cr, sc, cc, -ca,
+                           oc, orad, oa)
+    members = []
+    specs = [(alloc[0], alloc[1], sr, sc, orad, oc, oa, -oa,
+              hr["super_temperature"])]
+    if cr > 0.0:
+        specs.append((alloc[2], alloc[3], cr, cc, sr, sc, -ca, ca,
+                      hr.get("cede_temperature")))
+    for (S, ncell, radm, colm, rad_cut, col_cut, azi_cut, dphi, temp) in specs:
+        polar = (colm - radm < 0.0) or (colm + radm > math.pi)
+        th, ph, r, area, maxA, cg, eg = construct_cell_mesh(
+            polar, ncell, S, st["mass"], st["r_s"], st["R"], st["zeta"],
+            st["epsilon"], st["shape"], radm, colm, rad_cut, col_cut, azi_cut)
+        ph = ph + dphi
+        if hr.get("is_antiphased", False):
+            ph = ph + math.pi
+        u = np.ascontiguousarray(st["r_s"] / r)
+        t, defl, cosa, lag, maxD = compute_rays(st["r_s"], u, maxA,
+                                                int(hr.get("num_rays", 200)))
+        if t == 1:
+            return None, "RayError"
+        R_i = st["R"] / st["r_s"]
+        Cc = (1.0 / u - R_i) + np.log((1.0 / u - 1.0) / (R_i - 1.0))
+        Cc *= st["r_s"] / c_light
+        lag = (lag - Cc[:, None]) * (mode_frequency * 2.0 * math.pi)
+        radiates = (area > 0.0).astype(np.int32)
+        params = np.ones((int(S), int(S), 2))
+        params[..., 0] = temp
+        params[..., 1] = eg[:, None]
+        members.append(dict(S=int(S), theta=th, phi=ph, r=r, cellArea=area,
+                            maxAlpha=maxA, cos_gamma=cg, effGrav=eg,
+                            r_s_over_r=u, deflection=defl, cos_alpha=cosa,
+                            lag=np.ascontiguousarray(lag), maxDeflection=maxD,
+                            radiates=radiates, params=params))
+    return members, None
+
+
+def hot_region_integrate(st, hr, members, energies, atmosphere=None):
+    N_L = int(hr.get("num_leaves", 64))
+    N_P = int(hr.get("num_phases") or N_L)
+    leaves = np.linspace(0.0, 2.0 * math.pi, N_L)
+    phases = 2.0 * math.pi * np.linspace(0.0, 1.0, N_P)
+    out = []
+    for m in members:
+        t, flux = integrate(st["Omega"], st["r_s"], st["i"], m["cellArea"],
+                            m["r"], m["r_s_over_r"], m["theta"], m["phi"],
+                            m["params"], m["radiates"], m["deflection"],
+                            m["cos_alpha"], m["lag"], m["maxDeflection"],
+                            m["cos_gamma"], energies, leaves, phases,
+                            atm_ext=hr.get("atm_ext", 1),
+                            atmosphere=atmosphere,
+                            image_order_limit=hr.get("image_order_limit"),
+                            R=st["R"])
+        if t == 1:
+            return None
+        out.append(flux)
+    return out
+
+
+def loglikelihood(model, p=None, return_parts=False):
+    """One likelihood evaluation (Likelihood._driver, CustomSignal.__call__).
+
+    `model` is a dict: spacetime (frequency, star_shape), hot_regions (list
+    of dicts as for hot_region_embed, each with its parameter values),
+    instrument (matrix[N_ch, N_in], energy_edges[N_in+1]), data (counts,
+    phases, exposure_time), num_energies, support, marginal settings,
+    optional interstellar callable and atmosphere.
+    """
+    sp = model["spacetime"]
+    st = spacetime(sp["mass"], sp["radius"], sp["distance"],
+                   sp["cos_inclination"], sp["frequency"],
+                   sp.get("star_shape", "AGM_14"))
+    llzero = model.get("llzero", -1.0e90)
+    edges = model["instrument"]["energy_edges"]
+    energies = construct_energy_array(model["num_energies"], edges)
+    signals, phase_sets, shifts = [], [], []
+    for hr in model["hot_regions"]:
+        members, err = hot_region_embed(st, hr, sp["frequency"])
+        if err is not None:
+            return llzero * 0.5 if not return_parts else (llzero * 0.5, None)
+        fluxes = hot_region_integrate(st, hr, members, energies,
+                                      model.get("atmosphere"))
+        if fluxes is None:
+            return llzero * 0.5 if not return_parts else (llzero * 0.5, None)
+        integrated = None
+        for f in fluxes:
+            tmp = energy_integrator(f / st["d_sq"], np.log10(energies),
+                                    np.log10(edges))
+            integrated = tmp if integrated is None else integrated + tmp
+        if model.get("interstellar") is not None:
+            mids = 0.5 * (edges[:-1] + edges[1:])
+            integrated = integrated * model["interstellar"](mids)[:, None]
+        folded = model["instrument"]["matrix"] @ integrated
+        signals.append(np.ascontiguousarray(folded))
+        N_P = int(hr.get("num_phases") or hr.get("num_leaves", 64))
+        phase_sets.append(np.linspace(0.0, 1.0, N_P))
+        shifts.append(hr["phase_shift"])
+    data = model["data"]
+    counts = np.ascontiguousarray(data["counts"], dtype=np.float64)
+    pre = precomputation(counts.astype(np.int32))
+    support = model.get("support")
+    if support is None:
+        support = -1.0 * np.ones((counts.shape[0], 2))
+        support[:, 0] = 0.0
+    ms = model.get("marginal", {})
+    lnl, STAR, bml, bmls = eval_marginal_likelihood(
+        data["exposure_time"], data["phases"], counts, signals, phase_sets,
+        np.asarray(shifts, dtype=np.float64), pre, support,
+        ms.get("epsabs", 0.0), ms.get("epsrel", 1.0e-8),
+        ms.get("epsilon", 1.0e-3), ms.get("sigmas", 10.0), llzero)
+    if lnl <= llzero:
+        lnl = llzero * 0.5
+    if return_parts:
+        return lnl, dict(energies=energies, signals=signals, STAR=STAR)
+    return lnl

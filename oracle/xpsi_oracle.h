@@ -1,0 +1,114 @@
+/*
+ * CPU oracle for the X-PSI pulse-profile likelihood path.
+ * TEST INFRASTRUCTURE ONLY: only tests/, the smoke check and the benchmark's
+ * CPU-baseline leg load this library, and only as the checker.
+ */
+#ifndef XPSI_ORACLE_H
+#define XPSI_ORACLE_H
+#include <stddef.h>
+
+#define OR_SUCCESS 0
+#define OR_EFAILED 5
+#define OR_EMAXITER 11
+#define OR_EROUND 18
+#define OR_ESING 21
+#define OR_QAG_LIMIT 128
+
+/* relative tolerance used at the reference's cquad call sites */
+#define OR_MESH_EPSREL 1.0e-10
+#define OR_MARG_EPSREL 1.0e-11
+
+typedef double (*or_fn)(double, void *);
+
+/* numerics core */
+size_t or_interval(const double *xa, size_t n, double x);
+void or_steffen_init(const double *x, const double *y, size_t n,
+                     double *a, double *b, double *c, double *d);
+double or_steffen_eval(const double *x, size_t n, const double *a,
+                       const double *b, const double *c, const double *d,
+                       double xq);
+double or_steffen_deriv(const double *x, size_t n, const double *a,
+                        const double *b, const double *c, double xq);
+void or_akima_init(const double *x, const double *y, size_t n, int periodic,
+                   double *b, double *c, double *d, double *mbuf);
+double or_akima_eval(const double *x, const double *y, size_t n,
+                     const double *b, const double *c, const double *d,
+                     double xq);
+double or_akima_integ(const double *x, const double *y, size_t n,
+                      const double *b, const double *c, const double *d,
+                      double lo, double hi);
+void or_gk61(or_fn f, void *p, double a, double b, double *res,
+             double *err, double *rabs, double *rasc);
+int or_qag61(or_fn f, void *p, double a, double b, double epsabs,
+             double epsrel, int limit, double *result, double *abserr);
+
+/* cell mesh (mesh_tools.pyx, mesh.pyx, polar_mesh.pyx) */
+double or_effective_gravity(double mu, double R_eq, double zeta,
+                            double epsilon, int shape);
+void or_allocate_cells(size_t num_cells, size_t min_sqrt, size_t max_sqrt,
+                       double mass, double r_s, double R_eq, double Omega,
+                       double zeta, double epsilon, int shape,
+                       double superRadius, double cedeRadius,
+                       double superColatitude, double cedeColatitude,
+                       double superAzimuth, double holeColatitude,
+                       double holeRadius, double holeAzimuth, double *out4);
+/* outputs: theta[S*S], phi[S*S], r[S], cellArea[S*S], maxAlpha[S],
+   cos_gamma[S], effGrav[S] */
+int or_construct_cell_mesh(int polar, size_t numCell, size_t S, double mass,
+                           double r_s, double R_eq, double zeta,
+                           double epsilon, int shape, double cedeRadius,
+                           double cedeColatitude, double superRadius,
+                           double superColatitude, double superAzimuth,
+                           double *theta, double *phi, double *r,
+                           double *cellArea, double *maxAlpha,
+                           double *cos_gamma, double *effGrav);
+
+/* rays.pyx compute_rays: maxAlpha is updated in place */
+int or_compute_rays(size_t N_u, double r_s, const double *r_s_over_R,
+                    double *maxAlpha, size_t N_R, double *deflection,
+                    double *cos_alpha, double *lag, double *maxDeflection);
+
+/* integrator_for_azimuthal_invariance.pyx integrate (one thread) */
+typedef struct {
+    size_t ndims;          /* 4 */
+    const double *params[4];
+    size_t N[4];
+    const double *intensity;
+} or_atmosphere;
+
+int or_integrate(double R, double omega, double r_s, double inclination,
+                 size_t S_rows, size_t S_cols, const double *cellArea,
+                 const double *radial, const double *r_s_over_r,
+                 const double *theta, const double *phi,
+                 const double *srcCellParams, const int *radiates,
+                 size_t N_R, const double *deflection,
+                 const double *cos_alpha, const double *lag,
+                 const double *maxDeflection, const double *cos_gamma,
+                 size_t N_E, const double *energies, size_t N_L,
+                 const double *leaves, size_t N_P, const double *phases,
+                 int atm_ext, const or_atmosphere *atm, int image_order_limit,
+                 double *flux);
+
+/* energy_integrator.pyx: out[N_in, N_P] (already transposed) */
+void or_energy_integrator(size_t N_E, size_t N_P, const double *signal,
+                          const double *log10_energies, size_t N_edges,
+                          const double *log10_edges, double *out);
+
+/* default_background_marginalisation.pyx */
+void or_precomputation(size_t N_ch, size_t N_b, const int *counts,
+                       double *precomp);
+int or_eval_marginal_likelihood(double exposure_time, size_t N_b,
+                                const double *phases, size_t N_ch,
+                                const double *counts, size_t n_comp,
+                                const double *const *components,
+                                const size_t *comp_N_P,
+                                const double *const *component_phases,
+                                const double *phase_shifts,
+                                const double *precomp, const double *support,
+                                double epsabs, double epsrel, double epsilon,
+                                double sigmas, double llzero,
+                                const unsigned char *allow_negative,
+                                double slim, const double *background,
+                                double *lnl, double *STAR, double *B_ML,
+                                double *B_ML_supp);
+#endif
