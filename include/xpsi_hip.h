@@ -1,0 +1,162 @@
+/*
+ * C ABI of the MI355X (gfx950) X-PSI pulse-profile likelihood library.
+ *
+ * All array arguments are plain pointers with explicit sizes; "device"
+ * pointers are HIP device memory (e.g. torch.Tensor.data_ptr() on a ROCm
+ * device), "host" pointers are ordinary process memory.  `stream` is a
+ * hipStream_t (NULL = the default stream).  Every entry point returns 0 on
+ * success or a negative XH_E* code; xh_last_error() describes the failure.
+ * No entry point allocates device memory except xh_pipeline_create().
+ */
+#ifndef XPSI_HIP_H
+#define XPSI_HIP_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XH_OK 0
+#define XH_EINVAL (-1)
+#define XH_EHIP (-2)
+#define XH_ENOMEM (-3)
+#define XH_ERANGE (-4)
+
+/* Message for the last failing call on this thread. */
+const char* xh_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Drop-in kernels.  Each replaces one reference function; sizes follow the
+ * reference's array shapes.  Device pointers, C order, float64 unless noted.
+ * --------------------------------------------------------------------- */
+
+/* Replaces xpsi/cellmesh/integrator_for_azimuthal_invariance.pyx:70-98
+ * `integrate(...) -> (status, flux[N_E, N_P])` for the blackbody atmosphere
+ * (hot_atm_ext 1) without an elsewhere correction or disk.
+ * cellArea, theta, phi: [S, S]; radiates: int32 [S, S]; cellParams: [S, S, 2];
+ * radial, r_s_over_r, maxDeflection, cos_gamma: [S];
+ * deflection, cos_alpha, lag: [S, N_R]; energies [N_E] (keV);
+ * leaves [N_L]; phases [N_P] (radians); flux out [N_E, N_P].
+ * workspace: device scratch of xh_integrate_workspace_bytes() bytes.
+ * *status_out (host) = 0 success, 1 numerical failure (the reference's ERROR). */
+size_t xh_integrate_workspace_bytes(int S, int N_R, int N_L, int N_E, int N_P);
+int xh_integrate(double omega, double r_s, double inclination, int S,
+                 const double* cellArea, const double* radial,
+                 const double* r_s_over_r, const double* theta,
+                 const double* phi, const double* cellParams,
+                 const int* radiates, int N_R, const double* deflection,
+                 const double* cos_alpha, const double* lag,
+                 const double* maxDeflection, const double* cos_gamma,
+                 int N_E, const double* energies, int N_L,
+                 const double* leaves, int N_P, const double* phases,
+                 int hot_atm_ext, int image_order_limit, double* flux,
+                 void* workspace, size_t workspace_bytes, int* status_out,
+                 void* stream);
+
+/* Replaces xpsi/tools/energy_integrator.pyx:27-30,114
+ * `energy_integrator(N_Ts, signal[N_E, N_P], log10_E[N_E], log10_edges[N_in+1])`
+ * -> out [N_in, N_P]. */
+int xh_energy_integrator(int N_E, int N_P, const double* signal,
+                         const double* log10_energies, int N_edges,
+                         const double* log10_edges, double* out, void* stream);
+
+/* Replaces xpsi/Instrument.py:163-197 `matrix @ signal`:
+ * out[N_ch, N_P] = M[N_ch, N_in] (row stride ld_M) @ signal[N_in, N_P]. */
+int xh_fold(int N_ch, int N_in, int N_P, const double* M, int ld_M,
+            const double* signal, double* out, void* stream);
+
+/* Replaces xpsi/likelihoods/default_background_marginalisation.pyx:38-68
+ * `precomputation(int32 counts[N_ch, N_b]) -> -sum ln n!` (host memory). */
+int xh_precomputation(int N_ch, int N_b, const int* counts, double* out);
+
+/* Replaces default_background_marginalisation.pyx:450-466,731-734
+ * `eval_marginal_likelihood(...) -> (lnL, STAR, B_ML, B_ML_supp)` for
+ * n_comp components sharing one phase grid comp_phases [N_P] (cycles).
+ * components: [n_comp, N_ch, N_P]; phases [N_b+1]; counts [N_ch, N_b];
+ * shifts [n_comp]; precomp [N_ch]; support [N_ch, 2]; background [N_ch, N_b]
+ * or NULL.  lnl_out is a host pointer; STAR [N_ch, N_b], B_ML, B_ML_supp
+ * [N_ch] device outputs; workspace of xh_marginal_workspace_bytes(). */
+size_t xh_marginal_workspace_bytes(int N_ch);
+int xh_eval_marginal_likelihood(double exposure_time, int N_b,
+                                const double* phases, int N_ch,
+                                const double* counts, int n_comp,
+                                const double* components, int N_P,
+                                const double* comp_phases,
+                                const double* shifts, const double* precomp,
+                                const double* support, double epsabs,
+                                double epsrel, double epsilon, double sigmas,
+                                double llzero, int allow_negative,
+                                double slim, const double* background,
+                                double* lnl_out, double* STAR, double* B_ML,
+                                double* B_ML_supp, void* workspace,
+                                size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Whole-likelihood pipeline: mesh, rays, integration, registration and the
+ * marginal likelihood for a batch of parameter vectors, all on the device.
+ * Replaces the body of xpsi/Likelihood.py:297-432 (_driver) for models made
+ * of up to two circular hot regions (each optionally with a ceding ring).
+ * --------------------------------------------------------------------- */
+typedef struct xh_region_config {
+    int cede;                 /* ceding ring present */
+    int is_antiphased;
+    int sqrt_num_cells, min_sqrt_num_cells, max_sqrt_num_cells;
+} xh_region_config;
+
+typedef struct xh_model_config {
+    int n_regions;            /* 1 or 2 */
+    xh_region_config region[2];
+    int num_rays, num_leaves, num_phases, image_order_limit, atm_ext;
+    int star_shape;           /* 0 oblate (AGM_14), 1 sphere */
+    double frequency, mode_frequency;
+    int N_E;                  /* energies (host, keV) */
+    const double* energies;
+    int N_in;                 /* input bins: edges [N_in + 1] keV (host) */
+    const double* energy_edges;
+    const double* attenuation;/* [N_in] multiplier or NULL */
+    int N_ch;                 /* response [N_ch, N_in] (host) */
+    const double* matrix;
+    int N_b;                  /* data: phases [N_b+1], counts [N_ch, N_b] */
+    const double* data_phases;
+    const double* counts;
+    double exposure_time;
+    const double* support;    /* [N_ch, 2] or NULL (unbounded) */
+    double epsabs, epsrel, epsilon, sigmas, llzero, slim;
+    int max_batch;            /* requests per xh_pipeline_eval call */
+    int max_sqrt_cells;       /* buffer bound on mesh size (>= max_sqrt_num_cells) */
+} xh_model_config;
+
+/* One parameter vector.  Per region: phase_shift, super_colatitude,
+ * super_radius, super_temperature, cede_colatitude, cede_radius,
+ * cede_azimuth, cede_temperature (log10 K, angles in radians). */
+typedef struct xh_request {
+    double mass, radius, distance, cos_inclination;
+    double region[2][8];
+    int sqrt_num_cells[2];    /* 0 = the configured value */
+} xh_request;
+
+typedef struct xh_pipeline xh_pipeline;
+
+xh_pipeline* xh_pipeline_create(const xh_model_config* cfg);
+void xh_pipeline_destroy(xh_pipeline* p);
+/* Evaluate n <= max_batch requests; lnl_out host [n].  Blocks until done. */
+int xh_pipeline_eval(xh_pipeline* p, int n, const xh_request* reqs,
+                     double* lnl_out, void* stream);
+/* Same without waiting: results land in the pipeline's host buffer, read
+ * with xh_pipeline_results() after xh_pipeline_sync(). */
+int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs, void* stream);
+int xh_pipeline_sync(xh_pipeline* p);
+const double* xh_pipeline_results(xh_pipeline* p);
+
+/* Copy an intermediate array of request `req` to host memory for checks.
+ * what: 0 flux[member], 1 folded signal[region] [N_ch, N_P],
+ *       2 member sizes (S, status) as doubles, 3 cellArea[member] [S,S],
+ *       4 deflection[member] [S,N_R], 5 lag[member] [S,N_R],
+ *       6 theta[member] [S], 7 phi[member] [S], 8 STAR [N_ch,N_b]. */
+int xh_pipeline_fetch(xh_pipeline* p, int what, int req, int index,
+                      double* host_out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,101 @@
+"""ctypes binding to libxpsi_hip.so (declarations: include/xpsi_hip.h).
+
+The library is required: there is no CPU fallback.  Loading fails loudly
+when the shared object has not been built.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxpsi_hip.so")
+
+_lib = None
+
+d, i32, sz, vp = C.c_double, C.c_int, C.c_size_t, C.c_void_p
+
+
+class RegionConfig(C.Structure):
+    _fields_ = [("cede", i32), ("is_antiphased", i32), ("sqrt_num_cells", i32),
+                ("min_sqrt_num_cells", i32), ("max_sqrt_num_cells", i32)]
+
+
+class ModelConfig(C.Structure):
+    _fields_ = [("n_regions", i32), ("region", RegionConfig * 2),
+                ("num_rays", i32), ("num_leaves", i32), ("num_phases", i32),
+                ("image_order_limit", i32), ("atm_ext", i32), ("star_shape", i32),
+                ("frequency", d), ("mode_frequency", d),
+                ("N_E", i32), ("energies", vp),
+                ("N_in", i32), ("energy_edges", vp), ("attenuation", vp),
+                ("N_ch", i32), ("matrix", vp),
+                ("N_b", i32), ("data_phases", vp), ("counts", vp),
+                ("exposure_time", d), ("support", vp),
+                ("epsabs", d), ("epsrel", d), ("epsilon", d), ("sigmas", d),
+                ("llzero", d), ("slim", d),
+                ("max_batch", i32), ("max_sqrt_cells", i32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("mass", d), ("radius", d), ("distance", d), ("cos_inclination", d),
+                ("region", (d * 8) * 2), ("sqrt_num_cells", i32 * 2)]
+
+
+class XHError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise XHError("libxpsi_hip.so is not built; run `python -m xpsi_amd.build` "
+                      "(or __graft_entry__.build()) first")
+    L = C.CDLL(LIB_PATH)
+    L.xh_last_error.restype = C.c_char_p
+    L.xh_integrate_workspace_bytes.restype = sz
+    L.xh_integrate_workspace_bytes.argtypes = [i32] * 5
+    L.xh_integrate.restype = i32
+    L.xh_integrate.argtypes = ([d, d, d, i32] + [vp] * 7 + [i32] + [vp] * 5 +
+                               [i32, vp, i32, vp, i32, vp, i32, i32, vp, vp, sz,
+                                C.POINTER(i32), vp])
+    L.xh_energy_integrator.restype = i32
+    L.xh_energy_integrator.argtypes = [i32, i32, vp, vp, i32, vp, vp, vp]
+    L.xh_fold.restype = i32
+    L.xh_fold.argtypes = [i32, i32, i32, vp, i32, vp, vp, vp]
+    L.xh_precomputation.restype = i32
+    L.xh_precomputation.argtypes = [i32, i32, vp, vp]
+    L.xh_marginal_workspace_bytes.restype = sz
+    L.xh_marginal_workspace_bytes.argtypes = [i32]
+    L.xh_eval_marginal_likelihood.restype = i32
+    L.xh_eval_marginal_likelihood.argtypes = (
+        [d, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, d, d, d, d, d, i32, d,
+         vp, C.POINTER(d), vp, vp, vp, vp, sz, vp])
+    L.xh_pipeline_create.restype = vp
+    L.xh_pipeline_create.argtypes = [C.POINTER(ModelConfig)]
+    L.xh_pipeline_destroy.restype = None
+    L.xh_pipeline_destroy.argtypes = [vp]
+    L.xh_pipeline_eval.restype = i32
+    L.xh_pipeline_eval.argtypes = [vp, i32, C.POINTER(Request), vp, vp]
+    L.xh_pipeline_launch.restype = i32
+    L.xh_pipeline_launch.argtypes = [vp, i32, C.POINTER(Request), vp]
+    L.xh_pipeline_sync.restype = i32
+    L.xh_pipeline_sync.argtypes = [vp]
+    L.xh_pipeline_results.restype = C.POINTER(d)
+    L.xh_pipeline_results.argtypes = [vp]
+    L.xh_pipeline_fetch.restype = i32
+    L.xh_pipeline_fetch.argtypes = [vp, i32, i32, i32, vp, sz]
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != 0:
+        raise XHError("%s failed (%d): %s" % (what, rc, lib().xh_last_error().decode()))
+
+
+EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
+            "xh_energy_integrator", "xh_fold", "xh_precomputation",
+            "xh_marginal_workspace_bytes", "xh_eval_marginal_likelihood",
+            "xh_pipeline_create", "xh_pipeline_destroy", "xh_pipeline_eval",
+            "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
+            "xh_pipeline_fetch"]

@@ -1,0 +1,51 @@
+"""Drop-in for xpsi/cellmesh/integrator_for_azimuthal_invariance.pyx:70-665.
+
+`integrate(...)` keeps the reference signature (:70-98) and return value
+(:657-665): ``(0, flux[N_E, N_P])`` or ``(1, None)``.  It is bound where the
+reference binds its integrator (HotRegion._integrator, HotRegion.py:551-572).
+Supported: the blackbody atmosphere (hot_atm_ext 1) without an elsewhere
+correction or disk (R_in >= 1e6); other settings raise NotImplementedError.
+"""
+import ctypes as C
+
+import numpy as np
+
+from .._lib import lib, check
+from .. import _device as D
+
+
+def integrate(numThreads, R, omega, r_s, inclination, cellArea,
+              radialCoords_of_parallels, r_s_over_r, theta, phi, srcCellParams,
+              CELL_RADIATES, correction_srcCellParams, numRays, deflection,
+              cos_alpha, lag, maxDeflection, cos_gammaArray, energies, leaves,
+              phases, hot_atmosphere, elsewhere_atmosphere, hot_atm_ext,
+              else_atm_ext, beam_opt, image_order_limit=None, R_in=1e6):
+    if correction_srcCellParams is not None:
+        raise NotImplementedError("elsewhere correction is not supported")
+    if int(hot_atm_ext) != 1 or int(beam_opt) != 0 or R_in < 1e6:
+        raise NotImplementedError("only the blackbody atmosphere without beaming or disk")
+    S = int(np.shape(cellArea)[0])
+    N_R = int(np.shape(deflection)[1])
+    N_E, N_L, N_P = len(energies), len(leaves), len(phases)
+    dev = [D.to_dev(a) for a in (cellArea, radialCoords_of_parallels, r_s_over_r,
+                                  theta, phi, srcCellParams)]
+    rad = D.to_dev(CELL_RADIATES, np.int32)
+    rays = [D.to_dev(a) for a in (deflection, cos_alpha, lag, maxDeflection,
+                                   cos_gammaArray, energies, leaves, phases)]
+    flux = D.empty((N_E, N_P))
+    L = lib()
+    wsb = L.xh_integrate_workspace_bytes(S, N_R, N_L, N_E, N_P)
+    ws = D.empty((wsb + 7) // 8)
+    status = C.c_int(0)
+    rc = L.xh_integrate(float(omega), float(r_s), float(inclination), S,
+                        D.ptr(dev[0]), D.ptr(dev[1]), D.ptr(dev[2]), D.ptr(dev[3]),
+                        D.ptr(dev[4]), D.ptr(dev[5]), D.ptr(rad), N_R,
+                        D.ptr(rays[0]), D.ptr(rays[1]), D.ptr(rays[2]),
+                        D.ptr(rays[3]), D.ptr(rays[4]), N_E, D.ptr(rays[5]),
+                        N_L, D.ptr(rays[6]), N_P, D.ptr(rays[7]), 1,
+                        int(image_order_limit or 0), D.ptr(flux), D.ptr(ws),
+                        ws.numel() * 8, C.byref(status), D.stream_ptr())
+    check(rc, "xh_integrate")
+    if status.value != 0:
+        return (1, None)
+    return (0, flux.cpu().numpy())

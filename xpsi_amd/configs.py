@@ -1,0 +1,53 @@
+"""Model descriptions of the benchmark workloads.
+
+config1: one circular blackbody spot on an oblate (AGM_14) star, M = 1.4 Msun,
+R_eq = 12 km, 300 Hz, inclination 60 deg, colatitude 45 deg, radius 0.4 rad,
+log10 T = 6.1; sqrt_num_cells 32, 200 rays, 64 leaves, 128 energies over
+0.3-10 keV, 32 phase bins, 100 channels with Gaussian redistribution
+(sigma = 5% E) and an effective-area bump at 1 keV; Poisson counts for 1e6 s
+plus 5 background counts per channel and bin (numpy default_rng(42)).
+
+The response recipe (assumptions recorded alongside the data):
+  input bins: edges logspace(log10 0.3, log10 10, N_in + 1) keV;
+  channels: edges logspace(log10 0.3, log10 10, N_ch + 1) keV;
+  M[c, e] = A(E_e) * (Phi((hi_c - E_e)/(0.05 E_e)) - Phi((lo_c - E_e)/(0.05 E_e)))
+  with E_e the input-bin midpoint and A(E) = 1000 cm^2 exp(-(log10 E)^2 / (2 * 0.25^2)).
+"""
+import math
+import os
+
+import numpy as np
+from scipy.special import ndtr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def gaussian_response(N_ch, N_in, lo=0.3, hi=10.0, sigma_frac=0.05, area=1000.0, width=0.25):
+    in_edges = np.logspace(np.log10(lo), np.log10(hi), N_in + 1)
+    ch_edges = np.logspace(np.log10(lo), np.log10(hi), N_ch + 1)
+    Ee = 0.5 * (in_edges[:-1] + in_edges[1:])
+    A = area * np.exp(-np.log10(Ee) ** 2 / (2.0 * width ** 2))
+    sig = sigma_frac * Ee
+    M = A[None, :] * (ndtr((ch_edges[1:, None] - Ee[None, :]) / sig[None, :])
+                      - ndtr((ch_edges[:-1, None] - Ee[None, :]) / sig[None, :]))
+    return np.ascontiguousarray(M), in_edges, ch_edges
+
+
+def config1(counts=None):
+    M, in_edges, _ = gaussian_response(100, 128)
+    if counts is None:
+        path = os.path.join(GOLDEN, "config1.npz")
+        counts = np.load(path)["counts"] if os.path.exists(path) else np.zeros((100, 32))
+    return dict(
+        spacetime=dict(mass=1.4, radius=12.0, distance=1.0, cos_inclination=0.5,
+                       frequency=300.0, star_shape="AGM_14"),
+        hot_regions=[dict(super_colatitude=math.pi / 4.0, super_radius=0.4,
+                          super_temperature=6.1, phase_shift=0.0, sqrt_num_cells=32,
+                          min_sqrt_num_cells=10, max_sqrt_num_cells=80, num_rays=200,
+                          num_leaves=64, is_antiphased=False, image_order_limit=None)],
+        instrument=dict(matrix=M, energy_edges=in_edges),
+        data=dict(counts=counts, phases=np.linspace(0.0, 1.0, 33), exposure_time=1.0e6),
+        num_energies=128, support=None,
+        marginal=dict(epsabs=0.0, epsrel=1.0e-8, epsilon=1.0e-3, sigmas=10.0),
+        llzero=-1.0e90)

@@ -1,0 +1,551 @@
+// Host launch layer and C ABI (include/xpsi_hip.h).
+//
+// xh_pipeline owns every device buffer of a batch slot, sized once at
+// creation for the configured maxima; xh_pipeline_eval only fills the
+// per-request descriptors, launches the kernel sequence on one stream and
+// copies back one double per request.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "xh_common.hpp"
+#include "xh_kernels.hpp"
+#include "xh_types.hpp"
+#include "../../include/xpsi_hip.h"
+
+using namespace xh;
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) return fail(XH_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char* xh_last_error(void) { return g_err.c_str(); }
+
+// physical constants of xpsi/global_imports.py:67-88
+static const double kC = 2.99792458e8, kKPC = 3.08567758e19, kGM = 0.5 * 2.95325024e3,
+                    kG = 6.6740831e-11, kKM = 1.0e3;
+
+static int pick_ec(int N_L)
+{
+    // the profile and its three coefficient arrays for EC energies: <= 64 KiB
+    int ec = 64;
+    while (ec > 8 && (size_t)4 * N_L * ec * 8 > 65536) ec >>= 1;
+    return ec;
+}
+
+static size_t fold_lds(int N_L, int EC, int S_max)
+{
+    return ((size_t)N_L * EC + 3 * (size_t)(N_L - 1) * EC + 4 * (size_t)N_L + 2 * (size_t)S_max) * 8;
+}
+
+static size_t geo_lds(int N_R, int N_L) { return (5 * (size_t)N_R + 4 * (size_t)N_L) * 8 + (N_L / 2 + 2) * 4; }
+
+template <int EC>
+static hipError_t launch_fold_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)&k_profile_fold<EC, 16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_profile_fold<EC, 16>), g, dim3(256), lds, s, m, L, F);
+    return hipGetLastError();
+}
+
+static hipError_t launch_fold(int EC, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+{
+    switch (EC) {
+    case 64: return launch_fold_t<64>(g, lds, s, m, L, F);
+    case 32: return launch_fold_t<32>(g, lds, s, m, L, F);
+    case 16: return launch_fold_t<16>(g, lds, s, m, L, F);
+    default: return launch_fold_t<8>(g, lds, s, m, L, F);
+    }
+}
+
+static hipError_t fill(double* p, size_t n, double v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+static hipError_t fill_int(int* p, size_t n, int v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+// ============================================================= drop-ins
+
+extern "C" size_t xh_integrate_workspace_bytes(int S, int N_R, int N_L, int N_E, int N_P)
+{
+    (void)N_R; (void)N_E; (void)N_P;
+    const int IO = 8;
+    size_t b = 256;                                  // Member
+    b += 3 * (size_t)S * 8 + (size_t)S * 4;          // theta, logT, u(unused), ring_ok
+    b += (size_t)S * IO * 4 * N_L * 8;               // geo
+    b += (size_t)S * IO * 4;                         // valid
+    return b + 1024;
+}
+
+extern "C" int xh_integrate(double omega, double r_s, double inclination, int S, const double* cellArea,
+                            const double* radial, const double* r_s_over_r, const double* theta,
+                            const double* phi, const double* cellParams, const int* radiates, int N_R,
+                            const double* deflection, const double* cos_alpha, const double* lag,
+                            const double* maxDeflection, const double* cos_gamma, int N_E,
+                            const double* energies, int N_L, const double* leaves, int N_P,
+                            const double* phases, int hot_atm_ext, int image_order_limit, double* flux,
+                            void* workspace, size_t workspace_bytes, int* status_out, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (hot_atm_ext != 1) return fail(XH_EINVAL, "xh_integrate: only the blackbody atmosphere (1) is supported");
+    if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1) return fail(XH_EINVAL, "xh_integrate: bad sizes");
+    const int EC = pick_ec(N_L);
+    if (N_P > (256 / EC) * 16) return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
+    if (workspace_bytes < xh_integrate_workspace_bytes(S, N_R, N_L, N_E, N_P))
+        return fail(XH_EINVAL, "xh_integrate: workspace too small");
+    const int IO = 8;
+    char* w = (char*)workspace;
+    Member* d_m = (Member*)w; w += 256;
+    double* d_theta = (double*)w; w += (size_t)S * 8;
+    double* d_logT = (double*)w; w += (size_t)S * 8;
+    w += (size_t)S * 8;
+    int* d_ok = (int*)w; w += (size_t)S * 4;
+    w = (char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+    double* d_geo = (double*)w; w += (size_t)S * IO * 4 * N_L * 8;
+    int* d_valid = (int*)w;
+    Member hm{};
+    hm.Omega = omega; hm.r_s = r_s; hm.inclination = inclination; hm.active = 1; hm.S = S;
+    HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
+    Layout L{S, N_R, N_L, N_P, N_E, IO, image_order_limit > 0 ? image_order_limit : 0, 1};
+    hipLaunchKernelGGL(k_ring_prep, dim3(1), dim3(256), 0, s, S, theta, cellParams, radiates, d_theta, d_ok, d_logT);
+    HIPCHK(hipGetLastError());
+    RingArgs RA{d_theta, radial, r_s_over_r, cos_gamma, maxDeflection, deflection, cos_alpha, lag, leaves, d_ok,
+                d_geo, d_valid};
+    hipLaunchKernelGGL(k_geometry, dim3(S, IO, 1), dim3(256), geo_lds(N_R, N_L), s, d_m, L, RA);
+    HIPCHK(hipGetLastError());
+    FoldArgs FA{d_geo, d_valid, cellArea, radiates, phi, S, d_logT, energies, phases, flux, S};
+    HIPCHK(launch_fold(EC, dim3((N_E + EC - 1) / EC, 1, 1), fold_lds(N_L, EC, S), s, d_m, L, FA));
+    Member back{};
+    HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (back.status == 4) return fail(XH_ERANGE, "xh_integrate: more image orders than supported (8)");
+    *status_out = back.status ? 1 : 0;
+    return XH_OK;
+}
+
+extern "C" int xh_energy_integrator(int N_E, int N_P, const double* signal, const double* logE, int N_edges,
+                                    const double* logEdges, double* out, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (N_E < 5 || N_P < 1 || N_edges < 2) return fail(XH_EINVAL, "xh_energy_integrator: bad sizes");
+    // one "member" with d^2 = 1 carries the signal
+    static thread_local Member* d_m = nullptr;
+    static thread_local int* d_idx = nullptr;
+    if (!d_m) {
+        HIPCHK(hipMalloc(&d_m, sizeof(Member)));
+        HIPCHK(hipMalloc(&d_idx, 2 * sizeof(int)));
+        Member hm{};
+        hm.active = 1; hm.d_sq = 1.0;
+        int idx[2] = {0, -1};
+        HIPCHK(hipMemcpy(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_idx, idx, sizeof(idx), hipMemcpyHostToDevice));
+    }
+    EnergyArgs A{signal, logE, logEdges, nullptr, d_idx, out, N_E, N_P, N_edges, 2};
+    int n = (N_edges - 1) * N_P;
+    hipLaunchKernelGGL(k_energy_integrate, dim3((n + 255) / 256, 1), dim3(256), 0, s, d_m, A);
+    HIPCHK(hipGetLastError());
+    return XH_OK;
+}
+
+extern "C" int xh_fold(int N_ch, int N_in, int N_P, const double* M, int ld_M, const double* signal,
+                       double* out, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    FoldGemmArgs A{M, signal, out, N_ch, N_in, N_P, ld_M};
+    hipLaunchKernelGGL(k_fold_f64, dim3((N_P + 15) / 16, (N_ch + 15) / 16, 1), dim3(256), 0, s, A);
+    HIPCHK(hipGetLastError());
+    return XH_OK;
+}
+
+extern "C" int xh_precomputation(int N_ch, int N_b, const int* counts, double* out)
+{
+    for (int i = 0; i < N_ch; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < N_b; j++) acc += std::lgamma((double)(unsigned int)counts[(size_t)i * N_b + j] + 1.0);
+        out[i] = -acc;
+    }
+    return XH_OK;
+}
+
+extern "C" size_t xh_marginal_workspace_bytes(int N_ch) { return (size_t)N_ch * (8 + 4) + 64 + 256; }
+
+extern "C" int xh_eval_marginal_likelihood(double T, int N_b, const double* phases, int N_ch, const double* counts,
+                                           int n_comp, const double* comps, int N_P, const double* comp_phases,
+                                           const double* shifts, const double* precomp, const double* support,
+                                           double epsabs, double epsrel, double epsilon, double sigmas,
+                                           double llzero, int allow_negative, double slim,
+                                           const double* background, double* lnl_out, double* STAR,
+                                           double* B_ML, double* B_ML_supp, void* workspace,
+                                           size_t workspace_bytes, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (workspace_bytes < xh_marginal_workspace_bytes(N_ch)) return fail(XH_EINVAL, "workspace too small");
+    char* w = (char*)workspace;
+    double* d_ll = (double*)w; w += (size_t)N_ch * 8;
+    double* d_lnl = (double*)w; w += 64;
+    int* d_st = (int*)w;
+    MarginalArgs A{};
+    A.signals = comps; A.comp_phases = comp_phases; A.shifts = shifts; A.phases = phases; A.counts = counts;
+    A.precomp = precomp; A.support = support; A.background = background; A.request_status = nullptr;
+    A.ll = d_ll; A.status = d_st; A.B_ML = B_ML; A.B_ML_supp = B_ML_supp; A.STAR = STAR;
+    A.exposure = T; A.epsabs = epsabs; A.epsrel = epsrel; A.epsilon = epsilon; A.sigmas = sigmas;
+    A.llzero = llzero; A.slim = slim; A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp;
+    A.allow_negative = allow_negative; A.seed = 0;
+    HIPCHK(fill(STAR, (size_t)N_ch * N_b, 0.0, s));
+    HIPCHK(fill(B_ML, N_ch, 0.0, s));
+    HIPCHK(fill(B_ML_supp, N_ch, 0.0, s));
+    hipLaunchKernelGGL(k_marginal, dim3(N_ch, 1), dim3(64), 2 * N_b * 8, s, A);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_lnl_reduce, dim3(1), dim3(64), 0, s, A, (const Member*)nullptr, 0, d_lnl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(lnl_out, d_lnl, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return XH_OK;
+}
+
+// ============================================================= pipeline
+
+struct xh_pipeline {
+    xh_model_config cfg;
+    Layout L;
+    int nreg, mpr, maxb, N_in, N_ch, N_b, EC, table_n;
+    std::vector<void*> allocs;
+    Member* d_members;
+    RegionAlloc* d_regions;
+    double *d_area_tab, *d_total, *d_theta, *d_phi, *d_r, *d_cellArea, *d_maxAlpha, *d_cos_gamma, *d_effGrav;
+    double *d_u, *d_logT, *d_maxDefl, *d_defl, *d_cosa, *d_lag, *d_geo, *d_flux;
+    int *d_ring_ok, *d_valid, *d_member_index, *d_status;
+    double *d_energies, *d_logE, *d_logEdges, *d_atten, *d_leaves, *d_phases, *d_comp_phases;
+    double *d_integrated, *d_M, *d_folded, *d_data_phases, *d_counts, *d_precomp, *d_support, *d_shifts;
+    double *d_ll, *d_bml, *d_bmls, *d_star, *d_lnl;
+    Member* h_members;
+    RegionAlloc* h_regions;
+    double *h_shifts, *h_lnl;
+    hipStream_t stream;
+    int last_n;
+};
+
+template <class T>
+static int dalloc(xh_pipeline* p, T** ptr, size_t n)
+{
+    void* v = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&v, n * sizeof(T));
+    if (e != hipSuccess) return fail(XH_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    p->allocs.push_back(v);
+    *ptr = (T*)v;
+    return XH_OK;
+}
+
+extern "C" void xh_pipeline_destroy(xh_pipeline* p)
+{
+    if (!p) return;
+    for (void* v : p->allocs) (void)hipFree(v);
+    if (p->h_members) (void)hipHostFree(p->h_members);
+    if (p->h_regions) (void)hipHostFree(p->h_regions);
+    if (p->h_shifts) (void)hipHostFree(p->h_shifts);
+    if (p->h_lnl) (void)hipHostFree(p->h_lnl);
+    delete p;
+}
+
+#define PCHK(x) do { if ((x) != XH_OK) { xh_pipeline_destroy(p); return nullptr; } } while (0)
+#define PHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { fail(XH_EHIP, hipGetErrorString(e_)); xh_pipeline_destroy(p); return nullptr; } } while (0)
+
+extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
+{
+    if (!c || c->n_regions < 1 || c->n_regions > 2 || c->max_batch < 1 || c->N_E < 5 || c->num_leaves < 5 ||
+        c->num_rays < 3 || c->N_in < 1 || c->N_ch < 1 || c->N_b < 1) {
+        fail(XH_EINVAL, "xh_pipeline_create: bad configuration");
+        return nullptr;
+    }
+    xh_pipeline* p = new xh_pipeline();
+    std::memset((void*)p, 0, sizeof(*p) - sizeof(p->allocs) - sizeof(p->stream) - sizeof(p->last_n));
+    new (&p->allocs) std::vector<void*>();
+    p->cfg = *c;
+    p->nreg = c->n_regions;
+    p->mpr = 2 * c->n_regions;
+    p->maxb = c->max_batch;
+    p->N_in = c->N_in; p->N_ch = c->N_ch; p->N_b = c->N_b;
+    p->table_n = 1000;
+    int smax = c->max_sqrt_cells;
+    for (int r = 0; r < c->n_regions; r++) smax = std::max(smax, c->region[r].max_sqrt_num_cells);
+    smax += smax & 1;
+    const int N_P = c->num_phases > 0 ? c->num_phases : c->num_leaves;
+    const int IO = c->image_order_limit > 0 ? c->image_order_limit : 4;
+    p->L = Layout{smax, c->num_rays, c->num_leaves, N_P, c->N_E, IO, c->image_order_limit > 0 ? c->image_order_limit : 0,
+                  c->atm_ext};
+    p->EC = pick_ec(c->num_leaves);
+    if (N_P > (256 / p->EC) * 16) { fail(XH_EINVAL, "too many phases for the leaf count"); delete p; return nullptr; }
+    if (c->atm_ext != 1) { fail(XH_EINVAL, "pipeline: only the blackbody atmosphere is supported"); delete p; return nullptr; }
+    const size_t nm = (size_t)p->maxb * p->mpr, S = smax, NR = c->num_rays, NE = c->N_E;
+    const size_t ng = (size_t)p->maxb * p->nreg;
+    PCHK(dalloc(p, &p->d_members, nm));
+    PCHK(dalloc(p, &p->d_regions, ng));
+    PCHK(dalloc(p, &p->d_area_tab, nm * p->table_n));
+    PCHK(dalloc(p, &p->d_total, nm));
+    PCHK(dalloc(p, &p->d_theta, nm * S));
+    PCHK(dalloc(p, &p->d_phi, nm * S));
+    PCHK(dalloc(p, &p->d_r, nm * S));
+    PCHK(dalloc(p, &p->d_cellArea, nm * S * S));
+    PCHK(dalloc(p, &p->d_maxAlpha, nm * S));
+    PCHK(dalloc(p, &p->d_cos_gamma, nm * S));
+    PCHK(dalloc(p, &p->d_effGrav, nm * S));
+    PCHK(dalloc(p, &p->d_u, nm * S));
+    PCHK(dalloc(p, &p->d_logT, nm * S));
+    PCHK(dalloc(p, &p->d_maxDefl, nm * S));
+    PCHK(dalloc(p, &p->d_ring_ok, nm * S));
+    PCHK(dalloc(p, &p->d_defl, nm * S * NR));
+    PCHK(dalloc(p, &p->d_cosa, nm * S * NR));
+    PCHK(dalloc(p, &p->d_lag, nm * S * NR));
+    PCHK(dalloc(p, &p->d_geo, nm * S * IO * 4 * c->num_leaves));
+    PCHK(dalloc(p, &p->d_valid, nm * S * IO));
+    PCHK(dalloc(p, &p->d_flux, nm * NE * N_P));
+    PCHK(dalloc(p, &p->d_member_index, nm));
+    PCHK(dalloc(p, &p->d_energies, NE));
+    PCHK(dalloc(p, &p->d_logE, NE));
+    PCHK(dalloc(p, &p->d_logEdges, (size_t)c->N_in + 1));
+    PCHK(dalloc(p, &p->d_atten, (size_t)c->N_in));
+    PCHK(dalloc(p, &p->d_leaves, (size_t)c->num_leaves));
+    PCHK(dalloc(p, &p->d_phases, (size_t)N_P));
+    PCHK(dalloc(p, &p->d_comp_phases, (size_t)N_P));
+    PCHK(dalloc(p, &p->d_integrated, ng * c->N_in * N_P));
+    PCHK(dalloc(p, &p->d_M, (size_t)c->N_ch * c->N_in));
+    PCHK(dalloc(p, &p->d_folded, ng * c->N_ch * N_P));
+    PCHK(dalloc(p, &p->d_data_phases, (size_t)c->N_b + 1));
+    PCHK(dalloc(p, &p->d_counts, (size_t)c->N_ch * c->N_b));
+    PCHK(dalloc(p, &p->d_precomp, (size_t)c->N_ch));
+    PCHK(dalloc(p, &p->d_support, (size_t)c->N_ch * 2));
+    PCHK(dalloc(p, &p->d_shifts, ng));
+    PCHK(dalloc(p, &p->d_ll, (size_t)p->maxb * c->N_ch));
+    PCHK(dalloc(p, &p->d_status, (size_t)p->maxb * c->N_ch));
+    PCHK(dalloc(p, &p->d_bml, (size_t)p->maxb * c->N_ch));
+    PCHK(dalloc(p, &p->d_bmls, (size_t)p->maxb * c->N_ch));
+    PCHK(dalloc(p, &p->d_star, (size_t)p->maxb * c->N_ch * c->N_b));
+    PCHK(dalloc(p, &p->d_lnl, (size_t)p->maxb));
+    PHIP(hipHostMalloc((void**)&p->h_members, nm * sizeof(Member), hipHostMallocDefault));
+    PHIP(hipHostMalloc((void**)&p->h_regions, ng * sizeof(RegionAlloc), hipHostMallocDefault));
+    PHIP(hipHostMalloc((void**)&p->h_shifts, ng * sizeof(double), hipHostMallocDefault));
+    PHIP(hipHostMalloc((void**)&p->h_lnl, (size_t)p->maxb * sizeof(double), hipHostMallocDefault));
+    // constant inputs
+    std::vector<double> v(std::max<size_t>({NE, (size_t)c->N_in + 1, (size_t)c->num_leaves, (size_t)N_P}));
+    PHIP(hipMemcpy(p->d_energies, c->energies, NE * 8, hipMemcpyHostToDevice));
+    for (size_t i = 0; i < NE; i++) v[i] = std::log10(c->energies[i]);
+    PHIP(hipMemcpy(p->d_logE, v.data(), NE * 8, hipMemcpyHostToDevice));
+    for (int i = 0; i <= c->N_in; i++) v[i] = std::log10(c->energy_edges[i]);
+    PHIP(hipMemcpy(p->d_logEdges, v.data(), ((size_t)c->N_in + 1) * 8, hipMemcpyHostToDevice));
+    if (c->attenuation) PHIP(hipMemcpy(p->d_atten, c->attenuation, (size_t)c->N_in * 8, hipMemcpyHostToDevice));
+    // numpy.linspace(0, 2 pi, N_L) and 2 pi * numpy.linspace(0, 1, N_P)
+    const double two_pi = 2.0 * M_PI;
+    for (int i = 0; i < c->num_leaves; i++)
+        v[i] = (i == c->num_leaves - 1) ? two_pi : (double)i * (two_pi / (double)(c->num_leaves - 1)) + 0.0;
+    PHIP(hipMemcpy(p->d_leaves, v.data(), (size_t)c->num_leaves * 8, hipMemcpyHostToDevice));
+    std::vector<double> cyc(N_P);
+    for (int i = 0; i < N_P; i++) cyc[i] = (i == N_P - 1) ? 1.0 : (double)i * (1.0 / (double)(N_P - 1)) + 0.0;
+    PHIP(hipMemcpy(p->d_comp_phases, cyc.data(), (size_t)N_P * 8, hipMemcpyHostToDevice));
+    for (int i = 0; i < N_P; i++) v[i] = two_pi * cyc[i];
+    PHIP(hipMemcpy(p->d_phases, v.data(), (size_t)N_P * 8, hipMemcpyHostToDevice));
+    PHIP(hipMemcpy(p->d_M, c->matrix, (size_t)c->N_ch * c->N_in * 8, hipMemcpyHostToDevice));
+    PHIP(hipMemcpy(p->d_data_phases, c->data_phases, ((size_t)c->N_b + 1) * 8, hipMemcpyHostToDevice));
+    PHIP(hipMemcpy(p->d_counts, c->counts, (size_t)c->N_ch * c->N_b * 8, hipMemcpyHostToDevice));
+    std::vector<int> ic((size_t)c->N_ch * c->N_b);
+    for (size_t i = 0; i < ic.size(); i++) ic[i] = (int)c->counts[i];
+    std::vector<double> pre(c->N_ch);
+    xh_precomputation(c->N_ch, c->N_b, ic.data(), pre.data());
+    PHIP(hipMemcpy(p->d_precomp, pre.data(), (size_t)c->N_ch * 8, hipMemcpyHostToDevice));
+    std::vector<double> sup((size_t)c->N_ch * 2);
+    for (int i = 0; i < c->N_ch; i++) {
+        sup[2 * i] = c->support ? c->support[2 * i] : 0.0;
+        sup[2 * i + 1] = c->support ? c->support[2 * i + 1] : -1.0;
+    }
+    PHIP(hipMemcpy(p->d_support, sup.data(), sup.size() * 8, hipMemcpyHostToDevice));
+    std::vector<int> midx(nm);
+    for (size_t i = 0; i < nm; i++) midx[i] = (int)i;
+    PHIP(hipMemcpy(p->d_member_index, midx.data(), nm * sizeof(int), hipMemcpyHostToDevice));
+    return p;
+}
+
+// host-side spacetime and descriptors for one request (Spacetime.py:111-190,
+// HotRegion.py:774-871)
+static void describe(const xh_pipeline* p, int rq, const xh_request& r)
+{
+    const xh_model_config& c = p->cfg;
+    const double r_g = r.mass * kGM;
+    const double Msi = r.mass * kGM * (kC * kC) / kG;
+    const double R = r.radius * kKM;
+    const double Omega = 2.0 * M_PI * c.frequency;
+    const double d = r.distance * kKPC;
+    const double zeta = r_g / R;
+    const double eps = std::pow(Omega, 2.0) * std::pow(R, 3.0) / (kG * Msi);
+    const double incl = std::acos(r.cos_inclination);
+    for (int g = 0; g < p->nreg; g++) {
+        const double* q = r.region[g];
+        const double shift = q[0], sc = q[1], sr = q[2], sT = q[3];
+        const bool cede = c.region[g].cede != 0;
+        const double cc = cede ? q[4] : sc, cr = cede ? q[5] : 0.0, ca = cede ? q[6] : 0.0, cT = q[7];
+        const int gi = rq * p->nreg + g;
+        const int m0 = gi * 2, m1 = gi * 2 + 1;
+        for (int s = 0; s < 2; s++) {
+            Member& m = p->h_members[gi * 2 + s];
+            std::memset((void*)&m, 0, sizeof(Member));
+            m.R_eq = R; m.epsilon = eps; m.zeta = zeta; m.Omega = Omega; m.r_s = 2.0 * r_g;
+            m.inclination = incl; m.mode_freq = c.mode_frequency; m.d_sq = d * d; m.shape = c.star_shape;
+            m.antiphased = c.region[g].is_antiphased;
+            if (s == 0) {
+                m.active = 1;
+                m.cedeR = sr; m.cedeColat = sc; m.superR = 0.0; m.superColat = sc; m.superAzi = 0.0;
+                m.phi_offset = -0.0;
+                m.polar = (sc - sr < 0.0) || (sc + sr > M_PI);
+                m.logT = sT;
+            } else {
+                m.active = cede && cr > 0.0;
+                m.cedeR = cr; m.cedeColat = cc; m.superR = sr; m.superColat = sc; m.superAzi = -ca;
+                m.phi_offset = ca;
+                m.polar = (cc - cr < 0.0) || (cc + cr > M_PI);
+                m.logT = cT;
+            }
+        }
+        RegionAlloc& A = p->h_regions[gi];
+        int snc = r.sqrt_num_cells[g] > 0 ? r.sqrt_num_cells[g] : c.region[g].sqrt_num_cells;
+        A.superR = sr; A.cedeR = cr; A.superColat = sc; A.cedeColat = cc; A.superAzi = -ca;
+        A.holeColat = sc; A.holeR = 0.0; A.holeAzi = 0.0;
+        A.R_eq = R; A.Omega = Omega; A.zeta = zeta; A.epsilon = eps; A.shape = c.star_shape;
+        A.num_cells = snc * snc; A.min_sqrt = c.region[g].min_sqrt_num_cells;
+        A.max_sqrt = c.region[g].max_sqrt_num_cells;
+        A.member0 = m0; A.member1 = (cede && cr > 0.0) ? m1 : -1;
+        p->h_shifts[gi] = shift;
+    }
+}
+
+extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs, void* stream)
+{
+    if (!p || n < 1 || n > p->maxb) return fail(XH_EINVAL, "xh_pipeline_launch: batch size out of range");
+    hipStream_t s = (hipStream_t)stream;
+    p->stream = s;
+    p->last_n = n;
+    for (int rq = 0; rq < n; rq++) describe(p, rq, reqs[rq]);
+    const Layout L = p->L;
+    const int nm = n * p->mpr, ng = n * p->nreg;
+    const int S = L.S_max, N_P = L.N_P, N_E = L.N_E;
+    HIPCHK(hipMemcpyAsync(p->d_members, p->h_members, (size_t)nm * sizeof(Member), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(p->d_regions, p->h_regions, (size_t)ng * sizeof(RegionAlloc), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(p->d_shifts, p->h_shifts, (size_t)ng * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(fill(p->d_cellArea, (size_t)nm * S * S, 0.0, s));
+    HIPCHK(fill(p->d_flux, (size_t)nm * N_E * N_P, 0.0, s));
+    HIPCHK(fill_int(p->d_ring_ok, (size_t)nm * S, 0, s));
+    HIPCHK(fill_int(p->d_valid, (size_t)nm * S * L.IO_max, 0, s));
+    hipLaunchKernelGGL(k_allocate, dim3(ng), dim3(256), 0, s, p->d_regions, p->d_members, S);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mesh_table, dim3((p->table_n + 1 + 3) / 4, nm), dim3(256), 0, s, p->d_members,
+                       p->d_area_tab, p->d_total, p->table_n);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mesh_rings, dim3(S, nm), dim3(64), 0, s, p->d_members, p->d_area_tab, p->d_total,
+                       p->table_n, L, p->d_theta, p->d_phi, p->d_r, p->d_cellArea, p->d_maxAlpha,
+                       p->d_cos_gamma, p->d_effGrav);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_ring_finalize, dim3(nm), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_cellArea,
+                       p->d_u, p->d_ring_ok, p->d_logT);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_rays, dim3((L.N_R - 1 + 3) / 4, S, nm), dim3(256), 0, s, p->d_members, L, p->d_r,
+                       p->d_maxAlpha, p->d_cosa, p->d_defl, p->d_lag);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_rays_fix, dim3(nm), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_maxAlpha,
+                       p->d_defl, p->d_lag, p->d_maxDefl);
+    HIPCHK(hipGetLastError());
+    RingArgs RA{p->d_theta, p->d_r, p->d_u, p->d_cos_gamma, p->d_maxDefl, p->d_defl, p->d_cosa, p->d_lag,
+                p->d_leaves, p->d_ring_ok, p->d_geo, p->d_valid};
+    hipLaunchKernelGGL(k_geometry, dim3(S, L.IO_max, nm), dim3(256), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
+    HIPCHK(hipGetLastError());
+    // parallels per block: all of them when the batch fills the GPU
+    int rb = (n >= 32) ? S : 1;
+    FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_energies, p->d_phases,
+                p->d_flux, rb};
+    HIPCHK(launch_fold(p->EC, dim3((N_E + p->EC - 1) / p->EC, (S + rb - 1) / rb, nm), fold_lds(L.N_L, p->EC, S), s,
+                       p->d_members, L, FA));
+    EnergyArgs EA{p->d_flux, p->d_logE, p->d_logEdges, p->cfg.attenuation ? p->d_atten : nullptr,
+                  p->d_member_index, p->d_integrated, N_E, N_P, p->N_in + 1, 2};
+    hipLaunchKernelGGL(k_energy_integrate, dim3((p->N_in * N_P + 255) / 256, ng), dim3(256), 0, s, p->d_members, EA);
+    HIPCHK(hipGetLastError());
+    FoldGemmArgs GA{p->d_M, p->d_integrated, p->d_folded, p->N_ch, p->N_in, N_P, p->N_in};
+    hipLaunchKernelGGL(k_fold_f64, dim3((N_P + 15) / 16, (p->N_ch + 15) / 16, ng), dim3(256), 0, s, GA);
+    HIPCHK(hipGetLastError());
+    MarginalArgs MA{};
+    MA.signals = p->d_folded; MA.comp_phases = p->d_comp_phases; MA.shifts = p->d_shifts;
+    MA.phases = p->d_data_phases; MA.counts = p->d_counts; MA.precomp = p->d_precomp; MA.support = p->d_support;
+    MA.background = nullptr; MA.request_status = nullptr; MA.ll = p->d_ll; MA.status = p->d_status;
+    MA.B_ML = p->d_bml; MA.B_ML_supp = p->d_bmls; MA.STAR = p->d_star;
+    MA.exposure = p->cfg.exposure_time; MA.epsabs = p->cfg.epsabs; MA.epsrel = p->cfg.epsrel;
+    MA.epsilon = p->cfg.epsilon; MA.sigmas = p->cfg.sigmas; MA.llzero = p->cfg.llzero; MA.slim = p->cfg.slim;
+    MA.N_ch = p->N_ch; MA.N_b = p->N_b; MA.N_P = N_P; MA.n_comp = p->nreg; MA.allow_negative = 0; MA.seed = 0;
+    hipLaunchKernelGGL(k_marginal, dim3(p->N_ch, n), dim3(64), 2 * p->N_b * 8, s, MA);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_lnl_reduce, dim3(n), dim3(64), 0, s, MA, (const Member*)p->d_members, p->mpr, p->d_lnl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(p->h_lnl, p->d_lnl, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    return XH_OK;
+}
+
+extern "C" int xh_pipeline_sync(xh_pipeline* p)
+{
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return XH_OK;
+}
+
+extern "C" const double* xh_pipeline_results(xh_pipeline* p) { return p->h_lnl; }
+
+extern "C" int xh_pipeline_eval(xh_pipeline* p, int n, const xh_request* reqs, double* lnl_out, void* stream)
+{
+    int rc = xh_pipeline_launch(p, n, reqs, stream);
+    if (rc != XH_OK) return rc;
+    rc = xh_pipeline_sync(p);
+    if (rc != XH_OK) return rc;
+    std::memcpy(lnl_out, p->h_lnl, (size_t)n * 8);
+    return XH_OK;
+}
+
+extern "C" int xh_pipeline_fetch(xh_pipeline* p, int what, int req, int index, double* out, size_t n)
+{
+    const Layout L = p->L;
+    const size_t S = L.S_max;
+    const int mi = req * p->mpr + index;
+    const double* src = nullptr;
+    size_t cnt = 0;
+    Member m{};
+    HIPCHK(hipMemcpy(&m, p->d_members + mi, sizeof(Member), hipMemcpyDeviceToHost));
+    switch (what) {
+    case 0: src = p->d_flux + (size_t)mi * L.N_E * L.N_P; cnt = (size_t)L.N_E * L.N_P; break;
+    case 1: src = p->d_folded + ((size_t)req * p->nreg + index) * p->N_ch * L.N_P; cnt = (size_t)p->N_ch * L.N_P; break;
+    case 2: out[0] = m.S; if (n > 1) out[1] = m.status; if (n > 2) out[2] = m.numCell; return XH_OK;
+    case 3: src = p->d_cellArea + (size_t)mi * S * S; cnt = (size_t)m.S * m.S; break;
+    case 4: src = p->d_defl + (size_t)mi * S * L.N_R; cnt = (size_t)m.S * L.N_R; break;
+    case 5: src = p->d_lag + (size_t)mi * S * L.N_R; cnt = (size_t)m.S * L.N_R; break;
+    case 6: src = p->d_theta + (size_t)mi * S; cnt = m.S; break;
+    case 7: src = p->d_phi + (size_t)mi * S; cnt = m.S; break;
+    case 8: src = p->d_star + (size_t)req * p->N_ch * p->N_b; cnt = (size_t)p->N_ch * p->N_b; break;
+    default: return fail(XH_EINVAL, "xh_pipeline_fetch: unknown array");
+    }
+    if (cnt > n) cnt = n;
+    HIPCHK(hipMemcpy(out, src, cnt * 8, hipMemcpyDeviceToHost));
+    return XH_OK;
+}

@@ -1,0 +1,343 @@
+// Pulse integration over a hot-region mesh exploiting azimuthal invariance.
+//
+// Reference behaviour: xpsi/cellmesh/integrator_for_azimuthal_invariance.pyx
+// :70-665 (the leaf x energy loop :342-549 and the energy x cell x phase
+// spline loop :565-596), blackbody intensity hot_BB.pyx:54-98.
+//
+// k_geometry   one block per (parallel, image order, member): ray-table
+//              lookups (Steffen), Doppler factors and the invisible-phase
+//              padding of the leaf phases; writes PHASE/Z/mu/GEOM per leaf.
+// k_profile_fold one block per (energy chunk, parallel block, member): the
+//              specific-intensity profile and its periodic Akima coefficients
+//              are staged in LDS (interval-major, energy-minor) and every lane
+//              folds one energy over the parallel's cells at its phases,
+//              accumulating in registers.
+#include "xh_common.hpp"
+#include "xh_types.hpp"
+#include "xh_kernels.hpp"
+
+namespace xh {
+
+__device__ __forceinline__ double image_deflection(int order, double psi)
+{
+    double sgn = (order % 2 == 0) ? 1.0 : -1.0;
+    if (order % 2 != 0) return (double)(order + 1) * PI + sgn * psi;
+    return (double)order * PI + sgn * psi;
+}
+
+struct LeafGeo {
+    int calc;
+    double phase_lag, mu, sin_alpha, deriv, sin_psi, psi;
+};
+
+// geometry of leaf k for image order I (integrator :343-414)
+__device__ LeafGeo leaf_geometry(int I, double leaf, double inclination, double cos_i, double sin_i,
+                                 double cos_th, double sin_th, double th_over_pi, double cos_gamma,
+                                 double sin_gamma, double maxD, const double* D, const double* A,
+                                 const double* Lg, int N_R, const double* xalt, const double* yalt,
+                                 int nalt, bool need_all)
+{
+    LeafGeo g{};
+    double cos_psi = cos_i * cos_th + sin_i * sin_th * cos(leaf);
+    double psi = image_deflection(I, acos(cos_psi));
+    double sin_psi = sin(psi);
+    if (!eq12(psi, 0.0) && eq12(sin_psi, 0.0)) {
+        double ii = (cos_i >= 0.0) ? inclination + inclination * 1.0e-6 : inclination - inclination * 1.0e-6;
+        cos_psi = cos(ii) * cos_th + sin(ii) * sin_th * cos(leaf);
+        psi = image_deflection(I, acos(cos_psi));
+        sin_psi = sin(psi);
+    }
+    g.calc = 0;
+    g.psi = psi;
+    g.sin_psi = sin_psi;
+    if (!(psi <= maxD)) return g;
+    bool alt = (psi <= HALF_PI && cos_psi >= xalt[0]);
+    double ca = alt ? steffen_eval(xalt, yalt, nalt, cos_psi, nullptr) : steffen_eval(D, A, N_R, psi, nullptr);
+    double sa = sqrt(1.0 - ca * ca);
+    double mu = ca * cos_gamma;
+    if (!eq12(psi, 0.0)) {
+        double cos_delta = (cos_i - cos_th * cos_psi) / (sin_th * sin_psi);
+        if (th_over_pi < 0.5) mu = mu + sa * sin_gamma * cos_delta;
+        else mu = mu - sa * sin_gamma * cos_delta;
+    }
+    g.calc = mu > 0.0 ? 1 : 0;
+    if (!g.calc || !need_all) return g;
+    g.mu = mu;
+    g.sin_alpha = sa;
+    double deriv;
+    if (alt) {
+        steffen_eval(xalt, yalt, nalt, cos_psi, &deriv);
+    } else {
+        steffen_eval(D, A, N_R, psi, &deriv);
+        deriv = exp(log(fabs(deriv)) - log(fabs(sin_psi)));
+    }
+    g.deriv = deriv;
+    g.phase_lag = steffen_eval(D, Lg, N_R, psi, nullptr);
+    return g;
+}
+
+
+__global__ void __launch_bounds__(256) k_geometry(Member* __restrict__ members, Layout L, RingArgs A)
+{
+    extern __shared__ double sm[];
+    const int i = blockIdx.x, I = blockIdx.y, mi = blockIdx.z, tid = threadIdx.x;
+    Member* mp = members + mi;
+    const Member m = *mp;
+    const size_t vix = ((size_t)mi * L.S_max + i) * L.IO_max + I;
+    if (!m.active || m.status) return;
+    if (i >= m.S) return;
+    const size_t ro = (size_t)mi * L.S_max + i;
+    if (!A.ring_ok[ro]) { if (tid == 0) A.valid[vix] = 0; return; }
+    const double maxD = A.maxDefl[ro];
+    const int IO = L.image_limit > 0 ? L.image_limit : (int)ceil(maxD / PI);
+    if (I == 0 && tid == 0 && IO > L.IO_max) atomicMax(&mp->status, 4);
+    if (I >= IO) { if (tid == 0) A.valid[vix] = 0; return; }
+    const int N_R = L.N_R, N_L = L.N_L;
+    double* D = sm;
+    double* Ac = D + N_R;
+    double* Lg = Ac + N_R;
+    double* xalt = Lg + N_R;
+    double* yalt = xalt + N_R;
+    double* PH = yalt + N_R;            // N_L
+    double* Zs = PH + N_L;              // N_L
+    double* As = Zs + N_L;              // N_L
+    double* Gs = As + N_L;              // N_L
+    int* vis = (int*)(Gs + N_L);        // N_L/2+1
+    __shared__ int jn_s, any_vis, term;
+    const size_t tb = ro * N_R;
+    for (int q = tid; q < N_R; q += blockDim.x) {
+        D[q] = A.defl[tb + q];
+        Ac[q] = A.cos_alpha[tb + q];
+        Lg[q] = A.lag[tb + q];
+    }
+    if (tid == 0) { jn_s = N_R - 1; term = 0; }
+    __syncthreads();
+    for (int q = tid; q < N_R; q += blockDim.x)
+        if (D[q] > HALF_PI) atomicMin(&jn_s, q);
+    __syncthreads();
+    const int jn = jn_s, nalt = jn + 1;
+    for (int q = tid; q < nalt; q += blockDim.x) {
+        xalt[q] = cos(D[jn - q]);
+        yalt[q] = Ac[jn - q];
+    }
+    const double incl = m.inclination, cos_i = cos(incl), sin_i = sin(incl);
+    const double th = A.theta[ro], cos_th = cos(th), sin_th = sin(th), thp = th / PI;
+    const double cg = A.cos_gamma[ro], sg = sqrt(1.0 - cg * cg);
+    const double radius = A.r[ro], Grav_z = sqrt(1.0 - A.u[ro]);
+    const double beta = radius * m.Omega * sin_th / (C_SI * Grav_z);
+    const double Lorentz = sqrt(1.0 - beta * beta);
+    const int leaf_lim = (N_L % 2 == 0) ? N_L / 2 : (N_L + 1) / 2;
+    const double* leaves = A.leaves;
+    __syncthreads();
+    // an image order contributes only if every lower order has a visible leaf
+    for (int Ip = 0; Ip < I; Ip++) {
+        if (tid == 0) any_vis = 0;
+        __syncthreads();
+        for (int k = tid; k < leaf_lim; k += blockDim.x) {
+            LeafGeo g = leaf_geometry(Ip, leaves[k], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg,
+                                      maxD, D, Ac, Lg, N_R, xalt, yalt, nalt, false);
+            if (g.calc) any_vis = 1;
+        }
+        __syncthreads();
+        if (!any_vis) { if (tid == 0) A.valid[vix] = 0; return; }
+        __syncthreads();
+    }
+    for (int k = tid; k < N_L; k += blockDim.x) { Gs[k] = 0.0; Zs[k] = 1.0; As[k] = 0.0; }
+    __syncthreads();
+    for (int k = tid; k < leaf_lim; k += blockDim.x) {
+        LeafGeo g = leaf_geometry(I, leaves[k], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg, maxD,
+                                  D, Ac, Lg, N_R, xalt, yalt, nalt, true);
+        vis[k] = g.calc;
+        if (!g.calc) continue;
+        for (int ks = 0; ks < 2; ks++) {
+            bool take = (0 < k && k < leaf_lim - 1) || (k == 0 && ks == 0) ||
+                        (k == leaf_lim - 1 && N_L % 2 == 1 && ks == 0) ||
+                        (k == leaf_lim - 1 && N_L % 2 == 0);
+            if (!take) continue;
+            int kd = ks == 0 ? k : N_L - 1 - k;
+            double superlum, eta;
+            if (!eq12(g.psi, 0.0)) {
+                double cos_xi = g.sin_alpha * sin_i * sin(leaves[kd]) / g.sin_psi;
+                superlum = 1.0 + beta * cos_xi;
+                eta = Lorentz / superlum;
+            } else {
+                superlum = 1.0;
+                eta = Lorentz;
+            }
+            Zs[kd] = eta * Grav_z;
+            As[kd] = g.mu * eta;
+            Gs[kd] = g.mu * fabs(g.deriv) * Grav_z * eta * eta * eta / superlum;
+            PH[kd] = leaves[kd] + g.phase_lag;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // invisible-phase padding, in the reference's leaf order (:480-549)
+        int invis = 2, invis_phase = 0;
+        for (int k = 0; k < leaf_lim; k++) {
+            if (vis[k]) {
+                if (k == 0) {
+                    PH[N_L - 1] = PH[0] + TWO_PI;
+                    Zs[N_L - 1] = Zs[0]; As[N_L - 1] = As[0]; Gs[N_L - 1] = Gs[0];
+                } else if (invis == 2) {
+                    double step = leaves[k] / (double)k;
+                    for (int q = N_L - k; q < N_L; q++) PH[q] = PH[q - 1] + step;
+                    PH[0] = PH[N_L - 1] - TWO_PI;
+                    for (int q = 1; q < k; q++) PH[q] = PH[q - 1] + step;
+                } else if (invis == 1) {
+                    double step = (PH[k] - PH[invis_phase - 1]) / (double)(k - invis_phase + 1);
+                    for (int q = invis_phase; q < k; q++) PH[q] = PH[q - 1] + step;
+                    step = (PH[N_L - invis_phase] - PH[N_L - 1 - k]) / (double)(k - invis_phase + 1);
+                    for (int q = N_L - k; q < N_L - invis_phase; q++) PH[q] = PH[q - 1] + step;
+                }
+                invis = 0;
+            } else if (invis == 0) {
+                double step = (PH[N_L - k] - PH[k - 1]) / (double)(N_L - 2 * k + 1);
+                for (int q = k; q < N_L - k; q++) PH[q] = PH[q - 1] + step;
+                invis = 1;
+                invis_phase = k;
+            }
+        }
+        if (invis == 2) {
+            A.valid[vix] = 0;
+        } else {
+            for (int q = 1; q < N_L; q++)
+                if (PH[q] <= PH[q - 1]) { term = 1; break; }
+            A.valid[vix] = term ? 0 : 1;
+            if (term) atomicMax(&mp->status, 2);
+        }
+    }
+    __syncthreads();
+    double* out = A.geo + vix * 4 * N_L;
+    for (int q = tid; q < N_L; q += blockDim.x) {
+        out[q] = PH[q];
+        out[N_L + q] = Zs[q];
+        out[2 * N_L + q] = As[q];
+        out[3 * N_L + q] = Gs[q];
+    }
+}
+
+
+template <int EC, int MAXK>
+__global__ void __launch_bounds__(256) k_profile_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
+{
+    extern __shared__ double sm[];
+    const int mi = blockIdx.z, tid = threadIdx.x;
+    const Member m = members[mi];
+    if (!m.active || m.status) return;
+    const int S = m.S, N_L = L.N_L, N_P = L.N_P, N_E = L.N_E;
+    const int e0 = blockIdx.x * EC;
+    const int r0 = blockIdx.y * F.ring_block;
+    if (r0 >= S) return;
+    const int r1 = min(S, r0 + F.ring_block);
+    constexpr int PPAR = 256 / EC;
+    double* Y = sm;                       // [N_L][EC]
+    double* Bc = Y + N_L * EC;            // [N_L-1][EC]
+    double* Cc = Bc + (N_L - 1) * EC;
+    double* Dc = Cc + (N_L - 1) * EC;
+    double* PH = Dc + (N_L - 1) * EC;     // N_L
+    double* Zs = PH + N_L;
+    double* As = Zs + N_L;
+    double* Gs = As + N_L;
+    double* cphi = Gs + N_L;              // S_max
+    double* carea = cphi + L.S_max;       // S_max
+    const int p = tid % EC, slot = tid / EC;
+    const int pe = e0 + p;
+    const bool p_ok = pe < N_E;
+    const double E = p_ok ? F.energies[pe] : 1.0;
+    double ph[MAXK], acc[MAXK];
+#pragma unroll
+    for (int q = 0; q < MAXK; q++) {
+        int k = slot + q * PPAR;
+        ph[q] = k < N_P ? F.phases[k] : 0.0;
+        acc[q] = 0.0;
+    }
+    const size_t cbase = (size_t)mi * L.S_max * L.S_max;
+    for (int i = r0; i < r1; i++) {
+        const size_t ro = (size_t)mi * L.S_max + i;
+        __syncthreads();   // the previous parallel's fold is done with LDS
+        // cells of the parallel
+        for (int j = tid; j < S; j += 256) {
+            size_t c = cbase + (size_t)i * S + j;
+            double a = F.cellArea[c];
+            bool rad = F.radiates ? (F.radiates[c] == 1) : (a > 0.0);
+            carea[j] = rad ? a : __builtin_nan("");
+            cphi[j] = F.phi[(size_t)mi * L.S_max * (F.phi_row_stride ? L.S_max : 1) +
+                            (size_t)i * F.phi_row_stride + j];
+        }
+        const double kT = K_B_OVER_KEV * pow(10.0, F.logT[ro]);
+        for (int I = 0; I < L.IO_max; I++) {
+            const size_t vix = ro * L.IO_max + I;
+            if (!F.valid[vix]) break;
+            __syncthreads();
+            const double* g = F.geo + vix * 4 * N_L;
+            for (int q = tid; q < N_L; q += 256) {
+                PH[q] = g[q]; Zs[q] = g[N_L + q]; As[q] = g[2 * N_L + q]; Gs[q] = g[3 * N_L + q];
+            }
+            __syncthreads();
+            // specific-intensity profile (BB), zero where the leaf is hidden
+            for (int q = tid; q < N_L * EC; q += 256) {
+                int mm = q / EC, pp = q % EC;
+                double v = 0.0;
+                double geom = Gs[mm];
+                if (geom != 0.0 && e0 + pp < N_E) {
+                    double Ep = F.energies[e0 + pp] / Zs[mm];
+                    double I_E = Ep * Ep * Ep / (exp(Ep / kT) - 1.0);
+                    v = (I_E * BB_NORM - 0.0) * geom;
+                }
+                Y[q] = v;
+            }
+            __syncthreads();
+            // periodic Akima coefficients per interval and energy
+            for (int q = tid; q < (N_L - 1) * EC; q += 256) {
+                int mm = q / EC, pp = q % EC;
+                auto sec = [&](int k) {
+                    int kk = k < 0 ? k + (N_L - 1) : (k > N_L - 2 ? k - (N_L - 1) : k);
+                    return (Y[(kk + 1) * EC + pp] - Y[kk * EC + pp]) / (PH[kk + 1] - PH[kk]);
+                };
+                double b, c, d;
+                akima_piece(sec(mm - 2), sec(mm - 1), sec(mm), sec(mm + 1), sec(mm + 2),
+                            PH[mm + 1] - PH[mm], b, c, d);
+                Bc[q] = b; Cc[q] = c; Dc[q] = d;
+            }
+            __syncthreads();
+            const double lo = PH[0], hi = PH[N_L - 1];
+            for (int j = 0; j < S; j++) {
+                const double area = carea[j];
+                if (isnan(area)) continue;
+                const double sh = cphi[j];
+#pragma unroll
+                for (int q = 0; q < MAXK; q++) {
+                    int k = slot + q * PPAR;
+                    if (k >= N_P) break;
+                    double x = ph[q] + sh;
+                    int guard = 0;
+                    if (x > hi) { while (x > hi && guard++ < 64) x -= TWO_PI; }
+                    else if (x < lo) { while (x < lo && guard++ < 64) x += TWO_PI; }
+                    int idx = interval_of(PH, N_L, x);
+                    double t = x - PH[idx];
+                    int o = idx * EC + p;
+                    double v = Y[o] + t * (Bc[o] + t * (Cc[o] + t * Dc[o]));
+                    if (v > 0.0) acc[q] += area * v;
+                }
+            }
+        }
+    }
+    if (!p_ok) return;
+    double* fo = F.flux + (size_t)mi * N_E * N_P + (size_t)pe * N_P;
+    const double inv = 1.0 / (E * KEV);
+#pragma unroll
+    for (int q = 0; q < MAXK; q++) {
+        int k = slot + q * PPAR;
+        if (k >= N_P) break;
+        if (gridDim.y == 1) fo[k] = acc[q] / (E * KEV);
+        else atomicAdd(fo + k, acc[q] * inv);
+    }
+}
+
+template __global__ void k_profile_fold<64, 16>(const Member*, Layout, FoldArgs);
+template __global__ void k_profile_fold<32, 16>(const Member*, Layout, FoldArgs);
+template __global__ void k_profile_fold<16, 16>(const Member*, Layout, FoldArgs);
+template __global__ void k_profile_fold<8, 16>(const Member*, Layout, FoldArgs);
+
+}  // namespace xh

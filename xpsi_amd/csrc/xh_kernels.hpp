@@ -1,0 +1,92 @@
+// Kernel declarations and argument blocks shared by the launch code.
+#pragma once
+#include "xh_types.hpp"
+
+namespace xh {
+
+struct EnergyArgs {
+    const double* flux;         // [member][N_E][N_P]
+    const double* logE;         // N_E
+    const double* logEdges;     // N_edges
+    const double* atten;        // N_edges-1 or nullptr
+    const int* member_index;    // [region][members_per_region], -1 = none
+    double* out;                // [region][N_in][N_P]
+    int N_E, N_P, N_edges, members_per_region;
+};
+
+struct FoldGemmArgs {
+    const double* M;            // [N_ch][ld_M]
+    const double* in;           // [region][N_in][N_P]
+    double* out;                // [region][N_ch][N_P]
+    int N_ch, N_in, N_P, ld_M;
+};
+
+struct MarginalArgs {
+    const double* signals;      // [request][comp][N_ch][N_P]
+    const double* comp_phases;  // N_P (cycles)
+    const double* shifts;       // [request][comp]
+    const double* phases;       // N_b + 1
+    const double* counts;       // [N_ch][N_b]
+    const double* precomp;      // N_ch
+    const double* support;      // [N_ch][2]
+    const double* background;   // [N_ch][N_b] or nullptr
+    const int* request_status;  // [request] or nullptr
+    double* ll;                 // [request][N_ch]
+    int* status;                // [request][N_ch]
+    double* B_ML;               // [request][N_ch]
+    double* B_ML_supp;          // [request][N_ch]
+    double* STAR;               // [request][N_ch][N_b] or nullptr
+    double exposure, epsabs, epsrel, epsilon, sigmas, llzero, slim;
+    int N_ch, N_b, N_P, n_comp, allow_negative, seed;
+};
+
+__global__ void k_allocate(const RegionAlloc* regions, Member* members, int S_max);
+__global__ void k_mesh_table(const Member* members, double* area_tab, double* total, int table_n);
+__global__ void k_mesh_rings(const Member* members, const double* area_tab, const double* total,
+                             int table_n, Layout L, double* theta, double* phi, double* rr,
+                             double* cellArea, double* maxAlpha, double* cos_gamma, double* effGrav);
+__global__ void k_rays(const Member* members, Layout L, const double* rr, const double* maxAlpha_in,
+                       double* cos_alpha, double* defl_raw, double* lag_raw);
+__global__ void k_rays_fix(Member* members, Layout L, const double* rr, double* maxAlpha,
+                           double* defl, double* lag, double* maxDefl);
+__global__ void k_energy_integrate(const Member* members, EnergyArgs A);
+__global__ void k_fold_f64(FoldGemmArgs A);
+__global__ void k_marginal(MarginalArgs A);
+__global__ void k_lnl_reduce(MarginalArgs A, const Member* members, int members_per_request,
+                             double* lnl);
+
+}  // namespace xh
+
+namespace xh {
+__global__ void k_ring_finalize(const Member* members, Layout L, const double* rr,
+                                const double* cellArea, double* u, int* ring_ok, double* logT);
+__global__ void k_ring_prep(int S, const double* theta2d, const double* params, const int* radiates,
+                            double* theta, int* ring_ok, double* logT);
+// per-member, per-ring inputs
+struct RingArgs {
+    const double *theta, *r, *u, *cos_gamma, *maxDefl, *defl, *cos_alpha, *lag, *leaves;
+    const int* ring_ok;
+    double* geo;     // [member][ring][image][4][N_L]: PHASE, Z, ABB(mu eta), GEOM
+    int* valid;      // [member][ring][image]
+};
+
+struct FoldArgs {
+    const double* geo;
+    const int* valid;
+    const double* cellArea;   // [member][S_max*S_max] (row stride S)
+    const int* radiates;      // same layout or nullptr (then area > 0)
+    const double* phi;        // [member][...], see phi_row_stride
+    int phi_row_stride;       // 0: one azimuth row per member, else S
+    const double* logT;       // per member per ring [member][S_max]
+    const double* energies;   // N_E
+    const double* phases;     // N_P (radians)
+    double* flux;             // [member][N_E][N_P]
+    int ring_block;
+};
+
+__global__ void k_geometry(Member* members, Layout L, RingArgs A);
+template <int EC, int MAXK>
+__global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
+__global__ void k_fill(double* p, size_t n, double v);
+__global__ void k_fill_int(int* p, size_t n, int v);
+}

@@ -1,0 +1,571 @@
+// Hot-region cell mesh on the GPU.
+//
+// Reference behaviour: xpsi/cellmesh/mesh_tools.pyx:18-1099 (allocate_cells,
+// area/spot/cell integrals), mesh.pyx:23-417 (spot mesh), polar_mesh.pyx:
+// 23-430 (polar-cap mesh), effective_gravity_universal.pyx:10-71.
+// Every adaptive integral is one wavefront (wave_qag61); the 1000-point
+// equal-area table is 1000 independent waves; each ring is one wave whose
+// lanes classify the ring's cells and then integrate boundary cells in turn.
+#include "xh_common.hpp"
+#include "xh_types.hpp"
+#include "xh_kernels.hpp"
+
+namespace xh {
+
+__device__ __forceinline__ double radius_norm(double mu, double eps, double zeta, int shape)
+{
+    return shape == 0 ? 1.0 + eps * (-0.788 + 1.030 * zeta) * mu * mu : 1.0;
+}
+
+__device__ __forceinline__ double f_theta(double mu, double rn, double eps, double zeta, int shape)
+{
+    double d = -2.0 * eps * (-0.788 + 1.030 * zeta) * mu * sqrt(1.0 - mu * mu);
+    return shape == 0 ? d / (rn * sqrt(1.0 - 2.0 * zeta / rn)) : 0.0;
+}
+
+__device__ double effective_gravity(double mu, double R_eq, double x, double eps, int shape)
+{
+    double g0 = x * C_SI * C_SI / (R_eq * sqrt(1.0 - 2.0 * x));
+    double e2 = eps * eps;
+    double ce = -0.791 + 0.776 * x, cp = 1.138 - 1.431 * x;
+    double de = (-1.315 + 2.431 * x) * eps * x;
+    double dp = (0.653 - 2.864 * x) * eps * x;
+    double d60 = (13.47 - 27.13 * x) * eps * x;
+    double fe = -1.172 * x * e2, fp = 0.975 * x * e2;
+    double g = 1.0;
+    g += (ce + de + fe) * eps * (1.0 - mu * mu);
+    g += (cp + dp + fp - d60) * eps * mu * mu;
+    g += d60 * eps * fabs(mu);
+    return shape == 0 ? log10(g * g0) + 2.0 : log10(g0) + 2.0;
+}
+
+struct AreaFn {
+    double eps, zeta;
+    int av, shape;
+    __device__ double operator()(double th) const
+    {
+        if (eq12(th, 0.0)) return 0.0;
+        double mu = cos(th);
+        double r = radius_norm(mu, eps, zeta, shape);
+        double f = f_theta(mu, r, eps, zeta, shape);
+        double v = r * r * sqrt(1.0 + f * f) * sin(th);
+        return av ? th * v : v;
+    }
+};
+
+__device__ __forceinline__ double psi_of(double th, double ph, double TH)
+{
+    return acos(cos(TH) * cos(th) + sin(TH) * sin(th) * cos(ph));
+}
+
+__device__ __forceinline__ double phi_of(double th, double TH, double psi)
+{
+    double cp = cos(psi) - cos(TH) * cos(th);
+    cp /= sin(TH) * sin(th);
+    if (!(-1.0 <= cp && cp <= 1.0)) return -1.0;
+    return acos(cp);
+}
+
+__device__ __forceinline__ double span_in(double a, double b, double LB, double UB)
+{
+    int ac = (LB <= a && a <= UB), bc = (LB <= b && b <= UB);
+    if (!ac) ac = 2 * (a > UB);
+    if (!bc) bc = 2 * (b > UB);
+    if (ac == bc) return ac == 1 ? b - a : 0.0;
+    if (bc == 1 && ac == 0) return b - LB;
+    if (bc == 2 && ac == 1) return UB - a;
+    if (bc == 2 && ac == 0) return UB - LB;
+    return 0.0;
+}
+
+// azimuthal extent of (ceding region minus superseding region) at colatitude
+// th, clipped to the cell [pa, pb] when `cell` is set
+struct RegionFn {
+    double eps, zeta, colat, cedeR, superR, superAzi, superColat, pa, pb, R_eq, Omega;
+    int shape, lorentz, cell;
+    __device__ double span(double L, double U) const { return cell ? span_in(pa, pb, L, U) : U - L; }
+    __device__ double operator()(double th) const
+    {
+        if (eq12(th, 0.0)) return 0.0;
+        double mu = cos(th);
+        double r = radius_norm(mu, eps, zeta, shape);
+        double f = f_theta(mu, r, eps, zeta, shape);
+        double aLB, aUB, cLB, cUB, dphi = 0.0;
+        aUB = phi_of(th, colat, cedeR);
+        if (eq12(aUB, -1.0)) {
+            if (th + colat < cedeR) { aUB = PI; aLB = -PI; }
+            else return 0.0;
+        } else aLB = -aUB;
+        if (fabs(th - superColat) > superR) {
+            cLB = cUB = 0.0;
+        } else {
+            cUB = phi_of(th, superColat, superR);
+            if (eq12(cUB, -1.0)) return 0.0;
+            cLB = -cUB;
+            cUB += superAzi;
+            cLB += superAzi;
+            if (cUB > PI) cUB -= TWO_PI;
+            if (cLB < -PI) cLB += TWO_PI;
+        }
+        if (cUB >= cLB) {
+            if (cLB >= aUB || cUB <= aLB) dphi = span(aLB, aUB);
+            else if (cLB >= aLB && cUB <= aUB) dphi = span(aLB, cLB) + span(cUB, aUB);
+            else if (cLB < aLB && aLB < cUB && cUB <= aUB) dphi = span(cUB, aUB);
+            else if (aLB <= cLB && cLB < aUB && aUB < cUB) dphi = span(aLB, cLB);
+            else if (cLB < aLB && cUB > aUB) return 0.0;
+        } else {
+            if (cUB >= aUB || cLB <= aLB) return 0.0;
+            else if (cUB >= aLB && cLB <= aUB) dphi = span(cUB, cLB);
+            else if (cUB < aLB && aLB < cLB && cLB <= aUB) dphi = span(aLB, cLB);
+            else if (aLB <= cUB && cUB < aUB && aUB < cLB) dphi = span(cUB, aUB);
+            else if (cUB < aLB && cLB > aUB) dphi = span(aLB, aUB);
+        }
+        double w = 1.0;
+        if (!cell && lorentz) {
+            double rsr = 2.0 * zeta / r;
+            double beta = R_eq * r * Omega * sin(th) / (C_SI * sqrt(1.0 - rsr));
+            w = 1.0 / sqrt(1.0 - beta * beta);
+        }
+        return w * dphi * r * r * sqrt(1.0 + f * f) * sin(th);
+    }
+};
+
+__device__ double wave_area(double lo, double hi, double R_eq, double eps, double zeta, int shape, int av)
+{
+    AreaFn f{eps, zeta, av, shape};
+    double r, e;
+    wave_qag61(f, lo, hi, 0.0, MESH_EPSREL, QAG_LIMIT, r, e);
+    return R_eq * R_eq * r;
+}
+
+__device__ double wave_spot(double ta, double tb, double R_eq, double eps, double zeta, int shape,
+                            double colat, double cedeR, double superR, double superColat,
+                            double superAzi, double Omega)
+{
+    RegionFn f{eps, zeta, colat, cedeR, superR, superAzi, superColat, 0.0, 0.0, R_eq, Omega,
+               shape, 0, 0};
+    double r, e;
+    wave_qag61(f, ta, tb, 0.0, MESH_EPSREL, QAG_LIMIT, r, e);
+    return R_eq * R_eq * r;
+}
+
+__device__ double theta_given_phi(double ph, double colat, double rad, double lo, double hi)
+{
+    double x = sin(colat) * sin(ph) / sin(rad);
+    if (!(x <= 1.0)) return -1.0;
+    x = asin(x);
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) x = PI - x;
+        double th = tan(0.5 * (colat - rad)) / sin(0.5 * (x - ph));
+        th *= sin(0.5 * (x + ph));
+        th = atan(th);
+        th = th < 0.0 ? -2.0 * th : 2.0 * th;
+        if (lo <= th && th <= hi) return th;
+    }
+    return -1.0;
+}
+
+__device__ double cede_edge_phi(double th, double colat, double cedeR, double pa, double pb)
+{
+    double v = phi_of(th, colat, cedeR);
+    if (v >= 0.0 && !(pa <= v && v <= pb) && !(pa <= -v && -v <= pb)) v = -1.0;
+    return v;
+}
+
+__device__ double super_edge_phi(double th, double sColat, double sR, double sAzi, double pa,
+                                 double pb, bool strict_second)
+{
+    if (fabs(th - sColat) > sR) return -1.0;
+    double v = phi_of(th, sColat, sR);
+    if (v >= 0.0) {
+        v += sAzi;
+        if (v > PI) v -= TWO_PI;
+        if (v < -PI) v += TWO_PI;
+        if (!(pa <= v && v <= pb)) v = -1.0;
+        else if (v < 0.0) v = -v;
+    }
+    if (v < 0.0) {
+        v = phi_of(th, sColat, sR);
+        bool take = strict_second ? (v > 0.0) : (v >= 0.0);
+        if (take) {
+            v = -v + sAzi;
+            if (v > PI) v -= TWO_PI;
+            if (v < -PI) v += TWO_PI;
+            if (!(pa <= v && v <= pb)) v = -1.0;
+            else if (v < 0.0) v = -v;
+        }
+    }
+    return v;
+}
+
+__device__ __forceinline__ double take_first(double& c0, double& c1, double& c2, double& c3,
+                                             double ta, double tb)
+{
+    if (c0 >= 0.0) { double v = c0; c0 = -1.0; return v; }
+    if (c1 >= 0.0) { double v = c1; c1 = -1.0; return v; }
+    if (c2 >= 0.0) { c2 = -1.0; return ta; }
+    if (c3 >= 0.0) { c3 = -1.0; return tb; }
+    return -1.0;
+}
+
+// mesh_tools.pyx:429-725 (wave-cooperative)
+__device__ double wave_cell(double ta, double tb, double pa, double pb, double R_eq, double eps,
+                            double zeta, int shape, double colat, double cedeR, double superR,
+                            double superAzi, double superColat)
+{
+    RegionFn f{eps, zeta, colat, cedeR, superR, superAzi, superColat, pa, pb, R_eq, 0.0, shape, 0, 1};
+    double integral, e;
+    wave_qag61(f, ta, tb, 0.0, MESH_EPSREL, QAG_LIMIT, integral, e);
+    if (eq12(integral, 0.0)) {
+        double tl0 = theta_given_phi(pa, colat, cedeR, ta, tb);
+        double tl1 = theta_given_phi(pa - superAzi, superColat, superR, ta, tb);
+        double tr0 = theta_given_phi(pb, colat, cedeR, ta, tb);
+        double tr1 = theta_given_phi(pb - superAzi, superColat, superR, ta, tb);
+        double top0 = cede_edge_phi(ta, colat, cedeR, pa, pb);
+        double top1 = super_edge_phi(ta, superColat, superR, superAzi, pa, pb, true);
+        double bot0 = cede_edge_phi(tb, colat, cedeR, pa, pb);
+        double bot1 = super_edge_phi(tb, superColat, superR, superAzi, pa, pb, false);
+        double t0 = take_first(tl0, tr0, top0, bot0, ta, tb);
+        double t1 = take_first(tl0, tr0, top0, bot0, ta, tb);
+        double t2 = take_first(tl1, tr1, top1, bot1, ta, tb);
+        double t3 = take_first(tl1, tr1, top1, bot1, ta, tb);
+        double tmin = tb, tmax = ta;
+        double ts[4] = {t0, t1, t2, t3};
+        for (int q = 0; q < 4; q++) {
+            if (ts[q] >= 0.0) {
+                if (ts[q] < tmin) tmin = ts[q];
+                if (ts[q] > tmax) tmax = ts[q];
+            }
+        }
+        if (tmin < tmax) wave_qag61(f, tmin, tmax, 0.0, MESH_EPSREL, QAG_LIMIT, integral, e);
+    }
+    return R_eq * R_eq * integral;
+}
+
+__device__ __forceinline__ double lin(double start, double stop, int num, int k)
+{
+    if (num == 1) return start;
+    if (k == num - 1) return stop;
+    double step = (stop - start) / (double)(num - 1);
+    return (double)k * step + start;
+}
+
+__device__ void clamp_even(double& s, int lo, int hi)
+{
+    if (s < (double)lo) s = (double)lo;
+    else if (s > (double)hi) s = (double)hi;
+    if (((long)s) % 2 != 0) s += 1.0;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// One block of 4 waves per hot region: the up-to-four allocation integrals
+// run concurrently, then lane 0 of wave 0 sizes both members.
+__global__ void __launch_bounds__(256) k_allocate(const RegionAlloc* __restrict__ regions,
+                                                  Member* __restrict__ members, int S_max)
+{
+    const RegionAlloc R = regions[blockIdx.x];
+    const int wave = threadIdx.x >> 6;
+    __shared__ double vals[4];
+    __shared__ double lims_s[2], lims_c[2], cellfac[2];
+    // super region limits (possibly reflected through the equator)
+    double sColat = R.superColat, hColat = R.holeColat, s0, s1, sfac;
+    if (R.superColat - R.superR < 0.0 || R.superColat + R.superR > PI) {
+        s0 = 0.0;
+        if (R.superColat + R.superR > PI) {
+            s1 = PI - R.superColat + R.superR;
+            sColat = PI - R.superColat;
+            hColat = PI - R.holeColat;
+        } else s1 = R.superColat + R.superR;
+        sfac = TWO_PI;
+    } else {
+        s0 = R.superColat - R.superR;
+        s1 = R.superColat + R.superR;
+        sfac = 2.0 * asin(sin(R.superR) / sin(R.superColat));
+    }
+    bool cede = !eq12(R.cedeR, 0.0);
+    double cColat = R.cedeColat, sColat2 = R.superColat, c0 = 0.0, c1 = 0.0, cfac = 0.0;
+    if (cede) {
+        if (R.cedeColat - R.cedeR < 0.0 || R.cedeColat + R.cedeR > PI) {
+            c0 = 0.0;
+            if (R.cedeColat + R.cedeR > PI) {
+                c1 = PI - R.cedeColat + R.cedeR;
+                cColat = PI - R.cedeColat;
+                sColat2 = PI - R.superColat;
+            } else c1 = R.cedeColat + R.cedeR;
+            cfac = TWO_PI;
+        } else {
+            c0 = R.cedeColat - R.cedeR;
+            c1 = R.cedeColat + R.cedeR;
+            cfac = 2.0 * asin(sin(R.cedeR) / sin(R.cedeColat));
+        }
+    }
+    double v = 0.0;
+    if (wave == 0) v = wave_area(s0, s1, R.R_eq, R.epsilon, R.zeta, R.shape, 0);
+    else if (wave == 1)
+        v = wave_spot(s0, s1, R.R_eq, R.epsilon, R.zeta, R.shape, sColat, R.superR, R.holeR, hColat,
+                      R.holeAzi, R.Omega);
+    else if (wave == 2 && cede) v = wave_area(c0, c1, R.R_eq, R.epsilon, R.zeta, R.shape, 0);
+    else if (wave == 3 && cede)
+        v = wave_spot(c0, c1, R.R_eq, R.epsilon, R.zeta, R.shape, cColat, R.cedeR, R.superR, sColat2,
+                      R.superAzi, R.Omega);
+    if ((threadIdx.x & 63) == 0) vals[wave] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double s_cell = sfac * vals[0], s_area = vals[1];
+    double s_sqrt, s_num = 0.0, c_sqrt = 0.0, c_num = 0.0;
+    if (!cede) {
+        if (eq12(s_area, 0.0)) s_area = s_cell / 1000.0;
+        s_sqrt = ceil(sqrt((double)R.num_cells * s_cell / s_area));
+        clamp_even(s_sqrt, R.min_sqrt, R.max_sqrt);
+        s_num = s_sqrt * s_sqrt;
+    } else {
+        double c_cell = cfac * vals[2], c_area = vals[3];
+        if (eq12(s_area, 0.0)) s_area = s_cell / 1000.0;
+        if (eq12(c_area, 0.0)) c_area = c_cell / 1000.0;
+        double y = c_area / s_area - 1.0, sn, cn;
+        if (eq12(y, 0.0)) { sn = 0.5 * R.num_cells; cn = 0.5 * R.num_cells; }
+        else { sn = (double)R.num_cells * ((sqrt(1.0 + y) - 1.0) / y); cn = (double)R.num_cells - sn; }
+        s_sqrt = ceil(sqrt(sn * s_cell / s_area));
+        c_sqrt = ceil(sqrt(cn * c_cell / c_area));
+        clamp_even(s_sqrt, R.min_sqrt, R.max_sqrt);
+        clamp_even(c_sqrt, R.min_sqrt, R.max_sqrt);
+        s_num = s_sqrt * s_sqrt;
+        c_num = c_sqrt * c_sqrt;
+    }
+    Member* m0 = members + R.member0;
+    m0->S = (int)s_sqrt;
+    m0->numCell = s_num;
+    if (m0->S > S_max) m0->status = 3;
+    if (R.member1 >= 0) {
+        Member* m1 = members + R.member1;
+        m1->S = (int)c_sqrt;
+        m1->numCell = c_num;
+        if (m1->S > S_max) m1->status = 3;
+    }
+}
+
+__device__ __forceinline__ void member_lims(const Member& m, double& lo, double& hi, double& colat,
+                                            double& superColat, int& invert, double& dphi,
+                                            double& bphi)
+{
+    colat = m.cedeColat;
+    superColat = m.superColat;
+    invert = 0;
+    bphi = 0.0;
+    if (m.polar) {
+        lo = 0.0;
+        if (m.cedeColat + m.cedeR > PI) {
+            invert = 1;
+            hi = PI - m.cedeColat + m.cedeR;
+            colat = PI - m.cedeColat;
+            superColat = PI - m.superColat;
+        } else hi = m.cedeColat + m.cedeR;
+        dphi = TWO_PI / (double)m.S;
+    } else {
+        lo = m.cedeColat - m.cedeR;
+        hi = m.cedeColat + m.cedeR;
+        bphi = asin(sin(m.cedeR) / sin(m.cedeColat));
+        dphi = 2.0 * bphi / (double)m.S;
+    }
+}
+
+// Area table: wave q integrates the area between table colatitude q-1 and the
+// far boundary; wave 0 integrates the whole region (total cell area).
+__global__ void __launch_bounds__(256) k_mesh_table(const Member* __restrict__ members,
+                                                    double* __restrict__ area_tab,
+                                                    double* __restrict__ total, int table_n)
+{
+    const Member m = members[blockIdx.y];
+    if (!m.active || m.status) return;
+    int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q > table_n) return;
+    double lo, hi, colat, sc, dphi, bphi;
+    int inv;
+    member_lims(m, lo, hi, colat, sc, inv, dphi, bphi);
+    if (q == 0) {
+        double v = wave_area(m.polar ? 0.0 : lo, hi, m.R_eq, m.epsilon, m.zeta, m.shape, 0);
+        if ((threadIdx.x & 63) == 0) total[blockIdx.y] = v;
+    } else {
+        double ct = lin(hi, lo, table_n, q - 1);
+        double v = wave_area(ct, hi, m.R_eq, m.epsilon, m.zeta, m.shape, 0);
+        if ((threadIdx.x & 63) == 0) area_tab[(size_t)blockIdx.y * table_n + q - 1] = v;
+    }
+}
+
+// One wave per ring: parallels, centroid colatitude, ring properties and the
+// ring's cell areas (boundary cells integrated one after another).
+__global__ void __launch_bounds__(64) k_mesh_rings(const Member* __restrict__ members,
+                                                   const double* __restrict__ area_tab,
+                                                   const double* __restrict__ total, int table_n,
+                                                   Layout L, double* __restrict__ theta,
+                                                   double* __restrict__ phi, double* __restrict__ rr,
+                                                   double* __restrict__ cellArea,
+                                                   double* __restrict__ maxAlpha,
+                                                   double* __restrict__ cos_gamma,
+                                                   double* __restrict__ effGrav)
+{
+    const int mi = blockIdx.y, i = blockIdx.x, lane = threadIdx.x;
+    const Member m = members[mi];
+    if (!m.active || m.status) return;
+    const int S = m.S;
+    if (i >= S) return;
+    double lo, hi, colat, sColat, dphi, bphi;
+    int inv;
+    member_lims(m, lo, hi, colat, sColat, inv, dphi, bphi);
+    double cell = (m.polar ? TWO_PI : 2.0 * bphi) * total[mi];
+    cell /= m.numCell;
+    double eta = cell / dphi;
+    const double* at = area_tab + (size_t)mi * table_n;
+    // colatitude table is linspace(hi, lo, table_n) (descending); areas ascend
+    __shared__ double ct[1024];
+    for (int k = lane; k < table_n; k += 64) ct[k] = lin(hi, lo, table_n, k);
+    __syncthreads();
+    double i_eta = (double)S * eta, e_lo = 0.0, e_hi = 0.0;
+    for (int k = 0; k <= i && k < S - 1; k++) {
+        i_eta -= eta;
+        if (k == i - 1) e_lo = i_eta;
+        if (k == i) e_hi = i_eta;
+    }
+    double lower = (i == 0) ? lo : steffen_eval(at, ct, table_n, e_lo, nullptr);
+    double upper = (i == S - 1) ? hi : steffen_eval(at, ct, table_n, e_hi, nullptr);
+    double cc = wave_area(lower, upper, m.R_eq, m.epsilon, m.zeta, m.shape, 1) / eta;
+    double mu = cos(cc);
+    double rn = radius_norm(mu, m.epsilon, m.zeta, m.shape);
+    double fth = f_theta(mu, rn, m.epsilon, m.zeta, m.shape);
+    double cg = 1.0 / sqrt(1.0 + fth * fth);
+    const size_t ro = (size_t)mi * L.S_max, co = (size_t)mi * L.S_max * L.S_max;
+    if (lane == 0) {
+        rr[ro + i] = rn * m.R_eq;
+        effGrav[ro + i] = effective_gravity(mu, m.R_eq, m.zeta, m.epsilon, m.shape);
+        cos_gamma[ro + i] = cg;
+        maxAlpha[ro + i] = HALF_PI + acos(cg);
+        theta[ro + i] = inv ? PI - cc : cc;
+    }
+    double first_left = m.polar ? -PI : -bphi;
+    if (i == 0) {
+        for (int j = lane; j < S; j += 64) {
+            double p0 = m.polar ? lin(-PI + 0.5 * dphi, PI - 0.5 * dphi, S, j)
+                                : lin(-bphi + 0.5 * dphi, bphi - 0.5 * dphi, S, j);
+            double v = p0 + m.phi_offset;
+            if (m.antiphased) v += PI;
+            phi[ro + j] = v;
+        }
+    }
+    const double sR = m.superR, sAzi = m.superAzi, cR = m.cedeR;
+    double s1a = 0, s1b = 0, s2a = 0, s2b = 0, s3a = 0, s3b = 0, s4a = 0, s4b = 0;
+    if (sR > 0.0) {
+        if (sColat - sR < 0.0) {
+            s1a = sR - sColat; s1b = sAzi + PI; if (s1b > PI) s1b -= TWO_PI;
+            s2a = sColat + sR; s2b = sAzi;
+        } else if (sColat + sR > PI) {
+            s1a = sColat - sR; s1b = sAzi;
+            s2a = TWO_PI - sColat - sR; s2b = sAzi + PI; if (s2b > PI) s2b -= TWO_PI;
+        } else {
+            s1a = sColat - sR; s1b = sAzi;
+            s2a = acos(cos(sColat) / cos(sR));
+            s2b = sAzi + asin(sin(sR) / sin(sColat)); if (s2b > PI) s2b -= TWO_PI;
+            s3a = sColat + sR; s3b = sAzi;
+            s4a = s2a;
+            s4b = sAzi - asin(sin(sR) / sin(sColat)); if (s4b < -PI) s4b += TWO_PI;
+        }
+    }
+    const bool mirror = eq12(sColat, colat) && eq12(sAzi, 0.0);
+    const int j_max = mirror ? S / 2 : S;
+    double* ca = cellArea + co + (size_t)i * S;
+    // the reference accumulates left += delta_phi cell by cell
+    for (int base = 0; base < j_max; base += 64) {
+        int j = base + lane;
+        double left = first_left;
+        for (int q = 0; q < j && q < j_max; q++) left = left + dphi;
+        int kind = 0;     // 0 empty, 1 interior, 2 boundary
+        double right = left + dphi, mid = left + 0.5 * dphi;
+        if (j < j_max) {
+            int p1, p2, p3, p4, p5;
+            bool top_row = m.polar && i == 0;
+            if (top_row) {
+                p1 = p3 = (colat <= cR);
+                if (p1 == 1) { p1 = (sR <= sColat); p3 = p1; } else p1 = p3 = 2;
+            } else {
+                p1 = (psi_of(lower, left, colat) <= cR);
+                p3 = (psi_of(lower, right, colat) <= cR);
+                p1 = p1 == 1 ? (sR <= psi_of(lower, left - sAzi, sColat)) : 2;
+                p3 = p3 == 1 ? (sR <= psi_of(lower, right - sAzi, sColat)) : 2;
+            }
+            p2 = (psi_of(upper, left, colat) <= cR);
+            p4 = (psi_of(upper, right, colat) <= cR);
+            p5 = (psi_of(cc, mid, colat) <= cR);
+            p2 = p2 == 1 ? (sR <= psi_of(upper, left - sAzi, sColat)) : 2;
+            p4 = p4 == 1 ? (sR <= psi_of(upper, right - sAzi, sColat)) : 2;
+            p5 = p5 == 1 ? (sR <= psi_of(cc, mid - sAzi, sColat)) : 2;
+            if (!(p1 == p2 && p2 == p3 && p3 == p4 && p4 == p5)) kind = 2;
+            else if (sR > 0.0) {
+                auto inside = [&](double a, double b) {
+                    return (lower <= a && a <= upper) && (left <= b && b <= right);
+                };
+                if (inside(s1a, s1b) || inside(s2a, s2b) || inside(s3a, s3b) || inside(s4a, s4b)) kind = 2;
+                else if (p5 == 1) kind = 1;
+            } else if (p5 == 1) kind = 1;
+            double v = kind == 1 ? cell : 0.0;
+            ca[j] = v;
+            if (mirror) ca[S - 1 - j] = v;
+        }
+        // boundary cells: the whole wave integrates each in turn
+        unsigned long long todo = __ballot(kind == 2);
+        while (todo) {
+            int src = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            double l2 = wave_bcast(left, src);
+            double r2 = wave_bcast(right, src);
+            double v = wave_cell(lower, upper, l2, r2, m.R_eq, m.epsilon, m.zeta, m.shape, colat, cR,
+                                 sR, sAzi, sColat);
+            if (lane == src) {
+                ca[j] = v;
+                if (mirror) ca[S - 1 - j] = v;
+            }
+        }
+    }
+}
+
+}  // namespace xh
+
+namespace xh {
+
+// per-ring quantities the integrator needs, from the mesh of each member
+__global__ void __launch_bounds__(256) k_ring_finalize(const Member* __restrict__ members, Layout L,
+                                                       const double* __restrict__ rr,
+                                                       const double* __restrict__ cellArea,
+                                                       double* __restrict__ u, int* __restrict__ ring_ok,
+                                                       double* __restrict__ logT)
+{
+    const int mi = blockIdx.x;
+    const Member m = members[mi];
+    if (!m.active || m.status) return;
+    for (int i = threadIdx.x; i < m.S; i += blockDim.x) {
+        size_t ro = (size_t)mi * L.S_max + i;
+        u[ro] = m.r_s / rr[ro];
+        const double* ca = cellArea + (size_t)mi * L.S_max * L.S_max + (size_t)i * m.S;
+        int ok = 0;
+        for (int j = 0; j < m.S; j++) if (ca[j] > 0.0) { ok = 1; break; }
+        ring_ok[ro] = ok;
+        logT[ro] = m.logT;
+    }
+}
+
+// drop-in path: per-ring inputs from the reference's [S, S] arrays
+__global__ void __launch_bounds__(256) k_ring_prep(int S, const double* __restrict__ theta2d,
+                                                   const double* __restrict__ params,
+                                                   const int* __restrict__ radiates,
+                                                   double* __restrict__ theta, int* __restrict__ ring_ok,
+                                                   double* __restrict__ logT)
+{
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        int J = -1;
+        for (int j = 0; j < S; j++) if (radiates[(size_t)i * S + j] == 1) { J = j; break; }
+        ring_ok[i] = J >= 0;
+        logT[i] = J >= 0 ? params[((size_t)i * S + J) * 2] : 0.0;
+        theta[i] = theta2d[(size_t)i * S];
+    }
+}
+
+}  // namespace xh

@@ -1,0 +1,240 @@
+// Schwarzschild ray deflection and time-lag tables on the GPU.
+//
+// Reference behaviour: xpsi/cellmesh/rays.pyx:53-390 (integrands :61-88,
+// three ray branches :90-247, table construction and repair :249-390) and the
+// lag calibration of xpsi/HotRegion.py:880-901.  One wavefront per ray runs
+// the QAG-GK61 quadratures (epsrel 1e-12, 100 intervals, as the reference);
+// a second kernel applies the bad-ray repair, the last-ray check and the lag
+// calibration per member.
+#include "xh_common.hpp"
+#include "xh_types.hpp"
+#include "xh_kernels.hpp"
+
+namespace xh {
+
+struct InDef {
+    double rc;
+    __device__ double operator()(double x) const
+    {
+        return 1.0 / sqrt(2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (rc - 1.0));
+    }
+};
+struct InLag {
+    double rc;
+    __device__ double operator()(double x) const
+    {
+        double X = 1.0 / sqrt(2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (rc - 1.0));
+        return X * X / (x + X);
+    }
+};
+struct OutDef {
+    double sas, Rr;
+    __device__ double operator()(double x) const
+    {
+        return x / sqrt(1.0 - sas + x * x * sas * (2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (Rr - 1.0)));
+    }
+};
+struct OutLag {
+    double sas, Rr;
+    __device__ double operator()(double x) const
+    {
+        double f = sqrt(1.0 - sas + x * x * sas * (2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (Rr - 1.0)));
+        return x / (f + f * f);
+    }
+};
+
+template <class F>
+__device__ __forceinline__ int ray_quad(F f, double a, double b, double& r)
+{
+    double e;
+    return wave_qag61(f, a, b, 0.0, 1.0e-12, 100, r, e);
+}
+
+// deflection and lag (seconds) of the ray leaving radius r_s/u at cos(alpha)
+__device__ void wave_ray(double cos_alpha, double r_s, double u, double& defl, double& lag)
+{
+    const double b_ph = 3.0 * sqrt(3.0) / 2.0;
+    double sas = 1.0 - cos_alpha * cos_alpha;
+    double sa = sqrt(sas);
+    double alpha = acos(cos_alpha);
+    double b = sa / (u * sqrt(1.0 - u));
+    double d0 = __builtin_nan(""), l0 = __builtin_nan("");
+    int st = XH_SUCCESS;
+    if (b <= b_ph) {
+        OutDef fd{sas, 1.0 / u};
+        OutLag fl{sas, 1.0 / u};
+        st = ray_quad(fd, 0.0, 1.0, d0);
+        if (st == XH_EFAILED) d0 = __builtin_nan("");
+        else {
+            st = ray_quad(fl, 0.0, 1.0, l0);
+            if (st == XH_EFAILED) d0 = __builtin_nan("");
+        }
+        if (st != XH_EFAILED) {
+            d0 *= 2.0 * b * u;
+            l0 *= 2.0 * b * b * u * r_s;
+        }
+    } else if (alpha <= PI / 2.0) {
+        double rc, wR;
+        if (eq12(sa, 1.0)) { rc = 1.0 / u; wR = 0.0; }
+        else {
+            rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
+            wR = sqrt(1.0 - rc * u);
+        }
+        if (isnan(wR)) wR = 0.0;
+        InDef fd{rc};
+        InLag fl{rc};
+        st = ray_quad(fd, wR, 1.0, d0);
+        if (st == XH_EFAILED) d0 = __builtin_nan("");
+        else {
+            st = ray_quad(fl, wR, 1.0, l0);
+            if (st == XH_EFAILED) d0 = __builtin_nan("");
+        }
+        if (st != XH_EFAILED) {
+            d0 *= 2.0 * b / rc;
+            l0 *= 2.0 * b * b * r_s / rc;
+        }
+    } else {
+        double rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
+        double wR = sqrt(1.0 - rc * u);
+        if (isnan(wR)) wR = 0.0;
+        InDef fd{rc};
+        InLag fl{rc};
+        double D1 = 0, D0 = 0, L1 = 0, L0 = 0;
+        st = ray_quad(fd, 0.0, 1.0, D1);
+        if (st != XH_EFAILED) st = ray_quad(fd, wR, 1.0, D0);
+        if (st != XH_EFAILED) st = ray_quad(fl, 0.0, 1.0, L1);
+        if (st != XH_EFAILED) st = ray_quad(fl, wR, 1.0, L0);
+        if (st != XH_EFAILED) {
+            d0 = 2.0 * b * (2.0 * D1 - D0) / rc;
+            l0 = 2.0 * b * b * r_s * (2.0 * L1 - L0) / rc;
+            double cor = 1.0 / u - rc + log((1.0 / u - 1.0) / (rc - 1.0));
+            l0 += 2.0 * r_s * cor;
+        }
+    }
+    if (st != XH_EFAILED) l0 /= C_SI;
+    defl = d0;
+    lag = l0;
+}
+
+__device__ __forceinline__ double clamped_max_alpha(double maxA, double u)
+{
+    const double b_ph = 3.0 * sqrt(3.0) / 2.0, eps = 1.0e-8;
+    double extreme = 0.0;
+    if (u < 2.0 / 3.0) extreme = PI - asin(sqrt(1.0 - u) * b_ph * u);
+    else if (eq12(u, 2.0 / 3.0)) extreme = PI / 2.0;
+    else if (2.0 / 3.0 < u && u < 1.0) extreme = asin(sqrt(1.0 - u) * b_ph * u);
+    if (maxA >= (1.0 - eps) * extreme) maxA = (1.0 - eps) * extreme;
+    return maxA;
+}
+
+// wave per (ray j >= 1, parallel i, member); writes raw (unrepaired) values
+__global__ void __launch_bounds__(256) k_rays(const Member* __restrict__ members, Layout L,
+                                              const double* __restrict__ rr,
+                                              const double* __restrict__ maxAlpha_in,
+                                              double* __restrict__ cos_alpha,
+                                              double* __restrict__ defl_raw,
+                                              double* __restrict__ lag_raw)
+{
+    const int mi = blockIdx.z, i = blockIdx.y;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6) + 1;
+    const Member m = members[mi];
+    if (!m.active || m.status || i >= m.S || j >= L.N_R) return;
+    const size_t ro = (size_t)mi * L.S_max + i;
+    double u = m.r_s / rr[ro];
+    double maxA = clamped_max_alpha(maxAlpha_in[ro], u);
+    double inc = (1.0 - cos(maxA)) / ((double)L.N_R - 1.0);
+    double ca = 1.0 - (double)j * inc;
+    double d, l;
+    wave_ray(ca, m.r_s, u, d, l);
+    if ((threadIdx.x & 63) == 0) {
+        size_t o = ro * L.N_R + j;
+        cos_alpha[o] = ca;
+        defl_raw[o] = d;
+        lag_raw[o] = l;
+        if (j == 1) {
+            cos_alpha[ro * L.N_R] = 1.0;
+            defl_raw[ro * L.N_R] = 0.0;
+            lag_raw[ro * L.N_R] = 0.0;
+        }
+    }
+}
+
+__device__ __forceinline__ bool ray_bad(double d) { return isnan(d) || d < 0.0 || d > 100.0 * PI; }
+
+// per member: clamp maxAlpha, repair bad rays (rays.pyx:331-369, one-thread
+// order), check the last ray, calibrate the lag (HotRegion.py:880-901)
+__global__ void __launch_bounds__(256) k_rays_fix(Member* __restrict__ members, Layout L,
+                                                  const double* __restrict__ rr,
+                                                  double* __restrict__ maxAlpha,
+                                                  double* __restrict__ defl,
+                                                  double* __restrict__ lag,
+                                                  double* __restrict__ maxDefl)
+{
+    const int mi = blockIdx.x;
+    Member* mp = members + mi;
+    if (!mp->active || mp->status) return;
+    const int S = mp->S, N_R = L.N_R;
+    const double r_s = mp->r_s;
+    double* D = defl + (size_t)mi * L.S_max * N_R;
+    double* Lg = lag + (size_t)mi * L.S_max * N_R;
+    __shared__ int any_bad, last_bad;
+    if (threadIdx.x == 0) { any_bad = 0; last_bad = 0; }
+    __syncthreads();
+    for (int q = threadIdx.x; q < S * N_R; q += blockDim.x) {
+        int j = q % N_R;
+        if (j > 0 && j < N_R - 1 && ray_bad(D[q])) atomicOr(&any_bad, 1);
+        if (j == N_R - 1 && (isnan(D[q]) || D[q] <= D[q - 1])) atomicOr(&last_bad, 1);
+    }
+    __syncthreads();
+    if (any_bad) {
+        // rare path: the reference's sequential repair, counter kept across parallels
+        if (threadIdx.x == 0) {
+            int counter = 0, idx = 0, term = 0;
+            for (int i = 0; i < S && !term; i++) {
+                double* Di = D + (size_t)i * N_R;
+                double* Li = Lg + (size_t)i * N_R;
+                for (int j = 1; j < N_R; j++) {
+                    double o0 = Di[j], o1 = Li[j];
+                    if (j == N_R - 1 && (isnan(o0) || o0 <= Di[j - 1])) { term = 1; break; }
+                    else if (counter > 0 && !isnan(o0) && 0.0 <= o0 && o0 < 100.0 * PI && o0 > Di[idx]) {
+                        Di[j] = o0; Li[j] = o1;
+                        double delta = (double)counter + 1.0;
+                        for (int k = 1; k <= counter; k++) {
+                            Di[idx + k] = Di[j] * (double)k;
+                            Di[idx + k] += Di[idx] * (delta - (double)k);
+                            Di[idx + k] /= delta;
+                            Li[idx + k] = Li[j] * (double)k;
+                            Li[idx + k] += Li[idx] * (delta - (double)k);
+                            Li[idx + k] /= delta;
+                        }
+                        counter = 0;
+                    } else if (isnan(o0) || o0 < 0.0 || o0 > 100.0 * PI) {
+                        if (counter == 0) idx = j - 1;
+                        counter++;
+                    }
+                }
+            }
+            if (term) mp->status = 1;
+        }
+    } else if (last_bad) {
+        if (threadIdx.x == 0) mp->status = 1;
+    }
+    __syncthreads();
+    if (mp->status) return;
+    const double R_i = mp->R_eq / r_s, w = mp->mode_freq * TWO_PI;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        size_t ro = (size_t)mi * L.S_max + i;
+        double u = r_s / rr[ro];
+        maxAlpha[ro] = clamped_max_alpha(maxAlpha[ro], u);
+        maxDefl[ro] = D[(size_t)i * N_R + N_R - 1];
+    }
+    for (int q = threadIdx.x; q < S * N_R; q += blockDim.x) {
+        int i = q / N_R;
+        double u = r_s / rr[(size_t)mi * L.S_max + i];
+        double Cc = (1.0 / u - R_i) + log((1.0 / u - 1.0) / (R_i - 1.0));
+        Cc *= r_s / C_SI;
+        Lg[q] = (Lg[q] - Cc) * w;
+    }
+}
+
+}  // namespace xh

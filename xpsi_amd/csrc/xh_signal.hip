@@ -1,0 +1,320 @@
+// Signal registration and the background-marginalised likelihood on the GPU.
+//
+// Reference behaviour:
+//   energy integral  xpsi/tools/energy_integrator.pyx:27-114 (periodic Akima
+//                    in log10 E, upper edge clipped at the highest energy);
+//   fold             xpsi/Instrument.py:163-197 (matrix @ signal);
+//   phase binning    xpsi/tools/compute_expected_counts.pyx:66-197;
+//   marginal lnL     xpsi/likelihoods/default_background_marginalisation.pyx
+//                    :84-447 (Newton ML background, Gaussian bounds, quadrature).
+// The marginal kernel is one wavefront per (channel, request): lanes own phase
+// bins for the expected counts and Newton sums, then become Kronrod nodes.
+#include "xh_common.hpp"
+#include "xh_types.hpp"
+#include "xh_kernels.hpp"
+
+namespace xh {
+
+// out[region][j][k] = atten[j] * sum_m integral over bin j of the member's
+// specific flux (divided by d^2) in log10 E; one thread per (bin, phase).
+__global__ void __launch_bounds__(256) k_energy_integrate(const Member* __restrict__ members,
+                                                          EnergyArgs A)
+{
+    const int reg = blockIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N_in = A.N_edges - 1, N_P = A.N_P, N_E = A.N_E;
+    if (q >= N_in * N_P) return;
+    const int j = q / N_P, k = q % N_P;
+    const double maxE = A.logE[N_E - 1];
+    // bins past the first one reaching beyond the highest energy stay zero
+    if (j > 0 && A.logEdges[j] > maxE) { A.out[(size_t)reg * N_in * N_P + q] = 0.0; return; }
+    const double lo = A.logEdges[j];
+    const double hi = A.logEdges[j + 1] > maxE ? maxE : A.logEdges[j + 1];
+    double total = 0.0;
+    for (int s = 0; s < A.members_per_region; s++) {
+        const int mi = A.member_index[reg * A.members_per_region + s];
+        if (mi < 0) continue;
+        const Member m = members[mi];
+        if (!m.active) continue;
+        const double* fl = A.flux + (size_t)mi * N_E * N_P + k;
+        auto yv = [&](int e) {
+            return pow(10.0, A.logE[e]) * (fl[(size_t)e * N_P] / m.d_sq) * log(10.0);
+        };
+        double v;
+        if (lo > hi || lo < A.logE[0] || hi > maxE) v = __builtin_nan("");
+        else if (lo == hi) v = 0.0;
+        else {
+            int ia = interval_of(A.logE, N_E, lo), ib = interval_of(A.logE, N_E, hi);
+            v = 0.0;
+            for (int i = ia; i <= ib; i++) {
+                auto sec = [&](int kk) {
+                    int w = kk < 0 ? kk + (N_E - 1) : (kk > N_E - 2 ? kk - (N_E - 1) : kk);
+                    return (yv(w + 1) - yv(w)) / (A.logE[w + 1] - A.logE[w]);
+                };
+                double xl = A.logE[i], xh = A.logE[i + 1], h = xh - xl;
+                double b, c, d;
+                akima_piece(sec(i - 2), sec(i - 1), sec(i), sec(i + 1), sec(i + 2), h, b, c, d);
+                double yi = yv(i);
+                if (i == ia || i == ib) {
+                    double a1 = (i == ia) ? lo : xl;
+                    double a2 = (i == ib) ? hi : xh;
+                    v += akima_piece_integral(yi, b, c, d, xl, a1, a2);
+                } else {
+                    v += h * (yi + h * (0.5 * b + h * (c / 3.0 + 0.25 * d * h)));
+                }
+            }
+        }
+        total = (s == 0) ? v : total + v;
+    }
+    if (A.atten) total *= A.atten[j];
+    A.out[(size_t)reg * N_in * N_P + q] = total;
+}
+
+// folded[region][c][k] = sum_j M[c][j] * in[region][j][k]
+__global__ void __launch_bounds__(256) k_fold_f64(FoldGemmArgs A)
+{
+    __shared__ double Ms[16][65];
+    __shared__ double Xs[64][17];
+    const int reg = blockIdx.z;
+    const int c0 = blockIdx.y * 16, k0 = blockIdx.x * 16;
+    const int tc = threadIdx.x / 16, tk = threadIdx.x % 16;
+    const double* X = A.in + (size_t)reg * A.N_in * A.N_P;
+    double acc = 0.0;
+    for (int j0 = 0; j0 < A.N_in; j0 += 64) {
+        for (int q = threadIdx.x; q < 16 * 64; q += 256) {
+            int r = q / 64, cidx = q % 64;
+            int c = c0 + r, j = j0 + cidx;
+            Ms[r][cidx] = (c < A.N_ch && j < A.N_in) ? A.M[(size_t)c * A.ld_M + j] : 0.0;
+        }
+        for (int q = threadIdx.x; q < 64 * 16; q += 256) {
+            int r = q / 16, cidx = q % 16;
+            int j = j0 + r, k = k0 + cidx;
+            Xs[r][cidx] = (j < A.N_in && k < A.N_P) ? X[(size_t)j * A.N_P + k] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int jj = 0; jj < 64; jj++) acc += Ms[tc][jj] * Xs[jj][tk];
+        __syncthreads();
+    }
+    int c = c0 + tc, k = k0 + tk;
+    if (c < A.N_ch && k < A.N_P) A.out[((size_t)reg * A.N_ch + c) * A.N_P + k] = acc;
+}
+
+struct MargFn {
+    const double* star;
+    const double* data;
+    int n;
+    double SCALE, A;
+    __device__ double operator()(double B) const
+    {
+        double x = 0.0;
+        for (int j = 0; j < n; j++) {
+            double c = SCALE * (star[j] + B);
+            if (c > 0.0) x += data[j] * log(c) - c;
+            else if (eq12(c, 0.0) && eq12(data[j], 0.0)) {
+            } else return 0.0;
+        }
+        return exp(x - A);
+    }
+};
+
+// wave per (channel, request)
+__global__ void __launch_bounds__(64) k_marginal(MarginalArgs A)
+{
+    const int ch = blockIdx.x, rq = blockIdx.y, lane = threadIdx.x;
+    const int N_b = A.N_b;
+    extern __shared__ double sm[];
+    double* star = sm;              // N_b
+    double* data = star + N_b;      // N_b
+    const double n = (double)N_b, T = A.exposure, SCALE = T / n;
+    const double* counts = A.counts + (size_t)ch * N_b;
+    // expected count rate from each region, phase-binned (shifted, wrapped)
+    for (int j = lane; j < N_b; j += 64) {
+        double st = 0.0;
+        for (int c = 0; c < A.n_comp; c++) {
+            const double* pulse = A.signals + (((size_t)rq * A.n_comp + c) * A.N_ch + ch) * A.N_P;
+            const double* cph = A.comp_phases;
+            const double shift = A.shifts[rq * A.n_comp + c];
+            if (A.N_P > 1) {
+                double pa = A.phases[j] + shift, pb = A.phases[j + 1] + shift;
+                if (eq12(pb - pa, 1.0)) { pa = 0.0; pb = 1.0; }
+                else { pa -= floor(pa); pb -= floor(pb); }
+                if (pa < pb) {
+                    double v = akima_integ(cph, pulse, A.N_P, true, pa, pb);
+                    if (v > 0.0 || A.allow_negative) st += v;
+                } else {
+                    double v = akima_integ(cph, pulse, A.N_P, true, pa, 1.0);
+                    if (v > 0.0 || A.allow_negative) st += v;
+                    v = akima_integ(cph, pulse, A.N_P, true, 0.0, pb);
+                    if (v > 0.0 || A.allow_negative) st += v;
+                }
+            } else if (j == 0) {
+                st = pulse[0];
+            }
+        }
+        if (st < 0.0) st = 0.0;
+        if (A.background) st += A.background[(size_t)ch * N_b + j];
+        st *= n;
+        star[j] = st;
+        data[j] = counts[j];
+    }
+    __syncthreads();
+    double av_S = 0.0, av_D = 0.0;
+    for (int j = lane; j < N_b; j += 64) { av_S += star[j]; av_D += data[j]; }
+    av_S = wave_sum(av_S);
+    av_D = wave_sum(av_D);
+    const size_t ro = (size_t)rq * A.N_ch + ch;
+    int status = 0;
+    double ll = 0.0, B = 0.0;
+    if (!(A.slim < 0.0)) {
+        double limit = av_S * SCALE - A.slim * sqrt(av_S * SCALE) - av_D;
+        if (limit > 0.0) status = 1;
+    }
+    const double* sup = A.support + 2 * ch;
+    if (!status) {
+        av_S /= n;
+        av_D /= T;
+        if (eq12(av_D, 0.0) && eq12(av_S, 0.0)) {
+            double lower = 0.0;
+            if (lower < sup[0]) lower = sup[0];
+            double upper = 10.0 / T;
+            if (upper > sup[1] && sup[1] > 0.0) upper = sup[1];
+            B = 0.0;
+            ll = log((exp(-1.0 * lower * T) - exp(-1.0 * upper * T)) / T);
+        } else {
+            double B_min = 0.0;
+            B = av_D - av_S;
+            if (B <= B_min) {
+                double mc = -1.0;
+                for (int j = 0; j < N_b; j++) if (eq12(star[j], 0.0)) { mc = -2.0; break; }
+                if (eq12(mc, -2.0)) {
+                    for (int j = 0; j < N_b; j++)
+                        if ((eq12(mc, -2.0) && data[j] > 0.0) || (0.0 < data[j] && data[j] < mc)) mc = data[j];
+                }
+                if (!eq12(mc, -1.0)) {
+                    if (eq12(mc, -2.0)) { B = 0.0; B_min = 0.0; }
+                    else { B = 0.01 * mc / SCALE; B_min = 0.1 * B; }
+                } else B = B_min;
+            }
+            auto newton = [&](double Bv, double& stdv) {
+                double x = 0.0, y = 0.0;
+                for (int j = lane; j < N_b; j += 64) {
+                    y += data[j] / (star[j] + Bv);
+                    x += data[j] / pow(star[j] + Bv, 2.0);
+                }
+                x = wave_sum(x);
+                y = wave_sum(y);
+                y = 2.0 * T - 2.0 * y;
+                x *= 2.0;
+                stdv = sqrt(2.0 / x);
+                return -1.0 * y / x;
+            };
+            double stdv;
+            double dB = newton(B, stdv);
+            int counter = 0, iters = 0;
+            // the reference iterates without a cap; 10000 steps bound pathological inputs
+            while (fabs(dB) > A.epsilon * stdv && counter < 2 && iters++ < 10000) {
+                B += dB;
+                if (B < B_min) {
+                    if (B_min > 0.0) counter += 1; else counter = 2;
+                    B = B_min;
+                }
+                dB = newton(B, stdv);
+            }
+            double s = 0.0;
+            for (int j = lane; j < N_b; j += 64) s += data[j] / pow(star[j] + B, 2.0);
+            s = wave_sum(s);
+            s = s > 0.0 ? sqrt(1.0 / s) : 1e90;
+            double lower = B - A.sigmas * s, upper = B + A.sigmas * s;
+            if (lower < B_min) lower = B_min;
+            double Bi = B;
+            if (lower < sup[0]) {
+                lower = sup[0];
+                if (upper < sup[0] && sup[1] > 0.0) { upper = sup[1]; Bi = sup[0]; }
+                else if (upper < sup[0]) { upper = sup[0] + A.sigmas * s; Bi = sup[0]; }
+            }
+            if (upper > sup[1] && sup[1] > 0.0) {
+                upper = sup[1];
+                if (lower > sup[1]) { lower = sup[0]; Bi = sup[1]; }
+            }
+            if (Bi < lower) Bi = lower;
+            else if (Bi > upper) Bi = upper;
+            double Aval = 0.0;
+            for (int j = lane; j < N_b; j += 64) {
+                double c = SCALE * (star[j] + Bi);
+                if (c > 0.0) Aval += data[j] * log(c) - c;
+                else if (eq12(c, 0.0) && eq12(data[j], 0.0)) {
+                } else Aval += A.llzero;
+            }
+            Aval = wave_sum(Aval);
+            MargFn f{star, data, N_b, SCALE, Aval};
+            double result, abserr;
+            double er = A.epsrel < MARG_EPSREL ? A.epsrel : MARG_EPSREL;
+            wave_qag61(f, lower, upper, A.epsabs, er, QAG_LIMIT, result, abserr);
+            if (result > 0.0) ll = log(result) + Aval + A.precomp[ch];
+            else status = 2;
+        }
+    }
+    if (lane == 0) {
+        A.ll[ro] = ll;
+        A.status[ro] = status;
+        if (!status) {
+            A.B_ML[ro] = B * T;
+            double Bs = B;
+            if (B < sup[0]) Bs = sup[0];
+            else if (B > sup[1] && sup[1] > 0.0) Bs = sup[1];
+            A.B_ML_supp[ro] = Bs * T;
+        }
+    }
+    if (!status && A.STAR) {
+        double Bs = B;
+        if (B < sup[0]) Bs = sup[0];
+        else if (B > sup[1] && sup[1] > 0.0) Bs = sup[1];
+        for (int j = lane; j < N_b; j += 64) A.STAR[ro * N_b + j] = SCALE * (star[j] + Bs);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fill(double* p, size_t n, double v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_int(int* p, size_t n, int v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ordered sum over channels; a slim or failed channel makes the request
+// llzero-class (the reference's random draw near llzero)
+__global__ void __launch_bounds__(64) k_lnl_reduce(MarginalArgs A, const Member* __restrict__ members,
+                                                   int members_per_request, double* __restrict__ lnl)
+{
+    const int rq = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    int bad = 0;
+    for (int c = 0; c < A.N_ch; c++) {
+        size_t ro = (size_t)rq * A.N_ch + c;
+        if (A.status[ro]) { bad = 1; break; }
+        s += A.ll[ro];
+    }
+    if (A.request_status && A.request_status[rq]) bad = 1;
+    if (members)
+        for (int q = 0; q < members_per_request; q++) {
+            const Member& m = members[(size_t)rq * members_per_request + q];
+            if (m.active && m.status) bad = 1;
+        }
+    if (bad || s <= A.llzero) {
+        unsigned long long h = 0x9E3779B97F4A7C15ULL * (unsigned long long)(rq + 1 + A.seed);
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ULL;
+        h ^= h >> 27;
+        double u = (double)(h >> 11) / 9007199254740992.0;
+        s = A.llzero * (0.1 + 0.9 * u);
+    }
+    lnl[rq] = s;
+}
+
+}  // namespace xh

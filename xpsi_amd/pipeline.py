@@ -1,0 +1,174 @@
+"""Batched whole-likelihood evaluation on the GPU.
+
+LikelihoodPipeline holds one xh_pipeline (device buffers for `max_batch`
+parameter vectors) for a fixed model: instrument response, data, energy grid
+and resolution settings.  evaluate() maps a list of parameter vectors to
+log-likelihoods; each vector runs mesh construction, ray tables, pulse
+integration, energy integration, the response fold and the background
+marginalisation on the device (xpsi/Likelihood.py:297-432 for the reference
+flow).
+
+Model description (dict):
+  spacetime   frequency, star_shape ('AGM_14' | 'sphere')
+  hot_regions list of dicts: settings sqrt_num_cells, min/max_sqrt_num_cells,
+              num_rays, num_leaves, num_phases, is_antiphased, cede,
+              image_order_limit, atm_ext, plus default parameter values
+  instrument  matrix [N_ch, N_in], energy_edges [N_in + 1] (keV)
+  data        counts [N_ch, N_b], phases [N_b + 1], exposure_time
+  num_energies, support [N_ch, 2] or None, marginal settings, llzero,
+  interstellar: callable(mid energies) -> attenuation factors, or None
+A parameter vector is a dict: mass, radius, distance, cos_inclination and
+`hot_regions`: list of dicts phase_shift, super_colatitude, super_radius,
+super_temperature, cede_colatitude, cede_radius, cede_azimuth,
+cede_temperature; optional per-region sqrt_num_cells.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._lib import lib, check, ModelConfig, RegionConfig, Request, XHError
+
+_REGION_KEYS = ("phase_shift", "super_colatitude", "super_radius", "super_temperature",
+                "cede_colatitude", "cede_radius", "cede_azimuth", "cede_temperature")
+
+
+def construct_energy_array(num_energies, edges):
+    """Energy grid over one instrument waveband (Signal.py:1092-1197)."""
+    L, U = edges[0], edges[-1]
+    e = np.logspace(np.log10(L), np.log10(U), int(math.ceil(num_energies)), base=10.0)[::-1]
+    return np.ascontiguousarray(e[::-1])
+
+
+class LikelihoodPipeline:
+    def __init__(self, model, max_batch=1, max_sqrt_cells=0):
+        hrs = model["hot_regions"]
+        if not 1 <= len(hrs) <= 2:
+            raise XHError("one or two hot regions are supported")
+        h0 = hrs[0]
+        for h in hrs[1:]:
+            for key in ("num_rays", "num_leaves", "num_phases", "image_order_limit", "atm_ext"):
+                if h.get(key) != h0.get(key):
+                    raise XHError("hot regions must share %s" % key)
+        inst, data = model["instrument"], model["data"]
+        self.energies = construct_energy_array(model["num_energies"], inst["energy_edges"])
+        edges = np.ascontiguousarray(inst["energy_edges"], dtype=np.float64)
+        matrix = np.ascontiguousarray(inst["matrix"], dtype=np.float64)
+        counts = np.ascontiguousarray(data["counts"], dtype=np.float64)
+        dph = np.ascontiguousarray(data["phases"], dtype=np.float64)
+        support = model.get("support")
+        support = None if support is None else np.ascontiguousarray(support, dtype=np.float64)
+        att = None
+        if model.get("interstellar") is not None:
+            mids = 0.5 * (edges[:-1] + edges[1:])
+            att = np.ascontiguousarray(model["interstellar"](mids), dtype=np.float64)
+        self._keep = [self.energies, edges, matrix, counts, dph, support, att]
+        cfg = ModelConfig()
+        cfg.n_regions = len(hrs)
+        for k, h in enumerate(hrs):
+            cfg.region[k] = RegionConfig(int(bool(h.get("cede", False))),
+                                         int(bool(h.get("is_antiphased", False))),
+                                         int(h.get("sqrt_num_cells", 32)),
+                                         int(h.get("min_sqrt_num_cells", 10)),
+                                         int(h.get("max_sqrt_num_cells", 80)))
+        cfg.num_rays = int(h0.get("num_rays", 200))
+        cfg.num_leaves = int(h0.get("num_leaves", 64))
+        cfg.num_phases = int(h0.get("num_phases") or cfg.num_leaves)
+        cfg.image_order_limit = int(h0.get("image_order_limit") or 0)
+        cfg.atm_ext = int(h0.get("atm_ext", 1))
+        sp = model["spacetime"]
+        cfg.star_shape = {"AGM_14": 0, "sphere": 1}[sp.get("star_shape", "AGM_14")]
+        cfg.frequency = float(sp["frequency"])
+        cfg.mode_frequency = float(sp.get("mode_frequency", sp["frequency"]))
+        cfg.N_E = len(self.energies)
+        cfg.energies = self.energies.ctypes.data
+        cfg.N_in = len(edges) - 1
+        cfg.energy_edges = edges.ctypes.data
+        cfg.attenuation = None if att is None else att.ctypes.data
+        cfg.N_ch = matrix.shape[0]
+        cfg.matrix = matrix.ctypes.data
+        cfg.N_b = counts.shape[1]
+        cfg.data_phases = dph.ctypes.data
+        cfg.counts = counts.ctypes.data
+        cfg.exposure_time = float(data["exposure_time"])
+        cfg.support = None if support is None else support.ctypes.data
+        ms = model.get("marginal", {})
+        cfg.epsabs = float(ms.get("epsabs", 0.0))
+        cfg.epsrel = float(ms.get("epsrel", 1.0e-8))
+        cfg.epsilon = float(ms.get("epsilon", 1.0e-3))
+        cfg.sigmas = float(ms.get("sigmas", 10.0))
+        cfg.slim = float(ms.get("slim", 20.0))
+        cfg.llzero = float(model.get("llzero", -1.0e90))
+        cfg.max_batch = int(max_batch)
+        cfg.max_sqrt_cells = int(max_sqrt_cells)
+        self.model = model
+        self.max_batch = int(max_batch)
+        self.n_regions = len(hrs)
+        self.N_P = cfg.num_phases
+        self.N_ch, self.N_b = cfg.N_ch, cfg.N_b
+        self._h = lib().xh_pipeline_create(C.byref(cfg))
+        if not self._h:
+            raise XHError("xh_pipeline_create: " + lib().xh_last_error().decode())
+        self._reqs = (Request * self.max_batch)()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().xh_pipeline_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _fill(self, k, p):
+        r = self._reqs[k]
+        r.mass, r.radius = float(p["mass"]), float(p["radius"])
+        r.distance, r.cos_inclination = float(p["distance"]), float(p["cos_inclination"])
+        for g, hp in enumerate(p["hot_regions"]):
+            for q, key in enumerate(_REGION_KEYS):
+                v = hp.get(key)
+                if v is None:
+                    v = hp["super_colatitude"] if key == "cede_colatitude" else 0.0
+                r.region[g][q] = float(v)
+            r.sqrt_num_cells[g] = int(hp.get("sqrt_num_cells", 0))
+
+    def launch(self, params, stream=None):
+        n = len(params)
+        if n > self.max_batch:
+            raise XHError("batch of %d exceeds max_batch %d" % (n, self.max_batch))
+        for k, p in enumerate(params):
+            self._fill(k, p)
+        check(lib().xh_pipeline_launch(self._h, n, self._reqs, stream), "xh_pipeline_launch")
+        self._n = n
+
+    def wait(self):
+        check(lib().xh_pipeline_sync(self._h), "xh_pipeline_sync")
+        res = lib().xh_pipeline_results(self._h)
+        return np.array([res[k] for k in range(self._n)])
+
+    def evaluate(self, params, stream=None):
+        out = []
+        for s in range(0, len(params), self.max_batch):
+            self.launch(params[s:s + self.max_batch], stream)
+            out.append(self.wait())
+        return np.concatenate(out)
+
+    def fetch(self, what, req=0, index=0, size=None):
+        names = {"flux": 0, "folded": 1, "sizes": 2, "cellArea": 3, "deflection": 4,
+                 "lag": 5, "theta": 6, "phi": 7, "STAR": 8}
+        size = size or 4 * 1024 * 1024
+        buf = np.zeros(size)
+        check(lib().xh_pipeline_fetch(self._h, names[what], req, index, buf.ctypes.data, size),
+              "xh_pipeline_fetch")
+        return buf
+
+
+def params_from_model(model):
+    """The parameter vector stored in a model description."""
+    sp = model["spacetime"]
+    return dict(mass=sp["mass"], radius=sp["radius"], distance=sp["distance"],
+                cos_inclination=sp["cos_inclination"],
+                hot_regions=[{k: h.get(k) for k in _REGION_KEYS + ("sqrt_num_cells",)
+                              if h.get(k) is not None} for h in model["hot_regions"]])
