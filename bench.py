@@ -1,0 +1,161 @@
+"""Benchmark: batched lnL evaluations/s on one MI355X.
+
+Workload (the batched form): 4096 parameter vectors drawn from the prior with
+uneven resolutions (sqrt_num_cells in {24,32,48,64,96}, num_energies in
+{64,128,256}), one blackbody hot spot, 100-channel Gaussian response, 32
+phase bins; vectors failing the compactness prior are answered without
+evaluation and not counted.  A step evaluates every accepted vector once
+(one launch per energy count).  Beside it: the plain FP64 GPU path one
+request at a time (config-1 request) with p50/p99 latency, and the CPU
+oracle on the host cores.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--single", type=int, default=40, help="plain single-request evaluations")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--n", type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the likelihood path shards requests across GPUs "
+              "with no exchange (see DESIGN.md); run it with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from xpsi_amd import configs
+    from xpsi_amd.pipeline import LikelihoodPipeline, params_from_model
+
+    torch.cuda.init()
+    dev = torch.cuda.get_device_name(0)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    # ---------------- plain FP64 GPU path, one request at a time (config 1)
+    m1 = configs.config1()
+    p1 = params_from_model(m1)
+    pl1 = LikelihoodPipeline(m1, max_batch=1)
+    for _ in range(3):
+        pl1.evaluate([p1], stream)
+    lat = []
+    for _ in range(args.single):
+        t0 = time.perf_counter()
+        l1 = pl1.evaluate([p1], stream)[0]
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat) * 1e3
+    single_rate = 1e3 / lat.mean()
+
+    # ---------------- batched form
+    draws = configs.config3_draws(args.n)
+    acc = [d for d in draws if configs.prior_accepts(d["mass"], d["radius"], 300.0)]
+    groups = {}
+    for d in acc:
+        groups.setdefault(d["num_energies"], []).append(d)
+    pls = {}
+    for ne, g in sorted(groups.items()):
+        pls[ne] = LikelihoodPipeline(configs.config3_model(ne), max_batch=len(g))
+    res = {}
+
+    def step():
+        for ne, g in sorted(groups.items()):
+            pls[ne].launch(g, stream)
+        for ne, g in sorted(groups.items()):
+            res[ne] = pls[ne].wait()
+
+    for _ in range(args.warmup):
+        step()
+    for pl in pls.values():
+        pl.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stage = {k: 0.0 for k in LikelihoodPipeline.STAGES}
+    for pl in pls.values():
+        for k, v in pl.stage_ms().items():
+            stage[k] += v
+    # work of the dominant kernel (profile and phase fold), per step
+    evals = 0.0
+    for ne, g in sorted(groups.items()):
+        w = pls[ne].work(len(g))
+        evals += float(w.sum()) * ne * pls[ne].N_P
+    fold_ms_per_step = stage["profile_fold"] / args.steps
+    flops = 8.0 * evals
+    achieved = flops / (fold_ms_per_step * 1e-3) / 1e12
+    n_eval = len(acc)
+    value = n_eval * args.steps / dt
+    lnl_all = np.concatenate([res[ne] for ne in sorted(groups)])
+    finite = lnl_all > -1e89
+
+    # the same vectors alone on the single-request pipelines: identical results
+    alone_dev = 0.0
+    for ne, g in sorted(groups.items()):
+        pl_a = LikelihoodPipeline(configs.config3_model(ne), max_batch=1)
+        for k in range(0, len(g), max(1, len(g) // 4)):
+            la = pl_a.evaluate([g[k]], stream)[0]
+            if la > -1e89 and res[ne][k] > -1e89:
+                alone_dev = max(alone_dev, abs(la - res[ne][k]))
+        pl_a.close()
+
+    out = {
+        "metric": "batched lnL evals/s + p50/p99 latency vs plain FP64 GPU, at |dlnL|<1e-3",
+        "value": value, "unit": "lnL evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched form: %d prior draws (%d accepted), uneven "
+                               "sqrt_num_cells {24..96} x num_energies {64,128,256}, BB spot, "
+                               "100 ch x 32 bins" % (args.n, n_eval),
+                   "global_batch": n_eval, "parallelism": "none (1 GPU)"},
+        "plain_fp64_gpu_single": {"evals_per_s": single_rate, "p50_ms": float(np.percentile(lat, 50)),
+                                  "p99_ms": float(np.percentile(lat, 99)),
+                                  "workload": "config 1 request, batch 1", "lnl": float(l1)},
+        "speedup_vs_plain_fp64_gpu": value / single_rate,
+        "batch_latency_ms": {"p50": dt / args.steps * 1e3},
+        "max_abs_dlnl_alone_vs_batched": alone_dev,
+        "finite_fraction": float(finite.mean()),
+        "stage_ms_per_step": {k: v / args.steps for k, v in stage.items()},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_profile_fold", "evals_per_step": evals,
+                     "flop_per_eval": 8},
+        "device": dev,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        nc = 0
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            O.loglikelihood(m1)
+            nc += 1
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": nc / cdt, "unit": "lnL evals/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "%d config-1 likelihood evaluations (single request, "
+                                         "one thread) with the C oracle" % nc,
+                               "host_cpus": os.cpu_count()}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -148,6 +148,19 @@ int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs, void* stre
 int xh_pipeline_sync(xh_pipeline* p);
 const double* xh_pipeline_results(xh_pipeline* p);
 
+/* Per-stage device time: record HIP events around the kernel stages of every
+ * launch (on the launch stream) when on != 0; xh_pipeline_stage_ms() returns
+ * the accumulated milliseconds of mesh, rays, geometry, profile-and-fold,
+ * energy-and-response and marginal stages (6 values) since the last reset. */
+int xh_pipeline_set_timing(xh_pipeline* p, int on);
+int xh_pipeline_stage_ms(xh_pipeline* p, double* out, int n);
+/* Members (super + ceding) per request. */
+int xh_pipeline_members_per_request(xh_pipeline* p);
+/* Work accounting of the last launch of n requests: per member, the number
+ * of (radiating cell, parallel, image order) triples folded; the phase-fold
+ * evaluation count is that times N_E * N_P.  out: host [n * members]. */
+int xh_pipeline_work(xh_pipeline* p, int n, double* out);
+
 /* Copy an intermediate array of request `req` to host memory for checks.
  * what: 0 flux[member], 1 folded signal[region] [N_ch, N_P],
  *       2 member sizes (S, status) as doubles, 3 cellArea[member] [S,S],

@@ -82,6 +82,14 @@ def lib():
     L.xh_pipeline_sync.argtypes = [vp]
     L.xh_pipeline_results.restype = C.POINTER(d)
     L.xh_pipeline_results.argtypes = [vp]
+    L.xh_pipeline_set_timing.restype = i32
+    L.xh_pipeline_set_timing.argtypes = [vp, i32]
+    L.xh_pipeline_stage_ms.restype = i32
+    L.xh_pipeline_stage_ms.argtypes = [vp, vp, i32]
+    L.xh_pipeline_members_per_request.restype = i32
+    L.xh_pipeline_members_per_request.argtypes = [vp]
+    L.xh_pipeline_work.restype = i32
+    L.xh_pipeline_work.argtypes = [vp, i32, vp]
     L.xh_pipeline_fetch.restype = i32
     L.xh_pipeline_fetch.argtypes = [vp, i32, i32, i32, vp, sz]
     _lib = L
@@ -98,4 +106,5 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_marginal_workspace_bytes", "xh_eval_marginal_likelihood",
             "xh_pipeline_create", "xh_pipeline_destroy", "xh_pipeline_eval",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
-            "xh_pipeline_fetch"]
+            "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",
+            "xh_pipeline_members_per_request", "xh_pipeline_work"]

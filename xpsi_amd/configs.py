@@ -51,3 +51,63 @@ def config1(counts=None):
         num_energies=128, support=None,
         marginal=dict(epsabs=0.0, epsrel=1.0e-8, epsilon=1.0e-3, sigmas=10.0),
         llzero=-1.0e90)
+
+
+def prior_accepts(mass, radius, frequency):
+    """Compactness and shape cuts of examples/examples_fast/Modules/CustomPrior.py:49-66
+    (with the spacetime derived quantities of xpsi/Spacetime.py:111-190)."""
+    GM = 0.5 * 2.95325024e3
+    c, G = 2.99792458e8, 6.6740831e-11
+    if not radius <= 16.0:
+        return False
+    r_g = mass * GM
+    M = mass * GM * c * c / G
+    R = radius * 1.0e3
+    Omega = 2.0 * math.pi * frequency
+    zeta = r_g / R
+    eps = Omega ** 2.0 * R ** 3.0 / (G * M)
+    R_r_s = R / (2.0 * r_g)
+    R_p = 1.0 + eps * (-0.788 + 1.030 * zeta)
+    if R_p < 1.505 / R_r_s:
+        return False
+    arg = -1.0 / (3.0 * eps * (-0.788 + 1.030 * zeta))
+    if arg >= 0.0 and math.sqrt(arg) < 1.0:
+        return False
+    return True
+
+
+def config3_draws(n=4096, seed=3):
+    """Parameter vectors of the batched workload: uniform prior draws
+    (M 1-3, R 8-16 km, cos i 0-1, colatitude 0-pi, radius 0.05-1 rad,
+    log T 5.5-6.7, phase U[0,1), distance 1 kpc), sqrt_num_cells from
+    {24, 32, 48, 64, 96} and num_energies from {64, 128, 256}; drawn with
+    numpy.random.default_rng(seed) in that order."""
+    rng = np.random.default_rng(seed)
+    M = rng.uniform(1.0, 3.0, n)
+    R = rng.uniform(8.0, 16.0, n)
+    ci = rng.uniform(0.0, 1.0, n)
+    colat = rng.uniform(0.0, math.pi, n)
+    rho = rng.uniform(0.05, 1.0, n)
+    logT = rng.uniform(5.5, 6.7, n)
+    phase = rng.uniform(0.0, 1.0, n)
+    snc = rng.choice([24, 32, 48, 64, 96], n)
+    ne = rng.choice([64, 128, 256], n)
+    out = []
+    for k in range(n):
+        out.append(dict(mass=M[k], radius=R[k], distance=1.0, cos_inclination=ci[k],
+                        num_energies=int(ne[k]),
+                        hot_regions=[dict(phase_shift=phase[k], super_colatitude=colat[k],
+                                          super_radius=rho[k], super_temperature=logT[k],
+                                          sqrt_num_cells=int(snc[k]))]))
+    return out
+
+
+def config3_model(num_energies, counts=None):
+    """Model of the batched workload for one energy count: config-1 response,
+    data and atmosphere; mesh bound raised to 128 cells per side."""
+    m = config1(counts)
+    m["num_energies"] = int(num_energies)
+    h = m["hot_regions"][0]
+    h["max_sqrt_num_cells"] = 128
+    h["image_order_limit"] = None
+    return m

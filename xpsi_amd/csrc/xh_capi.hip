@@ -231,6 +231,11 @@ struct xh_pipeline {
     xh_model_config cfg;
     Layout L;
     int nreg, mpr, maxb, N_in, N_ch, N_b, EC, table_n;
+    int moff[3];                 // first member of each region within a request
+    bool timing;
+    hipEvent_t ev[16];
+    double stage_ms[8];
+    double* d_work;
     std::vector<void*> allocs;
     Member* d_members;
     RegionAlloc* d_regions;
@@ -263,6 +268,7 @@ extern "C" void xh_pipeline_destroy(xh_pipeline* p)
 {
     if (!p) return;
     for (void* v : p->allocs) (void)hipFree(v);
+    for (int e = 0; e < 16; e++) if (p->ev[e]) (void)hipEventDestroy(p->ev[e]);
     if (p->h_members) (void)hipHostFree(p->h_members);
     if (p->h_regions) (void)hipHostFree(p->h_regions);
     if (p->h_shifts) (void)hipHostFree(p->h_shifts);
@@ -280,12 +286,12 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         fail(XH_EINVAL, "xh_pipeline_create: bad configuration");
         return nullptr;
     }
-    xh_pipeline* p = new xh_pipeline();
-    std::memset((void*)p, 0, sizeof(*p) - sizeof(p->allocs) - sizeof(p->stream) - sizeof(p->last_n));
-    new (&p->allocs) std::vector<void*>();
+    xh_pipeline* p = new xh_pipeline();   // value-initialised: every field zero
     p->cfg = *c;
     p->nreg = c->n_regions;
-    p->mpr = 2 * c->n_regions;
+    p->moff[0] = 0;
+    for (int g = 0; g < c->n_regions; g++) p->moff[g + 1] = p->moff[g] + 1 + (c->region[g].cede ? 1 : 0);
+    p->mpr = p->moff[c->n_regions];
     p->maxb = c->max_batch;
     p->N_in = c->N_in; p->N_ch = c->N_ch; p->N_b = c->N_b;
     p->table_n = 1000;
@@ -293,7 +299,7 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     for (int r = 0; r < c->n_regions; r++) smax = std::max(smax, c->region[r].max_sqrt_num_cells);
     smax += smax & 1;
     const int N_P = c->num_phases > 0 ? c->num_phases : c->num_leaves;
-    const int IO = c->image_order_limit > 0 ? c->image_order_limit : 4;
+    const int IO = c->image_order_limit > 0 ? c->image_order_limit : 8;
     p->L = Layout{smax, c->num_rays, c->num_leaves, N_P, c->N_E, IO, c->image_order_limit > 0 ? c->image_order_limit : 0,
                   c->atm_ext};
     p->EC = pick_ec(c->num_leaves);
@@ -322,7 +328,8 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     PCHK(dalloc(p, &p->d_geo, nm * S * IO * 4 * c->num_leaves));
     PCHK(dalloc(p, &p->d_valid, nm * S * IO));
     PCHK(dalloc(p, &p->d_flux, nm * NE * N_P));
-    PCHK(dalloc(p, &p->d_member_index, nm));
+    PCHK(dalloc(p, &p->d_member_index, ng * 2));
+    PCHK(dalloc(p, &p->d_work, nm));
     PCHK(dalloc(p, &p->d_energies, NE));
     PCHK(dalloc(p, &p->d_logE, NE));
     PCHK(dalloc(p, &p->d_logEdges, (size_t)c->N_in + 1));
@@ -380,9 +387,16 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         sup[2 * i + 1] = c->support ? c->support[2 * i + 1] : -1.0;
     }
     PHIP(hipMemcpy(p->d_support, sup.data(), sup.size() * 8, hipMemcpyHostToDevice));
-    std::vector<int> midx(nm);
-    for (size_t i = 0; i < nm; i++) midx[i] = (int)i;
-    PHIP(hipMemcpy(p->d_member_index, midx.data(), nm * sizeof(int), hipMemcpyHostToDevice));
+    // energy integration groups: the members of each (request, region)
+    std::vector<int> midx(ng * 2);
+    for (int rq = 0; rq < p->maxb; rq++)
+        for (int g = 0; g < p->nreg; g++) {
+            int gi = rq * p->nreg + g, m0 = rq * p->mpr + p->moff[g];
+            midx[2 * gi] = m0;
+            midx[2 * gi + 1] = (p->moff[g + 1] - p->moff[g] > 1) ? m0 + 1 : -1;
+        }
+    PHIP(hipMemcpy(p->d_member_index, midx.data(), ng * 2 * sizeof(int), hipMemcpyHostToDevice));
+    for (int e = 0; e < 16; e++) PHIP(hipEventCreate(&p->ev[e]));
     return p;
 }
 
@@ -405,9 +419,10 @@ static void describe(const xh_pipeline* p, int rq, const xh_request& r)
         const bool cede = c.region[g].cede != 0;
         const double cc = cede ? q[4] : sc, cr = cede ? q[5] : 0.0, ca = cede ? q[6] : 0.0, cT = q[7];
         const int gi = rq * p->nreg + g;
-        const int m0 = gi * 2, m1 = gi * 2 + 1;
-        for (int s = 0; s < 2; s++) {
-            Member& m = p->h_members[gi * 2 + s];
+        const int m0 = rq * p->mpr + p->moff[g], m1 = m0 + 1;
+        const int nmem = p->moff[g + 1] - p->moff[g];
+        for (int s = 0; s < nmem; s++) {
+            Member& m = p->h_members[m0 + s];
             std::memset((void*)&m, 0, sizeof(Member));
             m.R_eq = R; m.epsilon = eps; m.zeta = zeta; m.Omega = Omega; m.r_s = 2.0 * r_g;
             m.inclination = incl; m.mode_freq = c.mode_frequency; m.d_sq = d * d; m.shape = c.star_shape;
@@ -433,7 +448,7 @@ static void describe(const xh_pipeline* p, int rq, const xh_request& r)
         A.R_eq = R; A.Omega = Omega; A.zeta = zeta; A.epsilon = eps; A.shape = c.star_shape;
         A.num_cells = snc * snc; A.min_sqrt = c.region[g].min_sqrt_num_cells;
         A.max_sqrt = c.region[g].max_sqrt_num_cells;
-        A.member0 = m0; A.member1 = (cede && cr > 0.0) ? m1 : -1;
+        A.member0 = m0; A.member1 = (nmem > 1 && cr > 0.0) ? m1 : -1;
         p->h_shifts[gi] = shift;
     }
 }
@@ -455,6 +470,7 @@ extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs,
     HIPCHK(fill(p->d_flux, (size_t)nm * N_E * N_P, 0.0, s));
     HIPCHK(fill_int(p->d_ring_ok, (size_t)nm * S, 0, s));
     HIPCHK(fill_int(p->d_valid, (size_t)nm * S * L.IO_max, 0, s));
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[0], s));
     hipLaunchKernelGGL(k_allocate, dim3(ng), dim3(256), 0, s, p->d_regions, p->d_members, S);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_mesh_table, dim3((p->table_n + 1 + 3) / 4, nm), dim3(256), 0, s, p->d_members,
@@ -467,6 +483,7 @@ extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs,
     hipLaunchKernelGGL(k_ring_finalize, dim3(nm), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_cellArea,
                        p->d_u, p->d_ring_ok, p->d_logT);
     HIPCHK(hipGetLastError());
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[1], s));
     hipLaunchKernelGGL(k_rays, dim3((L.N_R - 1 + 3) / 4, S, nm), dim3(256), 0, s, p->d_members, L, p->d_r,
                        p->d_maxAlpha, p->d_cosa, p->d_defl, p->d_lag);
     HIPCHK(hipGetLastError());
@@ -475,16 +492,19 @@ extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs,
     HIPCHK(hipGetLastError());
     RingArgs RA{p->d_theta, p->d_r, p->d_u, p->d_cos_gamma, p->d_maxDefl, p->d_defl, p->d_cosa, p->d_lag,
                 p->d_leaves, p->d_ring_ok, p->d_geo, p->d_valid};
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[2], s));
     hipLaunchKernelGGL(k_geometry, dim3(S, L.IO_max, nm), dim3(256), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
     HIPCHK(hipGetLastError());
     // parallels per block: all of them when the batch fills the GPU
     int rb = (n >= 32) ? S : 1;
     FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_energies, p->d_phases,
                 p->d_flux, rb};
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
     HIPCHK(launch_fold(p->EC, dim3((N_E + p->EC - 1) / p->EC, (S + rb - 1) / rb, nm), fold_lds(L.N_L, p->EC, S), s,
                        p->d_members, L, FA));
     EnergyArgs EA{p->d_flux, p->d_logE, p->d_logEdges, p->cfg.attenuation ? p->d_atten : nullptr,
                   p->d_member_index, p->d_integrated, N_E, N_P, p->N_in + 1, 2};
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[4], s));
     hipLaunchKernelGGL(k_energy_integrate, dim3((p->N_in * N_P + 255) / 256, ng), dim3(256), 0, s, p->d_members, EA);
     HIPCHK(hipGetLastError());
     FoldGemmArgs GA{p->d_M, p->d_integrated, p->d_folded, p->N_ch, p->N_in, N_P, p->N_in};
@@ -498,10 +518,12 @@ extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs,
     MA.exposure = p->cfg.exposure_time; MA.epsabs = p->cfg.epsabs; MA.epsrel = p->cfg.epsrel;
     MA.epsilon = p->cfg.epsilon; MA.sigmas = p->cfg.sigmas; MA.llzero = p->cfg.llzero; MA.slim = p->cfg.slim;
     MA.N_ch = p->N_ch; MA.N_b = p->N_b; MA.N_P = N_P; MA.n_comp = p->nreg; MA.allow_negative = 0; MA.seed = 0;
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[5], s));
     hipLaunchKernelGGL(k_marginal, dim3(p->N_ch, n), dim3(64), 2 * p->N_b * 8, s, MA);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_lnl_reduce, dim3(n), dim3(64), 0, s, MA, (const Member*)p->d_members, p->mpr, p->d_lnl);
     HIPCHK(hipGetLastError());
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[6], s));
     HIPCHK(hipMemcpyAsync(p->h_lnl, p->d_lnl, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     return XH_OK;
 }
@@ -509,6 +531,38 @@ extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs,
 extern "C" int xh_pipeline_sync(xh_pipeline* p)
 {
     HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->timing) {
+        for (int k = 0; k < 6; k++) {
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, p->ev[k], p->ev[k + 1]));
+            p->stage_ms[k] += ms;
+        }
+    }
+    return XH_OK;
+}
+
+extern "C" int xh_pipeline_set_timing(xh_pipeline* p, int on)
+{
+    p->timing = on != 0;
+    for (int k = 0; k < 8; k++) p->stage_ms[k] = 0.0;
+    return XH_OK;
+}
+
+extern "C" int xh_pipeline_stage_ms(xh_pipeline* p, double* out, int n)
+{
+    for (int k = 0; k < n && k < 8; k++) out[k] = p->stage_ms[k];
+    return XH_OK;
+}
+
+extern "C" int xh_pipeline_members_per_request(xh_pipeline* p) { return p->mpr; }
+
+extern "C" int xh_pipeline_work(xh_pipeline* p, int n, double* out)
+{
+    const int nm = n * p->mpr;
+    hipLaunchKernelGGL(k_work_count, dim3(nm), dim3(256), 0, p->stream, (const Member*)p->d_members, p->L,
+                       (const double*)p->d_cellArea, (const int*)p->d_valid, p->d_work);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, p->d_work, (size_t)nm * 8, hipMemcpyDeviceToHost));
     return XH_OK;
 }
 
@@ -528,7 +582,11 @@ extern "C" int xh_pipeline_fetch(xh_pipeline* p, int what, int req, int index, d
 {
     const Layout L = p->L;
     const size_t S = L.S_max;
-    const int mi = req * p->mpr + index;
+    if (req < 0 || req >= p->maxb) return fail(XH_EINVAL, "xh_pipeline_fetch: request out of range");
+    const bool by_region = (what == 1 || what == 8);
+    if (index < 0 || index >= (by_region ? p->nreg : p->mpr))
+        return fail(XH_EINVAL, "xh_pipeline_fetch: index out of range");
+    const int mi = req * p->mpr + (by_region ? 0 : index);
     const double* src = nullptr;
     size_t cnt = 0;
     Member m{};

@@ -335,6 +335,36 @@ __global__ void __launch_bounds__(256) k_profile_fold(const Member* __restrict__
     }
 }
 
+// per member: sum over parallels and contributing image orders of the number
+// of radiating cells (the cell x phase x energy evaluation count / (N_E N_P))
+__global__ void __launch_bounds__(256) k_work_count(const Member* __restrict__ members, Layout L,
+                                                    const double* __restrict__ cellArea,
+                                                    const int* __restrict__ valid, double* __restrict__ work)
+{
+    const int mi = blockIdx.x;
+    const Member m = members[mi];
+    __shared__ double acc;
+    if (threadIdx.x == 0) acc = 0.0;
+    __syncthreads();
+    if (m.active && !m.status) {
+        double w = 0.0;
+        for (int i = threadIdx.x; i < m.S; i += blockDim.x) {
+            const double* ca = cellArea + (size_t)mi * L.S_max * L.S_max + (size_t)i * m.S;
+            int c = 0;
+            for (int j = 0; j < m.S; j++) c += ca[j] > 0.0;
+            int imgs = 0;
+            for (int I = 0; I < L.IO_max; I++) {
+                if (!valid[((size_t)mi * L.S_max + i) * L.IO_max + I]) break;
+                imgs++;
+            }
+            w += (double)c * imgs;
+        }
+        atomicAdd(&acc, w);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) work[mi] = acc;
+}
+
 template __global__ void k_profile_fold<64, 16>(const Member*, Layout, FoldArgs);
 template __global__ void k_profile_fold<32, 16>(const Member*, Layout, FoldArgs);
 template __global__ void k_profile_fold<16, 16>(const Member*, Layout, FoldArgs);

@@ -88,5 +88,7 @@ __global__ void k_geometry(Member* members, Layout L, RingArgs A);
 template <int EC, int MAXK>
 __global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
 __global__ void k_fill(double* p, size_t n, double v);
+__global__ void k_work_count(const Member* members, Layout L, const double* cellArea, const int* valid,
+                             double* work);
 __global__ void k_fill_int(int* p, size_t n, int v);
 }

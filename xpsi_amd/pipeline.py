@@ -155,6 +155,22 @@ class LikelihoodPipeline:
             out.append(self.wait())
         return np.concatenate(out)
 
+    def set_timing(self, on=True):
+        check(lib().xh_pipeline_set_timing(self._h, int(on)), "xh_pipeline_set_timing")
+
+    STAGES = ("mesh", "rays", "geometry", "profile_fold", "energy_fold", "marginal")
+
+    def stage_ms(self):
+        out = np.zeros(6)
+        check(lib().xh_pipeline_stage_ms(self._h, out.ctypes.data, 6), "xh_pipeline_stage_ms")
+        return dict(zip(self.STAGES, out))
+
+    def work(self, n):
+        mpr = lib().xh_pipeline_members_per_request(self._h)
+        out = np.zeros(n * mpr)
+        check(lib().xh_pipeline_work(self._h, n, out.ctypes.data), "xh_pipeline_work")
+        return out.reshape(n, mpr)
+
     def fetch(self, what, req=0, index=0, size=None):
         names = {"flux": 0, "folded": 1, "sizes": 2, "cellArea": 3, "deflection": 4,
                  "lag": 5, "theta": 6, "phi": 7, "STAR": 8}
