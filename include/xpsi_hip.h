@@ -161,6 +161,11 @@ int xh_pipeline_members_per_request(xh_pipeline* p);
  * evaluation count is that times N_E * N_P.  out: host [n * members]. */
 int xh_pipeline_work(xh_pipeline* p, int n, double* out);
 
+/* Tuning: energies per block of the profile-and-fold kernel (8, 16, 32, 64;
+ * 0 = chosen from the leaf count).  Applies to pipelines created afterwards
+ * and to xh_integrate calls. */
+int xh_set_fold_energy_chunk(int ec);
+
 /* Copy an intermediate array of request `req` to host memory for checks.
  * what: 0 flux[member], 1 folded signal[region] [N_ch, N_P],
  *       2 member sizes (S, status) as doubles, 3 cellArea[member] [S,S],

@@ -90,6 +90,8 @@ def lib():
     L.xh_pipeline_members_per_request.argtypes = [vp]
     L.xh_pipeline_work.restype = i32
     L.xh_pipeline_work.argtypes = [vp, i32, vp]
+    L.xh_set_fold_energy_chunk.restype = i32
+    L.xh_set_fold_energy_chunk.argtypes = [i32]
     L.xh_pipeline_fetch.restype = i32
     L.xh_pipeline_fetch.argtypes = [vp, i32, i32, i32, vp, sz]
     _lib = L
@@ -107,4 +109,5 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_pipeline_create", "xh_pipeline_destroy", "xh_pipeline_eval",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",
-            "xh_pipeline_members_per_request", "xh_pipeline_work"]
+            "xh_pipeline_members_per_request", "xh_pipeline_work",
+            "xh_set_fold_energy_chunk"]

@@ -35,41 +35,73 @@ extern "C" const char* xh_last_error(void) { return g_err.c_str(); }
 static const double kC = 2.99792458e8, kKPC = 3.08567758e19, kGM = 0.5 * 2.95325024e3,
                     kG = 6.6740831e-11, kKM = 1.0e3;
 
+static int g_fold_ec = 0;   // 0: choose from the leaf count
+
 static int pick_ec(int N_L)
 {
-    // the profile and its three coefficient arrays for EC energies: <= 64 KiB
+    if (g_fold_ec) {
+        int ec = g_fold_ec;
+        while (ec > 8 && (size_t)5 * N_L * ec * 8 > 150 * 1024) ec >>= 1;
+        return ec;
+    }
+    // profile, slopes and three coefficient arrays for EC energies: <= 40 KiB,
+    // leaving room for three to four blocks per compute unit
     int ec = 64;
-    while (ec > 8 && (size_t)4 * N_L * ec * 8 > 65536) ec >>= 1;
+    while (ec > 8 && (size_t)5 * N_L * ec * 8 > 40960) ec >>= 1;
     return ec;
+}
+
+// tuning hook: energies per fold block (8, 16, 32 or 64; 0 = automatic)
+extern "C" int xh_set_fold_energy_chunk(int ec)
+{
+    if (ec != 0 && ec != 8 && ec != 16 && ec != 32 && ec != 64) return fail(XH_EINVAL, "chunk must be 0, 8, 16, 32 or 64");
+    g_fold_ec = ec;
+    return XH_OK;
 }
 
 static size_t fold_lds(int N_L, int EC, int S_max)
 {
-    return ((size_t)N_L * EC + 3 * (size_t)(N_L - 1) * EC + 4 * (size_t)N_L + 2 * (size_t)S_max) * 8;
+    return ((size_t)N_L * EC + 4 * (size_t)(N_L - 1) * EC + 4 * (size_t)N_L + 2 * (size_t)S_max) * 8;
 }
 
 static size_t geo_lds(int N_R, int N_L) { return (5 * (size_t)N_R + 4 * (size_t)N_L) * 8 + (N_L / 2 + 2) * 4; }
 
-template <int EC>
+template <int EC, int K>
 static hipError_t launch_fold_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void*)&k_profile_fold<EC, 16>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            160 * 1024);
+        (void)hipFuncSetAttribute((const void*)&k_profile_fold<EC, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((k_profile_fold<EC, 16>), g, dim3(256), lds, s, m, L, F);
+    hipLaunchKernelGGL((k_profile_fold<EC, K>), g, dim3(256), lds, s, m, L, F);
     return hipGetLastError();
+}
+
+// phases per lane: the smallest of 4, 8, 16 covering N_P at 256/EC phase slots
+static int fold_k(int EC, int N_P)
+{
+    int per = (N_P + (256 / EC) - 1) / (256 / EC);
+    return per <= 4 ? 4 : (per <= 8 ? 8 : 16);
+}
+
+template <int EC>
+static hipError_t launch_fold_e(int K, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+{
+    if (K == 4) return launch_fold_t<EC, 4>(g, lds, s, m, L, F);
+    if (K == 8) return launch_fold_t<EC, 8>(g, lds, s, m, L, F);
+    return launch_fold_t<EC, 16>(g, lds, s, m, L, F);
 }
 
 static hipError_t launch_fold(int EC, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
+    int K = fold_k(EC, L.N_P);
     switch (EC) {
-    case 64: return launch_fold_t<64>(g, lds, s, m, L, F);
-    case 32: return launch_fold_t<32>(g, lds, s, m, L, F);
-    case 16: return launch_fold_t<16>(g, lds, s, m, L, F);
-    default: return launch_fold_t<8>(g, lds, s, m, L, F);
+    case 64: return launch_fold_e<64>(K, g, lds, s, m, L, F);
+    case 32: return launch_fold_e<32>(K, g, lds, s, m, L, F);
+    case 16: return launch_fold_e<16>(K, g, lds, s, m, L, F);
+    default: return launch_fold_e<8>(K, g, lds, s, m, L, F);
     }
 }
 
@@ -476,7 +508,7 @@ extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs,
     hipLaunchKernelGGL(k_mesh_table, dim3((p->table_n + 1 + 3) / 4, nm), dim3(256), 0, s, p->d_members,
                        p->d_area_tab, p->d_total, p->table_n);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mesh_rings, dim3(S, nm), dim3(64), 0, s, p->d_members, p->d_area_tab, p->d_total,
+    hipLaunchKernelGGL(k_mesh_rings, dim3(S, nm), dim3(256), 0, s, p->d_members, p->d_area_tab, p->d_total,
                        p->table_n, L, p->d_theta, p->d_phi, p->d_r, p->d_cellArea, p->d_maxAlpha,
                        p->d_cos_gamma, p->d_effGrav);
     HIPCHK(hipGetLastError());

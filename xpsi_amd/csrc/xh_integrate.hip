@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(256) k_geometry(Member* __restrict__ members, 
 
 
 template <int EC, int MAXK>
-__global__ void __launch_bounds__(256) k_profile_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
+__global__ void __launch_bounds__(256, 4) k_profile_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
 {
     extern __shared__ double sm[];
     const int mi = blockIdx.z, tid = threadIdx.x;
@@ -231,11 +231,13 @@ __global__ void __launch_bounds__(256) k_profile_fold(const Member* __restrict__
     if (r0 >= S) return;
     const int r1 = min(S, r0 + F.ring_block);
     constexpr int PPAR = 256 / EC;
-    double* Y = sm;                       // [N_L][EC]
-    double* Bc = Y + N_L * EC;            // [N_L-1][EC]
-    double* Cc = Bc + (N_L - 1) * EC;
-    double* Dc = Cc + (N_L - 1) * EC;
-    double* PH = Dc + (N_L - 1) * EC;     // N_L
+    const int NI = N_L - 1;               // intervals
+    double* Y = sm;                       // [N_L][EC]  profile
+    double* Bc = Y + N_L * EC;            // [NI][EC]   Akima b
+    double* Cc = Bc + NI * EC;            // [NI][EC]   Akima c
+    double* Dc = Cc + NI * EC;            // [NI][EC]   Akima d
+    double* SL = Dc + NI * EC;            // [NI][EC]   secant slopes
+    double* PH = SL + NI * EC;            // N_L
     double* Zs = PH + N_L;
     double* As = Zs + N_L;
     double* Gs = As + N_L;
@@ -256,7 +258,6 @@ __global__ void __launch_bounds__(256) k_profile_fold(const Member* __restrict__
     for (int i = r0; i < r1; i++) {
         const size_t ro = (size_t)mi * L.S_max + i;
         __syncthreads();   // the previous parallel's fold is done with LDS
-        // cells of the parallel
         for (int j = tid; j < S; j += 256) {
             size_t c = cbase + (size_t)i * S + j;
             double a = F.cellArea[c];
@@ -288,20 +289,30 @@ __global__ void __launch_bounds__(256) k_profile_fold(const Member* __restrict__
                 Y[q] = v;
             }
             __syncthreads();
-            // periodic Akima coefficients per interval and energy
-            for (int q = tid; q < (N_L - 1) * EC; q += 256) {
+            // secant slopes, one division per interval and energy
+            for (int q = tid; q < NI * EC; q += 256) {
+                int mm = q / EC;
+                SL[q] = (Y[q + EC] - Y[q]) / (PH[mm + 1] - PH[mm]);
+            }
+            __syncthreads();
+            // periodic Akima coefficients (slopes wrap: m[-2] = m[NI-2], ...)
+            for (int q = tid; q < NI * EC; q += 256) {
                 int mm = q / EC, pp = q % EC;
-                auto sec = [&](int k) {
-                    int kk = k < 0 ? k + (N_L - 1) : (k > N_L - 2 ? k - (N_L - 1) : k);
-                    return (Y[(kk + 1) * EC + pp] - Y[kk * EC + pp]) / (PH[kk + 1] - PH[kk]);
+                auto sl = [&](int k) {
+                    int kk = k < 0 ? k + NI : (k >= NI ? k - NI : k);
+                    return SL[kk * EC + pp];
                 };
                 double b, c, d;
-                akima_piece(sec(mm - 2), sec(mm - 1), sec(mm), sec(mm + 1), sec(mm + 2),
-                            PH[mm + 1] - PH[mm], b, c, d);
-                Bc[q] = b; Cc[q] = c; Dc[q] = d;
+                akima_piece(sl(mm - 2), sl(mm - 1), sl(mm), sl(mm + 1), sl(mm + 2), PH[mm + 1] - PH[mm], b, c, d);
+                Bc[q] = b;
+                Cc[q] = c;
+                Dc[q] = d;
             }
             __syncthreads();
             const double lo = PH[0], hi = PH[N_L - 1];
+            int idx[MAXK];
+#pragma unroll
+            for (int q = 0; q < MAXK; q++) idx[q] = 0;
             for (int j = 0; j < S; j++) {
                 const double area = carea[j];
                 if (isnan(area)) continue;
@@ -314,9 +325,13 @@ __global__ void __launch_bounds__(256) k_profile_fold(const Member* __restrict__
                     int guard = 0;
                     if (x > hi) { while (x > hi && guard++ < 64) x -= TWO_PI; }
                     else if (x < lo) { while (x < lo && guard++ < 64) x += TWO_PI; }
-                    int idx = interval_of(PH, N_L, x);
-                    double t = x - PH[idx];
-                    int o = idx * EC + p;
+                    // interval search: x grows with the cell index between wraps
+                    int id = idx[q];
+                    if (x < PH[id]) id = 0;
+                    while (id < NI - 1 && PH[id + 1] <= x) id++;
+                    idx[q] = id;
+                    double t = x - PH[id];
+                    int o = id * EC + p;
                     double v = Y[o] + t * (Bc[o] + t * (Cc[o] + t * Dc[o]));
                     if (v > 0.0) acc[q] += area * v;
                 }
@@ -365,9 +380,10 @@ __global__ void __launch_bounds__(256) k_work_count(const Member* __restrict__ m
     if (threadIdx.x == 0) work[mi] = acc;
 }
 
-template __global__ void k_profile_fold<64, 16>(const Member*, Layout, FoldArgs);
-template __global__ void k_profile_fold<32, 16>(const Member*, Layout, FoldArgs);
-template __global__ void k_profile_fold<16, 16>(const Member*, Layout, FoldArgs);
-template __global__ void k_profile_fold<8, 16>(const Member*, Layout, FoldArgs);
+#define XH_FOLD_INST(EC, K) template __global__ void k_profile_fold<EC, K>(const Member*, Layout, FoldArgs);
+XH_FOLD_INST(64, 4) XH_FOLD_INST(64, 8) XH_FOLD_INST(64, 16)
+XH_FOLD_INST(32, 4) XH_FOLD_INST(32, 8) XH_FOLD_INST(32, 16)
+XH_FOLD_INST(16, 4) XH_FOLD_INST(16, 8) XH_FOLD_INST(16, 16)
+XH_FOLD_INST(8, 4) XH_FOLD_INST(8, 8) XH_FOLD_INST(8, 16)
 
 }  // namespace xh

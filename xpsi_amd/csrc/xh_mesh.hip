@@ -393,19 +393,20 @@ __global__ void __launch_bounds__(256) k_mesh_table(const Member* __restrict__ m
     }
 }
 
-// One wave per ring: parallels, centroid colatitude, ring properties and the
-// ring's cell areas (boundary cells integrated one after another).
-__global__ void __launch_bounds__(64) k_mesh_rings(const Member* __restrict__ members,
-                                                   const double* __restrict__ area_tab,
-                                                   const double* __restrict__ total, int table_n,
-                                                   Layout L, double* __restrict__ theta,
-                                                   double* __restrict__ phi, double* __restrict__ rr,
-                                                   double* __restrict__ cellArea,
-                                                   double* __restrict__ maxAlpha,
-                                                   double* __restrict__ cos_gamma,
-                                                   double* __restrict__ effGrav)
+// One block of four waves per ring: wave 0 finds the parallels and the
+// centroid colatitude, all threads classify the ring's cells, then the four
+// waves integrate the boundary cells between them.
+__global__ void __launch_bounds__(256) k_mesh_rings(const Member* __restrict__ members,
+                                                    const double* __restrict__ area_tab,
+                                                    const double* __restrict__ total, int table_n,
+                                                    Layout L, double* __restrict__ theta,
+                                                    double* __restrict__ phi, double* __restrict__ rr,
+                                                    double* __restrict__ cellArea,
+                                                    double* __restrict__ maxAlpha,
+                                                    double* __restrict__ cos_gamma,
+                                                    double* __restrict__ effGrav)
 {
-    const int mi = blockIdx.y, i = blockIdx.x, lane = threadIdx.x;
+    const int mi = blockIdx.y, i = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Member m = members[mi];
     if (!m.active || m.status) return;
     const int S = m.S;
@@ -415,36 +416,44 @@ __global__ void __launch_bounds__(64) k_mesh_rings(const Member* __restrict__ me
     member_lims(m, lo, hi, colat, sColat, inv, dphi, bphi);
     double cell = (m.polar ? TWO_PI : 2.0 * bphi) * total[mi];
     cell /= m.numCell;
-    double eta = cell / dphi;
+    const double eta = cell / dphi;
     const double* at = area_tab + (size_t)mi * table_n;
-    // colatitude table is linspace(hi, lo, table_n) (descending); areas ascend
     __shared__ double ct[1024];
-    for (int k = lane; k < table_n; k += 64) ct[k] = lin(hi, lo, table_n, k);
+    __shared__ double sh_lower, sh_upper, sh_cc;
+    __shared__ int nb;
+    __shared__ int bj[512];
+    for (int k = tid; k < table_n; k += 256) ct[k] = lin(hi, lo, table_n, k);
+    if (tid == 0) nb = 0;
     __syncthreads();
-    double i_eta = (double)S * eta, e_lo = 0.0, e_hi = 0.0;
-    for (int k = 0; k <= i && k < S - 1; k++) {
-        i_eta -= eta;
-        if (k == i - 1) e_lo = i_eta;
-        if (k == i) e_hi = i_eta;
-    }
-    double lower = (i == 0) ? lo : steffen_eval(at, ct, table_n, e_lo, nullptr);
-    double upper = (i == S - 1) ? hi : steffen_eval(at, ct, table_n, e_hi, nullptr);
-    double cc = wave_area(lower, upper, m.R_eq, m.epsilon, m.zeta, m.shape, 1) / eta;
-    double mu = cos(cc);
-    double rn = radius_norm(mu, m.epsilon, m.zeta, m.shape);
-    double fth = f_theta(mu, rn, m.epsilon, m.zeta, m.shape);
-    double cg = 1.0 / sqrt(1.0 + fth * fth);
     const size_t ro = (size_t)mi * L.S_max, co = (size_t)mi * L.S_max * L.S_max;
-    if (lane == 0) {
-        rr[ro + i] = rn * m.R_eq;
-        effGrav[ro + i] = effective_gravity(mu, m.R_eq, m.zeta, m.epsilon, m.shape);
-        cos_gamma[ro + i] = cg;
-        maxAlpha[ro + i] = HALF_PI + acos(cg);
-        theta[ro + i] = inv ? PI - cc : cc;
+    if (wave == 0) {
+        double i_eta = (double)S * eta, e_lo = 0.0, e_hi = 0.0;
+        for (int k = 0; k <= i && k < S - 1; k++) {
+            i_eta -= eta;
+            if (k == i - 1) e_lo = i_eta;
+            if (k == i) e_hi = i_eta;
+        }
+        double lower = (i == 0) ? lo : steffen_eval(at, ct, table_n, e_lo, nullptr);
+        double upper = (i == S - 1) ? hi : steffen_eval(at, ct, table_n, e_hi, nullptr);
+        double cc = wave_area(lower, upper, m.R_eq, m.epsilon, m.zeta, m.shape, 1) / eta;
+        if (lane == 0) {
+            sh_lower = lower; sh_upper = upper; sh_cc = cc;
+            double mu = cos(cc);
+            double rn = radius_norm(mu, m.epsilon, m.zeta, m.shape);
+            double fth = f_theta(mu, rn, m.epsilon, m.zeta, m.shape);
+            double cg = 1.0 / sqrt(1.0 + fth * fth);
+            rr[ro + i] = rn * m.R_eq;
+            effGrav[ro + i] = effective_gravity(mu, m.R_eq, m.zeta, m.epsilon, m.shape);
+            cos_gamma[ro + i] = cg;
+            maxAlpha[ro + i] = HALF_PI + acos(cg);
+            theta[ro + i] = inv ? PI - cc : cc;
+        }
     }
-    double first_left = m.polar ? -PI : -bphi;
+    __syncthreads();
+    const double lower = sh_lower, upper = sh_upper, cc = sh_cc;
+    const double first_left = m.polar ? -PI : -bphi;
     if (i == 0) {
-        for (int j = lane; j < S; j += 64) {
+        for (int j = tid; j < S; j += 256) {
             double p0 = m.polar ? lin(-PI + 0.5 * dphi, PI - 0.5 * dphi, S, j)
                                 : lin(-bphi + 0.5 * dphi, bphi - 0.5 * dphi, S, j);
             double v = p0 + m.phi_offset;
@@ -474,55 +483,57 @@ __global__ void __launch_bounds__(64) k_mesh_rings(const Member* __restrict__ me
     const int j_max = mirror ? S / 2 : S;
     double* ca = cellArea + co + (size_t)i * S;
     // the reference accumulates left += delta_phi cell by cell
-    for (int base = 0; base < j_max; base += 64) {
-        int j = base + lane;
+    auto left_of = [&](int j) {
         double left = first_left;
-        for (int q = 0; q < j && q < j_max; q++) left = left + dphi;
-        int kind = 0;     // 0 empty, 1 interior, 2 boundary
+        for (int q = 0; q < j; q++) left = left + dphi;
+        return left;
+    };
+    for (int j = tid; j < j_max; j += 256) {
+        double left = left_of(j);
         double right = left + dphi, mid = left + 0.5 * dphi;
-        if (j < j_max) {
-            int p1, p2, p3, p4, p5;
-            bool top_row = m.polar && i == 0;
-            if (top_row) {
-                p1 = p3 = (colat <= cR);
-                if (p1 == 1) { p1 = (sR <= sColat); p3 = p1; } else p1 = p3 = 2;
-            } else {
-                p1 = (psi_of(lower, left, colat) <= cR);
-                p3 = (psi_of(lower, right, colat) <= cR);
-                p1 = p1 == 1 ? (sR <= psi_of(lower, left - sAzi, sColat)) : 2;
-                p3 = p3 == 1 ? (sR <= psi_of(lower, right - sAzi, sColat)) : 2;
-            }
-            p2 = (psi_of(upper, left, colat) <= cR);
-            p4 = (psi_of(upper, right, colat) <= cR);
-            p5 = (psi_of(cc, mid, colat) <= cR);
-            p2 = p2 == 1 ? (sR <= psi_of(upper, left - sAzi, sColat)) : 2;
-            p4 = p4 == 1 ? (sR <= psi_of(upper, right - sAzi, sColat)) : 2;
-            p5 = p5 == 1 ? (sR <= psi_of(cc, mid - sAzi, sColat)) : 2;
-            if (!(p1 == p2 && p2 == p3 && p3 == p4 && p4 == p5)) kind = 2;
-            else if (sR > 0.0) {
-                auto inside = [&](double a, double b) {
-                    return (lower <= a && a <= upper) && (left <= b && b <= right);
-                };
-                if (inside(s1a, s1b) || inside(s2a, s2b) || inside(s3a, s3b) || inside(s4a, s4b)) kind = 2;
-                else if (p5 == 1) kind = 1;
-            } else if (p5 == 1) kind = 1;
-            double v = kind == 1 ? cell : 0.0;
+        int p1, p2, p3, p4, p5, kind = 0;
+        if (m.polar && i == 0) {
+            p1 = p3 = (colat <= cR);
+            if (p1 == 1) { p1 = (sR <= sColat); p3 = p1; } else p1 = p3 = 2;
+        } else {
+            p1 = (psi_of(lower, left, colat) <= cR);
+            p3 = (psi_of(lower, right, colat) <= cR);
+            p1 = p1 == 1 ? (sR <= psi_of(lower, left - sAzi, sColat)) : 2;
+            p3 = p3 == 1 ? (sR <= psi_of(lower, right - sAzi, sColat)) : 2;
+        }
+        p2 = (psi_of(upper, left, colat) <= cR);
+        p4 = (psi_of(upper, right, colat) <= cR);
+        p5 = (psi_of(cc, mid, colat) <= cR);
+        p2 = p2 == 1 ? (sR <= psi_of(upper, left - sAzi, sColat)) : 2;
+        p4 = p4 == 1 ? (sR <= psi_of(upper, right - sAzi, sColat)) : 2;
+        p5 = p5 == 1 ? (sR <= psi_of(cc, mid - sAzi, sColat)) : 2;
+        if (!(p1 == p2 && p2 == p3 && p3 == p4 && p4 == p5)) kind = 2;
+        else if (sR > 0.0) {
+            auto inside = [&](double a, double b) {
+                return (lower <= a && a <= upper) && (left <= b && b <= right);
+            };
+            if (inside(s1a, s1b) || inside(s2a, s2b) || inside(s3a, s3b) || inside(s4a, s4b)) kind = 2;
+            else if (p5 == 1) kind = 1;
+        } else if (p5 == 1) kind = 1;
+        double v = kind == 1 ? cell : 0.0;
+        ca[j] = v;
+        if (mirror) ca[S - 1 - j] = v;
+        if (kind == 2) {
+            int slot = atomicAdd(&nb, 1);
+            if (slot < 512) bj[slot] = j;
+        }
+    }
+    __syncthreads();
+    const int nbc = min(nb, 512);
+    for (int q = wave; q < nbc; q += 4) {
+        int j = bj[q];
+        double left = left_of(j);
+        double right = left + dphi;
+        double v = wave_cell(lower, upper, left, right, m.R_eq, m.epsilon, m.zeta, m.shape, colat, cR, sR, sAzi,
+                             sColat);
+        if (lane == 0) {
             ca[j] = v;
             if (mirror) ca[S - 1 - j] = v;
-        }
-        // boundary cells: the whole wave integrates each in turn
-        unsigned long long todo = __ballot(kind == 2);
-        while (todo) {
-            int src = __ffsll((long long)todo) - 1;
-            todo &= todo - 1;
-            double l2 = wave_bcast(left, src);
-            double r2 = wave_bcast(right, src);
-            double v = wave_cell(lower, upper, l2, r2, m.R_eq, m.epsilon, m.zeta, m.shape, colat, cR,
-                                 sR, sAzi, sColat);
-            if (lane == src) {
-                ca[j] = v;
-                if (mirror) ca[S - 1 - j] = v;
-            }
         }
     }
 }
