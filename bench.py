@@ -35,6 +35,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--fold-ec", type=int, default=0, help="energies per fold block (0 = auto)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the likelihood path shards requests across GPUs "
@@ -46,6 +47,9 @@ def main():
     from xpsi_amd.pipeline import LikelihoodPipeline, params_from_model
 
     torch.cuda.init()
+    if args.fold_ec:
+        from xpsi_amd import lib
+        lib().xh_set_fold_energy_chunk(args.fold_ec)
     dev = torch.cuda.get_device_name(0)
     stream = torch.cuda.current_stream().cuda_stream
 

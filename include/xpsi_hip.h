@@ -31,8 +31,12 @@ const char* xh_last_error(void);
  * --------------------------------------------------------------------- */
 
 /* Replaces xpsi/cellmesh/integrator_for_azimuthal_invariance.pyx:70-98
- * `integrate(...) -> (status, flux[N_E, N_P])` for the blackbody atmosphere
- * (hot_atm_ext 1) without an elsewhere correction or disk.
+ * `integrate(...) -> (status, flux[N_E, N_P])` for the blackbody (hot_atm_ext
+ * 1) and four-dimensional table (hot_atm_ext 2, hot_Num4D.pyx) atmospheres,
+ * without an elsewhere correction or disk.  The table is passed as device
+ * arrays: atm_axes = logT, logg, mu, log10(E/kT) concatenated, atm_N [4]
+ * (int32, device) their lengths, atm_I the values in C order (T, g, mu, E);
+ * all three may be NULL for the blackbody.
  * cellArea, theta, phi: [S, S]; radiates: int32 [S, S]; cellParams: [S, S, 2];
  * radial, r_s_over_r, maxDeflection, cos_gamma: [S];
  * deflection, cos_alpha, lag: [S, N_R]; energies [N_E] (keV);
@@ -49,7 +53,8 @@ int xh_integrate(double omega, double r_s, double inclination, int S,
                  const double* maxDeflection, const double* cos_gamma,
                  int N_E, const double* energies, int N_L,
                  const double* leaves, int N_P, const double* phases,
-                 int hot_atm_ext, int image_order_limit, double* flux,
+                 int hot_atm_ext, const double* atm_axes, const int* atm_N,
+                 const double* atm_I, int image_order_limit, double* flux,
                  void* workspace, size_t workspace_bytes, int* status_out,
                  void* stream);
 
@@ -107,6 +112,9 @@ typedef struct xh_model_config {
     int n_regions;            /* 1 or 2 */
     xh_region_config region[2];
     int num_rays, num_leaves, num_phases, image_order_limit, atm_ext;
+    const double* atm_axes[4];/* atm_ext 2: table axes (host) logT, logg, mu, log10(E/kT) */
+    int atm_N[4];
+    const double* atm_I;      /* table values (host), C order (T, g, mu, E) */
     int star_shape;           /* 0 oblate (AGM_14), 1 sphere */
     double frequency, mode_frequency;
     int N_E;                  /* energies (host, keV) */

@@ -47,5 +47,24 @@ def make_config1():
     print("config1: total counts %.4g" % counts.sum())
 
 
+def make_config2(atm):
+    """config2_<atm>.npz: counts[300, 32] from both hot regions, default_rng(42)."""
+    model = configs.config2(atm, counts=np.zeros((300, 32)))
+    lnl, parts = O.loglikelihood(model, return_parts=True)
+    rate = np.zeros((300, 32))
+    for sig, h in zip(parts["signals"], model["hot_regions"]):
+        rate += binned_rate(sig, np.linspace(0.0, 1.0, 128), model["data"]["phases"], h["phase_shift"])
+    expected = model["data"]["exposure_time"] * rate + 5.0
+    rng = np.random.default_rng(42)
+    counts = rng.poisson(expected).astype(np.float64)
+    np.savez_compressed(os.path.join(HERE, "config2_%s.npz" % atm), counts=counts, expected=expected)
+    print("config2 %s: total counts %.4g" % (atm, counts.sum()))
+
+
 if __name__ == "__main__":
-    make_config1()
+    which = sys.argv[1:] or ["config1", "config2"]
+    if "config1" in which:
+        make_config1()
+    if "config2" in which:
+        make_config2("BB")
+        make_config2("Num4D")

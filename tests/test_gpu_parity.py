@@ -156,3 +156,61 @@ def test_pipeline_examples_fast(ef_model):
 def test_pipeline_config1():
     from xpsi_amd import configs
     _pipeline_case(configs.config1(), "config1")
+
+
+def _region_parity(model, name, lnl_tol=LNL_ATOL):
+    """Full pipeline with every region and member checked against the oracle."""
+    from xpsi_amd.pipeline import LikelihoodPipeline, params_from_model
+    lnl_ref, parts = O.loglikelihood(model, return_parts=True)
+    say("pipeline %s: evaluating (%d regions)" % (name, len(model["hot_regions"])))
+    pl = LikelihoodPipeline(model, max_batch=1)
+    lnl = pl.evaluate([params_from_model(model)])[0]
+    say("pipeline %s: lnL gpu %.10f oracle %.10f" % (name, lnl, lnl_ref))
+    for r in range(len(model["hot_regions"])):
+        fo = pl.fetch("folded", 0, r, pl.N_ch * pl.N_P)[:pl.N_ch * pl.N_P].reshape(pl.N_ch, pl.N_P)
+        err = relerr(fo, parts["signals"][r])
+        say("pipeline %s: region %d folded rel err %.3e" % (name, r, err))
+        assert err < SIG_RTOL
+    assert abs(lnl - lnl_ref) < lnl_tol
+    pl.close()
+
+
+def test_pipeline_config2_bb_two_regions_cede():
+    from xpsi_amd import configs
+    _region_parity(configs.config2("BB"), "config2-BB")
+
+
+def test_pipeline_config2_num4d():
+    from xpsi_amd import configs
+    _region_parity(configs.config2("Num4D"), "config2-Num4D")
+
+
+def test_integrate_dropin_num4d():
+    from xpsi_amd import configs
+    from xpsi_amd.cellmesh.integrator_for_azimuthal_invariance import integrate
+    model = configs.config2("Num4D")
+    sp = model["spacetime"]
+    st = O.spacetime(sp["mass"], sp["radius"], sp["distance"], sp["cos_inclination"],
+                     sp["frequency"])
+    hr = model["hot_regions"][0]
+    members, err = O.hot_region_embed(st, hr, sp["frequency"])
+    atm = model["atmosphere"]
+    energies = O.construct_energy_array(256, model["instrument"]["energy_edges"])
+    leaves = np.linspace(0.0, 2.0 * math.pi, 128)
+    phases = 2.0 * math.pi * np.linspace(0.0, 1.0, 128)
+    for k, m in enumerate(members):
+        t, ref = O.integrate(st["Omega"], st["r_s"], st["i"], m["cellArea"], m["r"],
+                             m["r_s_over_r"], m["theta"], m["phi"], m["params"], m["radiates"],
+                             m["deflection"], m["cos_alpha"], m["lag"], m["maxDeflection"],
+                             m["cos_gamma"], energies, leaves, phases, atm_ext=2, atmosphere=atm)
+        assert t == 0
+        say("integrate drop-in Num4D member %d: S=%d" % (k, m["S"]))
+        status, flux = integrate(1, st["R"], st["Omega"], st["r_s"], st["i"], m["cellArea"], m["r"],
+                                 m["r_s_over_r"], m["theta"], m["phi"], m["params"],
+                                 m["radiates"], None, m["deflection"].shape[1], m["deflection"],
+                                 m["cos_alpha"], m["lag"], m["maxDeflection"], m["cos_gamma"],
+                                 energies, leaves, phases, atm, (), 2, None, 0)
+        assert status == 0
+        err = relerr(flux, ref)
+        say("integrate drop-in Num4D member %d rel err %.3e" % (k, err))
+        assert err < SIG_RTOL

@@ -22,7 +22,8 @@ class RegionConfig(C.Structure):
 class ModelConfig(C.Structure):
     _fields_ = [("n_regions", i32), ("region", RegionConfig * 2),
                 ("num_rays", i32), ("num_leaves", i32), ("num_phases", i32),
-                ("image_order_limit", i32), ("atm_ext", i32), ("star_shape", i32),
+                ("image_order_limit", i32), ("atm_ext", i32),
+                ("atm_axes", vp * 4), ("atm_N", i32 * 4), ("atm_I", vp), ("star_shape", i32),
                 ("frequency", d), ("mode_frequency", d),
                 ("N_E", i32), ("energies", vp),
                 ("N_in", i32), ("energy_edges", vp), ("attenuation", vp),
@@ -56,7 +57,7 @@ def lib():
     L.xh_integrate_workspace_bytes.argtypes = [i32] * 5
     L.xh_integrate.restype = i32
     L.xh_integrate.argtypes = ([d, d, d, i32] + [vp] * 7 + [i32] + [vp] * 5 +
-                               [i32, vp, i32, vp, i32, vp, i32, i32, vp, vp, sz,
+                               [i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, i32, vp, vp, sz,
                                 C.POINTER(i32), vp])
     L.xh_energy_integrator.restype = i32
     L.xh_energy_integrator.argtypes = [i32, i32, vp, vp, i32, vp, vp, vp]

@@ -3,8 +3,10 @@
 `integrate(...)` keeps the reference signature (:70-98) and return value
 (:657-665): ``(0, flux[N_E, N_P])`` or ``(1, None)``.  It is bound where the
 reference binds its integrator (HotRegion._integrator, HotRegion.py:551-572).
-Supported: the blackbody atmosphere (hot_atm_ext 1) without an elsewhere
-correction or disk (R_in >= 1e6); other settings raise NotImplementedError.
+Supported: the blackbody (hot_atm_ext 1) and four-dimensional table
+(hot_atm_ext 2, `hot_atmosphere` = (logT, logg, mu, logE, buf)) atmospheres
+without beaming, an elsewhere correction or a disk (R_in >= 1e6); other
+settings raise NotImplementedError.
 """
 import ctypes as C
 
@@ -22,8 +24,8 @@ def integrate(numThreads, R, omega, r_s, inclination, cellArea,
               else_atm_ext, beam_opt, image_order_limit=None, R_in=1e6):
     if correction_srcCellParams is not None:
         raise NotImplementedError("elsewhere correction is not supported")
-    if int(hot_atm_ext) != 1 or int(beam_opt) != 0 or R_in < 1e6:
-        raise NotImplementedError("only the blackbody atmosphere without beaming or disk")
+    if int(hot_atm_ext) not in (1, 2) or int(beam_opt) != 0 or R_in < 1e6:
+        raise NotImplementedError("blackbody or 4-D table atmospheres without beaming or disk")
     S = int(np.shape(cellArea)[0])
     N_R = int(np.shape(deflection)[1])
     N_E, N_L, N_P = len(energies), len(leaves), len(phases)
@@ -33,6 +35,11 @@ def integrate(numThreads, R, omega, r_s, inclination, cellArea,
     rays = [D.to_dev(a) for a in (deflection, cos_alpha, lag, maxDeflection,
                                    cos_gammaArray, energies, leaves, phases)]
     flux = D.empty((N_E, N_P))
+    axes = nax = tab = None
+    if int(hot_atm_ext) == 2:
+        axes = D.to_dev(np.concatenate([np.asarray(a, dtype=np.float64) for a in hot_atmosphere[:4]]))
+        nax = D.to_dev(np.array([len(a) for a in hot_atmosphere[:4]], dtype=np.int32), np.int32)
+        tab = D.to_dev(hot_atmosphere[4])
     L = lib()
     wsb = L.xh_integrate_workspace_bytes(S, N_R, N_L, N_E, N_P)
     ws = D.empty((wsb + 7) // 8)
@@ -42,7 +49,8 @@ def integrate(numThreads, R, omega, r_s, inclination, cellArea,
                         D.ptr(dev[4]), D.ptr(dev[5]), D.ptr(rad), N_R,
                         D.ptr(rays[0]), D.ptr(rays[1]), D.ptr(rays[2]),
                         D.ptr(rays[3]), D.ptr(rays[4]), N_E, D.ptr(rays[5]),
-                        N_L, D.ptr(rays[6]), N_P, D.ptr(rays[7]), 1,
+                        N_L, D.ptr(rays[6]), N_P, D.ptr(rays[7]), int(hot_atm_ext),
+                        D.ptr(axes), D.ptr(nax), D.ptr(tab),
                         int(image_order_limit or 0), D.ptr(flux), D.ptr(ws),
                         ws.numel() * 8, C.byref(status), D.stream_ptr())
     check(rc, "xh_integrate")

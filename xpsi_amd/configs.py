@@ -111,3 +111,69 @@ def config3_model(num_energies, counts=None):
     h["max_sqrt_num_cells"] = 128
     h["image_order_limit"] = None
     return m
+
+
+def num4d_table(seed=2):
+    """Synthetic four-dimensional atmosphere table (hot_Num4D.pyx layout):
+    axes log10 T (35, 5.1-6.8), log10 g (14, 13.7-15.0), mu (67, k/67 for
+    k = 1..67) and log10(E / kT) (166, -1.3-2.0); values I_nu / T^3 (cgs per
+    Hz) of a blackbody times (1 + 0.5 mu) times (1 + 0.05 n), with n a sum of
+    four low-frequency sinusoids over the normalised grid coordinates
+    (numpy default_rng(seed) draws their integer frequencies in {1, 2} and
+    phases), scaled so that max |n| <= 1.  Flattened C order (T, g, mu, E)."""
+    logT = np.linspace(5.1, 6.8, 35)
+    logg = np.linspace(13.7, 15.0, 14)
+    mu = np.arange(1, 68) / 67.0
+    logE = np.linspace(-1.3, 2.0, 166)
+    rng = np.random.default_rng(seed)
+    freqs = rng.integers(1, 3, size=(4, 4))
+    phs = rng.uniform(0.0, 2.0 * math.pi, size=4)
+    uT = np.linspace(0, 1, 35)[:, None, None, None]
+    ug = np.linspace(0, 1, 14)[None, :, None, None]
+    um = np.linspace(0, 1, 67)[None, None, :, None]
+    ue = np.linspace(0, 1, 166)[None, None, None, :]
+    n = np.zeros((35, 14, 67, 166))
+    for q in range(4):
+        f = freqs[q]
+        n += np.sin(2.0 * math.pi * (f[0] * uT + f[1] * ug + f[2] * um + f[3] * ue) + phs[q])
+    n /= 4.0
+    k_cgs, h_cgs, c_cgs = 1.380649e-16, 6.62607015e-27, 2.99792458e10
+    x = 10.0 ** logE
+    planck = 2.0 * k_cgs ** 3 / (c_cgs ** 2 * h_cgs ** 2) * x ** 3 / np.expm1(x)
+    buf = planck[None, None, None, :] * (1.0 + 0.5 * mu)[None, None, :, None] * (1.0 + 0.05 * n)
+    return (logT, logg, mu, logE, np.ascontiguousarray(buf.reshape(-1)))
+
+
+def config2(atmosphere="Num4D", counts=None):
+    """Primary hot region: superseding spot with a concentric ceding ring;
+    secondary: the same region antipodal (colatitude pi - theta, half a cycle
+    later).  theta 0.9, super radius 0.15, cede radius 0.35, log T 6.3 (super)
+    and 6.0 (cede); sqrt_num_cells 64 (max 128), 128 leaves, 256 energies over
+    0.3-10 keV, 300 channels x 256 inputs, 32 phase bins, 1e6 s."""
+    M, in_edges, _ = gaussian_response(300, 256)
+    atm_ext = 2 if atmosphere == "Num4D" else 1
+    name = "config2_%s.npz" % atmosphere
+    if counts is None:
+        path = os.path.join(GOLDEN, name)
+        counts = np.load(path)["counts"] if os.path.exists(path) else np.zeros((300, 32))
+    regions = []
+    for k in range(2):
+        regions.append(dict(super_colatitude=0.9 if k == 0 else math.pi - 0.9,
+                            super_radius=0.15, cede_radius=0.35, cede_colatitude=None,
+                            cede_azimuth=0.0, super_temperature=6.3, cede_temperature=6.0,
+                            phase_shift=0.0, cede=True, sqrt_num_cells=64,
+                            min_sqrt_num_cells=10, max_sqrt_num_cells=128, num_rays=200,
+                            num_leaves=128, is_antiphased=(k == 1), image_order_limit=None,
+                            atm_ext=atm_ext))
+    for r in regions:
+        r["cede_colatitude"] = r["super_colatitude"]
+    return dict(
+        spacetime=dict(mass=1.4, radius=12.0, distance=1.0, cos_inclination=0.5,
+                       frequency=300.0, star_shape="AGM_14"),
+        hot_regions=regions,
+        instrument=dict(matrix=M, energy_edges=in_edges),
+        data=dict(counts=counts, phases=np.linspace(0.0, 1.0, 33), exposure_time=1.0e6),
+        num_energies=256, support=None,
+        atmosphere=num4d_table() if atm_ext == 2 else None,
+        marginal=dict(epsabs=0.0, epsrel=1.0e-8, epsilon=1.0e-3, sigmas=10.0),
+        llzero=-1.0e90)

@@ -44,10 +44,10 @@ static int pick_ec(int N_L)
         while (ec > 8 && (size_t)5 * N_L * ec * 8 > 150 * 1024) ec >>= 1;
         return ec;
     }
-    // profile, slopes and three coefficient arrays for EC energies: <= 40 KiB,
-    // leaving room for three to four blocks per compute unit
-    int ec = 64;
-    while (ec > 8 && (size_t)5 * N_L * ec * 8 > 40960) ec >>= 1;
+    // 8 energies per block measured fastest at 64 leaves (more phase slots per
+    // wave, 20 KiB of LDS); keep the staged arrays within 40 KiB otherwise
+    int ec = 8;
+    while (ec < 64 && (size_t)5 * N_L * (ec * 2) * 8 <= 40960 && N_L < 16) ec <<= 1;
     return ec;
 }
 
@@ -61,21 +61,22 @@ extern "C" int xh_set_fold_energy_chunk(int ec)
 
 static size_t fold_lds(int N_L, int EC, int S_max)
 {
-    return ((size_t)N_L * EC + 4 * (size_t)(N_L - 1) * EC + 4 * (size_t)N_L + 2 * (size_t)S_max) * 8;
+    return ((size_t)N_L * EC + 4 * (size_t)(N_L - 1) * EC + 5 * (size_t)N_L + 2 * (size_t)S_max) * 8;
 }
 
-static size_t geo_lds(int N_R, int N_L) { return (5 * (size_t)N_R + 4 * (size_t)N_L) * 8 + (N_L / 2 + 2) * 4; }
+static size_t geo_lds(int N_R, int N_L) { return (5 * (size_t)N_R + 5 * (size_t)N_L) * 8 + (N_L / 2 + 2) * 4; }
 
-template <int EC, int K>
+template <int EC, int K, int ATM>
 static hipError_t launch_fold_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void*)&k_profile_fold<EC, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
+        hipError_t e = hipFuncSetAttribute((const void*)&k_profile_fold<EC, K, ATM>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+        if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL((k_profile_fold<EC, K>), g, dim3(256), lds, s, m, L, F);
+    hipLaunchKernelGGL((k_profile_fold<EC, K, ATM>), g, dim3(256), lds, s, m, L, F);
     return hipGetLastError();
 }
 
@@ -86,23 +87,30 @@ static int fold_k(int EC, int N_P)
     return per <= 4 ? 4 : (per <= 8 ? 8 : 16);
 }
 
-template <int EC>
+template <int EC, int ATM>
 static hipError_t launch_fold_e(int K, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
-    if (K == 4) return launch_fold_t<EC, 4>(g, lds, s, m, L, F);
-    if (K == 8) return launch_fold_t<EC, 8>(g, lds, s, m, L, F);
-    return launch_fold_t<EC, 16>(g, lds, s, m, L, F);
+    if (K == 4) return launch_fold_t<EC, 4, ATM>(g, lds, s, m, L, F);
+    if (K == 8) return launch_fold_t<EC, 8, ATM>(g, lds, s, m, L, F);
+    return launch_fold_t<EC, 16, ATM>(g, lds, s, m, L, F);
+}
+
+template <int ATM>
+static hipError_t launch_fold_a(int EC, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+{
+    int K = fold_k(EC, L.N_P);
+    switch (EC) {
+    case 64: return launch_fold_e<64, ATM>(K, g, lds, s, m, L, F);
+    case 32: return launch_fold_e<32, ATM>(K, g, lds, s, m, L, F);
+    case 16: return launch_fold_e<16, ATM>(K, g, lds, s, m, L, F);
+    default: return launch_fold_e<8, ATM>(K, g, lds, s, m, L, F);
+    }
 }
 
 static hipError_t launch_fold(int EC, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
-    int K = fold_k(EC, L.N_P);
-    switch (EC) {
-    case 64: return launch_fold_e<64>(K, g, lds, s, m, L, F);
-    case 32: return launch_fold_e<32>(K, g, lds, s, m, L, F);
-    case 16: return launch_fold_e<16>(K, g, lds, s, m, L, F);
-    default: return launch_fold_e<8>(K, g, lds, s, m, L, F);
-    }
+    if (lds > 150 * 1024) return hipErrorInvalidValue;
+    return L.atm_ext == 2 ? launch_fold_a<2>(EC, g, lds, s, m, L, F) : launch_fold_a<1>(EC, g, lds, s, m, L, F);
 }
 
 static hipError_t fill(double* p, size_t n, double v, hipStream_t s)
@@ -125,9 +133,9 @@ extern "C" size_t xh_integrate_workspace_bytes(int S, int N_R, int N_L, int N_E,
     (void)N_R; (void)N_E; (void)N_P;
     const int IO = 8;
     size_t b = 256;                                  // Member
-    b += 3 * (size_t)S * 8 + (size_t)S * 4;          // theta, logT, u(unused), ring_ok
-    b += (size_t)S * IO * 4 * N_L * 8;               // geo
-    b += (size_t)S * IO * 4;                         // valid
+    b += 3 * (size_t)S * 8 + (size_t)S * 4;          // theta, logT, logg, ring_ok
+    b += (size_t)S * IO * 5 * N_L * 8;               // geo
+    b += 2 * (size_t)S * IO * 4;                     // valid, lastq
     return b + 1024;
 }
 
@@ -137,11 +145,15 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
                             const double* deflection, const double* cos_alpha, const double* lag,
                             const double* maxDeflection, const double* cos_gamma, int N_E,
                             const double* energies, int N_L, const double* leaves, int N_P,
-                            const double* phases, int hot_atm_ext, int image_order_limit, double* flux,
-                            void* workspace, size_t workspace_bytes, int* status_out, void* stream)
+                            const double* phases, int hot_atm_ext, const double* atm_axes, const int* atm_N,
+                            const double* atm_I, int image_order_limit, double* flux, void* workspace,
+                            size_t workspace_bytes, int* status_out, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    if (hot_atm_ext != 1) return fail(XH_EINVAL, "xh_integrate: only the blackbody atmosphere (1) is supported");
+    if (hot_atm_ext != 1 && hot_atm_ext != 2)
+        return fail(XH_EINVAL, "xh_integrate: atmosphere extension must be 1 (blackbody) or 2 (4-D table)");
+    if (hot_atm_ext == 2 && (!atm_axes || !atm_N || !atm_I))
+        return fail(XH_EINVAL, "xh_integrate: the 4-D table atmosphere needs its axes and values");
     if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1) return fail(XH_EINVAL, "xh_integrate: bad sizes");
     const int EC = pick_ec(N_L);
     if (N_P > (256 / EC) * 16) return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
@@ -152,22 +164,33 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     Member* d_m = (Member*)w; w += 256;
     double* d_theta = (double*)w; w += (size_t)S * 8;
     double* d_logT = (double*)w; w += (size_t)S * 8;
-    w += (size_t)S * 8;
+    double* d_logg = (double*)w; w += (size_t)S * 8;
     int* d_ok = (int*)w; w += (size_t)S * 4;
     w = (char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
-    double* d_geo = (double*)w; w += (size_t)S * IO * 4 * N_L * 8;
-    int* d_valid = (int*)w;
+    double* d_geo = (double*)w; w += (size_t)S * IO * 5 * N_L * 8;
+    int* d_valid = (int*)w; w += (size_t)S * IO * 4;
+    int* d_lastq = (int*)w;
     Member hm{};
     hm.Omega = omega; hm.r_s = r_s; hm.inclination = inclination; hm.active = 1; hm.S = S;
     HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
-    Layout L{S, N_R, N_L, N_P, N_E, IO, image_order_limit > 0 ? image_order_limit : 0, 1};
-    hipLaunchKernelGGL(k_ring_prep, dim3(1), dim3(256), 0, s, S, theta, cellParams, radiates, d_theta, d_ok, d_logT);
+    Layout L{S, N_R, N_L, N_P, N_E, IO, image_order_limit > 0 ? image_order_limit : 0, hot_atm_ext};
+    hipLaunchKernelGGL(k_ring_prep, dim3(1), dim3(256), 0, s, S, theta, cellParams, radiates, d_theta, d_ok, d_logT,
+                       d_logg);
     HIPCHK(hipGetLastError());
     RingArgs RA{d_theta, radial, r_s_over_r, cos_gamma, maxDeflection, deflection, cos_alpha, lag, leaves, d_ok,
-                d_geo, d_valid};
+                d_geo, d_valid, d_lastq};
     hipLaunchKernelGGL(k_geometry, dim3(S, IO, 1), dim3(256), geo_lds(N_R, N_L), s, d_m, L, RA);
     HIPCHK(hipGetLastError());
-    FoldArgs FA{d_geo, d_valid, cellArea, radiates, phi, S, d_logT, energies, phases, flux, S};
+    FoldArgs FA{d_geo, d_valid, cellArea, radiates, phi, S, d_logT, d_logg, d_lastq, atm_axes, atm_I,
+                {0, 0, 0, 0}, energies, phases, flux, S};
+    if (hot_atm_ext == 2) {
+        int hN[4];
+        HIPCHK(hipMemcpy(hN, atm_N, sizeof(hN), hipMemcpyDeviceToHost));
+        for (int d = 0; d < 4; d++) {
+            if (hN[d] < 4) return fail(XH_EINVAL, "xh_integrate: table axes need at least 4 nodes");
+            FA.atm_N[d] = hN[d];
+        }
+    }
     HIPCHK(launch_fold(EC, dim3((N_E + EC - 1) / EC, 1, 1), fold_lds(N_L, EC, S), s, d_m, L, FA));
     Member back{};
     HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
@@ -273,7 +296,9 @@ struct xh_pipeline {
     RegionAlloc* d_regions;
     double *d_area_tab, *d_total, *d_theta, *d_phi, *d_r, *d_cellArea, *d_maxAlpha, *d_cos_gamma, *d_effGrav;
     double *d_u, *d_logT, *d_maxDefl, *d_defl, *d_cosa, *d_lag, *d_geo, *d_flux;
-    int *d_ring_ok, *d_valid, *d_member_index, *d_status;
+    int *d_ring_ok, *d_valid, *d_member_index, *d_status, *d_lastq;
+    double *d_atm_axes, *d_atm_I;
+    int atm_N[4];
     double *d_energies, *d_logE, *d_logEdges, *d_atten, *d_leaves, *d_phases, *d_comp_phases;
     double *d_integrated, *d_M, *d_folded, *d_data_phases, *d_counts, *d_precomp, *d_support, *d_shifts;
     double *d_ll, *d_bml, *d_bmls, *d_star, *d_lnl;
@@ -336,7 +361,12 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
                   c->atm_ext};
     p->EC = pick_ec(c->num_leaves);
     if (N_P > (256 / p->EC) * 16) { fail(XH_EINVAL, "too many phases for the leaf count"); delete p; return nullptr; }
-    if (c->atm_ext != 1) { fail(XH_EINVAL, "pipeline: only the blackbody atmosphere is supported"); delete p; return nullptr; }
+    if (c->atm_ext != 1 && c->atm_ext != 2) { fail(XH_EINVAL, "pipeline: atmosphere must be 1 (blackbody) or 2 (4-D table)"); delete p; return nullptr; }
+    if (c->atm_ext == 2) {
+        for (int d = 0; d < 4; d++)
+            if (!c->atm_axes[d] || c->atm_N[d] < 4) { fail(XH_EINVAL, "pipeline: 4-D table axes missing or too short"); delete p; return nullptr; }
+        if (!c->atm_I) { fail(XH_EINVAL, "pipeline: 4-D table values missing"); delete p; return nullptr; }
+    }
     const size_t nm = (size_t)p->maxb * p->mpr, S = smax, NR = c->num_rays, NE = c->N_E;
     const size_t ng = (size_t)p->maxb * p->nreg;
     PCHK(dalloc(p, &p->d_members, nm));
@@ -357,8 +387,21 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     PCHK(dalloc(p, &p->d_defl, nm * S * NR));
     PCHK(dalloc(p, &p->d_cosa, nm * S * NR));
     PCHK(dalloc(p, &p->d_lag, nm * S * NR));
-    PCHK(dalloc(p, &p->d_geo, nm * S * IO * 4 * c->num_leaves));
+    PCHK(dalloc(p, &p->d_geo, nm * S * IO * 5 * c->num_leaves));
     PCHK(dalloc(p, &p->d_valid, nm * S * IO));
+    PCHK(dalloc(p, &p->d_lastq, nm * S * IO));
+    if (c->atm_ext == 2) {
+        size_t na = 0, ni = 1;
+        for (int d = 0; d < 4; d++) { p->atm_N[d] = c->atm_N[d]; na += c->atm_N[d]; ni *= c->atm_N[d]; }
+        PCHK(dalloc(p, &p->d_atm_axes, na));
+        PCHK(dalloc(p, &p->d_atm_I, ni));
+        size_t off = 0;
+        for (int d = 0; d < 4; d++) {
+            PHIP(hipMemcpy(p->d_atm_axes + off, c->atm_axes[d], (size_t)c->atm_N[d] * 8, hipMemcpyHostToDevice));
+            off += c->atm_N[d];
+        }
+        PHIP(hipMemcpy(p->d_atm_I, c->atm_I, ni * 8, hipMemcpyHostToDevice));
+    }
     PCHK(dalloc(p, &p->d_flux, nm * NE * N_P));
     PCHK(dalloc(p, &p->d_member_index, ng * 2));
     PCHK(dalloc(p, &p->d_work, nm));
@@ -523,14 +566,15 @@ extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs,
                        p->d_defl, p->d_lag, p->d_maxDefl);
     HIPCHK(hipGetLastError());
     RingArgs RA{p->d_theta, p->d_r, p->d_u, p->d_cos_gamma, p->d_maxDefl, p->d_defl, p->d_cosa, p->d_lag,
-                p->d_leaves, p->d_ring_ok, p->d_geo, p->d_valid};
+                p->d_leaves, p->d_ring_ok, p->d_geo, p->d_valid, p->d_lastq};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[2], s));
     hipLaunchKernelGGL(k_geometry, dim3(S, L.IO_max, nm), dim3(256), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
     HIPCHK(hipGetLastError());
     // parallels per block: all of them when the batch fills the GPU
     int rb = (n >= 32) ? S : 1;
-    FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_energies, p->d_phases,
-                p->d_flux, rb};
+    FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_effGrav, p->d_lastq,
+                p->d_atm_axes, p->d_atm_I, {p->atm_N[0], p->atm_N[1], p->atm_N[2], p->atm_N[3]}, p->d_energies,
+                p->d_phases, p->d_flux, rb};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
     HIPCHK(launch_fold(p->EC, dim3((N_E + p->EC - 1) / p->EC, (S + rb - 1) / rb, nm), fold_lds(L.N_L, p->EC, S), s,
                        p->d_members, L, FA));

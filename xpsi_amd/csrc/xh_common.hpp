@@ -301,6 +301,33 @@ __device__ inline void wave_gk61(F& f, double a, double b, double& res, double& 
     err = scale_error(e, rabs, rasc);
 }
 
+// Two integrands on the same Gauss-Kronrod nodes (one pass, shared node
+// evaluation); results as wave_gk61 would give for each.
+template <class F>
+__device__ inline void wave_gk61_pair(F& f, double a, double b, double res[2], double err[2],
+                                      double rabs[2], double rasc[2])
+{
+    const int lane = lane_id();
+    const double ctr = 0.5 * (a + b), hl = 0.5 * (b - a), ahl = fabs(hl);
+    int k = lane < 30 ? lane : (lane < 60 ? lane - 30 : 30);
+    double wk = lane <= 60 ? WGK61[k] : 0.0;
+    double wg = (lane < 60 && (k & 1)) ? WG30[k >> 1] : 0.0;
+    double x = lane < 30 ? ctr - hl * XGK61[k] : (lane < 60 ? ctr + hl * XGK61[k] : ctr);
+    double f0 = 0.0, f1 = 0.0;
+    if (lane <= 60) f.both(x, f0, f1);
+    double rk0 = wave_sum(wk * f0), rk1 = wave_sum(wk * f1);
+    double rg0 = wave_sum(wg * f0), rg1 = wave_sum(wg * f1);
+    double ra0 = wave_sum(wk * fabs(f0)), ra1 = wave_sum(wk * fabs(f1));
+    double m0 = 0.5 * rk0, m1 = 0.5 * rk1;
+    double rs0 = wave_sum(lane <= 60 ? wk * fabs(f0 - m0) : 0.0);
+    double rs1 = wave_sum(lane <= 60 ? wk * fabs(f1 - m1) : 0.0);
+    res[0] = rk0 * hl; res[1] = rk1 * hl;
+    rabs[0] = ra0 * ahl; rabs[1] = ra1 * ahl;
+    rasc[0] = rs0 * ahl; rasc[1] = rs1 * ahl;
+    err[0] = scale_error((rk0 - rg0) * hl, rabs[0], rasc[0]);
+    err[1] = scale_error((rk1 - rg1) * hl, rabs[1], rasc[1]);
+}
+
 __device__ __forceinline__ bool qag_too_small(double a1, double a2, double b2)
 {
     double tmp = (1.0 + 100.0 * DBL_EPSILON) * (fabs(a2) + 1000.0 * DBL_MIN);
@@ -310,12 +337,26 @@ __device__ __forceinline__ bool qag_too_small(double a1, double a2, double b2)
 // QUADPACK QAG (bisect the interval of largest error) with GK61.
 // Wave-cooperative: all 64 lanes call with identical arguments.
 template <class F>
+__device__ inline int wave_qag61_from(F& f, double a, double b, double epsabs, double epsrel, int limit,
+                                      double r0, double e0, double ra0, double rs0, double& result,
+                                      double& abserr);
+
+template <class F>
 __device__ inline int wave_qag61(F& f, double a, double b, double epsabs, double epsrel, int limit,
                                  double& result, double& abserr)
 {
-    const int lane = lane_id();
     double r0, e0, ra0, rs0;
     wave_gk61(f, a, b, r0, e0, ra0, rs0);
+    return wave_qag61_from(f, a, b, epsabs, epsrel, limit, r0, e0, ra0, rs0, result, abserr);
+}
+
+// QAG continued from the first 61-point pass over [a, b]
+template <class F>
+__device__ inline int wave_qag61_from(F& f, double a, double b, double epsabs, double epsrel, int limit,
+                                      double r0, double e0, double ra0, double rs0, double& result,
+                                      double& abserr)
+{
+    const int lane = lane_id();
     double tol = fmax(epsabs, epsrel * fabs(r0));
     double roundoff = 50.0 * DBL_EPSILON * ra0;
     result = r0;

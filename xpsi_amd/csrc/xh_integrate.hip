@@ -102,7 +102,8 @@ __global__ void __launch_bounds__(256) k_geometry(Member* __restrict__ members, 
     double* Zs = PH + N_L;              // N_L
     double* As = Zs + N_L;              // N_L
     double* Gs = As + N_L;              // N_L
-    int* vis = (int*)(Gs + N_L);        // N_L/2+1
+    double* PV = Gs + N_L;              // N_L
+    int* vis = (int*)(PV + N_L);        // N_L/2+1
     __shared__ int jn_s, any_vis, term;
     const size_t tb = ro * N_R;
     for (int q = tid; q < N_R; q += blockDim.x) {
@@ -142,7 +143,7 @@ __global__ void __launch_bounds__(256) k_geometry(Member* __restrict__ members, 
         if (!any_vis) { if (tid == 0) A.valid[vix] = 0; return; }
         __syncthreads();
     }
-    for (int k = tid; k < N_L; k += blockDim.x) { Gs[k] = 0.0; Zs[k] = 1.0; As[k] = 0.0; }
+    for (int k = tid; k < N_L; k += blockDim.x) { Gs[k] = 0.0; Zs[k] = 1.0; As[k] = 0.0; PV[k] = -3.0; }
     __syncthreads();
     for (int k = tid; k < leaf_lim; k += blockDim.x) {
         LeafGeo g = leaf_geometry(I, leaves[k], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg, maxD,
@@ -198,6 +199,24 @@ __global__ void __launch_bounds__(256) k_geometry(Member* __restrict__ members, 
                 invis_phase = k;
             }
         }
+        // order of the intensity queries (leaf k ascending, then its mirror):
+        // PV[kd] = previous queried leaf, -1 = last query of the previous
+        // parallel/image, -2 = copied from leaf 0 (no query)
+        int prevq = -1;
+        for (int k = 0; k < leaf_lim; k++) {
+            if (!vis[k]) continue;
+            for (int ks = 0; ks < 2; ks++) {
+                bool take = (0 < k && k < leaf_lim - 1) || (k == 0 && ks == 0) ||
+                            (k == leaf_lim - 1 && N_L % 2 == 1 && ks == 0) ||
+                            (k == leaf_lim - 1 && N_L % 2 == 0);
+                if (!take) continue;
+                int kd = ks == 0 ? k : N_L - 1 - k;
+                PV[kd] = (double)prevq;
+                prevq = kd;
+            }
+            if (k == 0) PV[N_L - 1] = -2.0;
+        }
+        A.lastq[vix] = prevq;
         if (invis == 2) {
             A.valid[vix] = 0;
         } else {
@@ -208,17 +227,122 @@ __global__ void __launch_bounds__(256) k_geometry(Member* __restrict__ members, 
         }
     }
     __syncthreads();
-    double* out = A.geo + vix * 4 * N_L;
+    double* out = A.geo + vix * 5 * N_L;
     for (int q = tid; q < N_L; q += blockDim.x) {
         out[q] = PH[q];
         out[N_L + q] = Zs[q];
         out[2 * N_L + q] = As[q];
         out[3 * N_L + q] = Gs[q];
+        out[4 * N_L + q] = PV[q];
     }
 }
 
 
-template <int EC, int MAXK>
+// base node of a cubic Lagrange stencil for an in-table query: interval m
+// (x_m <= v < x_{m+1}) -> nodes m-1 .. m+2, kept inside the table
+__device__ __forceinline__ int n4_base(const double* x, int N, double v)
+{
+    if (v > x[N - 1]) return N - 4;
+    if (v < x[0]) return 0;
+    int m = interval_of(x, N, v) - 1;
+    return m < 0 ? 0 : (m > N - 4 ? N - 4 : m);
+}
+
+// Four-dimensional cubic Lagrange interpolation of the atmosphere table
+// (hot_Num4D.pyx:248-438).  The reference keeps its base node from call to
+// call: a query outside the table is clamped to the edge only when the
+// previous query's base node was not already at that edge, and extrapolated
+// otherwise.  `prev` is the previous query in the reference's order.
+__device__ double num4d_I(const FoldArgs& F, const double vec_in[4], const double prev[4], int has_prev)
+{
+    double v[4], Lw[4][4];
+    int bn[4];
+    const double* ax = F.atm_axes;
+    int S[4];
+    S[3] = 1;
+    S[2] = F.atm_N[3];
+    S[1] = F.atm_N[2] * F.atm_N[3];
+    S[0] = F.atm_N[1] * S[1];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        const double* x = ax;
+        const int N = F.atm_N[d];
+        ax += N;
+        double q = vec_in[d];
+        int pb = has_prev ? n4_base(x, N, prev[d]) : 0;
+        int b;
+        if (q > x[N - 1]) {
+            b = N - 4;
+            if (pb != N - 4) q = x[N - 1];
+        } else if (q < x[0]) {
+            b = 0;
+            if (pb != 0) q = x[0];
+        } else b = n4_base(x, N, q);
+        v[d] = q;
+        bn[d] = b;
+        const double x0 = x[b], x1 = x[b + 1], x2 = x[b + 2], x3 = x[b + 3];
+        double D0 = (q - x1) * (q - x2) * (q - x3), D1 = (q - x0) * (q - x2) * (q - x3);
+        double D2 = (q - x0) * (q - x1) * (q - x3), D3 = (q - x0) * (q - x1) * (q - x2);
+        double S0 = 1.0 / (x0 - x1) / (x0 - x2) / (x0 - x3), S1 = 1.0 / (x1 - x0) / (x1 - x2) / (x1 - x3);
+        double S2 = 1.0 / (x2 - x0) / (x2 - x1) / (x2 - x3), S3 = 1.0 / (x3 - x0) / (x3 - x1) / (x3 - x2);
+        Lw[d][0] = D0 * S0; Lw[d][1] = D1 * S1; Lw[d][2] = D2 * S2; Lw[d][3] = D3 * S3;
+    }
+    const double* T = F.atm_I + (size_t)bn[0] * S[0] + (size_t)bn[1] * S[1] + (size_t)bn[2] * S[2] + bn[3];
+    double I = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const double wab = Lw[0][a] * Lw[1][b];
+            const double* Tab = T + (size_t)a * S[0] + (size_t)b * S[1];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const double wabc = wab * Lw[2][c];
+                const double* Tc = Tab + (size_t)c * S[2];
+                I += wabc * (Lw[3][0] * Tc[0] + Lw[3][1] * Tc[1] + Lw[3][2] * Tc[2] + Lw[3][3] * Tc[3]);
+            }
+        }
+    if (I < 0.0) return 0.0;
+    return I * pow(10.0, 3.0 * v[0]);
+}
+
+// the query made just before parallel i, image I (the last query of the
+// nearest earlier contributing parallel/image), for the table atmosphere
+__device__ void prev_query_state(const FoldArgs& F, const Layout& L, int mi, int i, int I, double* out)
+{
+    out[0] = 0.0;
+    int ri = i, Ii = I - 1;
+    while (ri >= 0) {
+        const size_t ro = (size_t)mi * L.S_max + ri;
+        if (Ii < 0) {
+            // last contributing image of the previous parallel
+            ri--;
+            if (ri < 0) break;
+            const size_t r2 = (size_t)mi * L.S_max + ri;
+            Ii = -1;
+            for (int q = 0; q < L.IO_max; q++) {
+                if (!F.valid[r2 * L.IO_max + q]) break;
+                Ii = q;
+            }
+            continue;
+        }
+        const size_t vix = ro * L.IO_max + Ii;
+        int kd = F.lastq[vix];
+        if (kd >= 0) {
+            const double* g = F.geo + vix * 5 * L.N_L;
+            const double kT = K_B_OVER_KEV * pow(10.0, F.logT[ro]);
+            out[0] = 1.0;
+            out[1] = F.logT[ro];
+            out[2] = F.logg[ro];
+            out[3] = g[2 * L.N_L + kd];
+            out[4] = log10((F.energies[L.N_E - 1] / g[L.N_L + kd]) / kT);
+            return;
+        }
+        Ii--;
+    }
+}
+
+template <int EC, int MAXK, int ATM>
 __global__ void __launch_bounds__(256, 4) k_profile_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
 {
     extern __shared__ double sm[];
@@ -241,7 +365,8 @@ __global__ void __launch_bounds__(256, 4) k_profile_fold(const Member* __restric
     double* Zs = PH + N_L;
     double* As = Zs + N_L;
     double* Gs = As + N_L;
-    double* cphi = Gs + N_L;              // S_max
+    double* PV = Gs + N_L;
+    double* cphi = PV + N_L;              // S_max
     double* carea = cphi + L.S_max;       // S_max
     const int p = tid % EC, slot = tid / EC;
     const int pe = e0 + p;
@@ -255,6 +380,7 @@ __global__ void __launch_bounds__(256, 4) k_profile_fold(const Member* __restric
         acc[q] = 0.0;
     }
     const size_t cbase = (size_t)mi * L.S_max * L.S_max;
+    __shared__ double prevq[5];   // has-previous flag, then logT, logg, mu, log10(E/kT)
     for (int i = r0; i < r1; i++) {
         const size_t ro = (size_t)mi * L.S_max + i;
         __syncthreads();   // the previous parallel's fold is done with LDS
@@ -271,23 +397,46 @@ __global__ void __launch_bounds__(256, 4) k_profile_fold(const Member* __restric
             const size_t vix = ro * L.IO_max + I;
             if (!F.valid[vix]) break;
             __syncthreads();
-            const double* g = F.geo + vix * 4 * N_L;
+            const double* g = F.geo + vix * 5 * N_L;
             for (int q = tid; q < N_L; q += 256) {
                 PH[q] = g[q]; Zs[q] = g[N_L + q]; As[q] = g[2 * N_L + q]; Gs[q] = g[3 * N_L + q];
+                PV[q] = g[4 * N_L + q];
             }
+            if (ATM == 2 && tid == 0) prev_query_state(F, L, mi, i, I, prevq);
             __syncthreads();
-            // specific-intensity profile (BB), zero where the leaf is hidden
+            // specific-intensity profile, zero where the leaf is hidden
             for (int q = tid; q < N_L * EC; q += 256) {
                 int mm = q / EC, pp = q % EC;
                 double v = 0.0;
                 double geom = Gs[mm];
-                if (geom != 0.0 && e0 + pp < N_E) {
+                if (geom != 0.0 && e0 + pp < N_E && PV[mm] > -1.5) {
                     double Ep = F.energies[e0 + pp] / Zs[mm];
-                    double I_E = Ep * Ep * Ep / (exp(Ep / kT) - 1.0);
-                    v = (I_E * BB_NORM - 0.0) * geom;
+                    if (ATM == 2) {
+                        double vec[4], prev[4];
+                        int has_prev = 1;
+                        vec[0] = F.logT[ro]; vec[1] = F.logg[ro]; vec[2] = As[mm];
+                        vec[3] = log10(Ep / kT);
+                        if (e0 + pp > 0) {
+                            prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = vec[2];
+                            prev[3] = log10((F.energies[e0 + pp - 1] / Zs[mm]) / kT);
+                        } else if (PV[mm] > -0.5) {
+                            int kd = (int)PV[mm];
+                            prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = As[kd];
+                            prev[3] = log10((F.energies[N_E - 1] / Zs[kd]) / kT);
+                        } else if (prevq[0] > 0.5) {
+                            prev[0] = prevq[1]; prev[1] = prevq[2]; prev[2] = prevq[3]; prev[3] = prevq[4];
+                        } else has_prev = 0;
+                        v = (num4d_I(F, vec, prev, has_prev) * NUM4D_NORM - 0.0) * geom;
+                    } else {
+                        double I_E = Ep * Ep * Ep / (exp(Ep / kT) - 1.0);
+                        v = (I_E * BB_NORM - 0.0) * geom;
+                    }
                 }
                 Y[q] = v;
             }
+            __syncthreads();
+            for (int q = tid; q < EC; q += 256)
+                if (PV[N_L - 1] < -1.5) Y[(N_L - 1) * EC + q] = Y[q];
             __syncthreads();
             // secant slopes, one division per interval and energy
             for (int q = tid; q < NI * EC; q += 256) {
@@ -380,7 +529,9 @@ __global__ void __launch_bounds__(256) k_work_count(const Member* __restrict__ m
     if (threadIdx.x == 0) work[mi] = acc;
 }
 
-#define XH_FOLD_INST(EC, K) template __global__ void k_profile_fold<EC, K>(const Member*, Layout, FoldArgs);
+#define XH_FOLD_INST(EC, K)                                                                   \
+    template __global__ void k_profile_fold<EC, K, 1>(const Member*, Layout, FoldArgs);        \
+    template __global__ void k_profile_fold<EC, K, 2>(const Member*, Layout, FoldArgs);
 XH_FOLD_INST(64, 4) XH_FOLD_INST(64, 8) XH_FOLD_INST(64, 16)
 XH_FOLD_INST(32, 4) XH_FOLD_INST(32, 8) XH_FOLD_INST(32, 16)
 XH_FOLD_INST(16, 4) XH_FOLD_INST(16, 8) XH_FOLD_INST(16, 16)

@@ -61,13 +61,14 @@ namespace xh {
 __global__ void k_ring_finalize(const Member* members, Layout L, const double* rr,
                                 const double* cellArea, double* u, int* ring_ok, double* logT);
 __global__ void k_ring_prep(int S, const double* theta2d, const double* params, const int* radiates,
-                            double* theta, int* ring_ok, double* logT);
+                            double* theta, int* ring_ok, double* logT, double* logg);
 // per-member, per-ring inputs
 struct RingArgs {
     const double *theta, *r, *u, *cos_gamma, *maxDefl, *defl, *cos_alpha, *lag, *leaves;
     const int* ring_ok;
-    double* geo;     // [member][ring][image][4][N_L]: PHASE, Z, ABB(mu eta), GEOM
+    double* geo;     // [member][ring][image][5][N_L]: PHASE, Z, ABB(mu eta), GEOM, PREV
     int* valid;      // [member][ring][image]
+    int* lastq;      // [member][ring][image]: leaf index of the last intensity query
 };
 
 struct FoldArgs {
@@ -78,6 +79,11 @@ struct FoldArgs {
     const double* phi;        // [member][...], see phi_row_stride
     int phi_row_stride;       // 0: one azimuth row per member, else S
     const double* logT;       // per member per ring [member][S_max]
+    const double* logg;       // per member per ring [member][S_max] (table atmosphere)
+    const int* lastq;         // [member][ring][image]
+    const double* atm_axes;   // table axes: logT, logg, mu, log10(E/kT) concatenated
+    const double* atm_I;      // table values, C order (T, g, mu, E)
+    int atm_N[4];
     const double* energies;   // N_E
     const double* phases;     // N_P (radians)
     double* flux;             // [member][N_E][N_P]
@@ -85,7 +91,7 @@ struct FoldArgs {
 };
 
 __global__ void k_geometry(Member* members, Layout L, RingArgs A);
-template <int EC, int MAXK>
+template <int EC, int MAXK, int ATM>
 __global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
 __global__ void k_fill(double* p, size_t n, double v);
 __global__ void k_work_count(const Member* members, Layout L, const double* cellArea, const int* valid,

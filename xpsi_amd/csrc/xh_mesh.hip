@@ -568,13 +568,14 @@ __global__ void __launch_bounds__(256) k_ring_prep(int S, const double* __restri
                                                    const double* __restrict__ params,
                                                    const int* __restrict__ radiates,
                                                    double* __restrict__ theta, int* __restrict__ ring_ok,
-                                                   double* __restrict__ logT)
+                                                   double* __restrict__ logT, double* __restrict__ logg)
 {
     for (int i = threadIdx.x; i < S; i += blockDim.x) {
         int J = -1;
         for (int j = 0; j < S; j++) if (radiates[(size_t)i * S + j] == 1) { J = j; break; }
         ring_ok[i] = J >= 0;
         logT[i] = J >= 0 ? params[((size_t)i * S + J) * 2] : 0.0;
+        logg[i] = J >= 0 ? params[((size_t)i * S + J) * 2 + 1] : 0.0;
         theta[i] = theta2d[(size_t)i * S];
     }
 }

@@ -27,6 +27,15 @@ struct InLag {
         return X * X / (x + X);
     }
 };
+struct InBoth {
+    double rc;
+    __device__ void both(double x, double& d, double& l) const
+    {
+        double X = 1.0 / sqrt(2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (rc - 1.0));
+        d = X;
+        l = X * X / (x + X);
+    }
+};
 struct OutDef {
     double sas, Rr;
     __device__ double operator()(double x) const
@@ -42,12 +51,26 @@ struct OutLag {
         return x / (f + f * f);
     }
 };
+struct OutBoth {
+    double sas, Rr;
+    __device__ void both(double x, double& d, double& l) const
+    {
+        double f = sqrt(1.0 - sas + x * x * sas * (2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (Rr - 1.0)));
+        d = x / f;
+        l = x / (f + f * f);
+    }
+};
 
-template <class F>
-__device__ __forceinline__ int ray_quad(F f, double a, double b, double& r)
+// QAG (epsrel 1e-12, 100 intervals) of the deflection and lag integrands
+// over one interval; the first Gauss-Kronrod pass is shared by the two
+template <class FB, class FD, class FL>
+__device__ void ray_pair(FB& fb, FD& fd, FL& fl, double a, double b, double& d, double& l, int& st_d,
+                         int& st_l)
 {
-    double e;
-    return wave_qag61(f, a, b, 0.0, 1.0e-12, 100, r, e);
+    double r[2], e[2], ra[2], rs[2], ed, el;
+    wave_gk61_pair(fb, a, b, r, e, ra, rs);
+    st_d = wave_qag61_from(fd, a, b, 0.0, 1.0e-12, 100, r[0], e[0], ra[0], rs[0], d, ed);
+    st_l = wave_qag61_from(fl, a, b, 0.0, 1.0e-12, 100, r[1], e[1], ra[1], rs[1], l, el);
 }
 
 // deflection and lag (seconds) of the ray leaving radius r_s/u at cos(alpha)
@@ -59,17 +82,14 @@ __device__ void wave_ray(double cos_alpha, double r_s, double u, double& defl, d
     double alpha = acos(cos_alpha);
     double b = sa / (u * sqrt(1.0 - u));
     double d0 = __builtin_nan(""), l0 = __builtin_nan("");
-    int st = XH_SUCCESS;
+    int sd = XH_SUCCESS, sl = XH_SUCCESS;
     if (b <= b_ph) {
+        OutBoth fb{sas, 1.0 / u};
         OutDef fd{sas, 1.0 / u};
         OutLag fl{sas, 1.0 / u};
-        st = ray_quad(fd, 0.0, 1.0, d0);
-        if (st == XH_EFAILED) d0 = __builtin_nan("");
+        ray_pair(fb, fd, fl, 0.0, 1.0, d0, l0, sd, sl);
+        if (sd == XH_EFAILED || sl == XH_EFAILED) d0 = __builtin_nan("");
         else {
-            st = ray_quad(fl, 0.0, 1.0, l0);
-            if (st == XH_EFAILED) d0 = __builtin_nan("");
-        }
-        if (st != XH_EFAILED) {
             d0 *= 2.0 * b * u;
             l0 *= 2.0 * b * b * u * r_s;
         }
@@ -81,15 +101,12 @@ __device__ void wave_ray(double cos_alpha, double r_s, double u, double& defl, d
             wR = sqrt(1.0 - rc * u);
         }
         if (isnan(wR)) wR = 0.0;
+        InBoth fb{rc};
         InDef fd{rc};
         InLag fl{rc};
-        st = ray_quad(fd, wR, 1.0, d0);
-        if (st == XH_EFAILED) d0 = __builtin_nan("");
+        ray_pair(fb, fd, fl, wR, 1.0, d0, l0, sd, sl);
+        if (sd == XH_EFAILED || sl == XH_EFAILED) d0 = __builtin_nan("");
         else {
-            st = ray_quad(fl, wR, 1.0, l0);
-            if (st == XH_EFAILED) d0 = __builtin_nan("");
-        }
-        if (st != XH_EFAILED) {
             d0 *= 2.0 * b / rc;
             l0 *= 2.0 * b * b * r_s / rc;
         }
@@ -97,21 +114,24 @@ __device__ void wave_ray(double cos_alpha, double r_s, double u, double& defl, d
         double rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
         double wR = sqrt(1.0 - rc * u);
         if (isnan(wR)) wR = 0.0;
+        InBoth fb{rc};
         InDef fd{rc};
         InLag fl{rc};
         double D1 = 0, D0 = 0, L1 = 0, L0 = 0;
-        st = ray_quad(fd, 0.0, 1.0, D1);
-        if (st != XH_EFAILED) st = ray_quad(fd, wR, 1.0, D0);
-        if (st != XH_EFAILED) st = ray_quad(fl, 0.0, 1.0, L1);
-        if (st != XH_EFAILED) st = ray_quad(fl, wR, 1.0, L0);
-        if (st != XH_EFAILED) {
+        int s1, s2, s3, s4;
+        ray_pair(fb, fd, fl, 0.0, 1.0, D1, L1, s1, s3);
+        ray_pair(fb, fd, fl, wR, 1.0, D0, L0, s2, s4);
+        if (s1 == XH_EFAILED || s2 == XH_EFAILED || s3 == XH_EFAILED || s4 == XH_EFAILED) {
+            d0 = __builtin_nan("");
+            sd = XH_EFAILED;
+        } else {
             d0 = 2.0 * b * (2.0 * D1 - D0) / rc;
             l0 = 2.0 * b * b * r_s * (2.0 * L1 - L0) / rc;
             double cor = 1.0 / u - rc + log((1.0 / u - 1.0) / (rc - 1.0));
             l0 += 2.0 * r_s * cor;
         }
     }
-    if (st != XH_EFAILED) l0 /= C_SI;
+    if (sd != XH_EFAILED && sl != XH_EFAILED) l0 /= C_SI;
     defl = d0;
     lag = l0;
 }

@@ -76,6 +76,19 @@ class LikelihoodPipeline:
         cfg.num_phases = int(h0.get("num_phases") or cfg.num_leaves)
         cfg.image_order_limit = int(h0.get("image_order_limit") or 0)
         cfg.atm_ext = int(h0.get("atm_ext", 1))
+        if cfg.atm_ext == 2:
+            atm = model.get("atmosphere")
+            if atm is None:
+                raise XHError("atm_ext 2 needs model['atmosphere'] = (logT, logg, mu, logE, buf)")
+            axes = [np.ascontiguousarray(a, dtype=np.float64) for a in atm[:4]]
+            buf = np.ascontiguousarray(atm[4], dtype=np.float64)
+            if buf.size != np.prod([len(a) for a in axes]):
+                raise XHError("atmosphere buffer size does not match its axes")
+            self._keep += axes + [buf]
+            for d in range(4):
+                cfg.atm_axes[d] = axes[d].ctypes.data
+                cfg.atm_N[d] = len(axes[d])
+            cfg.atm_I = buf.ctypes.data
         sp = model["spacetime"]
         cfg.star_shape = {"AGM_14": 0, "sphere": 1}[sp.get("star_shape", "AGM_14")]
         cfg.frequency = float(sp["frequency"])
