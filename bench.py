@@ -36,6 +36,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--fold-ec", type=int, default=0, help="energies per fold block (0 = auto)")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the likelihood path shards requests across GPUs "
@@ -145,6 +147,8 @@ def main():
                      "flop_per_eval": 8},
         "device": dev,
     }
+    if not args.no_serving:
+        out["serving"] = serving(args, stream_free=True)
     if not args.no_cpu:
         from oracle import oracle as O
         t0 = time.perf_counter()
@@ -159,6 +163,53 @@ def main():
                                          "one thread) with the C oracle" % nc,
                                "host_cpus": os.cpu_count()}
     print(json.dumps(out))
+
+
+def serving(args, stream_free=True):
+    """Poisson arrivals of config-3 (128-energy) prior draws at several rates
+    into the continuous-batching server; latency = submit to result."""
+    from xpsi_amd import configs
+    from xpsi_amd.serve import LikelihoodServer
+    draws = [d for d in configs.config3_draws()
+             if configs.prior_accepts(d["mass"], d["radius"], 300.0) and d["num_energies"] == 128]
+    srv = LikelihoodServer(configs.config3_model(128), n_slots=4, slot_batch=128,
+                           hbm_budget_bytes=64e9, use_graphs=True)
+    # warm the graphs of common batch sizes
+    for d in draws[:256]:
+        srv.submit(d)
+    got = 0
+    t0 = time.perf_counter()
+    while got < 256 and time.perf_counter() - t0 < 60:
+        got += len(srv.poll()[0])
+    out = {}
+    rng = np.random.default_rng(4)
+    for rate in (2000.0, 5000.0, 10000.0, 20000.0):
+        n = int(rate * args.serve_seconds)
+        gaps = rng.exponential(1.0 / rate, n)
+        t_sub = np.cumsum(gaps)
+        lat, done = [], 0
+        t0 = time.perf_counter()
+        k = 0
+        while done < n:
+            now = time.perf_counter() - t0
+            while k < n and t_sub[k] <= now:
+                srv.submit(draws[k % len(draws)])
+                k += 1
+            ids, lnl, l = srv.poll()
+            done += len(ids)
+            lat.extend(l.tolist())
+            if time.perf_counter() - t0 > 120:
+                break
+        wall = time.perf_counter() - t0
+        lat = np.array(lat) * 1e3
+        out["%d_req_per_s" % int(rate)] = {
+            "offered": rate, "completed": int(done), "throughput": done / wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+    out["stats"] = srv.stats()
+    out["workload"] = ("config-3 prior draws with 128 energies, sqrt_num_cells in {24..96}, "
+                       "4 slots x 128, graphs on")
+    srv.close()
+    return out
 
 
 if __name__ == "__main__":

@@ -174,6 +174,27 @@ int xh_pipeline_work(xh_pipeline* p, int n, double* out);
  * and to xh_integrate calls. */
 int xh_set_fold_energy_chunk(int ec);
 
+/* Replay each launch as a hipGraph captured once per batch size (requires a
+ * non-default stream; timing mode launches eagerly). */
+int xh_pipeline_use_graphs(xh_pipeline* p, int on);
+
+/* ------------------------------------------------------------------------
+ * Continuous-batching server: n_slots pipelines of slot_batch requests, each
+ * on its own stream; a dispatcher thread admits queued requests into free
+ * slots while the in-flight device footprint stays within hbm_budget_bytes
+ * (<= 0: unlimited).  Additive to the reference API (its samplers call the
+ * likelihood one vector at a time, xpsi/Sample.py:32-130).
+ * --------------------------------------------------------------------- */
+typedef struct xh_server xh_server;
+xh_server* xh_server_create(const xh_model_config* cfg, int n_slots, int slot_batch,
+                            double hbm_budget_bytes, int use_graphs);
+int xh_server_submit(xh_server* s, const xh_request* req, long long* id_out);
+/* Finished requests: ids, lnL and submit-to-result latency in seconds. */
+int xh_server_poll(xh_server* s, long long* ids, double* lnl, double* latency_s, int max_n);
+/* queued, in-flight bytes, launched batches, completed, mean batch fill */
+int xh_server_stats(xh_server* s, double* out, int n);
+void xh_server_destroy(xh_server* s);
+
 /* Copy an intermediate array of request `req` to host memory for checks.
  * what: 0 flux[member], 1 folded signal[region] [N_ch, N_P],
  *       2 member sizes (S, status) as doubles, 3 cellArea[member] [S,S],

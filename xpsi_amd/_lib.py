@@ -93,6 +93,18 @@ def lib():
     L.xh_pipeline_work.argtypes = [vp, i32, vp]
     L.xh_set_fold_energy_chunk.restype = i32
     L.xh_set_fold_energy_chunk.argtypes = [i32]
+    L.xh_pipeline_use_graphs.restype = i32
+    L.xh_pipeline_use_graphs.argtypes = [vp, i32]
+    L.xh_server_create.restype = vp
+    L.xh_server_create.argtypes = [C.POINTER(ModelConfig), i32, i32, d, i32]
+    L.xh_server_submit.restype = i32
+    L.xh_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_longlong)]
+    L.xh_server_poll.restype = i32
+    L.xh_server_poll.argtypes = [vp, vp, vp, vp, i32]
+    L.xh_server_stats.restype = i32
+    L.xh_server_stats.argtypes = [vp, vp, i32]
+    L.xh_server_destroy.restype = None
+    L.xh_server_destroy.argtypes = [vp]
     L.xh_pipeline_fetch.restype = i32
     L.xh_pipeline_fetch.argtypes = [vp, i32, i32, i32, vp, sz]
     _lib = L
@@ -111,4 +123,5 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",
             "xh_pipeline_members_per_request", "xh_pipeline_work",
-            "xh_set_fold_energy_chunk"]
+            "xh_set_fold_energy_chunk", "xh_pipeline_use_graphs", "xh_server_create",
+            "xh_server_submit", "xh_server_poll", "xh_server_stats", "xh_server_destroy"]

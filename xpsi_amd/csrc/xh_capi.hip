@@ -76,6 +76,7 @@ static hipError_t launch_fold_t(dim3 g, size_t lds, hipStream_t s, const Member*
         if (e != hipSuccess) return e;
         attr = true;
     }
+    if (g.x == 0) return hipSuccess;     // configuration only
     hipLaunchKernelGGL((k_profile_fold<EC, K, ATM>), g, dim3(256), lds, s, m, L, F);
     return hipGetLastError();
 }
@@ -307,6 +308,8 @@ struct xh_pipeline {
     double *h_shifts, *h_lnl;
     hipStream_t stream;
     int last_n;
+    bool use_graphs;
+    std::vector<hipGraphExec_t> graphs;   // per batch size n (index n), null until captured
 };
 
 template <class T>
@@ -324,6 +327,7 @@ static int dalloc(xh_pipeline* p, T** ptr, size_t n)
 extern "C" void xh_pipeline_destroy(xh_pipeline* p)
 {
     if (!p) return;
+    for (hipGraphExec_t g : p->graphs) if (g) (void)hipGraphExecDestroy(g);
     for (void* v : p->allocs) (void)hipFree(v);
     for (int e = 0; e < 16; e++) if (p->ev[e]) (void)hipEventDestroy(p->ev[e]);
     if (p->h_members) (void)hipHostFree(p->h_members);
@@ -360,6 +364,14 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     p->L = Layout{smax, c->num_rays, c->num_leaves, N_P, c->N_E, IO, c->image_order_limit > 0 ? c->image_order_limit : 0,
                   c->atm_ext};
     p->EC = pick_ec(c->num_leaves);
+    {
+        // configure the fold kernel's LDS limit now, never inside a capture
+        Layout L0{};
+        L0.N_P = c->num_phases > 0 ? c->num_phases : c->num_leaves;
+        L0.atm_ext = c->atm_ext;
+        FoldArgs F0{};
+        PHIP(launch_fold(p->EC, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
+    }
     if (N_P > (256 / p->EC) * 16) { fail(XH_EINVAL, "too many phases for the leaf count"); delete p; return nullptr; }
     if (c->atm_ext != 1 && c->atm_ext != 2) { fail(XH_EINVAL, "pipeline: atmosphere must be 1 (blackbody) or 2 (4-D table)"); delete p; return nullptr; }
     if (c->atm_ext == 2) {
@@ -528,6 +540,8 @@ static void describe(const xh_pipeline* p, int rq, const xh_request& r)
     }
 }
 
+static int enqueue(xh_pipeline* p, int n, hipStream_t s);
+
 extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs, void* stream)
 {
     if (!p || n < 1 || n > p->maxb) return fail(XH_EINVAL, "xh_pipeline_launch: batch size out of range");
@@ -535,6 +549,28 @@ extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs,
     p->stream = s;
     p->last_n = n;
     for (int rq = 0; rq < n; rq++) describe(p, rq, reqs[rq]);
+    if (!p->use_graphs || p->timing || s == nullptr) return enqueue(p, n, s);
+    if ((int)p->graphs.size() <= n) p->graphs.resize(n + 1, nullptr);
+    if (!p->graphs[n]) {
+        // capture the whole device sequence once for this batch size; the
+        // host descriptors it copies from are refreshed before each replay
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue(p, n, s);
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != XH_OK) return rc;
+        HIPCHK(e);
+        hipGraphExec_t ge;
+        HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        p->graphs[n] = ge;
+    }
+    HIPCHK(hipGraphLaunch(p->graphs[n], s));
+    return XH_OK;
+}
+
+static int enqueue(xh_pipeline* p, int n, hipStream_t s)
+{
     const Layout L = p->L;
     const int nm = n * p->mpr, ng = n * p->nreg;
     const int S = L.S_max, N_P = L.N_P, N_E = L.N_E;
@@ -601,6 +637,12 @@ extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs,
     HIPCHK(hipGetLastError());
     if (p->timing) HIPCHK(hipEventRecord(p->ev[6], s));
     HIPCHK(hipMemcpyAsync(p->h_lnl, p->d_lnl, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    return XH_OK;
+}
+
+extern "C" int xh_pipeline_use_graphs(xh_pipeline* p, int on)
+{
+    p->use_graphs = on != 0;
     return XH_OK;
 }
 

@@ -33,6 +33,19 @@ _REGION_KEYS = ("phase_shift", "super_colatitude", "super_radius", "super_temper
                 "cede_colatitude", "cede_radius", "cede_azimuth", "cede_temperature")
 
 
+def fill_request(r, p):
+    """Copy a parameter-vector dict into an xh_request."""
+    r.mass, r.radius = float(p["mass"]), float(p["radius"])
+    r.distance, r.cos_inclination = float(p["distance"]), float(p["cos_inclination"])
+    for g, hp in enumerate(p["hot_regions"]):
+        for q, key in enumerate(_REGION_KEYS):
+            v = hp.get(key)
+            if v is None:
+                v = hp["super_colatitude"] if key == "cede_colatitude" else 0.0
+            r.region[g][q] = float(v)
+        r.sqrt_num_cells[g] = int(hp.get("sqrt_num_cells", 0))
+
+
 def construct_energy_array(num_energies, edges):
     """Energy grid over one instrument waveband (Signal.py:1092-1197)."""
     L, U = edges[0], edges[-1]
@@ -40,80 +53,88 @@ def construct_energy_array(num_energies, edges):
     return np.ascontiguousarray(e[::-1])
 
 
+def model_config(model, max_batch=1, max_sqrt_cells=0):
+    """Build the C configuration block for a model description; returns
+    (config, energies, keep-alive list of the host arrays it points to)."""
+    hrs = model["hot_regions"]
+    if not 1 <= len(hrs) <= 2:
+        raise XHError("one or two hot regions are supported")
+    h0 = hrs[0]
+    for h in hrs[1:]:
+        for key in ("num_rays", "num_leaves", "num_phases", "image_order_limit", "atm_ext"):
+            if h.get(key) != h0.get(key):
+                raise XHError("hot regions must share %s" % key)
+    inst, data = model["instrument"], model["data"]
+    energies = construct_energy_array(model["num_energies"], inst["energy_edges"])
+    edges = np.ascontiguousarray(inst["energy_edges"], dtype=np.float64)
+    matrix = np.ascontiguousarray(inst["matrix"], dtype=np.float64)
+    counts = np.ascontiguousarray(data["counts"], dtype=np.float64)
+    dph = np.ascontiguousarray(data["phases"], dtype=np.float64)
+    support = model.get("support")
+    support = None if support is None else np.ascontiguousarray(support, dtype=np.float64)
+    att = None
+    if model.get("interstellar") is not None:
+        mids = 0.5 * (edges[:-1] + edges[1:])
+        att = np.ascontiguousarray(model["interstellar"](mids), dtype=np.float64)
+    keep = [energies, edges, matrix, counts, dph, support, att]
+    cfg = ModelConfig()
+    cfg.n_regions = len(hrs)
+    for k, h in enumerate(hrs):
+        cfg.region[k] = RegionConfig(int(bool(h.get("cede", False))),
+                                     int(bool(h.get("is_antiphased", False))),
+                                     int(h.get("sqrt_num_cells", 32)),
+                                     int(h.get("min_sqrt_num_cells", 10)),
+                                     int(h.get("max_sqrt_num_cells", 80)))
+    cfg.num_rays = int(h0.get("num_rays", 200))
+    cfg.num_leaves = int(h0.get("num_leaves", 64))
+    cfg.num_phases = int(h0.get("num_phases") or cfg.num_leaves)
+    cfg.image_order_limit = int(h0.get("image_order_limit") or 0)
+    cfg.atm_ext = int(h0.get("atm_ext", 1))
+    if cfg.atm_ext == 2:
+        atm = model.get("atmosphere")
+        if atm is None:
+            raise XHError("atm_ext 2 needs model['atmosphere'] = (logT, logg, mu, logE, buf)")
+        axes = [np.ascontiguousarray(a, dtype=np.float64) for a in atm[:4]]
+        buf = np.ascontiguousarray(atm[4], dtype=np.float64)
+        if buf.size != np.prod([len(a) for a in axes]):
+            raise XHError("atmosphere buffer size does not match its axes")
+        keep += axes + [buf]
+        for d in range(4):
+            cfg.atm_axes[d] = axes[d].ctypes.data
+            cfg.atm_N[d] = len(axes[d])
+        cfg.atm_I = buf.ctypes.data
+    sp = model["spacetime"]
+    cfg.star_shape = {"AGM_14": 0, "sphere": 1}[sp.get("star_shape", "AGM_14")]
+    cfg.frequency = float(sp["frequency"])
+    cfg.mode_frequency = float(sp.get("mode_frequency", sp["frequency"]))
+    cfg.N_E = len(energies)
+    cfg.energies = energies.ctypes.data
+    cfg.N_in = len(edges) - 1
+    cfg.energy_edges = edges.ctypes.data
+    cfg.attenuation = None if att is None else att.ctypes.data
+    cfg.N_ch = matrix.shape[0]
+    cfg.matrix = matrix.ctypes.data
+    cfg.N_b = counts.shape[1]
+    cfg.data_phases = dph.ctypes.data
+    cfg.counts = counts.ctypes.data
+    cfg.exposure_time = float(data["exposure_time"])
+    cfg.support = None if support is None else support.ctypes.data
+    ms = model.get("marginal", {})
+    cfg.epsabs = float(ms.get("epsabs", 0.0))
+    cfg.epsrel = float(ms.get("epsrel", 1.0e-8))
+    cfg.epsilon = float(ms.get("epsilon", 1.0e-3))
+    cfg.sigmas = float(ms.get("sigmas", 10.0))
+    cfg.slim = float(ms.get("slim", 20.0))
+    cfg.llzero = float(model.get("llzero", -1.0e90))
+    cfg.max_batch = int(max_batch)
+    cfg.max_sqrt_cells = int(max_sqrt_cells)
+    return cfg, energies, keep
+
+
 class LikelihoodPipeline:
     def __init__(self, model, max_batch=1, max_sqrt_cells=0):
+        cfg, self.energies, self._keep = model_config(model, max_batch, max_sqrt_cells)
         hrs = model["hot_regions"]
-        if not 1 <= len(hrs) <= 2:
-            raise XHError("one or two hot regions are supported")
-        h0 = hrs[0]
-        for h in hrs[1:]:
-            for key in ("num_rays", "num_leaves", "num_phases", "image_order_limit", "atm_ext"):
-                if h.get(key) != h0.get(key):
-                    raise XHError("hot regions must share %s" % key)
-        inst, data = model["instrument"], model["data"]
-        self.energies = construct_energy_array(model["num_energies"], inst["energy_edges"])
-        edges = np.ascontiguousarray(inst["energy_edges"], dtype=np.float64)
-        matrix = np.ascontiguousarray(inst["matrix"], dtype=np.float64)
-        counts = np.ascontiguousarray(data["counts"], dtype=np.float64)
-        dph = np.ascontiguousarray(data["phases"], dtype=np.float64)
-        support = model.get("support")
-        support = None if support is None else np.ascontiguousarray(support, dtype=np.float64)
-        att = None
-        if model.get("interstellar") is not None:
-            mids = 0.5 * (edges[:-1] + edges[1:])
-            att = np.ascontiguousarray(model["interstellar"](mids), dtype=np.float64)
-        self._keep = [self.energies, edges, matrix, counts, dph, support, att]
-        cfg = ModelConfig()
-        cfg.n_regions = len(hrs)
-        for k, h in enumerate(hrs):
-            cfg.region[k] = RegionConfig(int(bool(h.get("cede", False))),
-                                         int(bool(h.get("is_antiphased", False))),
-                                         int(h.get("sqrt_num_cells", 32)),
-                                         int(h.get("min_sqrt_num_cells", 10)),
-                                         int(h.get("max_sqrt_num_cells", 80)))
-        cfg.num_rays = int(h0.get("num_rays", 200))
-        cfg.num_leaves = int(h0.get("num_leaves", 64))
-        cfg.num_phases = int(h0.get("num_phases") or cfg.num_leaves)
-        cfg.image_order_limit = int(h0.get("image_order_limit") or 0)
-        cfg.atm_ext = int(h0.get("atm_ext", 1))
-        if cfg.atm_ext == 2:
-            atm = model.get("atmosphere")
-            if atm is None:
-                raise XHError("atm_ext 2 needs model['atmosphere'] = (logT, logg, mu, logE, buf)")
-            axes = [np.ascontiguousarray(a, dtype=np.float64) for a in atm[:4]]
-            buf = np.ascontiguousarray(atm[4], dtype=np.float64)
-            if buf.size != np.prod([len(a) for a in axes]):
-                raise XHError("atmosphere buffer size does not match its axes")
-            self._keep += axes + [buf]
-            for d in range(4):
-                cfg.atm_axes[d] = axes[d].ctypes.data
-                cfg.atm_N[d] = len(axes[d])
-            cfg.atm_I = buf.ctypes.data
-        sp = model["spacetime"]
-        cfg.star_shape = {"AGM_14": 0, "sphere": 1}[sp.get("star_shape", "AGM_14")]
-        cfg.frequency = float(sp["frequency"])
-        cfg.mode_frequency = float(sp.get("mode_frequency", sp["frequency"]))
-        cfg.N_E = len(self.energies)
-        cfg.energies = self.energies.ctypes.data
-        cfg.N_in = len(edges) - 1
-        cfg.energy_edges = edges.ctypes.data
-        cfg.attenuation = None if att is None else att.ctypes.data
-        cfg.N_ch = matrix.shape[0]
-        cfg.matrix = matrix.ctypes.data
-        cfg.N_b = counts.shape[1]
-        cfg.data_phases = dph.ctypes.data
-        cfg.counts = counts.ctypes.data
-        cfg.exposure_time = float(data["exposure_time"])
-        cfg.support = None if support is None else support.ctypes.data
-        ms = model.get("marginal", {})
-        cfg.epsabs = float(ms.get("epsabs", 0.0))
-        cfg.epsrel = float(ms.get("epsrel", 1.0e-8))
-        cfg.epsilon = float(ms.get("epsilon", 1.0e-3))
-        cfg.sigmas = float(ms.get("sigmas", 10.0))
-        cfg.slim = float(ms.get("slim", 20.0))
-        cfg.llzero = float(model.get("llzero", -1.0e90))
-        cfg.max_batch = int(max_batch)
-        cfg.max_sqrt_cells = int(max_sqrt_cells)
         self.model = model
         self.max_batch = int(max_batch)
         self.n_regions = len(hrs)
@@ -136,16 +157,7 @@ class LikelihoodPipeline:
             pass
 
     def _fill(self, k, p):
-        r = self._reqs[k]
-        r.mass, r.radius = float(p["mass"]), float(p["radius"])
-        r.distance, r.cos_inclination = float(p["distance"]), float(p["cos_inclination"])
-        for g, hp in enumerate(p["hot_regions"]):
-            for q, key in enumerate(_REGION_KEYS):
-                v = hp.get(key)
-                if v is None:
-                    v = hp["super_colatitude"] if key == "cede_colatitude" else 0.0
-                r.region[g][q] = float(v)
-            r.sqrt_num_cells[g] = int(hp.get("sqrt_num_cells", 0))
+        fill_request(self._reqs[k], p)
 
     def launch(self, params, stream=None):
         n = len(params)
@@ -167,6 +179,9 @@ class LikelihoodPipeline:
             self.launch(params[s:s + self.max_batch], stream)
             out.append(self.wait())
         return np.concatenate(out)
+
+    def use_graphs(self, on=True):
+        check(lib().xh_pipeline_use_graphs(self._h, int(on)), "xh_pipeline_use_graphs")
 
     def set_timing(self, on=True):
         check(lib().xh_pipeline_set_timing(self._h, int(on)), "xh_pipeline_set_timing")
