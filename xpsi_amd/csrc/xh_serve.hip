@@ -197,7 +197,7 @@ extern "C" int xh_server_poll(xh_server* s, long long* ids, double* lnl, double*
 {
     std::lock_guard<std::mutex> lk(s->mu);
     int n = 0;
-    while (n < max_n && !s->done.empty()) {
+    while (n < max_n && n < (int)s->done.size()) {
         const Done& d = s->done[n];
         ids[n] = d.id;
         lnl[n] = d.lnl;
