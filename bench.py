@@ -5,11 +5,14 @@ uneven resolutions (sqrt_num_cells in {24,32,48,64,96}, num_energies in
 {64,128,256}), one blackbody hot spot, 100-channel Gaussian response, 32
 phase bins; vectors failing the compactness prior are answered without
 evaluation and not counted.  A step evaluates every accepted vector once
-(one launch per energy count).  Beside it: the plain FP64 GPU path one
-request at a time (config-1 request) with p50/p99 latency, and the CPU
-oracle on the host cores.
+(one launch per energy count).  The timed form is the precision form by
+default (--precision mixed); its |dlnL| against an FP64 pass over the same
+vectors is reported beside the rate.  Also reported: the plain FP64 GPU path
+one request at a time (config-1 request) with p50/p99 latency, the same
+config-1 request batched, Poisson-arrival serving, and the CPU oracle on the
+host cores.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu]
 """
 import argparse
 import json
@@ -23,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
@@ -38,6 +42,8 @@ def main():
     ap.add_argument("--fold-ec", type=int, default=0, help="energies per fold block (0 = auto)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--config1-batch", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the likelihood path shards requests across GPUs "
@@ -75,9 +81,12 @@ def main():
     groups = {}
     for d in acc:
         groups.setdefault(d["num_energies"], []).append(d)
-    pls = {}
-    for ne, g in sorted(groups.items()):
-        pls[ne] = LikelihoodPipeline(configs.config3_model(ne), max_batch=len(g))
+
+    def make(prec):
+        return {ne: LikelihoodPipeline(dict(configs.config3_model(ne), precision=prec), max_batch=len(g))
+                for ne, g in sorted(groups.items())}
+
+    pls = make(args.precision)
     res = {}
 
     def step():
@@ -113,40 +122,82 @@ def main():
     lnl_all = np.concatenate([res[ne] for ne in sorted(groups)])
     finite = lnl_all > -1e89
 
-    # the same vectors alone on the single-request pipelines: identical results
+    # accuracy of the measured form against FP64 on the same vectors (untimed)
+    if args.precision != "fp64":
+        for pl in pls.values():
+            pl.close()
+        ref64 = make("fp64")
+        l64 = np.concatenate([ref64[ne].evaluate(groups[ne], stream) for ne in sorted(groups)])
+        for pl in ref64.values():
+            pl.close()
+        both = finite & (l64 > -1e89)
+        d = np.abs(lnl_all[both] - l64[both])
+        dlnl = {"max": float(d.max()), "p50": float(np.percentile(d, 50)),
+                "p99": float(np.percentile(d, 99)), "n": int(both.sum()),
+                "same_llzero_class": bool(np.all(finite == (l64 > -1e89))),
+                "within_1e-3": bool(d.max() < 1e-3)}
+    else:
+        dlnl = {"max": 0.0, "note": "FP64 form measured"}
+
+    # the same vectors alone on single-request pipelines: identical results
     alone_dev = 0.0
     for ne, g in sorted(groups.items()):
-        pl_a = LikelihoodPipeline(configs.config3_model(ne), max_batch=1)
+        pl_a = LikelihoodPipeline(dict(configs.config3_model(ne), precision=args.precision), max_batch=1)
         for k in range(0, len(g), max(1, len(g) // 4)):
             la = pl_a.evaluate([g[k]], stream)[0]
             if la > -1e89 and res[ne][k] > -1e89:
                 alone_dev = max(alone_dev, abs(la - res[ne][k]))
         pl_a.close()
 
+    # batched config-1 requests: the single-request workload, batched
+    nb1 = args.config1_batch
+    plb = LikelihoodPipeline(dict(m1, precision=args.precision), max_batch=nb1)
+    plb.evaluate([p1] * nb1, stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lb = plb.evaluate([p1] * nb1, stream)
+    torch.cuda.synchronize()
+    batched1_rate = nb1 * args.steps / (time.perf_counter() - t0)
+    batched1_dlnl = float(np.max(np.abs(lb - l1)))
+    plb.close()
+
+    prec_desc = {"fp64": "f64",
+                 "mixed": "f32"}[args.precision]
     out = {
         "metric": "batched lnL evals/s + p50/p99 latency vs plain FP64 GPU, at |dlnL|<1e-3",
         "value": value, "unit": "lnL evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": prec_desc, "data": "synthetic",
+        "precision": {"fp64": "FP64 throughout",
+                      "mixed": "FP32 intensity + spline fold (f32 MFMA moment form), split-BF16 "
+                               "3-term response fold, FP64 rays/mesh/phase-bin sums/likelihood"}[
+                          args.precision],
         "config": {"workload": "batched form: %d prior draws (%d accepted), uneven "
                                "sqrt_num_cells {24..96} x num_energies {64,128,256}, BB spot, "
                                "100 ch x 32 bins" % (args.n, n_eval),
                    "global_batch": n_eval, "parallelism": "none (1 GPU)"},
+        "dlnl_vs_fp64": dlnl,
         "plain_fp64_gpu_single": {"evals_per_s": single_rate, "p50_ms": float(np.percentile(lat, 50)),
                                   "p99_ms": float(np.percentile(lat, 99)),
                                   "workload": "config 1 request, batch 1", "lnl": float(l1)},
         "speedup_vs_plain_fp64_gpu": value / single_rate,
+        "batched_config1": {"evals_per_s": batched1_rate, "batch": nb1,
+                            "speedup_vs_plain_fp64_gpu_same_workload": batched1_rate / single_rate,
+                            "max_abs_dlnl_vs_fp64_single": batched1_dlnl},
         "batch_latency_ms": {"p50": dt / args.steps * 1e3},
         "max_abs_dlnl_alone_vs_batched": alone_dev,
         "finite_fraction": float(finite.mean()),
         "stage_ms_per_step": {k: v / args.steps for k, v in stage.items()},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_profile_fold", "evals_per_step": evals,
+        "roofline": {"bound": "mfma", "achieved": achieved,
+                     "peak": FP32_MFMA_PEAK_TFLOPS if args.precision == "mixed" else FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "traffic": None,
+                     "kernel": "k_moment_fold", "evals_per_step": evals,
                      "flop_per_eval": 8},
         "device": dev,
     }
+    out["roofline"]["frac"] = out["roofline"]["achieved"] / out["roofline"]["peak"]
     if not args.no_serving:
         out["serving"] = serving(args, stream_free=True)
     if not args.no_cpu:
@@ -172,7 +223,7 @@ def serving(args, stream_free=True):
     from xpsi_amd.serve import LikelihoodServer
     draws = [d for d in configs.config3_draws()
              if configs.prior_accepts(d["mass"], d["radius"], 300.0) and d["num_energies"] == 128]
-    srv = LikelihoodServer(configs.config3_model(128), n_slots=4, slot_batch=128,
+    srv = LikelihoodServer(dict(configs.config3_model(128), precision=args.precision), n_slots=4, slot_batch=128,
                            hbm_budget_bytes=64e9, use_graphs=True)
     # warm the graphs of common batch sizes
     for d in draws[:256]:
@@ -207,7 +258,7 @@ def serving(args, stream_free=True):
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
     out["stats"] = srv.stats()
     out["workload"] = ("config-3 prior draws with 128 energies, sqrt_num_cells in {24..96}, "
-                       "4 slots x 128, graphs on")
+                       "4 slots x 128, graphs on, %s form" % args.precision)
     srv.close()
     return out
 

@@ -132,6 +132,12 @@ typedef struct xh_model_config {
     double epsabs, epsrel, epsilon, sigmas, llzero, slim;
     int max_batch;            /* requests per xh_pipeline_eval call */
     int max_sqrt_cells;       /* buffer bound on mesh size (>= max_sqrt_num_cells) */
+    int precision;            /* 0: FP64 throughout.  1: precision form - FP32
+                                 intensity and spline fold (matrix cores), split-BF16
+                                 response fold with FP32 accumulation; rays, phase-bin
+                                 accumulation and likelihood quadrature stay FP64 */
+    int fold_kernel;          /* 0 automatic (moment form when its LDS fits), 1 per-point
+                                 fold (FP64 only), 2 moment form */
 } xh_model_config;
 
 /* One parameter vector.  Per region: phase_shift, super_colatitude,
@@ -173,6 +179,12 @@ int xh_pipeline_work(xh_pipeline* p, int n, double* out);
  * 0 = chosen from the leaf count).  Applies to pipelines created afterwards
  * and to xh_integrate calls. */
 int xh_set_fold_energy_chunk(int ec);
+
+/* Fold kernel of xh_integrate: 0 automatic (moment form on the matrix cores
+ * when its LDS fits), 1 per-point evaluation, 2 moment form.  Both give the
+ * reference's sums (the moment form evaluates sign-changing spline pieces
+ * point by point, so the >0 clipping is kept). */
+int xh_set_fold_kernel(int kind);
 
 /* Replay each launch as a hipGraph captured once per batch size (requires a
  * non-default stream; timing mode launches eagerly). */

@@ -139,7 +139,7 @@ int or_integrate(double R, double omega, double r_s, double inclination,
                  const double *energies, size_t N_L, const double *leaves,
                  size_t N_P, const double *phases, int atm_ext,
                  const or_atmosphere *atm, int image_order_limit,
-                 double *flux)
+                 const double *correction, double *flux)
 {
     (void)R; (void)r_s;
     const double hpi = 0.5 * M_PI, two_pi = 2.0 * M_PI;
@@ -258,7 +258,9 @@ int or_integrate(double R, double omega, double r_s, double inclination,
                         for (size_t p = 0; p < N_E; p++) {
                             double Ep = energies[p] / Z;
                             double IE = (atm_ext == 2) ? n4_I(&n4, Ep, ABB, VEC) : bb_I(Ep, VEC);
-                            PROF[p * N_L + kd] = (IE * norm - 0.0) * GEOM;
+                            /* :469-478: the blackbody elsewhere intensity of the cell subtracted */
+                            double cIE = correction ? bb_I(Ep, correction + (i * S_cols + J) * 2) * BB_NORM : 0.0;
+                            PROF[p * N_L + kd] = (IE * norm - cIE) * GEOM;
                         }
                     }
                     if (k == 0) {
@@ -320,7 +322,7 @@ int or_integrate(double R, double omega, double r_s, double inclination,
                             break;
                         }
                         double v = or_akima_eval(PHASE, yp, N_L, ab, ab + N_L, ab + 2 * N_L, x);
-                        if (v > 0.0) flux[p * N_P + k] += area * v;
+                        if (v > 0.0 || correction) flux[p * N_P + k] += area * v;
                     }
                 }
             }

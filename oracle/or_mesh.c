@@ -539,3 +539,64 @@ int or_construct_cell_mesh(int polar, size_t numCell, size_t S, double mass,
     free(phis); free(colTab); free(areaTab); free(sa); free(par); free(colat);
     return 0;
 }
+
+/*
+ * global_mesh.pyx:21-147 (construct_closed_cellMesh): equal-area parallels
+ * over the whole star, S rings of S cells, every cell of area *cellArea.
+ * The northern S/2 ring colatitudes are the area-weighted mean colatitudes
+ * between parallels found by inverting a 1000-point area table (Steffen);
+ * the southern rings mirror them.
+ */
+int or_construct_closed_mesh(size_t S, double R_eq, double zeta, double eps,
+                             int shape, double *colat, double *phis,
+                             double *rr, double *cellArea, double *maxAlpha,
+                             double *cos_gamma, double *effGrav)
+{
+    enum { NTAB = 1000 };
+    const double hpi = 0.5 * M_PI;
+    size_t ncc = S / 2;
+    if (ncc < 2) return 1;
+    double ca = TWO_PI * integrate_area(0.0, hpi, R_eq, eps, zeta, shape, 0) / (double)(S * S);
+    ca *= 2.0;
+    *cellArea = ca;
+    double eta = ca * (double)S / TWO_PI;
+    double *colTab = (double *)malloc(NTAB * sizeof(double));
+    double *areaTab = (double *)malloc(NTAB * sizeof(double));
+    double *sa = (double *)malloc(4 * NTAB * sizeof(double));
+    double *par = (double *)malloc(ncc * sizeof(double));
+    for (size_t i = 0; i < NTAB; i++) colTab[i] = lin(hpi, 0.0, NTAB, i);
+    for (size_t i = 0; i < NTAB; i++)
+        areaTab[i] = integrate_area(colTab[i], hpi, R_eq, eps, zeta, shape, 0);
+    or_steffen_init(areaTab, colTab, NTAB, sa, sa + NTAB, sa + 2 * NTAB, sa + 3 * NTAB);
+    double i_eta = (double)ncc * eta;
+    for (size_t i = 0; i + 1 < ncc; i++) {
+        i_eta -= eta;
+        par[i] = or_steffen_eval(areaTab, NTAB, sa, sa + NTAB, sa + 2 * NTAB,
+                                 sa + 3 * NTAB, i_eta);
+    }
+    for (size_t i = 0; i < ncc; i++) {
+        double lo, hi;
+        if (i == ncc - 1) { lo = par[i - 1]; hi = hpi; }
+        else if (i == 0) { lo = 0.0; hi = par[0]; }
+        else { lo = par[i - 1]; hi = par[i]; }
+        colat[i] = integrate_area(lo, hi, R_eq, eps, zeta, shape, 1) / eta;
+        double mu = cos(colat[i]);
+        double rn = radius_norm(mu, eps, zeta, shape);
+        rr[i] = rn * R_eq;
+        effGrav[i] = or_effective_gravity(mu, R_eq, zeta, eps, shape);
+        double f = f_theta(mu, rn, eps, zeta, shape);
+        cos_gamma[i] = 1.0 / sqrt(1.0 + f * f);
+        maxAlpha[i] = hpi + acos(cos_gamma[i]);
+    }
+    double dphi = TWO_PI / (double)S;
+    for (size_t j = 0; j < S; j++) phis[j] = lin(-M_PI + 0.5 * dphi, M_PI - 0.5 * dphi, S, j);
+    for (size_t i = ncc; i < S; i++) {
+        colat[i] = M_PI - colat[S - i - 1];
+        rr[i] = rr[S - i - 1];
+        cos_gamma[i] = cos_gamma[S - i - 1];
+        effGrav[i] = effGrav[S - i - 1];
+        maxAlpha[i] = maxAlpha[S - i - 1];
+    }
+    free(colTab); free(areaTab); free(sa); free(par);
+    return 0;
+}

@@ -68,7 +68,13 @@ def lib():
         L.or_integrate.restype = i
         L.or_integrate.argtypes = ([d] * 4 + [sz, sz] + [_dp] * 6 + [_ip, sz] +
                                    [_dp] * 5 + [sz, _dp, sz, _dp, sz, _dp, i,
-                                                C.c_void_p, i, _dp])
+                                                C.c_void_p, i, C.c_void_p, _dp])
+        L.or_construct_closed_mesh.restype = i
+        L.or_construct_closed_mesh.argtypes = ([sz, d, d, d, i, _dp, _dp, _dp,
+                                                C.POINTER(C.c_double), _dp, _dp, _dp])
+        L.or_integrate_time_invariant.restype = i
+        L.or_integrate_time_invariant.argtypes = ([d, d, d, sz, d] + [_dp] * 4 +
+                                                  [d, sz] + [_dp] * 4 + [sz, _dp, i, _dp])
         L.or_energy_integrator.argtypes = [sz, sz, _dp, _dp, sz, _dp, _dp]
         L.or_precomputation.argtypes = [sz, sz, _ip, _dp]
         L.or_eval_marginal_likelihood.restype = i
@@ -162,7 +168,9 @@ class _Atm(C.Structure):
 def integrate(omega, r_s, inclination, cellArea, radial, r_s_over_r, theta,
               phi, cellParams, radiates, deflection, cos_alpha, lag,
               maxDeflection, cos_gamma, energies, leaves, phases, atm_ext=1,
-              atmosphere=None, image_order_limit=None, R=0.0):
+              atmosphere=None, image_order_limit=None, R=0.0, correction=None):
+    """integrator_for_azimuthal_invariance.integrate; `correction` is the
+    [S, S, 2] elsewhere parameter array (blackbody) or None."""
     S0, S1 = cellArea.shape
     N_R = deflection.shape[1]
     flux = np.zeros((len(energies), len(phases)))
@@ -186,8 +194,47 @@ def integrate(omega, r_s, inclination, cellArea, radial, r_s_over_r, theta,
                            _c(maxDeflection), _c(cos_gamma), len(energies),
                            _c(energies), len(leaves), _c(leaves), len(phases),
                            _c(phases), int(atm_ext), atm_ptr,
-                           int(image_order_limit or 0), flux)
+                           int(image_order_limit or 0),
+                           None if correction is None else correction.ctypes.data, flux)
     return t, flux
+
+
+def closed_mesh(st, S):
+    """global_mesh.pyx:21-147 for the star `st` with S rings of S cells."""
+    S = int(S)
+    colat, phis, r, maxA, cg, eg = (np.zeros(S) for _ in range(6))
+    ca = C.c_double(0.0)
+    t = lib().or_construct_closed_mesh(S, st["R"], st["zeta"], st["epsilon"], st["shape"],
+                                       colat, phis, r, C.byref(ca), maxA, cg, eg)
+    assert t == 0
+    return dict(S=S, colat=colat, phis=phis, r=r, cellArea=ca.value, maxAlpha=maxA,
+                cos_gamma=cg, effGrav=eg)
+
+
+def elsewhere_embed(st, es):
+    """Elsewhere.embed (xpsi/Elsewhere.py:254-384): closed mesh and rays.
+    `es`: sqrt_num_cells (default 64), num_rays (default 1000)."""
+    m = closed_mesh(st, es.get("sqrt_num_cells", 64))
+    u = np.ascontiguousarray(st["r_s"] / m["r"])
+    t, defl, cosa, lag, maxD = compute_rays(st["r_s"], u, m["maxAlpha"],
+                                            int(es.get("num_rays", 1000)))
+    if t == 1:
+        return None, "RayError"
+    m.update(r_s_over_r=u, deflection=defl, cos_alpha=cosa, maxDeflection=maxD)
+    return m, None
+
+
+def elsewhere_integrate(st, m, energies, logT, image_order_limit=None):
+    """Elsewhere.integrate (xpsi/Elsewhere.py:386-441): spectrum[N_E]."""
+    out = np.zeros(len(energies))
+    t = lib().or_integrate_time_invariant(st["Omega"], st["r_s"], st["i"], m["S"],
+                                          m["cellArea"], m["r"], m["r_s_over_r"],
+                                          m["colat"], m["phis"], float(logT),
+                                          m["deflection"].shape[1], m["deflection"],
+                                          m["cos_alpha"], m["maxDeflection"],
+                                          m["cos_gamma"], len(energies), _c(energies),
+                                          int(image_order_limit or 0), out)
+    return t, out
 
 
 # ------------------------------------------------------------ signal / lnL
@@ -317,13 +364,20 @@ def hot_region_embed(st, hr, mode_frequency):
     return members, None
 
 
-def hot_region_integrate(st, hr, members, energies, atmosphere=None):
+def hot_region_integrate(st, hr, members, energies, atmosphere=None, else_logT=None):
+    """HotRegion.integrate per member; with else_logT, every cell subtracts
+    the blackbody elsewhere intensity (HotRegion.py:1056-1069)."""
     N_L = int(hr.get("num_leaves", 64))
     N_P = int(hr.get("num_phases") or N_L)
     leaves = np.linspace(0.0, 2.0 * math.pi, N_L)
     phases = 2.0 * math.pi * np.linspace(0.0, 1.0, N_P)
     out = []
     for m in members:
+        corr = None
+        if else_logT is not None:
+            corr = np.ones_like(m["params"])
+            corr[..., 0] = else_logT
+            corr[..., 1] = m["effGrav"][:, None]
         t, flux = integrate(st["Omega"], st["r_s"], st["i"], m["cellArea"],
                             m["r"], m["r_s_over_r"], m["theta"], m["phi"],
                             m["params"], m["radiates"], m["deflection"],
@@ -332,7 +386,7 @@ def hot_region_integrate(st, hr, members, energies, atmosphere=None):
                             atm_ext=hr.get("atm_ext", 1),
                             atmosphere=atmosphere,
                             image_order_limit=hr.get("image_order_limit"),
-                            R=st["R"])
+                            R=st["R"], correction=corr)
         if t == 1:
             return None
         out.append(flux)
@@ -356,14 +410,29 @@ def loglikelihood(model, p=None, return_parts=False):
     edges = model["instrument"]["energy_edges"]
     energies = construct_energy_array(model["num_energies"], edges)
     signals, phase_sets, shifts = [], [], []
-    for hr in model["hot_regions"]:
+    es = model.get("elsewhere")
+    spectrum, else_logT = None, None
+    if es is not None:
+        # Photosphere.embed/integrate (xpsi/Photosphere.py:446-593)
+        em, err = elsewhere_embed(st, es)
+        if err is not None:
+            return llzero * 0.5 if not return_parts else (llzero * 0.5, None)
+        else_logT = es["elsewhere_temperature"]
+        t, spectrum = elsewhere_integrate(st, em, energies, else_logT,
+                                          es.get("image_order_limit"))
+        if t == 1:
+            return llzero * 0.5 if not return_parts else (llzero * 0.5, None)
+    for g, hr in enumerate(model["hot_regions"]):
         members, err = hot_region_embed(st, hr, sp["frequency"])
         if err is not None:
             return llzero * 0.5 if not return_parts else (llzero * 0.5, None)
         fluxes = hot_region_integrate(st, hr, members, energies,
-                                      model.get("atmosphere"))
+                                      model.get("atmosphere"), else_logT)
         if fluxes is None:
             return llzero * 0.5 if not return_parts else (llzero * 0.5, None)
+        if spectrum is not None and g == 0:
+            # the time-invariant spectrum joins the first component (Photosphere.py:590-592)
+            fluxes[0] = fluxes[0] + spectrum[:, None]
         integrated = None
         for f in fluxes:
             tmp = energy_integrator(f / st["d_sq"], np.log10(energies),

@@ -87,7 +87,23 @@ int or_integrate(double R, double omega, double r_s, double inclination,
                  size_t N_E, const double *energies, size_t N_L,
                  const double *leaves, size_t N_P, const double *phases,
                  int atm_ext, const or_atmosphere *atm, int image_order_limit,
-                 double *flux);
+                 const double *correction, double *flux);
+
+/* global_mesh.pyx: closed mesh; colat/rr/maxAlpha/cos_gamma/effGrav [S],
+   phis [S], *cellArea the common cell area */
+int or_construct_closed_mesh(size_t S, double R_eq, double zeta, double eps,
+                             int shape, double *colat, double *phis,
+                             double *rr, double *cellArea, double *maxAlpha,
+                             double *cos_gamma, double *effGrav);
+/* integrator_for_time_invariance.pyx (blackbody): flux[N_E] */
+int or_integrate_time_invariant(double omega, double r_s, double inclination,
+                                size_t S, double cellArea, const double *radial,
+                                const double *r_s_over_r, const double *colat,
+                                const double *phis, double logT, size_t N_R,
+                                const double *deflection, const double *cos_alpha,
+                                const double *maxDeflection, const double *cos_gammaArray,
+                                size_t N_E, const double *energies,
+                                int image_order_limit, double *flux);
 
 /* energy_integrator.pyx: out[N_in, N_P] (already transposed) */
 void or_energy_integrator(size_t N_E, size_t N_P, const double *signal,

@@ -45,7 +45,16 @@ def ef_model(examples_fast):
     return examples_fast_model(examples_fast)
 
 
-def test_integrate_dropin(ef_model):
+@pytest.fixture(params=["per-point", "moment"])
+def fold_kernel(request):
+    from xpsi_amd._lib import lib
+    kind = {"per-point": 1, "moment": 2}[request.param]
+    assert lib().xh_set_fold_kernel(kind) == 0
+    yield request.param
+    lib().xh_set_fold_kernel(0)
+
+
+def test_integrate_dropin(ef_model, fold_kernel):
     from xpsi_amd.cellmesh.integrator_for_azimuthal_invariance import integrate
     st, hr, members = oracle_member(ef_model)
     m = members[0]
@@ -57,7 +66,8 @@ def test_integrate_dropin(ef_model):
                          m["cos_alpha"], m["lag"], m["maxDeflection"], m["cos_gamma"], energies,
                          leaves, phases, image_order_limit=3)
     assert t == 0
-    say("integrate drop-in: S=%d N_R=%d N_E=128 N_L=64" % (m["S"], m["deflection"].shape[1]))
+    say("integrate drop-in (%s fold): S=%d N_R=%d N_E=128 N_L=64"
+        % (fold_kernel, m["S"], m["deflection"].shape[1]))
     status, flux = integrate(1, st["R"], st["Omega"], st["r_s"], st["i"], m["cellArea"], m["r"],
                              m["r_s_over_r"], m["theta"], m["phi"], m["params"], m["radiates"],
                              None, m["deflection"].shape[1], m["deflection"], m["cos_alpha"],

@@ -1,0 +1,107 @@
+"""GPU precision form against the FP64 path, and the two FP64 fold kernels
+against each other.
+
+The precision form computes the blackbody intensity, the spline
+coefficients, the interval moments and their products in FP32 (matrix
+cores), folds the response as split-BF16 with FP32 accumulation, and keeps
+the ray tables, the per-parallel phase-bin accumulation and the likelihood
+quadrature in FP64.  Its bar is |lnL - lnL_FP64| < 1e-3; the measured signal
+and lnL errors are printed per configuration.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.batch_cases import sample
+
+pytestmark = pytest.mark.gpu
+
+PREC_ATOL = 1.0e-3
+LNL_ATOL = 1.0e-8
+LLZERO = -1.0e90
+
+
+def say(msg):
+    print("[gpu] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _run(model, params, precision, fold_kernel="auto"):
+    from xpsi_amd.pipeline import LikelihoodPipeline
+    m = dict(model, precision=precision, fold_kernel=fold_kernel)
+    pl = LikelihoodPipeline(m, max_batch=len(params))
+    lnl = pl.evaluate(params)
+    nreg = len(model["hot_regions"])
+    folded = [[pl.fetch("folded", k, r, pl.N_ch * pl.N_P)[:pl.N_ch * pl.N_P] for r in range(nreg)]
+              for k in range(len(params))]
+    pl.close()
+    return np.asarray(lnl), folded
+
+
+def _single(model, name):
+    from xpsi_amd.pipeline import params_from_model
+    lnl_ref = O.loglikelihood(model)
+    p = [params_from_model(model)]
+    say("%s: FP64 moment form" % name)
+    l64, f64 = _run(model, p, "fp64", "moment")
+    say("%s: FP64 per-point fold" % name)
+    lpp, fpp = _run(model, p, "fp64", "per-point")
+    say("%s: precision form" % name)
+    l32, f32 = _run(model, p, "mixed")
+    for r in range(len(f64[0])):
+        e_pp = relerr(f64[0][r], fpp[0][r])
+        e_32 = relerr(f32[0][r], f64[0][r])
+        say("%s region %d: folded rel err moment-vs-per-point %.2e, precision-vs-FP64 %.2e"
+            % (name, r, e_pp, e_32))
+        assert e_pp < 1.0e-10
+    say("%s: lnL oracle %.10f fp64 %.10f per-point %.10f mixed %.10f |dlnL| %.3e"
+        % (name, lnl_ref, l64[0], lpp[0], l32[0], abs(l32[0] - l64[0])))
+    assert abs(l64[0] - lnl_ref) < LNL_ATOL
+    assert abs(lpp[0] - lnl_ref) < LNL_ATOL
+    assert abs(l32[0] - l64[0]) < PREC_ATOL
+
+
+def test_precision_config1():
+    from xpsi_amd import configs
+    _single(configs.config1(), "config1")
+
+
+def test_precision_config2_bb():
+    from xpsi_amd import configs
+    _single(configs.config2("BB"), "config2-BB")
+
+
+def test_precision_config2_num4d():
+    from xpsi_amd import configs
+    _single(configs.config2("Num4D"), "config2-Num4D")
+
+
+def test_precision_batched_sample():
+    from xpsi_amd import configs
+    cs = sample(24, max_cells=96, max_e=256)
+    groups = {}
+    for k, d in enumerate(cs):
+        groups.setdefault(d["num_energies"], []).append(k)
+    worst = 0.0
+    for ne, ks in sorted(groups.items()):
+        model = configs.config3_model(ne)
+        ps = [cs[k] for k in ks]
+        say("batch of %d draws, %d energies" % (len(ks), ne))
+        l64, _ = _run(model, ps, "fp64", "moment")
+        lpp, _ = _run(model, ps, "fp64", "per-point")
+        l32, _ = _run(model, ps, "mixed")
+        for k, a, b, c in zip(ks, l64, lpp, l32):
+            say("  draw %d: fp64 %.10g per-point %.10g mixed %.10g" % (k, a, b, c))
+            assert (a <= 0.1 * LLZERO) == (b <= 0.1 * LLZERO) == (c <= 0.1 * LLZERO)
+            if a > 0.1 * LLZERO:
+                assert abs(a - b) < LNL_ATOL
+                worst = max(worst, abs(c - a))
+                assert abs(c - a) < PREC_ATOL
+    say("batched sample: max |dlnL| precision vs FP64 %.3e" % worst)

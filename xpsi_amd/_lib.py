@@ -32,7 +32,8 @@ class ModelConfig(C.Structure):
                 ("exposure_time", d), ("support", vp),
                 ("epsabs", d), ("epsrel", d), ("epsilon", d), ("sigmas", d),
                 ("llzero", d), ("slim", d),
-                ("max_batch", i32), ("max_sqrt_cells", i32)]
+                ("max_batch", i32), ("max_sqrt_cells", i32),
+                ("precision", i32), ("fold_kernel", i32)]
 
 
 class Request(C.Structure):
@@ -93,6 +94,8 @@ def lib():
     L.xh_pipeline_work.argtypes = [vp, i32, vp]
     L.xh_set_fold_energy_chunk.restype = i32
     L.xh_set_fold_energy_chunk.argtypes = [i32]
+    L.xh_set_fold_kernel.restype = i32
+    L.xh_set_fold_kernel.argtypes = [i32]
     L.xh_pipeline_use_graphs.restype = i32
     L.xh_pipeline_use_graphs.argtypes = [vp, i32]
     L.xh_server_create.restype = vp
@@ -123,5 +126,5 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",
             "xh_pipeline_members_per_request", "xh_pipeline_work",
-            "xh_set_fold_energy_chunk", "xh_pipeline_use_graphs", "xh_server_create",
+            "xh_set_fold_energy_chunk", "xh_set_fold_kernel", "xh_pipeline_use_graphs", "xh_server_create",
             "xh_server_submit", "xh_server_poll", "xh_server_stats", "xh_server_destroy"]

@@ -114,6 +114,55 @@ static hipError_t launch_fold(int EC, dim3 g, size_t lds, hipStream_t s, const M
     return L.atm_ext == 2 ? launch_fold_a<2>(EC, g, lds, s, m, L, F) : launch_fold_a<1>(EC, g, lds, s, m, L, F);
 }
 
+// ----------------------------------------------- moment form (matrix cores)
+static int g_fold_kernel = 0;   // xh_integrate: 0 automatic, 1 per-point, 2 moment form
+static const size_t kMomentLdsMax = 160 * 1024;
+
+extern "C" int xh_set_fold_kernel(int kind)
+{
+    if (kind < 0 || kind > 2) return fail(XH_EINVAL, "fold kernel must be 0 (automatic), 1 (per-point) or 2 (moment)");
+    g_fold_kernel = kind;
+    return XH_OK;
+}
+
+// dynamic LDS of k_moment_fold (mirrors its carve-up; the moment area also
+// holds the 4 x 256 FP64 cross-wave reduction at the end)
+static size_t moment_lds(int N_L, int S_max, size_t rb)
+{
+    const size_t NI = (size_t)N_L - 1;
+    size_t head = (5 * (size_t)N_L + 2 * (size_t)S_max + 8) * 8 + 256 * rb + (size_t)((N_L + 1) & ~1) * 4;
+    return head + (size_t)N_L * 64 * rb + std::max(NI * 64 * rb, (size_t)4 * 256 * 8);
+}
+
+static dim3 moment_grid(int N_E, int N_P, int ring_blocks, int members)
+{
+    return dim3((unsigned)(((N_E + 15) / 16) * ((N_P + 15) / 16)), (unsigned)ring_blocks, (unsigned)members);
+}
+
+template <typename R, int ATM>
+static hipError_t launch_moment_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+{
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)&k_moment_fold<R, ATM>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMomentLdsMax);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    if (g.x == 0) return hipSuccess;     // configuration only
+    hipLaunchKernelGGL((k_moment_fold<R, ATM>), g, dim3(256), lds, s, m, L, F);
+    return hipGetLastError();
+}
+
+static hipError_t launch_moment(bool fp32, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+{
+    if (lds > kMomentLdsMax) return hipErrorInvalidValue;
+    if (fp32) return L.atm_ext == 2 ? launch_moment_t<float, 2>(g, lds, s, m, L, F)
+                                    : launch_moment_t<float, 1>(g, lds, s, m, L, F);
+    return L.atm_ext == 2 ? launch_moment_t<double, 2>(g, lds, s, m, L, F)
+                          : launch_moment_t<double, 1>(g, lds, s, m, L, F);
+}
+
 static hipError_t fill(double* p, size_t n, double v, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
@@ -157,7 +206,10 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
         return fail(XH_EINVAL, "xh_integrate: the 4-D table atmosphere needs its axes and values");
     if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1) return fail(XH_EINVAL, "xh_integrate: bad sizes");
     const int EC = pick_ec(N_L);
-    if (N_P > (256 / EC) * 16) return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
+    const bool moment = g_fold_kernel == 2 || (g_fold_kernel == 0 && moment_lds(N_L, S, 8) <= kMomentLdsMax);
+    if (g_fold_kernel == 2 && moment_lds(N_L, S, 8) > kMomentLdsMax)
+        return fail(XH_EINVAL, "xh_integrate: the moment-form fold does not fit LDS at this leaf count");
+    if (!moment && N_P > (256 / EC) * 16) return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
     if (workspace_bytes < xh_integrate_workspace_bytes(S, N_R, N_L, N_E, N_P))
         return fail(XH_EINVAL, "xh_integrate: workspace too small");
     const int IO = 8;
@@ -192,7 +244,17 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
             FA.atm_N[d] = hN[d];
         }
     }
-    HIPCHK(launch_fold(EC, dim3((N_E + EC - 1) / EC, 1, 1), fold_lds(N_L, EC, S), s, d_m, L, FA));
+    if (moment) {
+        // parallels split over up to 8 blocks per tile; partial sums meet by atomics
+        const int nrb = std::min(S, 8), rb = (S + nrb - 1) / nrb;
+        FA.ring_block = rb;
+        HIPCHK(fill(flux, (size_t)N_E * N_P, 0.0, s));
+        HIPCHK(launch_moment(false, dim3(0, 1, 1), 0, s, d_m, L, FA));
+        HIPCHK(launch_moment(false, moment_grid(N_E, N_P, (S + rb - 1) / rb, 1), moment_lds(N_L, S, 8), s, d_m, L,
+                             FA));
+    } else {
+        HIPCHK(launch_fold(EC, dim3((N_E + EC - 1) / EC, 1, 1), fold_lds(N_L, EC, S), s, d_m, L, FA));
+    }
     Member back{};
     HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -230,7 +292,7 @@ extern "C" int xh_fold(int N_ch, int N_in, int N_P, const double* M, int ld_M, c
 {
     hipStream_t s = (hipStream_t)stream;
     FoldGemmArgs A{M, signal, out, N_ch, N_in, N_P, ld_M};
-    hipLaunchKernelGGL(k_fold_f64, dim3((N_P + 15) / 16, (N_ch + 15) / 16, 1), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(k_fold_f64, dim3((N_P + 31) / 32, (N_ch + 31) / 32, 1), dim3(256), 0, s, A);
     HIPCHK(hipGetLastError());
     return XH_OK;
 }
@@ -287,6 +349,8 @@ struct xh_pipeline {
     xh_model_config cfg;
     Layout L;
     int nreg, mpr, maxb, N_in, N_ch, N_b, EC, table_n;
+    int fold_kind;               // 1 per-point fold, 2 moment form
+    bool fp32;                   // precision form
     int moff[3];                 // first member of each region within a request
     bool timing;
     hipEvent_t ev[16];
@@ -364,15 +428,31 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     p->L = Layout{smax, c->num_rays, c->num_leaves, N_P, c->N_E, IO, c->image_order_limit > 0 ? c->image_order_limit : 0,
                   c->atm_ext};
     p->EC = pick_ec(c->num_leaves);
+    if (c->precision < 0 || c->precision > 1 || c->fold_kernel < 0 || c->fold_kernel > 2) {
+        fail(XH_EINVAL, "pipeline: precision must be 0 or 1 and fold_kernel 0, 1 or 2"); delete p; return nullptr;
+    }
+    p->fp32 = c->precision == 1;
+    {
+        const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8) <= kMomentLdsMax;
+        if ((c->fold_kernel == 2 || p->fp32) && !fits) {
+            fail(XH_EINVAL, "pipeline: the moment-form fold does not fit LDS at this leaf count and mesh bound");
+            delete p; return nullptr;
+        }
+        if (p->fp32 && c->fold_kernel == 1) {
+            fail(XH_EINVAL, "pipeline: the precision form uses the moment-form fold"); delete p; return nullptr;
+        }
+        p->fold_kind = (c->fold_kernel == 1 || !fits) ? 1 : 2;
+    }
     {
         // configure the fold kernel's LDS limit now, never inside a capture
         Layout L0{};
         L0.N_P = c->num_phases > 0 ? c->num_phases : c->num_leaves;
         L0.atm_ext = c->atm_ext;
         FoldArgs F0{};
-        PHIP(launch_fold(p->EC, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
+        if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
+        else PHIP(launch_fold(p->EC, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
     }
-    if (N_P > (256 / p->EC) * 16) { fail(XH_EINVAL, "too many phases for the leaf count"); delete p; return nullptr; }
+    if (p->fold_kind == 1 && N_P > (256 / p->EC) * 16) { fail(XH_EINVAL, "too many phases for the leaf count"); delete p; return nullptr; }
     if (c->atm_ext != 1 && c->atm_ext != 2) { fail(XH_EINVAL, "pipeline: atmosphere must be 1 (blackbody) or 2 (4-D table)"); delete p; return nullptr; }
     if (c->atm_ext == 2) {
         for (int d = 0; d < 4; d++)
@@ -612,15 +692,22 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
                 p->d_atm_axes, p->d_atm_I, {p->atm_N[0], p->atm_N[1], p->atm_N[2], p->atm_N[3]}, p->d_energies,
                 p->d_phases, p->d_flux, rb};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
-    HIPCHK(launch_fold(p->EC, dim3((N_E + p->EC - 1) / p->EC, (S + rb - 1) / rb, nm), fold_lds(L.N_L, p->EC, S), s,
-                       p->d_members, L, FA));
+    if (p->fold_kind == 2)
+        HIPCHK(launch_moment(p->fp32, moment_grid(N_E, N_P, (S + rb - 1) / rb, nm),
+                             moment_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8), s, p->d_members, L, FA));
+    else
+        HIPCHK(launch_fold(p->EC, dim3((N_E + p->EC - 1) / p->EC, (S + rb - 1) / rb, nm), fold_lds(L.N_L, p->EC, S),
+                           s, p->d_members, L, FA));
     EnergyArgs EA{p->d_flux, p->d_logE, p->d_logEdges, p->cfg.attenuation ? p->d_atten : nullptr,
                   p->d_member_index, p->d_integrated, N_E, N_P, p->N_in + 1, 2};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[4], s));
     hipLaunchKernelGGL(k_energy_integrate, dim3((p->N_in * N_P + 255) / 256, ng), dim3(256), 0, s, p->d_members, EA);
     HIPCHK(hipGetLastError());
     FoldGemmArgs GA{p->d_M, p->d_integrated, p->d_folded, p->N_ch, p->N_in, N_P, p->N_in};
-    hipLaunchKernelGGL(k_fold_f64, dim3((N_P + 15) / 16, (p->N_ch + 15) / 16, ng), dim3(256), 0, s, GA);
+    if (p->fp32)
+        hipLaunchKernelGGL(k_fold_bf16x3, dim3((N_P + 31) / 32, (p->N_ch + 31) / 32, ng), dim3(256), 0, s, GA);
+    else
+        hipLaunchKernelGGL(k_fold_f64, dim3((N_P + 31) / 32, (p->N_ch + 31) / 32, ng), dim3(256), 0, s, GA);
     HIPCHK(hipGetLastError());
     MarginalArgs MA{};
     MA.signals = p->d_folded; MA.comp_phases = p->d_comp_phases; MA.shifts = p->d_shifts;

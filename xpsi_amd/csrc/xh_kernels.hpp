@@ -51,6 +51,7 @@ __global__ void k_rays_fix(Member* members, Layout L, const double* rr, double* 
                            double* defl, double* lag, double* maxDefl);
 __global__ void k_energy_integrate(const Member* members, EnergyArgs A);
 __global__ void k_fold_f64(FoldGemmArgs A);
+__global__ void k_fold_bf16x3(FoldGemmArgs A);
 __global__ void k_marginal(MarginalArgs A);
 __global__ void k_lnl_reduce(MarginalArgs A, const Member* members, int members_per_request,
                              double* lnl);
@@ -93,6 +94,8 @@ struct FoldArgs {
 __global__ void k_geometry(Member* members, Layout L, RingArgs A);
 template <int EC, int MAXK, int ATM>
 __global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
+template <typename R, int ATM>
+__global__ void k_moment_fold(const Member* members, Layout L, FoldArgs F);
 __global__ void k_fill(double* p, size_t n, double v);
 __global__ void k_work_count(const Member* members, Layout L, const double* cellArea, const int* valid,
                              double* work);

@@ -1,0 +1,330 @@
+// Moment form of the cell x phase x energy fold, on the matrix cores.
+//
+// Reference behaviour: the energy x cell x phase spline loop of
+// xpsi/cellmesh/integrator_for_azimuthal_invariance.pyx:565-596 and the
+// blackbody intensity of xpsi/surface_radiation_field/hot_BB.pyx:54-98.
+//
+// For one parallel and image order the reference evaluates, for every energy
+// k, the periodic Akima interpolant s_k of the profile at x = phase_p + phi_j
+// for every radiating cell j and model phase p, keeps values > 0 and adds
+// cellArea_j * s_k(x).  The interval i(j, p) holding x and the offset
+// t = x - PHASE[i] do not depend on the energy.  Where s_k is positive over
+// the whole interval the clipping is inactive and
+//
+//     sum_j A_j s_k(x_jp) = sum_i sum_m coef_k[i][m] * MOM[i][m][p],
+//     MOM[i][m][p]       = sum over cells j with i(j, p) = i of A_j t^m,
+//
+// the product of an [energy x (interval, power)] matrix with an
+// [(interval, power) x phase] matrix: one 16x16x4 MFMA per interval and
+// 16 x 16 tile.  Intervals where s_k changes sign are evaluated point by point
+// with the clipping, exactly as the reference does; intervals where s_k <= 0
+// throughout contribute nothing.
+//
+// One block of four waves owns 16 energies x 16 phases of one member and
+// walks the parallels of its ring block.  Per parallel and image: profile
+// (intensity x geometry) into LDS, secant slopes, Akima coefficients with a
+// sign class per (interval, energy), moments from the (cell, phase) queries
+// by LDS atomics, direct evaluation of sign-changing intervals, then the MFMA
+// sweep, intervals dealt round-robin to the waves.  Each parallel's tile is
+// added into FP64 accumulators, so the float form sums parallels in FP64.
+//
+// R = double: FP64 throughout (v_mfma_f64_16x16x4f64).
+// R = float: the precision form.  Blackbody intensity, spline coefficients,
+// moments and products in FP32 (v_mfma_f32_16x16x4f32); the phase offsets t
+// are formed in FP64 and the per-parallel partial sums are accumulated in
+// FP64.
+#include "xh_common.hpp"
+#include "xh_types.hpp"
+#include "xh_kernels.hpp"
+#include "xh_atm.hpp"
+
+namespace xh {
+
+template <typename R> struct Mfma;
+
+// lane l: A[l % 16][l / 16], B[l / 16][l % 16]; result element r of lane l is
+// D[4 r + l / 16][l % 16]
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * r + (l >> 4); }
+};
+
+// lane l: A[l % 16][l / 16], B[l / 16][l % 16]; result element r of lane l is
+// D[4 (l / 16) + r][l % 16]
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <typename R> struct Eps;
+template <> struct Eps<double> { static constexpr double v = DBL_EPSILON; };
+template <> struct Eps<float> { static constexpr float v = FLT_EPSILON; };
+
+// Akima (1970) piece from the five surrounding secant slopes, with the
+// degenerate-weight branches of the reference's interpolation library
+template <typename R>
+__device__ __forceinline__ void akima_piece_r(R mm2, R mm1, R m0, R mp1, R mp2, R h, R& b, R& c, R& d)
+{
+    const R one = (R)1, two = (R)2, three = (R)3;
+    R w = fabs(mp1 - m0) + fabs(mm1 - mm2);
+    if (w == (R)0) { b = m0; c = (R)0; d = (R)0; return; }
+    R wn = fabs(mp2 - mp1) + fabs(m0 - mm1);
+    R al = fabs(mm1 - mm2) / w;
+    R tn;
+    if (wn == (R)0) tn = m0;
+    else {
+        R aln = fabs(m0 - mm1) / wn;
+        tn = (one - aln) * m0 + aln * mp1;
+    }
+    b = (one - al) * mm1 + al * m0;
+    c = (three * m0 - two * b - tn) / h;
+    d = (b + tn - two * m0) / (h * h);
+}
+
+// Sign class of f(t) = a + t (b + t (c + t d)) on [0, h]:
+// 1 = positive at every point by more than the evaluation error,
+// 0 = identically zero or negative everywhere by more than that error,
+// 2 = otherwise (evaluated point by point).
+template <typename R>
+__device__ __forceinline__ int cubic_class(R a, R b, R c, R d, R h)
+{
+    if (a == (R)0 && b == (R)0 && c == (R)0 && d == (R)0) return 0;
+    const R scale = fabs(a) + h * (fabs(b) + h * (fabs(c) + h * fabs(d)));
+    const R tol = (R)64 * Eps<R>::v * scale;
+    auto f = [&](R t) { return a + t * (b + t * (c + t * d)); };
+    R fh = f(h);
+    R lo = fmin(a, fh), hi = fmax(a, fh);
+    // stationary points: 3 d t^2 + 2 c t + b = 0 (numerically stable roots)
+    R t1 = (R)-1, t2 = (R)-1;
+    if (d != (R)0) {
+        R disc = c * c - (R)3 * b * d;
+        if (disc >= (R)0) {
+            R q = -(c + copysign(sqrt(disc), c));
+            if (q != (R)0) { t1 = q / ((R)3 * d); t2 = b / q; }
+            else t1 = (R)0;
+        }
+    } else if (c != (R)0) {
+        t1 = -b / ((R)2 * c);
+    }
+    if (t1 > (R)0 && t1 < h) { R v = f(t1); lo = fmin(lo, v); hi = fmax(hi, v); }
+    if (t2 > (R)0 && t2 < h) { R v = f(t2); lo = fmin(lo, v); hi = fmax(hi, v); }
+    if (lo > tol) return 1;
+    if (hi < -tol) return 0;
+    return 2;
+}
+
+template <typename R, int ATM>
+__global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
+{
+    extern __shared__ __align__(16) double smd[];
+    const int mi = blockIdx.z, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const Member m = members[mi];
+    if (!m.active || m.status) return;
+    const int S = m.S, N_L = L.N_L, N_P = L.N_P, N_E = L.N_E, NI = N_L - 1;
+    const int n_ec = (N_E + 15) >> 4;
+    const int e0 = (blockIdx.x % n_ec) * 16, p0 = (blockIdx.x / n_ec) * 16;
+    const int r0 = blockIdx.y * F.ring_block;
+    if (r0 >= S) return;
+    const int r1 = min(S, r0 + F.ring_block);
+
+    double* PH = smd;                       // [N_L] leaf phases
+    double* Zs = PH + N_L;                  // [N_L] redshift factor
+    double* As = Zs + N_L;                  // [N_L] mu * eta
+    double* Gs = As + N_L;                  // [N_L] geometric factor
+    double* PV = Gs + N_L;                  // [N_L] previous queried leaf
+    double* cphi = PV + N_L;                // [S_max]
+    double* carea = cphi + L.S_max;         // [S_max], NaN = not radiating
+    double* ST = carea + L.S_max;           // [8] previous table query
+    R* DIR = (R*)(ST + 8);                  // [16][16] sign-changing pieces, direct sums
+    unsigned* MIX = (unsigned*)(DIR + 256); // [N_L (even)] energy bits of sign-changing pieces
+    R* CO = (R*)(MIX + ((N_L + 1) & ~1));   // [N_L][4][16] coefficient m of energy k
+    R* MOM = CO + N_L * 64;                 // [NI][4][16] moment m at phase p
+    R* SL = MOM;                            // [NI][16] secant slopes (before the moments)
+
+    DIR[tid] = (R)0;
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    const size_t cbase = (size_t)mi * L.S_max * L.S_max;
+    const int p = tid & 15, part = tid >> 4;
+    const int pg = p0 + p;
+    const double ph = pg < N_P ? F.phases[pg] : 0.0;
+    const int ja = (part * S) >> 4, jb = ((part + 1) * S) >> 4;
+
+    for (int i = r0; i < r1; i++) {
+        const size_t ro = (size_t)mi * L.S_max + i;
+        __syncthreads();   // the previous parallel is done with the cell arrays
+        for (int j = tid; j < S; j += 256) {
+            size_t c = cbase + (size_t)i * S + j;
+            double a = F.cellArea[c];
+            bool rad = F.radiates ? (F.radiates[c] == 1) : (a > 0.0);
+            carea[j] = rad ? a : __builtin_nan("");
+            cphi[j] = F.phi[(size_t)mi * L.S_max * (F.phi_row_stride ? L.S_max : 1) +
+                            (size_t)i * F.phi_row_stride + j];
+        }
+        const double kT = K_B_OVER_KEV * pow(10.0, F.logT[ro]);
+        for (int I = 0; I < L.IO_max; I++) {
+            const size_t vix = ro * L.IO_max + I;
+            if (!F.valid[vix]) break;
+            __syncthreads();   // the previous image's MFMA sweep is done with CO/MOM
+            const double* g = F.geo + vix * 5 * N_L;
+            for (int q = tid; q < N_L; q += 256) {
+                PH[q] = g[q]; Zs[q] = g[N_L + q]; As[q] = g[2 * N_L + q]; Gs[q] = g[3 * N_L + q];
+                PV[q] = g[4 * N_L + q];
+                MIX[q] = 0u;
+            }
+            if (ATM == 2 && tid == 0) prev_query_state(F, L, mi, i, I, ST);
+            __syncthreads();
+            // profile: specific intensity x geometry, zero where the leaf is hidden
+            for (int q = tid; q < N_L * 16; q += 256) {
+                const int mm = q >> 4, k = q & 15, e = e0 + k;
+                R v = (R)0;
+                const double geom = Gs[mm];
+                if (geom != 0.0 && e < N_E && PV[mm] > -1.5) {
+                    if (ATM == 2) {
+                        double Ep = F.energies[e] / Zs[mm];
+                        double vec[4], prev[4];
+                        int has_prev = 1;
+                        vec[0] = F.logT[ro]; vec[1] = F.logg[ro]; vec[2] = As[mm];
+                        vec[3] = log10(Ep / kT);
+                        if (e > 0) {
+                            prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = vec[2];
+                            prev[3] = log10((F.energies[e - 1] / Zs[mm]) / kT);
+                        } else if (PV[mm] > -0.5) {
+                            int kd = (int)PV[mm];
+                            prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = As[kd];
+                            prev[3] = log10((F.energies[N_E - 1] / Zs[kd]) / kT);
+                        } else if (ST[0] > 0.5) {
+                            prev[0] = ST[1]; prev[1] = ST[2]; prev[2] = ST[3]; prev[3] = ST[4];
+                        } else has_prev = 0;
+                        v = (R)((num4d_I(F, vec, prev, has_prev) * NUM4D_NORM - 0.0) * geom);
+                    } else if (sizeof(R) == 8) {
+                        double Ep = F.energies[e] / Zs[mm];
+                        double I_E = Ep * Ep * Ep / (exp(Ep / kT) - 1.0);
+                        v = (R)((I_E * BB_NORM - 0.0) * geom);
+                    } else {
+                        const float Ep = (float)F.energies[e] / (float)Zs[mm];
+                        const float I_E = Ep * Ep * Ep / (expf(Ep / (float)kT) - 1.0f);
+                        v = (R)(I_E * (float)BB_NORM * (float)geom);
+                    }
+                }
+                CO[mm * 64 + k] = v;
+            }
+            __syncthreads();
+            if (tid < 16 && PV[N_L - 1] < -1.5) CO[(N_L - 1) * 64 + tid] = CO[tid];
+            __syncthreads();
+            // secant slopes
+            for (int q = tid; q < NI * 16; q += 256) {
+                const int mm = q >> 4, k = q & 15;
+                SL[q] = (CO[(mm + 1) * 64 + k] - CO[mm * 64 + k]) / (R)(PH[mm + 1] - PH[mm]);
+            }
+            __syncthreads();
+            // periodic Akima coefficients and the sign class of each piece
+            for (int q = tid; q < NI * 16; q += 256) {
+                const int mm = q >> 4, k = q & 15;
+                auto sl = [&](int kk) {
+                    kk = kk < 0 ? kk + NI : (kk >= NI ? kk - NI : kk);
+                    return SL[kk * 16 + k];
+                };
+                const R h = (R)(PH[mm + 1] - PH[mm]);
+                R a = CO[mm * 64 + k], b, c, d;
+                akima_piece_r<R>(sl(mm - 2), sl(mm - 1), sl(mm), sl(mm + 1), sl(mm + 2), h, b, c, d);
+                const int cls = cubic_class<R>(a, b, c, d, h);
+                if (cls == 0) { a = (R)0; b = (R)0; c = (R)0; d = (R)0; }
+                else if (cls == 2) atomicOr(&MIX[mm], 1u << k);
+                CO[mm * 64 + k] = a;
+                CO[mm * 64 + 16 + k] = b;
+                CO[mm * 64 + 32 + k] = c;
+                CO[mm * 64 + 48 + k] = d;
+            }
+            __syncthreads();
+            for (int q = tid; q < NI * 64; q += 256) MOM[q] = (R)0;
+            __syncthreads();
+            // moments of the (cell, phase) queries; sign-changing pieces directly
+            if (pg < N_P) {
+                const double lo = PH[0], hi = PH[N_L - 1];
+                int id = -1, cur = -1;
+                R m0 = (R)0, m1 = (R)0, m2 = (R)0, m3 = (R)0;
+                for (int j = ja; j < jb; j++) {
+                    const double area = carea[j];
+                    if (isnan(area)) continue;
+                    double x = ph + cphi[j];
+                    int guard = 0;
+                    if (x > hi) { while (x > hi && guard++ < 64) x -= TWO_PI; }
+                    else if (x < lo) { while (x < lo && guard++ < 64) x += TWO_PI; }
+                    if (id < 0 || x < PH[id]) id = interval_of(PH, N_L, x);
+                    else while (id < NI - 1 && PH[id + 1] <= x) id++;
+                    const R t = (R)(x - PH[id]), ar = (R)area;
+                    if (id != cur) {
+                        if (cur >= 0) {
+                            atomicAdd(&MOM[cur * 64 + p], m0);
+                            atomicAdd(&MOM[cur * 64 + 16 + p], m1);
+                            atomicAdd(&MOM[cur * 64 + 32 + p], m2);
+                            atomicAdd(&MOM[cur * 64 + 48 + p], m3);
+                        }
+                        cur = id;
+                        m0 = (R)0; m1 = (R)0; m2 = (R)0; m3 = (R)0;
+                    }
+                    R at = ar;
+                    m0 += at; at *= t; m1 += at; at *= t; m2 += at; at *= t; m3 += at;
+                    unsigned mx = MIX[id];
+                    while (mx) {
+                        const int k = __ffs(mx) - 1;
+                        mx &= mx - 1u;
+                        const R* co = CO + id * 64 + k;
+                        const R v = co[0] + t * (co[16] + t * (co[32] + t * co[48]));
+                        if (v > (R)0) atomicAdd(&DIR[k * 16 + p], ar * v);
+                    }
+                }
+                if (cur >= 0) {
+                    atomicAdd(&MOM[cur * 64 + p], m0);
+                    atomicAdd(&MOM[cur * 64 + 16 + p], m1);
+                    atomicAdd(&MOM[cur * 64 + 32 + p], m2);
+                    atomicAdd(&MOM[cur * 64 + 48 + p], m3);
+                }
+            }
+            __syncthreads();
+            // sign-changing pieces are done: drop them from the product
+            for (int q = tid; q < NI * 16; q += 256) {
+                const int mm = q >> 4, k = q & 15;
+                if ((MIX[mm] >> k) & 1u) {
+                    CO[mm * 64 + k] = (R)0;
+                    CO[mm * 64 + 16 + k] = (R)0;
+                    CO[mm * 64 + 32 + k] = (R)0;
+                    CO[mm * 64 + 48 + k] = (R)0;
+                }
+            }
+            __syncthreads();
+            typename Mfma<R>::v4 acc = {(R)0, (R)0, (R)0, (R)0};
+            for (int idx = wave; idx < NI; idx += 4)
+                acc = Mfma<R>::step(CO[idx * 64 + lane], MOM[idx * 64 + lane], acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) tot[r] += (double)acc[r];
+        }
+    }
+    __syncthreads();
+    double* RED = (double*)MOM;   // [4 waves][16][16]; the host sizes LDS to hold it
+#pragma unroll
+    for (int r = 0; r < 4; r++) RED[wave * 256 + Mfma<R>::row(lane, r) * 16 + (lane & 15)] = tot[r];
+    __syncthreads();
+    const int k = tid >> 4, e = e0 + k;
+    if (e >= N_E || pg >= N_P) return;
+    const double s = RED[tid] + RED[256 + tid] + RED[512 + tid] + RED[768 + tid] + (double)DIR[tid];
+    const double E = F.energies[e];
+    double* fo = F.flux + (size_t)mi * N_E * N_P + (size_t)e * N_P + pg;
+    if (gridDim.y == 1) *fo = s / (E * KEV);
+    else atomicAdd(fo, s * (1.0 / (E * KEV)));
+}
+
+template __global__ void k_moment_fold<double, 1>(const Member*, Layout, FoldArgs);
+template __global__ void k_moment_fold<double, 2>(const Member*, Layout, FoldArgs);
+template __global__ void k_moment_fold<float, 1>(const Member*, Layout, FoldArgs);
+template __global__ void k_moment_fold<float, 2>(const Member*, Layout, FoldArgs);
+
+}  // namespace xh

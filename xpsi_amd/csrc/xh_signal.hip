@@ -70,34 +70,79 @@ __global__ void __launch_bounds__(256) k_energy_integrate(const Member* __restri
     A.out[(size_t)reg * N_in * N_P + q] = total;
 }
 
-// folded[region][c][k] = sum_j M[c][j] * in[region][j][k]
+// folded[region][c][k] = sum_j M[c][j] * in[region][j][k] on the matrix
+// cores (xpsi/Instrument.py:163-197).  A block of four waves owns a 32-channel
+// x 32-phase tile, one 16 x 16 sub-tile per wave.  The response matrix is
+// shared by every region of the batch and stays in L2.
+//
+// FP64: v_mfma_f64_16x16x4f64, lane l holds M[c0 + l%16][j + l/16] and
+// in[j + l/16][k0 + l%16]; result element r is row 4 r + l/16.
 __global__ void __launch_bounds__(256) k_fold_f64(FoldGemmArgs A)
 {
-    __shared__ double Ms[16][65];
-    __shared__ double Xs[64][17];
-    const int reg = blockIdx.z;
-    const int c0 = blockIdx.y * 16, k0 = blockIdx.x * 16;
-    const int tc = threadIdx.x / 16, tk = threadIdx.x % 16;
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const int reg = blockIdx.z, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = blockIdx.y * 32 + (wave >> 1) * 16, k0 = blockIdx.x * 32 + (wave & 1) * 16;
+    const int c = c0 + (lane & 15), k = k0 + (lane & 15), jl = lane >> 4;
     const double* X = A.in + (size_t)reg * A.N_in * A.N_P;
-    double acc = 0.0;
-    for (int j0 = 0; j0 < A.N_in; j0 += 64) {
-        for (int q = threadIdx.x; q < 16 * 64; q += 256) {
-            int r = q / 64, cidx = q % 64;
-            int c = c0 + r, j = j0 + cidx;
-            Ms[r][cidx] = (c < A.N_ch && j < A.N_in) ? A.M[(size_t)c * A.ld_M + j] : 0.0;
-        }
-        for (int q = threadIdx.x; q < 64 * 16; q += 256) {
-            int r = q / 16, cidx = q % 16;
-            int j = j0 + r, k = k0 + cidx;
-            Xs[r][cidx] = (j < A.N_in && k < A.N_P) ? X[(size_t)j * A.N_P + k] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 8
-        for (int jj = 0; jj < 64; jj++) acc += Ms[tc][jj] * Xs[jj][tk];
-        __syncthreads();
+    const bool c_ok = c < A.N_ch, k_ok = k < A.N_P;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int j0 = 0; j0 < A.N_in; j0 += 4) {
+        const int j = j0 + jl;
+        const bool j_ok = j < A.N_in;
+        const double a = (c_ok && j_ok) ? A.M[(size_t)c * A.ld_M + j] : 0.0;
+        const double b = (k_ok && j_ok) ? X[(size_t)j * A.N_P + k] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
-    int c = c0 + tc, k = k0 + tk;
-    if (c < A.N_ch && k < A.N_P) A.out[((size_t)reg * A.N_ch + c) * A.N_P + k] = acc;
+    if (!k_ok) return;
+    double* out = A.out + (size_t)reg * A.N_ch * A.N_P;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int cr = c0 + 4 * r + jl;
+        if (cr < A.N_ch) out[(size_t)cr * A.N_P + k] = acc[r];
+    }
+}
+
+// Precision form: split-BF16, three products.  Each operand x (rounded to
+// FP32) is split into hi = bf16(x) and lo = bf16(x - hi); the product is
+// lo_M hi_X + hi_M lo_X + hi_M hi_X, accumulated in FP32
+// (v_mfma_f32_16x16x32_bf16: lane l holds M[c0 + l%16][j + 8 (l/16) + v] and
+// in[j + 8 (l/16) + v][k0 + l%16], v = 0..7; result element r is row
+// 4 (l/16) + r).  The dropped lo_M lo_X term is below 2^-16 relative.
+__global__ void __launch_bounds__(256) k_fold_bf16x3(FoldGemmArgs A)
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+    const int reg = blockIdx.z, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = blockIdx.y * 32 + (wave >> 1) * 16, k0 = blockIdx.x * 32 + (wave & 1) * 16;
+    const int c = c0 + (lane & 15), k = k0 + (lane & 15), jl = 8 * (lane >> 4);
+    const double* X = A.in + (size_t)reg * A.N_in * A.N_P;
+    const bool c_ok = c < A.N_ch, k_ok = k < A.N_P;
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int j0 = 0; j0 < A.N_in; j0 += 32) {
+        bf8 ah, al, bh, bl;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            const int j = j0 + jl + v;
+            const bool j_ok = j < A.N_in;
+            const float a = (c_ok && j_ok) ? (float)A.M[(size_t)c * A.ld_M + j] : 0.f;
+            const float b = (k_ok && j_ok) ? (float)X[(size_t)j * A.N_P + k] : 0.f;
+            const __bf16 a1 = (__bf16)a, b1 = (__bf16)b;
+            ah[v] = a1;
+            al[v] = (__bf16)(a - (float)a1);
+            bh[v] = b1;
+            bl[v] = (__bf16)(b - (float)b1);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+    }
+    if (!k_ok) return;
+    double* out = A.out + (size_t)reg * A.N_ch * A.N_P;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int cr = c0 + 4 * (lane >> 4) + r;
+        if (cr < A.N_ch) out[(size_t)cr * A.N_P + k] = (double)acc[r];
+    }
 }
 
 struct MargFn {

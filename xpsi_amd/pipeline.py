@@ -17,6 +17,8 @@ Model description (dict):
   data        counts [N_ch, N_b], phases [N_b + 1], exposure_time
   num_energies, support [N_ch, 2] or None, marginal settings, llzero,
   interstellar: callable(mid energies) -> attenuation factors, or None
+  precision   'fp64' (default) or 'mixed' (see PRECISIONS)
+  fold_kernel 'auto' (default), 'moment' or 'per-point' (see FOLD_KERNELS)
 A parameter vector is a dict: mass, radius, distance, cos_inclination and
 `hot_regions`: list of dicts phase_shift, super_colatitude, super_radius,
 super_temperature, cede_colatitude, cede_radius, cede_azimuth,
@@ -51,6 +53,16 @@ def construct_energy_array(num_energies, edges):
     L, U = edges[0], edges[-1]
     e = np.logspace(np.log10(L), np.log10(U), int(math.ceil(num_energies)), base=10.0)[::-1]
     return np.ascontiguousarray(e[::-1])
+
+
+# "fp64": FP64 throughout.  "mixed": the precision form - FP32 intensity and
+# spline fold on the matrix cores, split-BF16 response fold with FP32
+# accumulation; rays, phase-bin accumulation and the likelihood stay FP64.
+PRECISIONS = {"fp64": 0, "mixed": 1}
+# "moment": spline fold as interval-moment products on the matrix cores;
+# "per-point": one spline evaluation per (cell, phase, energy); "auto": the
+# moment form when its LDS fits.
+FOLD_KERNELS = {"auto": 0, "per-point": 1, "moment": 2}
 
 
 def model_config(model, max_batch=1, max_sqrt_cells=0):
@@ -128,6 +140,14 @@ def model_config(model, max_batch=1, max_sqrt_cells=0):
     cfg.llzero = float(model.get("llzero", -1.0e90))
     cfg.max_batch = int(max_batch)
     cfg.max_sqrt_cells = int(max_sqrt_cells)
+    prec = model.get("precision", "fp64")
+    if prec not in PRECISIONS:
+        raise XHError("precision must be one of %s" % sorted(PRECISIONS))
+    cfg.precision = PRECISIONS[prec]
+    fk = model.get("fold_kernel", "auto")
+    if fk not in FOLD_KERNELS:
+        raise XHError("fold_kernel must be one of %s" % sorted(FOLD_KERNELS))
+    cfg.fold_kernel = FOLD_KERNELS[fk]
     return cfg, energies, keep
 
 
