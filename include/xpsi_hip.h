@@ -138,6 +138,14 @@ typedef struct xh_model_config {
                                  accumulation and likelihood quadrature stay FP64 */
     int fold_kernel;          /* 0 automatic (moment form when its LDS fits), 1 per-point
                                  fold (FP64 only), 2 moment form */
+    /* time-invariant blackbody emission from the rest of the surface
+       (xpsi/Elsewhere.py): a closed mesh of else_sqrt_num_cells^2 cells and
+       else_num_rays rays per ring; its spectrum joins the first hot region's
+       first component and every hot cell subtracts it (no clipping) */
+    int elsewhere;            /* 0 none, 1 present */
+    int else_sqrt_num_cells;  /* even, >= 10 (reference default 64) */
+    int else_num_rays;        /* reference default 1000 */
+    int else_image_order_limit; /* 0 = inferred from the deflection */
 } xh_model_config;
 
 /* One parameter vector.  Per region: phase_shift, super_colatitude,
@@ -147,6 +155,7 @@ typedef struct xh_request {
     double mass, radius, distance, cos_inclination;
     double region[2][8];
     int sqrt_num_cells[2];    /* 0 = the configured value */
+    double elsewhere_temperature; /* log10 K, used when the model has elsewhere */
 } xh_request;
 
 typedef struct xh_pipeline xh_pipeline;

@@ -1,5 +1,7 @@
 """Writes synthetic data fixtures for the benchmark workloads with the oracle.
 
+python tests/golden/make_configs.py [config1] [config2] [config5] [config5_full]
+
 config1.npz: counts[100, 32] ~ Poisson(exposure * phase-binned star rate + 5)
 with numpy.random.default_rng(42); the rate is the oracle's folded signal of
 xpsi_amd.configs.config1 integrated over each phase bin (the expected-count
@@ -61,10 +63,31 @@ def make_config2(atm):
     print("config2 %s: total counts %.4g" % (atm, counts.sum()))
 
 
+def make_config5(scale):
+    """config5_<scale>.npz: counts from both hot regions (with the elsewhere
+    spectrum and interstellar absorption), default_rng(42)."""
+    model = configs.config5(scale)
+    n_ch, n_b = model["data"]["counts"].shape
+    lnl, parts = O.loglikelihood(model, return_parts=True)
+    rate = np.zeros((n_ch, n_b))
+    n_p = model["hot_regions"][0]["num_leaves"]
+    for sig, h in zip(parts["signals"], model["hot_regions"]):
+        rate += binned_rate(sig, np.linspace(0.0, 1.0, n_p), model["data"]["phases"], h["phase_shift"])
+    expected = model["data"]["exposure_time"] * rate + 5.0
+    rng = np.random.default_rng(42)
+    counts = rng.poisson(expected).astype(np.float64)
+    np.savez_compressed(os.path.join(HERE, "config5_%s.npz" % scale), counts=counts, expected=expected)
+    print("config5 %s: total counts %.4g" % (scale, counts.sum()))
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["config1", "config2"]
+    which = sys.argv[1:] or ["config1", "config2", "config5"]
     if "config1" in which:
         make_config1()
     if "config2" in which:
         make_config2("BB")
         make_config2("Num4D")
+    if "config5" in which:
+        make_config5("test")
+    if "config5_full" in which:
+        make_config5("full")

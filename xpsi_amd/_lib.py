@@ -33,12 +33,15 @@ class ModelConfig(C.Structure):
                 ("epsabs", d), ("epsrel", d), ("epsilon", d), ("sigmas", d),
                 ("llzero", d), ("slim", d),
                 ("max_batch", i32), ("max_sqrt_cells", i32),
-                ("precision", i32), ("fold_kernel", i32)]
+                ("precision", i32), ("fold_kernel", i32),
+                ("elsewhere", i32), ("else_sqrt_num_cells", i32), ("else_num_rays", i32),
+                ("else_image_order_limit", i32)]
 
 
 class Request(C.Structure):
     _fields_ = [("mass", d), ("radius", d), ("distance", d), ("cos_inclination", d),
-                ("region", (d * 8) * 2), ("sqrt_num_cells", i32 * 2)]
+                ("region", (d * 8) * 2), ("sqrt_num_cells", i32 * 2),
+                ("elsewhere_temperature", d)]
 
 
 class XHError(RuntimeError):

@@ -177,3 +177,82 @@ def config2(atmosphere="Num4D", counts=None):
         atmosphere=num4d_table() if atm_ext == 2 else None,
         marginal=dict(epsabs=0.0, epsrel=1.0e-8, epsilon=1.0e-3, sigmas=10.0),
         llzero=-1.0e90)
+
+
+def interstellar(N_H=1.0e21, sigma0=2.0e-22, index=-2.5):
+    """Interstellar attenuation exp(-N_H sigma(E)), sigma = sigma0 (E/keV)^index
+    cm^2, evaluated at the input-bin midpoints (xpsi/Signal.py:435)."""
+    def attenuation(E):
+        return np.exp(-N_H * sigma0 * np.asarray(E, dtype=np.float64) ** index)
+    return attenuation
+
+
+def config5(scale="full", counts=None):
+    """Two hot regions of the config-2 geometry with blackbody emission (a
+    superseding spot with a concentric ceding ring, and its antipode half a
+    cycle later), the rest of the surface ("elsewhere") at log T 5.8 on a
+    closed mesh, and interstellar absorption with N_H = 1e21 cm^-2.
+
+    scale "full": sqrt_num_cells 256 (max 320), 256 leaves, 512 energies,
+    elsewhere 64 cells per side and 1000 rays, 300 channels x 256 inputs,
+    64 phase bins.  scale "test": sqrt_num_cells 16, 32 leaves, 64 energies,
+    elsewhere 16 cells per side and 200 rays, 60 channels x 64 inputs, 16 bins
+    (parity-test size)."""
+    full = scale == "full"
+    n_ch, n_in = (300, 256) if full else (60, 64)
+    M, in_edges, _ = gaussian_response(n_ch, n_in)
+    n_b = 64 if full else 16
+    if counts is None:
+        path = os.path.join(GOLDEN, "config5_%s.npz" % scale)
+        counts = np.load(path)["counts"] if os.path.exists(path) else np.zeros((n_ch, n_b))
+    regions = []
+    for k in range(2):
+        th = 0.9 if k == 0 else math.pi - 0.9
+        regions.append(dict(super_colatitude=th, super_radius=0.15, cede_radius=0.35,
+                            cede_colatitude=th, cede_azimuth=0.0, super_temperature=6.3,
+                            cede_temperature=6.0, phase_shift=0.0, cede=True,
+                            sqrt_num_cells=256 if full else 16, min_sqrt_num_cells=10,
+                            max_sqrt_num_cells=320 if full else 32, num_rays=200 if full else 100,
+                            num_leaves=256 if full else 32, is_antiphased=(k == 1),
+                            image_order_limit=None, atm_ext=1))
+    return dict(
+        spacetime=dict(mass=1.4, radius=12.0, distance=1.0, cos_inclination=0.5,
+                       frequency=300.0, star_shape="AGM_14"),
+        hot_regions=regions,
+        elsewhere=dict(sqrt_num_cells=64 if full else 16, num_rays=1000 if full else 200,
+                       image_order_limit=None, elsewhere_temperature=5.8),
+        interstellar=interstellar(),
+        instrument=dict(matrix=M, energy_edges=in_edges),
+        data=dict(counts=counts, phases=np.linspace(0.0, 1.0, n_b + 1), exposure_time=1.0e6),
+        num_energies=512 if full else 64, support=None,
+        marginal=dict(epsabs=0.0, epsrel=1.0e-8, epsilon=1.0e-3, sigmas=10.0),
+        llzero=-1.0e90)
+
+
+def config5_draws(n=1024, seed=5):
+    """Parameter vectors of the large workload: uniform draws of M 1-3,
+    R 8-16 km, cos i 0-1, primary colatitude 0-pi (the secondary region is
+    its antipode), super radius 0.05-0.5 rad, cede radius super + 0.05-0.5,
+    super log T 6.0-6.7, cede log T 5.9-6.5, phase U[0,1); elsewhere at
+    log T 5.8, distance 1 kpc; numpy.random.default_rng(seed) in that order."""
+    rng = np.random.default_rng(seed)
+    M = rng.uniform(1.0, 3.0, n)
+    R = rng.uniform(8.0, 16.0, n)
+    ci = rng.uniform(0.0, 1.0, n)
+    th = rng.uniform(0.0, math.pi, n)
+    rs = rng.uniform(0.05, 0.5, n)
+    rc = rs + rng.uniform(0.05, 0.5, n)
+    Ts = rng.uniform(6.0, 6.7, n)
+    Tc = rng.uniform(5.9, 6.5, n)
+    ph = rng.uniform(0.0, 1.0, n)
+    out = []
+    for k in range(n):
+        regs = []
+        for g in range(2):
+            t = th[k] if g == 0 else math.pi - th[k]
+            regs.append(dict(phase_shift=ph[k], super_colatitude=t, super_radius=rs[k],
+                             super_temperature=Ts[k], cede_colatitude=t, cede_radius=rc[k],
+                             cede_azimuth=0.0, cede_temperature=Tc[k]))
+        out.append(dict(mass=M[k], radius=R[k], distance=1.0, cos_inclination=ci[k],
+                        hot_regions=regs, elsewhere_temperature=5.8))
+    return out

@@ -374,6 +374,12 @@ struct xh_pipeline {
     int last_n;
     bool use_graphs;
     std::vector<hipGraphExec_t> graphs;   // per batch size n (index n), null until captured
+    // elsewhere component: one member per request on its own closed mesh
+    bool has_else;
+    Layout Le;
+    Member *d_else, *h_else;
+    double *d_else_tab, *d_else_total, *d_else_colat, *d_else_r, *d_else_maxA, *d_else_cg, *d_else_eg;
+    double *d_else_area, *d_else_defl, *d_else_cosa, *d_else_lag, *d_else_maxD, *d_else_part;
 };
 
 template <class T>
@@ -398,6 +404,7 @@ extern "C" void xh_pipeline_destroy(xh_pipeline* p)
     if (p->h_regions) (void)hipHostFree(p->h_regions);
     if (p->h_shifts) (void)hipHostFree(p->h_shifts);
     if (p->h_lnl) (void)hipHostFree(p->h_lnl);
+    if (p->h_else) (void)hipHostFree(p->h_else);
     delete p;
 }
 
@@ -518,6 +525,32 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     PCHK(dalloc(p, &p->d_bmls, (size_t)p->maxb * c->N_ch));
     PCHK(dalloc(p, &p->d_star, (size_t)p->maxb * c->N_ch * c->N_b));
     PCHK(dalloc(p, &p->d_lnl, (size_t)p->maxb));
+    if (c->elsewhere) {
+        if (c->else_sqrt_num_cells < 10 || c->else_sqrt_num_cells % 2 || c->else_num_rays < 3) {
+            fail(XH_EINVAL, "pipeline: elsewhere needs an even sqrt_num_cells >= 10 and >= 3 rays");
+            xh_pipeline_destroy(p); return nullptr;
+        }
+        p->has_else = true;
+        const int Se = c->else_sqrt_num_cells, NRe = c->else_num_rays;
+        p->Le = Layout{Se, NRe, c->num_leaves, N_P, c->N_E, 8,
+                       c->else_image_order_limit > 0 ? c->else_image_order_limit : 0, 1};
+        const size_t nb = (size_t)p->maxb;
+        PCHK(dalloc(p, &p->d_else, nb));
+        PCHK(dalloc(p, &p->d_else_tab, nb * p->table_n));
+        PCHK(dalloc(p, &p->d_else_total, nb));
+        PCHK(dalloc(p, &p->d_else_colat, nb * Se));
+        PCHK(dalloc(p, &p->d_else_r, nb * Se));
+        PCHK(dalloc(p, &p->d_else_maxA, nb * Se));
+        PCHK(dalloc(p, &p->d_else_cg, nb * Se));
+        PCHK(dalloc(p, &p->d_else_eg, nb * Se));
+        PCHK(dalloc(p, &p->d_else_area, nb));
+        PCHK(dalloc(p, &p->d_else_defl, nb * Se * NRe));
+        PCHK(dalloc(p, &p->d_else_cosa, nb * Se * NRe));
+        PCHK(dalloc(p, &p->d_else_lag, nb * Se * NRe));
+        PCHK(dalloc(p, &p->d_else_maxD, nb * Se));
+        PCHK(dalloc(p, &p->d_else_part, nb * Se * NE));
+        PHIP(hipHostMalloc((void**)&p->h_else, nb * sizeof(Member), hipHostMallocDefault));
+    }
     PHIP(hipHostMalloc((void**)&p->h_members, nm * sizeof(Member), hipHostMallocDefault));
     PHIP(hipHostMalloc((void**)&p->h_regions, ng * sizeof(RegionAlloc), hipHostMallocDefault));
     PHIP(hipHostMalloc((void**)&p->h_shifts, ng * sizeof(double), hipHostMallocDefault));
@@ -607,6 +640,7 @@ static void describe(const xh_pipeline* p, int rq, const xh_request& r)
                 m.polar = (cc - cr < 0.0) || (cc + cr > M_PI);
                 m.logT = cT;
             }
+            if (p->has_else) { m.correct = 1; m.else_logT = r.elsewhere_temperature; }
         }
         RegionAlloc& A = p->h_regions[gi];
         int snc = r.sqrt_num_cells[g] > 0 ? r.sqrt_num_cells[g] : c.region[g].sqrt_num_cells;
@@ -617,6 +651,15 @@ static void describe(const xh_pipeline* p, int rq, const xh_request& r)
         A.max_sqrt = c.region[g].max_sqrt_num_cells;
         A.member0 = m0; A.member1 = (nmem > 1 && cr > 0.0) ? m1 : -1;
         p->h_shifts[gi] = shift;
+    }
+    if (p->has_else) {
+        // Elsewhere.embed (xpsi/Elsewhere.py:254-303)
+        Member& e = p->h_else[rq];
+        std::memset((void*)&e, 0, sizeof(Member));
+        e.R_eq = R; e.epsilon = eps; e.zeta = zeta; e.Omega = Omega; e.r_s = 2.0 * r_g;
+        e.inclination = incl; e.mode_freq = c.mode_frequency; e.d_sq = d * d; e.shape = c.star_shape;
+        e.active = 1; e.S = c.else_sqrt_num_cells; e.numCell = (double)e.S * e.S;
+        e.logT = r.elsewhere_temperature;
     }
 }
 
@@ -657,6 +700,8 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     HIPCHK(hipMemcpyAsync(p->d_members, p->h_members, (size_t)nm * sizeof(Member), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(p->d_regions, p->h_regions, (size_t)ng * sizeof(RegionAlloc), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(p->d_shifts, p->h_shifts, (size_t)ng * 8, hipMemcpyHostToDevice, s));
+    if (p->has_else)
+        HIPCHK(hipMemcpyAsync(p->d_else, p->h_else, (size_t)n * sizeof(Member), hipMemcpyHostToDevice, s));
     HIPCHK(fill(p->d_cellArea, (size_t)nm * S * S, 0.0, s));
     HIPCHK(fill(p->d_flux, (size_t)nm * N_E * N_P, 0.0, s));
     HIPCHK(fill_int(p->d_ring_ok, (size_t)nm * S, 0, s));
@@ -674,6 +719,16 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     hipLaunchKernelGGL(k_ring_finalize, dim3(nm), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_cellArea,
                        p->d_u, p->d_ring_ok, p->d_logT);
     HIPCHK(hipGetLastError());
+    const Layout Le = p->Le;
+    if (p->has_else) {
+        hipLaunchKernelGGL(k_closed_table, dim3((p->table_n + 1 + 3) / 4, n), dim3(256), 0, s, p->d_else,
+                           p->d_else_tab, p->d_else_total, p->table_n);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_closed_rings, dim3(Le.S_max / 2, n), dim3(64), 0, s, p->d_else, p->d_else_tab,
+                           p->d_else_total, p->table_n, Le.S_max, p->d_else_colat, p->d_else_r, p->d_else_maxA,
+                           p->d_else_cg, p->d_else_eg, p->d_else_area);
+        HIPCHK(hipGetLastError());
+    }
     if (p->timing) HIPCHK(hipEventRecord(p->ev[1], s));
     hipLaunchKernelGGL(k_rays, dim3((L.N_R - 1 + 3) / 4, S, nm), dim3(256), 0, s, p->d_members, L, p->d_r,
                        p->d_maxAlpha, p->d_cosa, p->d_defl, p->d_lag);
@@ -681,11 +736,26 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     hipLaunchKernelGGL(k_rays_fix, dim3(nm), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_maxAlpha,
                        p->d_defl, p->d_lag, p->d_maxDefl);
     HIPCHK(hipGetLastError());
+    if (p->has_else) {
+        hipLaunchKernelGGL(k_rays, dim3((Le.N_R - 1 + 3) / 4, Le.S_max, n), dim3(256), 0, s, p->d_else, Le,
+                           p->d_else_r, p->d_else_maxA, p->d_else_cosa, p->d_else_defl, p->d_else_lag);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_rays_fix, dim3(n), dim3(256), 0, s, p->d_else, Le, p->d_else_r, p->d_else_maxA,
+                           p->d_else_defl, p->d_else_lag, p->d_else_maxD);
+        HIPCHK(hipGetLastError());
+    }
     RingArgs RA{p->d_theta, p->d_r, p->d_u, p->d_cos_gamma, p->d_maxDefl, p->d_defl, p->d_cosa, p->d_lag,
                 p->d_leaves, p->d_ring_ok, p->d_geo, p->d_valid, p->d_lastq};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[2], s));
     hipLaunchKernelGGL(k_geometry, dim3(S, L.IO_max, nm), dim3(256), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
     HIPCHK(hipGetLastError());
+    if (p->has_else) {
+        const size_t lds = (4 * (size_t)Le.N_R + 2 * (size_t)Le.S_max * 8) * 8;
+        hipLaunchKernelGGL(k_else_integrate, dim3(Le.S_max, n), dim3(256), lds, s, p->d_else, Le, p->d_else_colat,
+                           p->d_else_r, p->d_else_cg, p->d_else_maxD, p->d_else_defl, p->d_else_cosa,
+                           p->d_energies, N_E, p->d_else_part);
+        HIPCHK(hipGetLastError());
+    }
     // parallels per block: all of them when the batch fills the GPU
     int rb = (n >= 32) ? S : 1;
     FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_effGrav, p->d_lastq,
@@ -701,6 +771,11 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     EnergyArgs EA{p->d_flux, p->d_logE, p->d_logEdges, p->cfg.attenuation ? p->d_atten : nullptr,
                   p->d_member_index, p->d_integrated, N_E, N_P, p->N_in + 1, 2};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[4], s));
+    if (p->has_else) {
+        hipLaunchKernelGGL(k_else_add, dim3((N_E + 255) / 256, n), dim3(256), 0, s, p->d_else, Le, p->d_else_part,
+                           p->d_else_area, p->d_energies, N_E, N_P, p->d_members, p->mpr, p->d_flux);
+        HIPCHK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_energy_integrate, dim3((p->N_in * N_P + 255) / 256, ng), dim3(256), 0, s, p->d_members, EA);
     HIPCHK(hipGetLastError());
     FoldGemmArgs GA{p->d_M, p->d_integrated, p->d_folded, p->N_ch, p->N_in, N_P, p->N_in};
@@ -806,6 +881,26 @@ extern "C" int xh_pipeline_fetch(xh_pipeline* p, int what, int req, int index, d
     case 6: src = p->d_theta + (size_t)mi * S; cnt = m.S; break;
     case 7: src = p->d_phi + (size_t)mi * S; cnt = m.S; break;
     case 8: src = p->d_star + (size_t)req * p->N_ch * p->N_b; cnt = (size_t)p->N_ch * p->N_b; break;
+    case 9: {
+        // elsewhere spectrum [N_E]: ring partial sums x cell area x norm / (E keV)
+        if (!p->has_else) return fail(XH_EINVAL, "xh_pipeline_fetch: the model has no elsewhere component");
+        const Layout& Le = p->Le;
+        std::vector<double> part((size_t)Le.S_max * L.N_E), E(L.N_E);
+        double area = 0.0;
+        Member em{};
+        HIPCHK(hipMemcpy(&em, p->d_else + req, sizeof(Member), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(part.data(), p->d_else_part + (size_t)req * Le.S_max * L.N_E, part.size() * 8,
+                         hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&area, p->d_else_area + req, 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(E.data(), p->d_energies, (size_t)L.N_E * 8, hipMemcpyDeviceToHost));
+        const double bb_norm = 1.0e-7 * 5.040366110812353e22, kev = 1.60217662e-16;
+        for (size_t e = 0; e < (size_t)L.N_E && e < n; e++) {
+            double s = 0.0;
+            for (int i = 0; i < em.S; i++) s += part[(size_t)i * L.N_E + e];
+            out[e] = s * (area * bb_norm / (E[e] * kev));
+        }
+        return XH_OK;
+    }
     default: return fail(XH_EINVAL, "xh_pipeline_fetch: unknown array");
     }
     if (cnt > n) cnt = n;

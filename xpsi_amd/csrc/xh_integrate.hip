@@ -290,6 +290,7 @@ __global__ void __launch_bounds__(256, 4) k_profile_fold(const Member* __restric
                             (size_t)i * F.phi_row_stride + j];
         }
         const double kT = K_B_OVER_KEV * pow(10.0, F.logT[ro]);
+        const double kT_else = K_B_OVER_KEV * pow(10.0, m.else_logT);
         for (int I = 0; I < L.IO_max; I++) {
             const size_t vix = ro * L.IO_max + I;
             if (!F.valid[vix]) break;
@@ -323,10 +324,14 @@ __global__ void __launch_bounds__(256, 4) k_profile_fold(const Member* __restric
                         } else if (prevq[0] > 0.5) {
                             prev[0] = prevq[1]; prev[1] = prevq[2]; prev[2] = prevq[3]; prev[3] = prevq[4];
                         } else has_prev = 0;
-                        v = (num4d_I(F, vec, prev, has_prev) * NUM4D_NORM - 0.0) * geom;
+                        double corr = 0.0;
+                        if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
+                        v = (num4d_I(F, vec, prev, has_prev) * NUM4D_NORM - corr) * geom;
                     } else {
                         double I_E = Ep * Ep * Ep / (exp(Ep / kT) - 1.0);
-                        v = (I_E * BB_NORM - 0.0) * geom;
+                        double corr = 0.0;
+                        if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
+                        v = (I_E * BB_NORM - corr) * geom;
                     }
                 }
                 Y[q] = v;
@@ -379,7 +384,7 @@ __global__ void __launch_bounds__(256, 4) k_profile_fold(const Member* __restric
                     double t = x - PH[id];
                     int o = id * EC + p;
                     double v = Y[o] + t * (Bc[o] + t * (Cc[o] + t * Dc[o]));
-                    if (v > 0.0) acc[q] += area * v;
+                    if (v > 0.0 || m.correct) acc[q] += area * v;
                 }
             }
         }

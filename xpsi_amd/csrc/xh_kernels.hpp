@@ -92,6 +92,17 @@ struct FoldArgs {
 };
 
 __global__ void k_geometry(Member* members, Layout L, RingArgs A);
+// closed mesh and time-invariant spectrum of the elsewhere component
+__global__ void k_closed_table(const Member* members, double* area_tab, double* total, int table_n);
+__global__ void k_closed_rings(const Member* members, const double* area_tab, const double* total,
+                               int table_n, int S_max, double* colat, double* rr, double* maxAlpha,
+                               double* cos_gamma, double* effGrav, double* cellArea);
+__global__ void k_else_integrate(Member* members, Layout Le, const double* colat, const double* rr,
+                                 const double* cos_gamma, const double* maxDefl, const double* defl,
+                                 const double* cosa, const double* energies, int N_E, double* part);
+__global__ void k_else_add(const Member* else_members, Layout Le, const double* part,
+                           const double* cellArea, const double* energies, int N_E, int N_P,
+                           Member* hot_members, int members_per_request, double* flux);
 template <int EC, int MAXK, int ATM>
 __global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
 template <typename R, int ATM>

@@ -580,4 +580,83 @@ __global__ void __launch_bounds__(256) k_ring_prep(int S, const double* __restri
     }
 }
 
+// ---------------------------------------------------------------- closed mesh
+// Equal-area mesh of the whole star for the time-invariant "elsewhere"
+// component (xpsi/cellmesh/global_mesh.pyx:21-147).  k_closed_table: one
+// wave per point of the 1000-point area table (area from a colatitude to the
+// equator), plus the hemisphere area (point 0).
+__global__ void __launch_bounds__(256) k_closed_table(const Member* __restrict__ members,
+                                                      double* __restrict__ area_tab,
+                                                      double* __restrict__ total, int table_n)
+{
+    const Member m = members[blockIdx.y];
+    if (!m.active || m.status) return;
+    int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q > table_n) return;
+    double v;
+    if (q == 0) v = wave_area(0.0, HALF_PI, m.R_eq, m.epsilon, m.zeta, m.shape, 0);
+    else v = wave_area(lin(HALF_PI, 0.0, table_n, q - 1), HALF_PI, m.R_eq, m.epsilon, m.zeta, m.shape, 0);
+    if ((threadIdx.x & 63) == 0) {
+        if (q == 0) total[blockIdx.y] = v;
+        else area_tab[(size_t)blockIdx.y * table_n + q - 1] = v;
+    }
+}
+
+// One wave per northern ring: its bounding parallels by inverting the area
+// table (Steffen), the area-weighted mean colatitude between them, and the
+// ring's radius, surface tilt, gravity and maximum emission angle; the
+// southern ring S-1-i mirrors it.  cellArea[member] = the common cell area.
+__global__ void __launch_bounds__(64) k_closed_rings(const Member* __restrict__ members,
+                                                     const double* __restrict__ area_tab,
+                                                     const double* __restrict__ total, int table_n,
+                                                     int S_max, double* __restrict__ colat,
+                                                     double* __restrict__ rr, double* __restrict__ maxAlpha,
+                                                     double* __restrict__ cos_gamma,
+                                                     double* __restrict__ effGrav,
+                                                     double* __restrict__ cellArea)
+{
+    const int mi = blockIdx.y, i = blockIdx.x, lane = threadIdx.x;
+    const Member m = members[mi];
+    if (!m.active || m.status) return;
+    const int S = m.S, ncc = S / 2;
+    if (i >= ncc) return;
+    __shared__ double ct[1024];
+    for (int k = lane; k < table_n; k += 64) ct[k] = lin(HALF_PI, 0.0, table_n, k);
+    __syncthreads();
+    const double* at = area_tab + (size_t)mi * table_n;
+    double cell = TWO_PI * total[mi] / ((double)S * (double)S);
+    cell *= 2.0;
+    const double eta = cell * (double)S / TWO_PI;
+    // parallels[k] at i_eta = ncc eta - (k + 1) eta, by repeated subtraction
+    double i_eta = (double)ncc * eta, e_lo = 0.0, e_hi = 0.0;
+    for (int k = 0; k <= i && k < ncc - 1; k++) {
+        i_eta -= eta;
+        if (k == i - 1) e_lo = i_eta;
+        if (k == i) e_hi = i_eta;
+    }
+    const double lower = (i == 0) ? 0.0 : steffen_eval(at, ct, table_n, e_lo, nullptr);
+    const double upper = (i == ncc - 1) ? HALF_PI : steffen_eval(at, ct, table_n, e_hi, nullptr);
+    const double cc = wave_area(lower, upper, m.R_eq, m.epsilon, m.zeta, m.shape, 1) / eta;
+    if (lane == 0) {
+        const size_t ro = (size_t)mi * S_max;
+        const double mu = cos(cc);
+        const double rn = radius_norm(mu, m.epsilon, m.zeta, m.shape);
+        const double fth = f_theta(mu, rn, m.epsilon, m.zeta, m.shape);
+        const double cg = 1.0 / sqrt(1.0 + fth * fth);
+        const double eg = effective_gravity(mu, m.R_eq, m.zeta, m.epsilon, m.shape);
+        const int k = S - 1 - i;
+        colat[ro + i] = cc;
+        colat[ro + k] = PI - cc;
+        rr[ro + i] = rn * m.R_eq;
+        rr[ro + k] = rn * m.R_eq;
+        cos_gamma[ro + i] = cg;
+        cos_gamma[ro + k] = cg;
+        effGrav[ro + i] = eg;
+        effGrav[ro + k] = eg;
+        maxAlpha[ro + i] = HALF_PI + acos(cg);
+        maxAlpha[ro + k] = HALF_PI + acos(cg);
+        if (i == 0) cellArea[mi] = cell;
+    }
+}
+
 }  // namespace xh

@@ -169,6 +169,7 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                             (size_t)i * F.phi_row_stride + j];
         }
         const double kT = K_B_OVER_KEV * pow(10.0, F.logT[ro]);
+        const double kT_else = K_B_OVER_KEV * pow(10.0, m.else_logT);
         for (int I = 0; I < L.IO_max; I++) {
             const size_t vix = ro * L.IO_max + I;
             if (!F.valid[vix]) break;
@@ -203,14 +204,20 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                         } else if (ST[0] > 0.5) {
                             prev[0] = ST[1]; prev[1] = ST[2]; prev[2] = ST[3]; prev[3] = ST[4];
                         } else has_prev = 0;
-                        v = (R)((num4d_I(F, vec, prev, has_prev) * NUM4D_NORM - 0.0) * geom);
+                        double corr = 0.0;
+                        if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
+                        v = (R)((num4d_I(F, vec, prev, has_prev) * NUM4D_NORM - corr) * geom);
                     } else if (sizeof(R) == 8) {
                         double Ep = F.energies[e] / Zs[mm];
                         double I_E = Ep * Ep * Ep / (exp(Ep / kT) - 1.0);
-                        v = (R)((I_E * BB_NORM - 0.0) * geom);
+                        double corr = 0.0;
+                        if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
+                        v = (R)((I_E * BB_NORM - corr) * geom);
                     } else {
                         const float Ep = (float)F.energies[e] / (float)Zs[mm];
-                        const float I_E = Ep * Ep * Ep / (expf(Ep / (float)kT) - 1.0f);
+                        const float Ep3 = Ep * Ep * Ep;
+                        float I_E = Ep3 / (expf(Ep / (float)kT) - 1.0f);
+                        if (m.correct) I_E -= Ep3 / (expf(Ep / (float)kT_else) - 1.0f);
                         v = (R)(I_E * (float)BB_NORM * (float)geom);
                     }
                 }
@@ -235,7 +242,8 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                 const R h = (R)(PH[mm + 1] - PH[mm]);
                 R a = CO[mm * 64 + k], b, c, d;
                 akima_piece_r<R>(sl(mm - 2), sl(mm - 1), sl(mm), sl(mm + 1), sl(mm + 2), h, b, c, d);
-                const int cls = cubic_class<R>(a, b, c, d, h);
+                // with the elsewhere correction every value is kept (:593)
+                const int cls = m.correct ? 1 : cubic_class<R>(a, b, c, d, h);
                 if (cls == 0) { a = (R)0; b = (R)0; c = (R)0; d = (R)0; }
                 else if (cls == 2) atomicOr(&MIX[mm], 1u << k);
                 CO[mm * 64 + k] = a;

@@ -22,6 +22,10 @@ struct Member {
     double logT;        // log10 effective temperature of the member
     int status;         // 0 ok, 1 ray failure, 2 pulse failure, 3/4 size limits
     int antiphased;     // add pi to azimuths after the offset
+    // elsewhere correction: subtract the blackbody intensity at else_logT
+    // from every cell and keep negative spline values (correct != 0)
+    double else_logT;
+    int correct;
 };
 
 // Inputs of allocate_cells for one hot region (mesh_tools.pyx:892-910).

@@ -17,12 +17,15 @@ Model description (dict):
   data        counts [N_ch, N_b], phases [N_b + 1], exposure_time
   num_energies, support [N_ch, 2] or None, marginal settings, llzero,
   interstellar: callable(mid energies) -> attenuation factors, or None
+  elsewhere   None or dict: sqrt_num_cells (64), num_rays (1000),
+              image_order_limit, elsewhere_temperature (default value)
   precision   'fp64' (default) or 'mixed' (see PRECISIONS)
   fold_kernel 'auto' (default), 'moment' or 'per-point' (see FOLD_KERNELS)
 A parameter vector is a dict: mass, radius, distance, cos_inclination and
 `hot_regions`: list of dicts phase_shift, super_colatitude, super_radius,
 super_temperature, cede_colatitude, cede_radius, cede_azimuth,
-cede_temperature; optional per-region sqrt_num_cells.
+cede_temperature; optional per-region sqrt_num_cells; elsewhere_temperature
+when the model has an elsewhere component.
 """
 import ctypes as C
 import math
@@ -46,6 +49,7 @@ def fill_request(r, p):
                 v = hp["super_colatitude"] if key == "cede_colatitude" else 0.0
             r.region[g][q] = float(v)
         r.sqrt_num_cells[g] = int(hp.get("sqrt_num_cells", 0))
+    r.elsewhere_temperature = float(p.get("elsewhere_temperature", 0.0))
 
 
 def construct_energy_array(num_energies, edges):
@@ -148,6 +152,17 @@ def model_config(model, max_batch=1, max_sqrt_cells=0):
     if fk not in FOLD_KERNELS:
         raise XHError("fold_kernel must be one of %s" % sorted(FOLD_KERNELS))
     cfg.fold_kernel = FOLD_KERNELS[fk]
+    es = model.get("elsewhere")
+    if es is not None:
+        if es.get("atm_ext", "BB") != "BB":
+            raise XHError("the elsewhere component supports the blackbody atmosphere")
+        n = int(es.get("sqrt_num_cells", 64))
+        if n < 10 or n % 2:
+            raise XHError("elsewhere sqrt_num_cells must be an even integer >= 10")
+        cfg.elsewhere = 1
+        cfg.else_sqrt_num_cells = n
+        cfg.else_num_rays = int(es.get("num_rays", 1000))
+        cfg.else_image_order_limit = int(es.get("image_order_limit") or 0)
     return cfg, energies, keep
 
 
@@ -221,7 +236,7 @@ class LikelihoodPipeline:
 
     def fetch(self, what, req=0, index=0, size=None):
         names = {"flux": 0, "folded": 1, "sizes": 2, "cellArea": 3, "deflection": 4,
-                 "lag": 5, "theta": 6, "phi": 7, "STAR": 8}
+                 "lag": 5, "theta": 6, "phi": 7, "STAR": 8, "else_spectrum": 9}
         size = size or 4 * 1024 * 1024
         buf = np.zeros(size)
         check(lib().xh_pipeline_fetch(self._h, names[what], req, index, buf.ctypes.data, size),
@@ -235,4 +250,6 @@ def params_from_model(model):
     return dict(mass=sp["mass"], radius=sp["radius"], distance=sp["distance"],
                 cos_inclination=sp["cos_inclination"],
                 hot_regions=[{k: h.get(k) for k in _REGION_KEYS + ("sqrt_num_cells",)
-                              if h.get(k) is not None} for h in model["hot_regions"]])
+                              if h.get(k) is not None} for h in model["hot_regions"]],
+                **({"elsewhere_temperature": model["elsewhere"]["elsewhere_temperature"]}
+                   if model.get("elsewhere") is not None else {}))
