@@ -33,7 +33,11 @@ const char* xh_last_error(void);
 /* Replaces xpsi/cellmesh/integrator_for_azimuthal_invariance.pyx:70-98
  * `integrate(...) -> (status, flux[N_E, N_P])` for the blackbody (hot_atm_ext
  * 1) and four-dimensional table (hot_atm_ext 2, hot_Num4D.pyx) atmospheres,
- * without an elsewhere correction or disk.  The table is passed as device
+ * without a disk.  The reference's correction_srcCellParams (a blackbody
+ * elsewhere temperature uniform over the cells, HotRegion.py:1056-1069) is
+ * passed as correct = 1 and its log10 temperature else_logT; every cell then
+ * subtracts that intensity and negative values are kept (:469-478, :593).
+ * The table is passed as device
  * arrays: atm_axes = logT, logg, mu, log10(E/kT) concatenated, atm_N [4]
  * (int32, device) their lengths, atm_I the values in C order (T, g, mu, E);
  * all three may be NULL for the blackbody.
@@ -54,9 +58,31 @@ int xh_integrate(double omega, double r_s, double inclination, int S,
                  int N_E, const double* energies, int N_L,
                  const double* leaves, int N_P, const double* phases,
                  int hot_atm_ext, const double* atm_axes, const int* atm_N,
-                 const double* atm_I, int image_order_limit, double* flux,
-                 void* workspace, size_t workspace_bytes, int* status_out,
-                 void* stream);
+                 const double* atm_I, int image_order_limit, int correct,
+                 double else_logT, double* flux, void* workspace,
+                 size_t workspace_bytes, int* status_out, void* stream);
+
+/* Replaces xpsi/cellmesh/integrator_for_time_invariance.pyx:59-98
+ * `integrate(...) -> (status, flux[N_E])` for the blackbody elsewhere
+ * atmosphere (else_atm_ext 1).  S rings of S cells; cellArea: the common cell
+ * area (scalar); radial, colat (= theta[:, 0]), maxDeflection, cos_gamma: [S];
+ * phi: [S, S]; cellParams: [S, S, param_stride] with log10 T first;
+ * deflection, cos_alpha: [S, N_R]; energies [N_E] (keV); flux out [N_E].
+ * workspace: xh_integrate_time_invariant_workspace_bytes() bytes.
+ * *status_out = 0 success, 1 a deflection outside the ray table (ERROR). */
+size_t xh_integrate_time_invariant_workspace_bytes(int S, int N_E);
+int xh_integrate_time_invariant(double omega, double r_s, double inclination,
+                                int S, double cellArea, const double* radial,
+                                const double* colat, const double* phi,
+                                const double* cellParams, int param_stride,
+                                int N_R, const double* deflection,
+                                const double* cos_alpha,
+                                const double* maxDeflection,
+                                const double* cos_gamma, int N_E,
+                                const double* energies, int image_order_limit,
+                                double* flux, void* workspace,
+                                size_t workspace_bytes, int* status_out,
+                                void* stream);
 
 /* Replaces xpsi/tools/energy_integrator.pyx:27-30,114
  * `energy_integrator(N_Ts, signal[N_E, N_P], log10_E[N_E], log10_edges[N_in+1])`

@@ -91,3 +91,65 @@ def test_elsewhere_precision_form():
     l32 = LikelihoodPipeline(dict(model, precision="mixed"), max_batch=1).evaluate(p)[0]
     say("config5-test: lnL fp64 %.10f mixed %.10f |dlnL| %.3e" % (l64, l32, abs(l32 - l64)))
     assert abs(l32 - l64) < 1.0e-3
+
+
+def test_time_invariant_dropin():
+    import math
+    from xpsi_amd.cellmesh.integrator_for_time_invariance import integrate
+    st = O.spacetime(1.6, 11.0, 1.0, 0.3, 400.0)
+    em, err = O.elsewhere_embed(st, dict(sqrt_num_cells=32, num_rays=500))
+    assert err is None
+    E = np.logspace(np.log10(0.3), 1.0, 96)
+    t, ref = O.elsewhere_integrate(st, em, E, 5.9)
+    assert t == 0
+    S = em["S"]
+    theta = np.repeat(em["colat"][:, None], S, axis=1)
+    phi = np.repeat(em["phis"][None, :], S, axis=0)
+    params = np.ones((S, S, 2))
+    params[..., 0] = 5.9
+    params[..., 1] = em["effGrav"][:, None]
+    say("time-invariant drop-in: S=%d N_R=500 N_E=96" % S)
+    status, flux = integrate(1, st["R"], st["Omega"], st["r_s"], st["i"], S, em["cellArea"], em["r"],
+                             em["r_s_over_r"], theta, phi, params, 500, em["deflection"],
+                             em["cos_alpha"], em["maxDeflection"], em["cos_gamma"], E, (), 1)
+    assert status == 0
+    e = relerr(flux, ref)
+    say("time-invariant drop-in rel err %.3e" % e)
+    assert e < SIG_RTOL
+
+
+def test_integrate_dropin_with_correction():
+    import math
+    from xpsi_amd import configs
+    from xpsi_amd.cellmesh.integrator_for_azimuthal_invariance import integrate
+    model = configs.config1()
+    sp = model["spacetime"]
+    st = O.spacetime(sp["mass"], sp["radius"], sp["distance"], sp["cos_inclination"], sp["frequency"])
+    hr = dict(model["hot_regions"][0], sqrt_num_cells=24)
+    members, err = O.hot_region_embed(st, hr, sp["frequency"])
+    m = members[0]
+    E = O.construct_energy_array(64, model["instrument"]["energy_edges"])
+    leaves = np.linspace(0.0, 2.0 * math.pi, 64)
+    phases = 2.0 * math.pi * np.linspace(0.0, 1.0, 64)
+    corr = np.ones_like(m["params"])
+    corr[..., 0] = 5.8
+    corr[..., 1] = m["effGrav"][:, None]
+    t, ref = O.integrate(st["Omega"], st["r_s"], st["i"], m["cellArea"], m["r"], m["r_s_over_r"],
+                         m["theta"], m["phi"], m["params"], m["radiates"], m["deflection"],
+                         m["cos_alpha"], m["lag"], m["maxDeflection"], m["cos_gamma"], E, leaves,
+                         phases, correction=corr)
+    assert t == 0
+    for kind in (1, 2):
+        from xpsi_amd._lib import lib
+        lib().xh_set_fold_kernel(kind)
+        say("integrate drop-in with correction, fold kernel %d" % kind)
+        status, flux = integrate(1, st["R"], st["Omega"], st["r_s"], st["i"], m["cellArea"], m["r"],
+                                 m["r_s_over_r"], m["theta"], m["phi"], m["params"], m["radiates"],
+                                 corr, m["deflection"].shape[1], m["deflection"], m["cos_alpha"],
+                                 m["lag"], m["maxDeflection"], m["cos_gamma"], E, leaves, phases,
+                                 (), (), 1, 1, 0)
+        lib().xh_set_fold_kernel(0)
+        assert status == 0
+        e = relerr(flux, ref)
+        say("integrate drop-in with correction rel err %.3e" % e)
+        assert e < SIG_RTOL

@@ -61,8 +61,13 @@ def lib():
     L.xh_integrate_workspace_bytes.argtypes = [i32] * 5
     L.xh_integrate.restype = i32
     L.xh_integrate.argtypes = ([d, d, d, i32] + [vp] * 7 + [i32] + [vp] * 5 +
-                               [i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, i32, vp, vp, sz,
+                               [i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, i32, i32, d, vp, vp, sz,
                                 C.POINTER(i32), vp])
+    L.xh_integrate_time_invariant_workspace_bytes.restype = sz
+    L.xh_integrate_time_invariant_workspace_bytes.argtypes = [i32, i32]
+    L.xh_integrate_time_invariant.restype = i32
+    L.xh_integrate_time_invariant.argtypes = ([d, d, d, i32, d] + [vp] * 4 + [i32, i32] + [vp] * 4 +
+                                              [i32, vp, i32, vp, vp, sz, C.POINTER(i32), vp])
     L.xh_energy_integrator.restype = i32
     L.xh_energy_integrator.argtypes = [i32, i32, vp, vp, i32, vp, vp, vp]
     L.xh_fold.restype = i32
@@ -123,6 +128,7 @@ def check(rc, what):
 
 
 EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
+            "xh_integrate_time_invariant_workspace_bytes", "xh_integrate_time_invariant",
             "xh_energy_integrator", "xh_fold", "xh_precomputation",
             "xh_marginal_workspace_bytes", "xh_eval_marginal_likelihood",
             "xh_pipeline_create", "xh_pipeline_destroy", "xh_pipeline_eval",

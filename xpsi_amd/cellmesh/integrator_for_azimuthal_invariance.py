@@ -5,8 +5,10 @@
 reference binds its integrator (HotRegion._integrator, HotRegion.py:551-572).
 Supported: the blackbody (hot_atm_ext 1) and four-dimensional table
 (hot_atm_ext 2, `hot_atmosphere` = (logT, logg, mu, logE, buf)) atmospheres
-without beaming, an elsewhere correction or a disk (R_in >= 1e6); other
-settings raise NotImplementedError.
+without beaming or a disk (R_in >= 1e6), with or without the blackbody
+elsewhere correction (else_atm_ext 1) at a temperature uniform over the
+cells (Elsewhere._compute_cellParamVecs); other settings raise
+NotImplementedError.
 """
 import ctypes as C
 
@@ -22,8 +24,12 @@ def integrate(numThreads, R, omega, r_s, inclination, cellArea,
               cos_alpha, lag, maxDeflection, cos_gammaArray, energies, leaves,
               phases, hot_atmosphere, elsewhere_atmosphere, hot_atm_ext,
               else_atm_ext, beam_opt, image_order_limit=None, R_in=1e6):
+    correct, else_logT = 0, 0.0
     if correction_srcCellParams is not None:
-        raise NotImplementedError("elsewhere correction is not supported")
+        corr = np.asarray(correction_srcCellParams, dtype=np.float64)
+        if int(else_atm_ext) != 1 or not np.all(corr[..., 0] == corr.flat[0]):
+            raise NotImplementedError("elsewhere correction: blackbody at one temperature")
+        correct, else_logT = 1, float(corr.flat[0])
     if int(hot_atm_ext) not in (1, 2) or int(beam_opt) != 0 or R_in < 1e6:
         raise NotImplementedError("blackbody or 4-D table atmospheres without beaming or disk")
     S = int(np.shape(cellArea)[0])
@@ -51,7 +57,7 @@ def integrate(numThreads, R, omega, r_s, inclination, cellArea,
                         D.ptr(rays[3]), D.ptr(rays[4]), N_E, D.ptr(rays[5]),
                         N_L, D.ptr(rays[6]), N_P, D.ptr(rays[7]), int(hot_atm_ext),
                         D.ptr(axes), D.ptr(nax), D.ptr(tab),
-                        int(image_order_limit or 0), D.ptr(flux), D.ptr(ws),
+                        int(image_order_limit or 0), correct, else_logT, D.ptr(flux), D.ptr(ws),
                         ws.numel() * 8, C.byref(status), D.stream_ptr())
     check(rc, "xh_integrate")
     if status.value != 0:

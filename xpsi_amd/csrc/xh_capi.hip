@@ -163,6 +163,12 @@ static hipError_t launch_moment(bool fp32, dim3 g, size_t lds, hipStream_t s, co
                           : launch_moment_t<double, 1>(g, lds, s, m, L, F);
 }
 
+// dynamic LDS of k_else_integrate: ray tables, alternative tables, slots
+static size_t else_lds(const Layout& Le)
+{
+    return (4 * (size_t)Le.N_R + 2 * (size_t)Le.S_max * 8 + (size_t)Le.S_max) * 8;
+}
+
 static hipError_t fill(double* p, size_t n, double v, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
@@ -196,8 +202,9 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
                             const double* maxDeflection, const double* cos_gamma, int N_E,
                             const double* energies, int N_L, const double* leaves, int N_P,
                             const double* phases, int hot_atm_ext, const double* atm_axes, const int* atm_N,
-                            const double* atm_I, int image_order_limit, double* flux, void* workspace,
-                            size_t workspace_bytes, int* status_out, void* stream)
+                            const double* atm_I, int image_order_limit, int correct, double else_logT,
+                            double* flux, void* workspace, size_t workspace_bytes, int* status_out,
+                            void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (hot_atm_ext != 1 && hot_atm_ext != 2)
@@ -225,6 +232,7 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     int* d_lastq = (int*)w;
     Member hm{};
     hm.Omega = omega; hm.r_s = r_s; hm.inclination = inclination; hm.active = 1; hm.S = S;
+    hm.correct = correct ? 1 : 0; hm.else_logT = else_logT;
     HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
     Layout L{S, N_R, N_L, N_P, N_E, IO, image_order_limit > 0 ? image_order_limit : 0, hot_atm_ext};
     hipLaunchKernelGGL(k_ring_prep, dim3(1), dim3(256), 0, s, S, theta, cellParams, radiates, d_theta, d_ok, d_logT,
@@ -259,6 +267,48 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (back.status == 4) return fail(XH_ERANGE, "xh_integrate: more image orders than supported (8)");
+    *status_out = back.status ? 1 : 0;
+    return XH_OK;
+}
+
+extern "C" size_t xh_integrate_time_invariant_workspace_bytes(int S, int N_E)
+{
+    return 256 + (size_t)S * N_E * 8 + 256;
+}
+
+extern "C" int xh_integrate_time_invariant(double omega, double r_s, double inclination, int S, double cellArea,
+                                           const double* radial, const double* colat, const double* phi,
+                                           const double* cellParams, int param_stride, int N_R,
+                                           const double* deflection, const double* cos_alpha,
+                                           const double* maxDeflection, const double* cos_gamma, int N_E,
+                                           const double* energies, int image_order_limit, double* flux,
+                                           void* workspace, size_t workspace_bytes, int* status_out, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (S < 2 || N_R < 3 || N_E < 1 || param_stride < 1)
+        return fail(XH_EINVAL, "xh_integrate_time_invariant: bad sizes");
+    if (workspace_bytes < xh_integrate_time_invariant_workspace_bytes(S, N_E))
+        return fail(XH_EINVAL, "xh_integrate_time_invariant: workspace too small");
+    Layout Le{S, N_R, 0, 0, N_E, 8, image_order_limit > 0 ? image_order_limit : 0, 1};
+    if (else_lds(Le) > 64 * 1024)
+        return fail(XH_EINVAL, "xh_integrate_time_invariant: too many rays or cells for one block's LDS");
+    if (image_order_limit > 8) return fail(XH_EINVAL, "xh_integrate_time_invariant: at most 8 image orders");
+    char* w = (char*)workspace;
+    Member* d_m = (Member*)w; w += 256;
+    double* d_part = (double*)w;
+    Member hm{};
+    hm.Omega = omega; hm.r_s = r_s; hm.inclination = inclination; hm.active = 1; hm.S = S;
+    HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_else_integrate, dim3(S, 1), dim3(256), else_lds(Le), s, d_m, Le, colat, radial, cos_gamma,
+                       maxDeflection, deflection, cos_alpha, energies, N_E, phi, S, cellParams, param_stride, d_part);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_else_finalize, dim3((N_E + 255) / 256), dim3(256), 0, s, (const Member*)d_m, Le, d_part,
+                       cellArea, energies, N_E, flux);
+    HIPCHK(hipGetLastError());
+    Member back{};
+    HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (back.status == 4) return fail(XH_ERANGE, "xh_integrate_time_invariant: more than 8 image orders");
     *status_out = back.status ? 1 : 0;
     return XH_OK;
 }
@@ -750,10 +800,9 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     hipLaunchKernelGGL(k_geometry, dim3(S, L.IO_max, nm), dim3(256), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
     HIPCHK(hipGetLastError());
     if (p->has_else) {
-        const size_t lds = (4 * (size_t)Le.N_R + 2 * (size_t)Le.S_max * 8) * 8;
-        hipLaunchKernelGGL(k_else_integrate, dim3(Le.S_max, n), dim3(256), lds, s, p->d_else, Le, p->d_else_colat,
-                           p->d_else_r, p->d_else_cg, p->d_else_maxD, p->d_else_defl, p->d_else_cosa,
-                           p->d_energies, N_E, p->d_else_part);
+        hipLaunchKernelGGL(k_else_integrate, dim3(Le.S_max, n), dim3(256), else_lds(Le), s, p->d_else, Le,
+                           p->d_else_colat, p->d_else_r, p->d_else_cg, p->d_else_maxD, p->d_else_defl,
+                           p->d_else_cosa, p->d_energies, N_E, nullptr, 0, nullptr, 0, p->d_else_part);
         HIPCHK(hipGetLastError());
     }
     // parallels per block: all of them when the batch fills the GPU

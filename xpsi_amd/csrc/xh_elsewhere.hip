@@ -35,6 +35,8 @@ __global__ void __launch_bounds__(256) k_else_integrate(Member* __restrict__ mem
                                                         const double* __restrict__ defl,
                                                         const double* __restrict__ cosa,
                                                         const double* __restrict__ energies, int N_E,
+                                                        const double* __restrict__ phi, int phi_row_stride,
+                                                        const double* __restrict__ params, int param_stride,
                                                         double* __restrict__ part)
 {
     extern __shared__ double sm[];
@@ -50,6 +52,7 @@ __global__ void __launch_bounds__(256) k_else_integrate(Member* __restrict__ mem
     double* yalt = xalt + N_R;              // [N_R]
     double* Zs = yalt + N_R;                // [S_max * ELSE_IO_MAX] redshift factor
     double* Gs = Zs + Le.S_max * ELSE_IO_MAX;   // geometric factor, 0 = no image
+    double* Ks = Gs + Le.S_max * ELSE_IO_MAX;   // [S_max] kT of the cell
     __shared__ int jn_s, term;
     const size_t ro = (size_t)mi * Le.S_max + i;
     const size_t tb = ro * N_R;
@@ -75,10 +78,13 @@ __global__ void __launch_bounds__(256) k_else_integrate(Member* __restrict__ mem
     if (tid == 0 && IO > ELSE_IO_MAX) atomicMax(&mp->status, 4);
     const double dphi = TWO_PI / (double)S;
     for (int j = tid; j < S; j += 256) {
-        // cell azimuths: linspace(-pi + dphi/2, pi - dphi/2, S)
+        // cell azimuths: given, or the closed mesh's linspace(-pi + dphi/2, pi - dphi/2, S)
         double ph;
-        if (j == S - 1) ph = PI - 0.5 * dphi;
+        if (phi) ph = phi[(size_t)mi * S * (phi_row_stride ? S : 1) + (size_t)i * phi_row_stride + j];
+        else if (j == S - 1) ph = PI - 0.5 * dphi;
         else ph = (double)j * (((PI - 0.5 * dphi) - (-PI + 0.5 * dphi)) / (double)(S - 1)) + (-PI + 0.5 * dphi);
+        const double lt = params ? params[((size_t)i * S + j) * param_stride] : m.logT;
+        Ks[j] = K_B_OVER_KEV * pow(10.0, lt);
         const double c_psi0 = cos_i * cos_th + sin_i * sin_th * cos(ph);
         const double psi0 = acos(c_psi0);
         for (int I = 0; I < IO && I < ELSE_IO_MAX; I++) {
@@ -124,20 +130,35 @@ __global__ void __launch_bounds__(256) k_else_integrate(Member* __restrict__ mem
         if (tid == 0) atomicMax(&mp->status, 2);
         return;
     }
-    const double kT = K_B_OVER_KEV * pow(10.0, m.logT);
     double* out = part + ro * N_E;
     for (int e = tid; e < N_E; e += 256) {
         const double E = energies[e];
         double acc = 0.0;
-        for (int j = 0; j < S; j++)
+        for (int j = 0; j < S; j++) {
+            const double kT = Ks[j];
             for (int I = 0; I < ELSE_IO_MAX; I++) {
                 const double G = Gs[j * ELSE_IO_MAX + I];
                 if (G == 0.0) continue;
                 const double Ep = E / Zs[j * ELSE_IO_MAX + I];
                 acc += Ep * Ep * Ep / (exp(Ep / kT) - 1.0) * G;
             }
+        }
         out[e] = acc;
     }
+}
+
+// drop-in finalisation: spectrum[e] = sum over rings (in order) x cell area x
+// norm / (E keV) (integrator_for_time_invariance.pyx:318-324)
+__global__ void __launch_bounds__(256) k_else_finalize(const Member* __restrict__ member, Layout Le,
+                                                       const double* __restrict__ part, double cellArea,
+                                                       const double* __restrict__ energies, int N_E,
+                                                       double* __restrict__ out)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= N_E) return;
+    double s = 0.0;
+    for (int i = 0; i < member->S; i++) s += part[(size_t)i * N_E + e];
+    out[e] = s * (cellArea * BB_NORM / (energies[e] * KEV));
 }
 
 // else_members[r]: the elsewhere member of request r; hot member 0 of the

@@ -99,7 +99,10 @@ __global__ void k_closed_rings(const Member* members, const double* area_tab, co
                                double* cos_gamma, double* effGrav, double* cellArea);
 __global__ void k_else_integrate(Member* members, Layout Le, const double* colat, const double* rr,
                                  const double* cos_gamma, const double* maxDefl, const double* defl,
-                                 const double* cosa, const double* energies, int N_E, double* part);
+                                 const double* cosa, const double* energies, int N_E, const double* phi,
+                                 int phi_row_stride, const double* params, int param_stride, double* part);
+__global__ void k_else_finalize(const Member* member, Layout Le, const double* part, double cellArea,
+                                const double* energies, int N_E, double* out);
 __global__ void k_else_add(const Member* else_members, Layout Le, const double* part,
                            const double* cellArea, const double* energies, int N_E, int N_P,
                            Member* hot_members, int members_per_request, double* flux);
