@@ -44,6 +44,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--config1-batch", type=int, default=1024)
+    ap.add_argument("--c5-n", type=int, default=64, help="large-workload draws (0 = skip)")
+    ap.add_argument("--c5-steps", type=int, default=1)
+    ap.add_argument("--c5-check", type=int, default=16, help="large-workload draws re-run in FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the likelihood path shards requests across GPUs "
@@ -198,6 +201,8 @@ def main():
         "device": dev,
     }
     out["roofline"]["frac"] = out["roofline"]["achieved"] / out["roofline"]["peak"]
+    if args.c5_n > 0:
+        out["config5"] = config5_section(args, stream)
     if not args.no_serving:
         out["serving"] = serving(args, stream_free=True)
     if not args.no_cpu:
@@ -214,6 +219,45 @@ def main():
                                          "one thread) with the C oracle" % nc,
                                "host_cpus": os.cpu_count()}
     print(json.dumps(out))
+
+
+def config5_section(args, stream):
+    """The large workload: two hot regions (spot + ceding ring each) with an
+    elsewhere component and interstellar absorption, sqrt_num_cells 256, 256
+    leaves, 512 energies, 64 phase bins; prior draws evaluated in one launch
+    in the measured precision, and the first few again in FP64 for |dlnL|."""
+    import torch
+    from xpsi_amd import configs
+    from xpsi_amd.pipeline import LikelihoodPipeline
+    model = configs.config5("full")
+    draws = [d for d in configs.config5_draws(4 * args.c5_n)
+             if configs.prior_accepts(d["mass"], d["radius"], 300.0)][:args.c5_n]
+    pl = LikelihoodPipeline(dict(model, precision=args.precision), max_batch=len(draws))
+    pl.evaluate(draws, stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = max(1, args.c5_steps)
+    for _ in range(reps):
+        got = pl.evaluate(draws, stream)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    pl.close()
+    out = {"evals_per_s": len(draws) / dt, "batch": len(draws), "ms_per_batch": dt * 1e3,
+           "finite_fraction": float(np.mean(got > -1e89)),
+           "workload": "2 hot regions (super + cede) + elsewhere (64x64 cells, 1000 rays) + ISM, "
+                       "sqrt_num_cells 256, 256 leaves, 512 energies, 300 ch x 64 bins, %s form"
+                       % args.precision}
+    nchk = min(args.c5_check, len(draws))
+    if args.precision != "fp64" and nchk > 0:
+        pl64 = LikelihoodPipeline(dict(model, precision="fp64"), max_batch=nchk)
+        l64 = pl64.evaluate(draws[:nchk], stream)
+        pl64.close()
+        ok = (l64 > -1e89) & (got[:nchk] > -1e89)
+        d = np.abs(got[:nchk][ok] - l64[ok])
+        out["dlnl_vs_fp64"] = {"n": int(ok.sum()), "max": float(d.max()) if d.size else None,
+                               "p50": float(np.percentile(d, 50)) if d.size else None,
+                               "same_llzero_class": bool(np.all((l64 > -1e89) == (got[:nchk] > -1e89)))}
+    return out
 
 
 def serving(args, stream_free=True):

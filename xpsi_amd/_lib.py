@@ -35,7 +35,7 @@ class ModelConfig(C.Structure):
                 ("max_batch", i32), ("max_sqrt_cells", i32),
                 ("precision", i32), ("fold_kernel", i32),
                 ("elsewhere", i32), ("else_sqrt_num_cells", i32), ("else_num_rays", i32),
-                ("else_image_order_limit", i32)]
+                ("else_image_order_limit", i32), ("fold_terms", i32)]
 
 
 class Request(C.Structure):
@@ -55,6 +55,11 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise XHError("libxpsi_hip.so is not built; run `python -m xpsi_amd.build` "
                       "(or __graft_entry__.build()) first")
+    # PyTorch allocates the drop-ins' device buffers.  Load it first so that
+    # this library's libamdhip64 dependency resolves to the HIP runtime torch
+    # already holds: a second runtime instance in the same process finds no
+    # device once torch's has opened it.
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     L.xh_last_error.restype = C.c_char_p
     L.xh_integrate_workspace_bytes.restype = sz

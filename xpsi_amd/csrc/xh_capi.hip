@@ -401,6 +401,8 @@ struct xh_pipeline {
     int nreg, mpr, maxb, N_in, N_ch, N_b, EC, table_n;
     int fold_kind;               // 1 per-point fold, 2 moment form
     bool fp32;                   // precision form
+    int fold_terms;              // bf16 products of the precision form's response fold
+    __bf16 *d_Mh, *d_Mm, *d_Ml;  // three-term bf16 split of the response (precision form)
     int moff[3];                 // first member of each region within a request
     bool timing;
     hipEvent_t ev[16];
@@ -489,6 +491,10 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         fail(XH_EINVAL, "pipeline: precision must be 0 or 1 and fold_kernel 0, 1 or 2"); delete p; return nullptr;
     }
     p->fp32 = c->precision == 1;
+    p->fold_terms = c->fold_terms == 3 ? 3 : 6;
+    if (c->fold_terms != 0 && c->fold_terms != 3 && c->fold_terms != 6) {
+        fail(XH_EINVAL, "pipeline: fold_terms must be 0, 3 or 6"); delete p; return nullptr;
+    }
     {
         const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8) <= kMomentLdsMax;
         if ((c->fold_kernel == 2 || p->fp32) && !fits) {
@@ -624,6 +630,24 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     for (int i = 0; i < N_P; i++) v[i] = two_pi * cyc[i];
     PHIP(hipMemcpy(p->d_phases, v.data(), (size_t)N_P * 8, hipMemcpyHostToDevice));
     PHIP(hipMemcpy(p->d_M, c->matrix, (size_t)c->N_ch * c->N_in * 8, hipMemcpyHostToDevice));
+    if (p->fp32) {
+        // three-term bf16 split of the FP32-rounded response, once
+        const size_t nmat = (size_t)c->N_ch * c->N_in;
+        std::vector<__bf16> h(nmat), m(nmat), l(nmat);
+        for (size_t q = 0; q < nmat; q++) {
+            const float x = (float)c->matrix[q];
+            h[q] = (__bf16)x;
+            const float r = x - (float)h[q];
+            m[q] = (__bf16)r;
+            l[q] = (__bf16)(r - (float)m[q]);
+        }
+        PCHK(dalloc(p, &p->d_Mh, nmat));
+        PCHK(dalloc(p, &p->d_Mm, nmat));
+        PCHK(dalloc(p, &p->d_Ml, nmat));
+        PHIP(hipMemcpy(p->d_Mh, h.data(), nmat * sizeof(__bf16), hipMemcpyHostToDevice));
+        PHIP(hipMemcpy(p->d_Mm, m.data(), nmat * sizeof(__bf16), hipMemcpyHostToDevice));
+        PHIP(hipMemcpy(p->d_Ml, l.data(), nmat * sizeof(__bf16), hipMemcpyHostToDevice));
+    }
     PHIP(hipMemcpy(p->d_data_phases, c->data_phases, ((size_t)c->N_b + 1) * 8, hipMemcpyHostToDevice));
     PHIP(hipMemcpy(p->d_counts, c->counts, (size_t)c->N_ch * c->N_b * 8, hipMemcpyHostToDevice));
     std::vector<int> ic((size_t)c->N_ch * c->N_b);
@@ -828,8 +852,12 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     hipLaunchKernelGGL(k_energy_integrate, dim3((p->N_in * N_P + 255) / 256, ng), dim3(256), 0, s, p->d_members, EA);
     HIPCHK(hipGetLastError());
     FoldGemmArgs GA{p->d_M, p->d_integrated, p->d_folded, p->N_ch, p->N_in, N_P, p->N_in};
-    if (p->fp32)
-        hipLaunchKernelGGL(k_fold_bf16x3, dim3((N_P + 31) / 32, (p->N_ch + 31) / 32, ng), dim3(256), 0, s, GA);
+    if (p->fp32 && p->fold_terms == 3)
+        hipLaunchKernelGGL(k_fold_bf16<3>, dim3((N_P + 31) / 32, (p->N_ch + 31) / 32, ng), dim3(256), 0, s, GA,
+                           p->d_Mh, p->d_Mm, p->d_Ml);
+    else if (p->fp32)
+        hipLaunchKernelGGL(k_fold_bf16<6>, dim3((N_P + 31) / 32, (p->N_ch + 31) / 32, ng), dim3(256), 0, s, GA,
+                           p->d_Mh, p->d_Mm, p->d_Ml);
     else
         hipLaunchKernelGGL(k_fold_f64, dim3((N_P + 31) / 32, (p->N_ch + 31) / 32, ng), dim3(256), 0, s, GA);
     HIPCHK(hipGetLastError());

@@ -30,9 +30,10 @@
 //
 // R = double: FP64 throughout (v_mfma_f64_16x16x4f64).
 // R = float: the precision form.  Blackbody intensity, spline coefficients,
-// moments and products in FP32 (v_mfma_f32_16x16x4f32); the phase offsets t
-// are formed in FP64 and the per-parallel partial sums are accumulated in
-// FP64.
+// moments and products in FP32 (v_mfma_f32_16x16x4f32); the intensity's
+// arguments (E' = E/Z and E'/kT) and the phase offsets t are formed in FP64
+// and rounded once, the per-parallel partial sums are accumulated in FP64 and
+// the blackbody norm is applied in FP64 at the end.
 #include "xh_common.hpp"
 #include "xh_types.hpp"
 #include "xh_kernels.hpp"
@@ -149,8 +150,16 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
     R* MOM = CO + N_L * 64;                 // [NI][4][16] moment m at phase p
     R* SL = MOM;                            // [NI][16] secant slopes (before the moments)
 
+    // FP32 form: cell areas enter the moments relative to the ring's largest
+    // one.  The mesh is equal-area, so interior cells become exactly 1 and
+    // their common area multiplies each parallel's sums in FP64 (one FP32
+    // rounding of it would scale every bin alike).  Two slots alternate with
+    // the parallel's parity.
+    unsigned long long* ABITS = (unsigned long long*)(ST + 6);
+    if (tid < 2) ABITS[tid] = 0ull;
     DIR[tid] = (R)0;
     double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    double dtot = 0.0;                      // FP32 form: direct sums, scaled per parallel
     const size_t cbase = (size_t)mi * L.S_max * L.S_max;
     const int p = tid & 15, part = tid >> 4;
     const int pg = p0 + p;
@@ -160,6 +169,7 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
     for (int i = r0; i < r1; i++) {
         const size_t ro = (size_t)mi * L.S_max + i;
         __syncthreads();   // the previous parallel is done with the cell arrays
+        if (tid == 0) ABITS[(i + 1) & 1] = 0ull;
         for (int j = tid; j < S; j += 256) {
             size_t c = cbase + (size_t)i * S + j;
             double a = F.cellArea[c];
@@ -167,6 +177,7 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
             carea[j] = rad ? a : __builtin_nan("");
             cphi[j] = F.phi[(size_t)mi * L.S_max * (F.phi_row_stride ? L.S_max : 1) +
                             (size_t)i * F.phi_row_stride + j];
+            if (sizeof(R) == 4 && rad && a > 0.0) atomicMax(&ABITS[i & 1], (unsigned long long)__double_as_longlong(a));
         }
         const double kT = K_B_OVER_KEV * pow(10.0, F.logT[ro]);
         const double kT_else = K_B_OVER_KEV * pow(10.0, m.else_logT);
@@ -181,6 +192,9 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                 MIX[q] = 0u;
             }
             if (ATM == 2 && tid == 0) prev_query_state(F, L, mi, i, I, ST);
+            const double a_ref = (sizeof(R) == 4 && ABITS[i & 1]) ? __longlong_as_double((long long)ABITS[i & 1]) : 1.0;
+            if (sizeof(R) == 4 && I == 0)
+                for (int j = tid; j < S; j += 256) carea[j] = carea[j] / a_ref;
             __syncthreads();
             // profile: specific intensity x geometry, zero where the leaf is hidden
             for (int q = tid; q < N_L * 16; q += 256) {
@@ -214,11 +228,16 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                         if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
                         v = (R)((I_E * BB_NORM - corr) * geom);
                     } else {
-                        const float Ep = (float)F.energies[e] / (float)Zs[mm];
+                        // FP32 intensity from arguments rounded once each: a shared
+                        // rounding (of E, kT or the norm) would bias every bin alike,
+                        // so E' and E'/kT are formed in FP64 and the norm is applied
+                        // in FP64 after the sums
+                        const double Ep_d = F.energies[e] / Zs[mm];
+                        const float Ep = (float)Ep_d;
                         const float Ep3 = Ep * Ep * Ep;
-                        float I_E = Ep3 / (expf(Ep / (float)kT) - 1.0f);
-                        if (m.correct) I_E -= Ep3 / (expf(Ep / (float)kT_else) - 1.0f);
-                        v = (R)(I_E * (float)BB_NORM * (float)geom);
+                        float I_E = Ep3 / (expf((float)(Ep_d / kT)) - 1.0f);
+                        if (m.correct) I_E -= Ep3 / (expf((float)(Ep_d / kT_else)) - 1.0f);
+                        v = (R)(I_E * (float)geom);
                     }
                 }
                 CO[mm * 64 + k] = v;
@@ -298,6 +317,7 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                 }
             }
             __syncthreads();
+            if (sizeof(R) == 4) { dtot += (double)DIR[tid] * a_ref; DIR[tid] = (R)0; }
             // sign-changing pieces are done: drop them from the product
             for (int q = tid; q < NI * 16; q += 256) {
                 const int mm = q >> 4, k = q & 15;
@@ -313,7 +333,7 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
             for (int idx = wave; idx < NI; idx += 4)
                 acc = Mfma<R>::step(CO[idx * 64 + lane], MOM[idx * 64 + lane], acc);
 #pragma unroll
-            for (int r = 0; r < 4; r++) tot[r] += (double)acc[r];
+            for (int r = 0; r < 4; r++) tot[r] += (double)acc[r] * a_ref;
         }
     }
     __syncthreads();
@@ -323,7 +343,9 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
     __syncthreads();
     const int k = tid >> 4, e = e0 + k;
     if (e >= N_E || pg >= N_P) return;
-    const double s = RED[tid] + RED[256 + tid] + RED[512 + tid] + RED[768 + tid] + (double)DIR[tid];
+    double s = RED[tid] + RED[256 + tid] + RED[512 + tid] + RED[768 + tid] +
+               (sizeof(R) == 4 ? dtot : (double)DIR[tid]);
+    if (sizeof(R) == 4 && ATM == 1) s *= BB_NORM;
     const double E = F.energies[e];
     double* fo = F.flux + (size_t)mi * N_E * N_P + (size_t)e * N_P + pg;
     if (gridDim.y == 1) *fo = s / (E * KEV);

@@ -102,13 +102,19 @@ __global__ void __launch_bounds__(256) k_fold_f64(FoldGemmArgs A)
     }
 }
 
-// Precision form: split-BF16, three products.  Each operand x (rounded to
-// FP32) is split into hi = bf16(x) and lo = bf16(x - hi); the product is
-// lo_M hi_X + hi_M lo_X + hi_M hi_X, accumulated in FP32
-// (v_mfma_f32_16x16x32_bf16: lane l holds M[c0 + l%16][j + 8 (l/16) + v] and
+// Precision form: split-BF16 with FP32 accumulation.  Every operand x
+// (rounded to FP32) is a sum of three bf16 terms, x_h = bf16(x),
+// x_m = bf16(x - x_h), x_l = bf16(x - x_h - x_m); the response matrix is split
+// once on the host (Mh, Mm, Ml), the signal here.  NT = 3 keeps the products
+// M_m S_h + M_h S_m + M_h S_h (the two-term split, relative error ~2^-17);
+// NT = 6 adds M_l S_h + M_m S_m + M_h S_l (~2^-23), smallest terms first.
+// v_mfma_f32_16x16x32_bf16: lane l holds M[c0 + l%16][j + 8 (l/16) + v] and
 // in[j + 8 (l/16) + v][k0 + l%16], v = 0..7; result element r is row
-// 4 (l/16) + r).  The dropped lo_M lo_X term is below 2^-16 relative.
-__global__ void __launch_bounds__(256) k_fold_bf16x3(FoldGemmArgs A)
+// 4 (l/16) + r.
+template <int NT>
+__global__ void __launch_bounds__(256) k_fold_bf16(FoldGemmArgs A, const __bf16* __restrict__ Mh,
+                                                   const __bf16* __restrict__ Mm,
+                                                   const __bf16* __restrict__ Ml)
 {
     typedef float f4 __attribute__((ext_vector_type(4)));
     typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
@@ -117,23 +123,33 @@ __global__ void __launch_bounds__(256) k_fold_bf16x3(FoldGemmArgs A)
     const int c = c0 + (lane & 15), k = k0 + (lane & 15), jl = 8 * (lane >> 4);
     const double* X = A.in + (size_t)reg * A.N_in * A.N_P;
     const bool c_ok = c < A.N_ch, k_ok = k < A.N_P;
+    const __bf16 zero = (__bf16)0.0f;
     f4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int j0 = 0; j0 < A.N_in; j0 += 32) {
-        bf8 ah, al, bh, bl;
+        bf8 ah, am, al, bh, bm, bl;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
             const int j = j0 + jl + v;
             const bool j_ok = j < A.N_in;
-            const float a = (c_ok && j_ok) ? (float)A.M[(size_t)c * A.ld_M + j] : 0.f;
+            const size_t mo = (size_t)c * A.ld_M + j;
+            ah[v] = (c_ok && j_ok) ? Mh[mo] : zero;
+            am[v] = (c_ok && j_ok) ? Mm[mo] : zero;
+            al[v] = (c_ok && j_ok) ? Ml[mo] : zero;
             const float b = (k_ok && j_ok) ? (float)X[(size_t)j * A.N_P + k] : 0.f;
-            const __bf16 a1 = (__bf16)a, b1 = (__bf16)b;
-            ah[v] = a1;
-            al[v] = (__bf16)(a - (float)a1);
+            const __bf16 b1 = (__bf16)b;
+            const float r1 = b - (float)b1;
+            const __bf16 b2 = (__bf16)r1;
             bh[v] = b1;
-            bl[v] = (__bf16)(b - (float)b1);
+            bm[v] = b2;
+            bl[v] = (__bf16)(r1 - (float)b2);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+        if (NT == 6) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
     }
     if (!k_ok) return;
@@ -144,6 +160,9 @@ __global__ void __launch_bounds__(256) k_fold_bf16x3(FoldGemmArgs A)
         if (cr < A.N_ch) out[(size_t)cr * A.N_P + k] = (double)acc[r];
     }
 }
+
+template __global__ void k_fold_bf16<3>(FoldGemmArgs, const __bf16*, const __bf16*, const __bf16*);
+template __global__ void k_fold_bf16<6>(FoldGemmArgs, const __bf16*, const __bf16*, const __bf16*);
 
 struct MargFn {
     const double* star;
