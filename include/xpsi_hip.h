@@ -223,6 +223,11 @@ int xh_set_fold_energy_chunk(int ec);
  * point by point, so the >0 clipping is kept). */
 int xh_set_fold_kernel(int kind);
 
+/* Tuning: phases per block of the moment-form fold in the precision form
+ * (16, default, or 64 with 512 threads).  Applies to pipelines created
+ * afterwards. */
+int xh_set_moment_tile(int phases);
+
 /* Replay each launch as a hipGraph captured once per batch size (requires a
  * non-default stream; timing mode launches eagerly). */
 int xh_pipeline_use_graphs(xh_pipeline* p, int on);

@@ -6,7 +6,8 @@
  * (construct_polar_cellMesh) and
  * xpsi/surface_radiation_field/effective_gravity_universal.pyx:10-71.
  * The reference's gsl_integration_cquad(epsrel 1e-8) calls are replaced by
- * QAG-GK61 at OR_MESH_EPSREL.
+ * QAG-GK61 at OR_MESH_EPSREL; boundary-cell integrals are split at their
+ * kinks (cell_breakpoints) before the rule is applied.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -245,6 +246,68 @@ static double pick_first(double *c0, double *c1, double *c2, double *c3,
     return -1.0;
 }
 
+/* colatitudes t where the circle of angular radius rad about (TH, azimuth 0)
+   meets the meridian at azimuth ph: A cos t + B sin t = C */
+static void circle_meridian(double TH, double rad, double ph, double *t)
+{
+    double A = cos(TH), B = sin(TH) * cos(ph), C = cos(rad);
+    double Rn = sqrt(A * A + B * B);
+    t[0] = t[1] = -1.0;
+    if (Rn == 0.0 || fabs(C) > Rn) return;
+    double d = atan2(B, A), w = acos(C / Rn);
+    t[0] = d - w;
+    t[1] = d + w;
+}
+
+/*
+ * The cell integrand has kinks where a region boundary crosses a cell edge
+ * and square-root ends at a circle's extreme colatitudes.  The integral is
+ * taken as the sum over the pieces between those points (an exact
+ * decomposition: the rule then converges on each piece instead of bisecting
+ * toward the kinks, so the result is accurate well below the tolerance).
+ */
+static int cell_breakpoints(double ta, double tb, double pa, double pb,
+                            double colat, double cedeR, double superR,
+                            double superAzi, double superColat, double *bp)
+{
+    int n = 0;
+    double t2[2], cand[14];
+    int nc = 0;
+    circle_meridian(colat, cedeR, pa, t2); cand[nc++] = t2[0]; cand[nc++] = t2[1];
+    circle_meridian(colat, cedeR, pb, t2); cand[nc++] = t2[0]; cand[nc++] = t2[1];
+    cand[nc++] = colat - cedeR; cand[nc++] = colat + cedeR; cand[nc++] = cedeR - colat;
+    if (superR > 0.0) {
+        circle_meridian(superColat, superR, pa - superAzi, t2); cand[nc++] = t2[0]; cand[nc++] = t2[1];
+        circle_meridian(superColat, superR, pb - superAzi, t2); cand[nc++] = t2[0]; cand[nc++] = t2[1];
+        cand[nc++] = superColat - superR; cand[nc++] = superColat + superR;
+        cand[nc++] = superR - superColat;
+    }
+    for (int k = 0; k < nc; k++)
+        if (cand[k] > ta && cand[k] < tb) bp[n++] = cand[k];
+    for (int q = 1; q < n; q++) {
+        double v = bp[q];
+        int r = q - 1;
+        while (r >= 0 && bp[r] > v) { bp[r + 1] = bp[r]; r--; }
+        bp[r + 1] = v;
+    }
+    return n;
+}
+
+static double quad_pieces(region_par *p, double lo, double hi, const double *bp, int nbp)
+{
+    double total = 0.0, a = lo;
+    for (int k = 0; k <= nbp; k++) {
+        double b = hi;
+        if (k < nbp) {
+            if (!(bp[k] > a && bp[k] < hi)) continue;
+            b = bp[k];
+        }
+        total += mesh_quad(region_integrand, p, a, b);
+        a = b;
+    }
+    return total;
+}
+
 /* mesh_tools.pyx:429-725 */
 static double integrate_cell(double ta, double tb, double pa, double pb,
                              double R_eq, double eps, double zeta, int shape,
@@ -253,7 +316,9 @@ static double integrate_cell(double ta, double tb, double pa, double pb,
 {
     region_par p = {eps, zeta, colat, cedeR, superR, superAzi, superColat,
                     pa, pb, R_eq, 0.0, shape, 0, 1};
-    double integral = mesh_quad(region_integrand, &p, ta, tb);
+    double bp[14];
+    int nbp = cell_breakpoints(ta, tb, pa, pb, colat, cedeR, superR, superAzi, superColat, bp);
+    double integral = quad_pieces(&p, ta, tb, bp, nbp);
     if (eq12(integral, 0.0)) {
         double iv[2] = {ta, tb};
         double tl0 = theta_given_phi(pa, colat, cedeR, iv);
@@ -280,7 +345,7 @@ static double integrate_cell(double ta, double tb, double pa, double pb,
                 if (th[i] > tmax) tmax = th[i];
             }
         }
-        if (tmin < tmax) integral = mesh_quad(region_integrand, &p, tmin, tmax);
+        if (tmin < tmax) integral = quad_pieces(&p, tmin, tmax, bp, nbp);
     }
     return R_eq * R_eq * integral;
 }

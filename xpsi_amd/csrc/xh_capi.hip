@@ -125,43 +125,77 @@ extern "C" int xh_set_fold_kernel(int kind)
     return XH_OK;
 }
 
+// moment-form tile: PC phases per block (64 with 512 threads, else 16 with 256)
+struct MomentShape { int pc, nt; };
+
 // dynamic LDS of k_moment_fold (mirrors its carve-up; the moment area also
-// holds the 4 x 256 FP64 cross-wave reduction at the end)
-static size_t moment_lds(int N_L, int S_max, size_t rb)
+// holds the per-wave FP64 tiles of the cross-wave reduction at the end)
+static size_t moment_lds(int N_L, int S_max, size_t rb, MomentShape sh)
 {
-    const size_t NI = (size_t)N_L - 1;
-    size_t head = (5 * (size_t)N_L + 2 * (size_t)S_max + 8) * 8 + 256 * rb + (size_t)((N_L + 1) & ~1) * 4;
-    return head + (size_t)N_L * 64 * rb + std::max(NI * 64 * rb, (size_t)4 * 256 * 8);
+    const size_t NI = (size_t)N_L - 1, pc = (size_t)sh.pc, nw = (size_t)sh.nt / 64;
+    size_t head = (5 * (size_t)N_L + 2 * (size_t)S_max + 8) * 8 + 16 * pc * 8 + 16 * pc * rb +
+                  (size_t)((N_L + 1) & ~1) * 4;
+    return head + (size_t)N_L * 64 * rb + std::max(NI * 4 * pc * rb, nw * 256 * 8);
 }
 
-static dim3 moment_grid(int N_E, int N_P, int ring_blocks, int members)
+// phase tile of the moment form: 16 phases per block of 256 threads.  The
+// 64-phase, 512-thread tile (FP32 only) shares the per-parallel set-up four
+// ways but holds one block per CU; measured slower on the batched workload
+// (274 vs 241 ms per step), it is used only when selected with
+// xh_set_moment_tile(64).
+static int g_moment_pc = 16;
+
+extern "C" int xh_set_moment_tile(int phases)
 {
-    return dim3((unsigned)(((N_E + 15) / 16) * ((N_P + 15) / 16)), (unsigned)ring_blocks, (unsigned)members);
+    if (phases != 16 && phases != 64) return fail(XH_EINVAL, "moment tile must be 16 or 64 phases");
+    g_moment_pc = phases;
+    return XH_OK;
 }
 
-template <typename R, int ATM>
+static MomentShape moment_shape(bool fp32, int N_L, int S_max)
+{
+    if (fp32 && g_moment_pc == 64 && moment_lds(N_L, S_max, 4, MomentShape{64, 512}) <= kMomentLdsMax)
+        return MomentShape{64, 512};
+    return MomentShape{16, 256};
+}
+
+static dim3 moment_grid(int N_E, int N_P, int pc, int ring_blocks, int members)
+{
+    return dim3((unsigned)(((N_E + 15) / 16) * ((N_P + pc - 1) / pc)), (unsigned)ring_blocks, (unsigned)members);
+}
+
+template <typename R, int ATM, int PC, int NT>
 static hipError_t launch_moment_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)&k_moment_fold<R, ATM>,
+        hipError_t e = hipFuncSetAttribute((const void*)&k_moment_fold<R, ATM, PC, NT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMomentLdsMax);
         if (e != hipSuccess) return e;
         attr = true;
     }
     if (g.x == 0) return hipSuccess;     // configuration only
-    hipLaunchKernelGGL((k_moment_fold<R, ATM>), g, dim3(256), lds, s, m, L, F);
+    hipLaunchKernelGGL((k_moment_fold<R, ATM, PC, NT>), g, dim3(NT), lds, s, m, L, F);
     return hipGetLastError();
 }
 
-static hipError_t launch_moment(bool fp32, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+template <int ATM>
+static hipError_t launch_moment_a(bool fp32, MomentShape sh, dim3 g, size_t lds, hipStream_t s, const Member* m,
+                                  Layout L, FoldArgs F)
+{
+    if (!fp32) return launch_moment_t<double, ATM, 16, 256>(g, lds, s, m, L, F);
+    if (sh.pc == 64) return launch_moment_t<float, ATM, 64, 512>(g, lds, s, m, L, F);
+    return launch_moment_t<float, ATM, 16, 256>(g, lds, s, m, L, F);
+}
+
+static hipError_t launch_moment(bool fp32, MomentShape sh, dim3 g, size_t lds, hipStream_t s, const Member* m,
+                                Layout L, FoldArgs F)
 {
     if (lds > kMomentLdsMax) return hipErrorInvalidValue;
-    if (fp32) return L.atm_ext == 2 ? launch_moment_t<float, 2>(g, lds, s, m, L, F)
-                                    : launch_moment_t<float, 1>(g, lds, s, m, L, F);
-    return L.atm_ext == 2 ? launch_moment_t<double, 2>(g, lds, s, m, L, F)
-                          : launch_moment_t<double, 1>(g, lds, s, m, L, F);
+    return L.atm_ext == 2 ? launch_moment_a<2>(fp32, sh, g, lds, s, m, L, F)
+                          : launch_moment_a<1>(fp32, sh, g, lds, s, m, L, F);
 }
+
 
 // dynamic LDS of k_else_integrate: ray tables, alternative tables, slots
 static size_t else_lds(const Layout& Le)
@@ -213,8 +247,9 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
         return fail(XH_EINVAL, "xh_integrate: the 4-D table atmosphere needs its axes and values");
     if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1) return fail(XH_EINVAL, "xh_integrate: bad sizes");
     const int EC = pick_ec(N_L);
-    const bool moment = g_fold_kernel == 2 || (g_fold_kernel == 0 && moment_lds(N_L, S, 8) <= kMomentLdsMax);
-    if (g_fold_kernel == 2 && moment_lds(N_L, S, 8) > kMomentLdsMax)
+    const MomentShape msh{16, 256};
+    const bool moment = g_fold_kernel == 2 || (g_fold_kernel == 0 && moment_lds(N_L, S, 8, msh) <= kMomentLdsMax);
+    if (g_fold_kernel == 2 && moment_lds(N_L, S, 8, msh) > kMomentLdsMax)
         return fail(XH_EINVAL, "xh_integrate: the moment-form fold does not fit LDS at this leaf count");
     if (!moment && N_P > (256 / EC) * 16) return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
     if (workspace_bytes < xh_integrate_workspace_bytes(S, N_R, N_L, N_E, N_P))
@@ -257,8 +292,8 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
         const int nrb = std::min(S, 8), rb = (S + nrb - 1) / nrb;
         FA.ring_block = rb;
         HIPCHK(fill(flux, (size_t)N_E * N_P, 0.0, s));
-        HIPCHK(launch_moment(false, dim3(0, 1, 1), 0, s, d_m, L, FA));
-        HIPCHK(launch_moment(false, moment_grid(N_E, N_P, (S + rb - 1) / rb, 1), moment_lds(N_L, S, 8), s, d_m, L,
+        HIPCHK(launch_moment(false, msh, dim3(0, 1, 1), 0, s, d_m, L, FA));
+        HIPCHK(launch_moment(false, msh, moment_grid(N_E, N_P, msh.pc, (S + rb - 1) / rb, 1), moment_lds(N_L, S, 8, msh), s, d_m, L,
                              FA));
     } else {
         HIPCHK(launch_fold(EC, dim3((N_E + EC - 1) / EC, 1, 1), fold_lds(N_L, EC, S), s, d_m, L, FA));
@@ -400,6 +435,7 @@ struct xh_pipeline {
     Layout L;
     int nreg, mpr, maxb, N_in, N_ch, N_b, EC, table_n;
     int fold_kind;               // 1 per-point fold, 2 moment form
+    MomentShape msh;             // moment-form tile
     bool fp32;                   // precision form
     int fold_terms;              // bf16 products of the precision form's response fold
     __bf16 *d_Mh, *d_Mm, *d_Ml;  // three-term bf16 split of the response (precision form)
@@ -496,7 +532,8 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         fail(XH_EINVAL, "pipeline: fold_terms must be 0, 3 or 6"); delete p; return nullptr;
     }
     {
-        const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8) <= kMomentLdsMax;
+        p->msh = moment_shape(p->fp32, c->num_leaves, smax);
+        const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->msh) <= kMomentLdsMax;
         if ((c->fold_kernel == 2 || p->fp32) && !fits) {
             fail(XH_EINVAL, "pipeline: the moment-form fold does not fit LDS at this leaf count and mesh bound");
             delete p; return nullptr;
@@ -512,7 +549,7 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         L0.N_P = c->num_phases > 0 ? c->num_phases : c->num_leaves;
         L0.atm_ext = c->atm_ext;
         FoldArgs F0{};
-        if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
+        if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, p->msh, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
         else PHIP(launch_fold(p->EC, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
     }
     if (p->fold_kind == 1 && N_P > (256 / p->EC) * 16) { fail(XH_EINVAL, "too many phases for the leaf count"); delete p; return nullptr; }
@@ -836,8 +873,8 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
                 p->d_phases, p->d_flux, rb};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
     if (p->fold_kind == 2)
-        HIPCHK(launch_moment(p->fp32, moment_grid(N_E, N_P, (S + rb - 1) / rb, nm),
-                             moment_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8), s, p->d_members, L, FA));
+        HIPCHK(launch_moment(p->fp32, p->msh, moment_grid(N_E, N_P, p->msh.pc, (S + rb - 1) / rb, nm),
+                             moment_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8, p->msh), s, p->d_members, L, FA));
     else
         HIPCHK(launch_fold(p->EC, dim3((N_E + p->EC - 1) / p->EC, (S + rb - 1) / rb, nm), fold_lds(L.N_L, p->EC, S),
                            s, p->d_members, L, FA));

@@ -109,7 +109,7 @@ __global__ void k_else_add(const Member* else_members, Layout Le, const double* 
                            Member* hot_members, int members_per_request, double* flux);
 template <int EC, int MAXK, int ATM>
 __global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
-template <typename R, int ATM>
+template <typename R, int ATM, int PC, int NT>
 __global__ void k_moment_fold(const Member* members, Layout L, FoldArgs F);
 __global__ void k_fill(double* p, size_t n, double v);
 __global__ void k_work_count(const Member* members, Layout L, const double* cellArea, const int* valid,

@@ -208,14 +208,69 @@ __device__ __forceinline__ double take_first(double& c0, double& c1, double& c2,
     return -1.0;
 }
 
+// colatitudes where the circle of angular radius rad about (colatitude TH,
+// azimuth 0) meets the meridian at azimuth ph: cos(rad) = cos TH cos t +
+// sin TH sin t cos ph, i.e. A cos t + B sin t = C
+__device__ __forceinline__ void circle_meridian(double TH, double rad, double ph, double* t)
+{
+    const double A = cos(TH), B = sin(TH) * cos(ph), C = cos(rad);
+    const double Rn = sqrt(A * A + B * B);
+    t[0] = t[1] = -1.0;
+    if (Rn == 0.0 || fabs(C) > Rn) return;
+    const double d = atan2(B, A), w = acos(C / Rn);
+    t[0] = d - w;
+    t[1] = d + w;
+}
+
+// Sum of QAG-GK61 integrals over [lo, hi] split at the sorted points bp in
+// (lo, hi): the cell integrand's kinks (where a region boundary crosses a cell
+// edge) and square-root ends (a circle's extreme colatitudes) become piece
+// ends, where the rule converges in a pass or two instead of bisecting toward
+// them.  Any split is exact; a missed point only costs iterations.
+template <class Fn>
+__device__ double wave_pieces(const Fn& f, double lo, double hi, const double* bp, int nbp)
+{
+    double total = 0.0, a = lo;
+    for (int k = 0; k <= nbp; k++) {
+        double b = hi;
+        if (k < nbp) {
+            if (!(bp[k] > a && bp[k] < hi)) continue;
+            b = bp[k];
+        }
+        double r, e;
+        wave_qag61(f, a, b, 0.0, MESH_EPSREL, QAG_LIMIT, r, e);
+        total += r;
+        a = b;
+    }
+    return total;
+}
+
 // mesh_tools.pyx:429-725 (wave-cooperative)
 __device__ double wave_cell(double ta, double tb, double pa, double pb, double R_eq, double eps,
                             double zeta, int shape, double colat, double cedeR, double superR,
                             double superAzi, double superColat)
 {
     RegionFn f{eps, zeta, colat, cedeR, superR, superAzi, superColat, pa, pb, R_eq, 0.0, shape, 0, 1};
-    double integral, e;
-    wave_qag61(f, ta, tb, 0.0, MESH_EPSREL, QAG_LIMIT, integral, e);
+    // candidate breakpoints, sorted
+    double bp[16];
+    int nbp = 0;
+    auto add = [&](double t) { if (t > ta && t < tb && nbp < 16) bp[nbp++] = t; };
+    double t2[2];
+    circle_meridian(colat, cedeR, pa, t2); add(t2[0]); add(t2[1]);
+    circle_meridian(colat, cedeR, pb, t2); add(t2[0]); add(t2[1]);
+    add(colat - cedeR); add(colat + cedeR); add(cedeR - colat);
+    if (superR > 0.0) {
+        circle_meridian(superColat, superR, pa - superAzi, t2); add(t2[0]); add(t2[1]);
+        circle_meridian(superColat, superR, pb - superAzi, t2); add(t2[0]); add(t2[1]);
+        add(superColat - superR); add(superColat + superR); add(superR - superColat);
+    }
+    for (int q = 1; q < nbp; q++) {
+        double v = bp[q];
+        int r = q - 1;
+        while (r >= 0 && bp[r] > v) { bp[r + 1] = bp[r]; r--; }
+        bp[r + 1] = v;
+    }
+    double integral = wave_pieces(f, ta, tb, bp, nbp), e;
     if (eq12(integral, 0.0)) {
         double tl0 = theta_given_phi(pa, colat, cedeR, ta, tb);
         double tl1 = theta_given_phi(pa - superAzi, superColat, superR, ta, tb);
@@ -237,8 +292,9 @@ __device__ double wave_cell(double ta, double tb, double pa, double pb, double R
                 if (ts[q] > tmax) tmax = ts[q];
             }
         }
-        if (tmin < tmax) wave_qag61(f, tmin, tmax, 0.0, MESH_EPSREL, QAG_LIMIT, integral, e);
+        if (tmin < tmax) integral = wave_pieces(f, tmin, tmax, bp, nbp);
     }
+    (void)e;
     return R_eq * R_eq * integral;
 }
 

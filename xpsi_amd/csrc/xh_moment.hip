@@ -90,48 +90,40 @@ __device__ __forceinline__ void akima_piece_r(R mm2, R mm1, R m0, R mp1, R mp2, 
     d = (b + tn - two * m0) / (h * h);
 }
 
-// Sign class of f(t) = a + t (b + t (c + t d)) on [0, h]:
-// 1 = positive at every point by more than the evaluation error,
-// 0 = identically zero or negative everywhere by more than that error,
-// 2 = otherwise (evaluated point by point).
+// Sign class of f(t) = a + t (b + t (c + t d)) on [0, h] from its cubic
+// Bernstein coefficients (the curve lies in their convex hull):
+// 1 = every Bernstein coefficient above the evaluation-error margin, so f > 0
+//     at every point and the clipping cannot act;
+// 0 = identically zero, or every coefficient below minus the margin (f < 0);
+// 2 = otherwise, evaluated point by point with the clipping (conservative: a
+//     positive piece whose hull dips below the margin is evaluated exactly).
 template <typename R>
 __device__ __forceinline__ int cubic_class(R a, R b, R c, R d, R h)
 {
     if (a == (R)0 && b == (R)0 && c == (R)0 && d == (R)0) return 0;
-    const R scale = fabs(a) + h * (fabs(b) + h * (fabs(c) + h * fabs(d)));
+    const R bh = b * h, ch = c * h * h, dh = d * h * h * h;
+    const R scale = fabs(a) + fabs(bh) + fabs(ch) + fabs(dh);
     const R tol = (R)64 * Eps<R>::v * scale;
-    auto f = [&](R t) { return a + t * (b + t * (c + t * d)); };
-    R fh = f(h);
-    R lo = fmin(a, fh), hi = fmax(a, fh);
-    // stationary points: 3 d t^2 + 2 c t + b = 0 (numerically stable roots)
-    R t1 = (R)-1, t2 = (R)-1;
-    if (d != (R)0) {
-        R disc = c * c - (R)3 * b * d;
-        if (disc >= (R)0) {
-            R q = -(c + copysign(sqrt(disc), c));
-            if (q != (R)0) { t1 = q / ((R)3 * d); t2 = b / q; }
-            else t1 = (R)0;
-        }
-    } else if (c != (R)0) {
-        t1 = -b / ((R)2 * c);
-    }
-    if (t1 > (R)0 && t1 < h) { R v = f(t1); lo = fmin(lo, v); hi = fmax(hi, v); }
-    if (t2 > (R)0 && t2 < h) { R v = f(t2); lo = fmin(lo, v); hi = fmax(hi, v); }
+    const R b0 = a, b1 = a + bh / (R)3, b2 = a + (R)2 * bh / (R)3 + ch / (R)3, b3 = a + bh + ch + dh;
+    const R lo = fmin(fmin(b0, b1), fmin(b2, b3)), hi = fmax(fmax(b0, b1), fmax(b2, b3));
     if (lo > tol) return 1;
     if (hi < -tol) return 0;
     return 2;
 }
 
-template <typename R, int ATM>
-__global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
+// PC phases (16 or 64) and NT threads per block; the waves split the PC / 16
+// phase tiles and, within a tile, the intervals round-robin.
+template <typename R, int ATM, int PC, int NT>
+__global__ void __launch_bounds__(NT) k_moment_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
 {
+    constexpr int NW = NT / 64, NTILE = PC / 16, WPT = NW / NTILE, NPART = NT / PC;
     extern __shared__ __align__(16) double smd[];
     const int mi = blockIdx.z, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Member m = members[mi];
     if (!m.active || m.status) return;
     const int S = m.S, N_L = L.N_L, N_P = L.N_P, N_E = L.N_E, NI = N_L - 1;
     const int n_ec = (N_E + 15) >> 4;
-    const int e0 = (blockIdx.x % n_ec) * 16, p0 = (blockIdx.x / n_ec) * 16;
+    const int e0 = (blockIdx.x % n_ec) * 16, p0 = (blockIdx.x / n_ec) * PC;
     const int r0 = blockIdx.y * F.ring_block;
     if (r0 >= S) return;
     const int r1 = min(S, r0 + F.ring_block);
@@ -143,11 +135,12 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
     double* PV = Gs + N_L;                  // [N_L] previous queried leaf
     double* cphi = PV + N_L;                // [S_max]
     double* carea = cphi + L.S_max;         // [S_max], NaN = not radiating
-    double* ST = carea + L.S_max;           // [8] previous table query
-    R* DIR = (R*)(ST + 8);                  // [16][16] sign-changing pieces, direct sums
-    unsigned* MIX = (unsigned*)(DIR + 256); // [N_L (even)] energy bits of sign-changing pieces
+    double* ST = carea + L.S_max;           // [8] previous table query; area slots
+    double* D64 = ST + 8;                   // [16][PC] direct sums, FP64
+    R* DIR = (R*)(D64 + 16 * PC);           // [16][PC] direct sums of one parallel
+    unsigned* MIX = (unsigned*)(DIR + 16 * PC);   // [N_L (even)] energy bits of sign-changing pieces
     R* CO = (R*)(MIX + ((N_L + 1) & ~1));   // [N_L][4][16] coefficient m of energy k
-    R* MOM = CO + N_L * 64;                 // [NI][4][16] moment m at phase p
+    R* MOM = CO + N_L * 64;                 // [NI][4][PC] moment m at phase p
     R* SL = MOM;                            // [NI][16] secant slopes (before the moments)
 
     // FP32 form: cell areas enter the moments relative to the ring's largest
@@ -157,20 +150,20 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
     // the parallel's parity.
     unsigned long long* ABITS = (unsigned long long*)(ST + 6);
     if (tid < 2) ABITS[tid] = 0ull;
-    DIR[tid] = (R)0;
+    for (int q = tid; q < 16 * PC; q += NT) { DIR[q] = (R)0; D64[q] = 0.0; }
     double tot[4] = {0.0, 0.0, 0.0, 0.0};
-    double dtot = 0.0;                      // FP32 form: direct sums, scaled per parallel
     const size_t cbase = (size_t)mi * L.S_max * L.S_max;
-    const int p = tid & 15, part = tid >> 4;
+    const int p = tid % PC, part = tid / PC;
     const int pg = p0 + p;
     const double ph = pg < N_P ? F.phases[pg] : 0.0;
-    const int ja = (part * S) >> 4, jb = ((part + 1) * S) >> 4;
+    const int ja = (part * S) / NPART, jb = ((part + 1) * S) / NPART;
+    const int tile = wave % NTILE, sub = wave / NTILE;
 
     for (int i = r0; i < r1; i++) {
         const size_t ro = (size_t)mi * L.S_max + i;
         __syncthreads();   // the previous parallel is done with the cell arrays
         if (tid == 0) ABITS[(i + 1) & 1] = 0ull;
-        for (int j = tid; j < S; j += 256) {
+        for (int j = tid; j < S; j += NT) {
             size_t c = cbase + (size_t)i * S + j;
             double a = F.cellArea[c];
             bool rad = F.radiates ? (F.radiates[c] == 1) : (a > 0.0);
@@ -181,12 +174,13 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
         }
         const double kT = K_B_OVER_KEV * pow(10.0, F.logT[ro]);
         const double kT_else = K_B_OVER_KEV * pow(10.0, m.else_logT);
+        const double ikT = 1.0 / kT, ikT_else = 1.0 / kT_else;
         for (int I = 0; I < L.IO_max; I++) {
             const size_t vix = ro * L.IO_max + I;
             if (!F.valid[vix]) break;
             __syncthreads();   // the previous image's MFMA sweep is done with CO/MOM
             const double* g = F.geo + vix * 5 * N_L;
-            for (int q = tid; q < N_L; q += 256) {
+            for (int q = tid; q < N_L; q += NT) {
                 PH[q] = g[q]; Zs[q] = g[N_L + q]; As[q] = g[2 * N_L + q]; Gs[q] = g[3 * N_L + q];
                 PV[q] = g[4 * N_L + q];
                 MIX[q] = 0u;
@@ -194,10 +188,10 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
             if (ATM == 2 && tid == 0) prev_query_state(F, L, mi, i, I, ST);
             const double a_ref = (sizeof(R) == 4 && ABITS[i & 1]) ? __longlong_as_double((long long)ABITS[i & 1]) : 1.0;
             if (sizeof(R) == 4 && I == 0)
-                for (int j = tid; j < S; j += 256) carea[j] = carea[j] / a_ref;
+                for (int j = tid; j < S; j += NT) carea[j] = carea[j] / a_ref;
             __syncthreads();
             // profile: specific intensity x geometry, zero where the leaf is hidden
-            for (int q = tid; q < N_L * 16; q += 256) {
+            for (int q = tid; q < N_L * 16; q += NT) {
                 const int mm = q >> 4, k = q & 15, e = e0 + k;
                 R v = (R)0;
                 const double geom = Gs[mm];
@@ -235,8 +229,8 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                         const double Ep_d = F.energies[e] / Zs[mm];
                         const float Ep = (float)Ep_d;
                         const float Ep3 = Ep * Ep * Ep;
-                        float I_E = Ep3 / (expf((float)(Ep_d / kT)) - 1.0f);
-                        if (m.correct) I_E -= Ep3 / (expf((float)(Ep_d / kT_else)) - 1.0f);
+                        float I_E = Ep3 / (expf((float)(Ep_d * ikT)) - 1.0f);
+                        if (m.correct) I_E -= Ep3 / (expf((float)(Ep_d * ikT_else)) - 1.0f);
                         v = (R)(I_E * (float)geom);
                     }
                 }
@@ -246,13 +240,13 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
             if (tid < 16 && PV[N_L - 1] < -1.5) CO[(N_L - 1) * 64 + tid] = CO[tid];
             __syncthreads();
             // secant slopes
-            for (int q = tid; q < NI * 16; q += 256) {
+            for (int q = tid; q < NI * 16; q += NT) {
                 const int mm = q >> 4, k = q & 15;
                 SL[q] = (CO[(mm + 1) * 64 + k] - CO[mm * 64 + k]) / (R)(PH[mm + 1] - PH[mm]);
             }
             __syncthreads();
             // periodic Akima coefficients and the sign class of each piece
-            for (int q = tid; q < NI * 16; q += 256) {
+            for (int q = tid; q < NI * 16; q += NT) {
                 const int mm = q >> 4, k = q & 15;
                 auto sl = [&](int kk) {
                     kk = kk < 0 ? kk + NI : (kk >= NI ? kk - NI : kk);
@@ -271,7 +265,7 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                 CO[mm * 64 + 48 + k] = d;
             }
             __syncthreads();
-            for (int q = tid; q < NI * 64; q += 256) MOM[q] = (R)0;
+            for (int q = tid; q < NI * 4 * PC; q += NT) MOM[q] = (R)0;
             __syncthreads();
             // moments of the (cell, phase) queries; sign-changing pieces directly
             if (pg < N_P) {
@@ -290,10 +284,11 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                     const R t = (R)(x - PH[id]), ar = (R)area;
                     if (id != cur) {
                         if (cur >= 0) {
-                            atomicAdd(&MOM[cur * 64 + p], m0);
-                            atomicAdd(&MOM[cur * 64 + 16 + p], m1);
-                            atomicAdd(&MOM[cur * 64 + 32 + p], m2);
-                            atomicAdd(&MOM[cur * 64 + 48 + p], m3);
+                            R* mo = MOM + cur * 4 * PC + p;
+                            atomicAdd(mo, m0);
+                            atomicAdd(mo + PC, m1);
+                            atomicAdd(mo + 2 * PC, m2);
+                            atomicAdd(mo + 3 * PC, m3);
                         }
                         cur = id;
                         m0 = (R)0; m1 = (R)0; m2 = (R)0; m3 = (R)0;
@@ -306,20 +301,21 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
                         mx &= mx - 1u;
                         const R* co = CO + id * 64 + k;
                         const R v = co[0] + t * (co[16] + t * (co[32] + t * co[48]));
-                        if (v > (R)0) atomicAdd(&DIR[k * 16 + p], ar * v);
+                        if (v > (R)0) atomicAdd(&DIR[k * PC + p], ar * v);
                     }
                 }
                 if (cur >= 0) {
-                    atomicAdd(&MOM[cur * 64 + p], m0);
-                    atomicAdd(&MOM[cur * 64 + 16 + p], m1);
-                    atomicAdd(&MOM[cur * 64 + 32 + p], m2);
-                    atomicAdd(&MOM[cur * 64 + 48 + p], m3);
+                    R* mo = MOM + cur * 4 * PC + p;
+                    atomicAdd(mo, m0);
+                    atomicAdd(mo + PC, m1);
+                    atomicAdd(mo + 2 * PC, m2);
+                    atomicAdd(mo + 3 * PC, m3);
                 }
             }
             __syncthreads();
-            if (sizeof(R) == 4) { dtot += (double)DIR[tid] * a_ref; DIR[tid] = (R)0; }
+            for (int q = tid; q < 16 * PC; q += NT) { D64[q] += (double)DIR[q] * a_ref; DIR[q] = (R)0; }
             // sign-changing pieces are done: drop them from the product
-            for (int q = tid; q < NI * 16; q += 256) {
+            for (int q = tid; q < NI * 16; q += NT) {
                 const int mm = q >> 4, k = q & 15;
                 if ((MIX[mm] >> k) & 1u) {
                     CO[mm * 64 + k] = (R)0;
@@ -330,31 +326,37 @@ __global__ void __launch_bounds__(256, 2) k_moment_fold(const Member* __restrict
             }
             __syncthreads();
             typename Mfma<R>::v4 acc = {(R)0, (R)0, (R)0, (R)0};
-            for (int idx = wave; idx < NI; idx += 4)
-                acc = Mfma<R>::step(CO[idx * 64 + lane], MOM[idx * 64 + lane], acc);
+            const R* mb = MOM + (lane >> 4) * PC + tile * 16 + (lane & 15);
+            for (int idx = sub; idx < NI; idx += WPT)
+                acc = Mfma<R>::step(CO[idx * 64 + lane], mb[idx * 4 * PC], acc);
 #pragma unroll
             for (int r = 0; r < 4; r++) tot[r] += (double)acc[r] * a_ref;
         }
     }
     __syncthreads();
-    double* RED = (double*)MOM;   // [4 waves][16][16]; the host sizes LDS to hold it
+    double* RED = (double*)MOM;   // [NW][16][16]; the host sizes LDS to hold it
 #pragma unroll
     for (int r = 0; r < 4; r++) RED[wave * 256 + Mfma<R>::row(lane, r) * 16 + (lane & 15)] = tot[r];
     __syncthreads();
-    const int k = tid >> 4, e = e0 + k;
-    if (e >= N_E || pg >= N_P) return;
-    double s = RED[tid] + RED[256 + tid] + RED[512 + tid] + RED[768 + tid] +
-               (sizeof(R) == 4 ? dtot : (double)DIR[tid]);
-    if (sizeof(R) == 4 && ATM == 1) s *= BB_NORM;
-    const double E = F.energies[e];
-    double* fo = F.flux + (size_t)mi * N_E * N_P + (size_t)e * N_P + pg;
-    if (gridDim.y == 1) *fo = s / (E * KEV);
-    else atomicAdd(fo, s * (1.0 / (E * KEV)));
+    for (int q = tid; q < 16 * PC; q += NT) {
+        const int k = q / PC, pp = q % PC, e = e0 + k, pq = p0 + pp;
+        if (e >= N_E || pq >= N_P) continue;
+        const int tl = pp >> 4;
+        double s = D64[q];
+        for (int w = 0; w < WPT; w++) s += RED[(tl + w * NTILE) * 256 + k * 16 + (pp & 15)];
+        if (sizeof(R) == 4 && ATM == 1) s *= BB_NORM;
+        const double E = F.energies[e];
+        double* fo = F.flux + (size_t)mi * N_E * N_P + (size_t)e * N_P + pq;
+        if (gridDim.y == 1) *fo = s / (E * KEV);
+        else atomicAdd(fo, s * (1.0 / (E * KEV)));
+    }
 }
 
-template __global__ void k_moment_fold<double, 1>(const Member*, Layout, FoldArgs);
-template __global__ void k_moment_fold<double, 2>(const Member*, Layout, FoldArgs);
-template __global__ void k_moment_fold<float, 1>(const Member*, Layout, FoldArgs);
-template __global__ void k_moment_fold<float, 2>(const Member*, Layout, FoldArgs);
+#define XH_MOMENT_INST(R, PC, NT)                                                              \
+    template __global__ void k_moment_fold<R, 1, PC, NT>(const Member*, Layout, FoldArgs);     \
+    template __global__ void k_moment_fold<R, 2, PC, NT>(const Member*, Layout, FoldArgs);
+XH_MOMENT_INST(double, 16, 256)
+XH_MOMENT_INST(float, 16, 256)
+XH_MOMENT_INST(float, 64, 512)
 
 }  // namespace xh
