@@ -89,6 +89,10 @@ def main():
         return {ne: LikelihoodPipeline(dict(configs.config3_model(ne), precision=prec), max_batch=len(g))
                 for ne, g in sorted(groups.items())}
 
+    # tile shape of the fold kernel, measured on the 128-energy group (untimed)
+    from xpsi_amd.pipeline import autotune_fold
+    tune_ne = 128 if 128 in groups else sorted(groups)[0]
+    tuned = autotune_fold(dict(configs.config3_model(tune_ne), precision=args.precision), groups[tune_ne], stream)
     pls = make(args.precision)
     res = {}
 
@@ -198,6 +202,7 @@ def main():
                      "unit": "TFLOP/s", "traffic": None,
                      "kernel": "k_moment_fold", "evals_per_step": evals,
                      "flop_per_eval": 8},
+        "autotune_fold_tiles": tuned,
         "device": dev,
     }
     out["roofline"]["frac"] = out["roofline"]["achieved"] / out["roofline"]["peak"]

@@ -223,10 +223,11 @@ int xh_set_fold_energy_chunk(int ec);
  * point by point, so the >0 clipping is kept). */
 int xh_set_fold_kernel(int kind);
 
-/* Tuning: phases per block of the moment-form fold in the precision form
- * (16, default, or 64 with 512 threads).  Applies to pipelines created
- * afterwards. */
-int xh_set_moment_tile(int phases);
+/* Tuning: 16-phase tiles per block of the moment-form fold (1, 2 or 4; more
+ * tiles share one profile and coefficient set per parallel but hold more
+ * registers).  Applies to pipelines created afterwards and to xh_integrate;
+ * xpsi_amd.pipeline.autotune_fold measures the choices. */
+int xh_set_moment_tiles(int tiles);
 
 /* Replay each launch as a hipGraph captured once per batch size (requires a
  * non-default stream; timing mode launches eagerly). */

@@ -114,6 +114,9 @@ static hipError_t launch_fold(int EC, dim3 g, size_t lds, hipStream_t s, const M
     return L.atm_ext == 2 ? launch_fold_a<2>(EC, g, lds, s, m, L, F) : launch_fold_a<1>(EC, g, lds, s, m, L, F);
 }
 
+// blocks of the wave-per-ray QAG kernel (grid-stride over its work list)
+constexpr int kRayWaveBlocks = 2048;
+
 // ----------------------------------------------- moment form (matrix cores)
 static int g_fold_kernel = 0;   // xh_integrate: 0 automatic, 1 per-point, 2 moment form
 static const size_t kMomentLdsMax = 160 * 1024;
@@ -125,75 +128,65 @@ extern "C" int xh_set_fold_kernel(int kind)
     return XH_OK;
 }
 
-// moment-form tile: PC phases per block (64 with 512 threads, else 16 with 256)
-struct MomentShape { int pc, nt; };
+// moment-form block: 16 energies x 16 TPB phases (TPB 16-phase tiles sharing
+// one profile and coefficient set per parallel); TPB is a tuning choice
+static int g_moment_tpb = 1;
 
-// dynamic LDS of k_moment_fold (mirrors its carve-up; the moment area also
-// holds the per-wave FP64 tiles of the cross-wave reduction at the end)
-static size_t moment_lds(int N_L, int S_max, size_t rb, MomentShape sh)
+extern "C" int xh_set_moment_tiles(int tiles)
 {
-    const size_t NI = (size_t)N_L - 1, pc = (size_t)sh.pc, nw = (size_t)sh.nt / 64;
-    size_t head = (5 * (size_t)N_L + 2 * (size_t)S_max + 8) * 8 + 16 * pc * 8 + 16 * pc * rb +
-                  (size_t)((N_L + 1) & ~1) * 4;
-    return head + (size_t)N_L * 64 * rb + std::max(NI * 4 * pc * rb, nw * 256 * 8);
-}
-
-// phase tile of the moment form: 16 phases per block of 256 threads.  The
-// 64-phase, 512-thread tile (FP32 only) shares the per-parallel set-up four
-// ways but holds one block per CU; measured slower on the batched workload
-// (274 vs 241 ms per step), it is used only when selected with
-// xh_set_moment_tile(64).
-static int g_moment_pc = 16;
-
-extern "C" int xh_set_moment_tile(int phases)
-{
-    if (phases != 16 && phases != 64) return fail(XH_EINVAL, "moment tile must be 16 or 64 phases");
-    g_moment_pc = phases;
+    if (tiles != 1 && tiles != 2 && tiles != 4) return fail(XH_EINVAL, "moment tiles per block must be 1, 2 or 4");
+    g_moment_tpb = tiles;
     return XH_OK;
 }
 
-static MomentShape moment_shape(bool fp32, int N_L, int S_max)
+// dynamic LDS of k_moment_fold (mirrors its carve-up; the moment area also
+// holds the 4 x 256 FP64 cross-wave reduction at the end)
+static size_t moment_lds(int N_L, int S_max, size_t rb, int tpb)
 {
-    if (fp32 && g_moment_pc == 64 && moment_lds(N_L, S_max, 4, MomentShape{64, 512}) <= kMomentLdsMax)
-        return MomentShape{64, 512};
-    return MomentShape{16, 256};
+    const size_t NI = (size_t)N_L - 1, pcb = 16 * (size_t)tpb;
+    size_t head = (5 * (size_t)N_L + 2 * (size_t)S_max + 8) * 8 + 16 * pcb * 8 + (rb == 4 ? 16 * pcb * 4 : 0) +
+                  (size_t)((N_L + 1) & ~1) * 4;
+    return head + (size_t)N_L * 64 * rb + std::max(NI * 64 * rb, (size_t)4 * 256 * 8);
 }
 
-static dim3 moment_grid(int N_E, int N_P, int pc, int ring_blocks, int members)
+static dim3 moment_grid(int N_E, int N_P, int tpb, int ring_blocks, int members)
 {
-    return dim3((unsigned)(((N_E + 15) / 16) * ((N_P + pc - 1) / pc)), (unsigned)ring_blocks, (unsigned)members);
+    const int pcb = 16 * tpb;
+    return dim3((unsigned)(((N_E + 15) / 16) * ((N_P + pcb - 1) / pcb)), (unsigned)ring_blocks, (unsigned)members);
 }
 
-template <typename R, int ATM, int PC, int NT>
+template <typename R, int ATM, int TPB>
 static hipError_t launch_moment_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)&k_moment_fold<R, ATM, PC, NT>,
+        hipError_t e = hipFuncSetAttribute((const void*)&k_moment_fold<R, ATM, TPB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMomentLdsMax);
         if (e != hipSuccess) return e;
         attr = true;
     }
     if (g.x == 0) return hipSuccess;     // configuration only
-    hipLaunchKernelGGL((k_moment_fold<R, ATM, PC, NT>), g, dim3(NT), lds, s, m, L, F);
+    hipLaunchKernelGGL((k_moment_fold<R, ATM, TPB>), g, dim3(256), lds, s, m, L, F);
     return hipGetLastError();
 }
 
-template <int ATM>
-static hipError_t launch_moment_a(bool fp32, MomentShape sh, dim3 g, size_t lds, hipStream_t s, const Member* m,
-                                  Layout L, FoldArgs F)
+template <typename R, int ATM>
+static hipError_t launch_moment_a(int tpb, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L,
+                                  FoldArgs F)
 {
-    if (!fp32) return launch_moment_t<double, ATM, 16, 256>(g, lds, s, m, L, F);
-    if (sh.pc == 64) return launch_moment_t<float, ATM, 64, 512>(g, lds, s, m, L, F);
-    return launch_moment_t<float, ATM, 16, 256>(g, lds, s, m, L, F);
+    if (tpb == 4) return launch_moment_t<R, ATM, 4>(g, lds, s, m, L, F);
+    if (tpb == 2) return launch_moment_t<R, ATM, 2>(g, lds, s, m, L, F);
+    return launch_moment_t<R, ATM, 1>(g, lds, s, m, L, F);
 }
 
-static hipError_t launch_moment(bool fp32, MomentShape sh, dim3 g, size_t lds, hipStream_t s, const Member* m,
-                                Layout L, FoldArgs F)
+static hipError_t launch_moment(bool fp32, int tpb, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L,
+                                FoldArgs F)
 {
     if (lds > kMomentLdsMax) return hipErrorInvalidValue;
-    return L.atm_ext == 2 ? launch_moment_a<2>(fp32, sh, g, lds, s, m, L, F)
-                          : launch_moment_a<1>(fp32, sh, g, lds, s, m, L, F);
+    if (fp32) return L.atm_ext == 2 ? launch_moment_a<float, 2>(tpb, g, lds, s, m, L, F)
+                                    : launch_moment_a<float, 1>(tpb, g, lds, s, m, L, F);
+    return L.atm_ext == 2 ? launch_moment_a<double, 2>(tpb, g, lds, s, m, L, F)
+                          : launch_moment_a<double, 1>(tpb, g, lds, s, m, L, F);
 }
 
 
@@ -247,9 +240,9 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
         return fail(XH_EINVAL, "xh_integrate: the 4-D table atmosphere needs its axes and values");
     if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1) return fail(XH_EINVAL, "xh_integrate: bad sizes");
     const int EC = pick_ec(N_L);
-    const MomentShape msh{16, 256};
-    const bool moment = g_fold_kernel == 2 || (g_fold_kernel == 0 && moment_lds(N_L, S, 8, msh) <= kMomentLdsMax);
-    if (g_fold_kernel == 2 && moment_lds(N_L, S, 8, msh) > kMomentLdsMax)
+    const int tpb = g_moment_tpb;
+    const bool moment = g_fold_kernel == 2 || (g_fold_kernel == 0 && moment_lds(N_L, S, 8, tpb) <= kMomentLdsMax);
+    if (g_fold_kernel == 2 && moment_lds(N_L, S, 8, tpb) > kMomentLdsMax)
         return fail(XH_EINVAL, "xh_integrate: the moment-form fold does not fit LDS at this leaf count");
     if (!moment && N_P > (256 / EC) * 16) return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
     if (workspace_bytes < xh_integrate_workspace_bytes(S, N_R, N_L, N_E, N_P))
@@ -292,8 +285,8 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
         const int nrb = std::min(S, 8), rb = (S + nrb - 1) / nrb;
         FA.ring_block = rb;
         HIPCHK(fill(flux, (size_t)N_E * N_P, 0.0, s));
-        HIPCHK(launch_moment(false, msh, dim3(0, 1, 1), 0, s, d_m, L, FA));
-        HIPCHK(launch_moment(false, msh, moment_grid(N_E, N_P, msh.pc, (S + rb - 1) / rb, 1), moment_lds(N_L, S, 8, msh), s, d_m, L,
+        HIPCHK(launch_moment(false, tpb, dim3(0, 1, 1), 0, s, d_m, L, FA));
+        HIPCHK(launch_moment(false, tpb, moment_grid(N_E, N_P, tpb, (S + rb - 1) / rb, 1), moment_lds(N_L, S, 8, tpb), s, d_m, L,
                              FA));
     } else {
         HIPCHK(launch_fold(EC, dim3((N_E + EC - 1) / EC, 1, 1), fold_lds(N_L, EC, S), s, d_m, L, FA));
@@ -435,7 +428,7 @@ struct xh_pipeline {
     Layout L;
     int nreg, mpr, maxb, N_in, N_ch, N_b, EC, table_n;
     int fold_kind;               // 1 per-point fold, 2 moment form
-    MomentShape msh;             // moment-form tile
+    int tpb;                     // moment form: 16-phase tiles per block
     bool fp32;                   // precision form
     int fold_terms;              // bf16 products of the precision form's response fold
     __bf16 *d_Mh, *d_Mm, *d_Ml;  // three-term bf16 split of the response (precision form)
@@ -450,6 +443,7 @@ struct xh_pipeline {
     double *d_area_tab, *d_total, *d_theta, *d_phi, *d_r, *d_cellArea, *d_maxAlpha, *d_cos_gamma, *d_effGrav;
     double *d_u, *d_logT, *d_maxDefl, *d_defl, *d_cosa, *d_lag, *d_geo, *d_flux;
     int *d_ring_ok, *d_valid, *d_member_index, *d_status, *d_lastq;
+    int *d_raywork, *d_raycount, *d_else_raywork;   // rays left to the full QAG, and their counts
     double *d_atm_axes, *d_atm_I;
     int atm_N[4];
     double *d_energies, *d_logE, *d_logEdges, *d_atten, *d_leaves, *d_phases, *d_comp_phases;
@@ -532,8 +526,9 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         fail(XH_EINVAL, "pipeline: fold_terms must be 0, 3 or 6"); delete p; return nullptr;
     }
     {
-        p->msh = moment_shape(p->fp32, c->num_leaves, smax);
-        const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->msh) <= kMomentLdsMax;
+        p->tpb = g_moment_tpb;
+        while (p->tpb > 1 && moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->tpb) > kMomentLdsMax) p->tpb >>= 1;
+        const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->tpb) <= kMomentLdsMax;
         if ((c->fold_kernel == 2 || p->fp32) && !fits) {
             fail(XH_EINVAL, "pipeline: the moment-form fold does not fit LDS at this leaf count and mesh bound");
             delete p; return nullptr;
@@ -549,7 +544,7 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         L0.N_P = c->num_phases > 0 ? c->num_phases : c->num_leaves;
         L0.atm_ext = c->atm_ext;
         FoldArgs F0{};
-        if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, p->msh, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
+        if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, p->tpb, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
         else PHIP(launch_fold(p->EC, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
     }
     if (p->fold_kind == 1 && N_P > (256 / p->EC) * 16) { fail(XH_EINVAL, "too many phases for the leaf count"); delete p; return nullptr; }
@@ -576,6 +571,8 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     PCHK(dalloc(p, &p->d_logT, nm * S));
     PCHK(dalloc(p, &p->d_maxDefl, nm * S));
     PCHK(dalloc(p, &p->d_ring_ok, nm * S));
+    PCHK(dalloc(p, &p->d_raywork, nm * S * NR));
+    PCHK(dalloc(p, &p->d_raycount, 2));
     PCHK(dalloc(p, &p->d_defl, nm * S * NR));
     PCHK(dalloc(p, &p->d_cosa, nm * S * NR));
     PCHK(dalloc(p, &p->d_lag, nm * S * NR));
@@ -642,6 +639,7 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         PCHK(dalloc(p, &p->d_else_lag, nb * Se * NRe));
         PCHK(dalloc(p, &p->d_else_maxD, nb * Se));
         PCHK(dalloc(p, &p->d_else_part, nb * Se * NE));
+        PCHK(dalloc(p, &p->d_else_raywork, nb * Se * NRe));
         PHIP(hipHostMalloc((void**)&p->h_else, nb * sizeof(Member), hipHostMallocDefault));
     }
     PHIP(hipHostMalloc((void**)&p->h_members, nm * sizeof(Member), hipHostMallocDefault));
@@ -841,15 +839,24 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
         HIPCHK(hipGetLastError());
     }
     if (p->timing) HIPCHK(hipEventRecord(p->ev[1], s));
-    hipLaunchKernelGGL(k_rays, dim3((L.N_R - 1 + 3) / 4, S, nm), dim3(256), 0, s, p->d_members, L, p->d_r,
-                       p->d_maxAlpha, p->d_cosa, p->d_defl, p->d_lag);
+    // rays: lane-per-ray first pass, then the listed rays with full QAG
+    HIPCHK(fill_int(p->d_raycount, 2, 0, s));
+    hipLaunchKernelGGL(k_rays_lane, dim3((L.N_R - 1 + 63) / 64, S, nm), dim3(64), 0, s, p->d_members, L, p->d_r,
+                       p->d_maxAlpha, p->d_cosa, p->d_defl, p->d_lag, p->d_raywork, p->d_raycount);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_rays, dim3(kRayWaveBlocks), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_maxAlpha,
+                       p->d_defl, p->d_lag, p->d_raywork, p->d_raycount);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_rays_fix, dim3(nm), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_maxAlpha,
                        p->d_defl, p->d_lag, p->d_maxDefl);
     HIPCHK(hipGetLastError());
     if (p->has_else) {
-        hipLaunchKernelGGL(k_rays, dim3((Le.N_R - 1 + 3) / 4, Le.S_max, n), dim3(256), 0, s, p->d_else, Le,
-                           p->d_else_r, p->d_else_maxA, p->d_else_cosa, p->d_else_defl, p->d_else_lag);
+        hipLaunchKernelGGL(k_rays_lane, dim3((Le.N_R - 1 + 63) / 64, Le.S_max, n), dim3(64), 0, s, p->d_else, Le,
+                           p->d_else_r, p->d_else_maxA, p->d_else_cosa, p->d_else_defl, p->d_else_lag,
+                           p->d_else_raywork, p->d_raycount + 1);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_rays, dim3(kRayWaveBlocks), dim3(256), 0, s, p->d_else, Le, p->d_else_r,
+                           p->d_else_maxA, p->d_else_defl, p->d_else_lag, p->d_else_raywork, p->d_raycount + 1);
         HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_rays_fix, dim3(n), dim3(256), 0, s, p->d_else, Le, p->d_else_r, p->d_else_maxA,
                            p->d_else_defl, p->d_else_lag, p->d_else_maxD);
@@ -873,8 +880,8 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
                 p->d_phases, p->d_flux, rb};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
     if (p->fold_kind == 2)
-        HIPCHK(launch_moment(p->fp32, p->msh, moment_grid(N_E, N_P, p->msh.pc, (S + rb - 1) / rb, nm),
-                             moment_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8, p->msh), s, p->d_members, L, FA));
+        HIPCHK(launch_moment(p->fp32, p->tpb, moment_grid(N_E, N_P, p->tpb, (S + rb - 1) / rb, nm),
+                             moment_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8, p->tpb), s, p->d_members, L, FA));
     else
         HIPCHK(launch_fold(p->EC, dim3((N_E + p->EC - 1) / p->EC, (S + rb - 1) / rb, nm), fold_lds(L.N_L, p->EC, S),
                            s, p->d_members, L, FA));

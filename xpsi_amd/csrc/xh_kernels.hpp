@@ -45,8 +45,10 @@ __global__ void k_mesh_table(const Member* members, double* area_tab, double* to
 __global__ void k_mesh_rings(const Member* members, const double* area_tab, const double* total,
                              int table_n, Layout L, double* theta, double* phi, double* rr,
                              double* cellArea, double* maxAlpha, double* cos_gamma, double* effGrav);
+__global__ void k_rays_lane(const Member* members, Layout L, const double* rr, const double* maxAlpha_in,
+                            double* cos_alpha, double* defl_raw, double* lag_raw, int* work, int* nwork);
 __global__ void k_rays(const Member* members, Layout L, const double* rr, const double* maxAlpha_in,
-                       double* cos_alpha, double* defl_raw, double* lag_raw);
+                       double* defl_raw, double* lag_raw, const int* work, const int* nwork);
 __global__ void k_rays_fix(Member* members, Layout L, const double* rr, double* maxAlpha,
                            double* defl, double* lag, double* maxDefl);
 __global__ void k_energy_integrate(const Member* members, EnergyArgs A);
@@ -109,7 +111,7 @@ __global__ void k_else_add(const Member* else_members, Layout Le, const double* 
                            Member* hot_members, int members_per_request, double* flux);
 template <int EC, int MAXK, int ATM>
 __global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
-template <typename R, int ATM, int PC, int NT>
+template <typename R, int ATM, int TPB>
 __global__ void k_moment_fold(const Member* members, Layout L, FoldArgs F);
 __global__ void k_fill(double* p, size_t n, double v);
 __global__ void k_work_count(const Member* members, Layout L, const double* cellArea, const int* valid,

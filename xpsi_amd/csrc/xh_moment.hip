@@ -20,13 +20,14 @@
 // with the clipping, exactly as the reference does; intervals where s_k <= 0
 // throughout contribute nothing.
 //
-// One block of four waves owns 16 energies x 16 phases of one member and
+// One block of four waves owns 16 energies x 64 phases of one member and
 // walks the parallels of its ring block.  Per parallel and image: profile
 // (intensity x geometry) into LDS, secant slopes, Akima coefficients with a
-// sign class per (interval, energy), moments from the (cell, phase) queries
-// by LDS atomics, direct evaluation of sign-changing intervals, then the MFMA
-// sweep, intervals dealt round-robin to the waves.  Each parallel's tile is
-// added into FP64 accumulators, so the float form sums parallels in FP64.
+// sign class per (interval, energy); then per 16-phase tile, moments from the
+// (cell, phase) queries by LDS atomics, direct evaluation of sign-changing
+// pieces, and the MFMA sweep, intervals dealt round-robin to the waves.  Each
+// parallel's tile is added into FP64 accumulators, so the float form sums
+// parallels in FP64.
 //
 // R = double: FP64 throughout (v_mfma_f64_16x16x4f64).
 // R = float: the precision form.  Blackbody intensity, spline coefficients,
@@ -111,36 +112,41 @@ __device__ __forceinline__ int cubic_class(R a, R b, R c, R d, R h)
     return 2;
 }
 
-// PC phases (16 or 64) and NT threads per block; the waves split the PC / 16
-// phase tiles and, within a tile, the intervals round-robin.
-template <typename R, int ATM, int PC, int NT>
-__global__ void __launch_bounds__(NT) k_moment_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
+// One block of four waves owns 16 energies x TPB * 16 phases of one member.
+// Per parallel and image it builds the profile, Akima coefficients and sign
+// classes once, then walks its phase tiles: moments of the tile's queries
+// (one 16-phase moment buffer in LDS), direct sums of the sign-changing
+// pieces, and the MFMA sweep with those pieces masked out of the A operand.
+template <typename R, int ATM, int TPB>
+__global__ void __launch_bounds__(256) k_moment_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
 {
-    constexpr int NW = NT / 64, NTILE = PC / 16, WPT = NW / NTILE, NPART = NT / PC;
+    constexpr int PCB = 16 * TPB;           // phases per block
     extern __shared__ __align__(16) double smd[];
     const int mi = blockIdx.z, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Member m = members[mi];
     if (!m.active || m.status) return;
     const int S = m.S, N_L = L.N_L, N_P = L.N_P, N_E = L.N_E, NI = N_L - 1;
     const int n_ec = (N_E + 15) >> 4;
-    const int e0 = (blockIdx.x % n_ec) * 16, p0 = (blockIdx.x / n_ec) * PC;
+    const int e0 = (blockIdx.x % n_ec) * 16, p0 = (blockIdx.x / n_ec) * PCB;
     const int r0 = blockIdx.y * F.ring_block;
     if (r0 >= S) return;
     const int r1 = min(S, r0 + F.ring_block);
+    const int ntile = min(TPB, (N_P - p0 + 15) >> 4);
 
     double* PH = smd;                       // [N_L] leaf phases
-    double* Zs = PH + N_L;                  // [N_L] redshift factor
-    double* As = Zs + N_L;                  // [N_L] mu * eta
+    double* Zi = PH + N_L;                  // [N_L] 1 / redshift factor
+    double* As = Zi + N_L;                  // [N_L] mu * eta
     double* Gs = As + N_L;                  // [N_L] geometric factor
     double* PV = Gs + N_L;                  // [N_L] previous queried leaf
     double* cphi = PV + N_L;                // [S_max]
     double* carea = cphi + L.S_max;         // [S_max], NaN = not radiating
     double* ST = carea + L.S_max;           // [8] previous table query; area slots
-    double* D64 = ST + 8;                   // [16][PC] direct sums, FP64
-    R* DIR = (R*)(D64 + 16 * PC);           // [16][PC] direct sums of one parallel
-    unsigned* MIX = (unsigned*)(DIR + 16 * PC);   // [N_L (even)] energy bits of sign-changing pieces
+    double* D64 = ST + 8;                   // [16][PCB] direct sums, FP64
+    // [16][PCB] direct sums of one parallel (the FP64 form sums into D64 itself)
+    R* DIR = sizeof(R) == 8 ? (R*)D64 : (R*)(D64 + 16 * PCB);
+    unsigned* MIX = (unsigned*)(DIR + 16 * PCB);  // [N_L (even)] energy bits of sign-changing pieces
     R* CO = (R*)(MIX + ((N_L + 1) & ~1));   // [N_L][4][16] coefficient m of energy k
-    R* MOM = CO + N_L * 64;                 // [NI][4][PC] moment m at phase p
+    R* MOM = CO + N_L * 64;                 // [NI][4][16] moment m at phase p of one tile
     R* SL = MOM;                            // [NI][16] secant slopes (before the moments)
 
     // FP32 form: cell areas enter the moments relative to the ring's largest
@@ -150,20 +156,21 @@ __global__ void __launch_bounds__(NT) k_moment_fold(const Member* __restrict__ m
     // the parallel's parity.
     unsigned long long* ABITS = (unsigned long long*)(ST + 6);
     if (tid < 2) ABITS[tid] = 0ull;
-    for (int q = tid; q < 16 * PC; q += NT) { DIR[q] = (R)0; D64[q] = 0.0; }
-    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = tid; q < 16 * PCB; q += 256) { DIR[q] = (R)0; D64[q] = 0.0; }   // (one array for FP64)
+    double tot[TPB][4];
+#pragma unroll
+    for (int t = 0; t < TPB; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) tot[t][r] = 0.0;
     const size_t cbase = (size_t)mi * L.S_max * L.S_max;
-    const int p = tid % PC, part = tid / PC;
-    const int pg = p0 + p;
-    const double ph = pg < N_P ? F.phases[pg] : 0.0;
-    const int ja = (part * S) / NPART, jb = ((part + 1) * S) / NPART;
-    const int tile = wave % NTILE, sub = wave / NTILE;
+    const int p = tid & 15, part = tid >> 4;
+    const int ja = (part * S) >> 4, jb = ((part + 1) * S) >> 4;
 
     for (int i = r0; i < r1; i++) {
         const size_t ro = (size_t)mi * L.S_max + i;
         __syncthreads();   // the previous parallel is done with the cell arrays
         if (tid == 0) ABITS[(i + 1) & 1] = 0ull;
-        for (int j = tid; j < S; j += NT) {
+        for (int j = tid; j < S; j += 256) {
             size_t c = cbase + (size_t)i * S + j;
             double a = F.cellArea[c];
             bool rad = F.radiates ? (F.radiates[c] == 1) : (a > 0.0);
@@ -174,41 +181,40 @@ __global__ void __launch_bounds__(NT) k_moment_fold(const Member* __restrict__ m
         }
         const double kT = K_B_OVER_KEV * pow(10.0, F.logT[ro]);
         const double kT_else = K_B_OVER_KEV * pow(10.0, m.else_logT);
-        const double ikT = 1.0 / kT, ikT_else = 1.0 / kT_else;
         for (int I = 0; I < L.IO_max; I++) {
             const size_t vix = ro * L.IO_max + I;
             if (!F.valid[vix]) break;
             __syncthreads();   // the previous image's MFMA sweep is done with CO/MOM
             const double* g = F.geo + vix * 5 * N_L;
-            for (int q = tid; q < N_L; q += NT) {
-                PH[q] = g[q]; Zs[q] = g[N_L + q]; As[q] = g[2 * N_L + q]; Gs[q] = g[3 * N_L + q];
+            for (int q = tid; q < N_L; q += 256) {
+                PH[q] = g[q]; Zi[q] = 1.0 / g[N_L + q]; As[q] = g[2 * N_L + q]; Gs[q] = g[3 * N_L + q];
                 PV[q] = g[4 * N_L + q];
                 MIX[q] = 0u;
             }
             if (ATM == 2 && tid == 0) prev_query_state(F, L, mi, i, I, ST);
             const double a_ref = (sizeof(R) == 4 && ABITS[i & 1]) ? __longlong_as_double((long long)ABITS[i & 1]) : 1.0;
             if (sizeof(R) == 4 && I == 0)
-                for (int j = tid; j < S; j += NT) carea[j] = carea[j] / a_ref;
+                for (int j = tid; j < S; j += 256) carea[j] = carea[j] / a_ref;
             __syncthreads();
             // profile: specific intensity x geometry, zero where the leaf is hidden
-            for (int q = tid; q < N_L * 16; q += NT) {
+            for (int q = tid; q < N_L * 16; q += 256) {
                 const int mm = q >> 4, k = q & 15, e = e0 + k;
                 R v = (R)0;
                 const double geom = Gs[mm];
                 if (geom != 0.0 && e < N_E && PV[mm] > -1.5) {
+                    const double Ep = F.energies[e] * Zi[mm];
                     if (ATM == 2) {
-                        double Ep = F.energies[e] / Zs[mm];
                         double vec[4], prev[4];
                         int has_prev = 1;
                         vec[0] = F.logT[ro]; vec[1] = F.logg[ro]; vec[2] = As[mm];
                         vec[3] = log10(Ep / kT);
                         if (e > 0) {
                             prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = vec[2];
-                            prev[3] = log10((F.energies[e - 1] / Zs[mm]) / kT);
+                            prev[3] = log10((F.energies[e - 1] * Zi[mm]) / kT);
                         } else if (PV[mm] > -0.5) {
                             int kd = (int)PV[mm];
                             prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = As[kd];
-                            prev[3] = log10((F.energies[N_E - 1] / Zs[kd]) / kT);
+                            prev[3] = log10((F.energies[N_E - 1] * Zi[kd]) / kT);
                         } else if (ST[0] > 0.5) {
                             prev[0] = ST[1]; prev[1] = ST[2]; prev[2] = ST[3]; prev[3] = ST[4];
                         } else has_prev = 0;
@@ -216,7 +222,6 @@ __global__ void __launch_bounds__(NT) k_moment_fold(const Member* __restrict__ m
                         if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
                         v = (R)((num4d_I(F, vec, prev, has_prev) * NUM4D_NORM - corr) * geom);
                     } else if (sizeof(R) == 8) {
-                        double Ep = F.energies[e] / Zs[mm];
                         double I_E = Ep * Ep * Ep / (exp(Ep / kT) - 1.0);
                         double corr = 0.0;
                         if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
@@ -226,11 +231,10 @@ __global__ void __launch_bounds__(NT) k_moment_fold(const Member* __restrict__ m
                         // rounding (of E, kT or the norm) would bias every bin alike,
                         // so E' and E'/kT are formed in FP64 and the norm is applied
                         // in FP64 after the sums
-                        const double Ep_d = F.energies[e] / Zs[mm];
-                        const float Ep = (float)Ep_d;
-                        const float Ep3 = Ep * Ep * Ep;
-                        float I_E = Ep3 / (expf((float)(Ep_d * ikT)) - 1.0f);
-                        if (m.correct) I_E -= Ep3 / (expf((float)(Ep_d * ikT_else)) - 1.0f);
+                        const float Ef = (float)Ep;
+                        const float Ep3 = Ef * Ef * Ef;
+                        float I_E = Ep3 / (expf((float)(Ep / kT)) - 1.0f);
+                        if (m.correct) I_E -= Ep3 / (expf((float)(Ep / kT_else)) - 1.0f);
                         v = (R)(I_E * (float)geom);
                     }
                 }
@@ -240,13 +244,13 @@ __global__ void __launch_bounds__(NT) k_moment_fold(const Member* __restrict__ m
             if (tid < 16 && PV[N_L - 1] < -1.5) CO[(N_L - 1) * 64 + tid] = CO[tid];
             __syncthreads();
             // secant slopes
-            for (int q = tid; q < NI * 16; q += NT) {
+            for (int q = tid; q < NI * 16; q += 256) {
                 const int mm = q >> 4, k = q & 15;
                 SL[q] = (CO[(mm + 1) * 64 + k] - CO[mm * 64 + k]) / (R)(PH[mm + 1] - PH[mm]);
             }
             __syncthreads();
             // periodic Akima coefficients and the sign class of each piece
-            for (int q = tid; q < NI * 16; q += NT) {
+            for (int q = tid; q < NI * 16; q += 256) {
                 const int mm = q >> 4, k = q & 15;
                 auto sl = [&](int kk) {
                     kk = kk < 0 ? kk + NI : (kk >= NI ? kk - NI : kk);
@@ -264,99 +268,104 @@ __global__ void __launch_bounds__(NT) k_moment_fold(const Member* __restrict__ m
                 CO[mm * 64 + 32 + k] = c;
                 CO[mm * 64 + 48 + k] = d;
             }
-            __syncthreads();
-            for (int q = tid; q < NI * 4 * PC; q += NT) MOM[q] = (R)0;
-            __syncthreads();
-            // moments of the (cell, phase) queries; sign-changing pieces directly
-            if (pg < N_P) {
-                const double lo = PH[0], hi = PH[N_L - 1];
-                int id = -1, cur = -1;
-                R m0 = (R)0, m1 = (R)0, m2 = (R)0, m3 = (R)0;
-                for (int j = ja; j < jb; j++) {
-                    const double area = carea[j];
-                    if (isnan(area)) continue;
-                    double x = ph + cphi[j];
-                    int guard = 0;
-                    if (x > hi) { while (x > hi && guard++ < 64) x -= TWO_PI; }
-                    else if (x < lo) { while (x < lo && guard++ < 64) x += TWO_PI; }
-                    if (id < 0 || x < PH[id]) id = interval_of(PH, N_L, x);
-                    else while (id < NI - 1 && PH[id + 1] <= x) id++;
-                    const R t = (R)(x - PH[id]), ar = (R)area;
-                    if (id != cur) {
-                        if (cur >= 0) {
-                            R* mo = MOM + cur * 4 * PC + p;
-                            atomicAdd(mo, m0);
-                            atomicAdd(mo + PC, m1);
-                            atomicAdd(mo + 2 * PC, m2);
-                            atomicAdd(mo + 3 * PC, m3);
+            const double lo = PH[0], hi = PH[N_L - 1];
+            for (int t = 0; t < ntile; t++) {
+                __syncthreads();   // slopes / the previous tile's sweep are done with MOM
+                for (int q = tid; q < NI * 64; q += 256) MOM[q] = (R)0;
+                __syncthreads();
+                // moments of the tile's (cell, phase) queries; sign-changing pieces directly
+                const int pt = t * 16 + p, pg = p0 + pt;
+                if (pg < N_P) {
+                    const double ph = F.phases[pg];
+                    int id = -1, cur = -1;
+                    R m0 = (R)0, m1 = (R)0, m2 = (R)0, m3 = (R)0;
+                    for (int j = ja; j < jb; j++) {
+                        const double area = carea[j];
+                        if (isnan(area)) continue;
+                        double x = ph + cphi[j];
+                        int guard = 0;
+                        if (x > hi) { while (x > hi && guard++ < 64) x -= TWO_PI; }
+                        else if (x < lo) { while (x < lo && guard++ < 64) x += TWO_PI; }
+                        if (id < 0 || x < PH[id]) id = interval_of(PH, N_L, x);
+                        else while (id < NI - 1 && PH[id + 1] <= x) id++;
+                        const R tq = (R)(x - PH[id]), ar = (R)area;
+                        if (id != cur) {
+                            if (cur >= 0) {
+                                R* mo = MOM + cur * 64 + p;
+                                atomicAdd(mo, m0);
+                                atomicAdd(mo + 16, m1);
+                                atomicAdd(mo + 32, m2);
+                                atomicAdd(mo + 48, m3);
+                            }
+                            cur = id;
+                            m0 = (R)0; m1 = (R)0; m2 = (R)0; m3 = (R)0;
                         }
-                        cur = id;
-                        m0 = (R)0; m1 = (R)0; m2 = (R)0; m3 = (R)0;
+                        R at = ar;
+                        m0 += at; at *= tq; m1 += at; at *= tq; m2 += at; at *= tq; m3 += at;
+                        unsigned mx = MIX[id];
+                        while (mx) {
+                            const int k = __ffs(mx) - 1;
+                            mx &= mx - 1u;
+                            const R* co = CO + id * 64 + k;
+                            const R v = co[0] + tq * (co[16] + tq * (co[32] + tq * co[48]));
+                            if (v > (R)0) atomicAdd(&DIR[k * PCB + pt], ar * v);
+                        }
                     }
-                    R at = ar;
-                    m0 += at; at *= t; m1 += at; at *= t; m2 += at; at *= t; m3 += at;
-                    unsigned mx = MIX[id];
-                    while (mx) {
-                        const int k = __ffs(mx) - 1;
-                        mx &= mx - 1u;
-                        const R* co = CO + id * 64 + k;
-                        const R v = co[0] + t * (co[16] + t * (co[32] + t * co[48]));
-                        if (v > (R)0) atomicAdd(&DIR[k * PC + p], ar * v);
+                    if (cur >= 0) {
+                        R* mo = MOM + cur * 64 + p;
+                        atomicAdd(mo, m0);
+                        atomicAdd(mo + 16, m1);
+                        atomicAdd(mo + 32, m2);
+                        atomicAdd(mo + 48, m3);
                     }
                 }
-                if (cur >= 0) {
-                    R* mo = MOM + cur * 4 * PC + p;
-                    atomicAdd(mo, m0);
-                    atomicAdd(mo + PC, m1);
-                    atomicAdd(mo + 2 * PC, m2);
-                    atomicAdd(mo + 3 * PC, m3);
+                __syncthreads();
+                // the sweep: sign-changing pieces masked out of the A operand
+                typename Mfma<R>::v4 acc = {(R)0, (R)0, (R)0, (R)0};
+                const int kb = lane & 15;
+                for (int idx = wave; idx < NI; idx += 4) {
+                    R a = CO[idx * 64 + lane];
+                    if ((MIX[idx] >> kb) & 1u) a = (R)0;
+                    acc = Mfma<R>::step(a, MOM[idx * 64 + lane], acc);
                 }
-            }
-            __syncthreads();
-            for (int q = tid; q < 16 * PC; q += NT) { D64[q] += (double)DIR[q] * a_ref; DIR[q] = (R)0; }
-            // sign-changing pieces are done: drop them from the product
-            for (int q = tid; q < NI * 16; q += NT) {
-                const int mm = q >> 4, k = q & 15;
-                if ((MIX[mm] >> k) & 1u) {
-                    CO[mm * 64 + k] = (R)0;
-                    CO[mm * 64 + 16 + k] = (R)0;
-                    CO[mm * 64 + 32 + k] = (R)0;
-                    CO[mm * 64 + 48 + k] = (R)0;
-                }
-            }
-            __syncthreads();
-            typename Mfma<R>::v4 acc = {(R)0, (R)0, (R)0, (R)0};
-            const R* mb = MOM + (lane >> 4) * PC + tile * 16 + (lane & 15);
-            for (int idx = sub; idx < NI; idx += WPT)
-                acc = Mfma<R>::step(CO[idx * 64 + lane], mb[idx * 4 * PC], acc);
 #pragma unroll
-            for (int r = 0; r < 4; r++) tot[r] += (double)acc[r] * a_ref;
+                for (int tt = 0; tt < TPB; tt++)
+                    if (tt == t)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) tot[tt][r] += (double)acc[r] * a_ref;
+            }
+            if (sizeof(R) == 4) {
+                __syncthreads();
+                for (int q = tid; q < 16 * PCB; q += 256) { D64[q] += (double)DIR[q] * a_ref; DIR[q] = (R)0; }
+            }
         }
     }
-    __syncthreads();
-    double* RED = (double*)MOM;   // [NW][16][16]; the host sizes LDS to hold it
+    double* RED = (double*)MOM;   // [4 waves][16][16]; the host sizes LDS to hold it
+    for (int t = 0; t < ntile; t++) {
+        __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 4; r++) RED[wave * 256 + Mfma<R>::row(lane, r) * 16 + (lane & 15)] = tot[r];
-    __syncthreads();
-    for (int q = tid; q < 16 * PC; q += NT) {
-        const int k = q / PC, pp = q % PC, e = e0 + k, pq = p0 + pp;
-        if (e >= N_E || pq >= N_P) continue;
-        const int tl = pp >> 4;
-        double s = D64[q];
-        for (int w = 0; w < WPT; w++) s += RED[(tl + w * NTILE) * 256 + k * 16 + (pp & 15)];
-        if (sizeof(R) == 4 && ATM == 1) s *= BB_NORM;
-        const double E = F.energies[e];
-        double* fo = F.flux + (size_t)mi * N_E * N_P + (size_t)e * N_P + pq;
-        if (gridDim.y == 1) *fo = s / (E * KEV);
-        else atomicAdd(fo, s * (1.0 / (E * KEV)));
+        for (int tt = 0; tt < TPB; tt++)
+            if (tt == t)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    RED[wave * 256 + Mfma<R>::row(lane, r) * 16 + (lane & 15)] = tot[tt][r];
+        __syncthreads();
+        const int k = tid >> 4, pp = t * 16 + (tid & 15), e = e0 + k, pq = p0 + pp;
+        if (e < N_E && pq < N_P) {
+            double s = D64[k * PCB + pp] + RED[tid] + RED[256 + tid] + RED[512 + tid] + RED[768 + tid];
+            if (sizeof(R) == 4 && ATM == 1) s *= BB_NORM;
+            const double E = F.energies[e];
+            double* fo = F.flux + (size_t)mi * N_E * N_P + (size_t)e * N_P + pq;
+            if (gridDim.y == 1) *fo = s / (E * KEV);
+            else atomicAdd(fo, s * (1.0 / (E * KEV)));
+        }
     }
 }
 
-#define XH_MOMENT_INST(R, PC, NT)                                                              \
-    template __global__ void k_moment_fold<R, 1, PC, NT>(const Member*, Layout, FoldArgs);     \
-    template __global__ void k_moment_fold<R, 2, PC, NT>(const Member*, Layout, FoldArgs);
-XH_MOMENT_INST(double, 16, 256)
-XH_MOMENT_INST(float, 16, 256)
-XH_MOMENT_INST(float, 64, 512)
+#define XH_MOMENT_INST(R, TPB)                                                                 \
+    template __global__ void k_moment_fold<R, 1, TPB>(const Member*, Layout, FoldArgs);        \
+    template __global__ void k_moment_fold<R, 2, TPB>(const Member*, Layout, FoldArgs);
+XH_MOMENT_INST(double, 1) XH_MOMENT_INST(double, 2) XH_MOMENT_INST(double, 4)
+XH_MOMENT_INST(float, 1) XH_MOMENT_INST(float, 2) XH_MOMENT_INST(float, 4)
 
 }  // namespace xh

@@ -2,10 +2,12 @@
 //
 // Reference behaviour: xpsi/cellmesh/rays.pyx:53-390 (integrands :61-88,
 // three ray branches :90-247, table construction and repair :249-390) and the
-// lag calibration of xpsi/HotRegion.py:880-901.  One wavefront per ray runs
-// the QAG-GK61 quadratures (epsrel 1e-12, 100 intervals, as the reference);
-// a second kernel applies the bad-ray repair, the last-ray check and the lag
-// calibration per member.
+// lag calibration of xpsi/HotRegion.py:880-901.  The QAG-GK61 quadratures
+// (epsrel 1e-12, 100 intervals, as the reference) start with one lane per
+// ray: most rays are settled by QAG's first Gauss-Kronrod pass, which a lane
+// sums alone; the rest are listed and rerun with one wavefront per ray
+// through the full bisection.  A last kernel applies the bad-ray repair, the
+// last-ray check and the lag calibration per member.
 #include "xh_common.hpp"
 #include "xh_types.hpp"
 #include "xh_kernels.hpp"
@@ -147,34 +149,180 @@ __device__ __forceinline__ double clamped_max_alpha(double maxA, double u)
     return maxA;
 }
 
-// wave per (ray j >= 1, parallel i, member); writes raw (unrepaired) values
-__global__ void __launch_bounds__(256) k_rays(const Member* __restrict__ members, Layout L,
-                                              const double* __restrict__ rr,
-                                              const double* __restrict__ maxAlpha_in,
-                                              double* __restrict__ cos_alpha,
-                                              double* __restrict__ defl_raw,
-                                              double* __restrict__ lag_raw)
+// ---------------------------------------------------------------- lane per ray
+// The same 61-point Gauss-Kronrod pass as wave_gk61_pair, summed by one lane
+// over the nodes (uniform node and weight loads, no cross-lane reductions);
+// the mean-deviation term takes a second sweep over the nodes.
+template <class FB>
+__device__ void lane_gk61_pair(const FB& f, double a, double b, double res[2], double err[2], double rabs[2],
+                               double rasc[2])
+{
+    const double ctr = 0.5 * (a + b), hl = 0.5 * (b - a), ahl = fabs(hl);
+    double fc0, fc1;
+    f.both(ctr, fc0, fc1);
+    double rk0 = WGK61[30] * fc0, rk1 = WGK61[30] * fc1;
+    double rg0 = 0.0, rg1 = 0.0;
+    double ra0 = WGK61[30] * fabs(fc0), ra1 = WGK61[30] * fabs(fc1);
+    for (int k = 0; k < 30; k++) {
+        const double dx = hl * XGK61[k], wk = WGK61[k];
+        double fa0, fa1, fb0, fb1;
+        f.both(ctr - dx, fa0, fa1);
+        f.both(ctr + dx, fb0, fb1);
+        rk0 += wk * (fa0 + fb0);
+        rk1 += wk * (fa1 + fb1);
+        ra0 += wk * (fabs(fa0) + fabs(fb0));
+        ra1 += wk * (fabs(fa1) + fabs(fb1));
+        if (k & 1) {
+            const double wg = WG30[k >> 1];
+            rg0 += wg * (fa0 + fb0);
+            rg1 += wg * (fa1 + fb1);
+        }
+    }
+    const double m0 = 0.5 * rk0, m1 = 0.5 * rk1;
+    double rs0 = WGK61[30] * fabs(fc0 - m0), rs1 = WGK61[30] * fabs(fc1 - m1);
+    for (int k = 0; k < 30; k++) {
+        const double dx = hl * XGK61[k], wk = WGK61[k];
+        double fa0, fa1, fb0, fb1;
+        f.both(ctr - dx, fa0, fa1);
+        f.both(ctr + dx, fb0, fb1);
+        rs0 += wk * (fabs(fa0 - m0) + fabs(fb0 - m0));
+        rs1 += wk * (fabs(fa1 - m1) + fabs(fb1 - m1));
+    }
+    res[0] = rk0 * hl; res[1] = rk1 * hl;
+    rabs[0] = ra0 * ahl; rabs[1] = ra1 * ahl;
+    rasc[0] = rs0 * ahl; rasc[1] = rs1 * ahl;
+    err[0] = scale_error((rk0 - rg0) * hl, rabs[0], rasc[0]);
+    err[1] = scale_error((rk1 - rg1) * hl, rabs[1], rasc[1]);
+}
+
+// QAG's acceptance of its first pass (epsabs 0): the pass's result stands
+// when its error is within the tolerance (or is zero), or when it is below
+// the round-off level (QAG then returns that result with a round-off flag)
+__device__ __forceinline__ bool qag_first_pass_stands(double r, double e, double rabs, double rasc, double epsrel)
+{
+    const double tol = epsrel * fabs(r), roundoff = 50.0 * DBL_EPSILON * rabs;
+    if (e <= roundoff && e > tol) return true;
+    return (e <= tol && e != rasc) || e == 0.0;
+}
+
+template <class FB>
+__device__ __forceinline__ bool lane_ray_pair(const FB& fb, double a, double b, double& d, double& l)
+{
+    double r[2], e[2], ra[2], rs[2];
+    lane_gk61_pair(fb, a, b, r, e, ra, rs);
+    d = r[0];
+    l = r[1];
+    return qag_first_pass_stands(r[0], e[0], ra[0], rs[0], 1.0e-12) &&
+           qag_first_pass_stands(r[1], e[1], ra[1], rs[1], 1.0e-12);
+}
+
+// wave_ray's branches with every quadrature settled by its first pass;
+// returns false when some quadrature needs QAG's bisection
+__device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, double& lag)
+{
+    const double b_ph = 3.0 * sqrt(3.0) / 2.0;
+    double sas = 1.0 - cos_alpha * cos_alpha;
+    double sa = sqrt(sas);
+    double alpha = acos(cos_alpha);
+    double b = sa / (u * sqrt(1.0 - u));
+    double d0, l0;
+    if (b <= b_ph) {
+        OutBoth fb{sas, 1.0 / u};
+        if (!lane_ray_pair(fb, 0.0, 1.0, d0, l0)) return false;
+        d0 *= 2.0 * b * u;
+        l0 *= 2.0 * b * b * u * r_s;
+    } else if (alpha <= PI / 2.0) {
+        double rc, wR;
+        if (eq12(sa, 1.0)) { rc = 1.0 / u; wR = 0.0; }
+        else {
+            rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
+            wR = sqrt(1.0 - rc * u);
+        }
+        if (isnan(wR)) wR = 0.0;
+        InBoth fb{rc};
+        if (!lane_ray_pair(fb, wR, 1.0, d0, l0)) return false;
+        d0 *= 2.0 * b / rc;
+        l0 *= 2.0 * b * b * r_s / rc;
+    } else {
+        double rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
+        double wR = sqrt(1.0 - rc * u);
+        if (isnan(wR)) wR = 0.0;
+        InBoth fb{rc};
+        double D1, D0, L1, L0;
+        if (!lane_ray_pair(fb, 0.0, 1.0, D1, L1)) return false;
+        if (!lane_ray_pair(fb, wR, 1.0, D0, L0)) return false;
+        d0 = 2.0 * b * (2.0 * D1 - D0) / rc;
+        l0 = 2.0 * b * b * r_s * (2.0 * L1 - L0) / rc;
+        double cor = 1.0 / u - rc + log((1.0 / u - 1.0) / (rc - 1.0));
+        l0 += 2.0 * r_s * cor;
+    }
+    defl = d0;
+    lag = l0 / C_SI;
+    return true;
+}
+
+// lane per (ray j >= 1, parallel i, member): rays settled by the first pass
+// are written; the others go onto the work list for k_rays
+__global__ void __launch_bounds__(64) k_rays_lane(const Member* __restrict__ members, Layout L,
+                                                  const double* __restrict__ rr,
+                                                  const double* __restrict__ maxAlpha_in,
+                                                  double* __restrict__ cos_alpha,
+                                                  double* __restrict__ defl_raw,
+                                                  double* __restrict__ lag_raw,
+                                                  int* __restrict__ work, int* __restrict__ nwork)
 {
     const int mi = blockIdx.z, i = blockIdx.y;
-    const int j = blockIdx.x * 4 + (threadIdx.x >> 6) + 1;
+    const int j = blockIdx.x * 64 + threadIdx.x + 1;
     const Member m = members[mi];
     if (!m.active || m.status || i >= m.S || j >= L.N_R) return;
     const size_t ro = (size_t)mi * L.S_max + i;
-    double u = m.r_s / rr[ro];
-    double maxA = clamped_max_alpha(maxAlpha_in[ro], u);
-    double inc = (1.0 - cos(maxA)) / ((double)L.N_R - 1.0);
-    double ca = 1.0 - (double)j * inc;
+    const double u = m.r_s / rr[ro];
+    const double maxA = clamped_max_alpha(maxAlpha_in[ro], u);
+    const double inc = (1.0 - cos(maxA)) / ((double)L.N_R - 1.0);
+    const double ca = 1.0 - (double)j * inc;
+    const size_t o = ro * L.N_R + j;
+    cos_alpha[o] = ca;
     double d, l;
-    wave_ray(ca, m.r_s, u, d, l);
-    if ((threadIdx.x & 63) == 0) {
-        size_t o = ro * L.N_R + j;
-        cos_alpha[o] = ca;
+    if (lane_ray(ca, m.r_s, u, d, l)) {
         defl_raw[o] = d;
         lag_raw[o] = l;
-        if (j == 1) {
-            cos_alpha[ro * L.N_R] = 1.0;
-            defl_raw[ro * L.N_R] = 0.0;
-            lag_raw[ro * L.N_R] = 0.0;
+    } else {
+        const int slot = atomicAdd(nwork, 1);
+        work[slot] = (int)o;
+    }
+    if (j == 1) {
+        cos_alpha[ro * L.N_R] = 1.0;
+        defl_raw[ro * L.N_R] = 0.0;
+        lag_raw[ro * L.N_R] = 0.0;
+    }
+}
+
+// wave per listed ray (grid-stride over the work list): the full QAG with
+// bisection; writes raw (unrepaired) values
+__global__ void __launch_bounds__(256) k_rays(const Member* __restrict__ members, Layout L,
+                                              const double* __restrict__ rr,
+                                              const double* __restrict__ maxAlpha_in,
+                                              double* __restrict__ defl_raw,
+                                              double* __restrict__ lag_raw,
+                                              const int* __restrict__ work, const int* __restrict__ nwork)
+{
+    const int n = *nwork;
+    const int waves = gridDim.x * 4;
+    for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < n; w += waves) {
+        const size_t o = (size_t)work[w];
+        const int j = (int)(o % (size_t)L.N_R);
+        const size_t ro = o / (size_t)L.N_R;
+        const int mi = (int)(ro / (size_t)L.S_max);
+        const Member m = members[mi];
+        const double u = m.r_s / rr[ro];
+        const double maxA = clamped_max_alpha(maxAlpha_in[ro], u);
+        const double inc = (1.0 - cos(maxA)) / ((double)L.N_R - 1.0);
+        const double ca = 1.0 - (double)j * inc;
+        double d, l;
+        wave_ray(ca, m.r_s, u, d, l);
+        if ((threadIdx.x & 63) == 0) {
+            defl_raw[o] = d;
+            lag_raw[o] = l;
         }
     }
 }

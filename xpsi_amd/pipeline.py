@@ -259,3 +259,28 @@ def params_from_model(model):
                               if h.get(k) is not None} for h in model["hot_regions"]],
                 **({"elsewhere_temperature": model["elsewhere"]["elsewhere_temperature"]}
                    if model.get("elsewhere") is not None else {}))
+
+
+def autotune_fold(model, params, stream=None, candidates=(1, 2, 4), reps=2):
+    """Pick the moment-form fold's tiles per block for this model and batch.
+
+    For each candidate (16-phase tiles per block sharing one coefficient set
+    per parallel) a pipeline evaluates `params` once untimed and `reps` times
+    with HIP-event stage timing; the candidate with the least profile-and-fold
+    time is set with xh_set_moment_tiles for pipelines created afterwards.
+    Returns {candidate: fold ms per evaluation, "chosen": candidate}."""
+    L = lib()
+    times = {}
+    for c in candidates:
+        check(L.xh_set_moment_tiles(int(c)), "xh_set_moment_tiles")
+        pl = LikelihoodPipeline(model, max_batch=len(params))
+        pl.evaluate(params, stream)
+        pl.set_timing(True)
+        for _ in range(reps):
+            pl.evaluate(params, stream)
+        times[int(c)] = pl.stage_ms()["profile_fold"] / reps
+        pl.close()
+    best = min(times, key=times.get)
+    check(L.xh_set_moment_tiles(best), "xh_set_moment_tiles")
+    times["chosen"] = best
+    return times
