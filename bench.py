@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 FP32_MFMA_PEAK_TFLOPS = 157.3
+KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -85,15 +86,17 @@ def main():
     for d in acc:
         groups.setdefault(d["num_energies"], []).append(d)
 
+    # fold kernel and tile shape per energy group, measured on the group (untimed)
+    from xpsi_amd.pipeline import autotune_fold
+    tuned, pls = {}, {}
+    for ne, g in sorted(groups.items()):
+        mdl, tuned[ne] = autotune_fold(dict(configs.config3_model(ne), precision=args.precision), g, stream)
+        pls[ne] = LikelihoodPipeline(mdl, max_batch=len(g))
+
     def make(prec):
         return {ne: LikelihoodPipeline(dict(configs.config3_model(ne), precision=prec), max_batch=len(g))
                 for ne, g in sorted(groups.items())}
 
-    # tile shape of the fold kernel, measured on the 128-energy group (untimed)
-    from xpsi_amd.pipeline import autotune_fold
-    tune_ne = 128 if 128 in groups else sorted(groups)[0]
-    tuned = autotune_fold(dict(configs.config3_model(tune_ne), precision=args.precision), groups[tune_ne], stream)
-    pls = make(args.precision)
     res = {}
 
     def step():
@@ -200,7 +203,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved,
                      "peak": FP32_MFMA_PEAK_TFLOPS if args.precision == "mixed" else FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "traffic": None,
-                     "kernel": "k_moment_fold", "evals_per_step": evals,
+                     "kernel": "+".join(sorted({KERNEL_NAMES[t["chosen"].split("/")[0]] for t in tuned.values()})),
+                     "evals_per_step": evals,
                      "flop_per_eval": 8},
         "autotune_fold_tiles": tuned,
         "device": dev,

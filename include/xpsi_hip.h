@@ -162,8 +162,9 @@ typedef struct xh_model_config {
                                  intensity and spline fold (matrix cores), split-BF16
                                  response fold with FP32 accumulation; rays, phase-bin
                                  accumulation and likelihood quadrature stay FP64 */
-    int fold_kernel;          /* 0 automatic (moment form when its LDS fits), 1 per-point
-                                 fold (FP64 only), 2 moment form */
+    int fold_kernel;          /* 0 automatic (moment form, else energy lanes, else
+                                 per-point), 1 per-point fold (FP64 only), 2 moment form,
+                                 3 energy lanes; xpsi_amd.pipeline.autotune_fold measures */
     /* time-invariant blackbody emission from the rest of the surface
        (xpsi/Elsewhere.py): a closed mesh of else_sqrt_num_cells^2 cells and
        else_num_rays rays per ring; its spectrum joins the first hot region's
@@ -217,8 +218,9 @@ int xh_pipeline_work(xh_pipeline* p, int n, double* out);
  * and to xh_integrate calls. */
 int xh_set_fold_energy_chunk(int ec);
 
-/* Fold kernel of xh_integrate: 0 automatic (moment form on the matrix cores
- * when its LDS fits), 1 per-point evaluation, 2 moment form.  Both give the
+/* Fold kernel of xh_integrate: 0 automatic (the moment form when its LDS
+ * fits, else energy lanes, else per-point), 1 per-point evaluation,
+ * 2 moment form on the matrix cores, 3 energy lanes.  All give the
  * reference's sums (the moment form evaluates sign-changing spline pieces
  * point by point, so the >0 clipping is kept). */
 int xh_set_fold_kernel(int kind);
@@ -228,6 +230,10 @@ int xh_set_fold_kernel(int kind);
  * registers).  Applies to pipelines created afterwards and to xh_integrate;
  * xpsi_amd.pipeline.autotune_fold measures the choices. */
 int xh_set_moment_tiles(int tiles);
+
+/* Tuning: phases per wave of the energy-lane fold (8 or 16).  Applies to
+ * pipelines created afterwards and to xh_integrate. */
+int xh_set_lane_phases(int nq);
 
 /* Replay each launch as a hipGraph captured once per batch size (requires a
  * non-default stream; timing mode launches eagerly). */

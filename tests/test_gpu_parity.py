@@ -45,10 +45,10 @@ def ef_model(examples_fast):
     return examples_fast_model(examples_fast)
 
 
-@pytest.fixture(params=["per-point", "moment"])
+@pytest.fixture(params=["per-point", "moment", "lanes"])
 def fold_kernel(request):
     from xpsi_amd._lib import lib
-    kind = {"per-point": 1, "moment": 2}[request.param]
+    kind = {"per-point": 1, "moment": 2, "lanes": 3}[request.param]
     assert lib().xh_set_fold_kernel(kind) == 0
     yield request.param
     lib().xh_set_fold_kernel(0)

@@ -114,16 +114,73 @@ static hipError_t launch_fold(int EC, dim3 g, size_t lds, hipStream_t s, const M
     return L.atm_ext == 2 ? launch_fold_a<2>(EC, g, lds, s, m, L, F) : launch_fold_a<1>(EC, g, lds, s, m, L, F);
 }
 
+// ------------------------------------------------- energy-lane fold
+// dynamic LDS of k_lane_fold (mirrors its carve-up)
+static size_t lane_lds(int N_L, int S_max, size_t rb)
+{
+    const size_t head = ((5 * (size_t)N_L + 2 * (size_t)S_max + 8 + 3) & ~(size_t)3) * 8;
+    return head + 64 * rb + ((size_t)N_L - 1) * 64 * 4 * rb;
+}
+
+// phases per wave of the energy-lane fold (8 or 16); a tuning choice
+static int g_lane_nq = 8;
+
+extern "C" int xh_set_lane_phases(int nq)
+{
+    if (nq != 8 && nq != 16) return fail(XH_EINVAL, "phases per wave must be 8 or 16");
+    g_lane_nq = nq;
+    return XH_OK;
+}
+
+static dim3 lane_grid(int N_E, int N_P, int nq, int ring_blocks, int members)
+{
+    const int pb = 4 * nq;
+    return dim3((unsigned)(((N_E + 63) / 64) * ((N_P + pb - 1) / pb)), (unsigned)ring_blocks, (unsigned)members);
+}
+
+template <typename R, int ATM, int NQ>
+static hipError_t launch_lane_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+{
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)&k_lane_fold<R, ATM, NQ>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    if (g.x == 0) return hipSuccess;     // configuration only
+    hipLaunchKernelGGL((k_lane_fold<R, ATM, NQ>), g, dim3(256), lds, s, m, L, F);
+    return hipGetLastError();
+}
+
+template <typename R, int ATM>
+static hipError_t launch_lane_a(int nq, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+{
+    if (nq == 16) return launch_lane_t<R, ATM, 16>(g, lds, s, m, L, F);
+    return launch_lane_t<R, ATM, 8>(g, lds, s, m, L, F);
+}
+
+static hipError_t launch_lane(bool fp32, int nq, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L,
+                              FoldArgs F)
+{
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (fp32) return L.atm_ext == 2 ? launch_lane_a<float, 2>(nq, g, lds, s, m, L, F)
+                                    : launch_lane_a<float, 1>(nq, g, lds, s, m, L, F);
+    return L.atm_ext == 2 ? launch_lane_a<double, 2>(nq, g, lds, s, m, L, F)
+                          : launch_lane_a<double, 1>(nq, g, lds, s, m, L, F);
+}
+
 // blocks of the wave-per-ray QAG kernel (grid-stride over its work list)
 constexpr int kRayWaveBlocks = 2048;
 
 // ----------------------------------------------- moment form (matrix cores)
-static int g_fold_kernel = 0;   // xh_integrate: 0 automatic, 1 per-point, 2 moment form
+static int g_fold_kernel = 0;   // xh_integrate: 0 automatic, 1 per-point, 2 moment form, 3 energy lanes
 static const size_t kMomentLdsMax = 160 * 1024;
 
 extern "C" int xh_set_fold_kernel(int kind)
 {
-    if (kind < 0 || kind > 2) return fail(XH_EINVAL, "fold kernel must be 0 (automatic), 1 (per-point) or 2 (moment)");
+    if (kind < 0 || kind > 3)
+        return fail(XH_EINVAL, "fold kernel must be 0 (automatic), 1 (per-point), 2 (moment) or 3 (energy lanes)");
     g_fold_kernel = kind;
     return XH_OK;
 }
@@ -241,10 +298,16 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1) return fail(XH_EINVAL, "xh_integrate: bad sizes");
     const int EC = pick_ec(N_L);
     const int tpb = g_moment_tpb;
-    const bool moment = g_fold_kernel == 2 || (g_fold_kernel == 0 && moment_lds(N_L, S, 8, tpb) <= kMomentLdsMax);
+    const bool lane_fits = lane_lds(N_L, S, 8) <= 160 * 1024;
+    const bool moment_fits = moment_lds(N_L, S, 8, tpb) <= kMomentLdsMax;
+    const bool lane = g_fold_kernel == 3 || (g_fold_kernel == 0 && !moment_fits && lane_fits);
+    if (g_fold_kernel == 3 && !lane_fits)
+        return fail(XH_EINVAL, "xh_integrate: the energy-lane fold does not fit LDS at this leaf count");
+    const bool moment = !lane && (g_fold_kernel == 2 || (g_fold_kernel == 0 && moment_fits));
     if (g_fold_kernel == 2 && moment_lds(N_L, S, 8, tpb) > kMomentLdsMax)
         return fail(XH_EINVAL, "xh_integrate: the moment-form fold does not fit LDS at this leaf count");
-    if (!moment && N_P > (256 / EC) * 16) return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
+    if (!lane && !moment && N_P > (256 / EC) * 16)
+        return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
     if (workspace_bytes < xh_integrate_workspace_bytes(S, N_R, N_L, N_E, N_P))
         return fail(XH_EINVAL, "xh_integrate: workspace too small");
     const int IO = 8;
@@ -280,7 +343,15 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
             FA.atm_N[d] = hN[d];
         }
     }
-    if (moment) {
+    if (lane) {
+        const int nrb = std::min(S, 8), rb = (S + nrb - 1) / nrb;
+        FA.ring_block = rb;
+        HIPCHK(fill(flux, (size_t)N_E * N_P, 0.0, s));
+        const int nq = g_lane_nq;
+        HIPCHK(launch_lane(false, nq, dim3(0, 1, 1), 0, s, d_m, L, FA));
+        HIPCHK(launch_lane(false, nq, lane_grid(N_E, N_P, nq, (S + rb - 1) / rb, 1), lane_lds(N_L, S, 8), s, d_m, L,
+                           FA));
+    } else if (moment) {
         // parallels split over up to 8 blocks per tile; partial sums meet by atomics
         const int nrb = std::min(S, 8), rb = (S + nrb - 1) / nrb;
         FA.ring_block = rb;
@@ -429,6 +500,7 @@ struct xh_pipeline {
     int nreg, mpr, maxb, N_in, N_ch, N_b, EC, table_n;
     int fold_kind;               // 1 per-point fold, 2 moment form
     int tpb;                     // moment form: 16-phase tiles per block
+    int lane_nq;                 // energy-lane fold: phases per wave
     bool fp32;                   // precision form
     int fold_terms;              // bf16 products of the precision form's response fold
     __bf16 *d_Mh, *d_Mm, *d_Ml;  // three-term bf16 split of the response (precision form)
@@ -517,8 +589,8 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     p->L = Layout{smax, c->num_rays, c->num_leaves, N_P, c->N_E, IO, c->image_order_limit > 0 ? c->image_order_limit : 0,
                   c->atm_ext};
     p->EC = pick_ec(c->num_leaves);
-    if (c->precision < 0 || c->precision > 1 || c->fold_kernel < 0 || c->fold_kernel > 2) {
-        fail(XH_EINVAL, "pipeline: precision must be 0 or 1 and fold_kernel 0, 1 or 2"); delete p; return nullptr;
+    if (c->precision < 0 || c->precision > 1 || c->fold_kernel < 0 || c->fold_kernel > 3) {
+        fail(XH_EINVAL, "pipeline: precision must be 0 or 1 and fold_kernel 0, 1, 2 or 3"); delete p; return nullptr;
     }
     p->fp32 = c->precision == 1;
     p->fold_terms = c->fold_terms == 3 ? 3 : 6;
@@ -536,7 +608,16 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         if (p->fp32 && c->fold_kernel == 1) {
             fail(XH_EINVAL, "pipeline: the precision form uses the moment-form fold"); delete p; return nullptr;
         }
-        p->fold_kind = (c->fold_kernel == 1 || !fits) ? 1 : 2;
+        const bool lane_fits = lane_lds(c->num_leaves, smax, p->fp32 ? 4 : 8) <= 160 * 1024;
+        if (c->fold_kernel == 3 && !lane_fits) {
+            fail(XH_EINVAL, "pipeline: the energy-lane fold does not fit LDS at this leaf count and mesh bound");
+            delete p; return nullptr;
+        }
+        // automatic: the moment form, else the energy lanes, else per-point
+        if (c->fold_kernel == 3) p->fold_kind = 3;
+        else if (c->fold_kernel == 1) p->fold_kind = 1;
+        else if (fits) p->fold_kind = 2;
+        else p->fold_kind = lane_fits ? 3 : 1;
     }
     {
         // configure the fold kernel's LDS limit now, never inside a capture
@@ -544,7 +625,9 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         L0.N_P = c->num_phases > 0 ? c->num_phases : c->num_leaves;
         L0.atm_ext = c->atm_ext;
         FoldArgs F0{};
-        if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, p->tpb, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
+        p->lane_nq = g_lane_nq;
+        if (p->fold_kind == 3) PHIP(launch_lane(p->fp32, p->lane_nq, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
+        else if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, p->tpb, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
         else PHIP(launch_fold(p->EC, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
     }
     if (p->fold_kind == 1 && N_P > (256 / p->EC) * 16) { fail(XH_EINVAL, "too many phases for the leaf count"); delete p; return nullptr; }
@@ -874,12 +957,18 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
         HIPCHK(hipGetLastError());
     }
     // parallels per block: all of them when the batch fills the GPU
+    // (the energy-lane fold: 8 parallels per block, so that members of very
+    // different sizes finish in similar times; partial sums meet by atomics)
     int rb = (n >= 32) ? S : 1;
+    if (p->fold_kind == 3 && n >= 32) rb = std::min(S, 8);
     FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_effGrav, p->d_lastq,
                 p->d_atm_axes, p->d_atm_I, {p->atm_N[0], p->atm_N[1], p->atm_N[2], p->atm_N[3]}, p->d_energies,
                 p->d_phases, p->d_flux, rb};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
-    if (p->fold_kind == 2)
+    if (p->fold_kind == 3)
+        HIPCHK(launch_lane(p->fp32, p->lane_nq, lane_grid(N_E, N_P, p->lane_nq, (S + rb - 1) / rb, nm),
+                           lane_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8), s, p->d_members, L, FA));
+    else if (p->fold_kind == 2)
         HIPCHK(launch_moment(p->fp32, p->tpb, moment_grid(N_E, N_P, p->tpb, (S + rb - 1) / rb, nm),
                              moment_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8, p->tpb), s, p->d_members, L, FA));
     else

@@ -131,6 +131,27 @@ __device__ __forceinline__ void akima_piece(double mm2, double mm1, double m0, d
     d = (b + tn - 2.0 * m0) / (h * h);
 }
 
+// Akima (1970) piece from the five surrounding secant slopes, with the
+// degenerate-weight branches of the reference's interpolation library
+template <typename R>
+__device__ __forceinline__ void akima_piece_t(R mm2, R mm1, R m0, R mp1, R mp2, R h, R& b, R& c, R& d)
+{
+    const R one = (R)1, two = (R)2, three = (R)3;
+    R w = fabs(mp1 - m0) + fabs(mm1 - mm2);
+    if (w == (R)0) { b = m0; c = (R)0; d = (R)0; return; }
+    R wn = fabs(mp2 - mp1) + fabs(m0 - mm1);
+    R al = fabs(mm1 - mm2) / w;
+    R tn;
+    if (wn == (R)0) tn = m0;
+    else {
+        R aln = fabs(m0 - mm1) / wn;
+        tn = (one - aln) * m0 + aln * mp1;
+    }
+    b = (one - al) * mm1 + al * m0;
+    c = (three * m0 - two * b - tn) / h;
+    d = (b + tn - two * m0) / (h * h);
+}
+
 __device__ inline void akima_coefs(const double* x, const double* y, int n, bool periodic, int i,
                                    double& b, double& c, double& d)
 {

@@ -113,6 +113,8 @@ template <int EC, int MAXK, int ATM>
 __global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
 template <typename R, int ATM, int TPB>
 __global__ void k_moment_fold(const Member* members, Layout L, FoldArgs F);
+template <typename R, int ATM, int NQ>
+__global__ void k_lane_fold(const Member* members, Layout L, FoldArgs F);
 __global__ void k_fill(double* p, size_t n, double v);
 __global__ void k_work_count(const Member* members, Layout L, const double* cellArea, const int* valid,
                              double* work);

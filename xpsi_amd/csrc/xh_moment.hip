@@ -70,27 +70,6 @@ template <typename R> struct Eps;
 template <> struct Eps<double> { static constexpr double v = DBL_EPSILON; };
 template <> struct Eps<float> { static constexpr float v = FLT_EPSILON; };
 
-// Akima (1970) piece from the five surrounding secant slopes, with the
-// degenerate-weight branches of the reference's interpolation library
-template <typename R>
-__device__ __forceinline__ void akima_piece_r(R mm2, R mm1, R m0, R mp1, R mp2, R h, R& b, R& c, R& d)
-{
-    const R one = (R)1, two = (R)2, three = (R)3;
-    R w = fabs(mp1 - m0) + fabs(mm1 - mm2);
-    if (w == (R)0) { b = m0; c = (R)0; d = (R)0; return; }
-    R wn = fabs(mp2 - mp1) + fabs(m0 - mm1);
-    R al = fabs(mm1 - mm2) / w;
-    R tn;
-    if (wn == (R)0) tn = m0;
-    else {
-        R aln = fabs(m0 - mm1) / wn;
-        tn = (one - aln) * m0 + aln * mp1;
-    }
-    b = (one - al) * mm1 + al * m0;
-    c = (three * m0 - two * b - tn) / h;
-    d = (b + tn - two * m0) / (h * h);
-}
-
 // Sign class of f(t) = a + t (b + t (c + t d)) on [0, h] from its cubic
 // Bernstein coefficients (the curve lies in their convex hull):
 // 1 = every Bernstein coefficient above the evaluation-error margin, so f > 0
@@ -258,7 +237,7 @@ __global__ void __launch_bounds__(256) k_moment_fold(const Member* __restrict__ 
                 };
                 const R h = (R)(PH[mm + 1] - PH[mm]);
                 R a = CO[mm * 64 + k], b, c, d;
-                akima_piece_r<R>(sl(mm - 2), sl(mm - 1), sl(mm), sl(mm + 1), sl(mm + 2), h, b, c, d);
+                akima_piece_t<R>(sl(mm - 2), sl(mm - 1), sl(mm), sl(mm + 1), sl(mm + 2), h, b, c, d);
                 // with the elsewhere correction every value is kept (:593)
                 const int cls = m.correct ? 1 : cubic_class<R>(a, b, c, d, h);
                 if (cls == 0) { a = (R)0; b = (R)0; c = (R)0; d = (R)0; }

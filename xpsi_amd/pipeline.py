@@ -22,7 +22,7 @@ Model description (dict):
   precision   'fp64' (default) or 'mixed' (see PRECISIONS)
   fold_terms  bf16 products of the mixed-precision response fold: 6 (default)
               or 3
-  fold_kernel 'auto' (default), 'moment' or 'per-point' (see FOLD_KERNELS)
+  fold_kernel 'auto' (default), 'lanes', 'moment' or 'per-point' (see FOLD_KERNELS)
 A parameter vector is a dict: mass, radius, distance, cos_inclination and
 `hot_regions`: list of dicts phase_shift, super_colatitude, super_radius,
 super_temperature, cede_colatitude, cede_radius, cede_azimuth,
@@ -66,9 +66,10 @@ def construct_energy_array(num_energies, edges):
 # accumulation; rays, phase-bin accumulation and the likelihood stay FP64.
 PRECISIONS = {"fp64": 0, "mixed": 1}
 # "moment": spline fold as interval-moment products on the matrix cores;
-# "per-point": one spline evaluation per (cell, phase, energy); "auto": the
-# moment form when its LDS fits.
-FOLD_KERNELS = {"auto": 0, "per-point": 1, "moment": 2}
+# "lanes": energies across a wave's lanes, (cell, phase) queries in lockstep;
+# "per-point": one spline evaluation per (cell, phase, energy) lane; "auto":
+# moment, else lanes, else per-point (autotune_fold measures them).
+FOLD_KERNELS = {"auto": 0, "per-point": 1, "moment": 2, "lanes": 3}
 
 
 def model_config(model, max_batch=1, max_sqrt_cells=0):
@@ -261,26 +262,39 @@ def params_from_model(model):
                    if model.get("elsewhere") is not None else {}))
 
 
-def autotune_fold(model, params, stream=None, candidates=(1, 2, 4), reps=2):
-    """Pick the moment-form fold's tiles per block for this model and batch.
+def autotune_fold(model, params, stream=None, reps=2,
+                  candidates=(("moment", 1), ("moment", 2), ("moment", 4), ("lanes", 8), ("lanes", 16))):
+    """Pick the spline-fold kernel and its tile shape for this model and batch.
 
-    For each candidate (16-phase tiles per block sharing one coefficient set
-    per parallel) a pipeline evaluates `params` once untimed and `reps` times
-    with HIP-event stage timing; the candidate with the least profile-and-fold
-    time is set with xh_set_moment_tiles for pipelines created afterwards.
-    Returns {candidate: fold ms per evaluation, "chosen": candidate}."""
+    Candidates are (kernel, shape): the moment form with 1, 2 or 4 16-phase
+    tiles per block, or the energy-lane form with 8 or 16 phases per wave.
+    For each, a pipeline evaluates `params` once untimed and `reps` times with
+    HIP-event stage timing; the candidate with the least profile-and-fold
+    time wins.  Its shape is set for pipelines created afterwards, and the
+    returned model dict selects its kernel.  Returns (model, timings) with
+    timings {"kernel/shape": fold ms per evaluation, "chosen": "kernel/shape"}."""
     L = lib()
     times = {}
-    for c in candidates:
-        check(L.xh_set_moment_tiles(int(c)), "xh_set_moment_tiles")
-        pl = LikelihoodPipeline(model, max_batch=len(params))
+    for kernel, shape in candidates:
+        if kernel == "moment":
+            check(L.xh_set_moment_tiles(int(shape)), "xh_set_moment_tiles")
+        else:
+            check(L.xh_set_lane_phases(int(shape)), "xh_set_lane_phases")
+        try:
+            pl = LikelihoodPipeline(dict(model, fold_kernel=kernel), max_batch=len(params))
+        except XHError:
+            continue                      # this kernel does not fit the model
         pl.evaluate(params, stream)
         pl.set_timing(True)
         for _ in range(reps):
             pl.evaluate(params, stream)
-        times[int(c)] = pl.stage_ms()["profile_fold"] / reps
+        times["%s/%d" % (kernel, shape)] = pl.stage_ms()["profile_fold"] / reps
         pl.close()
     best = min(times, key=times.get)
-    check(L.xh_set_moment_tiles(best), "xh_set_moment_tiles")
+    kernel, shape = best.split("/")
+    if kernel == "moment":
+        check(L.xh_set_moment_tiles(int(shape)), "xh_set_moment_tiles")
+    else:
+        check(L.xh_set_lane_phases(int(shape)), "xh_set_lane_phases")
     times["chosen"] = best
-    return times
+    return dict(model, fold_kernel=kernel), times
