@@ -225,11 +225,20 @@ int xh_set_fold_energy_chunk(int ec);
  * point by point, so the >0 clipping is kept). */
 int xh_set_fold_kernel(int kind);
 
-/* Tuning: 16-phase tiles per block of the moment-form fold (1, 2 or 4; more
- * tiles share one profile and coefficient set per parallel but hold more
- * registers).  Applies to pipelines created afterwards and to xh_integrate;
+/* Tuning: block shape of the moment-form fold: T 16-phase tiles x T resident
+ * 16-energy chunks per block of eight waves (T = 1, 2 or 4; larger shapes
+ * share each parallel's moments and coefficients over more of the output but
+ * need more LDS and give fewer blocks).  0 = automatic: the largest shape
+ * whose LDS fits for batches of 32 members or more, 1 below.  Applies to
+ * pipelines created afterwards and to xh_integrate;
  * xpsi_amd.pipeline.autotune_fold measures the choices. */
-int xh_set_moment_tiles(int tiles);
+int xh_set_moment_shape(int tiles);
+
+/* Tuning: parallels per block of the moment-form and energy-lane folds for
+ * batches of 32 members or more (blocks of one member meet by atomic adds; 0 =
+ * automatic: all of a member's parallels in one moment-form block, 8 per
+ * energy-lane block).  Applies to later launches. */
+int xh_set_fold_rings(int rings);
 
 /* Tuning: phases per wave of the energy-lane fold (8 or 16).  Applies to
  * pipelines created afterwards and to xh_integrate. */

@@ -185,65 +185,107 @@ extern "C" int xh_set_fold_kernel(int kind)
     return XH_OK;
 }
 
-// moment-form block: 16 energies x 16 TPB phases (TPB 16-phase tiles sharing
-// one profile and coefficient set per parallel); TPB is a tuning choice
-static int g_moment_tpb = 1;
+// moment-form block shape: T 16-phase tiles x G resident 16-energy chunks
+// (T = G = 1, 2 or 4) per block of eight waves, moment_groups(T) groups of G
+// chunks per block.
+// 0 = automatic: for batches of 32 members or more the largest shape whose
+// LDS fits, else 1 (more, smaller blocks).
+static int g_moment_shape = 0;
 
-extern "C" int xh_set_moment_tiles(int tiles)
+extern "C" int xh_set_moment_shape(int tiles)
 {
-    if (tiles != 1 && tiles != 2 && tiles != 4) return fail(XH_EINVAL, "moment tiles per block must be 1, 2 or 4");
-    g_moment_tpb = tiles;
+    if (tiles != 0 && tiles != 1 && tiles != 2 && tiles != 4)
+        return fail(XH_EINVAL, "moment-form block shape must be 0 (automatic), 1, 2 or 4");
+    g_moment_shape = tiles;
     return XH_OK;
 }
 
-// dynamic LDS of k_moment_fold (mirrors its carve-up; the moment area also
-// holds the 4 x 256 FP64 cross-wave reduction at the end)
-static size_t moment_lds(int N_L, int S_max, size_t rb, int tpb)
+// dynamic LDS of k_moment_fold<R, ATM, T, T> (mirrors its carve-up; the
+// coefficient area also holds the 8 x 256 FP64 partials summed at the end)
+static size_t moment_lds(int N_L, int S_max, size_t rb, int T, int atm)
 {
-    const size_t NI = (size_t)N_L - 1, pcb = 16 * (size_t)tpb;
-    size_t head = (5 * (size_t)N_L + 2 * (size_t)S_max + 8) * 8 + 16 * pcb * 8 + (rb == 4 ? 16 * pcb * 4 : 0) +
-                  (size_t)((N_L + 1) & ~1) * 4;
-    return head + (size_t)N_L * 64 * rb + std::max(NI * 64 * rb, (size_t)4 * 256 * 8);
+    if (N_L > 256) return (size_t)-1;    // one list entry per thread of four waves
+    const size_t NL = (size_t)N_L, NI = NL - 1, G = (size_t)T;
+    size_t off = ((atm == 2 ? 4 : 3) * NL + (size_t)S_max + 8) * 8 + (size_t)S_max * rb + G * NL * 4 +
+                 ((NL + 7) / 8) * 4 + 32 * 4 + NL * 2 + 3 * NL * 2;
+    off = (off + 15) & ~(size_t)15;
+    size_t body = G * NL * 64 * rb + (size_t)T * NI * 64 * rb + std::max(G * NI * 16, G * (size_t)T * 256) * rb;
+    if (T * T < 8) body = std::max(body, (size_t)8 * 4 * 64 * 8);
+    return off + body;
 }
 
-static dim3 moment_grid(int N_E, int N_P, int tpb, int ring_blocks, int members)
+// parallels per fold block for batches of 32 members or more (the moment
+// form and the energy lanes; partial sums meet by atomics); 0 = automatic
+// (every parallel of a member in one moment-form block, 8 per lane block)
+static int g_fold_rings = 0;
+
+extern "C" int xh_set_fold_rings(int rings)
 {
-    const int pcb = 16 * tpb;
-    return dim3((unsigned)(((N_E + 15) / 16) * ((N_P + pcb - 1) / pcb)), (unsigned)ring_blocks, (unsigned)members);
+    if (rings < 0) return fail(XH_EINVAL, "parallels per fold block must be >= 0");
+    g_fold_rings = rings;
+    return XH_OK;
 }
 
-template <typename R, int ATM, int TPB>
+// the block shape of a launch over `members` members
+static int moment_shape(int N_L, int S_max, size_t rb, int atm, int members)
+{
+    if (g_moment_shape > 0) {
+        int T = g_moment_shape;
+        while (T > 1 && moment_lds(N_L, S_max, rb, T, atm) > kMomentLdsMax) T >>= 1;
+        return T;
+    }
+    if (members < 32) return 1;
+    for (int T = 4; T > 1; T >>= 1)
+        if (moment_lds(N_L, S_max, rb, T, atm) <= kMomentLdsMax) return T;
+    return 1;
+}
+
+static dim3 moment_grid(int N_E, int N_P, int T, int ring_blocks, int members)
+{
+    const int epb = 16 * T * moment_groups(T), ppb = 16 * T;
+    return dim3((unsigned)(((N_E + epb - 1) / epb) * ((N_P + ppb - 1) / ppb)), (unsigned)ring_blocks,
+                (unsigned)members);
+}
+
+template <typename R, int ATM, int T>
 static hipError_t launch_moment_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)&k_moment_fold<R, ATM, TPB>,
+        hipError_t e = hipFuncSetAttribute((const void*)&k_moment_fold<R, ATM, T, T>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMomentLdsMax);
         if (e != hipSuccess) return e;
         attr = true;
     }
     if (g.x == 0) return hipSuccess;     // configuration only
-    hipLaunchKernelGGL((k_moment_fold<R, ATM, TPB>), g, dim3(256), lds, s, m, L, F);
+    hipLaunchKernelGGL((k_moment_fold<R, ATM, T, T>), g, dim3(512), lds, s, m, L, F);
     return hipGetLastError();
 }
 
 template <typename R, int ATM>
-static hipError_t launch_moment_a(int tpb, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L,
-                                  FoldArgs F)
+static hipError_t launch_moment_a(int T, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
-    if (tpb == 4) return launch_moment_t<R, ATM, 4>(g, lds, s, m, L, F);
-    if (tpb == 2) return launch_moment_t<R, ATM, 2>(g, lds, s, m, L, F);
+    if (T == 4) return launch_moment_t<R, ATM, 4>(g, lds, s, m, L, F);
+    if (T == 2) return launch_moment_t<R, ATM, 2>(g, lds, s, m, L, F);
     return launch_moment_t<R, ATM, 1>(g, lds, s, m, L, F);
 }
 
-static hipError_t launch_moment(bool fp32, int tpb, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L,
+// T = 0: configure every shape (no launch)
+static hipError_t launch_moment(bool fp32, int T, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L,
                                 FoldArgs F)
 {
     if (lds > kMomentLdsMax) return hipErrorInvalidValue;
-    if (fp32) return L.atm_ext == 2 ? launch_moment_a<float, 2>(tpb, g, lds, s, m, L, F)
-                                    : launch_moment_a<float, 1>(tpb, g, lds, s, m, L, F);
-    return L.atm_ext == 2 ? launch_moment_a<double, 2>(tpb, g, lds, s, m, L, F)
-                          : launch_moment_a<double, 1>(tpb, g, lds, s, m, L, F);
+    if (T == 0) {
+        for (int t = 1; t <= 4; t <<= 1) {
+            hipError_t e = launch_moment(fp32, t, dim3(0, 1, 1), 0, s, m, L, F);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    if (fp32) return L.atm_ext == 2 ? launch_moment_a<float, 2>(T, g, lds, s, m, L, F)
+                                    : launch_moment_a<float, 1>(T, g, lds, s, m, L, F);
+    return L.atm_ext == 2 ? launch_moment_a<double, 2>(T, g, lds, s, m, L, F)
+                          : launch_moment_a<double, 1>(T, g, lds, s, m, L, F);
 }
 
 
@@ -297,14 +339,13 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
         return fail(XH_EINVAL, "xh_integrate: the 4-D table atmosphere needs its axes and values");
     if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1) return fail(XH_EINVAL, "xh_integrate: bad sizes");
     const int EC = pick_ec(N_L);
-    const int tpb = g_moment_tpb;
     const bool lane_fits = lane_lds(N_L, S, 8) <= 160 * 1024;
-    const bool moment_fits = moment_lds(N_L, S, 8, tpb) <= kMomentLdsMax;
+    const bool moment_fits = moment_lds(N_L, S, 8, 1, hot_atm_ext) <= kMomentLdsMax;
     const bool lane = g_fold_kernel == 3 || (g_fold_kernel == 0 && !moment_fits && lane_fits);
     if (g_fold_kernel == 3 && !lane_fits)
         return fail(XH_EINVAL, "xh_integrate: the energy-lane fold does not fit LDS at this leaf count");
     const bool moment = !lane && (g_fold_kernel == 2 || (g_fold_kernel == 0 && moment_fits));
-    if (g_fold_kernel == 2 && moment_lds(N_L, S, 8, tpb) > kMomentLdsMax)
+    if (g_fold_kernel == 2 && !moment_fits)
         return fail(XH_EINVAL, "xh_integrate: the moment-form fold does not fit LDS at this leaf count");
     if (!lane && !moment && N_P > (256 / EC) * 16)
         return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
@@ -356,9 +397,10 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
         const int nrb = std::min(S, 8), rb = (S + nrb - 1) / nrb;
         FA.ring_block = rb;
         HIPCHK(fill(flux, (size_t)N_E * N_P, 0.0, s));
-        HIPCHK(launch_moment(false, tpb, dim3(0, 1, 1), 0, s, d_m, L, FA));
-        HIPCHK(launch_moment(false, tpb, moment_grid(N_E, N_P, tpb, (S + rb - 1) / rb, 1), moment_lds(N_L, S, 8, tpb), s, d_m, L,
-                             FA));
+        const int T = moment_shape(N_L, S, 8, hot_atm_ext, 1);
+        HIPCHK(launch_moment(false, T, dim3(0, 1, 1), 0, s, d_m, L, FA));
+        HIPCHK(launch_moment(false, T, moment_grid(N_E, N_P, T, (S + rb - 1) / rb, 1), moment_lds(N_L, S, 8, T, hot_atm_ext), s,
+                             d_m, L, FA));
     } else {
         HIPCHK(launch_fold(EC, dim3((N_E + EC - 1) / EC, 1, 1), fold_lds(N_L, EC, S), s, d_m, L, FA));
     }
@@ -499,7 +541,7 @@ struct xh_pipeline {
     Layout L;
     int nreg, mpr, maxb, N_in, N_ch, N_b, EC, table_n;
     int fold_kind;               // 1 per-point fold, 2 moment form
-    int tpb;                     // moment form: 16-phase tiles per block
+    int shape;                   // moment form: block shape T (0 = by batch size)
     int lane_nq;                 // energy-lane fold: phases per wave
     bool fp32;                   // precision form
     int fold_terms;              // bf16 products of the precision form's response fold
@@ -598,9 +640,11 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         fail(XH_EINVAL, "pipeline: fold_terms must be 0, 3 or 6"); delete p; return nullptr;
     }
     {
-        p->tpb = g_moment_tpb;
-        while (p->tpb > 1 && moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->tpb) > kMomentLdsMax) p->tpb >>= 1;
-        const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->tpb) <= kMomentLdsMax;
+        // a tuned shape larger than this model's LDS allows falls back to smaller ones
+        p->shape = g_moment_shape;
+        while (p->shape > 1 && moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->shape, c->atm_ext) > kMomentLdsMax)
+            p->shape >>= 1;
+        const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->shape > 0 ? p->shape : 1, c->atm_ext) <= kMomentLdsMax;
         if ((c->fold_kernel == 2 || p->fp32) && !fits) {
             fail(XH_EINVAL, "pipeline: the moment-form fold does not fit LDS at this leaf count and mesh bound");
             delete p; return nullptr;
@@ -627,7 +671,7 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         FoldArgs F0{};
         p->lane_nq = g_lane_nq;
         if (p->fold_kind == 3) PHIP(launch_lane(p->fp32, p->lane_nq, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
-        else if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, p->tpb, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
+        else if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, 0, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
         else PHIP(launch_fold(p->EC, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
     }
     if (p->fold_kind == 1 && N_P > (256 / p->EC) * 16) { fail(XH_EINVAL, "too many phases for the leaf count"); delete p; return nullptr; }
@@ -960,7 +1004,8 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     // (the energy-lane fold: 8 parallels per block, so that members of very
     // different sizes finish in similar times; partial sums meet by atomics)
     int rb = (n >= 32) ? S : 1;
-    if (p->fold_kind == 3 && n >= 32) rb = std::min(S, 8);
+    if (n >= 32 && g_fold_rings > 0 && p->fold_kind != 1) rb = std::min(S, g_fold_rings);
+    else if (n >= 32 && p->fold_kind == 3) rb = std::min(S, 8);
     FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_effGrav, p->d_lastq,
                 p->d_atm_axes, p->d_atm_I, {p->atm_N[0], p->atm_N[1], p->atm_N[2], p->atm_N[3]}, p->d_energies,
                 p->d_phases, p->d_flux, rb};
@@ -969,8 +1014,12 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
         HIPCHK(launch_lane(p->fp32, p->lane_nq, lane_grid(N_E, N_P, p->lane_nq, (S + rb - 1) / rb, nm),
                            lane_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8), s, p->d_members, L, FA));
     else if (p->fold_kind == 2)
-        HIPCHK(launch_moment(p->fp32, p->tpb, moment_grid(N_E, N_P, p->tpb, (S + rb - 1) / rb, nm),
-                             moment_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8, p->tpb), s, p->d_members, L, FA));
+    {
+        const size_t rbytes = p->fp32 ? 4 : 8;
+        const int T = p->shape > 0 ? p->shape : moment_shape(L.N_L, L.S_max, rbytes, L.atm_ext, n);
+        HIPCHK(launch_moment(p->fp32, T, moment_grid(N_E, N_P, T, (S + rb - 1) / rb, nm),
+                             moment_lds(L.N_L, L.S_max, rbytes, T, L.atm_ext), s, p->d_members, L, FA));
+    }
     else
         HIPCHK(launch_fold(p->EC, dim3((N_E + p->EC - 1) / p->EC, (S + rb - 1) / rb, nm), fold_lds(L.N_L, p->EC, S),
                            s, p->d_members, L, FA));

@@ -94,6 +94,10 @@ struct FoldArgs {
     int ring_block;
 };
 
+// moment form: groups of T resident 16-energy chunks per block of shape T
+// (the accumulators of a wave stay within 16 doubles)
+__host__ __device__ constexpr int moment_groups(int T) { return T == 4 ? 2 : 4; }
+
 __global__ void k_geometry(Member* members, Layout L, RingArgs A);
 // closed mesh and time-invariant spectrum of the elsewhere component
 __global__ void k_closed_table(const Member* members, double* area_tab, double* total, int table_n);
@@ -111,7 +115,7 @@ __global__ void k_else_add(const Member* else_members, Layout Le, const double* 
                            Member* hot_members, int members_per_request, double* flux);
 template <int EC, int MAXK, int ATM>
 __global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
-template <typename R, int ATM, int TPB>
+template <typename R, int ATM, int T, int G>
 __global__ void k_moment_fold(const Member* members, Layout L, FoldArgs F);
 template <typename R, int ATM, int NQ>
 __global__ void k_lane_fold(const Member* members, Layout L, FoldArgs F);

@@ -263,23 +263,28 @@ def params_from_model(model):
 
 
 def autotune_fold(model, params, stream=None, reps=2,
-                  candidates=(("moment", 1), ("moment", 2), ("moment", 4), ("lanes", 8), ("lanes", 16))):
-    """Pick the spline-fold kernel and its tile shape for this model and batch.
+                  candidates=(("moment", 4), ("moment", 2), ("moment", 1), ("lanes", 8))):
+    """Pick the spline-fold kernel and its block shape for this model and batch.
 
-    Candidates are (kernel, shape): the moment form with 1, 2 or 4 16-phase
-    tiles per block, or the energy-lane form with 8 or 16 phases per wave.
-    For each, a pipeline evaluates `params` once untimed and `reps` times with
-    HIP-event stage timing; the candidate with the least profile-and-fold
-    time wins.  Its shape is set for pipelines created afterwards, and the
-    returned model dict selects its kernel.  Returns (model, timings) with
-    timings {"kernel/shape": fold ms per evaluation, "chosen": "kernel/shape"}."""
+    Candidates are (kernel, shape): the moment form with blocks of `shape`
+    16-phase tiles x `shape` resident 16-energy chunks, or the energy-lane
+    form with `shape` phases per wave.  For each, a pipeline evaluates
+    `params` once untimed and `reps` times with HIP-event stage timing; the
+    candidate with the least profile-and-fold time wins.  Its shape is set for
+    pipelines created afterwards, and the returned model dict selects its
+    kernel.  Returns (model, timings) with timings {"kernel/shape": fold ms
+    per evaluation, "chosen": "kernel/shape"}."""
     L = lib()
-    times = {}
-    for kernel, shape in candidates:
+
+    def select(kernel, shape):
         if kernel == "moment":
-            check(L.xh_set_moment_tiles(int(shape)), "xh_set_moment_tiles")
+            check(L.xh_set_moment_shape(int(shape)), "xh_set_moment_shape")
         else:
             check(L.xh_set_lane_phases(int(shape)), "xh_set_lane_phases")
+
+    times = {}
+    for kernel, shape in candidates:
+        select(kernel, shape)
         try:
             pl = LikelihoodPipeline(dict(model, fold_kernel=kernel), max_batch=len(params))
         except XHError:
@@ -292,9 +297,6 @@ def autotune_fold(model, params, stream=None, reps=2,
         pl.close()
     best = min(times, key=times.get)
     kernel, shape = best.split("/")
-    if kernel == "moment":
-        check(L.xh_set_moment_tiles(int(shape)), "xh_set_moment_tiles")
-    else:
-        check(L.xh_set_lane_phases(int(shape)), "xh_set_lane_phases")
+    select(kernel, int(shape))
     times["chosen"] = best
     return dict(model, fold_kernel=kernel), times
