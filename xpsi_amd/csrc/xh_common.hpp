@@ -298,8 +298,10 @@ __device__ __forceinline__ double scale_error(double err, double rabs, double ra
 // One 61-point Gauss-Kronrod pass; every lane of the wave must call it and
 // every lane receives the same results.  Lanes 0..29: centre - h*x_k,
 // lanes 30..59: centre + h*x_k, lane 60: centre, lanes 61..63 idle.
+// Out of line: inlined at each QAG step, the integrand's FP64 temporaries
+// multiply the caller's register count (and cut its occupancy).
 template <class F>
-__device__ inline void wave_gk61(F& f, double a, double b, double& res, double& err,
+__device__ __noinline__ void wave_gk61(F& f, double a, double b, double& res, double& err,
                                  double& rabs, double& rasc)
 {
     const int lane = lane_id();

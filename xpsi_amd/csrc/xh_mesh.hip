@@ -130,7 +130,7 @@ struct RegionFn {
     }
 };
 
-__device__ double wave_area(double lo, double hi, double R_eq, double eps, double zeta, int shape, int av)
+__device__ __noinline__ double wave_area(double lo, double hi, double R_eq, double eps, double zeta, int shape, int av)
 {
     AreaFn f{eps, zeta, av, shape};
     double r, e;
@@ -246,7 +246,7 @@ __device__ double wave_pieces(const Fn& f, double lo, double hi, const double* b
 }
 
 // mesh_tools.pyx:429-725 (wave-cooperative)
-__device__ double wave_cell(double ta, double tb, double pa, double pb, double R_eq, double eps,
+__device__ __noinline__ double wave_cell(double ta, double tb, double pa, double pb, double R_eq, double eps,
                             double zeta, int shape, double colat, double cedeR, double superR,
                             double superAzi, double superColat)
 {
@@ -451,8 +451,10 @@ __global__ void __launch_bounds__(256) k_mesh_table(const Member* __restrict__ m
 
 // One block of four waves per ring: wave 0 finds the parallels and the
 // centroid colatitude, all threads classify the ring's cells, then the four
-// waves integrate the boundary cells between them.
-__global__ void __launch_bounds__(256) k_mesh_rings(const Member* __restrict__ members,
+// waves integrate the boundary cells between them.  The phases are latency
+// bound, so the register budget is capped for three waves per SIMD (with the
+// integrals out of line).
+__global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict__ members,
                                                     const double* __restrict__ area_tab,
                                                     const double* __restrict__ total, int table_n,
                                                     Layout L, double* __restrict__ theta,
