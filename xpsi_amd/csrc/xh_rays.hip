@@ -14,52 +14,85 @@
 
 namespace xh {
 
+// 1/sqrt(q) and 1/d from the hardware estimates with two Newton steps each
+// (within an ulp or two of the correctly rounded values, at a third of the
+// cost of the IEEE sequences); non-finite estimates (q or d zero or not
+// positive) fall back to the IEEE operations
+__device__ __forceinline__ double rsqrt_nr(double q)
+{
+    double y = __builtin_amdgcn_rsq(q);
+    double e = fma(-q, y * y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-q, y * y, 1.0);
+    y = fma(0.5 * y, e, y);
+    return isfinite(y) && q > 0.0 ? y : 1.0 / sqrt(q);
+}
+__device__ __forceinline__ double rcp_nr(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    return isfinite(r) && d != 0.0 ? r : 1.0 / d;
+}
+
+// integrands of rays.pyx:61-88; k = 1 / (rc - 1) (or 1 / (R/r_s - 1)) is
+// formed once per ray
 struct InDef {
-    double rc;
+    double k;
     __device__ double operator()(double x) const
     {
-        return 1.0 / sqrt(2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (rc - 1.0));
+        const double w = 1.0 - x * x;
+        return rsqrt_nr(2.0 - x * x - w * w * k);
     }
 };
 struct InLag {
-    double rc;
+    double k;
     __device__ double operator()(double x) const
     {
-        double X = 1.0 / sqrt(2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (rc - 1.0));
-        return X * X / (x + X);
+        const double w = 1.0 - x * x;
+        const double X = rsqrt_nr(2.0 - x * x - w * w * k);
+        return X * X * rcp_nr(x + X);
     }
 };
 struct InBoth {
-    double rc;
+    double k;
     __device__ void both(double x, double& d, double& l) const
     {
-        double X = 1.0 / sqrt(2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (rc - 1.0));
+        const double w = 1.0 - x * x;
+        const double X = rsqrt_nr(2.0 - x * x - w * w * k);
         d = X;
-        l = X * X / (x + X);
+        l = X * X * rcp_nr(x + X);
     }
 };
 struct OutDef {
-    double sas, Rr;
+    double sas, k;
     __device__ double operator()(double x) const
     {
-        return x / sqrt(1.0 - sas + x * x * sas * (2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (Rr - 1.0)));
+        const double w = 1.0 - x * x;
+        return x * rsqrt_nr(1.0 - sas + x * x * sas * (2.0 - x * x - w * w * k));
     }
 };
 struct OutLag {
-    double sas, Rr;
+    double sas, k;
     __device__ double operator()(double x) const
     {
-        double f = sqrt(1.0 - sas + x * x * sas * (2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (Rr - 1.0)));
-        return x / (f + f * f);
+        const double w = 1.0 - x * x;
+        const double q = 1.0 - sas + x * x * sas * (2.0 - x * x - w * w * k);
+        const double ri = rsqrt_nr(q), f = q * ri;
+        return x * ri * rcp_nr(1.0 + f);
     }
 };
 struct OutBoth {
-    double sas, Rr;
+    double sas, k;
     __device__ void both(double x, double& d, double& l) const
     {
-        double f = sqrt(1.0 - sas + x * x * sas * (2.0 - x * x - (1.0 - x * x) * (1.0 - x * x) / (Rr - 1.0)));
-        d = x / f;
-        l = x / (f + f * f);
+        const double w = 1.0 - x * x;
+        const double q = 1.0 - sas + x * x * sas * (2.0 - x * x - w * w * k);
+        const double ri = rsqrt_nr(q), f = q * ri;
+        d = x * ri;
+        l = x * ri * rcp_nr(1.0 + f);
     }
 };
 
@@ -86,9 +119,9 @@ __device__ void wave_ray(double cos_alpha, double r_s, double u, double& defl, d
     double d0 = __builtin_nan(""), l0 = __builtin_nan("");
     int sd = XH_SUCCESS, sl = XH_SUCCESS;
     if (b <= b_ph) {
-        OutBoth fb{sas, 1.0 / u};
-        OutDef fd{sas, 1.0 / u};
-        OutLag fl{sas, 1.0 / u};
+        OutBoth fb{sas, 1.0 / (1.0 / u - 1.0)};
+        OutDef fd{sas, 1.0 / (1.0 / u - 1.0)};
+        OutLag fl{sas, 1.0 / (1.0 / u - 1.0)};
         ray_pair(fb, fd, fl, 0.0, 1.0, d0, l0, sd, sl);
         if (sd == XH_EFAILED || sl == XH_EFAILED) d0 = __builtin_nan("");
         else {
@@ -103,9 +136,9 @@ __device__ void wave_ray(double cos_alpha, double r_s, double u, double& defl, d
             wR = sqrt(1.0 - rc * u);
         }
         if (isnan(wR)) wR = 0.0;
-        InBoth fb{rc};
-        InDef fd{rc};
-        InLag fl{rc};
+        InBoth fb{1.0 / (rc - 1.0)};
+        InDef fd{1.0 / (rc - 1.0)};
+        InLag fl{1.0 / (rc - 1.0)};
         ray_pair(fb, fd, fl, wR, 1.0, d0, l0, sd, sl);
         if (sd == XH_EFAILED || sl == XH_EFAILED) d0 = __builtin_nan("");
         else {
@@ -116,9 +149,9 @@ __device__ void wave_ray(double cos_alpha, double r_s, double u, double& defl, d
         double rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
         double wR = sqrt(1.0 - rc * u);
         if (isnan(wR)) wR = 0.0;
-        InBoth fb{rc};
-        InDef fd{rc};
-        InLag fl{rc};
+        InBoth fb{1.0 / (rc - 1.0)};
+        InDef fd{1.0 / (rc - 1.0)};
+        InLag fl{1.0 / (rc - 1.0)};
         double D1 = 0, D0 = 0, L1 = 0, L0 = 0;
         int s1, s2, s3, s4;
         ray_pair(fb, fd, fl, 0.0, 1.0, D1, L1, s1, s3);
@@ -155,7 +188,7 @@ __device__ __forceinline__ double clamped_max_alpha(double maxA, double u)
 // the mean-deviation term takes a second sweep over the nodes.
 template <class FB>
 __device__ void lane_gk61_pair(const FB& f, double a, double b, double res[2], double err[2], double rabs[2],
-                               double rasc[2])
+                               double rasc[2], unsigned* __restrict__ sc)
 {
     const double ctr = 0.5 * (a + b), hl = 0.5 * (b - a), ahl = fabs(hl);
     double fc0, fc1;
@@ -163,6 +196,11 @@ __device__ void lane_gk61_pair(const FB& f, double a, double b, double res[2], d
     double rk0 = WGK61[30] * fc0, rk1 = WGK61[30] * fc1;
     double rg0 = 0.0, rg1 = 0.0;
     double ra0 = WGK61[30] * fabs(fc0), ra1 = WGK61[30] * fabs(fc1);
+    // node values relative to the centre value, parked in LDS (sc, stride 64)
+    // as bfloat16 pairs for the asc term: QUADPACK's resasc only scales the
+    // error estimate, and its relative error here stays below 2^-8
+    auto bf = [](double v) { return (unsigned)(__float_as_uint((float)v) >> 16); };
+    auto ub = [](unsigned h) { return (double)__uint_as_float(h << 16); };
     for (int k = 0; k < 30; k++) {
         const double dx = hl * XGK61[k], wk = WGK61[k];
         double fa0, fa1, fb0, fb1;
@@ -177,16 +215,17 @@ __device__ void lane_gk61_pair(const FB& f, double a, double b, double res[2], d
             rg0 += wg * (fa0 + fb0);
             rg1 += wg * (fa1 + fb1);
         }
+        sc[(2 * k) * 64] = bf(fa0 - fc0) | (bf(fb0 - fc0) << 16);
+        sc[(2 * k + 1) * 64] = bf(fa1 - fc1) | (bf(fb1 - fc1) << 16);
     }
     const double m0 = 0.5 * rk0, m1 = 0.5 * rk1;
-    double rs0 = WGK61[30] * fabs(fc0 - m0), rs1 = WGK61[30] * fabs(fc1 - m1);
+    const double c0 = fc0 - m0, c1 = fc1 - m1;
+    double rs0 = WGK61[30] * fabs(c0), rs1 = WGK61[30] * fabs(c1);
     for (int k = 0; k < 30; k++) {
-        const double dx = hl * XGK61[k], wk = WGK61[k];
-        double fa0, fa1, fb0, fb1;
-        f.both(ctr - dx, fa0, fa1);
-        f.both(ctr + dx, fb0, fb1);
-        rs0 += wk * (fabs(fa0 - m0) + fabs(fb0 - m0));
-        rs1 += wk * (fabs(fa1 - m1) + fabs(fb1 - m1));
+        const double wk = WGK61[k];
+        const unsigned p0 = sc[(2 * k) * 64], p1 = sc[(2 * k + 1) * 64];
+        rs0 += wk * (fabs(ub(p0 & 0xffffu) + c0) + fabs(ub(p0 >> 16) + c0));
+        rs1 += wk * (fabs(ub(p1 & 0xffffu) + c1) + fabs(ub(p1 >> 16) + c1));
     }
     res[0] = rk0 * hl; res[1] = rk1 * hl;
     rabs[0] = ra0 * ahl; rabs[1] = ra1 * ahl;
@@ -206,10 +245,10 @@ __device__ __forceinline__ bool qag_first_pass_stands(double r, double e, double
 }
 
 template <class FB>
-__device__ __forceinline__ bool lane_ray_pair(const FB& fb, double a, double b, double& d, double& l)
+__device__ __forceinline__ bool lane_ray_pair(const FB& fb, double a, double b, double& d, double& l, unsigned* sc)
 {
     double r[2], e[2], ra[2], rs[2];
-    lane_gk61_pair(fb, a, b, r, e, ra, rs);
+    lane_gk61_pair(fb, a, b, r, e, ra, rs, sc);
     d = r[0];
     l = r[1];
     return qag_first_pass_stands(r[0], e[0], ra[0], rs[0], 1.0e-12) &&
@@ -218,7 +257,7 @@ __device__ __forceinline__ bool lane_ray_pair(const FB& fb, double a, double b, 
 
 // wave_ray's branches with every quadrature settled by its first pass;
 // returns false when some quadrature needs QAG's bisection
-__device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, double& lag)
+__device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, double& lag, unsigned* sc)
 {
     const double b_ph = 3.0 * sqrt(3.0) / 2.0;
     double sas = 1.0 - cos_alpha * cos_alpha;
@@ -227,8 +266,8 @@ __device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, d
     double b = sa / (u * sqrt(1.0 - u));
     double d0, l0;
     if (b <= b_ph) {
-        OutBoth fb{sas, 1.0 / u};
-        if (!lane_ray_pair(fb, 0.0, 1.0, d0, l0)) return false;
+        OutBoth fb{sas, 1.0 / (1.0 / u - 1.0)};
+        if (!lane_ray_pair(fb, 0.0, 1.0, d0, l0, sc)) return false;
         d0 *= 2.0 * b * u;
         l0 *= 2.0 * b * b * u * r_s;
     } else if (alpha <= PI / 2.0) {
@@ -239,18 +278,18 @@ __device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, d
             wR = sqrt(1.0 - rc * u);
         }
         if (isnan(wR)) wR = 0.0;
-        InBoth fb{rc};
-        if (!lane_ray_pair(fb, wR, 1.0, d0, l0)) return false;
+        InBoth fb{1.0 / (rc - 1.0)};
+        if (!lane_ray_pair(fb, wR, 1.0, d0, l0, sc)) return false;
         d0 *= 2.0 * b / rc;
         l0 *= 2.0 * b * b * r_s / rc;
     } else {
         double rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
         double wR = sqrt(1.0 - rc * u);
         if (isnan(wR)) wR = 0.0;
-        InBoth fb{rc};
+        InBoth fb{1.0 / (rc - 1.0)};
         double D1, D0, L1, L0;
-        if (!lane_ray_pair(fb, 0.0, 1.0, D1, L1)) return false;
-        if (!lane_ray_pair(fb, wR, 1.0, D0, L0)) return false;
+        if (!lane_ray_pair(fb, 0.0, 1.0, D1, L1, sc)) return false;
+        if (!lane_ray_pair(fb, wR, 1.0, D0, L0, sc)) return false;
         d0 = 2.0 * b * (2.0 * D1 - D0) / rc;
         l0 = 2.0 * b * b * r_s * (2.0 * L1 - L0) / rc;
         double cor = 1.0 / u - rc + log((1.0 / u - 1.0) / (rc - 1.0));
@@ -283,7 +322,8 @@ __global__ void __launch_bounds__(64) k_rays_lane(const Member* __restrict__ mem
     const size_t o = ro * L.N_R + j;
     cos_alpha[o] = ca;
     double d, l;
-    if (lane_ray(ca, m.r_s, u, d, l)) {
+    __shared__ unsigned sc[60 * 64];   // per lane: 60 words at stride 64
+    if (lane_ray(ca, m.r_s, u, d, l, sc + threadIdx.x)) {
         defl_raw[o] = d;
         lag_raw[o] = l;
     } else {
