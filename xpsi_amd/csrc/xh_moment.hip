@@ -475,30 +475,50 @@ __global__ void __launch_bounds__(512) k_moment_fold(const Member* __restrict__ 
                     }
                     __syncthreads();
                 }
-                // the sweeps of the wave's pairs (tile wt, chunk wg) over their
-                // folded intervals: sign-changing pieces masked out of the A operand
+                // the sweeps of the wave's pairs over the folded intervals its
+                // tile reaches: the pairs share the tile (pair0 + u * 8 has
+                // tile pair0 % T), so one B operand feeds PPW independent
+                // accumulator chains; sign-changing pieces are masked out of A
+                {
+                    const int wt = pair0 % T;
+                    typename Mfma<R>::v4 sw[PPW];
+                    bool on[PPW];
+                    const R* cg[PPW];
+                    const unsigned* mxg[PPW];
 #pragma unroll
-                for (int u = 0; u < PPW; u++) {
-                    const int pair = pair0 + u * NW, wt = pair % T, wg = pair / T;
-                    if (e0 + wg * 16 >= N_E) continue;
-                    typename Mfma<R>::v4 sw = {(R)0, (R)0, (R)0, (R)0};
+                    for (int u = 0; u < PPW; u++) {
+                        const int wg = (pair0 + u * NW) / T;
+                        on[u] = e0 + wg * 16 < N_E;
+                        cg[u] = CO + wg * N_L * 64;
+                        mxg[u] = MIX + wg * N_L;
+                        sw[u] = typename Mfma<R>::v4{(R)0, (R)0, (R)0, (R)0};
+                    }
                     const int kb = lane & 15;
-                    const R* cg = CO + wg * N_L * 64;
                     const R* mt = MOM + wt * NI * 64;
-                    const unsigned* mxg = MIX + wg * N_L;
-                    for (int q = sub; q < n_i; q += WPP) {
-                        const int idx = IL[q];
-                        if (!((hits(idx) >> wt) & 1u)) continue;   // the tile's moments are zero
-                        R a = cg[idx * 64 + lane];
-                        if ((mxg[idx] >> kb) & 1u) a = (R)0;
-                        sw = Mfma<R>::step(a, mt[idx * 64 + lane], sw);
+                    if (on[0]) {
+                        for (int q = sub; q < n_i; q += WPP) {
+                            const int idx = IL[q];
+                            if (!((hits(idx) >> wt) & 1u)) continue;   // the tile's moments are zero
+                            const R bm = mt[idx * 64 + lane];
+#pragma unroll
+                            for (int u = 0; u < PPW; u++) {
+                                R a = cg[u][idx * 64 + lane];
+                                if ((mxg[u][idx] >> kb) & 1u) a = (R)0;
+                                sw[u] = Mfma<R>::step(a, bm, sw[u]);
+                            }
+                        }
                     }
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        double v = (double)sw[r];
-                        if (mixed && sub == 0)
-                            v += (double)DIR[(wg * 16 + Mfma<R>::row(lane, r)) * PT + wt * 16 + (lane & 15)];
-                        tot[u][gi][r] += v * a_ref;
+                    for (int u = 0; u < PPW; u++) {
+                        if (!on[u]) continue;
+                        const int wg = (pair0 + u * NW) / T;
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            double v = (double)sw[u][r];
+                            if (mixed && sub == 0)
+                                v += (double)DIR[(wg * 16 + Mfma<R>::row(lane, r)) * PT + wt * 16 + (lane & 15)];
+                            tot[u][gi][r] += v * a_ref;
+                        }
                     }
                 }
             }
