@@ -48,6 +48,9 @@ def main():
     ap.add_argument("--c5-n", type=int, default=64, help="large-workload draws (0 = skip)")
     ap.add_argument("--c5-steps", type=int, default=1)
     ap.add_argument("--c5-check", type=int, default=16, help="large-workload draws re-run in FP64")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB copied device to device once after the timed region: a plain copy of "
+                         "known size that calibrates the HBM byte counters of a profiling run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the likelihood path shards requests across GPUs "
@@ -182,7 +185,8 @@ def main():
         "dtype": prec_desc, "data": "synthetic",
         "precision": {"fp64": "FP64 throughout",
                       "mixed": "FP32 intensity + spline fold (f32 MFMA moment form), split-BF16 "
-                               "3-term response fold, FP64 rays/mesh/phase-bin sums/likelihood"}[
+                               "response fold (six products of a three-term split, FP32 "
+                               "accumulation), FP64 rays/mesh/phase-bin sums/likelihood"}[
                           args.precision],
         "config": {"workload": "batched form: %d prior draws (%d accepted), uneven "
                                "sqrt_num_cells {24..96} x num_energies {64,128,256}, BB spot, "
@@ -210,6 +214,14 @@ def main():
         "device": dev,
     }
     out["roofline"]["frac"] = out["roofline"]["achieved"] / out["roofline"]["peak"]
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = src.numel()
+        del src, dst
     if args.c5_n > 0:
         out["config5"] = config5_section(args, stream)
     if not args.no_serving:
