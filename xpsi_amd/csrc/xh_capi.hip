@@ -247,27 +247,42 @@ static dim3 moment_grid(int N_E, int N_P, int T, int ring_blocks, int members)
                 (unsigned)members);
 }
 
-template <typename R, int ATM, int T>
+template <typename R, int ATM, int T, int OCC>
 static hipError_t launch_moment_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)&k_moment_fold<R, ATM, T, T>,
+        hipError_t e = hipFuncSetAttribute((const void*)&k_moment_fold<R, ATM, T, T, OCC>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMomentLdsMax);
         if (e != hipSuccess) return e;
         attr = true;
     }
     if (g.x == 0) return hipSuccess;     // configuration only
-    hipLaunchKernelGGL((k_moment_fold<R, ATM, T, T>), g, dim3(512), lds, s, m, L, F);
+    hipLaunchKernelGGL((k_moment_fold<R, ATM, T, T, OCC>), g, dim3(512), lds, s, m, L, F);
     return hipGetLastError();
+}
+
+// shapes T < 4 with two blocks per CU in LDS use the variant whose register
+// budget is sized for four waves per SIMD
+template <typename R, int ATM, int T>
+static hipError_t launch_moment_o(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
+{
+    if constexpr (T < 4) {
+        if (g.x == 0) {
+            hipError_t e = launch_moment_t<R, ATM, T, 2>(g, 0, s, m, L, F);
+            return e != hipSuccess ? e : launch_moment_t<R, ATM, T, 4>(g, 0, s, m, L, F);
+        }
+        if (2 * lds <= kMomentLdsMax) return launch_moment_t<R, ATM, T, 4>(g, lds, s, m, L, F);
+    }
+    return launch_moment_t<R, ATM, T, 2>(g, lds, s, m, L, F);
 }
 
 template <typename R, int ATM>
 static hipError_t launch_moment_a(int T, dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
 {
-    if (T == 4) return launch_moment_t<R, ATM, 4>(g, lds, s, m, L, F);
-    if (T == 2) return launch_moment_t<R, ATM, 2>(g, lds, s, m, L, F);
-    return launch_moment_t<R, ATM, 1>(g, lds, s, m, L, F);
+    if (T == 4) return launch_moment_o<R, ATM, 4>(g, lds, s, m, L, F);
+    if (T == 2) return launch_moment_o<R, ATM, 2>(g, lds, s, m, L, F);
+    return launch_moment_o<R, ATM, 1>(g, lds, s, m, L, F);
 }
 
 // T = 0: configure every shape (no launch)

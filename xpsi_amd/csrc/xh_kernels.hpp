@@ -115,7 +115,7 @@ __global__ void k_else_add(const Member* else_members, Layout Le, const double* 
                            Member* hot_members, int members_per_request, double* flux);
 template <int EC, int MAXK, int ATM>
 __global__ void k_profile_fold(const Member* members, Layout L, FoldArgs F);
-template <typename R, int ATM, int T, int G>
+template <typename R, int ATM, int T, int G, int OCC>
 __global__ void k_moment_fold(const Member* members, Layout L, FoldArgs F);
 template <typename R, int ATM, int NQ>
 __global__ void k_lane_fold(const Member* members, Layout L, FoldArgs F);

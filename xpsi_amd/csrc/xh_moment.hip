@@ -199,8 +199,10 @@ __device__ __forceinline__ int compact_u16(bool flag, int value, unsigned short*
 // w, w + 8, ... over all listed intervals; with fewer, the 8 / (T G) waves of
 // a pair share its intervals.  Each wave adds each parallel's sweeps into
 // FP64 accumulators in registers; a pair's waves are summed at the end.
-template <typename R, int ATM, int T, int G>
-__global__ void __launch_bounds__(512) k_moment_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
+// OCC = waves per SIMD the register budget is sized for: 4 when two blocks
+// fit in LDS (the occupancy outweighs the spills the cap costs), else 2.
+template <typename R, int ATM, int T, int G, int OCC>
+__global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restrict__ members, Layout L, FoldArgs F)
 {
     constexpr int NT = 512, NW = 8, P = T * G;
     constexpr int PPW = P >= NW ? P / NW : 1, WPP = P >= NW ? 1 : NW / P;   // pairs per wave, waves per pair
@@ -554,10 +556,12 @@ __global__ void __launch_bounds__(512) k_moment_fold(const Member* __restrict__ 
     }
 }
 
-#define XH_MOMENT_INST(R, T, G)                                                             \
-    template __global__ void k_moment_fold<R, 1, T, G>(const Member*, Layout, FoldArgs);   \
-    template __global__ void k_moment_fold<R, 2, T, G>(const Member*, Layout, FoldArgs);
-XH_MOMENT_INST(double, 1, 1) XH_MOMENT_INST(double, 2, 2) XH_MOMENT_INST(double, 4, 4)
-XH_MOMENT_INST(float, 1, 1) XH_MOMENT_INST(float, 2, 2) XH_MOMENT_INST(float, 4, 4)
+#define XH_MOMENT_INST(R, T, OCC)                                                                \
+    template __global__ void k_moment_fold<R, 1, T, T, OCC>(const Member*, Layout, FoldArgs);   \
+    template __global__ void k_moment_fold<R, 2, T, T, OCC>(const Member*, Layout, FoldArgs);
+XH_MOMENT_INST(double, 1, 2) XH_MOMENT_INST(double, 2, 2) XH_MOMENT_INST(double, 4, 2)
+XH_MOMENT_INST(double, 1, 4) XH_MOMENT_INST(double, 2, 4)
+XH_MOMENT_INST(float, 1, 2) XH_MOMENT_INST(float, 2, 2) XH_MOMENT_INST(float, 4, 2)
+XH_MOMENT_INST(float, 1, 4) XH_MOMENT_INST(float, 2, 4)
 
 }  // namespace xh
