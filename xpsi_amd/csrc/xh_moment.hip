@@ -482,8 +482,10 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                 // tile pair0 % T), so one B operand feeds PPW independent
                 // accumulator chains; sign-changing pieces are masked out of A
                 {
+                    // alternate intervals feed two chains per pair (A, B), summed
+                    // in FP64: shorter FP32 accumulations and two MFMAs in flight
                     const int wt = pair0 % T;
-                    typename Mfma<R>::v4 sw[PPW];
+                    typename Mfma<R>::v4 swA[PPW], swB[PPW];
                     bool on[PPW];
                     const R* cg[PPW];
                     const unsigned* mxg[PPW];
@@ -493,22 +495,28 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                         on[u] = e0 + wg * 16 < N_E;
                         cg[u] = CO + wg * N_L * 64;
                         mxg[u] = MIX + wg * N_L;
-                        sw[u] = typename Mfma<R>::v4{(R)0, (R)0, (R)0, (R)0};
+                        swA[u] = typename Mfma<R>::v4{(R)0, (R)0, (R)0, (R)0};
+                        swB[u] = swA[u];
                     }
                     const int kb = lane & 15;
                     const R* mt = MOM + wt * NI * 64;
-                    if (on[0]) {
-                        for (int q = sub; q < n_i; q += WPP) {
-                            const int idx = IL[q];
-                            if (!((hits(idx) >> wt) & 1u)) continue;   // the tile's moments are zero
-                            const R bm = mt[idx * 64 + lane];
+                    auto sweep = [&](int idx, typename Mfma<R>::v4* sw) {
+                        if (!((hits(idx) >> wt) & 1u)) return;   // the tile's moments are zero
+                        const R bm = mt[idx * 64 + lane];
 #pragma unroll
-                            for (int u = 0; u < PPW; u++) {
-                                R a = cg[u][idx * 64 + lane];
-                                if ((mxg[u][idx] >> kb) & 1u) a = (R)0;
-                                sw[u] = Mfma<R>::step(a, bm, sw[u]);
-                            }
+                        for (int u = 0; u < PPW; u++) {
+                            R a = cg[u][idx * 64 + lane];
+                            if ((mxg[u][idx] >> kb) & 1u) a = (R)0;
+                            sw[u] = Mfma<R>::step(a, bm, sw[u]);
                         }
+                    };
+                    if (on[0]) {
+                        int q = sub;
+                        for (; q + WPP < n_i; q += 2 * WPP) {
+                            sweep(IL[q], swA);
+                            sweep(IL[q + WPP], swB);
+                        }
+                        if (q < n_i) sweep(IL[q], swA);
                     }
 #pragma unroll
                     for (int u = 0; u < PPW; u++) {
@@ -516,7 +524,7 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                         const int wg = (pair0 + u * NW) / T;
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
-                            double v = (double)sw[u][r];
+                            double v = (double)swA[u][r] + (double)swB[u][r];
                             if (mixed && sub == 0)
                                 v += (double)DIR[(wg * 16 + Mfma<R>::row(lane, r)) * PT + wt * 16 + (lane & 15)];
                             tot[u][gi][r] += v * a_ref;
