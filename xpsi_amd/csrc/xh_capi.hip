@@ -387,7 +387,7 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     HIPCHK(hipGetLastError());
     RingArgs RA{d_theta, radial, r_s_over_r, cos_gamma, maxDeflection, deflection, cos_alpha, lag, leaves, d_ok,
                 d_geo, d_valid, d_lastq};
-    hipLaunchKernelGGL(k_geometry, dim3(S, IO, 1), dim3(256), geo_lds(N_R, N_L), s, d_m, L, RA);
+    hipLaunchKernelGGL(k_geometry, dim3(S, 1, 1), dim3(128), geo_lds(N_R, N_L), s, d_m, L, RA);
     HIPCHK(hipGetLastError());
     FoldArgs FA{d_geo, d_valid, cellArea, radiates, phi, S, d_logT, d_logg, d_lastq, atm_axes, atm_I,
                 {0, 0, 0, 0}, energies, phases, flux, S};
@@ -1007,7 +1007,7 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     RingArgs RA{p->d_theta, p->d_r, p->d_u, p->d_cos_gamma, p->d_maxDefl, p->d_defl, p->d_cosa, p->d_lag,
                 p->d_leaves, p->d_ring_ok, p->d_geo, p->d_valid, p->d_lastq};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[2], s));
-    hipLaunchKernelGGL(k_geometry, dim3(S, L.IO_max, nm), dim3(256), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
+    hipLaunchKernelGGL(k_geometry, dim3(S, 1, nm), dim3(128), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
     HIPCHK(hipGetLastError());
     if (p->has_else) {
         hipLaunchKernelGGL(k_else_integrate, dim3(Le.S_max, n), dim3(256), else_lds(Le), s, p->d_else, Le,

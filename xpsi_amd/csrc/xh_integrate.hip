@@ -4,9 +4,10 @@
 // :70-665 (the leaf x energy loop :342-549 and the energy x cell x phase
 // spline loop :565-596), blackbody intensity hot_BB.pyx:54-98.
 //
-// k_geometry   one block per (parallel, image order, member): ray-table
-//              lookups (Steffen), Doppler factors and the invisible-phase
-//              padding of the leaf phases; writes PHASE/Z/mu/GEOM per leaf.
+// k_geometry   one block per (parallel, member), image orders in turn:
+//              ray-table lookups (Steffen), Doppler factors and the
+//              invisible-phase padding of the leaf phases; writes
+//              PHASE/Z/mu/GEOM per leaf.
 // k_profile_fold one block per (energy chunk, parallel block, member): the
 //              specific-intensity profile and its periodic Akima coefficients
 //              are staged in LDS (interval-major, energy-minor) and every lane
@@ -78,21 +79,29 @@ __device__ LeafGeo leaf_geometry(int I, double leaf, double inclination, double 
 }
 
 
-__global__ void __launch_bounds__(256) k_geometry(Member* __restrict__ members, Layout L, RingArgs A)
+// One block per (parallel, member): the ring's ray tables are staged once,
+// then the image orders are taken in turn.  An order contributes only when
+// every lower order has a visible leaf (the reference's image loop), so the
+// walk stops at the first order without one; later orders are marked invalid.
+__global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, Layout L, RingArgs A)
 {
     extern __shared__ double sm[];
-    const int i = blockIdx.x, I = blockIdx.y, mi = blockIdx.z, tid = threadIdx.x;
+    const int i = blockIdx.x, mi = blockIdx.z, tid = threadIdx.x, nt = blockDim.x;
     Member* mp = members + mi;
     const Member m = *mp;
-    const size_t vix = ((size_t)mi * L.S_max + i) * L.IO_max + I;
     if (!m.active || m.status) return;
     if (i >= m.S) return;
     const size_t ro = (size_t)mi * L.S_max + i;
-    if (!A.ring_ok[ro]) { if (tid == 0) A.valid[vix] = 0; return; }
+    const size_t vbase = ro * L.IO_max;
+    if (!A.ring_ok[ro]) {
+        for (int I = tid; I < L.IO_max; I += nt) A.valid[vbase + I] = 0;
+        return;
+    }
     const double maxD = A.maxDefl[ro];
     const int IO = L.image_limit > 0 ? L.image_limit : (int)ceil(maxD / PI);
-    if (I == 0 && tid == 0 && IO > L.IO_max) atomicMax(&mp->status, 4);
-    if (I >= IO) { if (tid == 0) A.valid[vix] = 0; return; }
+    if (tid == 0 && IO > L.IO_max) atomicMax(&mp->status, 4);
+    for (int I = tid; I < L.IO_max; I += nt)
+        if (I >= IO) A.valid[vbase + I] = 0;
     const int N_R = L.N_R, N_L = L.N_L;
     double* D = sm;
     double* Ac = D + N_R;
@@ -105,20 +114,20 @@ __global__ void __launch_bounds__(256) k_geometry(Member* __restrict__ members, 
     double* Gs = As + N_L;              // N_L
     double* PV = Gs + N_L;              // N_L
     int* vis = (int*)(PV + N_L);        // N_L/2+1
-    __shared__ int jn_s, any_vis, term;
+    __shared__ int jn_s, term, shown;
     const size_t tb = ro * N_R;
-    for (int q = tid; q < N_R; q += blockDim.x) {
+    for (int q = tid; q < N_R; q += nt) {
         D[q] = A.defl[tb + q];
         Ac[q] = A.cos_alpha[tb + q];
         Lg[q] = A.lag[tb + q];
     }
-    if (tid == 0) { jn_s = N_R - 1; term = 0; }
+    if (tid == 0) jn_s = N_R - 1;
     __syncthreads();
-    for (int q = tid; q < N_R; q += blockDim.x)
+    for (int q = tid; q < N_R; q += nt)
         if (D[q] > HALF_PI) atomicMin(&jn_s, q);
     __syncthreads();
     const int jn = jn_s, nalt = jn + 1;
-    for (int q = tid; q < nalt; q += blockDim.x) {
+    for (int q = tid; q < nalt; q += nt) {
         xalt[q] = cos(D[jn - q]);
         yalt[q] = Ac[jn - q];
     }
@@ -130,111 +139,109 @@ __global__ void __launch_bounds__(256) k_geometry(Member* __restrict__ members, 
     const double Lorentz = sqrt(1.0 - beta * beta);
     const int leaf_lim = (N_L % 2 == 0) ? N_L / 2 : (N_L + 1) / 2;
     const double* leaves = A.leaves;
-    __syncthreads();
-    // an image order contributes only if every lower order has a visible leaf
-    for (int Ip = 0; Ip < I; Ip++) {
-        if (tid == 0) any_vis = 0;
+    const int n_orders = min(IO, L.IO_max);
+    for (int I = 0; I < n_orders; I++) {
+        const size_t vix = vbase + I;
+        __syncthreads();   // the tables are staged; the previous order's outputs are written
+        for (int k = tid; k < N_L; k += nt) { Gs[k] = 0.0; Zs[k] = 1.0; As[k] = 0.0; PV[k] = -3.0; }
+        if (tid == 0) { term = 0; shown = 0; }
         __syncthreads();
-        for (int k = tid; k < leaf_lim; k += blockDim.x) {
-            LeafGeo g = leaf_geometry(Ip, leaves[k], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg,
-                                      maxD, D, Ac, Lg, N_R, xalt, yalt, nalt, false);
-            if (g.calc) any_vis = 1;
-        }
-        __syncthreads();
-        if (!any_vis) { if (tid == 0) A.valid[vix] = 0; return; }
-        __syncthreads();
-    }
-    for (int k = tid; k < N_L; k += blockDim.x) { Gs[k] = 0.0; Zs[k] = 1.0; As[k] = 0.0; PV[k] = -3.0; }
-    __syncthreads();
-    for (int k = tid; k < leaf_lim; k += blockDim.x) {
-        LeafGeo g = leaf_geometry(I, leaves[k], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg, maxD,
-                                  D, Ac, Lg, N_R, xalt, yalt, nalt, true);
-        vis[k] = g.calc;
-        if (!g.calc) continue;
-        for (int ks = 0; ks < 2; ks++) {
-            bool take = (0 < k && k < leaf_lim - 1) || (k == 0 && ks == 0) ||
-                        (k == leaf_lim - 1 && N_L % 2 == 1 && ks == 0) ||
-                        (k == leaf_lim - 1 && N_L % 2 == 0);
-            if (!take) continue;
-            int kd = ks == 0 ? k : N_L - 1 - k;
-            double superlum, eta;
-            if (!eq12(g.psi, 0.0)) {
-                double cos_xi = g.sin_alpha * sin_i * sin(leaves[kd]) / g.sin_psi;
-                superlum = 1.0 + beta * cos_xi;
-                eta = Lorentz / superlum;
-            } else {
-                superlum = 1.0;
-                eta = Lorentz;
-            }
-            Zs[kd] = eta * Grav_z;
-            As[kd] = g.mu * eta;
-            Gs[kd] = g.mu * fabs(g.deriv) * Grav_z * eta * eta * eta / superlum;
-            PH[kd] = leaves[kd] + g.phase_lag;
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        // invisible-phase padding, in the reference's leaf order (:480-549)
-        int invis = 2, invis_phase = 0;
-        for (int k = 0; k < leaf_lim; k++) {
-            if (vis[k]) {
-                if (k == 0) {
-                    PH[N_L - 1] = PH[0] + TWO_PI;
-                    Zs[N_L - 1] = Zs[0]; As[N_L - 1] = As[0]; Gs[N_L - 1] = Gs[0];
-                } else if (invis == 2) {
-                    double step = leaves[k] / (double)k;
-                    for (int q = N_L - k; q < N_L; q++) PH[q] = PH[q - 1] + step;
-                    PH[0] = PH[N_L - 1] - TWO_PI;
-                    for (int q = 1; q < k; q++) PH[q] = PH[q - 1] + step;
-                } else if (invis == 1) {
-                    double step = (PH[k] - PH[invis_phase - 1]) / (double)(k - invis_phase + 1);
-                    for (int q = invis_phase; q < k; q++) PH[q] = PH[q - 1] + step;
-                    step = (PH[N_L - invis_phase] - PH[N_L - 1 - k]) / (double)(k - invis_phase + 1);
-                    for (int q = N_L - k; q < N_L - invis_phase; q++) PH[q] = PH[q - 1] + step;
-                }
-                invis = 0;
-            } else if (invis == 0) {
-                double step = (PH[N_L - k] - PH[k - 1]) / (double)(N_L - 2 * k + 1);
-                for (int q = k; q < N_L - k; q++) PH[q] = PH[q - 1] + step;
-                invis = 1;
-                invis_phase = k;
-            }
-        }
-        // order of the intensity queries (leaf k ascending, then its mirror):
-        // PV[kd] = previous queried leaf, -1 = last query of the previous
-        // parallel/image, -2 = copied from leaf 0 (no query)
-        int prevq = -1;
-        for (int k = 0; k < leaf_lim; k++) {
-            if (!vis[k]) continue;
+        for (int k = tid; k < leaf_lim; k += nt) {
+            LeafGeo g = leaf_geometry(I, leaves[k], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg, maxD,
+                                      D, Ac, Lg, N_R, xalt, yalt, nalt, true);
+            vis[k] = g.calc;
+            if (!g.calc) continue;
             for (int ks = 0; ks < 2; ks++) {
                 bool take = (0 < k && k < leaf_lim - 1) || (k == 0 && ks == 0) ||
                             (k == leaf_lim - 1 && N_L % 2 == 1 && ks == 0) ||
                             (k == leaf_lim - 1 && N_L % 2 == 0);
                 if (!take) continue;
                 int kd = ks == 0 ? k : N_L - 1 - k;
-                PV[kd] = (double)prevq;
-                prevq = kd;
+                double superlum, eta;
+                if (!eq12(g.psi, 0.0)) {
+                    double cos_xi = g.sin_alpha * sin_i * sin(leaves[kd]) / g.sin_psi;
+                    superlum = 1.0 + beta * cos_xi;
+                    eta = Lorentz / superlum;
+                } else {
+                    superlum = 1.0;
+                    eta = Lorentz;
+                }
+                Zs[kd] = eta * Grav_z;
+                As[kd] = g.mu * eta;
+                Gs[kd] = g.mu * fabs(g.deriv) * Grav_z * eta * eta * eta / superlum;
+                PH[kd] = leaves[kd] + g.phase_lag;
             }
-            if (k == 0) PV[N_L - 1] = -2.0;
         }
-        A.lastq[vix] = prevq;
-        if (invis == 2) {
-            A.valid[vix] = 0;
-        } else {
-            for (int q = 1; q < N_L; q++)
-                if (PH[q] <= PH[q - 1]) { term = 1; break; }
-            A.valid[vix] = term ? 0 : 1;
-            if (term) atomicMax(&mp->status, 2);
+        __syncthreads();
+        if (tid == 0) {
+            // invisible-phase padding, in the reference's leaf order (:480-549)
+            int invis = 2, invis_phase = 0;
+            for (int k = 0; k < leaf_lim; k++) {
+                if (vis[k]) {
+                    if (k == 0) {
+                        PH[N_L - 1] = PH[0] + TWO_PI;
+                        Zs[N_L - 1] = Zs[0]; As[N_L - 1] = As[0]; Gs[N_L - 1] = Gs[0];
+                    } else if (invis == 2) {
+                        double step = leaves[k] / (double)k;
+                        for (int q = N_L - k; q < N_L; q++) PH[q] = PH[q - 1] + step;
+                        PH[0] = PH[N_L - 1] - TWO_PI;
+                        for (int q = 1; q < k; q++) PH[q] = PH[q - 1] + step;
+                    } else if (invis == 1) {
+                        double step = (PH[k] - PH[invis_phase - 1]) / (double)(k - invis_phase + 1);
+                        for (int q = invis_phase; q < k; q++) PH[q] = PH[q - 1] + step;
+                        step = (PH[N_L - invis_phase] - PH[N_L - 1 - k]) / (double)(k - invis_phase + 1);
+                        for (int q = N_L - k; q < N_L - invis_phase; q++) PH[q] = PH[q - 1] + step;
+                    }
+                    invis = 0;
+                } else if (invis == 0) {
+                    double step = (PH[N_L - k] - PH[k - 1]) / (double)(N_L - 2 * k + 1);
+                    for (int q = k; q < N_L - k; q++) PH[q] = PH[q - 1] + step;
+                    invis = 1;
+                    invis_phase = k;
+                }
+            }
+            // order of the intensity queries (leaf k ascending, then its mirror):
+            // PV[kd] = previous queried leaf, -1 = last query of the previous
+            // parallel/image, -2 = copied from leaf 0 (no query)
+            int prevq = -1;
+            for (int k = 0; k < leaf_lim; k++) {
+                if (!vis[k]) continue;
+                for (int ks = 0; ks < 2; ks++) {
+                    bool take = (0 < k && k < leaf_lim - 1) || (k == 0 && ks == 0) ||
+                                (k == leaf_lim - 1 && N_L % 2 == 1 && ks == 0) ||
+                                (k == leaf_lim - 1 && N_L % 2 == 0);
+                    if (!take) continue;
+                    int kd = ks == 0 ? k : N_L - 1 - k;
+                    PV[kd] = (double)prevq;
+                    prevq = kd;
+                }
+                if (k == 0) PV[N_L - 1] = -2.0;
+            }
+            A.lastq[vix] = prevq;
+            if (invis == 2) {
+                A.valid[vix] = 0;
+            } else {
+                shown = 1;
+                for (int q = 1; q < N_L; q++)
+                    if (PH[q] <= PH[q - 1]) { term = 1; break; }
+                A.valid[vix] = term ? 0 : 1;
+                if (term) atomicMax(&mp->status, 2);
+            }
         }
-    }
-    __syncthreads();
-    double* out = A.geo + vix * 5 * N_L;
-    for (int q = tid; q < N_L; q += blockDim.x) {
-        out[q] = PH[q];
-        out[N_L + q] = Zs[q];
-        out[2 * N_L + q] = As[q];
-        out[3 * N_L + q] = Gs[q];
-        out[4 * N_L + q] = PV[q];
+        __syncthreads();
+        double* out = A.geo + vix * 5 * N_L;
+        for (int q = tid; q < N_L; q += nt) {
+            out[q] = PH[q];
+            out[N_L + q] = Zs[q];
+            out[2 * N_L + q] = As[q];
+            out[3 * N_L + q] = Gs[q];
+            out[4 * N_L + q] = PV[q];
+        }
+        if (!shown) {
+            // no visible leaf at this order: the higher orders do not contribute
+            for (int q = I + 1 + tid; q < n_orders; q += nt) A.valid[vbase + q] = 0;
+            break;
+        }
     }
 }
 
