@@ -963,7 +963,7 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     hipLaunchKernelGGL(k_mesh_table, dim3((p->table_n + 1 + 3) / 4, nm), dim3(256), 0, s, p->d_members,
                        p->d_area_tab, p->d_total, p->table_n);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mesh_rings, dim3(S, nm), dim3(256), 0, s, p->d_members, p->d_area_tab, p->d_total,
+    hipLaunchKernelGGL(k_mesh_rings, dim3((S + 1) / 2, nm), dim3(256), 0, s, p->d_members, p->d_area_tab, p->d_total,
                        p->table_n, L, p->d_theta, p->d_phi, p->d_r, p->d_cellArea, p->d_maxAlpha,
                        p->d_cos_gamma, p->d_effGrav);
     HIPCHK(hipGetLastError());

@@ -449,11 +449,12 @@ __global__ void __launch_bounds__(256) k_mesh_table(const Member* __restrict__ m
     }
 }
 
-// One block of four waves per ring: wave 0 finds the parallels and the
-// centroid colatitude, all threads classify the ring's cells, then the four
-// waves integrate the boundary cells between them.  The phases are latency
-// bound, so the register budget is capped for three waves per SIMD (with the
-// integrals out of line).
+// One block of four waves per RPB (two) rings of a member: wave w < RPB finds
+// the parallels and the centroid colatitude of ring RPB b + w, all threads
+// classify the rings' cells, then the four waves integrate the rings' boundary
+// cells between them.  The phases are latency bound (serial FP64 work on one
+// wave), so rings share a block and the register budget is capped for three
+// waves per SIMD (with the integrals out of line).
 __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict__ members,
                                                     const double* __restrict__ area_tab,
                                                     const double* __restrict__ total, int table_n,
@@ -464,11 +465,13 @@ __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict_
                                                     double* __restrict__ cos_gamma,
                                                     double* __restrict__ effGrav)
 {
-    const int mi = blockIdx.y, i = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int RPB = 2;                  // rings per block
+    const int mi = blockIdx.y, i0 = blockIdx.x * RPB, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Member m = members[mi];
     if (!m.active || m.status) return;
     const int S = m.S;
-    if (i >= S) return;
+    if (i0 >= S) return;
+    const int nr = min(RPB, S - i0);
     double lo, hi, colat, sColat, dphi, bphi;
     int inv;
     member_lims(m, lo, hi, colat, sColat, inv, dphi, bphi);
@@ -477,14 +480,15 @@ __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict_
     const double eta = cell / dphi;
     const double* at = area_tab + (size_t)mi * table_n;
     __shared__ double ct[1024];
-    __shared__ double sh_lower, sh_upper, sh_cc;
+    __shared__ double sh_lower[RPB], sh_upper[RPB], sh_cc[RPB];
     __shared__ int nb;
-    __shared__ int bj[512];
+    __shared__ int bj[2048];                // boundary cells: (ring slot << 16) | j
     for (int k = tid; k < table_n; k += 256) ct[k] = lin(hi, lo, table_n, k);
     if (tid == 0) nb = 0;
     __syncthreads();
     const size_t ro = (size_t)mi * L.S_max, co = (size_t)mi * L.S_max * L.S_max;
-    if (wave == 0) {
+    if (wave < nr) {
+        const int i = i0 + wave;
         double i_eta = (double)S * eta, e_lo = 0.0, e_hi = 0.0;
         for (int k = 0; k <= i && k < S - 1; k++) {
             i_eta -= eta;
@@ -495,7 +499,7 @@ __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict_
         double upper = (i == S - 1) ? hi : steffen_eval(at, ct, table_n, e_hi, nullptr);
         double cc = wave_area(lower, upper, m.R_eq, m.epsilon, m.zeta, m.shape, 1) / eta;
         if (lane == 0) {
-            sh_lower = lower; sh_upper = upper; sh_cc = cc;
+            sh_lower[wave] = lower; sh_upper[wave] = upper; sh_cc[wave] = cc;
             double mu = cos(cc);
             double rn = radius_norm(mu, m.epsilon, m.zeta, m.shape);
             double fth = f_theta(mu, rn, m.epsilon, m.zeta, m.shape);
@@ -508,9 +512,8 @@ __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict_
         }
     }
     __syncthreads();
-    const double lower = sh_lower, upper = sh_upper, cc = sh_cc;
     const double first_left = m.polar ? -PI : -bphi;
-    if (i == 0) {
+    if (i0 == 0) {
         for (int j = tid; j < S; j += 256) {
             double p0 = m.polar ? lin(-PI + 0.5 * dphi, PI - 0.5 * dphi, S, j)
                                 : lin(-bphi + 0.5 * dphi, bphi - 0.5 * dphi, S, j);
@@ -539,14 +542,16 @@ __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict_
     }
     const bool mirror = eq12(sColat, colat) && eq12(sAzi, 0.0);
     const int j_max = mirror ? S / 2 : S;
-    double* ca = cellArea + co + (size_t)i * S;
     // the reference accumulates left += delta_phi cell by cell
     auto left_of = [&](int j) {
         double left = first_left;
         for (int q = 0; q < j; q++) left = left + dphi;
         return left;
     };
-    for (int j = tid; j < j_max; j += 256) {
+    for (int q = tid; q < nr * j_max; q += 256) {
+        const int r = q / j_max, j = q - r * j_max, i = i0 + r;
+        const double lower = sh_lower[r], upper = sh_upper[r], cc = sh_cc[r];
+        double* ca = cellArea + co + (size_t)i * S;
         double left = left_of(j);
         double right = left + dphi, mid = left + 0.5 * dphi;
         int p1, p2, p3, p4, p5, kind = 0;
@@ -578,13 +583,15 @@ __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict_
         if (mirror) ca[S - 1 - j] = v;
         if (kind == 2) {
             int slot = atomicAdd(&nb, 1);
-            if (slot < 512) bj[slot] = j;
+            if (slot < 2048) bj[slot] = (r << 16) | j;
         }
     }
     __syncthreads();
-    const int nbc = min(nb, 512);
+    const int nbc = min(nb, 2048);
     for (int q = wave; q < nbc; q += 4) {
-        int j = bj[q];
+        const int r = bj[q] >> 16, j = bj[q] & 0xffff, i = i0 + r;
+        const double lower = sh_lower[r], upper = sh_upper[r];
+        double* ca = cellArea + co + (size_t)i * S;
         double left = left_of(j);
         double right = left + dphi;
         double v = wave_cell(lower, upper, left, right, m.R_eq, m.epsilon, m.zeta, m.shape, colat, cR, sR, sAzi,
