@@ -163,27 +163,35 @@ __device__ __forceinline__ void walk_queries(const double* __restrict__ PH, int 
     }
 }
 
-// Block-wide stream compaction of up to 256 flagged values (one per thread of
-// the first four waves) in thread order; cnt holds 16 scratch ints.  Every
-// thread of the block calls it (it holds two barriers).  Returns the number of
-// flagged values.
+// Block-wide stream compaction of three flag sets at once (one value per
+// thread, up to 256 per set) in thread order; cnt holds 3 NW scratch ints.
+// Every thread of the block calls it (it holds two barriers).  Writes the
+// three counts to n[0..2].
 template <int NW>
-__device__ __forceinline__ int compact_u16(bool flag, int value, unsigned short* out, int* cnt)
+__device__ __forceinline__ void compact3_u16(bool f0, bool f1, bool f2, int value, unsigned short* o0,
+                                             unsigned short* o1, unsigned short* o2, int* cnt, int n[3])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long b = __ballot(flag);
-    const int rank = __popcll(b & ((1ull << lane) - 1ull));
-    if (lane == 0) cnt[wave] = __popcll(b);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long b0 = __ballot(f0), b1 = __ballot(f1), b2 = __ballot(f2);
+    if (lane == 0) {
+        cnt[wave] = __popcll(b0);
+        cnt[NW + wave] = __popcll(b1);
+        cnt[2 * NW + wave] = __popcll(b2);
+    }
     __syncthreads();
-    int base = 0, total = 0;
+    int base0 = 0, base1 = 0, base2 = 0, t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
     for (int w = 0; w < NW; w++) {
-        base += w < wave ? cnt[w] : 0;
-        total += cnt[w];
+        const int c0 = cnt[w], c1 = cnt[NW + w], c2 = cnt[2 * NW + w];
+        if (w < wave) { base0 += c0; base1 += c1; base2 += c2; }
+        t0 += c0; t1 += c1; t2 += c2;
     }
-    if (flag) out[base + rank] = (unsigned short)value;
+    if (f0) o0[base0 + __popcll(b0 & below)] = (unsigned short)value;
+    if (f1) o1[base1 + __popcll(b1 & below)] = (unsigned short)value;
+    if (f2) o2[base2 + __popcll(b2 & below)] = (unsigned short)value;
     __syncthreads();
-    return total;
+    n[0] = t0; n[1] = t1; n[2] = t2;
 }
 
 // One block of eight waves owns T 16-phase tiles of one member, the cells
@@ -231,7 +239,7 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
     R* carea = (R*)(ST + 8);                // [S_max], NaN = not radiating
     unsigned* MIX = (unsigned*)(carea + L.S_max);   // [G][N_L] energy bits of sign-changing pieces
     unsigned* HITW = MIX + G * N_L;         // [(N_L + 7) / 8] 4 bits per interval: tile t reaches it
-    int* CNT = (int*)(HITW + ((N_L + 7) >> 3));   // [32] compaction scratch, flags
+    int* CNT = (int*)(HITW + ((N_L + 7) >> 3));   // [32] compaction scratch (24), flag (28)
     short* PV = (short*)(CNT + 32);         // [N_L] previous queried leaf (-2: not queried)
     unsigned short* IL = (unsigned short*)(PV + N_L);   // [N_L] intervals folded
     unsigned short* SLL = IL + N_L;                       // [N_L] secants formed
@@ -241,7 +249,7 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
     R* MOM = CO + G * N_L * 64;             // [T][NI][4][16] moment m at phase p of tile t
     R* SL = MOM + T * NI * 64;              // [G][NI][16] secant slopes, then:
     R* DIR = SL;                            // [G][16][PT] direct sums of sign-changing pieces
-    int* MIXED = CNT + 16;                  // the group has sign-changing pieces (set, then read after a barrier)
+    int* MIXED = CNT + 28;                  // the group has sign-changing pieces (set, then read after a barrier)
 
     // FP32 form: cell areas enter the moments relative to the ring's largest
     // one.  The mesh is equal-area, so interior cells become exactly 1 and
@@ -306,6 +314,10 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
             if (ATM == 2 && tid == 0) prev_query_state(F, L, mi, i, I, ST);
             __syncthreads();
             // moments of the tiles' queries (energy independent); reached intervals
+            // first and last interval this thread's cells reach (in walk order,
+            // wrapping at most once): the direct pass below visits only
+            // threads whose range holds a sign-changing piece
+            int qfirst = -1, qlast = -1;
             if (pg < N_P) {
                 const int t = pp >> 4, col = pp & 15;
                 int mcur = -1;
@@ -322,6 +334,8 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                     if (entered) {
                         if (mcur >= 0) flush();
                         mcur = cur;
+                        if (qfirst < 0) qfirst = cur;
+                        qlast = cur;
                         m0 = (R)0; m1 = (R)0; m2 = (R)0; m3 = (R)0;
                     }
                     const R tq = (R)tq64;
@@ -362,9 +376,9 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                     }
                 }
             }
-            const int n_i = compact_u16<NW>(tid < NI && hits(tid) != 0u, tid, IL, CNT);
-            const int n_s = compact_u16<NW>(fsec, tid, SLL, CNT);
-            const int n_k = compact_u16<NW>(tid < N_L && fknot, tid, KL, CNT);
+            int ncount[3];
+            compact3_u16<NW>(tid < NI && hits(tid) != 0u, fsec, tid < N_L && fknot, tid, IL, SLL, KL, CNT, ncount);
+            const int n_i = ncount[0], n_s = ncount[1], n_k = ncount[2];
             if (n_i == 0) continue;
             const bool copy_last = PV[N_L - 1] < -1;   // the last leaf repeats the first
             for (int gi = 0; gi < ng; gi++) {
@@ -453,7 +467,19 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                     // sign-changing pieces, point by point with the clipping
                     for (int q = tid; q < G * 16 * PT; q += NT) DIR[q] = (R)0;
                     __syncthreads();
-                    if (pg < N_P) {
+                    bool need = false;
+                    if (pg < N_P && qfirst >= 0) {
+                        int id = qfirst;
+                        for (int n = 0; n < NI && !need; n++) {
+                            unsigned any = 0u;
+#pragma unroll
+                            for (int gg = 0; gg < G; gg++) any |= MIX[gg * N_L + id];
+                            need = any != 0u;
+                            if (id == qlast) break;
+                            id = id + 1 == NI ? 0 : id + 1;
+                        }
+                    }
+                    if (need) {
                         unsigned mx[G];
 #pragma unroll
                         for (int gg = 0; gg < G; gg++) mx[gg] = 0u;
