@@ -207,7 +207,7 @@ static size_t moment_lds(int N_L, int S_max, size_t rb, int T, int atm)
     if (N_L > 256) return (size_t)-1;    // one list entry per thread of four waves
     const size_t NL = (size_t)N_L, NI = NL - 1, G = (size_t)T;
     size_t off = ((atm == 2 ? 4 : 3) * NL + (size_t)S_max + 8) * 8 + (size_t)S_max * rb + G * NL * 4 +
-                 ((NL + 7) / 8) * 4 + 32 * 4 + NL * 2 + 3 * NL * 2;
+                 ((NL + 7) / 8) * 4 + 64 * 4 + NL * 2 + (3 + (size_t)T) * NL * 2;
     off = (off + 15) & ~(size_t)15;
     size_t body = G * NL * 64 * rb + (size_t)T * NI * 64 * rb + std::max(G * NI * 16, G * (size_t)T * 256) * rb;
     if (T * T < 8) body = std::max(body, (size_t)8 * 4 * 64 * 8);

@@ -163,35 +163,36 @@ __device__ __forceinline__ void walk_queries(const double* __restrict__ PH, int 
     }
 }
 
-// Block-wide stream compaction of three flag sets at once (one value per
-// thread, up to 256 per set) in thread order; cnt holds 3 NW scratch ints.
-// Every thread of the block calls it (it holds two barriers).  Writes the
-// three counts to n[0..2].
-template <int NW>
-__device__ __forceinline__ void compact3_u16(bool f0, bool f1, bool f2, int value, unsigned short* o0,
-                                             unsigned short* o1, unsigned short* o2, int* cnt, int n[3])
+// Block-wide stream compaction of NS flag sets at once (one value per thread,
+// up to 256 per set) in thread order; cnt holds NS NW scratch ints.  Every
+// thread of the block calls it (it holds two barriers).  Writes the counts to
+// n[0 .. NS-1].
+template <int NW, int NS>
+__device__ __forceinline__ void compact_u16(const bool (&f)[NS], int value, unsigned short* const (&o)[NS], int* cnt,
+                                            int (&n)[NS])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
-    const unsigned long long b0 = __ballot(f0), b1 = __ballot(f1), b2 = __ballot(f2);
-    if (lane == 0) {
-        cnt[wave] = __popcll(b0);
-        cnt[NW + wave] = __popcll(b1);
-        cnt[2 * NW + wave] = __popcll(b2);
-    }
-    __syncthreads();
-    int base0 = 0, base1 = 0, base2 = 0, t0 = 0, t1 = 0, t2 = 0;
+    unsigned long long b[NS];
 #pragma unroll
-    for (int w = 0; w < NW; w++) {
-        const int c0 = cnt[w], c1 = cnt[NW + w], c2 = cnt[2 * NW + w];
-        if (w < wave) { base0 += c0; base1 += c1; base2 += c2; }
-        t0 += c0; t1 += c1; t2 += c2;
+    for (int k = 0; k < NS; k++) {
+        b[k] = __ballot(f[k]);
+        if (lane == 0) cnt[k * NW + wave] = __popcll(b[k]);
     }
-    if (f0) o0[base0 + __popcll(b0 & below)] = (unsigned short)value;
-    if (f1) o1[base1 + __popcll(b1 & below)] = (unsigned short)value;
-    if (f2) o2[base2 + __popcll(b2 & below)] = (unsigned short)value;
     __syncthreads();
-    n[0] = t0; n[1] = t1; n[2] = t2;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        int base = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const int c = cnt[k * NW + w];
+            base += w < wave ? c : 0;
+            tot += c;
+        }
+        if (f[k]) o[k][base + __popcll(b[k] & below)] = (unsigned short)value;
+        n[k] = tot;
+    }
+    __syncthreads();
 }
 
 // One block of eight waves owns T 16-phase tiles of one member, the cells
@@ -239,17 +240,18 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
     R* carea = (R*)(ST + 8);                // [S_max], NaN = not radiating
     unsigned* MIX = (unsigned*)(carea + L.S_max);   // [G][N_L] energy bits of sign-changing pieces
     unsigned* HITW = MIX + G * N_L;         // [(N_L + 7) / 8] 4 bits per interval: tile t reaches it
-    int* CNT = (int*)(HITW + ((N_L + 7) >> 3));   // [32] compaction scratch (24), flag (28)
-    short* PV = (short*)(CNT + 32);         // [N_L] previous queried leaf (-2: not queried)
+    int* CNT = (int*)(HITW + ((N_L + 7) >> 3));   // [64] compaction scratch (56), flag (60)
+    short* PV = (short*)(CNT + 64);         // [N_L] previous queried leaf (-2: not queried)
     unsigned short* IL = (unsigned short*)(PV + N_L);   // [N_L] intervals folded
     unsigned short* SLL = IL + N_L;                       // [N_L] secants formed
     unsigned short* KL = SLL + N_L;                       // [N_L] knots evaluated
+    unsigned short* ILT = KL + N_L;                       // [T][N_L] folded intervals tile t reaches
     // (offsets from smd keep the LDS address space visible to the compiler)
-    R* CO = (R*)((char*)smd + ((((char*)(KL + N_L) - (char*)smd) + 15) & ~(ptrdiff_t)15));   // [G][N_L][4][16]
+    R* CO = (R*)((char*)smd + ((((char*)(ILT + T * N_L) - (char*)smd) + 15) & ~(ptrdiff_t)15));   // [G][N_L][4][16]
     R* MOM = CO + G * N_L * 64;             // [T][NI][4][16] moment m at phase p of tile t
     R* SL = MOM + T * NI * 64;              // [G][NI][16] secant slopes, then:
     R* DIR = SL;                            // [G][16][PT] direct sums of sign-changing pieces
-    int* MIXED = CNT + 28;                  // the group has sign-changing pieces (set, then read after a barrier)
+    int* MIXED = CNT + 60;                  // the group has sign-changing pieces (set, then read after a barrier)
 
     // FP32 form: cell areas enter the moments relative to the ring's largest
     // one.  The mesh is equal-area, so interior cells become exactly 1 and
@@ -376,8 +378,20 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                     }
                 }
             }
-            int ncount[3];
-            compact3_u16<NW>(tid < NI && hits(tid) != 0u, fsec, tid < N_L && fknot, tid, IL, SLL, KL, CNT, ncount);
+            // lists: folded intervals, secants, knots, and per tile the folded
+            // intervals it reaches (the others hold zero moments for it)
+            bool flags[3 + T];
+            unsigned short* outs[3 + T];
+            int ncount[3 + T];
+            flags[0] = tid < NI && hits(tid) != 0u; outs[0] = IL;
+            flags[1] = fsec; outs[1] = SLL;
+            flags[2] = tid < N_L && fknot; outs[2] = KL;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                flags[3 + t] = tid < NI && ((hits(tid) >> t) & 1u);
+                outs[3 + t] = ILT + t * N_L;
+            }
+            compact_u16<NW, 3 + T>(flags, tid, outs, CNT, ncount);
             const int n_i = ncount[0], n_s = ncount[1], n_k = ncount[2];
             if (n_i == 0) continue;
             const bool copy_last = PV[N_L - 1] < -1;   // the last leaf repeats the first
@@ -527,7 +541,6 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                     const int kb = lane & 15;
                     const R* mt = MOM + wt * NI * 64;
                     auto sweep = [&](int idx, typename Mfma<R>::v4* sw) {
-                        if (!((hits(idx) >> wt) & 1u)) return;   // the tile's moments are zero
                         const R bm = mt[idx * 64 + lane];
 #pragma unroll
                         for (int u = 0; u < PPW; u++) {
@@ -536,13 +549,19 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                             sw[u] = Mfma<R>::step(a, bm, sw[u]);
                         }
                     };
+                    // the tile's own list: no branch in the loop
+                    int n_t = ncount[3];
+#pragma unroll
+                    for (int t = 1; t < T; t++)
+                        if (wt == t) n_t = ncount[3 + t];
+                    const unsigned short* lt = ILT + wt * N_L;
                     if (on[0]) {
                         int q = sub;
-                        for (; q + WPP < n_i; q += 2 * WPP) {
-                            sweep(IL[q], swA);
-                            sweep(IL[q + WPP], swB);
+                        for (; q + WPP < n_t; q += 2 * WPP) {
+                            sweep(lt[q], swA);
+                            sweep(lt[q + WPP], swB);
                         }
-                        if (q < n_i) sweep(IL[q], swA);
+                        if (q < n_t) sweep(lt[q], swA);
                     }
 #pragma unroll
                     for (int u = 0; u < PPW; u++) {
