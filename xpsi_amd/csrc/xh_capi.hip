@@ -474,6 +474,7 @@ extern "C" int xh_energy_integrator(int N_E, int N_P, const double* signal, cons
 {
     hipStream_t s = (hipStream_t)stream;
     if (N_E < 5 || N_P < 1 || N_edges < 2) return fail(XH_EINVAL, "xh_energy_integrator: bad sizes");
+    if (N_E > 8192) return fail(XH_EINVAL, "xh_energy_integrator: at most 8192 energies (one LDS row)");
     // one "member" with d^2 = 1 carries the signal
     static thread_local Member* d_m = nullptr;
     static thread_local int* d_idx = nullptr;
@@ -488,7 +489,7 @@ extern "C" int xh_energy_integrator(int N_E, int N_P, const double* signal, cons
     }
     EnergyArgs A{signal, logE, logEdges, nullptr, d_idx, out, N_E, N_P, N_edges, 2};
     int n = (N_edges - 1) * N_P;
-    hipLaunchKernelGGL(k_energy_integrate, dim3((n + 255) / 256, 1), dim3(256), 0, s, d_m, A);
+    hipLaunchKernelGGL(k_energy_integrate, dim3((n + 255) / 256, 1), dim3(256), (size_t)N_E * 8, s, d_m, A);
     HIPCHK(hipGetLastError());
     return XH_OK;
 }
@@ -624,7 +625,7 @@ extern "C" void xh_pipeline_destroy(xh_pipeline* p)
 
 extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
 {
-    if (!c || c->n_regions < 1 || c->n_regions > 2 || c->max_batch < 1 || c->N_E < 5 || c->num_leaves < 5 ||
+    if (!c || c->n_regions < 1 || c->n_regions > 2 || c->max_batch < 1 || c->N_E < 5 || c->N_E > 8192 || c->num_leaves < 5 ||
         c->num_rays < 3 || c->N_in < 1 || c->N_ch < 1 || c->N_b < 1) {
         fail(XH_EINVAL, "xh_pipeline_create: bad configuration");
         return nullptr;
@@ -1046,7 +1047,8 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
                            p->d_else_area, p->d_energies, N_E, N_P, p->d_members, p->mpr, p->d_flux);
         HIPCHK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_energy_integrate, dim3((p->N_in * N_P + 255) / 256, ng), dim3(256), 0, s, p->d_members, EA);
+    hipLaunchKernelGGL(k_energy_integrate, dim3((p->N_in * N_P + 255) / 256, ng), dim3(256), (size_t)N_E * 8, s,
+                       p->d_members, EA);
     HIPCHK(hipGetLastError());
     FoldGemmArgs GA{p->d_M, p->d_integrated, p->d_folded, p->N_ch, p->N_in, N_P, p->N_in};
     if (p->fp32 && p->fold_terms == 3)

@@ -20,9 +20,12 @@ namespace xh {
 __global__ void __launch_bounds__(256) k_energy_integrate(const Member* __restrict__ members,
                                                           EnergyArgs A)
 {
+    extern __shared__ double wE[];          // [N_E] 10^logE, once per block
     const int reg = blockIdx.y;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int N_in = A.N_edges - 1, N_P = A.N_P, N_E = A.N_E;
+    for (int e = threadIdx.x; e < N_E; e += blockDim.x) wE[e] = pow(10.0, A.logE[e]);
+    __syncthreads();
     if (q >= N_in * N_P) return;
     const int j = q / N_P, k = q % N_P;
     const double maxE = A.logE[N_E - 1];
@@ -37,9 +40,7 @@ __global__ void __launch_bounds__(256) k_energy_integrate(const Member* __restri
         const Member m = members[mi];
         if (!m.active) continue;
         const double* fl = A.flux + (size_t)mi * N_E * N_P + k;
-        auto yv = [&](int e) {
-            return pow(10.0, A.logE[e]) * (fl[(size_t)e * N_P] / m.d_sq) * log(10.0);
-        };
+        auto yv = [&](int e) { return wE[e] * (fl[(size_t)e * N_P] / m.d_sq) * log(10.0); };
         double v;
         if (lo > hi || lo < A.logE[0] || hi > maxE) v = __builtin_nan("");
         else if (lo == hi) v = 0.0;
