@@ -184,9 +184,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": prec_desc, "data": "synthetic",
         "precision": {"fp64": "FP64 throughout",
-                      "mixed": "FP32 intensity + spline fold (f32 MFMA moment form), split-BF16 "
-                               "response fold (six products of a three-term split, FP32 "
-                               "accumulation), FP64 rays/mesh/phase-bin sums/likelihood"}[
+                      "mixed": "FP32 intensity + spline fold (f32 MFMA moment form), FP32 MFMA "
+                               "response fold with FP64 sums of 4-deep slices, FP64 "
+                               "rays/mesh/phase-bin sums/likelihood"}[
                           args.precision],
         "config": {"workload": "batched form: %d prior draws (%d accepted), uneven "
                                "sqrt_num_cells {24..96} x num_energies {64,128,256}, BB spot, "

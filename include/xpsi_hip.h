@@ -173,8 +173,9 @@ typedef struct xh_model_config {
     int else_sqrt_num_cells;  /* even, >= 10 (reference default 64) */
     int else_num_rays;        /* reference default 1000 */
     int else_image_order_limit; /* 0 = inferred from the deflection */
-    int fold_terms;           /* precision form's response fold: 3 or 6 bf16 products
-                                 (0 = 6); see xpsi_amd/csrc/xh_signal.hip */
+    int fold_terms;           /* precision form's response fold: 0 FP32 MFMA with FP64
+                                 sums per 4-deep slice; 3 or 6 bf16 products of a
+                                 three-term split; see xpsi_amd/csrc/xh_signal.hip */
 } xh_model_config;
 
 /* One parameter vector.  Per region: phase_shift, super_colatitude,

@@ -3,12 +3,13 @@ against each other.
 
 The precision form computes the blackbody intensity, the spline
 coefficients, the interval moments and their products in FP32 (matrix
-cores), folds the response as split-BF16 with FP32 accumulation, and keeps
-the ray tables, the per-parallel phase-bin accumulation and the likelihood
-quadrature in FP64.  Its bar is |lnL - lnL_FP64| < 1e-3; the measured signal
-and lnL errors are printed per configuration, for the default six-product
-response fold and, as a measurement only, the three-product one (whose
-~5e-6 signal error exceeds the bar on badly fitting draws).
+cores), folds the response on FP32 matrix cores with FP64 sums of each
+4-deep slice, and keeps the ray tables, the per-parallel phase-bin
+accumulation and the likelihood quadrature in FP64.  Its bar is
+|lnL - lnL_FP64| < 1e-3, asserted for the default response fold and for
+the six-product split-BF16 one; the three-product split-BF16 fold is
+printed as a measurement only (its ~5e-6 signal error exceeds the bar on
+badly fitting draws).
 """
 import sys
 
@@ -35,7 +36,7 @@ def relerr(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-def _run(model, params, precision, fold_kernel="auto", fold_terms=6):
+def _run(model, params, precision, fold_kernel="auto", fold_terms=0):
     from xpsi_amd.pipeline import LikelihoodPipeline
     m = dict(model, precision=precision, fold_kernel=fold_kernel, fold_terms=fold_terms)
     pl = LikelihoodPipeline(m, max_batch=len(params))
@@ -59,6 +60,7 @@ def _single(model, name):
     lmo, fmo = _run(model, p, "fp64", "moment")
     say("%s: precision form" % name)
     l32, f32 = _run(model, p, "mixed")
+    l6, _ = _run(model, p, "mixed", fold_terms=6)
     l3, f3 = _run(model, p, "mixed", fold_terms=3)
     for r in range(len(f64[0])):
         e_pp = max(relerr(f64[0][r], fpp[0][r]), relerr(fmo[0][r], fpp[0][r]))
@@ -68,12 +70,14 @@ def _single(model, name):
             "(3-product response fold %.2e)" % (name, r, e_pp, e_32, e_3))
         assert e_pp < 1.0e-10
     say("%s: lnL oracle %.10f fp64 %.10f per-point %.10f mixed %.10f |dlnL| %.3e "
-        "(3-product response fold %.3e)" % (name, lnl_ref, l64[0], lpp[0], l32[0],
-                                             abs(l32[0] - l64[0]), abs(l3[0] - l64[0])))
+        "(bf16 response fold: 6 products %.3e, 3 products %.3e)"
+        % (name, lnl_ref, l64[0], lpp[0], l32[0], abs(l32[0] - l64[0]), abs(l6[0] - l64[0]),
+           abs(l3[0] - l64[0])))
     assert abs(l64[0] - lnl_ref) < LNL_ATOL
     assert abs(lpp[0] - lnl_ref) < LNL_ATOL
     assert abs(lmo[0] - lnl_ref) < LNL_ATOL
     assert abs(l32[0] - l64[0]) < PREC_ATOL
+    assert abs(l6[0] - l64[0]) < PREC_ATOL
 
 
 def test_precision_config1():
@@ -97,7 +101,7 @@ def test_precision_batched_sample():
     groups = {}
     for k, d in enumerate(cs):
         groups.setdefault(d["num_energies"], []).append(k)
-    worst = worst3 = 0.0
+    worst = worst6 = worst3 = 0.0
     for ne, ks in sorted(groups.items()):
         model = configs.config3_model(ne)
         ps = [cs[k] for k in ks]
@@ -105,14 +109,18 @@ def test_precision_batched_sample():
         l64, _ = _run(model, ps, "fp64")
         lpp, _ = _run(model, ps, "fp64", "per-point")
         l32, _ = _run(model, ps, "mixed")
+        l6, _ = _run(model, ps, "mixed", fold_terms=6)
         l3, _ = _run(model, ps, "mixed", fold_terms=3)
-        for k, a, b, c, c3 in zip(ks, l64, lpp, l32, l3):
-            say("  draw %d: fp64 %.10g per-point %.10g mixed %.10g (3-product %.10g)" % (k, a, b, c, c3))
+        for k, a, b, c, c6, c3 in zip(ks, l64, lpp, l32, l6, l3):
+            say("  draw %d: fp64 %.10g per-point %.10g mixed %.10g (bf16 6-product %.10g, 3-product %.10g)"
+                % (k, a, b, c, c6, c3))
             assert (a <= 0.1 * LLZERO) == (b <= 0.1 * LLZERO) == (c <= 0.1 * LLZERO)
             if a > 0.1 * LLZERO:
                 assert abs(a - b) < LNL_ATOL
                 worst = max(worst, abs(c - a))
+                worst6 = max(worst6, abs(c6 - a))
                 worst3 = max(worst3, abs(c3 - a))
                 assert abs(c - a) < PREC_ATOL
-    say("batched sample: max |dlnL| precision vs FP64 %.3e (3-product response fold %.3e)"
-        % (worst, worst3))
+                assert abs(c6 - a) < PREC_ATOL
+    say("batched sample: max |dlnL| precision vs FP64 %.3e (bf16 response fold: 6 products %.3e, "
+        "3 products %.3e)" % (worst, worst6, worst3))

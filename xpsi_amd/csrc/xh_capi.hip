@@ -560,8 +560,9 @@ struct xh_pipeline {
     int shape;                   // moment form: block shape T (0 = by batch size)
     int lane_nq;                 // energy-lane fold: phases per wave
     bool fp32;                   // precision form
-    int fold_terms;              // bf16 products of the precision form's response fold
-    __bf16 *d_Mh, *d_Mm, *d_Ml;  // three-term bf16 split of the response (precision form)
+    int fold_terms;              // precision form's response fold: 0 FP32 MFMA, 3 or 6 bf16 products
+    float* d_Mf;                 // FP32-rounded response (precision form, fold_terms 0)
+    __bf16 *d_Mh, *d_Mm, *d_Ml;  // three-term bf16 split of the response (fold_terms 3 or 6)
     int moff[3];                 // first member of each region within a request
     bool timing;
     hipEvent_t ev[16];
@@ -651,7 +652,7 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         fail(XH_EINVAL, "pipeline: precision must be 0 or 1 and fold_kernel 0, 1, 2 or 3"); delete p; return nullptr;
     }
     p->fp32 = c->precision == 1;
-    p->fold_terms = c->fold_terms == 3 ? 3 : 6;
+    p->fold_terms = c->fold_terms;
     if (c->fold_terms != 0 && c->fold_terms != 3 && c->fold_terms != 6) {
         fail(XH_EINVAL, "pipeline: fold_terms must be 0, 3 or 6"); delete p; return nullptr;
     }
@@ -808,7 +809,13 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     for (int i = 0; i < N_P; i++) v[i] = two_pi * cyc[i];
     PHIP(hipMemcpy(p->d_phases, v.data(), (size_t)N_P * 8, hipMemcpyHostToDevice));
     PHIP(hipMemcpy(p->d_M, c->matrix, (size_t)c->N_ch * c->N_in * 8, hipMemcpyHostToDevice));
-    if (p->fp32) {
+    if (p->fp32 && p->fold_terms == 0) {
+        const size_t nmat = (size_t)c->N_ch * c->N_in;
+        std::vector<float> f(nmat);
+        for (size_t q = 0; q < nmat; q++) f[q] = (float)c->matrix[q];
+        PCHK(dalloc(p, &p->d_Mf, nmat));
+        PHIP(hipMemcpy(p->d_Mf, f.data(), nmat * sizeof(float), hipMemcpyHostToDevice));
+    } else if (p->fp32) {
         // three-term bf16 split of the FP32-rounded response, once
         const size_t nmat = (size_t)c->N_ch * c->N_in;
         std::vector<__bf16> h(nmat), m(nmat), l(nmat);
@@ -1051,7 +1058,9 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
                        p->d_members, EA);
     HIPCHK(hipGetLastError());
     FoldGemmArgs GA{p->d_M, p->d_integrated, p->d_folded, p->N_ch, p->N_in, N_P, p->N_in};
-    if (p->fp32 && p->fold_terms == 3)
+    if (p->fp32 && p->fold_terms == 0)
+        hipLaunchKernelGGL(k_fold_f32, dim3((N_P + 31) / 32, (p->N_ch + 31) / 32, ng), dim3(256), 0, s, GA, p->d_Mf);
+    else if (p->fp32 && p->fold_terms == 3)
         hipLaunchKernelGGL(k_fold_bf16<3>, dim3((N_P + 31) / 32, (p->N_ch + 31) / 32, ng), dim3(256), 0, s, GA,
                            p->d_Mh, p->d_Mm, p->d_Ml);
     else if (p->fp32)

@@ -53,6 +53,7 @@ __global__ void k_rays_fix(Member* members, Layout L, const double* rr, double* 
                            double* defl, double* lag, double* maxDefl);
 __global__ void k_energy_integrate(const Member* members, EnergyArgs A);
 __global__ void k_fold_f64(FoldGemmArgs A);
+__global__ void k_fold_f32(FoldGemmArgs A, const float* Mf);
 template <int NT>
 __global__ void k_fold_bf16(FoldGemmArgs A, const __bf16* Mh, const __bf16* Mm, const __bf16* Ml);
 __global__ void k_marginal(MarginalArgs A);

@@ -103,12 +103,51 @@ __global__ void __launch_bounds__(256) k_fold_f64(FoldGemmArgs A)
     }
 }
 
-// Precision form: split-BF16 with FP32 accumulation.  Every operand x
+// Precision form (default): FP32 operands on v_mfma_f32_16x16x4_f32.  On gfx950
+// that instruction rounds each product and partial sum to nearest, so every
+// 4-deep slice of the sum over input energies starts from zero and its result
+// joins an FP64 sum: no FP32 running sum over N_in terms (which drifts) and no
+// bf16 dot product (whose sum v_mfma_f32_16x16x32_bf16 truncates toward zero,
+// a bias that survives every split) is left in the path.
+// Lane l holds M[c0 + l%16][j + l/16] and in[j + l/16][k0 + l%16]; result
+// element r is row 4 (l/16) + r.
+__global__ void __launch_bounds__(256) k_fold_f32(FoldGemmArgs A, const float* __restrict__ Mf)
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int reg = blockIdx.z, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = blockIdx.y * 32 + (wave >> 1) * 16, k0 = blockIdx.x * 32 + (wave & 1) * 16;
+    const int c = c0 + (lane & 15), k = k0 + (lane & 15), jl = lane >> 4;
+    const double* X = A.in + (size_t)reg * A.N_in * A.N_P;
+    const bool c_ok = c < A.N_ch, k_ok = k < A.N_P;
+    const f4 z4 = {0.f, 0.f, 0.f, 0.f};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j0 = 0; j0 < A.N_in; j0 += 4) {
+        const int j = j0 + jl;
+        const bool j_ok = j < A.N_in;
+        const float a = (c_ok && j_ok) ? Mf[(size_t)c * A.ld_M + j] : 0.f;
+        const float b = (k_ok && j_ok) ? (float)X[(size_t)j * A.N_P + k] : 0.f;
+        const f4 t = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, z4, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r] += (double)t[r];
+    }
+    if (!k_ok) return;
+    double* out = A.out + (size_t)reg * A.N_ch * A.N_P;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int cr = c0 + 4 * (lane >> 4) + r;
+        if (cr < A.N_ch) out[(size_t)cr * A.N_P + k] = acc[r];
+    }
+}
+
+// Precision form, selectable: split-BF16 with FP32 accumulation.  Every operand x
 // (rounded to FP32) is a sum of three bf16 terms, x_h = bf16(x),
 // x_m = bf16(x - x_h), x_l = bf16(x - x_h - x_m); the response matrix is split
 // once on the host (Mh, Mm, Ml), the signal here.  NT = 3 keeps the products
 // M_m S_h + M_h S_m + M_h S_h (the two-term split, relative error ~2^-17);
 // NT = 6 adds M_l S_h + M_m S_m + M_h S_l (~2^-23), smallest terms first.
+// Each instruction truncates its 32-term dot product toward zero (only the
+// accumulator add rounds to nearest), so these forms read low by up to an
+// FP32 ulp per instruction; the lnL carries that as a one-signed bias.
 // v_mfma_f32_16x16x32_bf16: lane l holds M[c0 + l%16][j + 8 (l/16) + v] and
 // in[j + 8 (l/16) + v][k0 + l%16], v = 0..7; result element r is row
 // 4 (l/16) + r.

@@ -20,8 +20,8 @@ Model description (dict):
   elsewhere   None or dict: sqrt_num_cells (64), num_rays (1000),
               image_order_limit, elsewhere_temperature (default value)
   precision   'fp64' (default) or 'mixed' (see PRECISIONS)
-  fold_terms  bf16 products of the mixed-precision response fold: 6 (default)
-              or 3
+  fold_terms  mixed-precision response fold: 0 (default) FP32 MFMA with FP64
+              slice sums; 6 or 3 bf16 products of a three-term split
   fold_kernel 'auto' (default), 'lanes', 'moment' or 'per-point' (see FOLD_KERNELS)
 A parameter vector is a dict: mass, radius, distance, cos_inclination and
 `hot_regions`: list of dicts phase_shift, super_colatitude, super_radius,
@@ -155,9 +155,9 @@ def model_config(model, max_batch=1, max_sqrt_cells=0):
     if fk not in FOLD_KERNELS:
         raise XHError("fold_kernel must be one of %s" % sorted(FOLD_KERNELS))
     cfg.fold_kernel = FOLD_KERNELS[fk]
-    ft = int(model.get("fold_terms", 6))
-    if ft not in (3, 6):
-        raise XHError("fold_terms must be 3 or 6")
+    ft = int(model.get("fold_terms", 0))
+    if ft not in (0, 3, 6):
+        raise XHError("fold_terms must be 0, 3 or 6")
     cfg.fold_terms = ft
     es = model.get("elsewhere")
     if es is not None:
