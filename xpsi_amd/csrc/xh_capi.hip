@@ -1127,7 +1127,10 @@ extern "C" int xh_pipeline_work(xh_pipeline* p, int n, double* out)
     hipLaunchKernelGGL(k_work_count, dim3(nm), dim3(256), 0, p->stream, (const Member*)p->d_members, p->L,
                        (const double*)p->d_cellArea, (const int*)p->d_valid, p->d_work);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpy(out, p->d_work, (size_t)nm * 8, hipMemcpyDeviceToHost));
+    // on the pipeline's stream: a blocking copy on the null stream would not
+    // wait for a kernel on a non-blocking stream
+    HIPCHK(hipMemcpyAsync(out, p->d_work, (size_t)nm * 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
     return XH_OK;
 }
 
@@ -1155,6 +1158,9 @@ extern "C" int xh_pipeline_fetch(xh_pipeline* p, int what, int req, int index, d
     const double* src = nullptr;
     size_t cnt = 0;
     Member m{};
+    // the last launch may sit on a non-blocking stream, which the blocking
+    // copies below (null stream) would not wait for
+    HIPCHK(hipStreamSynchronize(p->stream));
     HIPCHK(hipMemcpy(&m, p->d_members + mi, sizeof(Member), hipMemcpyDeviceToHost));
     switch (what) {
     case 0: src = p->d_flux + (size_t)mi * L.N_E * L.N_P; cnt = (size_t)L.N_E * L.N_P; break;
