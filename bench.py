@@ -28,6 +28,11 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 FP32_MFMA_PEAK_TFLOPS = 157.3
 KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold"}
+# HBM bytes of k_moment_fold<float, 1, 2, 2, 4> on the default batched workload,
+# from rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0, factors from a
+# 512 MiB copy) over the three launches of one step (KiB): 256, 128 and 64
+# energies.  profiles/moment_fold_batched_default.md holds the runs.
+FOLD_TRAFFIC_KIB = {256: (17123810, 20011418), 128: (9762315, 9300707), 64: (6796957, 5185836)}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -149,6 +154,12 @@ def main():
             stage1[k] += v
     flops = 8.0 * evals
     achieved = flops / (stage1["profile_fold"] * 1e-3) / 1e12
+    # measured bytes per fold launch (mean of the step's launches), for the
+    # default workload and shape only
+    traffic = None
+    if (args.n == 4096 and args.precision == "mixed" and sorted(groups) == sorted(FOLD_TRAFFIC_KIB)
+            and all(t["chosen"] == "moment/2" for t in tuned.values())):
+        traffic = sum(2.0 * f + w for f, w in FOLD_TRAFFIC_KIB.values()) * 1024.0 / len(FOLD_TRAFFIC_KIB)
     achieved_overlapped = flops / (stage["profile_fold"] / args.steps * 1e-3) / 1e12
     n_eval = len(acc)
     value = n_eval * args.steps / dt
@@ -229,11 +240,12 @@ def main():
         "stage_ms_one_stream": stage1,
         "roofline": {"bound": "mfma", "achieved": achieved,
                      "peak": FP32_MFMA_PEAK_TFLOPS if args.precision == "mixed" else FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "traffic": None,
+                     "unit": "TFLOP/s", "traffic": traffic,
                      "kernel": "+".join(sorted({KERNEL_NAMES[t["chosen"].split("/")[0]] for t in tuned.values()})),
                      "evals_per_step": evals,
                      "flop_per_eval": 8,
                      "measured_on": "one-stream pass of the timed step (HIP events on the fold's stream)",
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE; mostly register spills)",
                      "achieved_overlapped": achieved_overlapped},
         "autotune_fold_tiles": tuned,
         "device": dev,
