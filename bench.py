@@ -59,8 +59,8 @@ def main():
                          "known size that calibrates the HBM byte counters of a profiling run")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the likelihood path shards requests across GPUs "
-              "with no exchange (see DESIGN.md); run it with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the likelihood path would shard requests across "
+              "GPUs with no exchange between them; run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
