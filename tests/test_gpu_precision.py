@@ -124,3 +124,17 @@ def test_precision_batched_sample():
                 assert abs(c6 - a) < PREC_ATOL
     say("batched sample: max |dlnL| precision vs FP64 %.3e (bf16 response fold: 6 products %.3e, "
         "3 products %.3e)" % (worst, worst6, worst3))
+
+
+def test_precision_ragged_tiles():
+    """40 leaves (the 32-phase tiles of the spline and response folds end
+    part-filled), 90 channels and 126 input bins (the response folds' 32-row
+    tiles and 4- and 32-deep K steps end part-filled): FP64 forms against the
+    oracle and each other, every precision-form response fold against FP64."""
+    from xpsi_amd import configs
+    base = configs.config1()
+    M, edges, _ = configs.gaussian_response(90, 126)
+    model = dict(base, hot_regions=[dict(base["hot_regions"][0], num_leaves=40)],
+                 instrument=dict(matrix=M, energy_edges=edges),
+                 data=dict(base["data"], counts=np.ascontiguousarray(base["data"]["counts"][:90])))
+    _single(model, "ragged (40 leaves, 90 x 126 response)")
