@@ -128,6 +128,17 @@ static double n4_I(n4_state *s, double E, double mu, const double *VEC)
     return I * pow(10.0, 3.0 * v[0]);
 }
 
+/* The interpolator over a query sequence, starting from the state each
+   integrate() call starts from (hot_Num4D.pyx:109-199): out[q] is
+   eval_hot_Num4D(E[q], mu[q], vec2[2q..2q+1]) without the norm. */
+void or_num4d_sequence(const or_atmosphere *a, size_t n, const double *E,
+                       const double *mu, const double *vec2, double *out)
+{
+    n4_state s;
+    n4_init(&s, a);
+    for (size_t q = 0; q < n; q++) out[q] = n4_I(&s, E[q], mu[q], vec2 + 2 * q);
+}
+
 int or_integrate(double R, double omega, double r_s, double inclination,
                  size_t S_rows, size_t S_cols, const double *cellArea,
                  const double *radial, const double *r_s_over_r,

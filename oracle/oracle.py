@@ -69,6 +69,7 @@ def lib():
         L.or_integrate.argtypes = ([d] * 4 + [sz, sz] + [_dp] * 6 + [_ip, sz] +
                                    [_dp] * 5 + [sz, _dp, sz, _dp, sz, _dp, i,
                                                 C.c_void_p, i, C.c_void_p, _dp])
+        L.or_num4d_sequence.argtypes = [C.c_void_p, sz, _dp, _dp, _dp, _dp]
         L.or_construct_closed_mesh.restype = i
         L.or_construct_closed_mesh.argtypes = ([sz, d, d, d, i, _dp, _dp, _dp,
                                                 C.POINTER(C.c_double), _dp, _dp, _dp])
@@ -177,16 +178,7 @@ def integrate(omega, r_s, inclination, cellArea, radial, r_s_over_r, theta,
     keep = []
     atm_ptr = None
     if atm_ext == 2:
-        arrs = [_c(a) for a in atmosphere]
-        keep.append(arrs)
-        st = _Atm()
-        st.ndims = 4
-        for k in range(4):
-            st.params[k] = arrs[k].ctypes.data
-            st.N[k] = len(arrs[k])
-        st.intensity = arrs[4].ctypes.data
-        keep.append(st)
-        atm_ptr = C.addressof(st)
+        atm_ptr = _atm_struct(atmosphere, keep)
     t = lib().or_integrate(R, omega, r_s, inclination, S0, S1, _c(cellArea),
                            _c(radial), _c(r_s_over_r), _c(theta), _c(phi),
                            _c(cellParams), _c(radiates, np.int32), N_R,
@@ -197,6 +189,32 @@ def integrate(omega, r_s, inclination, cellArea, radial, r_s_over_r, theta,
                            int(image_order_limit or 0),
                            None if correction is None else correction.ctypes.data, flux)
     return t, flux
+
+
+def _atm_struct(atmosphere, keep):
+    arrs = [_c(a) for a in atmosphere]
+    keep.append(arrs)
+    st = _Atm()
+    st.ndims = 4
+    for k in range(4):
+        st.params[k] = arrs[k].ctypes.data
+        st.N[k] = len(arrs[k])
+    st.intensity = arrs[4].ctypes.data
+    keep.append(st)
+    return C.addressof(st)
+
+
+def num4d_sequence(atmosphere, E, mu, logT, logg):
+    """hot_Num4D.pyx:248-438 evaluated for the queries (E[q], mu[q], logT[q],
+    logg[q]) in order, from the state an integrate() call starts with;
+    returns the intensities without the norm."""
+    keep = []
+    ptr = _atm_struct(atmosphere, keep)
+    E, mu = _c(E), _c(mu)
+    vec = _c(np.stack([np.broadcast_to(logT, E.shape), np.broadcast_to(logg, E.shape)], axis=1))
+    out = np.zeros(len(E))
+    lib().or_num4d_sequence(ptr, len(E), E, mu, vec, out)
+    return out
 
 
 def closed_mesh(st, S):

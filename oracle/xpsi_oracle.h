@@ -76,6 +76,10 @@ typedef struct {
     const double *intensity;
 } or_atmosphere;
 
+/* hot_Num4D.pyx:248-438 over a query sequence from a fresh state (no norm) */
+void or_num4d_sequence(const or_atmosphere *a, size_t n, const double *E,
+                       const double *mu, const double *vec2, double *out);
+
 int or_integrate(double R, double omega, double r_s, double inclination,
                  size_t S_rows, size_t S_cols, const double *cellArea,
                  const double *radial, const double *r_s_over_r,

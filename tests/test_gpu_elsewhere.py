@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from tests.bars import elem_relerr, norm_relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -22,9 +23,7 @@ def say(msg):
     sys.stdout.flush()
 
 
-def relerr(a, b):
-    a, b = np.asarray(a), np.asarray(b)
-    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+relerr = elem_relerr
 
 
 def test_elsewhere_spectrum_signals_and_lnl():

@@ -141,14 +141,6 @@ def test_rays_limits():
     assert np.all(np.diff(defl[1]) > 0)
 
 
-def test_blackbody_closed_form(examples_fast):
-    # one-parallel check of the integrator's intensity: compare BB profile value
-    kT = 1.38064852e-23 / 1.60217662e-16 * 10 ** 6.1
-    E = 2.0
-    val = E ** 3 / math.expm1(E / kT)
-    assert val == pytest.approx(E ** 3 / (math.exp(E / kT) - 1.0), rel=1e-12)
-
-
 def test_precomputation_is_minus_sum_lnfact():
     from scipy.special import gammaln
     rng = np.random.default_rng(3)

@@ -1,0 +1,292 @@
+"""Oracle functions that no reference output covers, pinned by mathematics.
+
+- The 4-D table atmosphere (xpsi/surface_radiation_field/hot_Num4D.pyx:248-438):
+  cubic Lagrange interpolation on four nodes per axis reproduces any
+  tensor-product cubic polynomial exactly, in range and when it extrapolates,
+  so a table sampled from such a polynomial P must return P at every query.
+  The base-node state machine (:300-351) then shows only through its clamp:
+  a query beyond an axis edge is clamped to the edge on the call that moves
+  the base node to the end of the axis, and extrapolated on later calls.
+- The ceding annulus (xpsi/cellmesh/mesh.pyx:23-417, polar_mesh.pyx:23-430):
+  on a spherical star its cells tile the cede cap minus the superseding cap.
+- The blackbody integrator (integrator_for_azimuthal_invariance.pyx:70-665):
+  in flat space, without rotation, a polar cap or ring seen pole-on has the
+  closed-form flux norm * B(E) * pi R^2 (sin^2 rho_out - sin^2 rho_in) / (E keV)
+  at every phase.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+K_B_OVER_KEV = 1.38064852e-23 / 1.60217662e-16
+BB_NORM = 1.0e-7 * 5.040366110812353e22
+KEV = 1.60217662e-16
+
+
+def _poly_table(seed, shape, negative=False):
+    """Axes (non-uniform) and a table sampled from a sum of three products of
+    per-axis cubics; returns (atmosphere tuple, P callable)."""
+    rng = np.random.default_rng(seed)
+    lims = [(5.1, 6.8), (13.7, 15.0), (0.05, 1.0), (-1.3, 2.0)]
+    axes = []
+    for (a, b), n in zip(lims, shape):
+        x = np.sort(rng.uniform(0.0, 1.0, n - 2))
+        axes.append(np.concatenate([[a], a + (b - a) * x, [b]]))
+    centre = [0.5 * (a + b) for a, b in lims]
+    coef = rng.uniform(-1.0, 1.0, (3, 4, 4))
+    const = 0.0 if negative else 40.0
+
+    def P(x0, x1, x2, x3):
+        xs = [x0, x1, x2, x3]
+        tot = const
+        for k in range(3):
+            term = 1.0
+            for d in range(4):
+                t = xs[d] - centre[d]
+                c = coef[k, d]
+                term = term * (c[0] + t * (c[1] + t * (c[2] + t * c[3])))
+            tot = tot + term
+        return tot
+
+    grid = np.meshgrid(*axes, indexing="ij")
+    table = np.ascontiguousarray(P(*grid).ravel())
+    return (axes[0], axes[1], axes[2], axes[3], table), P
+
+
+def _energy(logT, x3):
+    return K_B_OVER_KEV * 10.0 ** logT * 10.0 ** x3
+
+
+def test_num4d_reproduces_cubic_polynomials_in_range():
+    atm, P = _poly_table(11, (6, 5, 7, 9))
+    ax = atm[:4]
+    rng = np.random.default_rng(12)
+    n = 400
+    # random order: the base node walks up and down every axis
+    q = [rng.uniform(a[0], a[-1], n) for a in ax]
+    E = _energy(q[0], q[3])
+    out = O.num4d_sequence(atm, E, q[2], q[0], q[1])
+    x3 = np.log10(E / (K_B_OVER_KEV * 10.0 ** q[0]))
+    ref = P(q[0], q[1], q[2], x3) * 10.0 ** (3.0 * q[0])
+    assert np.all(ref > 0.0)
+    np.testing.assert_allclose(out, ref, rtol=1e-11)
+    # a transposed table (axes 2 and 3 swapped in memory) must not pass
+    bad = np.ascontiguousarray(atm[4].reshape(6, 5, 7, 9).transpose(0, 1, 3, 2).ravel())
+    bad_atm = (ax[0], ax[1], ax[3], ax[2], bad)
+    out_bad = O.num4d_sequence(bad_atm, E, q[2], q[0], q[1])
+    assert np.max(np.abs(out_bad / ref - 1.0)) > 1e-3
+
+
+def test_num4d_negative_values_clip_to_zero():
+    atm, P = _poly_table(13, (5, 5, 6, 7), negative=True)
+    ax = atm[:4]
+    rng = np.random.default_rng(14)
+    q = [rng.uniform(a[0], a[-1], 200) for a in ax]
+    E = _energy(q[0], q[3])
+    out = O.num4d_sequence(atm, E, q[2], q[0], q[1])
+    x3 = np.log10(E / (K_B_OVER_KEV * 10.0 ** q[0]))
+    ref = P(q[0], q[1], q[2], x3) * 10.0 ** (3.0 * q[0])
+    neg = ref < 0.0
+    assert neg.any() and (~neg).any()
+    assert np.all(out[neg] == 0.0)
+    np.testing.assert_allclose(out[~neg], ref[~neg], rtol=1e-9)
+
+
+def test_num4d_clamp_then_extrapolate():
+    atm, P = _poly_table(15, (6, 5, 7, 9))
+    ax = atm[:4]
+    lT, lg, mu = 6.0, 14.3, 0.6
+    x = ax[3]
+    top, bot = x[-1], x[0]
+    seq = [
+        (x[1] * 0.5 + x[2] * 0.5, None),    # in range, base node stays at 0
+        (top + 0.3, top),                   # moves the base node to the end: clamped
+        (top + 0.3, None),                  # base node already at the end: extrapolated
+        (top + 0.1, None),                  # extrapolated
+        (0.5 * (x[3] + x[4]), None),        # back in range
+        (top + 0.25, top),                  # moves to the end again: clamped
+        (bot - 0.2, bot),                   # moves the base node to 0: clamped
+        (bot - 0.2, None),                  # base node already at 0: extrapolated
+        (bot - 0.05, None),
+    ]
+    x3q = np.array([s[0] for s in seq])
+    E = _energy(lT, x3q)
+    out = O.num4d_sequence(atm, E, np.full(len(seq), mu), lT, lg)
+    for k, (q, clamp) in enumerate(seq):
+        xe = np.log10(E[k] / (K_B_OVER_KEV * 10.0 ** lT))
+        want = P(lT, lg, mu, clamp if clamp is not None else xe) * 10.0 ** (3.0 * lT)
+        assert out[k] == pytest.approx(want, rel=1e-11), (k, q, clamp)
+    # the clamped and the extrapolated values genuinely differ
+    assert abs(out[1] / out[2] - 1.0) > 1e-4 and abs(out[6] / out[7] - 1.0) > 1e-4
+    # a fresh state starts at base node 0: a first query below the axis extrapolates
+    out0 = O.num4d_sequence(atm, _energy(lT, np.array([bot - 0.2])), np.array([mu]), lT, lg)
+    xe = np.log10(_energy(lT, bot - 0.2) / (K_B_OVER_KEV * 10.0 ** lT))
+    assert out0[0] == pytest.approx(P(lT, lg, mu, xe) * 10.0 ** (3.0 * lT), rel=1e-11)
+
+
+def test_num4d_temperature_clamp_scales_with_the_clamped_value():
+    # the result is multiplied by 10^(3 logT) of the clamped logT, while
+    # log10(E/kT) is formed from the query's own logT (hot_Num4D.pyx:295-298,438)
+    atm, P = _poly_table(16, (6, 5, 7, 9))
+    ax = atm[:4]
+    lg, mu = 14.1, 0.4
+    lTs = np.array([5.5, ax[0][-1] + 0.1])
+    x3 = np.array([0.3, 0.3])
+    E = _energy(lTs, x3)
+    out = O.num4d_sequence(atm, E, np.full(2, mu), lTs, lg)
+    lt_c = ax[0][-1]
+    want = P(lt_c, lg, mu, 0.3) * 10.0 ** (3.0 * lt_c)
+    assert out[1] == pytest.approx(want, rel=1e-10)
+
+
+def _sphere():
+    return O.spacetime(1.4, 12.0, 1.0, 0.5, 300.0, "sphere")
+
+
+@pytest.mark.parametrize("case", ["concentric", "offset", "polar"])
+def test_ceding_annulus_area_closed_form(case):
+    st = _sphere()
+    if case == "concentric":
+        hr = dict(super_colatitude=0.9, super_radius=0.15, cede_colatitude=0.9, cede_radius=0.35)
+    elif case == "offset":
+        hr = dict(super_colatitude=0.95, super_radius=0.12, cede_colatitude=1.0, cede_radius=0.4,
+                  cede_azimuth=0.08)
+    else:
+        hr = dict(super_colatitude=0.15, super_radius=0.1, cede_colatitude=0.2, cede_radius=0.35)
+    hr.update(super_temperature=6.3, cede_temperature=6.0, cede=True, phase_shift=0.0,
+              sqrt_num_cells=32, num_rays=64, num_leaves=32)
+    members, err = O.hot_region_embed(st, hr, 300.0)
+    assert err is None and len(members) == 2
+    R = st["R"]
+    sup = members[0]["cellArea"].sum()
+    ced = members[1]["cellArea"].sum()
+    cap = lambda rho: 2.0 * math.pi * R * R * (1.0 - math.cos(rho))
+    assert sup == pytest.approx(cap(hr["super_radius"]), rel=1e-8)
+    assert ced == pytest.approx(cap(hr["cede_radius"]) - cap(hr["super_radius"]), rel=1e-8)
+    # the hole is really cut: the annulus has cells that do not radiate
+    assert np.count_nonzero(members[1]["cellArea"] == 0.0) > 0
+
+
+def _flat_pole_on(rho_in, rho_out, S_sqrt, logT):
+    """A pole-centred cap (rho_in = 0) or ring on a nearly massless,
+    non-rotating sphere seen from the pole, integrated by the oracle."""
+    st = O.spacetime(1e-7, 12.0, 1.0, 1.0, 1e-9, "sphere")
+    if rho_in > 0.0:
+        hr = dict(super_colatitude=0.0, super_radius=rho_in, cede_colatitude=0.0,
+                  cede_radius=rho_out, cede=True, cede_temperature=logT)
+    else:
+        hr = dict(super_colatitude=0.0, super_radius=rho_out)
+    hr.update(super_temperature=logT, phase_shift=0.0, sqrt_num_cells=S_sqrt,
+              min_sqrt_num_cells=10, max_sqrt_num_cells=400, num_rays=400, num_leaves=32)
+    members, err = O.hot_region_embed(st, hr, st["f"])
+    assert err is None
+    energies = np.array([0.5, 1.0, 2.0, 4.0])
+    fl = O.hot_region_integrate(st, hr, members, energies)
+    flux = fl[-1]
+    kT = K_B_OVER_KEV * 10.0 ** logT
+    B = energies ** 3 / np.expm1(energies / kT)
+    R = st["R"]
+    exact = BB_NORM * B * math.pi * R * R * (math.sin(rho_out) ** 2 - math.sin(rho_in) ** 2)
+    exact = exact / (energies * KEV)
+    return flux, exact
+
+
+def test_blackbody_flat_pole_on_closed_form():
+    errs = []
+    for S in (16, 48):
+        flux, exact = _flat_pole_on(0.0, 0.5, S, 6.2)
+        # the same at every phase (axisymmetric view), equal to the closed form
+        assert np.all(np.abs(flux / flux[:, :1] - 1.0) < 1e-9)
+        errs.append(np.max(np.abs(flux[:, 0] / exact - 1.0)))
+    # converges with the mesh: the residual is the cell quadrature's error
+    assert errs[1] < 2e-5 and errs[1] < errs[0] / 4.0
+    flux, exact = _flat_pole_on(0.2, 0.5, 48, 6.2)
+    assert np.all(np.abs(flux / flux[:, :1] - 1.0) < 1e-9)
+    assert np.max(np.abs(flux[:, 0] / exact - 1.0)) < 3e-5
+
+
+def test_energy_integrator_power_law():
+    # the integral over an input bin of 10^logE F ln10 d(logE) is the integral
+    # of F dE; for F = E^-1.5 that is 2 (lo^-0.5 - hi^-0.5), to the spline's error
+    E = np.logspace(np.log10(0.3), np.log10(10.0), 200)
+    F = E ** -1.5
+    sig = np.repeat(F[:, None], 3, axis=1) * np.array([1.0, 2.0, 0.5])[None, :]
+    edges = np.logspace(np.log10(0.35), np.log10(12.0), 30)
+    out = O.energy_integrator(sig, np.log10(E), np.log10(edges))
+    hi = np.minimum(edges[1:], E[-1])
+    exact = 2.0 * (edges[:-1] ** -0.5 - hi ** -0.5)
+    inside = edges[:-1] < E[-1]
+    # the reference's periodic Akima end conditions distort the last two
+    # energy intervals (a non-periodic spectrum); away from them the bins agree
+    away = (edges[:-1] > E[2]) & (edges[1:] < E[-3])
+    np.testing.assert_allclose(out[away, 0], exact[away], rtol=2e-6)
+    np.testing.assert_allclose(out[:, 1], 2.0 * out[:, 0], rtol=1e-14)   # linear in the signal
+    assert np.all(out[~inside] == 0.0)                                   # beyond the grid
+
+
+def _marginal_brute(T, N_b, counts, star_rate, support):
+    """sum over channels of log integral over B of prod_j Poisson(d_j | c_j(B)),
+    c_j = T star_rate_j + (T / N_b) B, on the support (upper < 0: unbounded)."""
+    from scipy import integrate as si
+    from scipy.special import gammaln
+    tot = 0.0
+    for i in range(counts.shape[0]):
+        d = counts[i]
+        lam = lambda B: T * star_rate[i] + (T / N_b) * B
+        logf = lambda B: float(np.sum(d * np.log(lam(B)) - lam(B)))
+        lo = max(support[i, 0], 0.0)
+        hi = support[i, 1] if support[i, 1] > 0.0 else 1e3
+        # the integrand's peak and width, then quadrature over +-40 widths of it
+        from scipy import optimize as so
+        Bp = so.minimize_scalar(lambda b: -logf(b), bounds=(lo, hi), method="bounded",
+                                options=dict(xatol=1e-12)).x
+        w = math.sqrt(1.0 / max(np.sum(d / lam(Bp) ** 2) * (T / N_b) ** 2, 1e-300))
+        a, b = max(lo, Bp - 40.0 * w), min(hi, Bp + 40.0 * w)
+        A = logf(Bp)
+        val, _ = si.quad(lambda x: math.exp(logf(x) - A), a, b, points=[Bp], limit=400,
+                         epsabs=0.0, epsrel=1e-12)
+        tot += math.log(val) + A - float(np.sum(gammaln(d + 1.0)))
+    return tot
+
+
+@pytest.mark.parametrize("bounded", [False, True])
+def test_marginal_likelihood_brute_force(bounded):
+    # expected counts from the phase-bin integral of a shifted periodic pulse
+    # (bins wrap at the cycle's end), then the background integral by quadrature
+    rng = np.random.default_rng(21)
+    T, N_b, N_ch, NP = 1000.0, 8, 3, 48
+    ph = np.linspace(0.0, 1.0, NP)
+    amp = np.array([0.02, 0.05, 0.01])
+    pulses = amp[:, None] * (1.0 + 0.6 * np.sin(2.0 * math.pi * ph))[None, :]
+    shift = 0.3
+    edges = np.linspace(0.0, 1.0, N_b + 1)
+    # the bin integral of each channel's pulse, by quadrature of the interpolant
+    from scipy import integrate as si
+    star = np.zeros((N_ch, N_b))
+    for i in range(N_ch):
+        sp = O.akima(ph, pulses[i], True)
+        f = lambda x: O.akima_eval(sp, x - math.floor(x))
+        for j in range(N_b):
+            a, b = edges[j] + shift, edges[j + 1] + shift
+            pts = [math.floor(a) + 1.0] if math.floor(a) != math.floor(b) else None
+            star[i, j] = si.quad(f, a, b, points=pts, limit=200, epsrel=1e-13)[0]
+    B_true = np.array([0.2, 0.1, 0.05])
+    lam = T * star + (T / N_b) * B_true[:, None]
+    counts = rng.poisson(lam).astype(np.float64)
+    if bounded:
+        support = np.stack([0.5 * B_true, 1.5 * B_true], axis=1)
+    else:
+        support = np.zeros((N_ch, 2))
+        support[:, 1] = -1.0
+    pre = O.precomputation(counts.astype(np.int32))
+    lnl, STAR, bml, bmls = O.eval_marginal_likelihood(T, edges, counts, [pulses], [ph],
+                                                      np.array([shift]), pre, support)
+    ref = _marginal_brute(T, N_b, counts, star, support)
+    assert lnl == pytest.approx(ref, abs=1e-7)
+    # a dropped shift moves the answer by far more than the bar
+    lnl0, *_ = O.eval_marginal_likelihood(T, edges, counts, [pulses], [ph], np.array([0.0]),
+                                          pre, support)
+    assert abs(lnl0 - ref) > 1e-3
