@@ -43,12 +43,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--single", type=int, default=40, help="plain single-request evaluations")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--fold-ec", type=int, default=0, help="energies per fold block (0 = auto)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--one-stream", action="store_true", help="launch every energy group on one stream")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-batch", type=int, default=512)
+    ap.add_argument("--serve-wait", type=float, default=0.002, help="latency cap of a partial batch (s)")
+    ap.add_argument("--serve-budget", type=float, default=200e9, help="device footprint budget (bytes)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--config1-batch", type=int, default=1024)
     ap.add_argument("--c5-n", type=int, default=64, help="large-workload draws (0 = skip)")
@@ -62,6 +66,9 @@ def main():
         print("bench.py measures one GPU: the likelihood path would shard requests across "
               "GPUs with no exchange between them; run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    # the CPU baseline first: its process pool forks before the GPU is touched
+    cpu = None if args.no_cpu else cpu_baseline(args)
 
     import torch
     from xpsi_amd import configs
@@ -275,20 +282,82 @@ def main():
         out["config5"] = config5_section(args, stream)
     if not args.no_serving:
         out["serving"] = serving(args, stream_free=True)
-    if not args.no_cpu:
-        from oracle import oracle as O
-        t0 = time.perf_counter()
-        nc = 0
-        while time.perf_counter() - t0 < args.cpu_seconds:
-            O.loglikelihood(m1)
-            nc += 1
-        cdt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": nc / cdt, "unit": "lnL evals/s", "cores": 1,
-                               "kind": "port",
-                               "sample": "%d config-1 likelihood evaluations (single request, "
-                                         "one thread) with the C oracle" % nc,
-                               "host_cpus": os.cpu_count()}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+        out["speedup_vs_cpu_baseline"] = value / cpu["value"]
     print(json.dumps(out))
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_worker(seconds):
+    """One request per core: config-1 evaluations on one thread for `seconds`."""
+    from xpsi_amd import configs
+    from oracle import oracle as O
+    O.select_build("fast")
+    O.set_threads(1)
+    m1 = configs.config1()
+    O.loglikelihood(m1)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        O.loglikelihood(m1)
+        n += 1
+    return n, time.perf_counter() - t0
+
+
+def cpu_baseline(args):
+    """The C oracle built with the reference's flags (-O3 -march=native
+    -funroll-loops -fopenmp, xpsi setup.py:118-126; OpenMP over rings as the
+    reference's prange) on the host's cores, config-1 request, three legs:
+    one thread (the reference's production setting, Likelihood.py:43-50),
+    OpenMP over all cores, and one request per core in a process pool (the
+    reference's one-MPI-process-per-core sampling)."""
+    import multiprocessing as mp
+    from xpsi_amd import configs
+    from oracle import oracle as O
+    affinity = len(os.sched_getaffinity(0))
+    cores = max(1, min(16, affinity))          # the GPU box's CPU share is 16
+    budget = args.cpu_seconds
+    # one request per core (forked before any OpenMP thread pool or GPU use)
+    ctx = mp.get_context("fork")
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [budget / 3.0] * cores)
+    per_core_rate = sum(n / dt for n, dt in res)
+    O.select_build("fast")
+    m1 = configs.config1()
+
+    def lat_leg(threads, nmin, seconds):
+        O.set_threads(threads)
+        O.loglikelihood(m1)
+        lat, t0 = [], time.perf_counter()
+        while len(lat) < nmin or time.perf_counter() - t0 < seconds:
+            t1 = time.perf_counter()
+            O.loglikelihood(m1)
+            lat.append(time.perf_counter() - t1)
+            if len(lat) >= nmin and time.perf_counter() - t0 > seconds:
+                break
+        lat = np.array(lat) * 1e3
+        return {"threads": threads, "n": len(lat), "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "evals_per_s": 1e3 / float(lat.mean())}
+
+    one = lat_leg(1, 30, budget / 3.0)
+    allc = lat_leg(cores, 30, budget / 6.0)
+    O.set_threads(1)
+    return {"value": one["evals_per_s"], "unit": "lnL evals/s", "cores": 1, "kind": "port",
+            "sample": "config-1 requests one at a time on one thread (%d evaluations), the C "
+                      "oracle built -O3 -march=native -funroll-loops -fopenmp" % one["n"],
+            "single_thread": one, "all_cores_openmp": allc,
+            "one_request_per_core": {"processes": cores, "evals_per_s": per_core_rate,
+                                     "seconds": budget / 3.0},
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity": affinity}
 
 
 def config5_section(args, stream):
@@ -331,27 +400,58 @@ def config5_section(args, stream):
 
 
 def serving(args, stream_free=True):
-    """Poisson arrivals of config-3 (128-energy) prior draws at several rates
-    into the continuous-batching server; latency = submit to result."""
+    """The serving form on its workload: a stream of config-2-model requests
+    (two regions with ceding rings, 4-D table atmosphere, 256 energies, 300
+    channels, 128 leaves) with mixed sqrt_num_cells {32..96}, into the
+    continuous-batching server (batches of 1..slot_batch formed under a
+    latency cap, admitted by device footprint, graphs captured for every batch
+    size beforehand).  Reported: the sustained maximum (a closed-loop burst),
+    then Poisson arrivals at 2k-20k req/s with p50/p99 submit-to-result
+    latency; the plain FP64 single-request loop of the same requests beside."""
     from xpsi_amd import configs
+    from xpsi_amd.pipeline import LikelihoodPipeline
     from xpsi_amd.serve import LikelihoodServer
-    draws = [d for d in configs.config3_draws()
-             if configs.prior_accepts(d["mass"], d["radius"], 300.0) and d["num_energies"] == 128]
-    srv = LikelihoodServer(dict(configs.config3_model(128), precision=args.precision), n_slots=4, slot_batch=128,
-                           hbm_budget_bytes=64e9, use_graphs=True)
-    # warm the graphs of common batch sizes
-    for d in draws[:256]:
-        srv.submit(d)
-    got = 0
+    model = configs.config4_model()
+    draws = [d for d in configs.config4_draws(4096)
+             if configs.prior_accepts(d["mass"], d["radius"], 300.0)]
+    # the plain FP64 single-request loop on the same requests
+    pl = LikelihoodPipeline(model, max_batch=1)
+    for d in draws[:3]:
+        pl.evaluate([d])
+    lat1 = []
+    for d in draws[:args.single]:
+        t0 = time.perf_counter()
+        pl.evaluate([d])
+        lat1.append(time.perf_counter() - t0)
+    pl.close()
+    lat1 = np.array(lat1) * 1e3
+    srv = LikelihoodServer(dict(model, precision=args.precision), n_slots=args.serve_slots,
+                           slot_batch=args.serve_batch, hbm_budget_bytes=args.serve_budget,
+                           use_graphs=True, max_wait_s=args.serve_wait)
     t0 = time.perf_counter()
-    while got < 256 and time.perf_counter() - t0 < 60:
-        got += len(srv.poll()[0])
-    out = {}
+    srv.warm()
+    warm_s = time.perf_counter() - t0
+
+    def drain(n, timeout=120.0):
+        lat, done, t0 = [], 0, time.perf_counter()
+        while done < n and time.perf_counter() - t0 < timeout:
+            ids, lnl, l = srv.poll()
+            done += len(ids)
+            lat.extend(l.tolist())
+        return done, lat
+
+    # sustained maximum: a burst of requests submitted at once
+    nb = min(len(draws), 2 * args.serve_batch * args.serve_slots)
+    t0 = time.perf_counter()
+    for d in draws[:nb]:
+        srv.submit(d)
+    done, _ = drain(nb)
+    sustained = done / (time.perf_counter() - t0)
+    out = {"sustained_max_req_per_s": sustained, "burst": nb}
     rng = np.random.default_rng(4)
     for rate in (2000.0, 5000.0, 10000.0, 20000.0):
         n = int(rate * args.serve_seconds)
-        gaps = rng.exponential(1.0 / rate, n)
-        t_sub = np.cumsum(gaps)
+        t_sub = np.cumsum(rng.exponential(1.0 / rate, n))
         lat, done = [], 0
         t0 = time.perf_counter()
         k = 0
@@ -363,17 +463,28 @@ def serving(args, stream_free=True):
             ids, lnl, l = srv.poll()
             done += len(ids)
             lat.extend(l.tolist())
-            if time.perf_counter() - t0 > 120:
+            if time.perf_counter() - t0 > 60:
                 break
         wall = time.perf_counter() - t0
         lat = np.array(lat) * 1e3
         out["%d_req_per_s" % int(rate)] = {
-            "offered": rate, "completed": int(done), "throughput": done / wall,
+            "offered": rate, "submitted": int(k), "completed": int(done), "throughput": done / wall,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
-    out["stats"] = srv.stats()
-    out["workload"] = ("config-3 prior draws with 128 energies, sqrt_num_cells in {24..96}, "
-                       "4 slots x 128, graphs on, %s form" % args.precision)
+        if done < n:
+            drain(k - done)
+    st = srv.stats()
     srv.close()
+    out["stats"] = st
+    out["graph_warm_s"] = warm_s
+    out["plain_fp64_single_same_requests"] = {
+        "evals_per_s": 1e3 / float(lat1.mean()), "p50_ms": float(np.percentile(lat1, 50)),
+        "p99_ms": float(np.percentile(lat1, 99)), "n": int(len(lat1))}
+    out["speedup_sustained_vs_plain_fp64_single"] = sustained * float(lat1.mean()) / 1e3
+    out["workload"] = ("config-2 model requests (2 regions + ceding rings, 4-D table, 256 energies, "
+                       "300 ch), sqrt_num_cells {32,48,64,96}; %d slots x %d, latency cap %.1f ms, "
+                       "HBM budget %.0f GB, graphs on, %s form"
+                       % (args.serve_slots, args.serve_batch, args.serve_wait * 1e3,
+                          args.serve_budget / 1e9, args.precision))
     return out
 
 

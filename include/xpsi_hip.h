@@ -6,7 +6,9 @@
  * device), "host" pointers are ordinary process memory.  `stream` is a
  * hipStream_t (NULL = the default stream).  Every entry point returns 0 on
  * success or a negative XH_E* code; xh_last_error() describes the failure.
- * No entry point allocates device memory except xh_pipeline_create().
+ * Device memory is allocated only by xh_pipeline_create() and
+ * xh_server_create() (the server's pipelines); the drop-ins work in the
+ * caller's buffers and workspaces.
  */
 #ifndef XPSI_HIP_H
 #define XPSI_HIP_H
@@ -159,9 +161,10 @@ typedef struct xh_model_config {
     int max_batch;            /* requests per xh_pipeline_eval call */
     int max_sqrt_cells;       /* buffer bound on mesh size (>= max_sqrt_num_cells) */
     int precision;            /* 0: FP64 throughout.  1: precision form - FP32
-                                 intensity and spline fold (matrix cores), split-BF16
-                                 response fold with FP32 accumulation; rays, phase-bin
-                                 accumulation and likelihood quadrature stay FP64 */
+                                 intensity and spline fold (matrix cores), response fold
+                                 on FP32 MFMA with FP64 sums per 4-deep slice (fold_terms
+                                 selects split-BF16 instead); rays, phase-bin accumulation
+                                 and likelihood quadrature stay FP64 */
     int fold_kernel;          /* 0 automatic (moment form, else energy lanes, else
                                  per-point), 1 per-point fold (FP64 only), 2 moment form,
                                  3 energy lanes; xpsi_amd.pipeline.autotune_fold measures */
@@ -192,7 +195,11 @@ typedef struct xh_pipeline xh_pipeline;
 
 xh_pipeline* xh_pipeline_create(const xh_model_config* cfg);
 void xh_pipeline_destroy(xh_pipeline* p);
-/* Evaluate n <= max_batch requests; lnl_out host [n].  Blocks until done. */
+/* Evaluate n <= max_batch requests; lnl_out host [n].  Blocks until done.
+ * Every launch draws its own llzero-class values (requests that fail or fall
+ * below llzero get llzero * U(0.1, 1), fresh per launch, as the reference's
+ * random_near_llzero).  A launch while the previous one is unwaited first
+ * waits for it (the pinned descriptors and results are reused). */
 int xh_pipeline_eval(xh_pipeline* p, int n, const xh_request* reqs,
                      double* lnl_out, void* stream);
 /* Same without waiting: results land in the pipeline's host buffer, read
@@ -248,6 +255,9 @@ int xh_set_lane_phases(int nq);
 /* Replay each launch as a hipGraph captured once per batch size (requires a
  * non-default stream; timing mode launches eagerly). */
 int xh_pipeline_use_graphs(xh_pipeline* p, int on);
+/* Capture (without running) the graph of batch size n on stream, so that a
+ * later launch of that size replays at once. */
+int xh_pipeline_capture(xh_pipeline* p, int n, void* stream);
 
 /* ------------------------------------------------------------------------
  * Continuous-batching server: n_slots pipelines of slot_batch requests, each
@@ -262,8 +272,18 @@ xh_server* xh_server_create(const xh_model_config* cfg, int n_slots, int slot_ba
 int xh_server_submit(xh_server* s, const xh_request* req, long long* id_out);
 /* Finished requests: ids, lnL and submit-to-result latency in seconds. */
 int xh_server_poll(xh_server* s, long long* ids, double* lnl, double* latency_s, int max_n);
-/* queued, in-flight bytes, launched batches, completed, mean batch fill */
+/* queued, in-flight bytes, launched batches, completed, mean batch fill,
+ * peak in-flight bytes, largest batch, a failed batch seen (0/1): out[<= 8] */
 int xh_server_stats(xh_server* s, double* out, int n);
+/* Batch formation: a free slot takes a partial batch only once the oldest
+ * queued request has waited max_wait_s seconds, or when no slot is busy
+ * (default 0: dispatch at once).  A full batch is always dispatched. */
+int xh_server_set_policy(xh_server* s, double max_wait_s);
+/* Capture every slot's graphs for every batch size (servers created with
+ * use_graphs), so that no batch pays for a capture while serving. */
+int xh_server_warm(xh_server* s);
+/* Device footprint (bytes) the admission control charges for a request. */
+double xh_server_footprint(xh_server* s, const xh_request* req);
 void xh_server_destroy(xh_server* s);
 
 /* Copy an intermediate array of request `req` to host memory for checks.

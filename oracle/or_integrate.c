@@ -17,7 +17,6 @@
 static const double C_SI = 2.99792458e8;
 static const double KEV = 1.60217662e-16;
 static const double K_B = 1.38064852e-23;
-static const double H_KEV = 4.135667662e-18;
 static int eq12(double x, double y) { return fabs(x - y) < 1.0e-12; }
 
 /* rays.pyx:53-57 */
@@ -157,6 +156,15 @@ int or_integrate(double R, double omega, double r_s, double inclination,
     double sin_i = sin(inclination), cos_i = cos(inclination);
     size_t leaf_lim = (N_L % 2 == 0) ? N_L / 2 : (N_L + 1) / 2;
     int image_order = image_order_limit > 0 ? image_order_limit : 0;
+    memset(flux, 0, N_E * N_P * sizeof(double));
+    double norm = (atm_ext == 2) ? NUM4D_NORM : BB_NORM;
+    int terminate = 0;
+    /* Built with OpenMP (the CPU baseline), rings are shared out over threads
+       as the reference's prange does, each thread with its own scratch, table
+       state and flux, summed at the end (:605-612).  Without it this is the
+       one-thread restatement, summing in ring order. */
+#pragma omp parallel
+    {
     double *PHASE = (double *)malloc(N_L * sizeof(double));
     double *PROF = (double *)calloc(N_E * N_L, sizeof(double));
     double *cdefl = (double *)malloc(N_R * sizeof(double));
@@ -166,12 +174,13 @@ int or_integrate(double R, double omega, double r_s, double inclination,
     double *st = (double *)malloc(4 * N_R * sizeof(double));   /* cos psi -> cos a */
     double *ab = (double *)malloc(3 * N_L * sizeof(double));
     double *mb = (double *)malloc((N_L + 3) * sizeof(double));
-    memset(flux, 0, N_E * N_P * sizeof(double));
-    double norm = (atm_ext == 2) ? NUM4D_NORM : BB_NORM;
+    double *acc = (double *)calloc(N_E * N_P, sizeof(double));
     n4_state n4;
     if (atm_ext == 2) n4_init(&n4, atm);
-    int terminate = 0;
-    for (size_t i = 0; i < S_rows && !terminate; i++) {
+#pragma omp for schedule(dynamic)
+    for (long ir = 0; ir < (long)S_rows; ir++) {
+        const size_t i = (size_t)ir;
+        if (terminate) continue;
         size_t J = 0, j;
         for (j = 0; j < S_cols; j++) if (radiates[i * S_cols + j] == 1) { J = j; break; }
         if (j == S_cols) continue;
@@ -333,16 +342,19 @@ int or_integrate(double R, double omega, double r_s, double inclination,
                             break;
                         }
                         double v = or_akima_eval(PHASE, yp, N_L, ab, ab + N_L, ab + 2 * N_L, x);
-                        if (v > 0.0 || correction) flux[p * N_P + k] += area * v;
+                        if (v > 0.0 || correction) acc[p * N_P + k] += area * v;
                     }
                 }
             }
             if (terminate) break;
         }
     }
+#pragma omp critical
+    for (size_t q = 0; q < N_E * N_P; q++) flux[q] += acc[q];
+    free(PHASE); free(PROF); free(cdefl); free(calt); free(sa); free(sl);
+    free(st); free(ab); free(mb); free(acc);
+    }
     for (size_t p = 0; p < N_E; p++)
         for (size_t k = 0; k < N_P; k++) flux[p * N_P + k] /= energies[p] * KEV;
-    free(PHASE); free(PROF); free(cdefl); free(calt); free(sa); free(sl);
-    free(st); free(ab); free(mb);
     return terminate;
 }

@@ -367,3 +367,18 @@ int or_qag61(or_fn f, void *p, double a, double b, double epsabs,
     if (it == limit) return OR_EMAXITER;
     return OR_EFAILED;
 }
+
+/* Threads of the OpenMP build (the CPU baseline); the checker build has one. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}

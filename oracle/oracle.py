@@ -20,6 +20,11 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+# "checker": one thread, no FMA contraction (the parity reference);
+# "fast": the same sources built with the reference's flags and OpenMP (the
+# benchmark's CPU baseline)
+_BUILD = "checker"
+_FILES = {"checker": "liboracle.so", "fast": "liboracle_fast.so"}
 
 # xpsi/global_imports.py:67-88
 c_light = 2.99792458e8
@@ -38,14 +43,31 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def select_build(name):
+    """Use the "checker" or the "fast" (OpenMP, -O3 -march=native) build;
+    takes effect for calls made afterwards."""
+    global _LIB, _BUILD
+    if name not in _FILES:
+        raise ValueError(name)
+    if name != _BUILD:
+        _BUILD, _LIB = name, None
+
+
+def set_threads(n):
+    """OpenMP threads of the fast build (returns the count now in use)."""
+    return lib().or_set_threads(int(n))
+
+
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "_build", "liboracle.so")
+        path = os.path.join(_HERE, "_build", _FILES[_BUILD])
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)
         d, sz, i = C.c_double, C.c_size_t, C.c_int
+        L.or_set_threads.restype = i
+        L.or_set_threads.argtypes = [i]
         L.or_interval.restype = sz
         L.or_interval.argtypes = [_dp, sz, d]
         L.or_steffen_init.argtypes = [_dp, _dp, sz, _dp, _dp, _dp, _dp]

@@ -42,6 +42,10 @@ void or_gk61(or_fn f, void *p, double a, double b, double *res,
 int or_qag61(or_fn f, void *p, double a, double b, double epsabs,
              double epsrel, int limit, double *result, double *abserr);
 
+/* threads of the OpenMP baseline build (returns the count in use; 1 in the
+   checker build) */
+int or_set_threads(int n);
+
 /* cell mesh (mesh_tools.pyx, mesh.pyx, polar_mesh.pyx) */
 double or_effective_gravity(double mu, double R_eq, double zeta,
                             double epsilon, int shape);

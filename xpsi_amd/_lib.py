@@ -127,6 +127,14 @@ def lib():
     L.xh_server_stats.argtypes = [vp, vp, i32]
     L.xh_server_destroy.restype = None
     L.xh_server_destroy.argtypes = [vp]
+    L.xh_server_warm.restype = i32
+    L.xh_server_warm.argtypes = [vp]
+    L.xh_pipeline_capture.restype = i32
+    L.xh_pipeline_capture.argtypes = [vp, i32, vp]
+    L.xh_server_set_policy.restype = i32
+    L.xh_server_set_policy.argtypes = [vp, d]
+    L.xh_server_footprint.restype = d
+    L.xh_server_footprint.argtypes = [vp, C.POINTER(Request)]
     L.xh_pipeline_fetch.restype = i32
     L.xh_pipeline_fetch.argtypes = [vp, i32, i32, i32, vp, sz]
     _lib = L
@@ -147,4 +155,5 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",
             "xh_pipeline_members_per_request", "xh_pipeline_work",
             "xh_set_fold_energy_chunk", "xh_set_fold_kernel", "xh_set_moment_shape", "xh_set_fold_rings", "xh_set_lane_phases", "xh_pipeline_use_graphs", "xh_server_create",
-            "xh_server_submit", "xh_server_poll", "xh_server_stats", "xh_server_destroy"]
+            "xh_server_submit", "xh_server_poll", "xh_server_stats", "xh_server_destroy",
+            "xh_server_set_policy", "xh_server_footprint", "xh_server_warm", "xh_pipeline_capture"]

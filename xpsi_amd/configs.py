@@ -256,3 +256,42 @@ def config5_draws(n=1024, seed=5):
         out.append(dict(mass=M[k], radius=R[k], distance=1.0, cos_inclination=ci[k],
                         hot_regions=regs, elsewhere_temperature=5.8))
     return out
+
+
+def config4_model(atmosphere="Num4D"):
+    """Model of the serving workload: the config-2 model (two regions with
+    ceding rings, 4-D table atmosphere, 256 energies, 300 channels, 128
+    leaves); requests carry their own mesh resolution."""
+    return config2(atmosphere)
+
+
+def config4_draws(n=512, seed=4):
+    """Request stream of the serving workload: uniform draws of M 1-3,
+    R 8-16 km, cos i 0-1, primary colatitude 0-pi (the secondary region is
+    its antipode), super radius 0.05-0.4 rad, cede radius super + 0.05-0.4,
+    super log T 6.0-6.6, cede log T 5.8-6.4 (inside the table's range),
+    phase U[0,1), distance 1 kpc, and sqrt_num_cells from {32, 48, 64, 96}
+    for both regions; numpy.random.default_rng(seed) in that order."""
+    rng = np.random.default_rng(seed)
+    M = rng.uniform(1.0, 3.0, n)
+    R = rng.uniform(8.0, 16.0, n)
+    ci = rng.uniform(0.0, 1.0, n)
+    th = rng.uniform(0.0, math.pi, n)
+    rs = rng.uniform(0.05, 0.4, n)
+    rc = rs + rng.uniform(0.05, 0.4, n)
+    Ts = rng.uniform(6.0, 6.6, n)
+    Tc = rng.uniform(5.8, 6.4, n)
+    ph = rng.uniform(0.0, 1.0, n)
+    snc = rng.choice([32, 48, 64, 96], n)
+    out = []
+    for k in range(n):
+        regs = []
+        for g in range(2):
+            t = th[k] if g == 0 else math.pi - th[k]
+            regs.append(dict(phase_shift=ph[k], super_colatitude=t, super_radius=rs[k],
+                             super_temperature=Ts[k], cede_colatitude=t, cede_radius=rc[k],
+                             cede_azimuth=0.0, cede_temperature=Tc[k],
+                             sqrt_num_cells=int(snc[k])))
+        out.append(dict(mass=M[k], radius=R[k], distance=1.0, cos_inclination=ci[k],
+                        hot_regions=regs))
+    return out

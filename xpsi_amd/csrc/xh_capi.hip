@@ -5,6 +5,8 @@
 // per-request descriptors, launches the kernel sequence on one stream and
 // copies back one double per request.
 #include <hip/hip_runtime.h>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -32,6 +34,17 @@ static int fail(int code, const std::string& msg)
 extern "C" const char* xh_last_error(void) { return g_err.c_str(); }
 
 // physical constants of xpsi/global_imports.py:67-88
+// Seeds of the llzero-class draws: a process-wide counter mixed with the
+// start time, so every launch and every process draws afresh (the reference
+// draws from numpy's global generator, xpsi/Likelihood.py:268-272).
+static std::atomic<unsigned long long> g_seed_counter{0};
+static unsigned long long next_seed()
+{
+    static const unsigned long long base =
+        (unsigned long long)std::chrono::steady_clock::now().time_since_epoch().count() * 0x9E3779B97F4A7C15ULL;
+    return base + g_seed_counter.fetch_add(1) + 1;
+}
+
 static const double kC = 2.99792458e8, kKPC = 3.08567758e19, kGM = 0.5 * 2.95325024e3,
                     kG = 6.6740831e-11, kKM = 1.0e3;
 
@@ -366,6 +379,14 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
         return fail(XH_EINVAL, "xh_integrate: too many phases for the leaf count");
     if (workspace_bytes < xh_integrate_workspace_bytes(S, N_R, N_L, N_E, N_P))
         return fail(XH_EINVAL, "xh_integrate: workspace too small");
+    int hN[4] = {0, 0, 0, 0};
+    if (hot_atm_ext == 2) {
+        // axis lengths first, in stream order, before anything is enqueued
+        HIPCHK(hipMemcpyAsync(hN, atm_N, sizeof(hN), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int d = 0; d < 4; d++)
+            if (hN[d] < 4) return fail(XH_EINVAL, "xh_integrate: table axes need at least 4 nodes");
+    }
     const int IO = 8;
     char* w = (char*)workspace;
     Member* d_m = (Member*)w; w += 256;
@@ -391,14 +412,8 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     HIPCHK(hipGetLastError());
     FoldArgs FA{d_geo, d_valid, cellArea, radiates, phi, S, d_logT, d_logg, d_lastq, atm_axes, atm_I,
                 {0, 0, 0, 0}, energies, phases, flux, S};
-    if (hot_atm_ext == 2) {
-        int hN[4];
-        HIPCHK(hipMemcpy(hN, atm_N, sizeof(hN), hipMemcpyDeviceToHost));
-        for (int d = 0; d < 4; d++) {
-            if (hN[d] < 4) return fail(XH_EINVAL, "xh_integrate: table axes need at least 4 nodes");
-            FA.atm_N[d] = hN[d];
-        }
-    }
+    if (hot_atm_ext == 2)
+        for (int d = 0; d < 4; d++) FA.atm_N[d] = hN[d];
     if (lane) {
         const int nrb = std::min(S, 8), rb = (S + nrb - 1) / nrb;
         FA.ring_block = rb;
@@ -475,21 +490,11 @@ extern "C" int xh_energy_integrator(int N_E, int N_P, const double* signal, cons
     hipStream_t s = (hipStream_t)stream;
     if (N_E < 5 || N_P < 1 || N_edges < 2) return fail(XH_EINVAL, "xh_energy_integrator: bad sizes");
     if (N_E > 8192) return fail(XH_EINVAL, "xh_energy_integrator: at most 8192 energies (one LDS row)");
-    // one "member" with d^2 = 1 carries the signal
-    static thread_local Member* d_m = nullptr;
-    static thread_local int* d_idx = nullptr;
-    if (!d_m) {
-        HIPCHK(hipMalloc(&d_m, sizeof(Member)));
-        HIPCHK(hipMalloc(&d_idx, 2 * sizeof(int)));
-        Member hm{};
-        hm.active = 1; hm.d_sq = 1.0;
-        int idx[2] = {0, -1};
-        HIPCHK(hipMemcpy(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_idx, idx, sizeof(idx), hipMemcpyHostToDevice));
-    }
-    EnergyArgs A{signal, logE, logEdges, nullptr, d_idx, out, N_E, N_P, N_edges, 2};
+    // no member table: the kernel integrates the signal itself (d^2 = 1)
+    EnergyArgs A{signal, logE, logEdges, nullptr, nullptr, out, N_E, N_P, N_edges, 1};
     int n = (N_edges - 1) * N_P;
-    hipLaunchKernelGGL(k_energy_integrate, dim3((n + 255) / 256, 1), dim3(256), (size_t)N_E * 8, s, d_m, A);
+    hipLaunchKernelGGL(k_energy_integrate, dim3((n + 255) / 256, 1), dim3(256), (size_t)N_E * 8, s,
+                       (const Member*)nullptr, A);
     HIPCHK(hipGetLastError());
     return XH_OK;
 }
@@ -537,7 +542,7 @@ extern "C" int xh_eval_marginal_likelihood(double T, int N_b, const double* phas
     A.ll = d_ll; A.status = d_st; A.B_ML = B_ML; A.B_ML_supp = B_ML_supp; A.STAR = STAR;
     A.exposure = T; A.epsabs = epsabs; A.epsrel = epsrel; A.epsilon = epsilon; A.sigmas = sigmas;
     A.llzero = llzero; A.slim = slim; A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp;
-    A.allow_negative = allow_negative; A.seed = 0;
+    A.allow_negative = allow_negative; A.seed = next_seed(); A.seed_ptr = nullptr;
     HIPCHK(fill(STAR, (size_t)N_ch * N_b, 0.0, s));
     HIPCHK(fill(B_ML, N_ch, 0.0, s));
     HIPCHK(fill(B_ML_supp, N_ch, 0.0, s));
@@ -583,6 +588,8 @@ struct xh_pipeline {
     Member* h_members;
     RegionAlloc* h_regions;
     double *h_shifts, *h_lnl;
+    unsigned long long *h_seed, *d_seed;   // llzero-class draw of each launch (pinned host -> device)
+    bool inflight;                         // a launch not yet waited for
     hipStream_t stream;
     int last_n;
     bool use_graphs;
@@ -617,6 +624,7 @@ extern "C" void xh_pipeline_destroy(xh_pipeline* p)
     if (p->h_regions) (void)hipHostFree(p->h_regions);
     if (p->h_shifts) (void)hipHostFree(p->h_shifts);
     if (p->h_lnl) (void)hipHostFree(p->h_lnl);
+    if (p->h_seed) (void)hipHostFree(p->h_seed);
     if (p->h_else) (void)hipHostFree(p->h_else);
     delete p;
 }
@@ -790,6 +798,9 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     PHIP(hipHostMalloc((void**)&p->h_regions, ng * sizeof(RegionAlloc), hipHostMallocDefault));
     PHIP(hipHostMalloc((void**)&p->h_shifts, ng * sizeof(double), hipHostMallocDefault));
     PHIP(hipHostMalloc((void**)&p->h_lnl, (size_t)p->maxb * sizeof(double), hipHostMallocDefault));
+    PHIP(hipHostMalloc((void**)&p->h_seed, sizeof(unsigned long long), hipHostMallocDefault));
+    PCHK(dalloc(p, &p->d_seed, 1));
+    p->h_seed[0] = next_seed();
     // constant inputs
     std::vector<double> v(std::max<size_t>({NE, (size_t)c->N_in + 1, (size_t)c->num_leaves, (size_t)N_P}));
     PHIP(hipMemcpy(p->d_energies, c->energies, NE * 8, hipMemcpyHostToDevice));
@@ -923,32 +934,54 @@ static void describe(const xh_pipeline* p, int rq, const xh_request& r)
 }
 
 static int enqueue(xh_pipeline* p, int n, hipStream_t s);
+static int capture_graph(xh_pipeline* p, int n, hipStream_t s);
 
 extern "C" int xh_pipeline_launch(xh_pipeline* p, int n, const xh_request* reqs, void* stream)
 {
     if (!p || n < 1 || n > p->maxb) return fail(XH_EINVAL, "xh_pipeline_launch: batch size out of range");
     hipStream_t s = (hipStream_t)stream;
+    // the pinned descriptors and results are shared with the previous launch:
+    // let it finish before they are rewritten
+    if (p->inflight) {
+        HIPCHK(hipStreamSynchronize(p->stream));
+        p->inflight = false;
+    }
     p->stream = s;
     p->last_n = n;
     for (int rq = 0; rq < n; rq++) describe(p, rq, reqs[rq]);
+    p->h_seed[0] = next_seed();
+    p->inflight = true;
     if (!p->use_graphs || p->timing || s == nullptr) return enqueue(p, n, s);
-    if ((int)p->graphs.size() <= n) p->graphs.resize(n + 1, nullptr);
-    if (!p->graphs[n]) {
-        // capture the whole device sequence once for this batch size; the
-        // host descriptors it copies from are refreshed before each replay
-        hipGraph_t g;
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue(p, n, s);
-        hipError_t e = hipStreamEndCapture(s, &g);
-        if (rc != XH_OK) return rc;
-        HIPCHK(e);
-        hipGraphExec_t ge;
-        HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
-        p->graphs[n] = ge;
-    }
+    int rc = capture_graph(p, n, s);
+    if (rc != XH_OK) return rc;
     HIPCHK(hipGraphLaunch(p->graphs[n], s));
     return XH_OK;
+}
+
+// Capture the whole device sequence once for batch size n; the host
+// descriptors it copies from are refreshed before each replay.
+static int capture_graph(xh_pipeline* p, int n, hipStream_t s)
+{
+    if ((int)p->graphs.size() <= n) p->graphs.resize(n + 1, nullptr);
+    if (p->graphs[n]) return XH_OK;
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(p, n, s);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != XH_OK) return rc;
+    HIPCHK(e);
+    hipGraphExec_t ge;
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    p->graphs[n] = ge;
+    return XH_OK;
+}
+
+extern "C" int xh_pipeline_capture(xh_pipeline* p, int n, void* stream)
+{
+    if (!p || n < 1 || n > p->maxb) return fail(XH_EINVAL, "xh_pipeline_capture: batch size out of range");
+    if (stream == nullptr) return fail(XH_EINVAL, "xh_pipeline_capture: needs a non-default stream");
+    return capture_graph(p, n, (hipStream_t)stream);
 }
 
 static int enqueue(xh_pipeline* p, int n, hipStream_t s)
@@ -959,6 +992,7 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     HIPCHK(hipMemcpyAsync(p->d_members, p->h_members, (size_t)nm * sizeof(Member), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(p->d_regions, p->h_regions, (size_t)ng * sizeof(RegionAlloc), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(p->d_shifts, p->h_shifts, (size_t)ng * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(p->d_seed, p->h_seed, sizeof(unsigned long long), hipMemcpyHostToDevice, s));
     if (p->has_else)
         HIPCHK(hipMemcpyAsync(p->d_else, p->h_else, (size_t)n * sizeof(Member), hipMemcpyHostToDevice, s));
     HIPCHK(fill(p->d_cellArea, (size_t)nm * S * S, 0.0, s));
@@ -1076,7 +1110,8 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     MA.B_ML = p->d_bml; MA.B_ML_supp = p->d_bmls; MA.STAR = p->d_star;
     MA.exposure = p->cfg.exposure_time; MA.epsabs = p->cfg.epsabs; MA.epsrel = p->cfg.epsrel;
     MA.epsilon = p->cfg.epsilon; MA.sigmas = p->cfg.sigmas; MA.llzero = p->cfg.llzero; MA.slim = p->cfg.slim;
-    MA.N_ch = p->N_ch; MA.N_b = p->N_b; MA.N_P = N_P; MA.n_comp = p->nreg; MA.allow_negative = 0; MA.seed = 0;
+    MA.N_ch = p->N_ch; MA.N_b = p->N_b; MA.N_P = N_P; MA.n_comp = p->nreg; MA.allow_negative = 0;
+    MA.seed = 0; MA.seed_ptr = p->d_seed;
     if (p->timing) HIPCHK(hipEventRecord(p->ev[5], s));
     hipLaunchKernelGGL(k_marginal, dim3(p->N_ch, n), dim3(64), 2 * p->N_b * 8, s, MA);
     HIPCHK(hipGetLastError());
@@ -1096,6 +1131,7 @@ extern "C" int xh_pipeline_use_graphs(xh_pipeline* p, int on)
 extern "C" int xh_pipeline_sync(xh_pipeline* p)
 {
     HIPCHK(hipStreamSynchronize(p->stream));
+    p->inflight = false;
     if (p->timing) {
         for (int k = 0; k < 6; k++) {
             float ms = 0.0f;

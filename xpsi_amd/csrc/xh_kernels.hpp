@@ -37,7 +37,11 @@ struct MarginalArgs {
     double* B_ML_supp;          // [request][N_ch]
     double* STAR;               // [request][N_ch][N_b] or nullptr
     double exposure, epsabs, epsrel, epsilon, sigmas, llzero, slim;
-    int N_ch, N_b, N_P, n_comp, allow_negative, seed;
+    int N_ch, N_b, N_P, n_comp, allow_negative;
+    // draw of the llzero-class values: *seed_ptr when set (device memory the
+    // host refreshes before each launch, so replayed graphs draw afresh), else seed
+    unsigned long long seed;
+    const unsigned long long* seed_ptr;
 };
 
 __global__ void k_allocate(const RegionAlloc* regions, Member* members, int S_max);

@@ -3,12 +3,24 @@
 // A server owns `n_slots` pipelines, each with its own HIP stream and device
 // buffers for `slot_batch` requests.  Requests are submitted from any thread
 // into a FIFO; one dispatcher thread
-//   - harvests slots whose batch has finished (event query, no blocking),
-//   - admits waiting requests into every free slot, in arrival order, while
-//     the in-flight HBM footprint stays within the budget, and launches the
-//     batch on the slot's stream (optionally as a replayed hipGraph).
+//   - harvests slots whose batch has finished (event query, no blocking; a
+//     failed batch completes its requests with NaN and records the error),
+//   - forms a batch for a free slot when a full batch is queued, when the
+//     oldest queued request has waited max_wait, or when no slot is busy
+//     (an idle device never waits), and
+//   - admits queued requests into it in arrival order while the device
+//     footprint of all batches in flight stays within the budget, then
+//     launches it on the slot's stream (optionally as a replayed hipGraph).
 // Batches on different slots overlap on the device; a request arriving while
 // others run is admitted as soon as a slot and memory are free.
+//
+// The footprint of a request is the device memory its work occupies in a
+// slot's buffers: per member (super and ceding ring of each region) the mesh
+// (S^2 cell areas, S-sized parallel arrays), the ray tables (3 S N_R), the
+// per-parallel geometry (5 S N_L per image order, two orders assumed) and the
+// flux (N_E N_P), with S the request's sqrt_num_cells; per request the
+// energy-integrated and folded signals and the likelihood's per-channel
+// arrays.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -55,8 +67,15 @@ struct Slot {
 struct xh_server {
     std::vector<Slot> slots;
     int slot_batch = 1;
-    double budget = 0.0, inflight = 0.0;
-    double bytes_per_cell = 0.0, bytes_fixed = 0.0;
+    double budget = 0.0, inflight = 0.0, peak_inflight = 0.0;
+    double max_wait = 0.0;       // seconds a partial batch may wait for more requests
+    int max_batch_seen = 0;
+    bool use_graphs = false;
+    // footprint model (bytes), from the model configuration
+    int n_regions = 1;
+    int members[2] = {1, 1};     // members per region (super + ceding ring)
+    int snc[2] = {32, 32};       // configured sqrt_num_cells per region
+    double per_cell = 0.0, per_side = 0.0, per_member = 0.0, per_request = 0.0;
     std::deque<Pending> queue;
     std::vector<Done> done;
     std::mutex mu;
@@ -71,14 +90,12 @@ struct xh_server {
 
 static double footprint(const xh_server* s, const xh_request& r)
 {
-    // device bytes the request's work touches: mesh cells, ray tables and
-    // per-parallel geometry scale with the mesh size
-    double cells = 0.0;
-    for (int g = 0; g < 2; g++) {
-        int n = r.sqrt_num_cells[g] > 0 ? r.sqrt_num_cells[g] : 32;
-        cells += (double)n * n;
+    double b = s->per_request;
+    for (int g = 0; g < s->n_regions; g++) {
+        const double n = r.sqrt_num_cells[g] > 0 ? r.sqrt_num_cells[g] : s->snc[g];
+        b += s->members[g] * (s->per_member + s->per_side * n + s->per_cell * n * n);
     }
-    return s->bytes_fixed + s->bytes_per_cell * cells;
+    return b;
 }
 
 static void serve_loop(xh_server* s)
@@ -87,15 +104,19 @@ static void serve_loop(xh_server* s)
     while (!s->stop.load()) {
         bool progressed = false;
         // harvest finished slots
+        int busy = 0;
         for (auto& sl : s->slots) {
             if (!sl.busy) continue;
-            if (hipEventQuery(sl.done) != hipSuccess) continue;
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipErrorNotReady) { busy++; continue; }
+            const bool ok = q == hipSuccess;
             const double* r = xh_pipeline_results(sl.pl);
             auto now = Clock::now();
             std::lock_guard<std::mutex> lk(s->mu);
+            if (!ok) s->error = std::string("batch failed: ") + hipGetErrorString(q);
             for (size_t k = 0; k < sl.batch.size(); k++) {
                 double lat = std::chrono::duration<double>(now - sl.batch[k].t_submit).count();
-                s->done.push_back({sl.batch[k].id, r[k], lat});
+                s->done.push_back({sl.batch[k].id, ok ? r[k] : 0.0 / 0.0, lat});
             }
             s->completed += (long long)sl.batch.size();
             s->inflight -= sl.bytes;
@@ -109,6 +130,11 @@ static void serve_loop(xh_server* s)
             {
                 std::lock_guard<std::mutex> lk(s->mu);
                 if (s->queue.empty()) break;
+                // batch formation: a full batch, a request at its wait limit,
+                // or an idle device
+                const double waited =
+                    std::chrono::duration<double>(Clock::now() - s->queue.front().t_submit).count();
+                if ((int)s->queue.size() < s->slot_batch && waited < s->max_wait && busy > 0) break;
                 sl.bytes = 0.0;
                 while (!s->queue.empty() && (int)sl.batch.size() < s->slot_batch) {
                     const Pending& q = s->queue.front();
@@ -120,12 +146,15 @@ static void serve_loop(xh_server* s)
                 }
                 if (sl.batch.empty()) continue;
                 s->inflight += sl.bytes;
+                if (s->inflight > s->peak_inflight) s->peak_inflight = s->inflight;
+                if ((int)sl.batch.size() > s->max_batch_seen) s->max_batch_seen = (int)sl.batch.size();
             }
             sl.reqs.resize(sl.batch.size());
             for (size_t k = 0; k < sl.batch.size(); k++) sl.reqs[k] = sl.batch[k].req;
             int rc = xh_pipeline_launch(sl.pl, (int)sl.reqs.size(), sl.reqs.data(), sl.stream);
             if (rc == XH_OK && hipEventRecord(sl.done, sl.stream) == hipSuccess) {
                 sl.busy = true;
+                busy++;
                 s->launched_batches++;
                 s->batch_fill += (double)sl.batch.size() / s->slot_batch;
             } else {
@@ -153,9 +182,16 @@ extern "C" xh_server* xh_server_create(const xh_model_config* cfg, int n_slots, 
     s->budget = hbm_budget_bytes > 0.0 ? hbm_budget_bytes : 1e300;
     const double NR = cfg->num_rays, NL = cfg->num_leaves, NE = cfg->N_E;
     const double NP = cfg->num_phases > 0 ? cfg->num_phases : cfg->num_leaves;
-    // per mesh cell: area + per-parallel ray tables and geometry amortised
-    s->bytes_per_cell = 8.0 * (1.0 + 3.0 * NR / 64.0 + 5.0 * NL * 2.0 / 64.0);
-    s->bytes_fixed = 8.0 * (1000.0 + NE * NP * 2.0 + (double)cfg->N_ch * NP * 2.0);
+    s->n_regions = cfg->n_regions;
+    for (int g = 0; g < cfg->n_regions && g < 2; g++) {
+        s->members[g] = cfg->region[g].cede ? 2 : 1;
+        s->snc[g] = cfg->region[g].sqrt_num_cells;
+    }
+    s->per_cell = 8.0;                                          // cell areas
+    s->per_side = 8.0 * (10.0 + 3.0 * NR + 2.0 * 5.0 * NL) + 4.0 * 2.0 * 2.0;
+    s->per_member = 8.0 * NE * NP + 256.0;                      // flux, descriptor
+    s->per_request = 8.0 * (cfg->n_regions * ((double)cfg->N_in + cfg->N_ch) * NP) +
+                     8.0 * (double)cfg->N_ch * (cfg->N_b + 4.0) + 4.0 * cfg->N_ch;
     xh_model_config c = *cfg;
     c.max_batch = slot_batch;
     s->slots.resize(n_slots);
@@ -172,6 +208,7 @@ extern "C" xh_server* xh_server_create(const xh_model_config* cfg, int n_slots, 
             return nullptr;
         }
         xh_pipeline_use_graphs(sl.pl, use_graphs);
+        s->use_graphs = use_graphs != 0;
     }
     s->worker = std::thread(serve_loop, s);
     return s;
@@ -208,14 +245,44 @@ extern "C" int xh_server_poll(xh_server* s, long long* ids, double* lnl, double*
     return n;
 }
 
-// stats: queued, in-flight bytes, launched batches, completed, mean batch fill
+// stats: queued, in-flight bytes, launched batches, completed, mean batch
+// fill, peak in-flight bytes, largest batch, failed batches recorded (0/1)
 extern "C" int xh_server_stats(xh_server* s, double* out, int n)
 {
     std::lock_guard<std::mutex> lk(s->mu);
-    double v[5] = {(double)s->queue.size(), s->inflight, (double)s->launched_batches, (double)s->completed,
-                   s->launched_batches ? s->batch_fill / s->launched_batches : 0.0};
-    for (int k = 0; k < n && k < 5; k++) out[k] = v[k];
+    double v[8] = {(double)s->queue.size(), s->inflight, (double)s->launched_batches, (double)s->completed,
+                   s->launched_batches ? s->batch_fill / s->launched_batches : 0.0, s->peak_inflight,
+                   (double)s->max_batch_seen, s->error.empty() ? 0.0 : 1.0};
+    for (int k = 0; k < n && k < 8; k++) out[k] = v[k];
     return XH_OK;
+}
+
+extern "C" int xh_server_warm(xh_server* s)
+{
+    if (!s) return XH_EINVAL;
+    if (!s->use_graphs) return XH_OK;
+    std::lock_guard<std::mutex> lk(s->mu);   // the dispatcher launches nothing meanwhile
+    for (auto& sl : s->slots) {
+        if (sl.busy && hipEventSynchronize(sl.done) != hipSuccess) return XH_EHIP;
+        for (int n = 1; n <= s->slot_batch; n++) {
+            int rc = xh_pipeline_capture(sl.pl, n, sl.stream);
+            if (rc != XH_OK) return rc;
+        }
+    }
+    return XH_OK;
+}
+
+extern "C" int xh_server_set_policy(xh_server* s, double max_wait_s)
+{
+    if (!s || !(max_wait_s >= 0.0)) return XH_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->max_wait = max_wait_s;
+    return XH_OK;
+}
+
+extern "C" double xh_server_footprint(xh_server* s, const xh_request* req)
+{
+    return (s && req) ? footprint(s, *req) : 0.0;
 }
 
 extern "C" void xh_server_destroy(xh_server* s)

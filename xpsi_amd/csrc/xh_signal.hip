@@ -35,12 +35,18 @@ __global__ void __launch_bounds__(256) k_energy_integrate(const Member* __restri
     const double hi = A.logEdges[j + 1] > maxE ? maxE : A.logEdges[j + 1];
     double total = 0.0;
     for (int s = 0; s < A.members_per_region; s++) {
-        const int mi = A.member_index[reg * A.members_per_region + s];
-        if (mi < 0) continue;
-        const Member m = members[mi];
-        if (!m.active) continue;
+        // without a member table (the drop-in): region reg's own signal, d^2 = 1
+        int mi = reg;
+        double d_sq = 1.0;
+        if (A.member_index) {
+            mi = A.member_index[reg * A.members_per_region + s];
+            if (mi < 0) continue;
+            const Member& m = members[mi];
+            if (!m.active) continue;
+            d_sq = m.d_sq;
+        } else if (s > 0) break;
         const double* fl = A.flux + (size_t)mi * N_E * N_P + k;
-        auto yv = [&](int e) { return wE[e] * (fl[(size_t)e * N_P] / m.d_sq) * log(10.0); };
+        auto yv = [&](int e) { return wE[e] * (fl[(size_t)e * N_P] / d_sq) * log(10.0); };
         double v;
         if (lo > hi || lo < A.logE[0] || hi > maxE) v = __builtin_nan("");
         else if (lo == hi) v = 0.0;
@@ -411,10 +417,14 @@ __global__ void __launch_bounds__(64) k_lnl_reduce(MarginalArgs A, const Member*
             if (m.active && m.status) bad = 1;
         }
     if (bad || s <= A.llzero) {
-        unsigned long long h = 0x9E3779B97F4A7C15ULL * (unsigned long long)(rq + 1 + A.seed);
-        h ^= h >> 31;
+        // splitmix64 of (launch seed, request): a fresh draw per launch and request
+        const unsigned long long sd = A.seed_ptr ? *A.seed_ptr : A.seed;
+        unsigned long long h = sd * 0x100000001B3ULL + (unsigned long long)(rq + 1) * 0x9E3779B97F4A7C15ULL;
+        h ^= h >> 30;
         h *= 0xBF58476D1CE4E5B9ULL;
         h ^= h >> 27;
+        h *= 0x94D049BB133111EBULL;
+        h ^= h >> 31;
         double u = (double)(h >> 11) / 9007199254740992.0;
         s = A.llzero * (0.1 + 0.9 * u);
     }

@@ -62,8 +62,9 @@ def construct_energy_array(num_energies, edges):
 
 
 # "fp64": FP64 throughout.  "mixed": the precision form - FP32 intensity and
-# spline fold on the matrix cores, split-BF16 response fold with FP32
-# accumulation; rays, phase-bin accumulation and the likelihood stay FP64.
+# spline fold on the matrix cores, response fold on FP32 MFMA with FP64 sums
+# per 4-deep slice (fold_terms 6 or 3: split-BF16 instead); rays, phase-bin
+# accumulation and the likelihood stay FP64.
 PRECISIONS = {"fp64": 0, "mixed": 1}
 # "moment": spline fold as interval-moment products on the matrix cores;
 # "lanes": energies across a wave's lanes, (cell, phase) queries in lockstep;
@@ -202,16 +203,23 @@ class LikelihoodPipeline:
         fill_request(self._reqs[k], p)
 
     def launch(self, params, stream=None):
+        """Start one batch; its results are read by wait().  A pipeline holds
+        one batch at a time (its pinned descriptors and results are reused),
+        so a second launch before wait() raises."""
         n = len(params)
         if n > self.max_batch:
             raise XHError("batch of %d exceeds max_batch %d" % (n, self.max_batch))
+        if getattr(self, "_inflight", False):
+            raise XHError("launch: the previous batch has not been waited for")
         for k, p in enumerate(params):
             self._fill(k, p)
         check(lib().xh_pipeline_launch(self._h, n, self._reqs, stream), "xh_pipeline_launch")
         self._n = n
+        self._inflight = True
 
     def wait(self):
         check(lib().xh_pipeline_sync(self._h), "xh_pipeline_sync")
+        self._inflight = False
         res = lib().xh_pipeline_results(self._h)
         return np.array([res[k] for k in range(self._n)])
 
