@@ -16,6 +16,16 @@ def say(msg):
     sys.stdout.flush()
 
 
+def same_lnl(got, ref, tol):
+    """Finite values agree within tol; llzero-class values (random draws in
+    [llzero, 0.1 llzero], fresh per launch) agree as a class."""
+    fin = ref > -1e89
+    assert np.array_equal(fin, got > -1e89)
+    assert np.all((got[~fin] >= -1e90) & (got[~fin] <= -1e89))
+    if fin.any():
+        assert np.max(np.abs(got[fin] - ref[fin])) < tol
+
+
 def _draws(ne, n):
     from xpsi_amd import configs
     out = [d for d in configs.config3_draws()
@@ -38,9 +48,9 @@ def test_graph_replay_matches_eager():
     for rep in range(3):
         say("graph replay %d (batch 12, then 5)" % rep)
         got = pl.evaluate(draws, s.cuda_stream)
-        assert np.array_equal(got, ref) or np.max(np.abs(got - ref)) < 1e-9
+        same_lnl(got, ref, 1e-9)
         got5 = pl.evaluate(draws[:5], s.cuda_stream)
-        assert np.max(np.abs(got5 - ref[:5])) < 1e-9
+        same_lnl(got5, ref[:5], 1e-9)
         # eager work and a synchronisation between the replays
         eager.evaluate(draws[3:6])
         torch.cuda.synchronize()
