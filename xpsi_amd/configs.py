@@ -268,8 +268,8 @@ def config4_model(atmosphere="Num4D"):
 def config4_draws(n=512, seed=4):
     """Request stream of the serving workload: uniform draws of M 1-3,
     R 8-16 km, cos i 0-1, primary colatitude 0-pi (the secondary region is
-    its antipode), super radius 0.05-0.4 rad, cede radius super + 0.05-0.4,
-    super log T 6.0-6.6, cede log T 5.8-6.4 (inside the table's range),
+    its antipode), super radius 0.05-0.3 rad, cede radius super + 0.05-0.3,
+    super log T 5.9-6.35, cede log T 5.8-6.1 (inside the table's range),
     phase U[0,1), distance 1 kpc, and sqrt_num_cells from {32, 48, 64, 96}
     for both regions; numpy.random.default_rng(seed) in that order."""
     rng = np.random.default_rng(seed)
@@ -277,10 +277,10 @@ def config4_draws(n=512, seed=4):
     R = rng.uniform(8.0, 16.0, n)
     ci = rng.uniform(0.0, 1.0, n)
     th = rng.uniform(0.0, math.pi, n)
-    rs = rng.uniform(0.05, 0.4, n)
-    rc = rs + rng.uniform(0.05, 0.4, n)
-    Ts = rng.uniform(6.0, 6.6, n)
-    Tc = rng.uniform(5.8, 6.4, n)
+    rs = rng.uniform(0.05, 0.3, n)
+    rc = rs + rng.uniform(0.05, 0.3, n)
+    Ts = rng.uniform(5.9, 6.35, n)
+    Tc = rng.uniform(5.8, 6.1, n)
     ph = rng.uniform(0.0, 1.0, n)
     snc = rng.choice([32, 48, 64, 96], n)
     out = []
