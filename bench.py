@@ -36,6 +36,14 @@ FOLD_TRAFFIC_KIB = {256: (17123810, 20011418), 128: (9762315, 9300707), 64: (679
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    """Progress on stderr (the JSON line on stdout stays the only output there)."""
+    print("[bench %7.1fs] %s" % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
+
+
+_T0 = time.perf_counter()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,7 +58,7 @@ def main():
     ap.add_argument("--one-stream", action="store_true", help="launch every energy group on one stream")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-batch", type=int, default=512)
+    ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-wait", type=float, default=0.002, help="latency cap of a partial batch (s)")
     ap.add_argument("--serve-budget", type=float, default=200e9, help="device footprint budget (bytes)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
@@ -69,6 +77,7 @@ def main():
 
     # the CPU baseline first: its process pool forks before the GPU is touched
     cpu = None if args.no_cpu else cpu_baseline(args)
+    log("cpu baseline done")
 
     import torch
     from xpsi_amd import configs
@@ -94,6 +103,7 @@ def main():
         lat.append(time.perf_counter() - t0)
     lat = np.array(lat) * 1e3
     single_rate = 1e3 / lat.mean()
+    log("plain FP64 single request: %.3f ms" % lat.mean())
 
     # ---------------- batched form
     draws = configs.config3_draws(args.n)
@@ -126,6 +136,7 @@ def main():
         for ne in order:
             out[ne] = pls[ne].wait()
 
+    log("autotuned fold shapes: %s" % {ne: t["chosen"] for ne, t in tuned.items()})
     for _ in range(args.warmup):
         step()
     for pl in pls.values():
@@ -136,6 +147,7 @@ def main():
         step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    log("batched form: %.1f ms per step" % (dt / args.steps * 1e3))
     stage = {k: 0.0 for k in LikelihoodPipeline.STAGES}
     for pl in pls.values():
         for k, v in pl.stage_ms().items():
@@ -221,6 +233,7 @@ def main():
         lb = step1()
     torch.cuda.synchronize()
     batched1_rate = nb1 * steps1 / (time.perf_counter() - t0)
+    log("batched config-1 requests: %.0f lnL/s" % batched1_rate)
     batched1_dlnl = float(np.max(np.abs(lb - l1)))
     for pl in plbs:
         pl.close()
@@ -280,8 +293,10 @@ def main():
         del src, dst
     if args.c5_n > 0:
         out["config5"] = config5_section(args, stream)
+        log("large workload done")
     if not args.no_serving:
         out["serving"] = serving(args, stream_free=True)
+        log("serving done")
     if cpu is not None:
         out["cpu_baseline"] = cpu
         out["speedup_vs_cpu_baseline"] = value / cpu["value"]
@@ -326,11 +341,23 @@ def cpu_baseline(args):
     affinity = len(os.sched_getaffinity(0))
     cores = max(1, min(16, affinity))          # the GPU box's CPU share is 16
     budget = args.cpu_seconds
-    # one request per core (forked before any OpenMP thread pool or GPU use)
-    ctx = mp.get_context("fork")
-    with ctx.Pool(cores) as pool:
-        res = pool.map(_cpu_worker, [budget / 3.0] * cores)
+    # one request per core: fresh interpreters (spawned before this process
+    # touches the GPU), one BLAS/OpenMP thread each
+    ctx = mp.get_context("spawn")
+    saved = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS")}
+    os.environ.update(OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1")
+    try:
+        log("cpu: %d worker processes for %.1f s" % (cores, budget / 3.0))
+        with ctx.Pool(cores) as pool:
+            res = pool.map(_cpu_worker, [budget / 3.0] * cores)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     per_core_rate = sum(n / dt for n, dt in res)
+    log("cpu: one request per core on %d processes: %.1f lnL/s" % (cores, per_core_rate))
     O.select_build("fast")
     m1 = configs.config1()
 
@@ -349,7 +376,9 @@ def cpu_baseline(args):
                 "p99_ms": float(np.percentile(lat, 99)), "evals_per_s": 1e3 / float(lat.mean())}
 
     one = lat_leg(1, 30, budget / 3.0)
+    log("cpu: one thread %.1f ms per evaluation" % one["p50_ms"])
     allc = lat_leg(cores, 30, budget / 6.0)
+    log("cpu: %d OpenMP threads %.1f ms per evaluation" % (cores, allc["p50_ms"]))
     O.set_threads(1)
     return {"value": one["evals_per_s"], "unit": "lnL evals/s", "cores": 1, "kind": "port",
             "sample": "config-1 requests one at a time on one thread (%d evaluations), the C "
@@ -428,9 +457,11 @@ def serving(args, stream_free=True):
     srv = LikelihoodServer(dict(model, precision=args.precision), n_slots=args.serve_slots,
                            slot_batch=args.serve_batch, hbm_budget_bytes=args.serve_budget,
                            use_graphs=True, max_wait_s=args.serve_wait)
+    log("serving: plain single requests %.3f ms; capturing graphs" % float(lat1.mean()))
     t0 = time.perf_counter()
     srv.warm()
     warm_s = time.perf_counter() - t0
+    log("serving: graphs captured in %.1f s" % warm_s)
 
     def drain(n, timeout=120.0):
         lat, done, t0 = [], 0, time.perf_counter()
@@ -448,6 +479,7 @@ def serving(args, stream_free=True):
     done, _ = drain(nb)
     sustained = done / (time.perf_counter() - t0)
     out = {"sustained_max_req_per_s": sustained, "burst": nb}
+    log("serving: sustained %.0f req/s" % sustained)
     rng = np.random.default_rng(4)
     for rate in (2000.0, 5000.0, 10000.0, 20000.0):
         n = int(rate * args.serve_seconds)
@@ -470,6 +502,8 @@ def serving(args, stream_free=True):
         out["%d_req_per_s" % int(rate)] = {
             "offered": rate, "submitted": int(k), "completed": int(done), "throughput": done / wall,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+        log("serving at %d req/s: p50 %.2f ms p99 %.2f ms" % (rate, np.percentile(lat, 50),
+                                                               np.percentile(lat, 99)))
         if done < n:
             drain(k - done)
     st = srv.stats()
