@@ -124,6 +124,30 @@ int xh_eval_marginal_likelihood(double exposure_time, int N_b,
                                 double* B_ML_supp, void* workspace,
                                 size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference: xh_eval_marginal_likelihood without the
+ * workspace argument, for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its scratch is a static device array of the library, one per
+ * GPU, holding XH_NOALLOC_MAX_CH channels; more channels fail with
+ * XH_ERANGE.  Calls that overlap on one GPU share that array and race.
+ * It allocates nothing; results equal xh_eval_marginal_likelihood's. */
+#define XH_NOALLOC_MAX_CH 16384
+int xh_eval_marginal_likelihood_noalloc(double exposure_time, int N_b,
+                                        const double* phases, int N_ch,
+                                        const double* counts, int n_comp,
+                                        const double* components, int N_P,
+                                        const double* comp_phases,
+                                        const double* shifts,
+                                        const double* precomp,
+                                        const double* support, double epsabs,
+                                        double epsrel, double epsilon,
+                                        double sigmas, double llzero,
+                                        int allow_negative, double slim,
+                                        const double* background,
+                                        double* lnl_out, double* STAR,
+                                        double* B_ML, double* B_ML_supp,
+                                        void* stream);
+
 /* ------------------------------------------------------------------------
  * Whole-likelihood pipeline: mesh, rays, integration, registration and the
  * marginal likelihood for a batch of parameter vectors, all on the device.

@@ -85,6 +85,10 @@ def lib():
     L.xh_eval_marginal_likelihood.argtypes = (
         [d, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, d, d, d, d, d, i32, d,
          vp, C.POINTER(d), vp, vp, vp, vp, sz, vp])
+    L.xh_eval_marginal_likelihood_noalloc.restype = i32
+    L.xh_eval_marginal_likelihood_noalloc.argtypes = (
+        [d, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, d, d, d, d, d, i32, d,
+         vp, C.POINTER(d), vp, vp, vp, vp])
     L.xh_pipeline_create.restype = vp
     L.xh_pipeline_create.argtypes = [C.POINTER(ModelConfig)]
     L.xh_pipeline_destroy.restype = None
@@ -150,6 +154,7 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_integrate_time_invariant_workspace_bytes", "xh_integrate_time_invariant",
             "xh_energy_integrator", "xh_fold", "xh_precomputation",
             "xh_marginal_workspace_bytes", "xh_eval_marginal_likelihood",
+            "xh_eval_marginal_likelihood_noalloc",
             "xh_pipeline_create", "xh_pipeline_destroy", "xh_pipeline_eval",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",

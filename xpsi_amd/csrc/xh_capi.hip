@@ -32,6 +32,7 @@ static int fail(int code, const std::string& msg)
     } while (0)
 
 extern "C" const char* xh_last_error(void) { return g_err.c_str(); }
+int xh_fail(int code, const std::string& msg) { return fail(code, msg); }
 
 // physical constants of xpsi/global_imports.py:67-88
 // Seeds of the llzero-class draws: a process-wide counter mixed with the
