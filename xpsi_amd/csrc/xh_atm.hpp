@@ -18,6 +18,18 @@ __device__ __forceinline__ int n4_base(const double* x, int N, double v)
     return m < 0 ? 0 : (m > N - 4 ? N - 4 : m);
 }
 
+// Cubic Lagrange weights on the four nodes b .. b+3 at q (extrapolating
+// when q lies outside them), as hot_Num4D.pyx:356-402 forms DIFF x SPACE.
+__device__ __forceinline__ void n4_weights(const double* x, int b, double q, double w[4])
+{
+    const double x0 = x[b], x1 = x[b + 1], x2 = x[b + 2], x3 = x[b + 3];
+    const double D0 = (q - x1) * (q - x2) * (q - x3), D1 = (q - x0) * (q - x2) * (q - x3);
+    const double D2 = (q - x0) * (q - x1) * (q - x3), D3 = (q - x0) * (q - x1) * (q - x2);
+    const double S0 = 1.0 / (x0 - x1) / (x0 - x2) / (x0 - x3), S1 = 1.0 / (x1 - x0) / (x1 - x2) / (x1 - x3);
+    const double S2 = 1.0 / (x2 - x0) / (x2 - x1) / (x2 - x3), S3 = 1.0 / (x3 - x0) / (x3 - x1) / (x3 - x2);
+    w[0] = D0 * S0; w[1] = D1 * S1; w[2] = D2 * S2; w[3] = D3 * S3;
+}
+
 // Four-dimensional cubic Lagrange interpolation of the atmosphere table
 // (hot_Num4D.pyx:248-438).  The reference keeps its base node from call to
 // call: a query outside the table is clamped to the edge only when the
@@ -50,12 +62,7 @@ __device__ inline double num4d_I(const FoldArgs& F, const double vec_in[4], cons
         } else b = n4_base(x, N, q);
         v[d] = q;
         bn[d] = b;
-        const double x0 = x[b], x1 = x[b + 1], x2 = x[b + 2], x3 = x[b + 3];
-        double D0 = (q - x1) * (q - x2) * (q - x3), D1 = (q - x0) * (q - x2) * (q - x3);
-        double D2 = (q - x0) * (q - x1) * (q - x3), D3 = (q - x0) * (q - x1) * (q - x2);
-        double S0 = 1.0 / (x0 - x1) / (x0 - x2) / (x0 - x3), S1 = 1.0 / (x1 - x0) / (x1 - x2) / (x1 - x3);
-        double S2 = 1.0 / (x2 - x0) / (x2 - x1) / (x2 - x3), S3 = 1.0 / (x3 - x0) / (x3 - x1) / (x3 - x2);
-        Lw[d][0] = D0 * S0; Lw[d][1] = D1 * S1; Lw[d][2] = D2 * S2; Lw[d][3] = D3 * S3;
+        n4_weights(x, b, q, Lw[d]);
     }
     const double* T = F.atm_I + (size_t)bn[0] * S[0] + (size_t)bn[1] * S[1] + (size_t)bn[2] * S[2] + bn[3];
     double I = 0.0;
@@ -74,6 +81,94 @@ __device__ inline double num4d_I(const FoldArgs& F, const double vec_in[4], cons
         }
     if (I < 0.0) return 0.0;
     return I * pow(10.0, 3.0 * v[0]);
+}
+
+// The table contracted over (log T, log g) for one parallel: slab[c][l] =
+// sum_ab w_a(log T) w_b(log g) I[bT + a][bg + b][c][l] over every mu node c
+// and energy node l, with the base nodes and weights of the parallel's own
+// log T and log g (extrapolating outside the table).  A query whose
+// predecessor is the same leaf's previous energy has these base nodes and no
+// clamp on the first three axes (hot_Num4D.pyx:300-351), so its value is a
+// bicubic over (mu, log10(E/kT)) of the slab.
+__device__ __forceinline__ void n4_ring_weights(const FoldArgs& F, double logT, double logg, int& b0, int& b1,
+                                                double w0[4], double w1[4])
+{
+    const double* ax0 = F.atm_axes;
+    const double* ax1 = ax0 + F.atm_N[0];
+    b0 = n4_base(ax0, F.atm_N[0], logT);
+    b1 = n4_base(ax1, F.atm_N[1], logg);
+    n4_weights(ax0, b0, logT, w0);
+    n4_weights(ax1, b1, logg, w1);
+}
+
+// Weights as n4_weights with the denominators read from a table (Sb = the
+// four of base node b).
+__device__ __forceinline__ void n4_weights_pre(const double* x, int b, double q, const double* Sb, double w[4])
+{
+    const double x0 = x[b], x1 = x[b + 1], x2 = x[b + 2], x3 = x[b + 3];
+    w[0] = (q - x1) * (q - x2) * (q - x3) * Sb[0];
+    w[1] = (q - x0) * (q - x2) * (q - x3) * Sb[1];
+    w[2] = (q - x0) * (q - x1) * (q - x3) * Sb[2];
+    w[3] = (q - x0) * (q - x1) * (q - x2) * Sb[3];
+}
+
+// interval index i of v (x_i <= v < x_{i+1}, the last interval closed),
+// moved up from a previous index at or below it (v only grows)
+__device__ __forceinline__ int interval_from(const double* x, int N, double v, int i)
+{
+    while (i < N - 2 && v >= x[i + 1]) i++;
+    return i;
+}
+
+// One leaf's run of consecutive energies e0 .. e0 + n - 1 (e0 >= 1) on the
+// parallel's slab: q_k = log10(E'_k / kT) increasing.  The energy axis clamps
+// as the reference's base-node rule does (a query beyond an edge is clamped
+// when its predecessor's base node was not already there).  out[k]: the value
+// times 10^(3 log T) (p10), clipped at zero.
+template <int RUN>
+__device__ __forceinline__ void n4_slab_run(const FoldArgs& F, const double* slab, double mu, const double* q,
+                                            double qp, int n, double p10, double* out)
+{
+    const double* x2 = F.atm_axes + F.atm_N[0] + F.atm_N[1];
+    const double* x3 = x2 + F.atm_N[2];
+    const int N2 = F.atm_N[2], N3 = F.atm_N[3];
+    const double* S2 = F.atm_S + 4 * ((F.atm_N[0] - 3) + (F.atm_N[1] - 3));
+    const double* S3 = S2 + 4 * (N2 - 3);
+    const int b2 = n4_base(x2, N2, mu);
+    double w2[4];
+    n4_weights_pre(x2, b2, mu, S2 + 4 * b2, w2);
+    const double* rows = slab + (size_t)b2 * N3;
+    // predecessor's base node, then this run's interval walk
+    int pb = n4_base(x3, N3, qp);
+    int iv = qp < x3[0] ? 0 : interval_of(x3, N3, qp);
+#pragma unroll
+    for (int k = 0; k < RUN; k++) {
+        if (k >= n) break;
+        double v = q[k];
+        int b;
+        if (v > x3[N3 - 1]) {
+            b = N3 - 4;
+            if (pb != N3 - 4) v = x3[N3 - 1];
+        } else if (v < x3[0]) {
+            b = 0;
+            if (pb != 0) v = x3[0];
+        } else {
+            iv = interval_from(x3, N3, q[k], iv);
+            b = iv - 1 < 0 ? 0 : (iv - 1 > N3 - 4 ? N3 - 4 : iv - 1);
+        }
+        double w3[4];
+        n4_weights_pre(x3, b, v, S3 + 4 * b, w3);
+        const double* c = rows + b;
+        double I = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const double* r = c + (size_t)m * N3;
+            I += w2[m] * (w3[0] * r[0] + w3[1] * r[1] + w3[2] * r[2] + w3[3] * r[3]);
+        }
+        out[k] = I < 0.0 ? 0.0 : I * p10;
+        // this query's own base node, for the next query's clamp rule
+        pb = q[k] > x3[N3 - 1] ? N3 - 4 : (q[k] < x3[0] ? 0 : b);
+    }
 }
 
 // the query made just before parallel i, image I (the last query of the

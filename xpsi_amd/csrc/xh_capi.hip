@@ -582,6 +582,8 @@ struct xh_pipeline {
     int *d_ring_ok, *d_valid, *d_member_index, *d_status, *d_lastq;
     int *d_raywork, *d_raycount, *d_else_raywork;   // rays left to the full QAG, and their counts
     double *d_atm_axes, *d_atm_I;
+    double* d_slab;              // table contracted over (log T, log g) per parallel (moment form)
+    double* d_atm_S;             // Lagrange denominators per axis and base node
     int atm_N[4];
     double *d_energies, *d_logE, *d_logEdges, *d_atten, *d_leaves, *d_phases, *d_comp_phases;
     double *d_integrated, *d_M, *d_folded, *d_data_phases, *d_counts, *d_precomp, *d_support, *d_shifts;
@@ -743,6 +745,26 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
             off += c->atm_N[d];
         }
         PHIP(hipMemcpy(p->d_atm_I, c->atm_I, ni * 8, hipMemcpyHostToDevice));
+        if (p->fold_kind == 2) PCHK(dalloc(p, &p->d_slab, nm * S * (size_t)c->atm_N[2] * c->atm_N[3]));
+        // 1 / prod_{m != k} (x_{b+k} - x_{b+m}) per axis and base node, in the
+        // order hot_Num4D.pyx:385-402 divides
+        std::vector<double> Sd;
+        for (int d = 0; d < 4; d++) {
+            const double* x = c->atm_axes[d];
+            for (int b = 0; b + 3 < c->atm_N[d]; b++)
+                for (int k = 0; k < 4; k++) {
+                    double v = 1.0;
+                    bool first = true;
+                    for (int mm = 0; mm < 4; mm++) {
+                        if (mm == k) continue;
+                        if (first) { v = 1.0 / (x[b + k] - x[b + mm]); first = false; }
+                        else v /= x[b + k] - x[b + mm];
+                    }
+                    Sd.push_back(v);
+                }
+        }
+        PCHK(dalloc(p, &p->d_atm_S, Sd.size()));
+        PHIP(hipMemcpy(p->d_atm_S, Sd.data(), Sd.size() * 8, hipMemcpyHostToDevice));
     }
     PCHK(dalloc(p, &p->d_flux, nm * NE * N_P));
     PCHK(dalloc(p, &p->d_member_index, ng * 2));
@@ -1066,8 +1088,15 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     else if (n >= 32 && p->fold_kind == 3) rb = std::min(S, 8);
     FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_effGrav, p->d_lastq,
                 p->d_atm_axes, p->d_atm_I, {p->atm_N[0], p->atm_N[1], p->atm_N[2], p->atm_N[3]}, p->d_energies,
-                p->d_phases, p->d_flux, rb};
+                p->d_phases, p->d_flux, rb, nullptr};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
+    if (p->d_slab && p->fold_kind == 2) {
+        hipLaunchKernelGGL(k_table_slab, dim3(8, S, nm), dim3(256), 0, s, (const Member*)p->d_members, L, FA,
+                           p->d_slab);
+        HIPCHK(hipGetLastError());
+        FA.slab = p->d_slab;
+        FA.atm_S = p->d_atm_S;
+    }
     if (p->fold_kind == 3)
         HIPCHK(launch_lane(p->fp32, p->lane_nq, lane_grid(N_E, N_P, p->lane_nq, (S + rb - 1) / rb, nm),
                            lane_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8), s, p->d_members, L, FA));

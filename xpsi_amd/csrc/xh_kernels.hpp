@@ -97,6 +97,13 @@ struct FoldArgs {
     const double* phases;     // N_P (radians)
     double* flux;             // [member][N_E][N_P]
     int ring_block;
+    // table atmosphere contracted over log T and log g per parallel,
+    // [member][S_max][N_mu][N_E'] (k_table_slab), or nullptr
+    const double* slab;
+    // per axis d and base node b (0 .. N_d - 4) the four Lagrange
+    // denominators 1 / prod_{m != k} (x_{b+k} - x_{b+m}), axes one after
+    // another; or nullptr (then formed at each query)
+    const double* atm_S;
 };
 
 // moment form: groups of T resident 16-energy chunks per block of shape T.
@@ -108,6 +115,8 @@ struct FoldArgs {
 __host__ __device__ constexpr int moment_groups(int T) { return T == 4 ? 2 : (T == 2 ? 8 : 4); }
 
 __global__ void k_geometry(Member* members, Layout L, RingArgs A);
+// the table atmosphere contracted over (log T, log g) for every parallel
+__global__ void k_table_slab(const Member* members, Layout L, FoldArgs F, double* slab);
 // closed mesh and time-invariant spectrum of the elsewhere component
 __global__ void k_closed_table(const Member* members, double* area_tab, double* total, int table_n);
 __global__ void k_closed_rings(const Member* members, const double* area_tab, const double* total,

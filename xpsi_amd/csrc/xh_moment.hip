@@ -300,6 +300,7 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
         }
         const double kT = K_B_OVER_KEV * pow(10.0, F.logT[ro]);
         const double kT_else = K_B_OVER_KEV * pow(10.0, m.else_logT);
+        const double p10 = ATM == 2 ? pow(10.0, 3.0 * F.logT[ro]) : 0.0;   // the table's 10^(3 log T)
         for (int I = 0; I < L.IO_max; I++) {
             const size_t vix = ro * L.IO_max + I;
             if (!F.valid[vix]) break;
@@ -402,6 +403,68 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                 for (int q = tid; q < n_i * G; q += NT) MIX[(q % G) * N_L + IL[q / G]] = 0u;
                 // profile: specific intensity x geometry, zero where the leaf is hidden
                 const double Ee = Eth[gi];
+                // the table atmosphere for a query with no predecessor on its own leaf
+                // (a leaf's first energy): the reference's base-node history in full
+                auto table_direct = [&](int mm, int e, double Ep) -> double {
+                    double vec[4], prev[4];
+                    int has_prev = 1;
+                    vec[0] = F.logT[ro]; vec[1] = F.logg[ro]; vec[2] = As[mm];
+                    vec[3] = log10(Ep / kT);
+                    if (e > 0) {
+                        prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = vec[2];
+                        prev[3] = log10((F.energies[e - 1] * Zi[mm]) / kT);
+                    } else if (PV[mm] >= 0) {
+                        int kd = PV[mm];
+                        prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = As[kd];
+                        prev[3] = log10((F.energies[N_E - 1] * Zi[kd]) / kT);
+                    } else if (ST[0] > 0.5) {
+                        prev[0] = ST[1]; prev[1] = ST[2]; prev[2] = ST[3]; prev[3] = ST[4];
+                    } else has_prev = 0;
+                    return num4d_I(F, vec, prev, has_prev);
+                };
+                if (ATM == 2) {
+                    // runs of eight consecutive energies of one leaf: a leaf's later
+                    // energies from the parallel's contracted table (their predecessor
+                    // query is the same leaf's previous energy, so log T, log g and mu
+                    // keep their base nodes and never clamp); a leaf's first energy
+                    // queries the table with the full history
+                    constexpr int RUN = 8;
+                    constexpr int NRUN = EG / RUN;
+                    const double* slab = (F.slab && F.atm_S) ? F.slab + ro * (size_t)F.atm_N[2] * F.atm_N[3] : nullptr;
+                    for (int q = tid; q < n_k * NRUN; q += NT) {
+                        const int ms = KL[q / NRUN], mm = (ms == N_L - 1 && copy_last) ? 0 : ms;
+                        const int k0 = (q % NRUN) * RUN;
+                        const double geom = Gs[mm], zi = Zi[mm];
+                        const bool vis = geom != 0.0 && PV[mm] >= -1;
+                        double Iv[RUN];
+                        int ks = k0;   // first energy of the run served by the slab
+                        if (vis && slab) {
+                            if (e0 + ks == 0) ks++;
+                            const int n = min(k0 + RUN, N_E - e0) - ks;
+                            if (n > 0) {
+                                double qv[RUN];
+#pragma unroll
+                                for (int k = 0; k < RUN; k++)
+                                    qv[k] = k < n ? log10((F.energies[e0 + ks + k] * zi) / kT) : 0.0;
+                                const double qp = log10((F.energies[e0 + ks - 1] * zi) / kT);
+                                n4_slab_run<RUN>(F, slab, As[mm], qv, qp, n, p10, Iv + (ks - k0));
+                            }
+                        }
+#pragma unroll
+                        for (int k = 0; k < RUN; k++) {
+                            const int ke = k0 + k, e = e0 + ke;
+                            R v = (R)0;
+                            if (vis && e < N_E) {
+                                const double Ep = F.energies[e] * zi;
+                                const double I = (slab && ke >= ks) ? Iv[k] : table_direct(mm, e, Ep);
+                                double corr = 0.0;
+                                if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
+                                v = (R)((I * NUM4D_NORM - corr) * geom);
+                            }
+                            CO[((ke >> 4) * N_L + ms) * 64 + (ke & 15)] = v;
+                        }
+                    }
+                } else {
                 for (int q = tid; q < n_k * EG; q += NT) {
                     const int ms = KL[q / EG], mm = (ms == N_L - 1 && copy_last) ? 0 : ms;
                     const int ke = q % EG, e = e0 + ke;
@@ -409,25 +472,7 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                     const double geom = Gs[mm];
                     if (geom != 0.0 && e < N_E && PV[mm] >= -1) {
                         const double Ep = Ee * Zi[mm];
-                        if (ATM == 2) {
-                            double vec[4], prev[4];
-                            int has_prev = 1;
-                            vec[0] = F.logT[ro]; vec[1] = F.logg[ro]; vec[2] = As[mm];
-                            vec[3] = log10(Ep / kT);
-                            if (e > 0) {
-                                prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = vec[2];
-                                prev[3] = log10((F.energies[e - 1] * Zi[mm]) / kT);
-                            } else if (PV[mm] >= 0) {
-                                int kd = PV[mm];
-                                prev[0] = vec[0]; prev[1] = vec[1]; prev[2] = As[kd];
-                                prev[3] = log10((F.energies[N_E - 1] * Zi[kd]) / kT);
-                            } else if (ST[0] > 0.5) {
-                                prev[0] = ST[1]; prev[1] = ST[2]; prev[2] = ST[3]; prev[3] = ST[4];
-                            } else has_prev = 0;
-                            double corr = 0.0;
-                            if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
-                            v = (R)((num4d_I(F, vec, prev, has_prev) * NUM4D_NORM - corr) * geom);
-                        } else if (sizeof(R) == 8) {
+                        if (sizeof(R) == 8) {
                             double I_E = Ep * Ep * Ep / (exp(Ep / kT) - 1.0);
                             double corr = 0.0;
                             if (m.correct) corr = Ep * Ep * Ep / (exp(Ep / kT_else) - 1.0) * BB_NORM;
@@ -445,6 +490,7 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                         }
                     }
                     CO[((ke >> 4) * N_L + ms) * 64 + (ke & 15)] = v;
+                }
                 }
                 __syncthreads();
                 // secant slopes
@@ -606,6 +652,34 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                 }
             }
         }
+    }
+}
+
+// The table contracted over (log T, log g) for every contributing parallel:
+// slab[member][ring][mu node][energy node], FP64.  Blocks (x) share out one
+// parallel's mu x energy plane; consecutive threads read consecutive energy
+// nodes of the 16 (log T, log g) rows.
+__global__ void __launch_bounds__(256) k_table_slab(const Member* __restrict__ members, Layout L, FoldArgs F,
+                                                    double* __restrict__ slab)
+{
+    const int i = blockIdx.y, mi = blockIdx.z;
+    const Member& m = members[mi];
+    if (!m.active || m.status || i >= m.S) return;
+    const size_t ro = (size_t)mi * L.S_max + i;
+    if (!F.valid[ro * L.IO_max]) return;   // the parallel folds nothing
+    int b0, b1;
+    double w0[4], w1[4];
+    n4_ring_weights(F, F.logT[ro], F.logg[ro], b0, b1, w0, w1);
+    const size_t s1 = (size_t)F.atm_N[2] * F.atm_N[3], s0 = (size_t)F.atm_N[1] * s1;
+    const double* T = F.atm_I + (size_t)b0 * s0 + (size_t)b1 * s1;
+    double* out = slab + ro * s1;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < s1; k += (size_t)gridDim.x * blockDim.x) {
+        double c = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) c += w0[a] * w1[b] * T[(size_t)a * s0 + (size_t)b * s1 + k];
+        out[k] = c;
     }
 }
 
