@@ -86,6 +86,57 @@ int xh_integrate_time_invariant(double omega, double r_s, double inclination,
                                 size_t workspace_bytes, int* status_out,
                                 void* stream);
 
+/* Replaces xpsi/cellmesh/rays.pyx:249-390 `compute_rays(N_T, N_u, r_s,
+ * r_s_over_R[N_u], maxAlpha[N_u], N_R) -> (terminate, deflection, cos_alpha,
+ * lag, maxDeflection)`: per u = r_s/R, N_R rays with cos(alpha) uniform from 1
+ * to cos(maxAlpha); deflection and (uncalibrated) lag by adaptive
+ * Gauss-Kronrod quadrature to 1e-12 relative.  maxAlpha (device) is clamped
+ * in place as the reference does; bad rays are repaired linearly in the
+ * reference's order.  deflection, cos_alpha, lag: [N_u, N_R]; maxDeflection
+ * [N_u] (device).  *terminate_out (host) = 1 when a parallel's last ray
+ * fails (the reference's ERROR).  workspace of xh_compute_rays_workspace_bytes(). */
+size_t xh_compute_rays_workspace_bytes(int N_u, int N_R);
+int xh_compute_rays(int N_u, double r_s, const double* r_s_over_R,
+                    double* maxAlpha, int N_R, double* deflection,
+                    double* cos_alpha, double* lag, double* maxDeflection,
+                    void* workspace, size_t workspace_bytes,
+                    int* terminate_out, void* stream);
+
+/* Replaces xpsi/cellmesh/mesh_tools.pyx:892-1099 `allocate_cells(...) ->
+ * (super_sqrt_num_cells, super_num_cells, cede_sqrt_num_cells,
+ * cede_num_cells)`, the four values written to out4 (host); same argument
+ * order.  workspace of xh_allocate_cells_workspace_bytes(). */
+size_t xh_allocate_cells_workspace_bytes(void);
+int xh_allocate_cells(int num_cells, int min_sqrt_num_cells,
+                      int max_sqrt_num_cells, double mass, double r_s,
+                      double R_eq, double Omega, double zeta, double epsilon,
+                      int star_shape_ind, double superRadius,
+                      double cedeRadius, double superColatitude,
+                      double cedeColatitude, double superAzimuth,
+                      double holeColatitude, double holeRadius,
+                      double holeAzimuth, double* out4, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* Replaces xpsi/cellmesh/mesh.pyx:23-417 construct_spot_cellMesh (polar = 0)
+ * and xpsi/cellmesh/polar_mesh.pyx:23-430 construct_polar_cellMesh (polar =
+ * 1): (theta, phi, r, cellAreas, maxEmissionAngle, cos_gamma, effGrav) for
+ * the region of cedeRadius at cedeColatitude less the superseding region.
+ * Outputs (device): theta, phi, cellArea [S, S]; r, maxEmissionAngle,
+ * cos_gamma, effGrav [S].  *status_out (host) 0 on success.
+ * workspace of xh_cell_mesh_workspace_bytes(S). */
+size_t xh_cell_mesh_workspace_bytes(int S);
+int xh_construct_cell_mesh(int polar, double numCell, int S, double mass,
+                           double r_s, double R_eq, double zeta,
+                           double epsilon, int star_shape_ind,
+                           double cedeRadius, double cedeColatitude,
+                           double superRadius, double superColatitude,
+                           double superAzimuth, double* theta, double* phi,
+                           double* r, double* cellArea,
+                           double* maxEmissionAngle, double* cos_gamma,
+                           double* effGrav, void* workspace,
+                           size_t workspace_bytes, int* status_out,
+                           void* stream);
+
 /* Replaces xpsi/tools/energy_integrator.pyx:27-30,114
  * `energy_integrator(N_Ts, signal[N_E, N_P], log10_E[N_E], log10_edges[N_in+1])`
  * -> out [N_in, N_P]. */
@@ -123,6 +174,29 @@ int xh_eval_marginal_likelihood(double exposure_time, int N_b,
                                 double* lnl_out, double* STAR, double* B_ML,
                                 double* B_ML_supp, void* workspace,
                                 size_t workspace_bytes, void* stream);
+
+/* Replaces xpsi/likelihoods/_poisson_likelihood_given_background.pyx:14-113
+ * `poisson_likelihood_given_background(...) -> (lnL, EXPEC)`: the Poisson
+ * likelihood of the counts given the star's signal plus a known background.
+ * components [n_comp, N_ch, N_P] on one phase grid comp_phases [N_P]
+ * (cycles); phases [N_b+1]; counts [N_ch, N_b]; shifts [n_comp];
+ * background [N_ch, N_b] count rates (required); precomp [N_ch] (the
+ * -sum ln n! of xh_precomputation) or NULL.  EXPEC [N_ch, N_b] (device) gets
+ * (star + background) x exposure_time; lnl_out is a host pointer.  A bin with
+ * data but no expected counts adds -1e90 (0.1 + 0.9 U), U uniform, fresh per
+ * call.  workspace of xh_poisson_workspace_bytes(). */
+size_t xh_poisson_workspace_bytes(int N_ch);
+int xh_poisson_likelihood_given_background(double exposure_time, int N_b,
+                                           const double* phases, int N_ch,
+                                           const double* counts, int n_comp,
+                                           const double* components, int N_P,
+                                           const double* comp_phases,
+                                           const double* shifts,
+                                           const double* background,
+                                           const double* precomp,
+                                           int allow_negative, double* lnl_out,
+                                           double* EXPEC, void* workspace,
+                                           size_t workspace_bytes, void* stream);
 
 /* No counterpart in the reference: xh_eval_marginal_likelihood without the
  * workspace argument, for callers that cannot allocate device memory (code

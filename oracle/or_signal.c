@@ -178,6 +178,44 @@ static int channel_marginal(margs *a, double precomp, double *LL,
     return 1;
 }
 
+/* compute_expected_counts.pyx:66-197: channel i's expected count rate in
+   each phase bin (shifted, wrapped, negative bin integrals dropped unless
+   allowed, then clipped at zero), added into st[N_b] (zeroed by the caller);
+   cb [3 (max N_P + 1)] and mb [max N_P + 4] are scratch */
+static void channel_rate(size_t i, size_t N_b, const double *phases, size_t n_comp,
+                         const double *const *comps, const size_t *comp_N_P,
+                         const double *const *comp_phases, const double *shifts,
+                         const unsigned char *allow_neg, double *cb, double *mb,
+                         double *st)
+{
+    for (size_t c = 0; c < n_comp; c++) {
+        size_t NP = comp_N_P[c];
+        const double *pulse = comps[c] + i * NP;
+        if (NP > 1) {
+            const double *ph = comp_phases[c];
+            or_akima_init(ph, pulse, NP, 1, cb, cb + NP, cb + 2 * NP, mb);
+            for (size_t j = 0; j < N_b; j++) {
+                double pa = phases[j] + shifts[c], pb = phases[j + 1] + shifts[c];
+                if (eq12(pb - pa, 1.0)) { pa = 0.0; pb = 1.0; }
+                else { pa -= floor(pa); pb -= floor(pb); }
+                double v;
+                if (pa < pb) {
+                    v = or_akima_integ(ph, pulse, NP, cb, cb + NP, cb + 2 * NP, pa, pb);
+                    if (v > 0.0 || allow_neg[c]) st[j] += v;
+                } else {
+                    v = or_akima_integ(ph, pulse, NP, cb, cb + NP, cb + 2 * NP, pa, 1.0);
+                    if (v > 0.0 || allow_neg[c]) st[j] += v;
+                    v = or_akima_integ(ph, pulse, NP, cb, cb + NP, cb + 2 * NP, 0.0, pb);
+                    if (v > 0.0 || allow_neg[c]) st[j] += v;
+                }
+            }
+        } else {
+            st[0] = pulse[0];
+        }
+    }
+    for (size_t j = 0; j < N_b; j++) if (st[j] < 0.0) st[j] = 0.0;
+}
+
 int or_eval_marginal_likelihood(double T_exp, size_t N_b, const double *phases,
                                 size_t N_ch, const double *counts,
                                 size_t n_comp, const double *const *comps,
@@ -203,33 +241,7 @@ int or_eval_marginal_likelihood(double T_exp, size_t N_b, const double *phases,
     a.n = N_b; a.SCALE = SCALE; a.T_exp = T_exp;
     for (size_t i = 0; i < N_ch; i++) {
         double *st = STAR + i * N_b;
-        /* compute_expected_counts.pyx:66-197 */
-        for (size_t c = 0; c < n_comp; c++) {
-            size_t NP = comp_N_P[c];
-            const double *pulse = comps[c] + i * NP;
-            if (NP > 1) {
-                const double *ph = comp_phases[c];
-                or_akima_init(ph, pulse, NP, 1, cb, cb + NP, cb + 2 * NP, mb);
-                for (size_t j = 0; j < N_b; j++) {
-                    double pa = phases[j] + shifts[c], pb = phases[j + 1] + shifts[c];
-                    if (eq12(pb - pa, 1.0)) { pa = 0.0; pb = 1.0; }
-                    else { pa -= floor(pa); pb -= floor(pb); }
-                    double v;
-                    if (pa < pb) {
-                        v = or_akima_integ(ph, pulse, NP, cb, cb + NP, cb + 2 * NP, pa, pb);
-                        if (v > 0.0 || allow_neg[c]) st[j] += v;
-                    } else {
-                        v = or_akima_integ(ph, pulse, NP, cb, cb + NP, cb + 2 * NP, pa, 1.0);
-                        if (v > 0.0 || allow_neg[c]) st[j] += v;
-                        v = or_akima_integ(ph, pulse, NP, cb, cb + NP, cb + 2 * NP, 0.0, pb);
-                        if (v > 0.0 || allow_neg[c]) st[j] += v;
-                    }
-                }
-            } else {
-                st[0] = pulse[0];
-            }
-        }
-        for (size_t j = 0; j < N_b; j++) if (st[j] < 0.0) st[j] = 0.0;
+        channel_rate(i, N_b, phases, n_comp, comps, comp_N_P, comp_phases, shifts, allow_neg, cb, mb, st);
         double av_D = 0.0, av_S = 0.0;
         for (size_t j = 0; j < N_b; j++) {
             if (background) st[j] += background[i * N_b + j];
@@ -249,6 +261,52 @@ int or_eval_marginal_likelihood(double T_exp, size_t N_b, const double *phases,
                               epsilon, sigmas, llzero, support + 2 * i,
                               B_ML + i, B_ML_supp + i))
             break;
+    }
+    free(cb); free(mb);
+    *lnl = LL;
+    return 0;
+}
+
+/* xpsi/likelihoods/_poisson_likelihood_given_background.pyx:14-113 with
+   compute_expected_counts.pyx:199-315: EXPEC = (star rate + background rate)
+   x exposure per channel and bin; lnL = sum over channels of
+   neg_sum_ln_data_factorial (when given) plus, per bin, d log EXPEC - EXPEC;
+   a bin with no expected counts but data adds -1e90 (0.1 + 0.9 U). */
+int or_poisson_likelihood_given_background(double T_exp, size_t N_b, const double *phases,
+                                           size_t N_ch, const double *counts, size_t n_comp,
+                                           const double *const *comps, const size_t *comp_N_P,
+                                           const double *const *comp_phases,
+                                           const double *shifts, const double *background,
+                                           const double *precomp,
+                                           const unsigned char *allow_neg,
+                                           double *lnl, double *EXPEC)
+{
+    size_t maxP = 0;
+    for (size_t c = 0; c < n_comp; c++) if (comp_N_P[c] > maxP) maxP = comp_N_P[c];
+    double *cb = (double *)malloc(3 * (maxP + 1) * sizeof(double));
+    double *mb = (double *)malloc((maxP + 4) * sizeof(double));
+    memset(EXPEC, 0, N_ch * N_b * sizeof(double));
+    double LL = 0.0;
+    for (size_t i = 0; i < N_ch; i++) {
+        double *ex = EXPEC + i * N_b;
+        channel_rate(i, N_b, phases, n_comp, comps, comp_N_P, comp_phases, shifts, allow_neg, cb, mb, ex);
+        for (size_t j = 0; j < N_b; j++) {
+            ex[j] += background[i * N_b + j];
+            ex[j] *= T_exp;
+        }
+    }
+    for (size_t i = 0; i < N_ch; i++) {
+        if (precomp) LL += precomp[i];
+        for (size_t j = 0; j < N_b; j++) {
+            double e = EXPEC[i * N_b + j], d = counts[i * N_b + j];
+            if (e > 0.0) {
+                LL -= e;
+                LL += d * log(e);
+            } else if (d == 0.0 && e == 0.0) {
+            } else {
+                LL += -1.0e90 * (0.1 + 0.9 * unit_draw());
+            }
+        }
     }
     free(cb); free(mb);
     *lnl = LL;

@@ -105,6 +105,10 @@ def lib():
             [d, sz, _dp, sz, _dp, sz, C.c_void_p, C.c_void_p, C.c_void_p, _dp,
              _dp, _dp, d, d, d, d, d, C.c_void_p, d, C.c_void_p,
              C.POINTER(C.c_double), _dp, _dp, _dp])
+        L.or_poisson_likelihood_given_background.restype = i
+        L.or_poisson_likelihood_given_background.argtypes = (
+            [d, sz, _dp, sz, _dp, sz, C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp,
+             C.c_void_p, C.c_void_p, C.POINTER(C.c_double), _dp])
         _LIB = L
     return _LIB
 
@@ -320,6 +324,31 @@ def eval_marginal_likelihood(exposure_time, phases, counts, components,
         epsilon, sigmas, llzero, an.ctypes.data, slim,
         None if bg is None else bg.ctypes.data, C.byref(lnl), STAR, bml, bmls)
     return lnl.value, STAR, bml, bmls
+
+
+def poisson_likelihood_given_background(exposure_time, phases, counts, components,
+                                        component_phases, phase_shifts, background,
+                                        neg_sum_ln_data_factorial=None, allow_negative=False):
+    """_poisson_likelihood_given_background.pyx:14-113: (lnL, EXPEC)."""
+    comps = [_c(cmp) for cmp in components]
+    cph = [_c(p) for p in component_phases]
+    nc = len(comps)
+    N_ch, N_b = counts.shape
+    cp = (C.c_void_p * nc)(*[cmp.ctypes.data for cmp in comps])
+    pp = (C.c_void_p * nc)(*[p.ctypes.data for p in cph])
+    nps = (C.c_size_t * nc)(*[cmp.shape[1] for cmp in comps])
+    if isinstance(allow_negative, bool):
+        an = np.full(nc, int(allow_negative), dtype=np.uint8)
+    else:
+        an = np.asarray(allow_negative, dtype=np.uint8)
+    pre = None if neg_sum_ln_data_factorial is None else _c(neg_sum_ln_data_factorial)
+    EXPEC = np.zeros((N_ch, N_b))
+    lnl = C.c_double(0.0)
+    lib().or_poisson_likelihood_given_background(
+        float(exposure_time), N_b, _c(phases), N_ch, _c(counts), nc, cp, nps, pp,
+        _c(phase_shifts), _c(background), None if pre is None else pre.ctypes.data,
+        an.ctypes.data, C.byref(lnl), EXPEC)
+    return lnl.value, EXPEC
 
 
 # ---------------------------------------------------------- model pipeline

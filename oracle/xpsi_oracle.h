@@ -135,4 +135,17 @@ int or_eval_marginal_likelihood(double exposure_time, size_t N_b,
                                 double slim, const double *background,
                                 double *lnl, double *STAR, double *B_ML,
                                 double *B_ML_supp);
+/* _poisson_likelihood_given_background.pyx: lnL and EXPEC [N_ch, N_b];
+   background [N_ch, N_b] count rates; precomp [N_ch] or NULL */
+int or_poisson_likelihood_given_background(double exposure_time, size_t N_b,
+                                           const double *phases, size_t N_ch,
+                                           const double *counts, size_t n_comp,
+                                           const double *const *components,
+                                           const size_t *comp_N_P,
+                                           const double *const *component_phases,
+                                           const double *phase_shifts,
+                                           const double *background,
+                                           const double *precomp,
+                                           const unsigned char *allow_negative,
+                                           double *lnl, double *EXPEC);
 #endif

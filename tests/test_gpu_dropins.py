@@ -1,0 +1,101 @@
+"""GPU: the ray and mesh drop-ins against the oracle (signals to 1e-10
+element by element): compute_rays (rays.pyx:249-390, maxAlpha clamped in
+place), allocate_cells (mesh_tools.pyx:892-1099) and the spot and polar cell
+meshes (mesh.pyx:23-417, polar_mesh.pyx:23-430), with ceding rings (the
+superseding region cut out) and an omitted hole."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.bars import SIG_RTOL, elem_relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print("[gpu] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def star():
+    return O.spacetime(1.4, 12.0, 1.0, 0.5, 300.0, "AGM_14")
+
+
+CASES = [
+    # (name, super radius, cede radius, super colat, cede colat, super azimuth, hole colat, hole radius, hole azimuth)
+    ("spot", 0.4, 0.0, 0.8, 0.8, 0.0, 0.8, 0.0, 0.0),
+    ("spot-cede", 0.15, 0.35, 0.9, 0.9, 0.0, 0.9, 0.0, 0.0),
+    ("spot-cede-offset", 0.12, 0.4, 0.95, 1.0, -0.08, 0.95, 0.0, 0.0),
+    ("polar-cede", 0.1, 0.35, 0.15, 0.2, 0.0, 0.15, 0.0, 0.0),
+    ("south-polar", 0.3, 0.0, math.pi - 0.1, math.pi - 0.1, 0.0, math.pi - 0.1, 0.0, 0.0),
+    ("spot-hole", 0.5, 0.0, 1.2, 1.2, 0.0, 1.25, 0.15, 0.1),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_allocate_and_mesh_dropins(star, case):
+    from xpsi_amd.cellmesh.mesh import construct_polar_cellMesh, construct_spot_cellMesh
+    from xpsi_amd.cellmesh.mesh_tools import allocate_cells
+    name, sr, cr, sc, cc, sa, hc, hr, ha = case
+    st = star
+    args = (32 * 32, 10, 80, st["M"], st["r_s"], st["R"], st["Omega"], st["zeta"], st["epsilon"])
+    ref = O.allocate_cells(*args, st["shape"], sr, cr, sc, cc, sa, hc, hr, ha)
+    got = allocate_cells(*args, st["shape"], sr, cr, sc, cc, sa, hc, hr, ha)
+    say("allocate_cells %s: gpu %s oracle %s" % (name, got, ref))
+    assert tuple(got) == tuple(ref)
+    members = [(got[0], got[1], sr, sc, hr, hc, ha)]
+    if cr > 0.0:
+        members.append((got[2], got[3], cr, cc, sr, sc, sa))
+    for S, ncell, radm, colm, rcut, ccut, acut in members:
+        polar = (colm - radm < 0.0) or (colm + radm > math.pi)
+        fn = construct_polar_cellMesh if polar else construct_spot_cellMesh
+        say("mesh %s: %s S=%d" % (name, "polar" if polar else "spot", int(S)))
+        out = fn(1, ncell, int(S), 1.4, st["r_s"], st["R"], st["zeta"], st["epsilon"], st["shape"], radm,
+                 colm, rcut, ccut, acut)
+        ref_m = O.construct_cell_mesh(polar, ncell, int(S), 1.4, st["r_s"], st["R"], st["zeta"],
+                                      st["epsilon"], st["shape"], radm, colm, rcut, ccut, acut)
+        for label, a, b in zip(("theta", "phi", "r", "cellArea", "maxAlpha", "cos_gamma", "effGrav"), out,
+                               ref_m):
+            e = elem_relerr(a, b)
+            say("  %s rel err %.3e" % (label, e))
+            assert a.shape == b.shape and e < SIG_RTOL, label
+
+
+def test_compute_rays_dropin(star):
+    """Parallels at u 0.25-0.45 with emission bounds from pi/2 up (as the
+    mesh gives them), and ultra-compact ones (u 0.66-0.7) whose bound lies
+    beyond the photon orbit's and is clamped in place.  Rays of a clamped
+    parallel end within 1e-8 of the photon orbit, where the deflection
+    integrand is nearly singular and adaptive quadrature at epsrel 1e-12
+    settles only to about 1e-9 (in either implementation): their last two rays
+    are held to 1e-8, every other element to 1e-10."""
+    from xpsi_amd.cellmesh.rays import compute_rays
+    st = star
+    rng = np.random.default_rng(9)
+    u = np.concatenate([np.sort(rng.uniform(0.25, 0.45, 36)), [0.66, 0.67, 0.68, 0.7]])
+    maxA = np.concatenate([0.5 * math.pi + rng.uniform(0.0, 0.25, 36), np.full(4, 1.7)])
+    mg, mr = maxA.copy(), maxA.copy()
+    say("compute_rays: 40 parallels x 150 rays")
+    t, d, ca, lg, md = compute_rays(1, 40, st["r_s"], u, mg, 150)
+    tr, dr, car, lgr, mdr = O.compute_rays(st["r_s"], u, mr, 150)
+    assert t == tr == 0
+    clamped = mr != maxA
+    assert clamped[36:].all() and not clamped[:36].any()
+    assert elem_relerr(mg, mr) < 1e-15
+    edge = np.zeros(d.shape, dtype=bool)
+    edge[clamped, -2:] = True
+    for label, a, b in (("deflection", d, dr), ("lag", lg, lgr), ("cos_alpha", ca, car)):
+        e_in = elem_relerr(np.where(edge, 0.0, a), np.where(edge, 0.0, b))
+        e_edge = float(np.max(np.abs(a[edge] / b[edge] - 1.0)))
+        k = np.unravel_index(np.argmax(np.where(edge, 0.0, np.abs(a - b) / np.maximum(np.abs(b), 1e-300))), a.shape)
+        say("  %s rel err %.3e (worst at parallel %d ray %d), last rays of clamped parallels %.3e"
+            % (label, e_in, k[0], k[1], e_edge))
+        assert e_in < SIG_RTOL, label
+        assert e_edge < 1e-8, label
+    e = elem_relerr(md[~clamped], mdr[~clamped])
+    say("  maxDeflection rel err %.3e" % e)
+    assert e < SIG_RTOL

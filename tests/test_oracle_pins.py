@@ -290,3 +290,60 @@ def test_marginal_likelihood_brute_force(bounded):
     lnl0, *_ = O.eval_marginal_likelihood(T, edges, counts, [pulses], [ph], np.array([0.0]),
                                           pre, support)
     assert abs(lnl0 - ref) > 1e-3
+
+
+def test_poisson_likelihood_given_background_brute_force():
+    # expected counts = exposure x (phase-bin integral of the shifted pulse +
+    # background rate); lnL = sum d log(lambda) - lambda - ln d!
+    from scipy import integrate as si
+    from scipy.special import gammaln
+    rng = np.random.default_rng(31)
+    T, N_b, N_ch, NP = 2000.0, 10, 4, 40
+    ph = np.linspace(0.0, 1.0, NP)
+    amp = np.array([0.03, 0.01, 0.05, 0.02])
+    pulses = amp[:, None] * (1.0 + 0.7 * np.cos(2.0 * math.pi * (ph - 0.2)))[None, :]
+    second = 0.5 * amp[:, None] * (1.0 + 0.4 * np.sin(4.0 * math.pi * ph))[None, :]
+    shifts = np.array([0.15, -0.4])
+    edges = np.linspace(0.0, 1.0, N_b + 1)
+    star = np.zeros((N_ch, N_b))
+    for comp, sh in ((pulses, shifts[0]), (second, shifts[1])):
+        for i in range(N_ch):
+            sp = O.akima(ph, comp[i], True)
+            f = lambda x: O.akima_eval(sp, x - math.floor(x))
+            for j in range(N_b):
+                a, b = edges[j] + sh, edges[j + 1] + sh
+                # 4-point Gauss-Legendre on every cubic piece (exact) between the
+                # knots, shifted into the integration range
+                knots = np.concatenate([ph + k for k in range(math.floor(a), math.floor(b) + 1)])
+                cuts = np.unique(np.concatenate([[a, b], knots[(knots > a) & (knots < b)]]))
+                gx, gw = np.polynomial.legendre.leggauss(4)
+                tot = 0.0
+                for lo, hi in zip(cuts[:-1], cuts[1:]):
+                    xs = 0.5 * (hi - lo) * gx + 0.5 * (hi + lo)
+                    tot += 0.5 * (hi - lo) * sum(w * f(x) for x, w in zip(xs, gw))
+                star[i, j] += tot
+    bg = rng.uniform(0.001, 0.004, (N_ch, N_b))
+    lam = T * (star + bg)
+    counts = rng.poisson(lam).astype(np.float64)
+    pre = O.precomputation(counts.astype(np.int32))
+    lnl, EXPEC = O.poisson_likelihood_given_background(T, edges, counts, [pulses, second], [ph, ph],
+                                                       shifts, bg, pre)
+    np.testing.assert_allclose(EXPEC, lam, rtol=1e-11)
+    ref = float(np.sum(counts * np.log(lam) - lam - gammaln(counts + 1.0)))
+    assert lnl == pytest.approx(ref, abs=1e-8)
+    # without the factorial term
+    lnl0, _ = O.poisson_likelihood_given_background(T, edges, counts, [pulses, second], [ph, ph],
+                                                    shifts, bg)
+    assert lnl0 == pytest.approx(ref + float(np.sum(gammaln(counts + 1.0))), abs=1e-8)
+    # a bin with data but no expected counts: a draw in [-1e90, -1e89]
+    zero = np.zeros_like(pulses)
+    c2 = np.zeros_like(counts)
+    c2[0, 0] = 3.0
+    lz, _ = O.poisson_likelihood_given_background(T, edges, c2, [zero], [ph], shifts[:1],
+                                                  np.zeros_like(bg))
+    assert -1e90 <= lz <= -1e89
+    # every such bin adds its own draw
+    c2[1, 3] = 1.0
+    lz2, _ = O.poisson_likelihood_given_background(T, edges, c2, [zero], [ph], shifts[:1],
+                                                   np.zeros_like(bg))
+    assert -2e90 <= lz2 <= -2e89

@@ -85,6 +85,24 @@ def lib():
     L.xh_eval_marginal_likelihood.argtypes = (
         [d, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, d, d, d, d, d, i32, d,
          vp, C.POINTER(d), vp, vp, vp, vp, sz, vp])
+    L.xh_compute_rays_workspace_bytes.restype = sz
+    L.xh_compute_rays_workspace_bytes.argtypes = [i32, i32]
+    L.xh_compute_rays.restype = i32
+    L.xh_compute_rays.argtypes = [i32, d, vp, vp, i32, vp, vp, vp, vp, vp, sz, C.POINTER(i32), vp]
+    L.xh_allocate_cells_workspace_bytes.restype = sz
+    L.xh_allocate_cells_workspace_bytes.argtypes = []
+    L.xh_allocate_cells.restype = i32
+    L.xh_allocate_cells.argtypes = [i32, i32, i32] + [d] * 6 + [i32] + [d] * 8 + [vp, vp, sz, vp]
+    L.xh_cell_mesh_workspace_bytes.restype = sz
+    L.xh_cell_mesh_workspace_bytes.argtypes = [i32]
+    L.xh_construct_cell_mesh.restype = i32
+    L.xh_construct_cell_mesh.argtypes = ([i32, d, i32] + [d] * 5 + [i32] + [d] * 5 + [vp] * 7 +
+                                         [vp, sz, C.POINTER(i32), vp])
+    L.xh_poisson_workspace_bytes.restype = sz
+    L.xh_poisson_workspace_bytes.argtypes = [i32]
+    L.xh_poisson_likelihood_given_background.restype = i32
+    L.xh_poisson_likelihood_given_background.argtypes = (
+        [d, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, i32, C.POINTER(d), vp, vp, sz, vp])
     L.xh_eval_marginal_likelihood_noalloc.restype = i32
     L.xh_eval_marginal_likelihood_noalloc.argtypes = (
         [d, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, d, d, d, d, d, i32, d,
@@ -155,6 +173,9 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_energy_integrator", "xh_fold", "xh_precomputation",
             "xh_marginal_workspace_bytes", "xh_eval_marginal_likelihood",
             "xh_eval_marginal_likelihood_noalloc",
+            "xh_poisson_workspace_bytes", "xh_poisson_likelihood_given_background",
+            "xh_compute_rays_workspace_bytes", "xh_compute_rays", "xh_allocate_cells_workspace_bytes",
+            "xh_allocate_cells", "xh_cell_mesh_workspace_bytes", "xh_construct_cell_mesh",
             "xh_pipeline_create", "xh_pipeline_destroy", "xh_pipeline_eval",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",

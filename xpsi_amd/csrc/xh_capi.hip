@@ -50,6 +50,7 @@ static const double kC = 2.99792458e8, kKPC = 3.08567758e19, kGM = 0.5 * 2.95325
                     kG = 6.6740831e-11, kKM = 1.0e3;
 
 static int g_fold_ec = 0;   // 0: choose from the leaf count
+static bool eq12_host(double x, double y) { return std::fabs(x - y) < 1.0e-12; }
 
 static int pick_ec(int N_L)
 {
@@ -485,6 +486,143 @@ extern "C" int xh_integrate_time_invariant(double omega, double r_s, double incl
     return XH_OK;
 }
 
+extern "C" size_t xh_compute_rays_workspace_bytes(int N_u, int N_R)
+{
+    return 256 + (size_t)N_u * 8 + (size_t)N_u * N_R * 4 + 64 + 256;
+}
+
+extern "C" int xh_compute_rays(int N_u, double r_s, const double* r_s_over_R, double* maxAlpha, int N_R,
+                               double* deflection, double* cos_alpha, double* lag, double* maxDeflection,
+                               void* workspace, size_t workspace_bytes, int* terminate_out, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (N_u < 1 || N_R < 3) return fail(XH_EINVAL, "xh_compute_rays: bad sizes");
+    if (workspace_bytes < xh_compute_rays_workspace_bytes(N_u, N_R))
+        return fail(XH_EINVAL, "xh_compute_rays: workspace too small");
+    char* w = (char*)workspace;
+    Member* d_m = (Member*)w; w += 256;
+    double* d_rr = (double*)w; w += (size_t)N_u * 8;
+    int* d_work = (int*)w; w += (size_t)N_u * N_R * 4;
+    w = (char*)(((uintptr_t)w + 63) & ~(uintptr_t)63);
+    int* d_count = (int*)w;
+    Member hm{};
+    hm.r_s = r_s; hm.active = 1; hm.S = N_u;
+    HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
+    // radial coordinates r = r_s / u, as the kernels take them
+    hipLaunchKernelGGL(k_over, dim3((N_u + 255) / 256), dim3(256), 0, s, d_rr, r_s_over_R, (size_t)N_u, r_s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(fill_int(d_count, 2, 0, s));
+    Layout L{N_u, N_R, 0, 0, 0, 8, 0, 1};
+    hipLaunchKernelGGL(k_rays_lane, dim3((N_R - 1 + 63) / 64, N_u, 1), dim3(64), 0, s, (const Member*)d_m, L,
+                       (const double*)d_rr, (const double*)maxAlpha, cos_alpha, deflection, lag, d_work, d_count,
+                       r_s_over_R);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_rays, dim3(kRayWaveBlocks), dim3(256), 0, s, (const Member*)d_m, L, (const double*)d_rr,
+                       (const double*)maxAlpha, deflection, lag, (const int*)d_work, (const int*)d_count,
+                       r_s_over_R);
+    HIPCHK(hipGetLastError());
+    // bad rays repaired in the reference's order, maxAlpha clamped in place
+    // (rays.pyx:310-369); the lag is returned uncalibrated
+    hipLaunchKernelGGL(k_rays_fix, dim3(1), dim3(256), 0, s, d_m, L, (const double*)d_rr, maxAlpha, deflection, lag,
+                       maxDeflection, 0, r_s_over_R);
+    HIPCHK(hipGetLastError());
+    Member back{};
+    HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *terminate_out = back.status == 1 ? 1 : 0;
+    return XH_OK;
+}
+
+extern "C" size_t xh_allocate_cells_workspace_bytes(void) { return 256 + 2 * 256 + 256; }
+
+extern "C" int xh_allocate_cells(int num_cells, int min_sqrt_num_cells, int max_sqrt_num_cells, double mass,
+                                 double r_s, double R_eq, double Omega, double zeta, double epsilon,
+                                 int star_shape_ind, double superRadius, double cedeRadius,
+                                 double superColatitude, double cedeColatitude, double superAzimuth,
+                                 double holeColatitude, double holeRadius, double holeAzimuth, double* out4,
+                                 void* workspace, size_t workspace_bytes, void* stream)
+{
+    (void)mass; (void)r_s;
+    hipStream_t s = (hipStream_t)stream;
+    if (num_cells < 1 || min_sqrt_num_cells < 1 || max_sqrt_num_cells < min_sqrt_num_cells)
+        return fail(XH_EINVAL, "xh_allocate_cells: bad cell counts");
+    if (workspace_bytes < xh_allocate_cells_workspace_bytes())
+        return fail(XH_EINVAL, "xh_allocate_cells: workspace too small");
+    char* w = (char*)workspace;
+    RegionAlloc* d_r = (RegionAlloc*)w; w += 256;
+    Member* d_m = (Member*)w;
+    RegionAlloc hr{};
+    hr.superR = superRadius; hr.cedeR = cedeRadius; hr.superColat = superColatitude; hr.cedeColat = cedeColatitude;
+    hr.superAzi = superAzimuth; hr.holeColat = holeColatitude; hr.holeR = holeRadius; hr.holeAzi = holeAzimuth;
+    hr.R_eq = R_eq; hr.Omega = Omega; hr.zeta = zeta; hr.epsilon = epsilon; hr.shape = star_shape_ind;
+    hr.num_cells = num_cells; hr.min_sqrt = min_sqrt_num_cells; hr.max_sqrt = max_sqrt_num_cells;
+    hr.member0 = 0; hr.member1 = eq12_host(cedeRadius, 0.0) ? -1 : 1;
+    Member hm[2] = {};
+    hm[0].active = 1; hm[1].active = 1;
+    HIPCHK(hipMemcpyAsync(d_r, &hr, sizeof(RegionAlloc), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_m, hm, 2 * sizeof(Member), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_allocate, dim3(1), dim3(256), 0, s, (const RegionAlloc*)d_r, d_m, max_sqrt_num_cells);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hm, d_m, 2 * sizeof(Member), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    out4[0] = (double)hm[0].S;
+    out4[1] = hm[0].numCell;
+    out4[2] = hr.member1 >= 0 ? (double)hm[1].S : 0.0;
+    out4[3] = hr.member1 >= 0 ? hm[1].numCell : 0.0;
+    return XH_OK;
+}
+
+extern "C" size_t xh_cell_mesh_workspace_bytes(int S)
+{
+    return 256 + (1000 + 64) * 8 + 2 * (size_t)S * 8 + 256;
+}
+
+extern "C" int xh_construct_cell_mesh(int polar, double numCell, int S, double mass, double r_s, double R_eq,
+                                      double zeta, double epsilon, int star_shape_ind, double cedeRadius,
+                                      double cedeColatitude, double superRadius, double superColatitude,
+                                      double superAzimuth, double* theta, double* phi, double* r, double* cellArea,
+                                      double* maxEmissionAngle, double* cos_gamma, double* effGrav, void* workspace,
+                                      size_t workspace_bytes, int* status_out, void* stream)
+{
+    (void)mass;
+    hipStream_t s = (hipStream_t)stream;
+    if (S < 2 || S > 32767) return fail(XH_EINVAL, "xh_construct_cell_mesh: bad size");
+    if (workspace_bytes < xh_cell_mesh_workspace_bytes(S))
+        return fail(XH_EINVAL, "xh_construct_cell_mesh: workspace too small");
+    const int table_n = 1000;
+    char* w = (char*)workspace;
+    Member* d_m = (Member*)w; w += 256;
+    double* d_tab = (double*)w; w += (size_t)table_n * 8;
+    double* d_total = (double*)w; w += 64 * 8;
+    double* d_th = (double*)w; w += (size_t)S * 8;
+    double* d_ph = (double*)w;
+    Member hm{};
+    hm.R_eq = R_eq; hm.epsilon = epsilon; hm.zeta = zeta; hm.r_s = r_s; hm.shape = star_shape_ind; hm.active = 1;
+    hm.cedeR = cedeRadius; hm.cedeColat = cedeColatitude; hm.superR = superRadius; hm.superColat = superColatitude;
+    hm.superAzi = superAzimuth; hm.polar = polar ? 1 : 0; hm.S = S; hm.numCell = numCell;
+    HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
+    HIPCHK(fill(cellArea, (size_t)S * S, 0.0, s));
+    Layout L{S, 3, 0, 0, 0, 8, 0, 1};
+    hipLaunchKernelGGL(k_mesh_table, dim3((table_n + 1 + 3) / 4, 1), dim3(256), 0, s, (const Member*)d_m, d_tab,
+                       d_total, table_n);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mesh_rings, dim3((S + 1) / 2, 1), dim3(256), 0, s, (const Member*)d_m,
+                       (const double*)d_tab, (const double*)d_total, table_n, L, d_th, d_ph, r, cellArea,
+                       maxEmissionAngle, cos_gamma, effGrav);
+    HIPCHK(hipGetLastError());
+    // the reference's [S, S] colatitude and azimuth arrays
+    const unsigned nb = (unsigned)(((size_t)S * S + 255) / 256);
+    hipLaunchKernelGGL(k_spread, dim3(nb), dim3(256), 0, s, theta, (const double*)d_th, S, S, 1);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_spread, dim3(nb), dim3(256), 0, s, phi, (const double*)d_ph, S, S, 0);
+    HIPCHK(hipGetLastError());
+    Member back{};
+    HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *status_out = back.status;
+    return XH_OK;
+}
+
 extern "C" int xh_energy_integrator(int N_E, int N_P, const double* signal, const double* logE, int N_edges,
                                     const double* logEdges, double* out, void* stream)
 {
@@ -550,6 +688,39 @@ extern "C" int xh_eval_marginal_likelihood(double T, int N_b, const double* phas
     hipLaunchKernelGGL(k_marginal, dim3(N_ch, 1), dim3(64), 2 * N_b * 8, s, A);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_lnl_reduce, dim3(1), dim3(64), 0, s, A, (const Member*)nullptr, 0, d_lnl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(lnl_out, d_lnl, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return XH_OK;
+}
+
+extern "C" size_t xh_poisson_workspace_bytes(int N_ch) { return (size_t)N_ch * (8 + 4) + 64 + 256; }
+
+extern "C" int xh_poisson_likelihood_given_background(double T, int N_b, const double* phases, int N_ch,
+                                                      const double* counts, int n_comp, const double* comps,
+                                                      int N_P, const double* comp_phases, const double* shifts,
+                                                      const double* background, const double* precomp,
+                                                      int allow_negative, double* lnl_out, double* EXPEC,
+                                                      void* workspace, size_t workspace_bytes, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (N_ch < 1 || N_b < 1 || n_comp < 1 || N_P < 1)
+        return fail(XH_EINVAL, "xh_poisson_likelihood_given_background: bad sizes");
+    if (!background) return fail(XH_EINVAL, "xh_poisson_likelihood_given_background: the background is required");
+    if (workspace_bytes < xh_poisson_workspace_bytes(N_ch))
+        return fail(XH_EINVAL, "xh_poisson_likelihood_given_background: workspace too small");
+    char* w = (char*)workspace;
+    double* d_ll = (double*)w; w += (size_t)N_ch * 8;
+    double* d_lnl = (double*)w; w += 64;
+    int* d_st = (int*)w;
+    MarginalArgs A{};
+    A.signals = comps; A.comp_phases = comp_phases; A.shifts = shifts; A.phases = phases; A.counts = counts;
+    A.precomp = precomp; A.background = background; A.ll = d_ll; A.status = d_st; A.STAR = EXPEC;
+    A.exposure = T; A.llzero = -1.0e90; A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp;
+    A.allow_negative = allow_negative; A.seed = next_seed(); A.seed_ptr = nullptr;
+    hipLaunchKernelGGL(k_poisson_bg, dim3(N_ch, 1), dim3(64), 2 * (size_t)N_b * 8, s, A);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_poisson_reduce, dim3(1), dim3(64), 0, s, A, d_lnl);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(lnl_out, d_lnl, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1049,24 +1220,26 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     // rays: lane-per-ray first pass, then the listed rays with full QAG
     HIPCHK(fill_int(p->d_raycount, 2, 0, s));
     hipLaunchKernelGGL(k_rays_lane, dim3((L.N_R - 1 + 63) / 64, S, nm), dim3(64), 0, s, p->d_members, L, p->d_r,
-                       p->d_maxAlpha, p->d_cosa, p->d_defl, p->d_lag, p->d_raywork, p->d_raycount);
+                       p->d_maxAlpha, p->d_cosa, p->d_defl, p->d_lag, p->d_raywork, p->d_raycount,
+                       (const double*)nullptr);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_rays, dim3(kRayWaveBlocks), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_maxAlpha,
-                       p->d_defl, p->d_lag, p->d_raywork, p->d_raycount);
+                       p->d_defl, p->d_lag, p->d_raywork, p->d_raycount, (const double*)nullptr);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_rays_fix, dim3(nm), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_maxAlpha,
-                       p->d_defl, p->d_lag, p->d_maxDefl);
+                       p->d_defl, p->d_lag, p->d_maxDefl, 1, (const double*)nullptr);
     HIPCHK(hipGetLastError());
     if (p->has_else) {
         hipLaunchKernelGGL(k_rays_lane, dim3((Le.N_R - 1 + 63) / 64, Le.S_max, n), dim3(64), 0, s, p->d_else, Le,
                            p->d_else_r, p->d_else_maxA, p->d_else_cosa, p->d_else_defl, p->d_else_lag,
-                           p->d_else_raywork, p->d_raycount + 1);
+                           p->d_else_raywork, p->d_raycount + 1, (const double*)nullptr);
         HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_rays, dim3(kRayWaveBlocks), dim3(256), 0, s, p->d_else, Le, p->d_else_r,
-                           p->d_else_maxA, p->d_else_defl, p->d_else_lag, p->d_else_raywork, p->d_raycount + 1);
+                           p->d_else_maxA, p->d_else_defl, p->d_else_lag, p->d_else_raywork, p->d_raycount + 1,
+                           (const double*)nullptr);
         HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_rays_fix, dim3(n), dim3(256), 0, s, p->d_else, Le, p->d_else_r, p->d_else_maxA,
-                           p->d_else_defl, p->d_else_lag, p->d_else_maxD);
+                           p->d_else_defl, p->d_else_lag, p->d_else_maxD, 0, (const double*)nullptr);
         HIPCHK(hipGetLastError());
     }
     RingArgs RA{p->d_theta, p->d_r, p->d_u, p->d_cos_gamma, p->d_maxDefl, p->d_defl, p->d_cosa, p->d_lag,

@@ -50,11 +50,13 @@ __global__ void k_mesh_rings(const Member* members, const double* area_tab, cons
                              int table_n, Layout L, double* theta, double* phi, double* rr,
                              double* cellArea, double* maxAlpha, double* cos_gamma, double* effGrav);
 __global__ void k_rays_lane(const Member* members, Layout L, const double* rr, const double* maxAlpha_in,
-                            double* cos_alpha, double* defl_raw, double* lag_raw, int* work, int* nwork);
+                            double* cos_alpha, double* defl_raw, double* lag_raw, int* work, int* nwork,
+                            const double* uu);
 __global__ void k_rays(const Member* members, Layout L, const double* rr, const double* maxAlpha_in,
-                       double* defl_raw, double* lag_raw, const int* work, const int* nwork);
+                       double* defl_raw, double* lag_raw, const int* work, const int* nwork,
+                       const double* uu);
 __global__ void k_rays_fix(Member* members, Layout L, const double* rr, double* maxAlpha,
-                           double* defl, double* lag, double* maxDefl);
+                           double* defl, double* lag, double* maxDefl, int calibrate, const double* uu);
 __global__ void k_energy_integrate(const Member* members, EnergyArgs A);
 __global__ void k_fold_f64(FoldGemmArgs A);
 __global__ void k_fold_f32(FoldGemmArgs A, const float* Mf);
@@ -115,6 +117,10 @@ struct FoldArgs {
 __host__ __device__ constexpr int moment_groups(int T) { return T == 4 ? 2 : (T == 2 ? 8 : 4); }
 
 __global__ void k_geometry(Member* members, Layout L, RingArgs A);
+__global__ void k_poisson_bg(MarginalArgs A);
+__global__ void k_over(double* out, const double* x, size_t n, double a);
+__global__ void k_spread(double* out, const double* v, int rows, int cols, int by_row);
+__global__ void k_poisson_reduce(MarginalArgs A, double* lnl);
 // the table atmosphere contracted over (log T, log g) for every parallel
 __global__ void k_table_slab(const Member* members, Layout L, FoldArgs F, double* slab);
 // closed mesh and time-invariant spectrum of the elsewhere component

@@ -308,14 +308,15 @@ __global__ void __launch_bounds__(64) k_rays_lane(const Member* __restrict__ mem
                                                   double* __restrict__ cos_alpha,
                                                   double* __restrict__ defl_raw,
                                                   double* __restrict__ lag_raw,
-                                                  int* __restrict__ work, int* __restrict__ nwork)
+                                                  int* __restrict__ work, int* __restrict__ nwork,
+                                                  const double* __restrict__ uu)
 {
     const int mi = blockIdx.z, i = blockIdx.y;
     const int j = blockIdx.x * 64 + threadIdx.x + 1;
     const Member m = members[mi];
     if (!m.active || m.status || i >= m.S || j >= L.N_R) return;
     const size_t ro = (size_t)mi * L.S_max + i;
-    const double u = m.r_s / rr[ro];
+    const double u = uu ? uu[ro] : m.r_s / rr[ro];
     const double maxA = clamped_max_alpha(maxAlpha_in[ro], u);
     const double inc = (1.0 - cos(maxA)) / ((double)L.N_R - 1.0);
     const double ca = 1.0 - (double)j * inc;
@@ -344,7 +345,8 @@ __global__ void __launch_bounds__(256) k_rays(const Member* __restrict__ members
                                               const double* __restrict__ maxAlpha_in,
                                               double* __restrict__ defl_raw,
                                               double* __restrict__ lag_raw,
-                                              const int* __restrict__ work, const int* __restrict__ nwork)
+                                              const int* __restrict__ work, const int* __restrict__ nwork,
+                                              const double* __restrict__ uu)
 {
     const int n = *nwork;
     const int waves = gridDim.x * 4;
@@ -354,7 +356,7 @@ __global__ void __launch_bounds__(256) k_rays(const Member* __restrict__ members
         const size_t ro = o / (size_t)L.N_R;
         const int mi = (int)(ro / (size_t)L.S_max);
         const Member m = members[mi];
-        const double u = m.r_s / rr[ro];
+        const double u = uu ? uu[ro] : m.r_s / rr[ro];
         const double maxA = clamped_max_alpha(maxAlpha_in[ro], u);
         const double inc = (1.0 - cos(maxA)) / ((double)L.N_R - 1.0);
         const double ca = 1.0 - (double)j * inc;
@@ -376,7 +378,8 @@ __global__ void __launch_bounds__(256) k_rays_fix(Member* __restrict__ members, 
                                                   double* __restrict__ maxAlpha,
                                                   double* __restrict__ defl,
                                                   double* __restrict__ lag,
-                                                  double* __restrict__ maxDefl)
+                                                  double* __restrict__ maxDefl, int calibrate,
+                                                  const double* __restrict__ uu)
 {
     const int mi = blockIdx.x;
     Member* mp = members + mi;
@@ -432,10 +435,11 @@ __global__ void __launch_bounds__(256) k_rays_fix(Member* __restrict__ members, 
     const double R_i = mp->R_eq / r_s, w = mp->mode_freq * TWO_PI;
     for (int i = threadIdx.x; i < S; i += blockDim.x) {
         size_t ro = (size_t)mi * L.S_max + i;
-        double u = r_s / rr[ro];
+        double u = uu ? uu[ro] : r_s / rr[ro];
         maxAlpha[ro] = clamped_max_alpha(maxAlpha[ro], u);
         maxDefl[ro] = D[(size_t)i * N_R + N_R - 1];
     }
+    if (!calibrate) return;   // compute_rays itself returns the raw lag
     for (int q = threadIdx.x; q < S * N_R; q += blockDim.x) {
         int i = q / N_R;
         double u = r_s / rr[(size_t)mi * L.S_max + i];

@@ -228,6 +228,38 @@ struct MargFn {
     }
 };
 
+// The expected count rate of request rq in channel ch and phase bin j:
+// each component's phase interpolant integrated over the bin shifted by the
+// component's phase shift (wrapped; a full cycle taken whole), negative bin
+// integrals dropped unless allowed, the total clipped at zero
+// (xpsi/tools/compute_expected_counts.pyx:66-197).
+__device__ double bin_rate(const MarginalArgs& A, int rq, int ch, int j)
+{
+    double st = 0.0;
+    for (int c = 0; c < A.n_comp; c++) {
+        const double* pulse = A.signals + (((size_t)rq * A.n_comp + c) * A.N_ch + ch) * A.N_P;
+        const double* cph = A.comp_phases;
+        const double shift = A.shifts[rq * A.n_comp + c];
+        if (A.N_P > 1) {
+            double pa = A.phases[j] + shift, pb = A.phases[j + 1] + shift;
+            if (eq12(pb - pa, 1.0)) { pa = 0.0; pb = 1.0; }
+            else { pa -= floor(pa); pb -= floor(pb); }
+            if (pa < pb) {
+                double v = akima_integ(cph, pulse, A.N_P, true, pa, pb);
+                if (v > 0.0 || A.allow_negative) st += v;
+            } else {
+                double v = akima_integ(cph, pulse, A.N_P, true, pa, 1.0);
+                if (v > 0.0 || A.allow_negative) st += v;
+                v = akima_integ(cph, pulse, A.N_P, true, 0.0, pb);
+                if (v > 0.0 || A.allow_negative) st += v;
+            }
+        } else if (j == 0) {
+            st = pulse[0];
+        }
+    }
+    return st < 0.0 ? 0.0 : st;
+}
+
 // wave per (channel, request)
 __global__ void __launch_bounds__(64) k_marginal(MarginalArgs A)
 {
@@ -240,29 +272,7 @@ __global__ void __launch_bounds__(64) k_marginal(MarginalArgs A)
     const double* counts = A.counts + (size_t)ch * N_b;
     // expected count rate from each region, phase-binned (shifted, wrapped)
     for (int j = lane; j < N_b; j += 64) {
-        double st = 0.0;
-        for (int c = 0; c < A.n_comp; c++) {
-            const double* pulse = A.signals + (((size_t)rq * A.n_comp + c) * A.N_ch + ch) * A.N_P;
-            const double* cph = A.comp_phases;
-            const double shift = A.shifts[rq * A.n_comp + c];
-            if (A.N_P > 1) {
-                double pa = A.phases[j] + shift, pb = A.phases[j + 1] + shift;
-                if (eq12(pb - pa, 1.0)) { pa = 0.0; pb = 1.0; }
-                else { pa -= floor(pa); pb -= floor(pb); }
-                if (pa < pb) {
-                    double v = akima_integ(cph, pulse, A.N_P, true, pa, pb);
-                    if (v > 0.0 || A.allow_negative) st += v;
-                } else {
-                    double v = akima_integ(cph, pulse, A.N_P, true, pa, 1.0);
-                    if (v > 0.0 || A.allow_negative) st += v;
-                    v = akima_integ(cph, pulse, A.N_P, true, 0.0, pb);
-                    if (v > 0.0 || A.allow_negative) st += v;
-                }
-            } else if (j == 0) {
-                st = pulse[0];
-            }
-        }
-        if (st < 0.0) st = 0.0;
+        double st = bin_rate(A, rq, ch, j);
         if (A.background) st += A.background[(size_t)ch * N_b + j];
         st *= n;
         star[j] = st;
@@ -390,6 +400,19 @@ __global__ void __launch_bounds__(256) k_fill(double* p, size_t n, double v)
     if (i < n) p[i] = v;
 }
 
+__global__ void __launch_bounds__(256) k_over(double* out, const double* x, size_t n, double a)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a / x[i];
+}
+
+// rows [rows][cols] of a per-row (by_row) or per-column value: out[i][j] = v[i] or v[j]
+__global__ void __launch_bounds__(256) k_spread(double* out, const double* v, int rows, int cols, int by_row)
+{
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < (size_t)rows * cols) out[q] = by_row ? v[q / cols] : v[q % cols];
+}
+
 __global__ void __launch_bounds__(256) k_fill_int(int* p, size_t n, int v)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,6 +450,75 @@ __global__ void __launch_bounds__(64) k_lnl_reduce(MarginalArgs A, const Member*
         h ^= h >> 31;
         double u = (double)(h >> 11) / 9007199254740992.0;
         s = A.llzero * (0.1 + 0.9 * u);
+    }
+    lnl[rq] = s;
+}
+
+// Poisson likelihood given a background (the expected counts are the star's
+// plus the background's count rate times the exposure), wave per (channel,
+// request): lanes form each bin's expected counts and its term, then one lane
+// sums the channel in bin order (from the channel's -sum ln d! when given) and
+// counts the bins with data but no expected counts.  ll [request][channel],
+// status [request][channel] = such bins; STAR holds the expected counts.
+// (xpsi/likelihoods/_poisson_likelihood_given_background.pyx:14-113)
+__global__ void __launch_bounds__(64) k_poisson_bg(MarginalArgs A)
+{
+    const int ch = blockIdx.x, rq = blockIdx.y, lane = threadIdx.x;
+    const int N_b = A.N_b;
+    extern __shared__ double sm[];
+    double* ex = sm;                // N_b expected counts
+    double* tm = ex + N_b;          // N_b terms d log(ex) - ex
+    const double* counts = A.counts + (size_t)ch * N_b;
+    const size_t ro = (size_t)rq * A.N_ch + ch;
+    int bad = 0;
+    for (int j = lane; j < N_b; j += 64) {
+        const double e = (bin_rate(A, rq, ch, j) + A.background[(size_t)ch * N_b + j]) * A.exposure;
+        const double d = counts[j];
+        ex[j] = e;
+        if (A.STAR) A.STAR[ro * N_b + j] = e;
+        if (e > 0.0) tm[j] = d * log(e);
+        else {
+            tm[j] = 0.0;
+            if (!(d == 0.0 && e == 0.0)) bad++;
+        }
+    }
+    bad = (int)wave_sum((double)bad);
+    __syncthreads();
+    if (lane == 0) {
+        double s = A.precomp ? A.precomp[ch] : 0.0;
+        for (int j = 0; j < N_b; j++) {
+            if (ex[j] > 0.0) {
+                s -= ex[j];
+                s += tm[j];
+            }
+        }
+        A.ll[ro] = s;
+        A.status[ro] = bad;
+    }
+}
+
+// ordered sum over channels; every bin with data but no expected counts adds
+// its own draw llzero (0.1 + 0.9 U) (the reference's -1e90 (0.1 + 0.9 U))
+__global__ void __launch_bounds__(64) k_poisson_reduce(MarginalArgs A, double* __restrict__ lnl)
+{
+    const int rq = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    const unsigned long long sd = A.seed_ptr ? *A.seed_ptr : A.seed;
+    double s = 0.0;
+    unsigned long long k = 0;
+    for (int c = 0; c < A.N_ch; c++) {
+        const size_t ro = (size_t)rq * A.N_ch + c;
+        s += A.ll[ro];
+        for (int b = 0; b < A.status[ro]; b++, k++) {
+            unsigned long long h = sd * 0x100000001B3ULL + (unsigned long long)(rq + 1) * 0x9E3779B97F4A7C15ULL +
+                                   k * 0xD1B54A32D192ED03ULL;
+            h ^= h >> 30;
+            h *= 0xBF58476D1CE4E5B9ULL;
+            h ^= h >> 27;
+            h *= 0x94D049BB133111EBULL;
+            h ^= h >> 31;
+            s += A.llzero * (0.1 + 0.9 * ((double)(h >> 11) / 9007199254740992.0));
+        }
     }
     lnl[rq] = s;
 }
