@@ -602,10 +602,41 @@ __global__ void __launch_bounds__(512, OCC) k_moment_fold(const Member* __restri
                         if (wt == t) n_t = ncount[3 + t];
                     const unsigned short* lt = ILT + wt * N_L;
                     if (on[0]) {
+                        // software-pipelined: the operands of the next two
+                        // intervals are read while this pair's MFMAs run
+                        struct Ops { R b; R a[PPW]; };
+                        auto fetch = [&](int idx, Ops& o) {
+                            o.b = mt[idx * 64 + lane];
+#pragma unroll
+                            for (int u = 0; u < PPW; u++) {
+                                R a = cg[u][idx * 64 + lane];
+                                if ((mxg[u][idx] >> kb) & 1u) a = (R)0;
+                                o.a[u] = a;
+                            }
+                        };
                         int q = sub;
-                        for (; q + WPP < n_t; q += 2 * WPP) {
-                            sweep(lt[q], swA);
-                            sweep(lt[q + WPP], swB);
+                        if (q + WPP < n_t) {
+                            Ops c0, c1;
+                            fetch(lt[q], c0);
+                            fetch(lt[q + WPP], c1);
+                            for (; q + 3 * WPP < n_t; q += 2 * WPP) {
+                                Ops n0, n1;
+                                fetch(lt[q + 2 * WPP], n0);
+                                fetch(lt[q + 3 * WPP], n1);
+#pragma unroll
+                                for (int u = 0; u < PPW; u++) {
+                                    swA[u] = Mfma<R>::step(c0.a[u], c0.b, swA[u]);
+                                    swB[u] = Mfma<R>::step(c1.a[u], c1.b, swB[u]);
+                                }
+                                c0 = n0;
+                                c1 = n1;
+                            }
+#pragma unroll
+                            for (int u = 0; u < PPW; u++) {
+                                swA[u] = Mfma<R>::step(c0.a[u], c0.b, swA[u]);
+                                swB[u] = Mfma<R>::step(c1.a[u], c1.b, swB[u]);
+                            }
+                            q += 2 * WPP;
                         }
                         if (q < n_t) sweep(lt[q], swA);
                     }
