@@ -357,6 +357,18 @@ int xh_pipeline_use_graphs(xh_pipeline* p, int on);
  * later launch of that size replays at once. */
 int xh_pipeline_capture(xh_pipeline* p, int n, void* stream);
 
+/* Prior inverse sampling for a batch of proposals (xpsi/Prior.py:84-160 with
+ * the examples' compactness prior, examples/examples_fast/Modules/CustomPrior.py:49-66):
+ * params[q][k] = lo_k + (hi_k - lo_k) hypercube[q][k] for n points of d
+ * coordinates (device, [n, d]); bounds [d, 2] (device); logprior[q] = 0 when
+ * every coordinate lies within its bounds and, when idx_mass and idx_radius
+ * are >= 0, the star of that mass (Msun) and equatorial radius (km) spinning
+ * at `frequency` Hz passes the compactness cuts, else -inf. */
+int xh_prior_inverse_sample(int n, int d, const double* hypercube,
+                            const double* bounds, int idx_mass, int idx_radius,
+                            double frequency, double* params, double* logprior,
+                            void* stream);
+
 /* ------------------------------------------------------------------------
  * Continuous-batching server: n_slots pipelines of slot_batch requests, each
  * on its own stream; a dispatcher thread admits queued requests into free

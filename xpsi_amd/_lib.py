@@ -98,6 +98,8 @@ def lib():
     L.xh_construct_cell_mesh.restype = i32
     L.xh_construct_cell_mesh.argtypes = ([i32, d, i32] + [d] * 5 + [i32] + [d] * 5 + [vp] * 7 +
                                          [vp, sz, C.POINTER(i32), vp])
+    L.xh_prior_inverse_sample.restype = i32
+    L.xh_prior_inverse_sample.argtypes = [i32, i32, vp, vp, i32, i32, d, vp, vp, vp]
     L.xh_poisson_workspace_bytes.restype = sz
     L.xh_poisson_workspace_bytes.argtypes = [i32]
     L.xh_poisson_likelihood_given_background.restype = i32
@@ -176,6 +178,7 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_poisson_workspace_bytes", "xh_poisson_likelihood_given_background",
             "xh_compute_rays_workspace_bytes", "xh_compute_rays", "xh_allocate_cells_workspace_bytes",
             "xh_allocate_cells", "xh_cell_mesh_workspace_bytes", "xh_construct_cell_mesh",
+            "xh_prior_inverse_sample",
             "xh_pipeline_create", "xh_pipeline_destroy", "xh_pipeline_eval",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",

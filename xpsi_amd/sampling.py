@@ -19,6 +19,13 @@ Kept from ``Likelihood.__call__``:
   driver does (:305-336), and it is passed on unchanged;
 - otherwise the log prior, when a prior is given, is added (:507-511).
 
+``Likelihood`` keeps the reference's one-vector call itself,
+``__call__(p=None, reinitialise=False, force=False)``, with its change
+detection (a vector equal to the last one returns the stored value unless
+``force``), and ``serve_prior_draws`` feeds batches of prior proposals drawn
+on the device (``xpsi_amd.prior.DevicePrior``) into the continuous-batching
+server.
+
 Vector order: ``parameter_names(model)``.  With one hot region it is the
 order of the reference's config-1 style models (mass, radius, distance,
 cos_inclination, phase_shift, super_colatitude, super_radius,
@@ -133,3 +140,72 @@ class VectorizedLikelihood:
 
     def close(self):
         self._pl.close()
+
+
+class Likelihood:
+    """The reference's sampler-facing call, ``Likelihood.__call__(p=None,
+    reinitialise=False, force=False) -> float`` (xpsi/Likelihood.py:450-513):
+    prior gate, one device evaluation, llzero guard, prior added; a vector
+    equal to the previous one returns the stored value unless ``force``
+    (the reference's parameter-change detection, ParameterSubspace.py:267-274).
+    ``batch(P)`` evaluates many vectors at once (an addition to the reference
+    API).  ``p=None`` re-evaluates the current vector."""
+
+    def __init__(self, model, prior=None, names=None, max_batch=1024, seed=None):
+        self.model = model
+        self.names = parameter_names(model) if names is None else list(names)
+        self.prior = prior
+        log_prior = None
+        if prior is not None:
+            log_prior = prior if callable(prior) else None
+        self._one = VectorizedLikelihood(model, max_batch=1, names=self.names, log_prior=log_prior, seed=seed)
+        self._many = None
+        self._max_batch = int(max_batch)
+        self._log_prior = log_prior
+        self._seed = seed
+        self._last_p = None
+        self._last = None
+
+    def __call__(self, p=None, reinitialise=False, force=False):
+        if reinitialise:
+            self._last_p, self._last = None, None
+        if p is None:
+            if self._last_p is None:
+                raise XHError("no parameter vector set")
+            p = self._last_p
+            force = True
+        p = np.asarray(p, dtype=np.float64)
+        if not force and self._last_p is not None and np.array_equal(p, self._last_p):
+            return self._last
+        self._last = float(self._one(p))
+        self._last_p = p.copy()
+        return self._last
+
+    def batch(self, P):
+        if self._many is None:
+            self._many = VectorizedLikelihood(self.model, max_batch=self._max_batch, names=self.names,
+                                              log_prior=self._log_prior, seed=self._seed)
+        return self._many(P)
+
+    def close(self):
+        self._one.close()
+        if self._many is not None:
+            self._many.close()
+
+
+def serve_prior_draws(server, model, prior, n, rng=None, names=None, batch=512):
+    """Draw n proposals from the prior on the device in batches of hypercube
+    points, submit those in its support to the server as they come, and
+    return (vectors [n, d], request ids [n]).  The results come back through
+    ``server.poll()``."""
+    names = parameter_names(model) if names is None else list(names)
+    rng = np.random.default_rng() if rng is None else rng
+    vecs, ids = [], []
+    while len(ids) < n:
+        P, lp = prior.inverse_sample(rng.random((batch, len(prior))))
+        for v in P[np.isfinite(lp)]:
+            if len(ids) == n:
+                break
+            ids.append(server.submit(vector_to_params(model, v, names)))
+            vecs.append(v)
+    return np.array(vecs), np.array(ids)
