@@ -198,6 +198,43 @@ int xh_poisson_likelihood_given_background(double exposure_time, int N_b,
                                            double* EXPEC, void* workspace,
                                            size_t workspace_bytes, void* stream);
 
+/* Replace xpsi/tools/synthesise.pyx:40-148 `synthesise_exposure(...) ->
+ * (EXPEC, SYNTHETIC, SCALE_BACKGROUND)` and :152-276
+ * `synthesise_given_total_count_number(...) -> (EXPEC, SYNTHETIC, SCALE_STAR,
+ * SCALE_BACKGROUND / SCALE_STAR)`: expected counts of the phase-binned star
+ * signal plus the background scaled to expected_background_counts, and one
+ * Poisson realisation per (channel, bin) drawn on the device from `seed`
+ * (splitmix64 streams; sequential inversion below mean 30, Hormann's PTRS
+ * above; the reference's GSL generator is not reproduced, only the
+ * distribution).  components [n_comp, N_ch, N_P] on one phase grid
+ * comp_phases [N_P]; phases [N_b+1]; background [N_ch, N_b] expected counts;
+ * EXPEC, SYNTHETIC [N_ch, N_b] device outputs; the scales to host memory
+ * (scales_out: two values).  workspace of xh_synthesise_workspace_bytes(). */
+size_t xh_synthesise_workspace_bytes(int N_ch, int N_b);
+int xh_synthesise_exposure(double exposure_time, int N_b, const double* phases,
+                           int N_ch, int n_comp, const double* components,
+                           int N_P, const double* comp_phases,
+                           const double* shifts,
+                           double expected_background_counts,
+                           const double* background, int allow_negative,
+                           unsigned long long seed, double* EXPEC,
+                           double* SYNTHETIC, double* scale_background,
+                           void* workspace, size_t workspace_bytes,
+                           void* stream);
+int xh_synthesise_given_total_count_number(int N_b, const double* phases,
+                                           double expected_star_counts,
+                                           int N_ch, int n_comp,
+                                           const double* components, int N_P,
+                                           const double* comp_phases,
+                                           const double* shifts,
+                                           double expected_background_counts,
+                                           const double* background,
+                                           int allow_negative,
+                                           unsigned long long seed,
+                                           double* EXPEC, double* SYNTHETIC,
+                                           double* scales_out, void* workspace,
+                                           size_t workspace_bytes, void* stream);
+
 /* No counterpart in the reference: xh_eval_marginal_likelihood without the
  * workspace argument, for callers that cannot allocate device memory (code
  * under stream capture, tight serving loops, harnesses that forbid

@@ -727,6 +727,75 @@ extern "C" int xh_poisson_likelihood_given_background(double T, int N_b, const d
     return XH_OK;
 }
 
+extern "C" size_t xh_synthesise_workspace_bytes(int N_ch, int N_b) { return (size_t)N_ch * N_b * 8 + 256; }
+
+// shared body of the two synthesis entry points (mode 0: given the exposure,
+// mode 1: given the total expected star counts)
+static int synthesise(int mode, double T, double n_star, int N_b, const double* phases, int N_ch, int n_comp,
+                      const double* comps, int N_P, const double* comp_phases, const double* shifts, double n_bg,
+                      const double* background, int allow_negative, unsigned long long seed, double* EXPEC,
+                      double* SYNTH, double* scales_out, void* workspace, size_t workspace_bytes, hipStream_t s)
+{
+    if (N_ch < 1 || N_b < 1 || n_comp < 1 || N_P < 1) return fail(XH_EINVAL, "synthesise: bad sizes");
+    if (!background) return fail(XH_EINVAL, "synthesise: the background is required");
+    if (workspace_bytes < xh_synthesise_workspace_bytes(N_ch, N_b))
+        return fail(XH_EINVAL, "synthesise: workspace too small");
+    double* rate = (double*)workspace;
+    double* scal = rate + (size_t)N_ch * N_b;
+    MarginalArgs A{};
+    A.signals = comps; A.comp_phases = comp_phases; A.shifts = shifts; A.phases = phases;
+    A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp; A.allow_negative = allow_negative;
+    hipLaunchKernelGGL(k_star_rate, dim3(N_ch), dim3(64), 0, s, A, rate);
+    HIPCHK(hipGetLastError());
+    const int n = N_ch * N_b;
+    hipLaunchKernelGGL(k_synth_totals, dim3(1), dim3(64), 0, s, (const double*)rate, background, n,
+                       mode == 1 ? 1.0e5 : 1.0, scal);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_synth_draw, dim3((n + 255) / 256), dim3(256), 0, s, mode, (const double*)rate, background, n,
+                       T, n_star, n_bg, scal, seed, EXPEC, SYNTH);
+    HIPCHK(hipGetLastError());
+    double h[4];
+    HIPCHK(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    scales_out[0] = h[2];
+    scales_out[1] = h[3];
+    return XH_OK;
+}
+
+extern "C" int xh_synthesise_exposure(double exposure_time, int N_b, const double* phases, int N_ch, int n_comp,
+                                      const double* comps, int N_P, const double* comp_phases,
+                                      const double* shifts, double expected_background_counts,
+                                      const double* background, int allow_negative, unsigned long long seed,
+                                      double* EXPEC, double* SYNTHETIC, double* scale_background,
+                                      void* workspace, size_t workspace_bytes, void* stream)
+{
+    double sc[2];
+    int rc = synthesise(0, exposure_time, 0.0, N_b, phases, N_ch, n_comp, comps, N_P, comp_phases, shifts,
+                        expected_background_counts, background, allow_negative, seed, EXPEC, SYNTHETIC, sc,
+                        workspace, workspace_bytes, (hipStream_t)stream);
+    if (rc == XH_OK) *scale_background = sc[0];
+    return rc;
+}
+
+extern "C" int xh_synthesise_given_total_count_number(int N_b, const double* phases, double expected_star_counts,
+                                                      int N_ch, int n_comp, const double* comps, int N_P,
+                                                      const double* comp_phases, const double* shifts,
+                                                      double expected_background_counts, const double* background,
+                                                      int allow_negative, unsigned long long seed, double* EXPEC,
+                                                      double* SYNTHETIC, double* scales_out, void* workspace,
+                                                      size_t workspace_bytes, void* stream)
+{
+    double sc[2];
+    int rc = synthesise(1, 0.0, expected_star_counts, N_b, phases, N_ch, n_comp, comps, N_P, comp_phases, shifts,
+                        expected_background_counts, background, allow_negative, seed, EXPEC, SYNTHETIC, sc,
+                        workspace, workspace_bytes, (hipStream_t)stream);
+    if (rc == XH_OK) {
+        scales_out[0] = sc[1];               // SCALE_STAR: the exposure is 1e5 s times it
+        scales_out[1] = sc[0] / sc[1];       // SCALE_BACKGROUND / SCALE_STAR
+    }
+    return rc;
+}
+
 // ============================================================= pipeline
 
 struct xh_pipeline {

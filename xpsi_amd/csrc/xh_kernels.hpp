@@ -118,6 +118,10 @@ __host__ __device__ constexpr int moment_groups(int T) { return T == 4 ? 2 : (T 
 
 __global__ void k_geometry(Member* members, Layout L, RingArgs A);
 __global__ void k_poisson_bg(MarginalArgs A);
+__global__ void k_star_rate(MarginalArgs A, double* out);
+__global__ void k_synth_totals(const double* rate, const double* bg, int n, double mul, double* scal);
+__global__ void k_synth_draw(int mode, const double* rate, const double* bg, int n, double T, double n_star,
+                             double n_bg, double* scal, unsigned long long seed, double* expec, double* synth);
 __global__ void k_over(double* out, const double* x, size_t n, double a);
 __global__ void k_spread(double* out, const double* v, int rows, int cols, int by_row);
 __global__ void k_poisson_reduce(MarginalArgs A, double* lnl);

@@ -523,4 +523,102 @@ __global__ void __launch_bounds__(64) k_poisson_reduce(MarginalArgs A, double* _
     lnl[rq] = s;
 }
 
+
+// ---------------------------------------------------------------- synthesis
+// Poisson realisations of the expected counts
+// (xpsi/tools/synthesise.pyx:40-276).  Random numbers: splitmix64 (Steele,
+// Lea and Flood 2014) of a per-bin stream (seed, bin index); Poisson variates
+// by sequential inversion below mean 30 and by Hormann's transformed
+// rejection with squeeze (PTRS, Hormann 1993) above.
+struct SplitMix {
+    unsigned long long s;
+    __device__ unsigned long long next()
+    {
+        unsigned long long z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    __device__ double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+__device__ double poisson_draw(double mu, SplitMix& r)
+{
+    if (!(mu > 0.0)) return 0.0;
+    if (mu < 30.0) {
+        // inversion: walk the cumulative distribution from zero
+        double p = exp(-mu), c = p, u = r.uniform();
+        double k = 0.0;
+        while (u > c && k < 1000.0) {
+            k += 1.0;
+            p *= mu / k;
+            c += p;
+        }
+        return k;
+    }
+    // PTRS: a hat from the transformed uniform U, the squeeze accepting most
+    // draws at once, the exact log-density test for the rest
+    const double smu = sqrt(mu), lmu = log(mu);
+    const double b = 0.931 + 2.53 * smu, a = -0.059 + 0.02483 * b;
+    const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4), v_r = 0.9277 - 3.6224 / (b - 2.0);
+    for (int it = 0; it < 1000; it++) {
+        const double U = r.uniform() - 0.5, V = r.uniform();
+        const double us = 0.5 - fabs(U);
+        const double k = floor((2.0 * a / us + b) * U + mu + 0.43);
+        if (us >= 0.07 && V <= v_r) return k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (log(V) + log(inv_alpha) - log(a / (us * us) + b) <= -mu + k * lmu - lgamma(k + 1.0)) return k;
+    }
+    return floor(mu + 0.5);   // not reached in practice (acceptance above 0.9 per try)
+}
+
+// the clipped star count rate per (channel, bin) into out [N_ch][N_b]
+__global__ void __launch_bounds__(64) k_star_rate(MarginalArgs A, double* __restrict__ out)
+{
+    const int ch = blockIdx.x;
+    for (int j = threadIdx.x; j < A.N_b; j += 64) out[(size_t)ch * A.N_b + j] = bin_rate(A, 0, ch, j);
+}
+
+// ordered totals (the reference's sequential loops): scal[0] background sum,
+// scal[1] sum of the star counts rate x mul
+__global__ void __launch_bounds__(64) k_synth_totals(const double* __restrict__ rate,
+                                                     const double* __restrict__ bg, int n, double mul, double* scal)
+{
+    if (threadIdx.x != 0) return;
+    double b = 0.0, st = 0.0;
+    for (int q = 0; q < n; q++) { st += rate[q] * mul; b += bg[q]; }
+    scal[0] = b;
+    scal[1] = st;
+}
+
+// expected counts and one Poisson draw per (channel, bin).  mode 0
+// (synthesise_exposure): EXPEC = (rate + bg * sb / T) * T with
+// sb = n_bg / sum(bg); mode 1 (synthesise_given_total_count_number):
+// EXPEC = (rate * 1e5) * ss + bg * sb, ss = n_star / sum(rate * 1e5).
+// scal[2], scal[3] receive sb and ss.
+__global__ void __launch_bounds__(256) k_synth_draw(int mode, const double* __restrict__ rate,
+                                                    const double* __restrict__ bg, int n, double T, double n_star,
+                                                    double n_bg, double* scal, unsigned long long seed,
+                                                    double* __restrict__ expec, double* __restrict__ synth)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const double B = scal[0];
+    const double sb = B == 0.0 ? 0.0 : n_bg / B;
+    double ss = 0.0;
+    if (mode == 1) ss = n_star / scal[1];
+    if (q == 0) { scal[2] = sb; scal[3] = ss; }
+    if (q >= n) return;
+    double e;
+    if (mode == 0) {
+        e = rate[q];
+        e += bg[q] * sb / T;
+        e *= T;
+    } else {
+        e = (rate[q] * 1.0e5) * ss + bg[q] * sb;
+    }
+    expec[q] = e;
+    SplitMix r{seed ^ (0xD1B54A32D192ED03ULL * (unsigned long long)(q + 1))};
+    synth[q] = poisson_draw(e, r);
+}
+
 }  // namespace xh
