@@ -99,3 +99,18 @@ def test_compute_rays_dropin(star):
     e = elem_relerr(md[~clamped], mdr[~clamped])
     say("  maxDeflection rel err %.3e" % e)
     assert e < SIG_RTOL
+
+
+def test_instrument_call_folds_on_device():
+    # Instrument.__call__(signal, irange, orange) (xpsi/Instrument.py:163-197):
+    # matrix[orange, irange] @ signal, here through xh_fold
+    from xpsi_amd.instrument import Instrument
+    rng = np.random.default_rng(5)
+    M = rng.uniform(0.0, 1.0, (120, 90))
+    inst = Instrument(M, np.linspace(0.3, 10.0, 91), np.arange(120))
+    sig = rng.uniform(0.0, 2.0, (70, 33))
+    out = inst(sig, (10, 80), (5, 115))
+    ref = M[5:115, 10:80] @ sig
+    e = elem_relerr(out, ref)
+    say("Instrument fold rel err %.3e" % e)
+    assert out.shape == (110, 33) and e < 1e-13
