@@ -6,9 +6,9 @@
  * device), "host" pointers are ordinary process memory.  `stream` is a
  * hipStream_t (NULL = the default stream).  Every entry point returns 0 on
  * success or a negative XH_E* code; xh_last_error() describes the failure.
- * Device memory is allocated only by xh_pipeline_create() and
- * xh_server_create() (the server's pipelines); the drop-ins work in the
- * caller's buffers and workspaces.
+ * Device memory is allocated only by xh_pipeline_create(),
+ * xh_server_create() (the server's pipelines) and xh_atm_init() (a table's
+ * device copy); the drop-ins work in the caller's buffers and workspaces.
  */
 #ifndef XPSI_HIP_H
 #define XPSI_HIP_H
@@ -85,6 +85,26 @@ int xh_integrate_time_invariant(double omega, double r_s, double inclination,
                                 double* flux, void* workspace,
                                 size_t workspace_bytes, int* status_out,
                                 void* stream);
+
+/* Replaces the atmosphere extension point of the integrators,
+ * xpsi/surface_radiation_field/hot_wrapper.pyx:72-252 (init_hot,
+ * eval_hot_I, eval_hot_norm, free_hot), for the blackbody (ext 1,
+ * hot_BB.pyx:54-98) and the 4-D table (ext 2, hot_Num4D.pyx:66-460; axes
+ * logT, logg, mu, log10(E/kT) host arrays of lengths N[4], values I host,
+ * C order (T, g, mu, E) as preload.pyx:6-44 lays them out).  xh_atm_eval_I
+ * evaluates n queries at once (device arrays E [n] keV, mu [n], VEC [n, 2] =
+ * (log10 T, log10 g)) into out [n], the intensity without the norm, with the
+ * values one reference thread returns evaluating the queries in order from
+ * a fresh init_hot (the table's base-node history included); xh_atm_norm
+ * returns eval_hot_norm's factor.  xh_atm_init allocates the table's device
+ * copy; xh_atm_free releases it. */
+typedef struct xh_atmosphere xh_atmosphere;
+xh_atmosphere* xh_atm_init(int ext, const double* const* axes, const int* N,
+                           const double* I);
+int xh_atm_eval_I(xh_atmosphere* a, int n, const double* E, const double* mu,
+                  const double* VEC, double* out, void* stream);
+double xh_atm_norm(const xh_atmosphere* a);
+int xh_atm_free(xh_atmosphere* a);
 
 /* Replaces xpsi/cellmesh/rays.pyx:249-390 `compute_rays(N_T, N_u, r_s,
  * r_s_over_R[N_u], maxAlpha[N_u], N_R) -> (terminate, deflection, cos_alpha,

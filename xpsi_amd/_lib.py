@@ -98,6 +98,14 @@ def lib():
     L.xh_construct_cell_mesh.restype = i32
     L.xh_construct_cell_mesh.argtypes = ([i32, d, i32] + [d] * 5 + [i32] + [d] * 5 + [vp] * 7 +
                                          [vp, sz, C.POINTER(i32), vp])
+    L.xh_atm_init.restype = vp
+    L.xh_atm_init.argtypes = [i32, vp, vp, vp]
+    L.xh_atm_eval_I.restype = i32
+    L.xh_atm_eval_I.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.xh_atm_norm.restype = d
+    L.xh_atm_norm.argtypes = [vp]
+    L.xh_atm_free.restype = i32
+    L.xh_atm_free.argtypes = [vp]
     L.xh_synthesise_workspace_bytes.restype = sz
     L.xh_synthesise_workspace_bytes.argtypes = [i32, i32]
     L.xh_synthesise_exposure.restype = i32
@@ -187,7 +195,8 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_compute_rays_workspace_bytes", "xh_compute_rays", "xh_allocate_cells_workspace_bytes",
             "xh_allocate_cells", "xh_cell_mesh_workspace_bytes", "xh_construct_cell_mesh",
             "xh_prior_inverse_sample", "xh_synthesise_workspace_bytes", "xh_synthesise_exposure",
-            "xh_synthesise_given_total_count_number",
+            "xh_synthesise_given_total_count_number", "xh_atm_init", "xh_atm_eval_I", "xh_atm_norm",
+            "xh_atm_free",
             "xh_pipeline_create", "xh_pipeline_destroy", "xh_pipeline_eval",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",
