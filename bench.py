@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--one-stream", action="store_true", help="launch every energy group on one stream")
     ap.add_argument("--no-graphs", action="store_true", help="launch the batched step's kernels eagerly")
-    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-wait", type=float, default=0.002, help="latency cap of a partial batch (s)")
@@ -451,8 +451,9 @@ def serving(args, stream_free=True):
     continuous-batching server (batches of 1..slot_batch formed under a
     latency cap, admitted by device footprint, graphs captured for every batch
     size beforehand).  Reported: the sustained maximum (a closed-loop burst),
-    then Poisson arrivals at 2k-20k req/s with p50/p99 submit-to-result
-    latency; the plain FP64 single-request loop of the same requests beside."""
+    then Poisson arrivals offered at 25%, 50%, 80% and 95% of that maximum with
+    p50/p99 submit-to-result latency; the plain FP64 single-request loop of
+    the same requests beside."""
     from xpsi_amd import configs
     from xpsi_amd.pipeline import LikelihoodPipeline
     from xpsi_amd.serve import LikelihoodServer
@@ -497,8 +498,9 @@ def serving(args, stream_free=True):
     out = {"sustained_max_req_per_s": sustained, "burst": nb}
     log("serving: sustained %.0f req/s" % sustained)
     rng = np.random.default_rng(4)
-    for rate in (2000.0, 5000.0, 10000.0, 20000.0):
-        n = int(rate * args.serve_seconds)
+    for load in (0.25, 0.5, 0.8, 0.95):
+        rate = load * sustained
+        n = max(64, int(rate * args.serve_seconds))
         t_sub = np.cumsum(rng.exponential(1.0 / rate, n))
         lat, done = [], 0
         t0 = time.perf_counter()
@@ -515,11 +517,11 @@ def serving(args, stream_free=True):
                 break
         wall = time.perf_counter() - t0
         lat = np.array(lat) * 1e3
-        out["%d_req_per_s" % int(rate)] = {
-            "offered": rate, "submitted": int(k), "completed": int(done), "throughput": done / wall,
+        out["load_%.2f" % load] = {
+            "offered_req_per_s": rate, "submitted": int(k), "completed": int(done), "throughput": done / wall,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
-        log("serving at %d req/s: p50 %.2f ms p99 %.2f ms" % (rate, np.percentile(lat, 50),
-                                                               np.percentile(lat, 99)))
+        log("serving at %.0f req/s (%.0f%% load): p50 %.2f ms p99 %.2f ms"
+            % (rate, 100 * load, np.percentile(lat, 50), np.percentile(lat, 99)))
         if done < n:
             drain(k - done)
     st = srv.stats()
