@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 FP32_MFMA_PEAK_TFLOPS = 157.3
-KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold"}
+KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold", "waves": "k_wave_fold"}
 # HBM bytes of k_moment_fold<float, 1, 2, 2, 4> on the default batched workload,
 # from rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0, factors from a
 # 512 MiB copy) over the three launches of one step (KiB): 256, 128 and 64
@@ -57,6 +57,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--one-stream", action="store_true", help="launch every energy group on one stream")
     ap.add_argument("--no-graphs", action="store_true", help="launch the batched step's kernels eagerly")
+    ap.add_argument("--fold", default="auto",
+                    help="spline fold of the batched form: auto (autotuned per energy group) or kernel/shape, "
+                         "e.g. waves/2 or moment/2")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
@@ -119,7 +122,9 @@ def main():
     from xpsi_amd.pipeline import autotune_fold
     tuned, pls = {}, {}
     for ne, g in sorted(groups.items()):
-        mdl, tuned[ne] = autotune_fold(dict(configs.config3_model(ne), precision=args.precision), g, stream)
+        cands = None if args.fold == "auto" else ((args.fold.split("/")[0], int(args.fold.split("/")[1])),)
+        mdl, tuned[ne] = autotune_fold(dict(configs.config3_model(ne), precision=args.precision), g, stream,
+                                       **({} if cands is None else {"candidates": cands}))
         pls[ne] = LikelihoodPipeline(mdl, max_batch=len(g))
         if not args.no_graphs:
             pls[ne].use_graphs(True)   # each group's step replayed as one captured hipGraph
@@ -142,6 +147,8 @@ def main():
             out[ne] = pls[ne].wait()
 
     log("autotuned fold shapes: %s" % {ne: t["chosen"] for ne, t in tuned.items()})
+    log("fold ms per group and candidate: %s" % {ne: {k: round(v, 2) for k, v in t.items() if k != "chosen"}
+                                                  for ne, t in tuned.items()})
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

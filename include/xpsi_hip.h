@@ -320,9 +320,13 @@ typedef struct xh_model_config {
                                  on FP32 MFMA with FP64 sums per 4-deep slice (fold_terms
                                  selects split-BF16 instead); rays, phase-bin accumulation
                                  and likelihood quadrature stay FP64 */
-    int fold_kernel;          /* 0 automatic (moment form, else energy lanes, else
-                                 per-point), 1 per-point fold (FP64 only), 2 moment form,
-                                 3 energy lanes; xpsi_amd.pipeline.autotune_fold measures */
+    int fold_kernel;          /* 0 automatic (the wave fold for the precision form with
+                                 the blackbody, else the moment form, else energy lanes,
+                                 else per-point), 1 per-point fold (FP64 only), 2 moment form,
+                                 3 energy lanes, 4 wave-autonomous moment fold (precision
+                                 form, blackbody; one wavefront per 64 energies x 64 phases
+                                 x a block of parallels, no block barriers);
+                                 xpsi_amd.pipeline.autotune_fold measures */
     /* time-invariant blackbody emission from the rest of the surface
        (xpsi/Elsewhere.py): a closed mesh of else_sqrt_num_cells^2 cells and
        else_num_rays rays per ring; its spectrum joins the first hot region's
@@ -400,8 +404,14 @@ int xh_set_moment_shape(int tiles);
 /* Tuning: parallels per block of the moment-form and energy-lane folds for
  * batches of 32 members or more (blocks of one member meet by atomic adds; 0 =
  * automatic: all of a member's parallels in one moment-form block, 8 per
- * energy-lane block).  Applies to later launches. */
+ * energy-lane block).  Applies to pipelines created afterwards (and to later
+ * xh_integrate calls). */
 int xh_set_fold_rings(int rings);
+
+/* Tuning: parallels per wavefront of the wave fold (fold_kernel 4), at any
+ * batch size (0 = automatic: 1, measured fastest on the batched workload).
+ * Applies to pipelines created afterwards. */
+int xh_set_wave_fold_rings(int rings);
 
 /* Tuning: phases per wave of the energy-lane fold (8 or 16).  Applies to
  * pipelines created afterwards and to xh_integrate. */

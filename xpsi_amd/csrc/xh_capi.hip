@@ -241,6 +241,16 @@ extern "C" int xh_set_fold_rings(int rings)
     return XH_OK;
 }
 
+// parallels per wavefront of the wave fold (0 = automatic: 1)
+static int g_wave_rings = 0;
+
+extern "C" int xh_set_wave_fold_rings(int rings)
+{
+    if (rings < 0) return fail(XH_EINVAL, "parallels per wave-fold wavefront must be >= 0");
+    g_wave_rings = rings;
+    return XH_OK;
+}
+
 // the block shape of a launch over `members` members
 static int moment_shape(int N_L, int S_max, size_t rb, int atm, int members)
 {
@@ -802,7 +812,9 @@ struct xh_pipeline {
     xh_model_config cfg;
     Layout L;
     int nreg, mpr, maxb, N_in, N_ch, N_b, EC, table_n;
-    int fold_kind;               // 1 per-point fold, 2 moment form
+    int fold_kind;               // 1 per-point fold, 2 moment form, 3 energy lanes, 4 wave fold
+    int fold_rings;              // parallels per fold block (0 = automatic), fixed at creation
+    int wave_rings;              // parallels per wave-fold wavefront (0 = automatic)
     int shape;                   // moment form: block shape T (0 = by batch size)
     int lane_nq;                 // energy-lane fold: phases per wave
     bool fp32;                   // precision form
@@ -899,8 +911,12 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     p->L = Layout{smax, c->num_rays, c->num_leaves, N_P, c->N_E, IO, c->image_order_limit > 0 ? c->image_order_limit : 0,
                   c->atm_ext};
     p->EC = pick_ec(c->num_leaves);
-    if (c->precision < 0 || c->precision > 1 || c->fold_kernel < 0 || c->fold_kernel > 3) {
-        fail(XH_EINVAL, "pipeline: precision must be 0 or 1 and fold_kernel 0, 1, 2 or 3"); delete p; return nullptr;
+    if (c->precision < 0 || c->precision > 1 || c->fold_kernel < 0 || c->fold_kernel > 4) {
+        fail(XH_EINVAL, "pipeline: precision must be 0 or 1 and fold_kernel 0 .. 4"); delete p; return nullptr;
+    }
+    if (c->fold_kernel == 4 && (c->precision != 1 || c->atm_ext != 1 || c->num_leaves < 5)) {
+        fail(XH_EINVAL, "pipeline: the wave fold runs the precision form with the blackbody and at least 5 leaves");
+        delete p; return nullptr;
     }
     p->fp32 = c->precision == 1;
     p->fold_terms = c->fold_terms;
@@ -910,6 +926,8 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     {
         // a tuned shape larger than this model's LDS allows falls back to smaller ones
         p->shape = g_moment_shape;
+        p->fold_rings = g_fold_rings;
+        p->wave_rings = g_wave_rings;
         while (p->shape > 1 && moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->shape, c->atm_ext) > kMomentLdsMax)
             p->shape >>= 1;
         const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->shape > 0 ? p->shape : 1, c->atm_ext) <= kMomentLdsMax;
@@ -926,8 +944,16 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
             delete p; return nullptr;
         }
         // automatic: the moment form, else the energy lanes, else per-point
-        if (c->fold_kernel == 3) p->fold_kind = 3;
+        // automatic: the wave fold for the precision form with the blackbody
+        // (measured fastest on every energy group of the batched workload),
+        // else the block moment form, else energy lanes, else per-point
+        const bool wave_ok = p->fp32 && c->atm_ext == 1 && c->num_leaves >= 5 &&
+                             wave_fold_lds(c->num_leaves, smax) <= 64 * 1024;
+        if (c->fold_kernel == 4) p->fold_kind = 4;
+        else if (c->fold_kernel == 3) p->fold_kind = 3;
         else if (c->fold_kernel == 1) p->fold_kind = 1;
+        else if (c->fold_kernel == 2) p->fold_kind = 2;
+        else if (wave_ok) p->fold_kind = 4;
         else if (fits) p->fold_kind = 2;
         else p->fold_kind = lane_fits ? 3 : 1;
     }
@@ -938,7 +964,12 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         L0.atm_ext = c->atm_ext;
         FoldArgs F0{};
         p->lane_nq = g_lane_nq;
-        if (p->fold_kind == 3) PHIP(launch_lane(p->fp32, p->lane_nq, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
+        if (p->fold_kind == 4) {
+            if (wave_fold_lds(c->num_leaves, smax) > 64 * 1024) {
+                fail(XH_EINVAL, "pipeline: the wave fold's LDS exceeds 64 KiB at this leaf count and mesh bound");
+                delete p; return nullptr;
+            }
+        } else if (p->fold_kind == 3) PHIP(launch_lane(p->fp32, p->lane_nq, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
         else if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, 0, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
         else PHIP(launch_fold(p->EC, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
     }
@@ -1326,7 +1357,8 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     // (the energy-lane fold: 8 parallels per block, so that members of very
     // different sizes finish in similar times; partial sums meet by atomics)
     int rb = (n >= 32) ? S : 1;
-    if (n >= 32 && g_fold_rings > 0 && p->fold_kind != 1) rb = std::min(S, g_fold_rings);
+    if (p->fold_kind == 4) rb = std::min(S, p->wave_rings > 0 ? p->wave_rings : 1);
+    else if (n >= 32 && p->fold_rings > 0 && p->fold_kind != 1) rb = std::min(S, p->fold_rings);
     else if (n >= 32 && p->fold_kind == 3) rb = std::min(S, 8);
     FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_effGrav, p->d_lastq,
                 p->d_atm_axes, p->d_atm_I, {p->atm_N[0], p->atm_N[1], p->atm_N[2], p->atm_N[3]}, p->d_energies,
@@ -1339,7 +1371,13 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
         FA.slab = p->d_slab;
         FA.atm_S = p->d_atm_S;
     }
-    if (p->fold_kind == 3)
+    if (p->fold_kind == 4) {
+        const unsigned chunks = (unsigned)(((N_E + 63) / 64) * ((N_P + 63) / 64));
+        hipLaunchKernelGGL(k_wave_fold, dim3((unsigned)((S + rb - 1) / rb), chunks, nm), dim3(64),
+                           wave_fold_lds(L.N_L, L.S_max), s, p->d_members, L, FA);
+        HIPCHK(hipGetLastError());
+    }
+    else if (p->fold_kind == 3)
         HIPCHK(launch_lane(p->fp32, p->lane_nq, lane_grid(N_E, N_P, p->lane_nq, (S + rb - 1) / rb, nm),
                            lane_lds(L.N_L, L.S_max, p->fp32 ? 4 : 8), s, p->d_members, L, FA));
     else if (p->fold_kind == 2)

@@ -112,6 +112,23 @@ __device__ __forceinline__ double akima_m(const double* x, const double* y, int 
     return 3.0 * sec(n - 2) - 2.0 * sec(n - 3); // k == n
 }
 
+// A cell's phase point wrapped into the leaves' span [lo, hi] as the
+// integrator wraps it (integrator_for_azimuthal_invariance.pyx:565-596):
+// while (x > hi) x -= 2 pi; while (x < lo) x += 2 pi
+__device__ __forceinline__ double wrap_phase(double x, double lo, double hi)
+{
+    if (x > hi) x -= TWO_PI;
+    else if (x < lo) x += TWO_PI;
+    if (x > hi || x < lo) {
+        int guard = 0;
+#pragma unroll 1
+        while (x > hi && guard++ < 64) x -= TWO_PI;
+#pragma unroll 1
+        while (x < lo && guard++ < 64) x += TWO_PI;
+    }
+    return x;
+}
+
 // Akima piece i from slopes m[i-2..i+2]: y = y_i + t(b + t(c + t d))
 __device__ __forceinline__ void akima_piece(double mm2, double mm1, double m0, double mp1, double mp2,
                                             double h, double& b, double& c, double& d)

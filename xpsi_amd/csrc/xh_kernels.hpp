@@ -108,6 +108,15 @@ struct FoldArgs {
     const double* atm_S;
 };
 
+// wave-autonomous moment fold (precision form, blackbody): one wavefront per
+// (ring block, 64 energies x 64 phases, member); flux zeroed beforehand
+__global__ void k_wave_fold(Member* members, Layout L, FoldArgs F);
+inline size_t wave_fold_lds(int N_L, int S_max)
+{
+    return (((size_t)(3 * N_L + S_max) * 8 + (size_t)(S_max + 2 * N_L) * 4) + 15) & ~(size_t)15;
+}
+
+
 // moment form: groups of T resident 16-energy chunks per block of shape T.
 // Shape 2 takes 8 groups (256 energies), so a 256-energy member shares one
 // block's per-parallel set-up per phase tile pair; the wave's 32 FP64

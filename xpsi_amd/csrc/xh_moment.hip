@@ -140,15 +140,7 @@ __device__ __forceinline__ void walk_queries(const double* __restrict__ PH, int 
         const R caj = can;
         if (j + 1 < jb) { cpn = cphi[j + 1]; can = carea[j + 1]; }
         if (isnan(caj)) continue;
-        double x = ph + cpj;
-        // while (x > hi) x -= 2 pi; while (x < lo) x += 2 pi
-        if (x > hi) x -= TWO_PI;
-        else if (x < lo) x += TWO_PI;
-        if (x > hi || x < lo) {
-            int guard = 0;
-            while (x > hi && guard++ < 64) x -= TWO_PI;
-            while (x < lo && guard++ < 64) x += TWO_PI;
-        }
+        const double x = wrap_phase(ph + cpj, lo, hi);
         bool entered = false;
         if (!(x >= blo && x < bhi)) {
             int id = cur;

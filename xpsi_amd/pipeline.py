@@ -22,7 +22,7 @@ Model description (dict):
   precision   'fp64' (default) or 'mixed' (see PRECISIONS)
   fold_terms  mixed-precision response fold: 0 (default) FP32 MFMA with FP64
               slice sums; 6 or 3 bf16 products of a three-term split
-  fold_kernel 'auto' (default), 'lanes', 'moment' or 'per-point' (see FOLD_KERNELS)
+  fold_kernel 'auto' (default), 'waves', 'lanes', 'moment' or 'per-point' (see FOLD_KERNELS)
 A parameter vector is a dict: mass, radius, distance, cos_inclination and
 `hot_regions`: list of dicts phase_shift, super_colatitude, super_radius,
 super_temperature, cede_colatitude, cede_radius, cede_azimuth,
@@ -70,7 +70,7 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 # "lanes": energies across a wave's lanes, (cell, phase) queries in lockstep;
 # "per-point": one spline evaluation per (cell, phase, energy) lane; "auto":
 # moment, else lanes, else per-point (autotune_fold measures them).
-FOLD_KERNELS = {"auto": 0, "per-point": 1, "moment": 2, "lanes": 3}
+FOLD_KERNELS = {"auto": 0, "per-point": 1, "moment": 2, "lanes": 3, "waves": 4}
 
 
 def model_config(model, max_batch=1, max_sqrt_cells=0):
@@ -271,12 +271,14 @@ def params_from_model(model):
 
 
 def autotune_fold(model, params, stream=None, reps=2,
-                  candidates=(("moment", 4), ("moment", 2), ("moment", 1), ("lanes", 8))):
+                  candidates=(("waves", 2), ("waves", 1), ("moment", 4), ("moment", 2), ("moment", 1),
+                              ("lanes", 8))):
     """Pick the spline-fold kernel and its block shape for this model and batch.
 
     Candidates are (kernel, shape): the moment form with blocks of `shape`
     16-phase tiles x `shape` resident 16-energy chunks, or the energy-lane
-    form with `shape` phases per wave.  For each, a pipeline evaluates
+    form with `shape` phases per wave, or the wave-autonomous moment fold
+    with `shape` parallels per wave (precision form, blackbody).  For each, a pipeline evaluates
     `params` once untimed and `reps` times with HIP-event stage timing; the
     candidate with the least profile-and-fold time wins.  Its shape is set for
     pipelines created afterwards, and the returned model dict selects its
@@ -285,9 +287,11 @@ def autotune_fold(model, params, stream=None, reps=2,
     L = lib()
 
     def select(kernel, shape):
-        if kernel == "moment":
+        if kernel == "waves":
+            check(L.xh_set_wave_fold_rings(int(shape)), "xh_set_wave_fold_rings")
+        elif kernel == "moment":
             check(L.xh_set_moment_shape(int(shape)), "xh_set_moment_shape")
-        else:
+        elif kernel == "lanes":
             check(L.xh_set_lane_phases(int(shape)), "xh_set_lane_phases")
 
     times = {}
