@@ -113,7 +113,7 @@ struct FoldArgs {
 __global__ void k_wave_fold(Member* members, Layout L, FoldArgs F);
 inline size_t wave_fold_lds(int N_L, int S_max)
 {
-    return (((size_t)(3 * N_L + S_max) * 8 + (size_t)(S_max + 2 * N_L) * 4) + 15) & ~(size_t)15;
+    return (((size_t)(3 * N_L + S_max) * 8 + (size_t)(S_max + 2 * N_L) * 4 + (size_t)N_L) + 15) & ~(size_t)15;
 }
 
 

@@ -120,7 +120,7 @@ __device__ __forceinline__ int piece_class(float a, float b, float c, float d, f
 
 // grid: x = ring block, y = energy chunk x phase chunk (64 each), z = member;
 // 64 threads; dynamic LDS wave_fold_lds(N_L, S_max)
-__global__ void __launch_bounds__(64) k_wave_fold(Member* members, Layout L, FoldArgs F)
+__global__ void __launch_bounds__(64, 2) k_wave_fold(Member* members, Layout L, FoldArgs F)
 {
     const int mi = blockIdx.z, lane = threadIdx.x;
     if (!members[mi].active || members[mi].status) return;
@@ -137,8 +137,9 @@ __global__ void __launch_bounds__(64) k_wave_fold(Member* members, Layout L, Fol
     double2* KN = (double2*)wsm;             // [N_L] knot: (geometric factor, 0 where hidden; 1 / redshift)
     double* PH = (double*)(KN + N_L);        // [N_L] leaf phases
     double* CPHI = PH + N_L;                 // [S_max] cell azimuths
-    float2* KH = (float2*)(CPHI + L.S_max);  // [N_L] interval width in FP32 (.x; .y unused)
+    float2* KH = (float2*)(CPHI + L.S_max);  // [N_L] interval (width, 1 / width) in FP32
     float* CAREA = (float*)(KH + N_L);       // [S_max] area / largest area, NaN = not radiating
+    unsigned char* LIVE = (unsigned char*)(CAREA + L.S_max);   // [N_L] piece's stencil shows a leaf
 
     // energy lane
     const int e = e0 + lane;
@@ -196,8 +197,19 @@ __global__ void __launch_bounds__(64) k_wave_fold(Member* members, Layout L, Fol
                 PH[q] = g[q];
                 if (q < NI) {
                     const double w = g[q + 1] - g[q];
-                    KH[q] = make_float2((float)w, 0.f);
+                    KH[q] = make_float2((float)w, (float)(1.0 / w));
                 }
+            }
+            __syncthreads();
+            // pieces whose Akima stencil (secants i-2 .. i+2, periodic) touches a
+            // visible leaf; the others are identically zero
+            for (int q = lane; q < NI; q += 64) {
+                bool any_vis = false;
+                for (int k = q - 2; k <= q + 2; k++) {
+                    const int sk = k < 0 ? k + NI : (k >= NI ? k - NI : k);
+                    any_vis = any_vis || KN[sk].x != 0.0 || KN[sk + 1].x != 0.0;
+                }
+                LIVE[q] = any_vis ? 1 : 0;
             }
             __syncthreads();
             const double lo = PH[0], hi = PH[N_L - 1];
@@ -257,54 +269,39 @@ __global__ void __launch_bounds__(64) k_wave_fold(Member* members, Layout L, Fol
             // window: y_i, y_{i+1}, y_{i+2} and secants m_{i-2} .. m_{i+1}
             const float yl2 = knot(KN[NI - 2]), yl1 = knot(KN[NI - 1]), yl0 = knot(KN[NI]);
             float ya = knot(KN[0]), yb = knot(KN[1]), yk = knot(KN[2]);
-            float sm2 = (yl1 - yl2) / KH[NI - 2].x, sm1 = (yl0 - yl1) / KH[NI - 1].x;
-            float s0 = (yb - ya) / KH[0].x, s1 = (yk - yb) / KH[1].x;
+            // secants and the cubic's coefficients by the interval's reciprocal
+            // width, rounded once from FP64: a fixed factor per interval whose
+            // sign varies from interval to interval (no bias shared by the bins)
+            float sm2 = (yl1 - yl2) * KH[NI - 2].y, sm1 = (yl0 - yl1) * KH[NI - 1].y;
+            float s0 = (yb - ya) * KH[0].y, s1 = (yk - yb) * KH[1].y;
             const float s0_first = s0, s1_first = s1;
             // Akima weights carried from piece to piece: w of piece i is wn of
-            // piece i - 1, and its al that piece's aln
-            float w_c = fabsf(s1 - s0) + fabsf(sm1 - sm2);
-            float al_c = w_c != 0.f ? fabsf(sm1 - sm2) / w_c : 0.f;
+            // piece i - 1, and its al that piece's aln (recomputed after a
+            // skipped piece)
+            float w_c = 0.f, al_c = 0.f;
+            bool carry = false;
             double2 kn_next = KN[min(3, NI)];
 
             for (int iv = 0; iv < NI; iv++) {
-                // -- lanes = energies: the piece's coefficients and class
+                // -- lanes = energies: the next knot and secant (the window slides always)
                 const int k2 = iv + 2;
                 const double2 kn_now = kn_next;
                 if (k2 + 2 <= NI) kn_next = KN[k2 + 2];   // read a step ahead
                 float s2, ynew = 0.f;
                 if (k2 < NI) {
                     ynew = knot(kn_now);
-                    s2 = (ynew - yk) / KH[k2].x;
+                    s2 = (ynew - yk) * KH[k2].y;
                 } else s2 = k2 == NI ? s0_first : s1_first;
-                const float h = KH[iv].x;
-                const float ca = ya;
-                float cb, cc, cd;
-                const float wn = fabsf(s2 - s1) + fabsf(s0 - sm1);
-                const float aln = wn != 0.f ? fabsf(s0 - sm1) / wn : 0.f;
-                if (w_c == 0.f) { cb = s0; cc = 0.f; cd = 0.f; }
-                else {
-                    const float tn = wn == 0.f ? s0 : (1.f - aln) * s0 + aln * s1;
-                    cb = (1.f - al_c) * sm1 + al_c * s0;
-                    cc = (3.f * s0 - 2.f * cb - tn) / h;
-                    cd = (cb + tn - 2.f * s0) / (h * h);
-                }
-                w_c = wn;
-                al_c = aln;
-                const int cls = correct ? 1 : piece_class(ca, cb, cc, cd, h);
-                float P[4];
-                const bool on = cls == 1 && e_ok;
-                P[0] = on ? ca : 0.f; P[1] = on ? cb : 0.f; P[2] = on ? cc : 0.f; P[3] = on ? cd : 0.f;
-                const unsigned long long nzA = __ballot(on && (ca != 0.f || cb != 0.f || cc != 0.f || cd != 0.f));
-                const unsigned long long mixA = __ballot(cls == 2 && e_ok);
+                const bool piece_live = LIVE[iv] != 0;
 
-                // -- lanes = phases: this interval's cells and their moments
+                // -- lanes = phases: this interval's cells (their moments if the piece can fold)
                 const double lb = PH[iv], ub = iv < NI - 1 ? PH[iv + 1] : __builtin_inf();
                 float M[4] = {0.f, 0.f, 0.f, 0.f};
                 const int c_first = cur;
                 int cnt = 0;
                 while (xn < ub) {
                     unsorted = unsorted || xn < lb || xn > hi;
-                    if (!isnan(an)) {
+                    if (piece_live && !isnan(an)) {
                         const float tq = (float)(xn - lb);
                         float at = an;
                         M[0] += at; at *= tq; M[1] += at; at *= tq; M[2] += at; at *= tq; M[3] += at;
@@ -319,42 +316,70 @@ __global__ void __launch_bounds__(64) k_wave_fold(Member* members, Layout L, Fol
                 const int c_last = cur;
                 const unsigned long long nzB = __ballot(cnt > 0);
 
-                if (nzA && nzB) {
-                    row_transpose(P);   // P[c]: A operand of energy chunk c
-                    row_transpose(M);   // M[t]: B operand of phase tile t
-#pragma unroll
-                    for (int c = 0; c < 4; c++)
-#pragma unroll
-                        for (int t = 0; t < 4; t++)
-                            acc[c][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(P[c], M[t], acc[c][t], 0, 0, 0);
-                    any = true;
-                }
+                const float wn = fabsf(s2 - s1) + fabsf(s0 - sm1);
+                if (piece_live && nzB) {
+                    // -- lanes = energies: the piece's coefficients and class
+                    if (!carry) {
+                        w_c = fabsf(s1 - s0) + fabsf(sm1 - sm2);
+                        al_c = w_c != 0.f ? fabsf(sm1 - sm2) / w_c : 0.f;
+                    }
+                    const float h = KH[iv].x, ih = KH[iv].y;
+                    const float ca = ya;
+                    float cb, cc, cd;
+                    const float aln = wn != 0.f ? fabsf(s0 - sm1) / wn : 0.f;
+                    if (w_c == 0.f) { cb = s0; cc = 0.f; cd = 0.f; }
+                    else {
+                        const float tn = wn == 0.f ? s0 : (1.f - aln) * s0 + aln * s1;
+                        cb = (1.f - al_c) * sm1 + al_c * s0;
+                        cc = (3.f * s0 - 2.f * cb - tn) * ih;
+                        cd = (cb + tn - 2.f * s0) * (ih * ih);
+                    }
+                    w_c = wn;
+                    al_c = aln;
+                    carry = true;
+                    const int cls = correct ? 1 : piece_class(ca, cb, cc, cd, h);
+                    float P[4];
+                    const bool on = cls == 1 && e_ok;
+                    P[0] = on ? ca : 0.f; P[1] = on ? cb : 0.f; P[2] = on ? cc : 0.f; P[3] = on ? cd : 0.f;
+                    const unsigned long long nzA =
+                        __ballot(on && (ca != 0.f || cb != 0.f || cc != 0.f || cd != 0.f));
+                    const unsigned long long mixA = __ballot(cls == 2 && e_ok);
 
-                // -- sign-changing pieces, point by point with the clipping
-                if (mixA && nzB) {
-                    unsigned long long mb = mixA;
-                    while (mb) {
-                        const int el = __ffsll((long long)mb) - 1;
-                        mb &= mb - 1ull;
-                        const float a = readlane_f(ca, el), b = readlane_f(cb, el), c = readlane_f(cc, el),
-                                    d = readlane_f(cd, el);
-                        const double sc = a_ref * readlane_d(esc, el);
-                        float sum = 0.f;
-                        if (cnt > 0) {
-                            for (int k = c_first; k < c_last; k++) {
-                                const int j = cell(k);
-                                const float ar = CAREA[j];
-                                if (isnan(ar)) continue;
-                                const float tq = (float)(xof(j) - lb);
-                                const float v = a + tq * (b + tq * (c + tq * d));
-                                if (v > 0.f) sum += ar * v;
+                    if (nzA) {
+                        row_transpose(P);   // P[c]: A operand of energy chunk c
+                        row_transpose(M);   // M[t]: B operand of phase tile t
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+#pragma unroll
+                            for (int t = 0; t < 4; t++)
+                                acc[c][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(P[c], M[t], acc[c][t], 0, 0, 0);
+                        any = true;
+                    }
+
+                    // -- sign-changing pieces, point by point with the clipping
+                    if (mixA) {
+                        unsigned long long mb = mixA;
+                        while (mb) {
+                            const int el = __ffsll((long long)mb) - 1;
+                            mb &= mb - 1ull;
+                            const float a = readlane_f(ca, el), b = readlane_f(cb, el), c = readlane_f(cc, el),
+                                        d = readlane_f(cd, el);
+                            const double sc = a_ref * readlane_d(esc, el);
+                            float sum = 0.f;
+                            if (cnt > 0) {
+                                for (int k = c_first; k < c_last; k++) {
+                                    const int j = cell(k);
+                                    const float ar = CAREA[j];
+                                    if (isnan(ar)) continue;
+                                    const float tq = (float)(xof(j) - lb);
+                                    const float v = a + tq * (b + tq * (c + tq * d));
+                                    if (v > 0.f) sum += ar * v;
+                                }
                             }
-                        }
-                        if (sum != 0.f && p_ok) {
-                            unsafeAtomicAdd(flux + (size_t)(e0 + el) * N_P + p, (double)sum * sc);
+                            if (sum != 0.f && p_ok) unsafeAtomicAdd(flux + (size_t)(e0 + el) * N_P + p, (double)sum * sc);
                         }
                     }
-                }
+                } else carry = false;
                 // slide the window
                 ya = yb; yb = yk; yk = ynew;
                 sm2 = sm1; sm1 = s0; s0 = s1; s1 = s2;
