@@ -28,11 +28,16 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 FP32_MFMA_PEAK_TFLOPS = 157.3
 KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold", "waves": "k_wave_fold"}
-# HBM bytes of k_moment_fold<float, 1, 2, 2, 4> on the default batched workload,
-# from rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0, factors from a
-# 512 MiB copy) over the three launches of one step (KiB): 256, 128 and 64
-# energies.  profiles/moment_fold_batched_default.md holds the runs.
-FOLD_TRAFFIC_KIB = {256: (17123810, 20011418), 128: (9762315, 9300707), 64: (6796957, 5185836)}
+# Bytes per fold launch on the default batched workload (4096 prior draws,
+# precision form) from rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0;
+# factors from a 512 MiB device copy), KiB per launch for the 256-, 128- and
+# 64-energy launches of one step, per autotuned fold (profiles/wave_fold_batched_default.md,
+# profiles/moment_fold_batched_default.md hold the runs).
+# The wave fold's writes are its FP64 atomic additions into the flux.
+FOLD_TRAFFIC_KIB = {
+    "waves/1": {256: (113018, 7930018), 128: (102858, 3573981), 64: (110550, 1910209)},
+    "moment/2": {256: (17123810, 20011418), 128: (9762315, 9300707), 64: (6796957, 5185836)},
+}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -192,9 +197,11 @@ def main():
     # measured bytes per fold launch (mean of the step's launches), for the
     # default workload and shape only
     traffic = None
-    if (args.n == 4096 and args.precision == "mixed" and sorted(groups) == sorted(FOLD_TRAFFIC_KIB)
-            and all(t["chosen"] == "moment/2" for t in tuned.values())):
-        traffic = sum(2.0 * f + w for f, w in FOLD_TRAFFIC_KIB.values()) * 1024.0 / len(FOLD_TRAFFIC_KIB)
+    chosen = {t["chosen"] for t in tuned.values()}
+    if args.n == 4096 and args.precision == "mixed" and len(chosen) == 1:
+        tab = FOLD_TRAFFIC_KIB.get(chosen.pop())
+        if tab is not None and sorted(groups) == sorted(tab):
+            traffic = sum(2.0 * f + w for f, w in tab.values()) * 1024.0 / len(tab)
     achieved_overlapped = flops / (stage["profile_fold"] * 1e-3) / 1e12
     n_eval = len(acc)
     value = n_eval * args.steps / dt
@@ -294,7 +301,9 @@ def main():
                      "evals_per_step": evals,
                      "flop_per_eval": 8,
                      "measured_on": "one-stream pass of the timed step (HIP events on the fold's stream)",
-                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE; mostly register spills)",
+                     "traffic_unit": "bytes per launch, mean over the step's launches (rocprofv3 FETCH_SIZE x 2 + "
+                                     "WRITE_SIZE, scaled by a 512 MiB copy; the wave fold's writes are its FP64 "
+                                     "atomic flux additions, the block fold's mostly register spills)",
                      "achieved_overlapped": achieved_overlapped},
         "autotune_fold_tiles": tuned,
         "device": dev,
