@@ -582,9 +582,34 @@ extern "C" int xh_allocate_cells(int num_cells, int min_sqrt_num_cells, int max_
     return XH_OK;
 }
 
+// The cell mesh after the area table: parallels' limits, cell kinds and the
+// boundary-cell list, then the boundary cells (k_ring_bounds, k_mesh_classify,
+// k_mesh_cells); blist holds members x S x S entries, every cell's worst case.
+static hipError_t launch_mesh_cells(const Member* m, const double* tab, const double* tot, int table_n, Layout L,
+                                    int nm, int S, double* theta, double* phi, double* r, double* cellArea,
+                                    double* maxAlpha, double* cos_gamma, double* effGrav, double* ringlim,
+                                    int2* blist, int* bcount, hipStream_t s)
+{
+    hipError_t e = fill_int(bcount, 1, 0, s);
+    if (e != hipSuccess) return e;
+    const dim3 g((unsigned)((S + 3) / 4), (unsigned)nm);
+    hipLaunchKernelGGL(k_ring_bounds, g, dim3(256), 0, s, m, tab, tot, table_n, L, theta, r, maxAlpha, cos_gamma,
+                       effGrav, ringlim);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mesh_classify, g, dim3(256), 0, s, m, tot, L, (const double*)ringlim, phi, cellArea, blist,
+                       bcount);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const unsigned nb = (unsigned)std::max(1, std::min(2048, nm * S / 8));
+    hipLaunchKernelGGL(k_mesh_cells, dim3(nb), dim3(256), 0, s, m, tot, L, (const double*)ringlim, cellArea,
+                       (const int2*)blist, (const int*)bcount);
+    return hipGetLastError();
+}
+
 extern "C" size_t xh_cell_mesh_workspace_bytes(int S)
 {
-    return 256 + (1000 + 64) * 8 + 2 * (size_t)S * 8 + 256;
+    // member, area table, totals, colatitudes, azimuths, parallel limits,
+    // boundary-cell list and its count
+    return 256 + (1000 + 64) * 8 + 2 * (size_t)S * 8 + (size_t)S * 3 * 8 + (size_t)S * S * 8 + 256 + 256;
 }
 
 extern "C" int xh_construct_cell_mesh(int polar, double numCell, int S, double mass, double r_s, double R_eq,
@@ -605,7 +630,10 @@ extern "C" int xh_construct_cell_mesh(int polar, double numCell, int S, double m
     double* d_tab = (double*)w; w += (size_t)table_n * 8;
     double* d_total = (double*)w; w += 64 * 8;
     double* d_th = (double*)w; w += (size_t)S * 8;
-    double* d_ph = (double*)w;
+    double* d_ph = (double*)w; w += (size_t)S * 8;
+    double* d_rl = (double*)w; w += (size_t)S * 3 * 8;
+    int* d_bc = (int*)w; w += 256;
+    int2* d_bl = (int2*)w;
     Member hm{};
     hm.R_eq = R_eq; hm.epsilon = epsilon; hm.zeta = zeta; hm.r_s = r_s; hm.shape = star_shape_ind; hm.active = 1;
     hm.cedeR = cedeRadius; hm.cedeColat = cedeColatitude; hm.superR = superRadius; hm.superColat = superColatitude;
@@ -616,10 +644,8 @@ extern "C" int xh_construct_cell_mesh(int polar, double numCell, int S, double m
     hipLaunchKernelGGL(k_mesh_table, dim3((table_n + 1 + 3) / 4, 1), dim3(256), 0, s, (const Member*)d_m, d_tab,
                        d_total, table_n);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mesh_rings, dim3((S + 1) / 2, 1), dim3(256), 0, s, (const Member*)d_m,
-                       (const double*)d_tab, (const double*)d_total, table_n, L, d_th, d_ph, r, cellArea,
-                       maxEmissionAngle, cos_gamma, effGrav);
-    HIPCHK(hipGetLastError());
+    HIPCHK(launch_mesh_cells(d_m, d_tab, d_total, table_n, L, 1, S, d_th, d_ph, r, cellArea, maxEmissionAngle,
+                             cos_gamma, effGrav, d_rl, d_bl, d_bc, s));
     // the reference's [S, S] colatitude and azimuth arrays
     const unsigned nb = (unsigned)(((size_t)S * S + 255) / 256);
     hipLaunchKernelGGL(k_spread, dim3(nb), dim3(256), 0, s, theta, (const double*)d_th, S, S, 1);
@@ -830,6 +856,9 @@ struct xh_pipeline {
     Member* d_members;
     RegionAlloc* d_regions;
     double *d_area_tab, *d_total, *d_theta, *d_phi, *d_r, *d_cellArea, *d_maxAlpha, *d_cos_gamma, *d_effGrav;
+    double* d_ringlim;           // per parallel (lower, upper, centroid) colatitude
+    int2* d_blist;               // boundary cells of the batch (member, ring << 16 | column)
+    int* d_bcount;
     double *d_u, *d_logT, *d_maxDefl, *d_defl, *d_cosa, *d_lag, *d_geo, *d_flux;
     int *d_ring_ok, *d_valid, *d_member_index, *d_status, *d_lastq;
     int *d_raywork, *d_raycount, *d_else_raywork;   // rays left to the full QAG, and their counts
@@ -990,6 +1019,9 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     PCHK(dalloc(p, &p->d_phi, nm * S));
     PCHK(dalloc(p, &p->d_r, nm * S));
     PCHK(dalloc(p, &p->d_cellArea, nm * S * S));
+    PCHK(dalloc(p, &p->d_ringlim, nm * S * 3));
+    PCHK(dalloc(p, &p->d_blist, nm * S * S));
+    PCHK(dalloc(p, &p->d_bcount, 1));
     PCHK(dalloc(p, &p->d_maxAlpha, nm * S));
     PCHK(dalloc(p, &p->d_cos_gamma, nm * S));
     PCHK(dalloc(p, &p->d_effGrav, nm * S));
@@ -1299,10 +1331,9 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     hipLaunchKernelGGL(k_mesh_table, dim3((p->table_n + 1 + 3) / 4, nm), dim3(256), 0, s, p->d_members,
                        p->d_area_tab, p->d_total, p->table_n);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mesh_rings, dim3((S + 1) / 2, nm), dim3(256), 0, s, p->d_members, p->d_area_tab, p->d_total,
-                       p->table_n, L, p->d_theta, p->d_phi, p->d_r, p->d_cellArea, p->d_maxAlpha,
-                       p->d_cos_gamma, p->d_effGrav);
-    HIPCHK(hipGetLastError());
+    HIPCHK(launch_mesh_cells(p->d_members, p->d_area_tab, p->d_total, p->table_n, L, (int)nm, S, p->d_theta,
+                             p->d_phi, p->d_r, p->d_cellArea, p->d_maxAlpha, p->d_cos_gamma, p->d_effGrav,
+                             p->d_ringlim, p->d_blist, p->d_bcount, s));
     hipLaunchKernelGGL(k_ring_finalize, dim3(nm), dim3(256), 0, s, p->d_members, L, p->d_r, p->d_cellArea,
                        p->d_u, p->d_ring_ok, p->d_logT);
     HIPCHK(hipGetLastError());

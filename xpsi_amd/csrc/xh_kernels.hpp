@@ -46,9 +46,13 @@ struct MarginalArgs {
 
 __global__ void k_allocate(const RegionAlloc* regions, Member* members, int S_max);
 __global__ void k_mesh_table(const Member* members, double* area_tab, double* total, int table_n);
-__global__ void k_mesh_rings(const Member* members, const double* area_tab, const double* total,
-                             int table_n, Layout L, double* theta, double* phi, double* rr,
-                             double* cellArea, double* maxAlpha, double* cos_gamma, double* effGrav);
+__global__ void k_ring_bounds(const Member* members, const double* area_tab, const double* total, int table_n,
+                              Layout L, double* theta, double* rr, double* maxAlpha, double* cos_gamma,
+                              double* effGrav, double* ringlim);
+__global__ void k_mesh_classify(const Member* members, const double* total, Layout L, const double* ringlim,
+                                double* phi, double* cellArea, int2* blist, int* bcount);
+__global__ void k_mesh_cells(const Member* members, const double* total, Layout L, const double* ringlim,
+                             double* cellArea, const int2* blist, const int* bcount);
 __global__ void k_rays_lane(const Member* members, Layout L, const double* rr, const double* maxAlpha_in,
                             double* cos_alpha, double* defl_raw, double* lag_raw, int* work, int* nwork,
                             const double* uu);

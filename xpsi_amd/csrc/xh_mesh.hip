@@ -449,29 +449,35 @@ __global__ void __launch_bounds__(256) k_mesh_table(const Member* __restrict__ m
     }
 }
 
-// One block of four waves per RPB (two) rings of a member: wave w < RPB finds
-// the parallels and the centroid colatitude of ring RPB b + w, all threads
-// classify the rings' cells, then the four waves integrate the rings' boundary
-// cells between them.  The phases are latency bound (serial FP64 work on one
-// wave), so rings share a block and the register budget is capped for three
-// waves per SIMD (with the integrals out of line).
-__global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict__ members,
-                                                    const double* __restrict__ area_tab,
-                                                    const double* __restrict__ total, int table_n,
-                                                    Layout L, double* __restrict__ theta,
-                                                    double* __restrict__ phi, double* __restrict__ rr,
-                                                    double* __restrict__ cellArea,
-                                                    double* __restrict__ maxAlpha,
-                                                    double* __restrict__ cos_gamma,
-                                                    double* __restrict__ effGrav)
+// The cell mesh of a member, in three launches:
+//   k_ring_bounds   one wavefront per parallel: its bounding colatitudes from
+//                   the equal-area table (Steffen inversion) and its centroid
+//                   colatitude (an area-weighted integral), and the per-ring
+//                   outputs;
+//   k_mesh_classify one wavefront per parallel, lanes over its cells: each
+//                   cell's kind from its corners and centre (mesh.pyx's rule),
+//                   full and empty cells written, boundary cells appended to
+//                   a list;
+//   k_mesh_cells    wavefronts in a grid-stride loop over the list: each
+//                   boundary cell's area by the split QAG integral.
+// Splitting the parallels' set-up from the cells keeps every wavefront busy
+// (the set-up is serial FP64 work of one wavefront per parallel), and the list
+// spreads the boundary cells, a few per parallel on most and a whole row on
+// the parallels at the regions' top and bottom, evenly over the chip.
+
+// (lower, upper, centroid) colatitudes of each parallel: ringlim[(member * S_max + ring) * 3 + k]
+__global__ void __launch_bounds__(256) k_ring_bounds(const Member* __restrict__ members,
+                                                     const double* __restrict__ area_tab,
+                                                     const double* __restrict__ total, int table_n, Layout L,
+                                                     double* __restrict__ theta, double* __restrict__ rr,
+                                                     double* __restrict__ maxAlpha, double* __restrict__ cos_gamma,
+                                                     double* __restrict__ effGrav, double* __restrict__ ringlim)
 {
-    constexpr int RPB = 2;                  // rings per block
-    const int mi = blockIdx.y, i0 = blockIdx.x * RPB, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int mi = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
+    const int i = blockIdx.x * 4 + (tid >> 6);
     const Member m = members[mi];
-    if (!m.active || m.status) return;
+    if (!m.active || m.status || blockIdx.x * 4 >= m.S) return;
     const int S = m.S;
-    if (i0 >= S) return;
-    const int nr = min(RPB, S - i0);
     double lo, hi, colat, sColat, dphi, bphi;
     int inv;
     member_lims(m, lo, hi, colat, sColat, inv, dphi, bphi);
@@ -480,41 +486,81 @@ __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict_
     const double eta = cell / dphi;
     const double* at = area_tab + (size_t)mi * table_n;
     __shared__ double ct[1024];
-    __shared__ double sh_lower[RPB], sh_upper[RPB], sh_cc[RPB];
-    __shared__ int nb;
-    __shared__ int bj[2048];                // boundary cells: (ring slot << 16) | j
     for (int k = tid; k < table_n; k += 256) ct[k] = lin(hi, lo, table_n, k);
-    if (tid == 0) nb = 0;
     __syncthreads();
-    const size_t ro = (size_t)mi * L.S_max, co = (size_t)mi * L.S_max * L.S_max;
-    if (wave < nr) {
-        const int i = i0 + wave;
-        double i_eta = (double)S * eta, e_lo = 0.0, e_hi = 0.0;
-        for (int k = 0; k <= i && k < S - 1; k++) {
-            i_eta -= eta;
-            if (k == i - 1) e_lo = i_eta;
-            if (k == i) e_hi = i_eta;
-        }
-        double lower = (i == 0) ? lo : steffen_eval(at, ct, table_n, e_lo, nullptr);
-        double upper = (i == S - 1) ? hi : steffen_eval(at, ct, table_n, e_hi, nullptr);
-        double cc = wave_area(lower, upper, m.R_eq, m.epsilon, m.zeta, m.shape, 1) / eta;
-        if (lane == 0) {
-            sh_lower[wave] = lower; sh_upper[wave] = upper; sh_cc[wave] = cc;
-            double mu = cos(cc);
-            double rn = radius_norm(mu, m.epsilon, m.zeta, m.shape);
-            double fth = f_theta(mu, rn, m.epsilon, m.zeta, m.shape);
-            double cg = 1.0 / sqrt(1.0 + fth * fth);
-            rr[ro + i] = rn * m.R_eq;
-            effGrav[ro + i] = effective_gravity(mu, m.R_eq, m.zeta, m.epsilon, m.shape);
-            cos_gamma[ro + i] = cg;
-            maxAlpha[ro + i] = HALF_PI + acos(cg);
-            theta[ro + i] = inv ? PI - cc : cc;
-        }
+    if (i >= S) return;
+    const size_t ro = (size_t)mi * L.S_max;
+    // the reference subtracts eta ring by ring
+    double i_eta = (double)S * eta, e_lo = 0.0, e_hi = 0.0;
+    for (int k = 0; k <= i && k < S - 1; k++) {
+        i_eta -= eta;
+        if (k == i - 1) e_lo = i_eta;
+        if (k == i) e_hi = i_eta;
     }
-    __syncthreads();
+    const double lower = (i == 0) ? lo : steffen_eval(at, ct, table_n, e_lo, nullptr);
+    const double upper = (i == S - 1) ? hi : steffen_eval(at, ct, table_n, e_hi, nullptr);
+    const double cc = wave_area(lower, upper, m.R_eq, m.epsilon, m.zeta, m.shape, 1) / eta;
+    if (lane == 0) {
+        double* rl = ringlim + (ro + i) * 3;
+        rl[0] = lower; rl[1] = upper; rl[2] = cc;
+        double mu = cos(cc);
+        double rn = radius_norm(mu, m.epsilon, m.zeta, m.shape);
+        double fth = f_theta(mu, rn, m.epsilon, m.zeta, m.shape);
+        double cg = 1.0 / sqrt(1.0 + fth * fth);
+        rr[ro + i] = rn * m.R_eq;
+        effGrav[ro + i] = effective_gravity(mu, m.R_eq, m.zeta, m.epsilon, m.shape);
+        cos_gamma[ro + i] = cg;
+        maxAlpha[ro + i] = HALF_PI + acos(cg);
+        theta[ro + i] = inv ? PI - cc : cc;
+    }
+}
+
+// psi_of(th, ph, TH) <= R as the cosine of psi against cos R: the same
+// argument psi_of forms, and its NaN (an argument beyond [-1, 1]) counts as
+// outside, as the comparison with a NaN angle does
+__device__ __forceinline__ bool within(double cTH, double sTH, double cth, double sth, double cph, double cR)
+{
+    const double arg = cTH * cth + sTH * sth * cph;
+    return arg <= 1.0 && arg >= cR;
+}
+__device__ __forceinline__ bool beyond(double cTH, double sTH, double cth, double sth, double cph, double cR)
+{
+    const double arg = cTH * cth + sTH * sth * cph;
+    return arg >= -1.0 && arg <= 1.0 && arg <= cR;
+}
+
+// azimuth of the left edge of column j: the reference accumulates
+// left += delta_phi cell by cell
+__device__ __forceinline__ double left_edge(double first_left, double dphi, int j)
+{
+    double left = first_left;
+    for (int q = 0; q < j; q++) left = left + dphi;
+    return left;
+}
+
+// one wavefront per parallel, lanes over its cells; blist entries (member,
+// ring << 16 | column)
+__global__ void __launch_bounds__(256) k_mesh_classify(const Member* __restrict__ members,
+                                                       const double* __restrict__ total, Layout L,
+                                                       const double* __restrict__ ringlim,
+                                                       double* __restrict__ phi, double* __restrict__ cellArea,
+                                                       int2* __restrict__ blist, int* __restrict__ bcount)
+{
+    const int mi = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
+    const int i = blockIdx.x * 4 + (tid >> 6);
+    const Member m = members[mi];
+    if (!m.active || m.status) return;
+    const int S = m.S;
+    if (i >= S) return;
+    double lo, hi, colat, sColat, dphi, bphi;
+    int inv;
+    member_lims(m, lo, hi, colat, sColat, inv, dphi, bphi);
+    double cell = (m.polar ? TWO_PI : 2.0 * bphi) * total[mi];
+    cell /= m.numCell;
+    const size_t ro = (size_t)mi * L.S_max, co = (size_t)mi * L.S_max * L.S_max;
     const double first_left = m.polar ? -PI : -bphi;
-    if (i0 == 0) {
-        for (int j = tid; j < S; j += 256) {
+    if (i == 0) {
+        for (int j = lane; j < S; j += 64) {
             double p0 = m.polar ? lin(-PI + 0.5 * dphi, PI - 0.5 * dphi, S, j)
                                 : lin(-bphi + 0.5 * dphi, bphi - 0.5 * dphi, S, j);
             double v = p0 + m.phi_offset;
@@ -522,6 +568,8 @@ __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict_
             phi[ro + j] = v;
         }
     }
+    const double* rl = ringlim + (ro + i) * 3;
+    const double lower = rl[0], upper = rl[1], cc = rl[2];
     const double sR = m.superR, sAzi = m.superAzi, cR = m.cedeR;
     double s1a = 0, s1b = 0, s2a = 0, s2b = 0, s3a = 0, s3b = 0, s4a = 0, s4b = 0;
     if (sR > 0.0) {
@@ -542,61 +590,84 @@ __global__ void __launch_bounds__(256, 3) k_mesh_rings(const Member* __restrict_
     }
     const bool mirror = eq12(sColat, colat) && eq12(sAzi, 0.0);
     const int j_max = mirror ? S / 2 : S;
-    // the reference accumulates left += delta_phi cell by cell
-    auto left_of = [&](int j) {
-        double left = first_left;
-        for (int q = 0; q < j; q++) left = left + dphi;
-        return left;
-    };
-    for (int q = tid; q < nr * j_max; q += 256) {
-        const int r = q / j_max, j = q - r * j_max, i = i0 + r;
-        const double lower = sh_lower[r], upper = sh_upper[r], cc = sh_cc[r];
-        double* ca = cellArea + co + (size_t)i * S;
-        double left = left_of(j);
-        double right = left + dphi, mid = left + 0.5 * dphi;
-        int p1, p2, p3, p4, p5, kind = 0;
-        if (m.polar && i == 0) {
-            p1 = p3 = (colat <= cR);
-            if (p1 == 1) { p1 = (sR <= sColat); p3 = p1; } else p1 = p3 = 2;
-        } else {
-            p1 = (psi_of(lower, left, colat) <= cR);
-            p3 = (psi_of(lower, right, colat) <= cR);
-            p1 = p1 == 1 ? (sR <= psi_of(lower, left - sAzi, sColat)) : 2;
-            p3 = p3 == 1 ? (sR <= psi_of(lower, right - sAzi, sColat)) : 2;
+    // the parallel's and the circles' sines and cosines, once
+    const double cLo = cos(lower), sLo = sin(lower), cUp = cos(upper), sUp = sin(upper);
+    const double cCc = cos(cc), sCc = sin(cc);
+    const double cCo = cos(colat), sCo = sin(colat), cSu = cos(sColat), sSu = sin(sColat);
+    const double ccR = cos(cR), csR = cos(sR);
+    double* ca = cellArea + co + (size_t)i * S;
+    for (int j0 = 0; j0 < j_max; j0 += 64) {
+        const int j = j0 + lane;
+        int kind = 0;
+        if (j < j_max) {
+            const double left = left_edge(first_left, dphi, j);
+            const double right = left + dphi, mid = left + 0.5 * dphi;
+            const double cl = cos(left), cr = cos(right), cm = cos(mid);
+            const double cls = cos(left - sAzi), crs = cos(right - sAzi), cms = cos(mid - sAzi);
+            int p1, p2, p3, p4, p5;
+            if (m.polar && i == 0) {
+                p1 = p3 = (colat <= cR);
+                if (p1 == 1) { p1 = (sR <= sColat); p3 = p1; } else p1 = p3 = 2;
+            } else {
+                p1 = within(cCo, sCo, cLo, sLo, cl, ccR);
+                p3 = within(cCo, sCo, cLo, sLo, cr, ccR);
+                p1 = p1 == 1 ? beyond(cSu, sSu, cLo, sLo, cls, csR) : 2;
+                p3 = p3 == 1 ? beyond(cSu, sSu, cLo, sLo, crs, csR) : 2;
+            }
+            p2 = within(cCo, sCo, cUp, sUp, cl, ccR);
+            p4 = within(cCo, sCo, cUp, sUp, cr, ccR);
+            p5 = within(cCo, sCo, cCc, sCc, cm, ccR);
+            p2 = p2 == 1 ? beyond(cSu, sSu, cUp, sUp, cls, csR) : 2;
+            p4 = p4 == 1 ? beyond(cSu, sSu, cUp, sUp, crs, csR) : 2;
+            p5 = p5 == 1 ? beyond(cSu, sSu, cCc, sCc, cms, csR) : 2;
+            if (!(p1 == p2 && p2 == p3 && p3 == p4 && p4 == p5)) kind = 2;
+            else if (sR > 0.0) {
+                auto inside = [&](double a, double b) {
+                    return (lower <= a && a <= upper) && (left <= b && b <= right);
+                };
+                if (inside(s1a, s1b) || inside(s2a, s2b) || inside(s3a, s3b) || inside(s4a, s4b)) kind = 2;
+                else if (p5 == 1) kind = 1;
+            } else if (p5 == 1) kind = 1;
+            const double v = kind == 1 ? cell : 0.0;
+            ca[j] = v;
+            if (mirror) ca[S - 1 - j] = v;
         }
-        p2 = (psi_of(upper, left, colat) <= cR);
-        p4 = (psi_of(upper, right, colat) <= cR);
-        p5 = (psi_of(cc, mid, colat) <= cR);
-        p2 = p2 == 1 ? (sR <= psi_of(upper, left - sAzi, sColat)) : 2;
-        p4 = p4 == 1 ? (sR <= psi_of(upper, right - sAzi, sColat)) : 2;
-        p5 = p5 == 1 ? (sR <= psi_of(cc, mid - sAzi, sColat)) : 2;
-        if (!(p1 == p2 && p2 == p3 && p3 == p4 && p4 == p5)) kind = 2;
-        else if (sR > 0.0) {
-            auto inside = [&](double a, double b) {
-                return (lower <= a && a <= upper) && (left <= b && b <= right);
-            };
-            if (inside(s1a, s1b) || inside(s2a, s2b) || inside(s3a, s3b) || inside(s4a, s4b)) kind = 2;
-            else if (p5 == 1) kind = 1;
-        } else if (p5 == 1) kind = 1;
-        double v = kind == 1 ? cell : 0.0;
-        ca[j] = v;
-        if (mirror) ca[S - 1 - j] = v;
-        if (kind == 2) {
-            int slot = atomicAdd(&nb, 1);
-            if (slot < 2048) bj[slot] = (r << 16) | j;
+        // boundary cells onto the list, one atomic per wavefront
+        const unsigned long long bm = __ballot(kind == 2);
+        if (bm) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(bcount, __popcll(bm));
+            base = __shfl(base, 0, 64);
+            if (kind == 2) blist[base + __popcll(bm & ((1ull << lane) - 1ull))] = make_int2(mi, (i << 16) | j);
         }
     }
-    __syncthreads();
-    const int nbc = min(nb, 2048);
-    for (int q = wave; q < nbc; q += 4) {
-        const int r = bj[q] >> 16, j = bj[q] & 0xffff, i = i0 + r;
-        const double lower = sh_lower[r], upper = sh_upper[r];
-        double* ca = cellArea + co + (size_t)i * S;
-        double left = left_of(j);
-        double right = left + dphi;
-        double v = wave_cell(lower, upper, left, right, m.R_eq, m.epsilon, m.zeta, m.shape, colat, cR, sR, sAzi,
-                             sColat);
+}
+
+// wavefronts over the boundary-cell list (grid-stride): each cell's area
+__global__ void __launch_bounds__(256) k_mesh_cells(const Member* __restrict__ members, const double* __restrict__ total,
+                                                    Layout L, const double* __restrict__ ringlim,
+                                                    double* __restrict__ cellArea, const int2* __restrict__ blist,
+                                                    const int* __restrict__ bcount)
+{
+    const int lane = threadIdx.x & 63;
+    const int nw = gridDim.x * 4;
+    const int n = *bcount;
+    for (int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < n; q += nw) {
+        const int2 e = blist[q];
+        const int mi = e.x, i = e.y >> 16, j = e.y & 0xffff;
+        const Member m = members[mi];
+        const int S = m.S;
+        double lo, hi, colat, sColat, dphi, bphi;
+        int inv;
+        member_lims(m, lo, hi, colat, sColat, inv, dphi, bphi);
+        const double* rl = ringlim + ((size_t)mi * L.S_max + i) * 3;
+        const double first_left = m.polar ? -PI : -bphi;
+        const double left = left_edge(first_left, dphi, j), right = left + dphi;
+        const double v = wave_cell(rl[0], rl[1], left, right, m.R_eq, m.epsilon, m.zeta, m.shape, colat, m.cedeR,
+                                   m.superR, m.superAzi, sColat);
+        const bool mirror = eq12(sColat, colat) && eq12(m.superAzi, 0.0);
         if (lane == 0) {
+            double* ca = cellArea + (size_t)mi * L.S_max * L.S_max + (size_t)i * S;
             ca[j] = v;
             if (mirror) ca[S - 1 - j] = v;
         }

@@ -187,8 +187,8 @@ extern "C" xh_server* xh_server_create(const xh_model_config* cfg, int n_slots, 
         s->members[g] = cfg->region[g].cede ? 2 : 1;
         s->snc[g] = cfg->region[g].sqrt_num_cells;
     }
-    s->per_cell = 8.0;                                          // cell areas
-    s->per_side = 8.0 * (10.0 + 3.0 * NR + 2.0 * 5.0 * NL) + 4.0 * 2.0 * 2.0;
+    s->per_cell = 8.0 + 8.0;                                    // cell areas, boundary-cell list
+    s->per_side = 8.0 * (10.0 + 3.0 + 3.0 * NR + 2.0 * 5.0 * NL) + 4.0 * 2.0 * 2.0;
     s->per_member = 8.0 * NE * NP + 256.0;                      // flux, descriptor
     s->per_request = 8.0 * (cfg->n_regions * ((double)cfg->N_in + cfg->N_ch) * NP) +
                      8.0 * (double)cfg->N_ch * (cfg->N_b + 4.0) + 4.0 * cfg->N_ch;
