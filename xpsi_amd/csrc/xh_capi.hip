@@ -641,7 +641,7 @@ extern "C" int xh_construct_cell_mesh(int polar, double numCell, int S, double m
     HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
     HIPCHK(fill(cellArea, (size_t)S * S, 0.0, s));
     Layout L{S, 3, 0, 0, 0, 8, 0, 1};
-    hipLaunchKernelGGL(k_mesh_table, dim3((table_n + 1 + 3) / 4, 1), dim3(256), 0, s, (const Member*)d_m, d_tab,
+    hipLaunchKernelGGL(k_mesh_table, dim3(1), dim3(1024), 0, s, (const Member*)d_m, d_tab,
                        d_total, table_n);
     HIPCHK(hipGetLastError());
     HIPCHK(launch_mesh_cells(d_m, d_tab, d_total, table_n, L, 1, S, d_th, d_ph, r, cellArea, maxEmissionAngle,
@@ -1328,7 +1328,7 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     if (p->timing) HIPCHK(hipEventRecord(p->ev[0], s));
     hipLaunchKernelGGL(k_allocate, dim3(ng), dim3(256), 0, s, p->d_regions, p->d_members, S);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mesh_table, dim3((p->table_n + 1 + 3) / 4, nm), dim3(256), 0, s, p->d_members,
+    hipLaunchKernelGGL(k_mesh_table, dim3(nm), dim3(1024), 0, s, p->d_members,
                        p->d_area_tab, p->d_total, p->table_n);
     HIPCHK(hipGetLastError());
     HIPCHK(launch_mesh_cells(p->d_members, p->d_area_tab, p->d_total, p->table_n, L, (int)nm, S, p->d_theta,

@@ -426,27 +426,50 @@ __device__ __forceinline__ void member_lims(const Member& m, double& lo, double&
     }
 }
 
-// Area table: wave q integrates the area between table colatitude q-1 and the
-// far boundary; wave 0 integrates the whole region (total cell area).
-__global__ void __launch_bounds__(256) k_mesh_table(const Member* __restrict__ members,
-                                                    double* __restrict__ area_tab,
-                                                    double* __restrict__ total, int table_n)
+// Area table (mesh.pyx:79-115): entry q is the area from table colatitude q
+// (linspace(hi, lo, table_n)) to the far boundary hi, and the total is the
+// area over [lo, hi].  The reference integrates each entry separately; here
+// entry q is the running sum of the integrals over the table's intervals
+// [ct_q, ct_(q-1)], each under (hi - lo) / 999 wide, where the smooth area
+// integrand makes a 7-point Gauss-Legendre rule exact to rounding.  One block
+// per member: a lane per interval, then a block-wide inclusive scan.
+__global__ void __launch_bounds__(1024) k_mesh_table(const Member* __restrict__ members,
+                                                     double* __restrict__ area_tab,
+                                                     double* __restrict__ total, int table_n)
 {
-    const Member m = members[blockIdx.y];
+    // Gauss-Legendre, 7 points on [-1, 1] (abscissae and weights to 16 digits)
+    constexpr double GX[4] = {0.0, 0.4058451513773971669, 0.7415311855993944399, 0.9491079123427585245};
+    constexpr double GW[4] = {0.4179591836734693878, 0.3818300505051189449, 0.2797053914892766679,
+                              0.1294849661688696933};
+    const int mi = blockIdx.x, tid = threadIdx.x;
+    const Member m = members[mi];
     if (!m.active || m.status) return;
-    int q = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (q > table_n) return;
     double lo, hi, colat, sc, dphi, bphi;
     int inv;
     member_lims(m, lo, hi, colat, sc, inv, dphi, bphi);
-    if (q == 0) {
-        double v = wave_area(m.polar ? 0.0 : lo, hi, m.R_eq, m.epsilon, m.zeta, m.shape, 0);
-        if ((threadIdx.x & 63) == 0) total[blockIdx.y] = v;
-    } else {
-        double ct = lin(hi, lo, table_n, q - 1);
-        double v = wave_area(ct, hi, m.R_eq, m.epsilon, m.zeta, m.shape, 0);
-        if ((threadIdx.x & 63) == 0) area_tab[(size_t)blockIdx.y * table_n + q - 1] = v;
+    AreaFn f{m.epsilon, m.zeta, 0, m.shape};
+    __shared__ double part[1024];
+    double v = 0.0;
+    if (tid >= 1 && tid < table_n) {
+        const double a = lin(hi, lo, table_n, tid), b = lin(hi, lo, table_n, tid - 1);
+        const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+        double sum = GW[0] * f(c);
+#pragma unroll
+        for (int k = 1; k < 4; k++) sum += GW[k] * (f(c - h * GX[k]) + f(c + h * GX[k]));
+        v = sum * h;
     }
+    part[tid] = v;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele)
+    for (int o = 1; o < 1024; o <<= 1) {
+        const double add = tid >= o ? part[tid - o] : 0.0;
+        __syncthreads();
+        part[tid] += add;
+        __syncthreads();
+    }
+    const double R2 = m.R_eq * m.R_eq;   // as wave_area scales its integral
+    if (tid < table_n) area_tab[(size_t)mi * table_n + tid] = R2 * part[tid];
+    if (tid == 0) total[mi] = R2 * part[table_n - 1];
 }
 
 // The cell mesh of a member, in three launches:
