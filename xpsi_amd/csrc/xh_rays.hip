@@ -184,11 +184,17 @@ __device__ __forceinline__ double clamped_max_alpha(double maxA, double u)
 
 // ---------------------------------------------------------------- lane per ray
 // The same 61-point Gauss-Kronrod pass as wave_gk61_pair, summed by one lane
-// over the nodes (uniform node and weight loads, no cross-lane reductions);
-// the mean-deviation term takes a second sweep over the nodes.
+// over the nodes (uniform node and weight loads, no cross-lane reductions), in
+// one sweep.  QUADPACK's resasc, sum w_k |f_k - m| with m the Kronrod mean, is
+// known only after the sweep; the sweep sums w_k |f_k - f_c| against the
+// centre value instead, and since the weights sum to 2 the true resasc lies
+// within 2 |m - f_c| of that (times the half-length).  resasc enters only
+// QAG's error estimate, never the result, so the pass's acceptance is decided
+// over that whole range and a ray whose decision the range leaves open goes
+// to the full QAG (k_rays).
 template <class FB>
-__device__ void lane_gk61_pair(const FB& f, double a, double b, double res[2], double err[2], double rabs[2],
-                               double rasc[2], unsigned* __restrict__ sc)
+__device__ void lane_gk61_pair(const FB& f, double a, double b, double res[2], double eraw[2], double rabs[2],
+                               double rasc_lo[2], double rasc_hi[2])
 {
     const double ctr = 0.5 * (a + b), hl = 0.5 * (b - a), ahl = fabs(hl);
     double fc0, fc1;
@@ -196,13 +202,15 @@ __device__ void lane_gk61_pair(const FB& f, double a, double b, double res[2], d
     double rk0 = WGK61[30] * fc0, rk1 = WGK61[30] * fc1;
     double rg0 = 0.0, rg1 = 0.0;
     double ra0 = WGK61[30] * fabs(fc0), ra1 = WGK61[30] * fabs(fc1);
-    // node values relative to the centre value, parked in LDS (sc, stride 64)
-    // as bfloat16 pairs for the asc term: QUADPACK's resasc only scales the
-    // error estimate, and its relative error here stays below 2^-8
-    auto bf = [](double v) { return (unsigned)(__float_as_uint((float)v) >> 16); };
-    auto ub = [](unsigned h) { return (double)__uint_as_float(h << 16); };
+    double rc0 = 0.0, rc1 = 0.0;   // sum w_k |f_k - f_c|
+    // the next node's abscissa and weights are read one step ahead, so the
+    // sweep does not wait on the scalar loads
+    double xk = XGK61[0], wk = WGK61[0];
+#pragma unroll 1
     for (int k = 0; k < 30; k++) {
-        const double dx = hl * XGK61[k], wk = WGK61[k];
+        const double xn = XGK61[k + 1], wn = WGK61[k + 1];
+        const double wg = (k & 1) ? WG30[k >> 1] : 0.0;
+        const double dx = hl * xk;
         double fa0, fa1, fb0, fb1;
         f.both(ctr - dx, fa0, fa1);
         f.both(ctr + dx, fb0, fb1);
@@ -210,28 +218,22 @@ __device__ void lane_gk61_pair(const FB& f, double a, double b, double res[2], d
         rk1 += wk * (fa1 + fb1);
         ra0 += wk * (fabs(fa0) + fabs(fb0));
         ra1 += wk * (fabs(fa1) + fabs(fb1));
+        rc0 += wk * (fabs(fa0 - fc0) + fabs(fb0 - fc0));
+        rc1 += wk * (fabs(fa1 - fc1) + fabs(fb1 - fc1));
         if (k & 1) {
-            const double wg = WG30[k >> 1];
             rg0 += wg * (fa0 + fb0);
             rg1 += wg * (fa1 + fb1);
         }
-        sc[(2 * k) * 64] = bf(fa0 - fc0) | (bf(fb0 - fc0) << 16);
-        sc[(2 * k + 1) * 64] = bf(fa1 - fc1) | (bf(fb1 - fc1) << 16);
+        xk = xn;
+        wk = wn;
     }
-    const double m0 = 0.5 * rk0, m1 = 0.5 * rk1;
-    const double c0 = fc0 - m0, c1 = fc1 - m1;
-    double rs0 = WGK61[30] * fabs(c0), rs1 = WGK61[30] * fabs(c1);
-    for (int k = 0; k < 30; k++) {
-        const double wk = WGK61[k];
-        const unsigned p0 = sc[(2 * k) * 64], p1 = sc[(2 * k + 1) * 64];
-        rs0 += wk * (fabs(ub(p0 & 0xffffu) + c0) + fabs(ub(p0 >> 16) + c0));
-        rs1 += wk * (fabs(ub(p1 & 0xffffu) + c1) + fabs(ub(p1 >> 16) + c1));
-    }
+    const double d0 = 2.0 * fabs(fc0 - 0.5 * rk0), d1 = 2.0 * fabs(fc1 - 0.5 * rk1);
+    const double sl0 = 8.0 * DBL_EPSILON * (rc0 + d0), sl1 = 8.0 * DBL_EPSILON * (rc1 + d1);   // rounding slack
     res[0] = rk0 * hl; res[1] = rk1 * hl;
+    eraw[0] = (rk0 - rg0) * hl; eraw[1] = (rk1 - rg1) * hl;
     rabs[0] = ra0 * ahl; rabs[1] = ra1 * ahl;
-    rasc[0] = rs0 * ahl; rasc[1] = rs1 * ahl;
-    err[0] = scale_error((rk0 - rg0) * hl, rabs[0], rasc[0]);
-    err[1] = scale_error((rk1 - rg1) * hl, rabs[1], rasc[1]);
+    rasc_lo[0] = fmax(rc0 - d0 - sl0, 0.0) * ahl; rasc_lo[1] = fmax(rc1 - d1 - sl1, 0.0) * ahl;
+    rasc_hi[0] = (rc0 + d0 + sl0) * ahl; rasc_hi[1] = (rc1 + d1 + sl1) * ahl;
 }
 
 // QAG's acceptance of its first pass (epsabs 0): the pass's result stands
@@ -244,20 +246,36 @@ __device__ __forceinline__ bool qag_first_pass_stands(double r, double e, double
     return (e <= tol && e != rasc) || e == 0.0;
 }
 
-template <class FB>
-__device__ __forceinline__ bool lane_ray_pair(const FB& fb, double a, double b, double& d, double& l, unsigned* sc)
+// The acceptance over resasc in [lo, hi]: the scaled error is monotone on
+// either side of resasc = 200 |E| (where QUADPACK's scaling saturates), so the
+// acceptance is settled when it agrees at lo, hi and that point.  Returns
+// true only when the pass stands for every resasc in the range.
+__device__ __forceinline__ bool first_pass_stands(double r, double eraw, double rabs, double lo, double hi,
+                                                  double epsrel)
 {
-    double r[2], e[2], ra[2], rs[2];
-    lane_gk61_pair(fb, a, b, r, e, ra, rs, sc);
+    auto P = [&](double rasc) {
+        return qag_first_pass_stands(r, scale_error(eraw, rabs, rasc), rabs, rasc, epsrel);
+    };
+    if (!P(lo) || !P(hi)) return false;
+    const double kink = 200.0 * fabs(eraw);
+    if (lo < kink && kink < hi && !P(kink)) return false;
+    return true;
+}
+
+template <class FB>
+__device__ __forceinline__ bool lane_ray_pair(const FB& fb, double a, double b, double& d, double& l)
+{
+    double r[2], e[2], ra[2], lo[2], hi[2];
+    lane_gk61_pair(fb, a, b, r, e, ra, lo, hi);
     d = r[0];
     l = r[1];
-    return qag_first_pass_stands(r[0], e[0], ra[0], rs[0], 1.0e-12) &&
-           qag_first_pass_stands(r[1], e[1], ra[1], rs[1], 1.0e-12);
+    return first_pass_stands(r[0], e[0], ra[0], lo[0], hi[0], 1.0e-12) &&
+           first_pass_stands(r[1], e[1], ra[1], lo[1], hi[1], 1.0e-12);
 }
 
 // wave_ray's branches with every quadrature settled by its first pass;
 // returns false when some quadrature needs QAG's bisection
-__device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, double& lag, unsigned* sc)
+__device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, double& lag)
 {
     const double b_ph = 3.0 * sqrt(3.0) / 2.0;
     double sas = 1.0 - cos_alpha * cos_alpha;
@@ -267,7 +285,7 @@ __device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, d
     double d0, l0;
     if (b <= b_ph) {
         OutBoth fb{sas, 1.0 / (1.0 / u - 1.0)};
-        if (!lane_ray_pair(fb, 0.0, 1.0, d0, l0, sc)) return false;
+        if (!lane_ray_pair(fb, 0.0, 1.0, d0, l0)) return false;
         d0 *= 2.0 * b * u;
         l0 *= 2.0 * b * b * u * r_s;
     } else if (alpha <= PI / 2.0) {
@@ -279,7 +297,7 @@ __device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, d
         }
         if (isnan(wR)) wR = 0.0;
         InBoth fb{1.0 / (rc - 1.0)};
-        if (!lane_ray_pair(fb, wR, 1.0, d0, l0, sc)) return false;
+        if (!lane_ray_pair(fb, wR, 1.0, d0, l0)) return false;
         d0 *= 2.0 * b / rc;
         l0 *= 2.0 * b * b * r_s / rc;
     } else {
@@ -288,8 +306,8 @@ __device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, d
         if (isnan(wR)) wR = 0.0;
         InBoth fb{1.0 / (rc - 1.0)};
         double D1, D0, L1, L0;
-        if (!lane_ray_pair(fb, 0.0, 1.0, D1, L1, sc)) return false;
-        if (!lane_ray_pair(fb, wR, 1.0, D0, L0, sc)) return false;
+        if (!lane_ray_pair(fb, 0.0, 1.0, D1, L1)) return false;
+        if (!lane_ray_pair(fb, wR, 1.0, D0, L0)) return false;
         d0 = 2.0 * b * (2.0 * D1 - D0) / rc;
         l0 = 2.0 * b * b * r_s * (2.0 * L1 - L0) / rc;
         double cor = 1.0 / u - rc + log((1.0 / u - 1.0) / (rc - 1.0));
@@ -323,8 +341,7 @@ __global__ void __launch_bounds__(64) k_rays_lane(const Member* __restrict__ mem
     const size_t o = ro * L.N_R + j;
     cos_alpha[o] = ca;
     double d, l;
-    __shared__ unsigned sc[60 * 64];   // per lane: 60 words at stride 64
-    if (lane_ray(ca, m.r_s, u, d, l, sc + threadIdx.x)) {
+    if (lane_ray(ca, m.r_s, u, d, l)) {
         defl_raw[o] = d;
         lag_raw[o] = l;
     } else {
