@@ -178,8 +178,10 @@ int xh_precomputation(int N_ch, int N_b, const int* counts, double* out);
  * n_comp components sharing one phase grid comp_phases [N_P] (cycles).
  * components: [n_comp, N_ch, N_P]; phases [N_b+1]; counts [N_ch, N_b];
  * shifts [n_comp]; precomp [N_ch]; support [N_ch, 2]; background [N_ch, N_b]
- * or NULL.  lnl_out is a host pointer; STAR [N_ch, N_b], B_ML, B_ML_supp
- * [N_ch] device outputs; workspace of xh_marginal_workspace_bytes(). */
+ * or NULL; allow_negative nonzero lets every component keep negative bin
+ * integrals.  lnl_out is a host pointer; STAR [N_ch, N_b], B_ML, B_ML_supp
+ * [N_ch] device outputs; workspace of xh_marginal_workspace_bytes().  At most
+ * 32 components. */
 size_t xh_marginal_workspace_bytes(int N_ch);
 int xh_eval_marginal_likelihood(double exposure_time, int N_b,
                                 const double* phases, int N_ch,
@@ -194,6 +196,23 @@ int xh_eval_marginal_likelihood(double exposure_time, int N_b,
                                 double* lnl_out, double* STAR, double* B_ML,
                                 double* B_ML_supp, void* workspace,
                                 size_t workspace_bytes, void* stream);
+
+/* xh_eval_marginal_likelihood with the components as the reference takes them
+ * (xpsi/tools/compute_expected_counts.pyx:26-197): each component on its own
+ * phase grid, and allow_negative per component.  components: the components'
+ * [N_ch][comp_num_phases[c]] blocks one after another (device);
+ * comp_num_phases: phases per component (device, n_comp ints); comp_phases:
+ * the grids one after another (device); allow_negative: n_comp flags (host,
+ * nullptr = none).  At most 32 components. */
+int xh_eval_marginal_likelihood_ex(double exposure_time, int N_b, const double* phases, int N_ch,
+                                   const double* counts, int n_comp, const double* components,
+                                   const int* comp_num_phases, const double* comp_phases,
+                                   const double* shifts, const double* precomp, const double* support,
+                                   double epsabs, double epsrel, double epsilon, double sigmas,
+                                   double llzero, const int* allow_negative, double slim,
+                                   const double* background, double* lnl_out, double* STAR,
+                                   double* B_ML, double* B_ML_supp, void* workspace,
+                                   size_t workspace_bytes, void* stream);
 
 /* Replaces xpsi/likelihoods/_poisson_likelihood_given_background.pyx:14-113
  * `poisson_likelihood_given_background(...) -> (lnL, EXPEC)`: the Poisson
@@ -217,6 +236,17 @@ int xh_poisson_likelihood_given_background(double exposure_time, int N_b,
                                            int allow_negative, double* lnl_out,
                                            double* EXPEC, void* workspace,
                                            size_t workspace_bytes, void* stream);
+
+/* xh_poisson_likelihood_given_background with components on their own phase
+ * grids and allow_negative per component, laid out as for
+ * xh_eval_marginal_likelihood_ex. */
+int xh_poisson_likelihood_given_background_ex(double exposure_time, int N_b, const double* phases, int N_ch,
+                                              const double* counts, int n_comp, const double* components,
+                                              const int* comp_num_phases, const double* comp_phases,
+                                              const double* shifts, const double* background,
+                                              const double* precomp, const int* allow_negative,
+                                              double* lnl_out, double* EXPEC, void* workspace,
+                                              size_t workspace_bytes, void* stream);
 
 /* Replace xpsi/tools/synthesise.pyx:40-148 `synthesise_exposure(...) ->
  * (EXPEC, SYNTHETIC, SCALE_BACKGROUND)` and :152-276

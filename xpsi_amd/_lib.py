@@ -85,6 +85,13 @@ def lib():
     L.xh_eval_marginal_likelihood.argtypes = (
         [d, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, d, d, d, d, d, i32, d,
          vp, C.POINTER(d), vp, vp, vp, vp, sz, vp])
+    L.xh_eval_marginal_likelihood_ex.restype = i32
+    L.xh_eval_marginal_likelihood_ex.argtypes = (
+        [d, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, d, d, d, d, d, vp, d,
+         vp, C.POINTER(d), vp, vp, vp, vp, sz, vp])
+    L.xh_poisson_likelihood_given_background_ex.restype = i32
+    L.xh_poisson_likelihood_given_background_ex.argtypes = (
+        [d, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, C.POINTER(d), vp, vp, sz, vp])
     L.xh_compute_rays_workspace_bytes.restype = sz
     L.xh_compute_rays_workspace_bytes.argtypes = [i32, i32]
     L.xh_compute_rays.restype = i32
@@ -191,9 +198,10 @@ def check(rc, what):
 EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_integrate_time_invariant_workspace_bytes", "xh_integrate_time_invariant",
             "xh_energy_integrator", "xh_fold", "xh_precomputation",
-            "xh_marginal_workspace_bytes", "xh_eval_marginal_likelihood",
+            "xh_marginal_workspace_bytes", "xh_eval_marginal_likelihood", "xh_eval_marginal_likelihood_ex",
             "xh_eval_marginal_likelihood_noalloc",
             "xh_poisson_workspace_bytes", "xh_poisson_likelihood_given_background",
+            "xh_poisson_likelihood_given_background_ex",
             "xh_compute_rays_workspace_bytes", "xh_compute_rays", "xh_allocate_cells_workspace_bytes",
             "xh_allocate_cells", "xh_cell_mesh_workspace_bytes", "xh_construct_cell_mesh",
             "xh_prior_inverse_sample", "xh_synthesise_workspace_bytes", "xh_synthesise_exposure",

@@ -696,28 +696,31 @@ extern "C" int xh_precomputation(int N_ch, int N_b, const int* counts, double* o
 
 extern "C" size_t xh_marginal_workspace_bytes(int N_ch) { return (size_t)N_ch * (8 + 4) + 64 + 256; }
 
-extern "C" int xh_eval_marginal_likelihood(double T, int N_b, const double* phases, int N_ch, const double* counts,
-                                           int n_comp, const double* comps, int N_P, const double* comp_phases,
-                                           const double* shifts, const double* precomp, const double* support,
-                                           double epsabs, double epsrel, double epsilon, double sigmas,
-                                           double llzero, int allow_negative, double slim,
-                                           const double* background, double* lnl_out, double* STAR,
-                                           double* B_ML, double* B_ML_supp, void* workspace,
-                                           size_t workspace_bytes, void* stream)
+// shared body of the two marginal drop-ins: comp_np (device, per component)
+// or nullptr for one shared grid of N_P phases; allow: bit c lets component c
+// keep negative bin integrals
+static int marginal_impl(double T, int N_b, const double* phases, int N_ch, const double* counts, int n_comp,
+                         const double* comps, int N_P, const int* comp_np, const double* comp_phases,
+                         const double* shifts, const double* precomp, const double* support, double epsabs,
+                         double epsrel, double epsilon, double sigmas, double llzero, unsigned allow, double slim,
+                         const double* background, double* lnl_out, double* STAR, double* B_ML, double* B_ML_supp,
+                         void* workspace, size_t workspace_bytes, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
+    if (n_comp < 1 || n_comp > 32) return fail(XH_EINVAL, "marginal likelihood: 1 to 32 components");
     if (workspace_bytes < xh_marginal_workspace_bytes(N_ch)) return fail(XH_EINVAL, "workspace too small");
     char* w = (char*)workspace;
     double* d_ll = (double*)w; w += (size_t)N_ch * 8;
     double* d_lnl = (double*)w; w += 64;
     int* d_st = (int*)w;
     MarginalArgs A{};
-    A.signals = comps; A.comp_phases = comp_phases; A.shifts = shifts; A.phases = phases; A.counts = counts;
+    A.signals = comps; A.comp_phases = comp_phases; A.comp_np = comp_np; A.shifts = shifts; A.phases = phases;
+    A.counts = counts;
     A.precomp = precomp; A.support = support; A.background = background; A.request_status = nullptr;
     A.ll = d_ll; A.status = d_st; A.B_ML = B_ML; A.B_ML_supp = B_ML_supp; A.STAR = STAR;
     A.exposure = T; A.epsabs = epsabs; A.epsrel = epsrel; A.epsilon = epsilon; A.sigmas = sigmas;
     A.llzero = llzero; A.slim = slim; A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp;
-    A.allow_negative = allow_negative; A.seed = next_seed(); A.seed_ptr = nullptr;
+    A.allow_negative = allow; A.seed = next_seed(); A.seed_ptr = nullptr;
     HIPCHK(fill(STAR, (size_t)N_ch * N_b, 0.0, s));
     HIPCHK(fill(B_ML, N_ch, 0.0, s));
     HIPCHK(fill(B_ML_supp, N_ch, 0.0, s));
@@ -730,17 +733,53 @@ extern "C" int xh_eval_marginal_likelihood(double T, int N_b, const double* phas
     return XH_OK;
 }
 
+static unsigned allow_mask(int n_comp, const int* allow_negative)
+{
+    unsigned m = 0u;
+    for (int c = 0; allow_negative && c < n_comp && c < 32; c++)
+        if (allow_negative[c]) m |= 1u << c;
+    return m;
+}
+
+extern "C" int xh_eval_marginal_likelihood(double T, int N_b, const double* phases, int N_ch, const double* counts,
+                                           int n_comp, const double* comps, int N_P, const double* comp_phases,
+                                           const double* shifts, const double* precomp, const double* support,
+                                           double epsabs, double epsrel, double epsilon, double sigmas,
+                                           double llzero, int allow_negative, double slim,
+                                           const double* background, double* lnl_out, double* STAR,
+                                           double* B_ML, double* B_ML_supp, void* workspace,
+                                           size_t workspace_bytes, void* stream)
+{
+    return marginal_impl(T, N_b, phases, N_ch, counts, n_comp, comps, N_P, nullptr, comp_phases, shifts, precomp,
+                         support, epsabs, epsrel, epsilon, sigmas, llzero, allow_negative ? ~0u : 0u, slim,
+                         background, lnl_out, STAR, B_ML, B_ML_supp, workspace, workspace_bytes, stream);
+}
+
+extern "C" int xh_eval_marginal_likelihood_ex(double T, int N_b, const double* phases, int N_ch,
+                                              const double* counts, int n_comp, const double* comps,
+                                              const int* comp_num_phases, const double* comp_phases,
+                                              const double* shifts, const double* precomp, const double* support,
+                                              double epsabs, double epsrel, double epsilon, double sigmas,
+                                              double llzero, const int* allow_negative, double slim,
+                                              const double* background, double* lnl_out, double* STAR,
+                                              double* B_ML, double* B_ML_supp, void* workspace,
+                                              size_t workspace_bytes, void* stream)
+{
+    if (!comp_num_phases) return fail(XH_EINVAL, "xh_eval_marginal_likelihood_ex: phases per component required");
+    return marginal_impl(T, N_b, phases, N_ch, counts, n_comp, comps, 0, comp_num_phases, comp_phases, shifts,
+                         precomp, support, epsabs, epsrel, epsilon, sigmas, llzero, allow_mask(n_comp, allow_negative),
+                         slim, background, lnl_out, STAR, B_ML, B_ML_supp, workspace, workspace_bytes, stream);
+}
+
 extern "C" size_t xh_poisson_workspace_bytes(int N_ch) { return (size_t)N_ch * (8 + 4) + 64 + 256; }
 
-extern "C" int xh_poisson_likelihood_given_background(double T, int N_b, const double* phases, int N_ch,
-                                                      const double* counts, int n_comp, const double* comps,
-                                                      int N_P, const double* comp_phases, const double* shifts,
-                                                      const double* background, const double* precomp,
-                                                      int allow_negative, double* lnl_out, double* EXPEC,
-                                                      void* workspace, size_t workspace_bytes, void* stream)
+static int poisson_impl(double T, int N_b, const double* phases, int N_ch, const double* counts, int n_comp,
+                        const double* comps, int N_P, const int* comp_np, const double* comp_phases,
+                        const double* shifts, const double* background, const double* precomp, unsigned allow,
+                        double* lnl_out, double* EXPEC, void* workspace, size_t workspace_bytes, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    if (N_ch < 1 || N_b < 1 || n_comp < 1 || N_P < 1)
+    if (N_ch < 1 || N_b < 1 || n_comp < 1 || n_comp > 32 || (!comp_np && N_P < 1))
         return fail(XH_EINVAL, "xh_poisson_likelihood_given_background: bad sizes");
     if (!background) return fail(XH_EINVAL, "xh_poisson_likelihood_given_background: the background is required");
     if (workspace_bytes < xh_poisson_workspace_bytes(N_ch))
@@ -750,10 +789,11 @@ extern "C" int xh_poisson_likelihood_given_background(double T, int N_b, const d
     double* d_lnl = (double*)w; w += 64;
     int* d_st = (int*)w;
     MarginalArgs A{};
-    A.signals = comps; A.comp_phases = comp_phases; A.shifts = shifts; A.phases = phases; A.counts = counts;
+    A.signals = comps; A.comp_phases = comp_phases; A.comp_np = comp_np; A.shifts = shifts; A.phases = phases;
+    A.counts = counts;
     A.precomp = precomp; A.background = background; A.ll = d_ll; A.status = d_st; A.STAR = EXPEC;
     A.exposure = T; A.llzero = -1.0e90; A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp;
-    A.allow_negative = allow_negative; A.seed = next_seed(); A.seed_ptr = nullptr;
+    A.allow_negative = allow; A.seed = next_seed(); A.seed_ptr = nullptr;
     hipLaunchKernelGGL(k_poisson_bg, dim3(N_ch, 1), dim3(64), 2 * (size_t)N_b * 8, s, A);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_poisson_reduce, dim3(1), dim3(64), 0, s, A, d_lnl);
@@ -761,6 +801,32 @@ extern "C" int xh_poisson_likelihood_given_background(double T, int N_b, const d
     HIPCHK(hipMemcpyAsync(lnl_out, d_lnl, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return XH_OK;
+}
+
+extern "C" int xh_poisson_likelihood_given_background(double T, int N_b, const double* phases, int N_ch,
+                                                      const double* counts, int n_comp, const double* comps,
+                                                      int N_P, const double* comp_phases, const double* shifts,
+                                                      const double* background, const double* precomp,
+                                                      int allow_negative, double* lnl_out, double* EXPEC,
+                                                      void* workspace, size_t workspace_bytes, void* stream)
+{
+    return poisson_impl(T, N_b, phases, N_ch, counts, n_comp, comps, N_P, nullptr, comp_phases, shifts, background,
+                        precomp, allow_negative ? ~0u : 0u, lnl_out, EXPEC, workspace, workspace_bytes, stream);
+}
+
+extern "C" int xh_poisson_likelihood_given_background_ex(double T, int N_b, const double* phases, int N_ch,
+                                                         const double* counts, int n_comp, const double* comps,
+                                                         const int* comp_num_phases, const double* comp_phases,
+                                                         const double* shifts, const double* background,
+                                                         const double* precomp, const int* allow_negative,
+                                                         double* lnl_out, double* EXPEC, void* workspace,
+                                                         size_t workspace_bytes, void* stream)
+{
+    if (!comp_num_phases)
+        return fail(XH_EINVAL, "xh_poisson_likelihood_given_background_ex: phases per component required");
+    return poisson_impl(T, N_b, phases, N_ch, counts, n_comp, comps, 0, comp_num_phases, comp_phases, shifts,
+                        background, precomp, allow_mask(n_comp, allow_negative), lnl_out, EXPEC, workspace,
+                        workspace_bytes, stream);
 }
 
 extern "C" size_t xh_synthesise_workspace_bytes(int N_ch, int N_b) { return (size_t)N_ch * N_b * 8 + 256; }
@@ -780,7 +846,7 @@ static int synthesise(int mode, double T, double n_star, int N_b, const double* 
     double* scal = rate + (size_t)N_ch * N_b;
     MarginalArgs A{};
     A.signals = comps; A.comp_phases = comp_phases; A.shifts = shifts; A.phases = phases;
-    A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp; A.allow_negative = allow_negative;
+    A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp; A.allow_negative = allow_negative ? ~0u : 0u;
     hipLaunchKernelGGL(k_star_rate, dim3(N_ch), dim3(64), 0, s, A, rate);
     HIPCHK(hipGetLastError());
     const int n = N_ch * N_b;
@@ -1451,7 +1517,7 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     MA.B_ML = p->d_bml; MA.B_ML_supp = p->d_bmls; MA.STAR = p->d_star;
     MA.exposure = p->cfg.exposure_time; MA.epsabs = p->cfg.epsabs; MA.epsrel = p->cfg.epsrel;
     MA.epsilon = p->cfg.epsilon; MA.sigmas = p->cfg.sigmas; MA.llzero = p->cfg.llzero; MA.slim = p->cfg.slim;
-    MA.N_ch = p->N_ch; MA.N_b = p->N_b; MA.N_P = N_P; MA.n_comp = p->nreg; MA.allow_negative = 0;
+    MA.N_ch = p->N_ch; MA.N_b = p->N_b; MA.N_P = N_P; MA.n_comp = p->nreg; MA.allow_negative = 0u;
     MA.seed = 0; MA.seed_ptr = p->d_seed;
     if (p->timing) HIPCHK(hipEventRecord(p->ev[5], s));
     hipLaunchKernelGGL(k_marginal, dim3(p->N_ch, n), dim3(64), 2 * p->N_b * 8, s, MA);

@@ -22,8 +22,9 @@ struct FoldGemmArgs {
 };
 
 struct MarginalArgs {
-    const double* signals;      // [request][comp][N_ch][N_P]
-    const double* comp_phases;  // N_P (cycles)
+    const double* signals;      // [request][comp][N_ch][N_P] (comp_np: component c's block is N_ch x comp_np[c])
+    const double* comp_phases;  // N_P (cycles), shared; with comp_np the components' grids one after another
+    const int* comp_np;         // [comp] phases per component, or nullptr (every component on the shared grid)
     const double* shifts;       // [request][comp]
     const double* phases;       // N_b + 1
     const double* counts;       // [N_ch][N_b]
@@ -37,7 +38,8 @@ struct MarginalArgs {
     double* B_ML_supp;          // [request][N_ch]
     double* STAR;               // [request][N_ch][N_b] or nullptr
     double exposure, epsabs, epsrel, epsilon, sigmas, llzero, slim;
-    int N_ch, N_b, N_P, n_comp, allow_negative;
+    int N_ch, N_b, N_P, n_comp;
+    unsigned allow_negative;    // bit c: component c keeps negative bin integrals
     // draw of the llzero-class values: *seed_ptr when set (device memory the
     // host refreshes before each launch, so replayed graphs draw afresh), else seed
     unsigned long long seed;

@@ -235,27 +235,38 @@ struct MargFn {
 // (xpsi/tools/compute_expected_counts.pyx:66-197).
 __device__ double bin_rate(const MarginalArgs& A, int rq, int ch, int j)
 {
+    // the request's signals: components one after another, each [N_ch][its phases]
+    int P_tot = A.n_comp * A.N_P;
+    if (A.comp_np) {
+        P_tot = 0;
+        for (int c = 0; c < A.n_comp; c++) P_tot += A.comp_np[c];
+    }
+    const double* sig = A.signals + (size_t)rq * A.N_ch * P_tot;
+    int off = 0;   // phases of the components before c
     double st = 0.0;
     for (int c = 0; c < A.n_comp; c++) {
-        const double* pulse = A.signals + (((size_t)rq * A.n_comp + c) * A.N_ch + ch) * A.N_P;
-        const double* cph = A.comp_phases;
+        const int np = A.comp_np ? A.comp_np[c] : A.N_P;
+        const double* pulse = sig + (size_t)A.N_ch * off + (size_t)ch * np;
+        const double* cph = A.comp_np ? A.comp_phases + off : A.comp_phases;
         const double shift = A.shifts[rq * A.n_comp + c];
-        if (A.N_P > 1) {
+        const bool neg_ok = (A.allow_negative >> c) & 1u;
+        if (np > 1) {
             double pa = A.phases[j] + shift, pb = A.phases[j + 1] + shift;
             if (eq12(pb - pa, 1.0)) { pa = 0.0; pb = 1.0; }
             else { pa -= floor(pa); pb -= floor(pb); }
             if (pa < pb) {
-                double v = akima_integ(cph, pulse, A.N_P, true, pa, pb);
-                if (v > 0.0 || A.allow_negative) st += v;
+                double v = akima_integ(cph, pulse, np, true, pa, pb);
+                if (v > 0.0 || neg_ok) st += v;
             } else {
-                double v = akima_integ(cph, pulse, A.N_P, true, pa, 1.0);
-                if (v > 0.0 || A.allow_negative) st += v;
-                v = akima_integ(cph, pulse, A.N_P, true, 0.0, pb);
-                if (v > 0.0 || A.allow_negative) st += v;
+                double v = akima_integ(cph, pulse, np, true, pa, 1.0);
+                if (v > 0.0 || neg_ok) st += v;
+                v = akima_integ(cph, pulse, np, true, 0.0, pb);
+                if (v > 0.0 || neg_ok) st += v;
             }
         } else if (j == 0) {
             st = pulse[0];
         }
+        off += np;
     }
     return st < 0.0 ? 0.0 : st;
 }

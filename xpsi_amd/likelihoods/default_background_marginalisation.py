@@ -4,7 +4,8 @@ precomputation (:38-68) and eval_marginal_likelihood (:450-734) keep the
 reference signatures and return values.  The marginal over each channel's
 background is integrated with QAG-GK61 to a relative tolerance of
 min(epsrel, 1e-11); llzero-class outputs are drawn near llzero as the
-reference does.  Components must share one phase grid.
+reference does.  Components may lie on different phase grids and take
+allow_negative per component, as the reference's do.
 """
 import ctypes as C
 
@@ -12,6 +13,7 @@ import numpy as np
 
 from .._lib import lib, check
 from .. import _device as D
+from ._components import allow_flags, pack
 
 
 def precomputation(data):
@@ -39,19 +41,10 @@ def eval_marginal_likelihood(exposure_time, phases, counts, components,
             raise TypeError('Background upper limit cannot be set to 0.')
         if support[i, 1] > 0 and support[i, 1] - support[i, 0] < 0:
             raise TypeError('Background upper limit must be higher than the lower limit.')
-    ph0 = np.asarray(component_phases[0], dtype=np.float64)
-    for cp in component_phases[1:]:
-        if not np.array_equal(np.asarray(cp), ph0):
-            raise NotImplementedError("components must share one phase grid")
-    if not isinstance(allow_negative, bool):
-        if len(set(bool(a) for a in allow_negative)) != 1:
-            raise NotImplementedError("per-component allow_negative is not supported")
-        allow_negative = bool(allow_negative[0])
-    comps = np.ascontiguousarray(np.stack([np.asarray(c, dtype=np.float64) for c in components]))
-    n_comp, _, N_P = comps.shape
-    dc = D.to_dev(comps)
+    flags = allow_flags(allow_negative, len(components))
+    shared, n_comp, N_P, dc, dcp, dnp = pack(components, component_phases, N_ch)
     dph, dcnt = D.to_dev(phases), D.to_dev(counts)
-    dcp, dsh = D.to_dev(ph0), D.to_dev(np.asarray(phase_shifts, dtype=np.float64))
+    dsh = D.to_dev(np.asarray(phase_shifts, dtype=np.float64))
     dpre, dsup = D.to_dev(neg_sum_ln_data_factorial), D.to_dev(support)
     dbg = None if background is None else D.to_dev(background)
     STAR = D.empty((N_ch, N_b))
@@ -60,11 +53,20 @@ def eval_marginal_likelihood(exposure_time, phases, counts, components,
     wsb = L.xh_marginal_workspace_bytes(N_ch)
     ws = D.empty((wsb + 7) // 8)
     lnl = C.c_double(0.0)
-    rc = L.xh_eval_marginal_likelihood(
-        float(exposure_time), N_b, D.ptr(dph), N_ch, D.ptr(dcnt), n_comp, D.ptr(dc),
-        N_P, D.ptr(dcp), D.ptr(dsh), D.ptr(dpre), D.ptr(dsup), float(epsabs),
-        float(epsrel), float(epsilon), float(sigmas), float(llzero),
-        int(allow_negative), float(slim), D.ptr(dbg), C.byref(lnl), D.ptr(STAR),
-        D.ptr(bml), D.ptr(bmls), D.ptr(ws), ws.numel() * 8, D.stream_ptr())
-    check(rc, "xh_eval_marginal_likelihood")
+    head = (float(exposure_time), N_b, D.ptr(dph), N_ch, D.ptr(dcnt), n_comp, D.ptr(dc))
+    mid = (D.ptr(dsh), D.ptr(dpre), D.ptr(dsup), float(epsabs), float(epsrel), float(epsilon), float(sigmas),
+           float(llzero))
+    tail = (float(slim), D.ptr(dbg), C.byref(lnl), D.ptr(STAR), D.ptr(bml), D.ptr(bmls), D.ptr(ws),
+            ws.numel() * 8, D.stream_ptr())
+    if shared and len(set(flags)) == 1:
+        rc = L.xh_eval_marginal_likelihood(*head, N_P, D.ptr(dcp), *mid, int(flags[0]), *tail)
+        check(rc, "xh_eval_marginal_likelihood")
+    else:
+        if shared:
+            # one grid, per-component flags: the same layout, the grid repeated
+            dnp = D.to_dev(np.full(n_comp, N_P, dtype=np.int32), np.int32)
+            dcp = D.to_dev(np.tile(dcp.cpu().numpy(), n_comp))
+        af = C.cast((C.c_int * n_comp)(*[int(f) for f in flags]), C.c_void_p)
+        rc = L.xh_eval_marginal_likelihood_ex(*head, D.ptr(dnp), D.ptr(dcp), *mid, af, *tail)
+        check(rc, "xh_eval_marginal_likelihood_ex")
     return (lnl.value, STAR.cpu().numpy(), bml.cpu().numpy(), bmls.cpu().numpy())
