@@ -358,3 +358,246 @@ int or_integrate(double R, double omega, double r_s, double inclination,
         for (size_t k = 0; k < N_P; k++) flux[p * N_P + k] /= energies[p] * KEV;
     return terminate;
 }
+
+/* xpsi/cellmesh/integrator.pyx:48-667, the integrator without azimuthal
+ * invariance, for the blackbody (atm_ext 1): every cell carries its own
+ * parameter vector (cellParams [S_rows][S_cols][2], log10 T first) and, with
+ * the correction, its own elsewhere temperature (correction, same layout).
+ * Per parallel and image the leaves' redshift Z, mu eta (ABB), geometric
+ * factor GEOM and phase are formed over all N_L leaves (:316-512; invisible
+ * stretches padded with linearly stepped phase, Z and ABB and zero GEOM);
+ * then for each radiating cell and model phase GEOM is interpolated with the
+ * periodic Akima spline and, where positive, Z with a Steffen spline
+ * (:531-592; the blackbody ignores the interpolated ABB, which only beamed
+ * atmospheres read), and every energy's intensity at E / Z is added with the cell's
+ * area times GEOM.  Flux [N_E][N_P], divided by E keV at the end (:601-606).
+ * Rings are summed in order (one thread). */
+int or_integrate_general(double omega, double inclination, size_t S_rows, size_t S_cols,
+                         const double *cellArea, const double *radial, const double *r_s_over_r,
+                         const double *theta, const double *phi, const double *cellParams,
+                         const int *radiates, size_t N_R, const double *deflection,
+                         const double *cos_alpha, const double *lag, const double *maxDeflection,
+                         const double *cos_gammaArray, size_t N_E, const double *energies, size_t N_L,
+                         const double *leaves, size_t N_P, const double *phases, int image_order_limit,
+                         const double *correction, double *flux)
+{
+    const double hpi = 0.5 * M_PI, two_pi = 2.0 * M_PI;
+    double sin_i = sin(inclination), cos_i = cos(inclination);
+    size_t leaf_lim = (N_L % 2 == 0) ? N_L / 2 : (N_L + 1) / 2;
+    int image_order = image_order_limit > 0 ? image_order_limit : 0;
+    memset(flux, 0, N_E * N_P * sizeof(double));
+    double *acc = (double *)calloc(N_E * N_P, sizeof(double));
+    double *PHASE = (double *)calloc(N_L, sizeof(double));
+    double *ZZ = (double *)calloc(N_L, sizeof(double));
+    double *AB = (double *)calloc(N_L, sizeof(double));
+    double *GE = (double *)calloc(N_L, sizeof(double));
+    double *cdefl = (double *)malloc(N_R * sizeof(double));
+    double *calt = (double *)malloc(N_R * sizeof(double));
+    double *sa = (double *)malloc(4 * N_R * sizeof(double));
+    double *sl = (double *)malloc(4 * N_R * sizeof(double));
+    double *st = (double *)malloc(4 * N_R * sizeof(double));
+    double *gb = (double *)malloc(3 * N_L * sizeof(double));   /* Akima, GEOM */
+    double *mb = (double *)malloc((N_L + 3) * sizeof(double));
+    double *zs = (double *)malloc(4 * N_L * sizeof(double));   /* Steffen, Z */
+    int terminate = 0;
+    for (size_t i = 0; i < S_rows && !terminate; i++) {
+        size_t j;
+        for (j = 0; j < S_cols; j++) if (radiates[i * S_cols + j] == 1) break;
+        if (j == S_cols) continue;
+        const double *D = deflection + i * N_R, *A = cos_alpha + i * N_R, *L = lag + i * N_R;
+        size_t jn = 0;
+        while (D[jn] <= hpi) jn++;
+        for (size_t k = 0; k < N_R; k++) {
+            cdefl[k] = cos(D[N_R - k - 1]);
+            calt[k] = A[N_R - k - 1];
+        }
+        const double *xalt = cdefl + (N_R - jn - 1), *yalt = calt + (N_R - jn - 1);
+        size_t nalt = jn + 1;
+        or_steffen_init(xalt, yalt, nalt, st, st + N_R, st + 2 * N_R, st + 3 * N_R);
+        or_steffen_init(D, A, N_R, sa, sa + N_R, sa + 2 * N_R, sa + 3 * N_R);
+        or_steffen_init(D, L, N_R, sl, sl + N_R, sl + 2 * N_R, sl + 3 * N_R);
+        double radius = radial[i];
+        double Grav_z = sqrt(1.0 - r_s_over_r[i]);
+        double cos_gamma = cos_gammaArray[i];
+        double sin_gamma = sqrt(1.0 - cos_gamma * cos_gamma);
+        double th_i = theta[i * S_cols];
+        double cos_th = cos(th_i), sin_th = sin(th_i), th_over_pi = th_i / M_PI;
+        double beta = radius * omega * sin_th / (C_SI * Grav_z);
+        double Lorentz = sqrt(1.0 - beta * beta);
+        double c_alpha = -1.0, deriv = -1.0;
+        int IO = image_order == 0 ? (int)ceil(maxDeflection[i] / M_PI) : image_order;
+        for (int I = 0; I < IO; I++) {
+            int invis = 2;
+            size_t invis_phase = 0;
+            double step, zstep, astep;
+            for (size_t k = 0; k < N_L; k++) {
+                double cos_psi = cos_i * cos_th + sin_i * sin_th * cos(leaves[k]);
+                double psi = image_deflection(I, acos(cos_psi));
+                double sin_psi = sin(psi);
+                if (!eq12(psi, 0.0) && eq12(sin_psi, 0.0)) {
+                    double ii = (cos_i >= 0.0) ? inclination + inclination * 1.0e-6
+                                               : inclination - inclination * 1.0e-6;
+                    cos_psi = cos(ii) * cos_th + sin(ii) * sin_th * cos(leaves[k]);
+                    psi = image_deflection(I, acos(cos_psi));
+                    sin_psi = sin(psi);
+                }
+                int calc = 0;
+                double mu = 0.0, sin_alpha = 0.0;
+                if (psi <= maxDeflection[i]) {
+                    if (psi < D[0] || psi > D[N_R - 1]) {
+                        printf("Interpolation error: deflection = %.16e\n", psi);
+                        terminate = 1;
+                        break;
+                    }
+                    if (psi <= hpi && cos_psi >= xalt[0])
+                        c_alpha = or_steffen_eval(xalt, nalt, st, st + N_R, st + 2 * N_R, st + 3 * N_R, cos_psi);
+                    else
+                        c_alpha = or_steffen_eval(D, N_R, sa, sa + N_R, sa + 2 * N_R, sa + 3 * N_R, psi);
+                    sin_alpha = sqrt(1.0 - c_alpha * c_alpha);
+                    mu = c_alpha * cos_gamma;
+                    if (!eq12(psi, 0.0)) {
+                        double cos_delta = (cos_i - cos_th * cos_psi) / (sin_th * sin_psi);
+                        if (th_over_pi < 0.5) mu = mu + sin_alpha * sin_gamma * cos_delta;
+                        else mu = mu - sin_alpha * sin_gamma * cos_delta;
+                    }
+                    calc = (mu > 0.0) ? 1 : 0;
+                }
+                if (calc) {
+                    if (psi <= hpi && cos_psi >= xalt[0]) {
+                        deriv = or_steffen_deriv(xalt, nalt, st, st + N_R, st + 2 * N_R, cos_psi);
+                    } else {
+                        deriv = or_steffen_deriv(D, N_R, sa, sa + N_R, sa + 2 * N_R, psi);
+                        deriv = exp(log(fabs(deriv)) - log(fabs(sin_psi)));
+                    }
+                    if (psi < D[0] || psi > D[N_R - 1]) {
+                        printf("Interpolation error: deflection = %.16e\n", psi);
+                        terminate = 1;
+                        break;
+                    }
+                    double plag = or_steffen_eval(D, N_R, sl, sl + N_R, sl + 2 * N_R, sl + 3 * N_R, psi);
+                    for (int ks = 0; ks < 2; ks++) {
+                        int take = (0 < k && k < leaf_lim - 1) || (k == 0 && ks == 0) ||
+                                   (k == leaf_lim - 1 && N_L % 2 == 1 && ks == 0) ||
+                                   (k == leaf_lim - 1 && N_L % 2 == 0);
+                        if (!take) continue;
+                        size_t kd = ks == 0 ? k : N_L - 1 - k;
+                        double cos_xi, superlum, eta;
+                        if (!eq12(psi, 0.0)) {
+                            cos_xi = sin_alpha * sin_i * sin(leaves[kd]) / sin_psi;
+                            superlum = 1.0 + beta * cos_xi;
+                            eta = Lorentz / superlum;
+                        } else {
+                            superlum = 1.0;
+                            eta = Lorentz;
+                        }
+                        ZZ[kd] = eta * Grav_z;
+                        AB[kd] = mu * eta;
+                        GE[kd] = mu * fabs(deriv) * Grav_z * eta * eta * eta / superlum;
+                        PHASE[kd] = leaves[kd] + plag;
+                    }
+                    if (k == 0) {
+                        PHASE[N_L - 1] = PHASE[0] + two_pi;
+                        ZZ[N_L - 1] = ZZ[0];
+                        AB[N_L - 1] = AB[0];
+                        GE[N_L - 1] = GE[0];
+                    } else if (invis == 2) {
+                        step = leaves[k] / (double)k;
+                        zstep = (ZZ[k] - ZZ[N_L - k - 1]) / (2.0 * (double)k);
+                        astep = (AB[k] - AB[N_L - k - 1]) / (2.0 * (double)k);
+                        for (size_t m = N_L - k; m < N_L; m++) {
+                            PHASE[m] = PHASE[m - 1] + step;
+                            ZZ[m] = ZZ[m - 1] + zstep;
+                            AB[m] = AB[m - 1] + astep;
+                            GE[m] = 0.0;
+                        }
+                        PHASE[0] = PHASE[N_L - 1] - two_pi;
+                        ZZ[0] = ZZ[N_L - 1];
+                        AB[0] = AB[N_L - 1];
+                        GE[0] = GE[N_L - 1];
+                        for (size_t m = 1; m < k; m++) {
+                            PHASE[m] = PHASE[m - 1] + step;
+                            ZZ[m] = ZZ[m - 1] + zstep;
+                            AB[m] = AB[m - 1] + astep;
+                            GE[m] = 0.0;
+                        }
+                    } else if (invis == 1) {
+                        step = (PHASE[k] - PHASE[invis_phase - 1]) / (double)(k - invis_phase + 1);
+                        zstep = (ZZ[k] - ZZ[invis_phase - 1]) / (double)(k - invis_phase + 1);
+                        astep = (AB[k] - AB[invis_phase - 1]) / (double)(k - invis_phase + 1);
+                        for (size_t m = invis_phase; m < k; m++) {
+                            PHASE[m] = PHASE[m - 1] + step;
+                            ZZ[m] = ZZ[m - 1] + zstep;
+                            AB[m] = AB[m - 1] + astep;
+                        }
+                        step = (PHASE[N_L - invis_phase] - PHASE[N_L - 1 - k]) / (double)(k - invis_phase + 1);
+                        zstep = (ZZ[N_L - invis_phase] - ZZ[N_L - 1 - k]) / (double)(k - invis_phase + 1);
+                        astep = (AB[N_L - invis_phase] - AB[N_L - 1 - k]) / (double)(k - invis_phase + 1);
+                        for (size_t m = N_L - k; m < N_L - invis_phase; m++) {
+                            PHASE[m] = PHASE[m - 1] + step;
+                            ZZ[m] = ZZ[m - 1] + zstep;
+                            AB[m] = AB[m - 1] + astep;
+                        }
+                    }
+                    invis = 0;
+                } else if (invis == 0) {
+                    step = (PHASE[N_L - k] - PHASE[k - 1]) / (double)(N_L - 2 * k + 1);
+                    zstep = (ZZ[N_L - k] - ZZ[k - 1]) / (double)(N_L - 2 * k + 1);
+                    astep = (AB[N_L - k] - AB[k - 1]) / (double)(N_L - 2 * k + 1);
+                    for (size_t m = k; m < N_L - k; m++) {
+                        PHASE[m] = PHASE[m - 1] + step;
+                        ZZ[m] = ZZ[m - 1] + zstep;
+                        AB[m] = AB[m - 1] + astep;
+                        GE[m] = 0.0;
+                    }
+                    invis = 1;
+                    invis_phase = k;
+                }
+            }
+            if (terminate) break;
+            if (invis == 2) break;
+            for (size_t m = 1; m < N_L; m++) {
+                if (PHASE[m] <= PHASE[m - 1]) {
+                    printf("Interpolation error: phases are not strictly increasing.\n");
+                    terminate = 1;
+                    break;
+                }
+            }
+            if (terminate) break;
+            or_akima_init(PHASE, GE, N_L, 1, gb, gb + N_L, gb + 2 * N_L, mb);
+            or_steffen_init(PHASE, ZZ, N_L, zs, zs + N_L, zs + 2 * N_L, zs + 3 * N_L);
+            for (size_t jj = 0; jj < S_cols && !terminate; jj++) {
+                if (radiates[i * S_cols + jj] != 1) continue;
+                const double shift = phi[i * S_cols + jj], area = cellArea[i * S_cols + jj];
+                const double *VEC = cellParams + (i * S_cols + jj) * 2;
+                const double *CV = correction ? correction + (i * S_cols + jj) * 2 : NULL;
+                for (size_t k = 0; k < N_P; k++) {
+                    double x = phases[k] + shift;
+                    if (x > PHASE[N_L - 1]) { while (x > PHASE[N_L - 1]) x -= two_pi; }
+                    else if (x < PHASE[0]) { while (x < PHASE[0]) x += two_pi; }
+                    if (x < PHASE[0] || x > PHASE[N_L - 1]) {
+                        printf("Interpolation error: phase = %.16e\n", x);
+                        terminate = 1;
+                        break;
+                    }
+                    const double G = or_akima_eval(PHASE, GE, N_L, gb, gb + N_L, gb + 2 * N_L, x);
+                    if (G > 0.0) {
+                        const double Zq = or_steffen_eval(PHASE, N_L, zs, zs + N_L, zs + 2 * N_L, zs + 3 * N_L, x);
+                        for (size_t p = 0; p < N_E; p++) {
+                            const double Ep = energies[p] / Zq;
+                            const double IE = bb_I(Ep, VEC) * BB_NORM;
+                            const double cIE = CV ? bb_I(Ep, CV) * BB_NORM : 0.0;
+                            acc[p * N_P + k] += area * (IE - cIE) * G;
+                        }
+                    }
+                }
+            }
+            if (terminate) break;
+        }
+    }
+    for (size_t q = 0; q < N_E * N_P; q++) flux[q] = acc[q];
+    for (size_t p = 0; p < N_E; p++)
+        for (size_t k = 0; k < N_P; k++) flux[p * N_P + k] /= energies[p] * KEV;
+    free(acc); free(PHASE); free(ZZ); free(AB); free(GE); free(cdefl); free(calt); free(sa); free(sl);
+    free(st); free(gb); free(mb); free(zs);
+    return terminate;
+}

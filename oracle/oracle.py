@@ -91,6 +91,10 @@ def lib():
         L.or_integrate.argtypes = ([d] * 4 + [sz, sz] + [_dp] * 6 + [_ip, sz] +
                                    [_dp] * 5 + [sz, _dp, sz, _dp, sz, _dp, i,
                                                 C.c_void_p, i, C.c_void_p, _dp])
+        L.or_integrate_general.restype = i
+        L.or_integrate_general.argtypes = ([d, d, sz, sz] + [_dp] * 6 + [_ip, sz] +
+                                           [_dp] * 5 + [sz, _dp, sz, _dp, sz, _dp, i,
+                                                        C.c_void_p, _dp])
         L.or_num4d_sequence.argtypes = [C.c_void_p, sz, _dp, _dp, _dp, _dp]
         L.or_construct_closed_mesh.restype = i
         L.or_construct_closed_mesh.argtypes = ([sz, d, d, d, i, _dp, _dp, _dp,
@@ -214,6 +218,27 @@ def integrate(omega, r_s, inclination, cellArea, radial, r_s_over_r, theta,
                            _c(phases), int(atm_ext), atm_ptr,
                            int(image_order_limit or 0),
                            None if correction is None else correction.ctypes.data, flux)
+    return t, flux
+
+
+def integrate_general(omega, inclination, cellArea, radial, r_s_over_r, theta, phi,
+                      cellParams, radiates, deflection, cos_alpha, lag, maxDeflection,
+                      cos_gamma, energies, leaves, phases, image_order_limit=None,
+                      correction=None):
+    """cellmesh/integrator.integrate for the blackbody: cellParams [S, S, 2]
+    per cell (log10 T, ...), `correction` the [S, S, 2] elsewhere parameter
+    array or None.  Returns (status, flux [N_E, N_P])."""
+    S0, S1 = cellArea.shape
+    N_R = deflection.shape[1]
+    flux = np.zeros((len(energies), len(phases)))
+    corr = None if correction is None else _c(correction)
+    t = lib().or_integrate_general(omega, inclination, S0, S1, _c(cellArea), _c(radial),
+                                   _c(r_s_over_r), _c(theta), _c(phi), _c(cellParams),
+                                   _c(radiates, np.int32), N_R, _c(deflection), _c(cos_alpha),
+                                   _c(lag), _c(maxDeflection), _c(cos_gamma), len(energies),
+                                   _c(energies), len(leaves), _c(leaves), len(phases),
+                                   _c(phases), int(image_order_limit or 0),
+                                   None if corr is None else corr.ctypes.data, flux)
     return t, flux
 
 

@@ -97,6 +97,16 @@ int or_integrate(double R, double omega, double r_s, double inclination,
                  int atm_ext, const or_atmosphere *atm, int image_order_limit,
                  const double *correction, double *flux);
 
+/* xpsi/cellmesh/integrator.pyx:48-667 (blackbody): per-cell parameters */
+int or_integrate_general(double omega, double inclination, size_t S_rows, size_t S_cols,
+                         const double *cellArea, const double *radial, const double *r_s_over_r,
+                         const double *theta, const double *phi, const double *cellParams,
+                         const int *radiates, size_t N_R, const double *deflection,
+                         const double *cos_alpha, const double *lag, const double *maxDeflection,
+                         const double *cos_gammaArray, size_t N_E, const double *energies, size_t N_L,
+                         const double *leaves, size_t N_P, const double *phases, int image_order_limit,
+                         const double *correction, double *flux);
+
 /* global_mesh.pyx: closed mesh; colat/rr/maxAlpha/cos_gamma/effGrav [S],
    phis [S], *cellArea the common cell area */
 int or_construct_closed_mesh(size_t S, double R_eq, double zeta, double eps,

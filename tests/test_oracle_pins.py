@@ -13,6 +13,15 @@
   in flat space, without rotation, a polar cap or ring seen pole-on has the
   closed-form flux norm * B(E) * pi R^2 (sin^2 rho_out - sin^2 rho_in) / (E keV)
   at every phase.
+- The integrator without azimuthal invariance (xpsi/cellmesh/integrator.pyx
+  :48-667), per-cell temperatures: seen pole-on in flat space a cap split into
+  two azimuthal halves at two temperatures gives the mean of the two closed
+  forms, and a per-cell elsewhere correction the difference of two; on an
+  inclined, rotating, oblate star with one temperature it converges with the
+  leaf count to the invariant integrator (independent interpolation: GEOM and
+  Z there against the specific-intensity profile here); and rolling a
+  temperature map by one cell along a parallel of a pole-centred mesh shifts
+  the pulse by that cell's azimuth step.
 """
 import math
 
@@ -347,3 +356,130 @@ def test_poisson_likelihood_given_background_brute_force():
     lz2, _ = O.poisson_likelihood_given_background(T, edges, c2, [zero], [ph], shifts[:1],
                                                    np.zeros_like(bg))
     assert -2e90 <= lz2 <= -2e89
+
+
+def _general(st, hr, params=None, correction=None, N_L=64, phases=None, invariant=False):
+    """Both integrators on member 0 of `hr`; params/correction are callables
+    of the member returning [S, S, 2] arrays (default: the member's own)."""
+    members, err = O.hot_region_embed(st, hr, st["f"])
+    assert err is None
+    m = members[0]
+    energies = np.array([0.5, 1.0, 2.0, 4.0])
+    leaves = np.linspace(0.0, 2.0 * math.pi, N_L)
+    if phases is None:
+        phases = 2.0 * math.pi * np.linspace(0.0, 1.0, N_L)
+    P = m["params"] if params is None else params(m)
+    C = None if correction is None else correction(m)
+    geo = (m["cellArea"], m["r"], m["r_s_over_r"], m["theta"], m["phi"])
+    rays = (m["deflection"], m["cos_alpha"], m["lag"], m["maxDeflection"], m["cos_gamma"])
+    t, f = O.integrate_general(st["Omega"], st["i"], *geo, P, m["radiates"], *rays, energies, leaves,
+                               phases, correction=C)
+    assert t == 0
+    if not invariant:
+        return f, m
+    t2, f2 = O.integrate(st["Omega"], st["r_s"], st["i"], *geo, m["params"], m["radiates"], *rays,
+                         energies, leaves, phases, R=st["R"], correction=C)
+    assert t2 == 0
+    return f, f2
+
+
+def _flat_cap(S):
+    st = O.spacetime(1e-7, 12.0, 1.0, 1.0, 1e-9, "sphere")
+    hr = dict(super_colatitude=0.0, super_radius=0.5, super_temperature=6.2, phase_shift=0.0,
+              sqrt_num_cells=S, min_sqrt_num_cells=10, max_sqrt_num_cells=400, num_rays=400,
+              num_leaves=32)
+    return st, hr
+
+
+def _closed(st, logT, rho=0.5):
+    energies = np.array([0.5, 1.0, 2.0, 4.0])
+    kT = K_B_OVER_KEV * 10.0 ** logT
+    B = energies ** 3 / np.expm1(energies / kT)
+    return BB_NORM * B * math.pi * st["R"] ** 2 * math.sin(rho) ** 2 / (energies * KEV)
+
+
+def test_general_integrator_two_temperature_halves_pole_on():
+    st, hr = _flat_cap(48)
+
+    def halves(m):
+        P = m["params"].copy()
+        P[:, : P.shape[1] // 2, 0] = 6.0
+        P[:, P.shape[1] // 2:, 0] = 6.3
+        return P
+
+    f, m = _general(st, hr, params=halves, N_L=32)
+    assert np.all(np.abs(f / f[:, :1] - 1.0) < 1e-9)
+    exact = 0.5 * (_closed(st, 6.0) + _closed(st, 6.3))
+    assert np.max(np.abs(f[:, 0] / exact - 1.0)) < 3e-5
+    # one temperature everywhere: the invariant integrator's value, to rounding
+    g, h = _general(st, hr, N_L=32, invariant=True)
+    assert np.max(np.abs(g / h - 1.0)) < 1e-11
+
+
+def test_general_integrator_per_cell_correction_pole_on():
+    st, hr = _flat_cap(48)
+
+    def corr(m):
+        C = np.ones_like(m["params"])
+        C[..., 0] = 6.0
+        return C
+
+    f, f_inv = _general(st, hr, correction=corr, N_L=32, invariant=True)
+    exact = _closed(st, 6.2) - _closed(st, 6.0)
+    assert np.max(np.abs(f[:, 0] / exact - 1.0)) < 3e-5
+    assert np.max(np.abs(f / f_inv - 1.0)) < 1e-11
+
+
+def test_general_integrator_converges_to_invariant_inclined():
+    st = O.spacetime(1.4, 12.0, 1.0, math.cos(1.0), 300.0, "AGM_14")
+    hr = dict(super_colatitude=1.2, super_radius=0.8, super_temperature=6.3, phase_shift=0.0,
+              sqrt_num_cells=32, min_sqrt_num_cells=10, max_sqrt_num_cells=100, num_rays=512,
+              num_leaves=64)
+    errs = []
+    for N_L in (64, 128):
+        f, f_inv = _general(st, hr, N_L=N_L, invariant=True)
+        errs.append(np.max(np.abs(f - f_inv)) / np.max(np.abs(f_inv)))
+    assert errs[0] < 2e-6 and errs[1] < errs[0] / 3.0
+
+
+def test_general_integrator_rolled_map_shifts_phase():
+    st = O.spacetime(1.4, 12.0, 1.0, math.cos(1.0), 300.0, "AGM_14")
+    hr = dict(super_colatitude=0.0, super_radius=0.6, super_temperature=6.2, phase_shift=0.0,
+              sqrt_num_cells=24, min_sqrt_num_cells=10, max_sqrt_num_cells=100, num_rays=512,
+              num_leaves=64)
+    members, _ = O.hot_region_embed(st, hr, st["f"])
+    S = members[0]["phi"].shape[1]
+    dphi = 2.0 * math.pi / S
+    assert np.allclose(np.diff(members[0]["phi"], axis=1), dphi)
+    phases = dphi * np.arange(S + 1)
+
+    def sector(m, roll=0):
+        P = m["params"].copy()
+        P[:, :, 0] = 6.0
+        P[:, S // 3: S // 3 + 4, 0] = 6.35
+        return np.roll(P, roll, axis=1)
+
+    f, _ = _general(st, hr, params=sector, phases=phases)
+    g, _ = _general(st, hr, params=lambda m: sector(m, 1), phases=phases)
+    # the map is not axisymmetric: the pulse has a shape to shift
+    assert np.max(f) / np.min(f) > 1.05
+    assert np.max(np.abs(g[:, :-1] - f[:, 1:])) < 1e-10 * np.max(np.abs(f))
+    # and where the sector sits: the invariant integrator restricted to the
+    # sector's cells at its temperature plus the rest at theirs
+    m = members[0]
+    energies = np.array([0.5, 1.0, 2.0, 4.0])
+    leaves = np.linspace(0.0, 2.0 * math.pi, 64)
+    geo = (m["cellArea"], m["r"], m["r_s_over_r"], m["theta"], m["phi"])
+    rays = (m["deflection"], m["cos_alpha"], m["lag"], m["maxDeflection"], m["cos_gamma"])
+    hot = np.zeros_like(m["radiates"])
+    hot[:, S // 3: S // 3 + 4] = 1
+    parts = []
+    for logT, mask in ((6.35, hot), (6.0, 1 - hot)):
+        P = m["params"].copy()
+        P[..., 0] = logT
+        t, fl = O.integrate(st["Omega"], st["r_s"], st["i"], *geo, P, mask * m["radiates"], *rays,
+                            energies, leaves, phases, R=st["R"])
+        assert t == 0
+        parts.append(fl)
+    ref = parts[0] + parts[1]
+    assert np.max(np.abs(f - ref)) < 5e-6 * np.max(np.abs(ref))
