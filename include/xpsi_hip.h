@@ -64,6 +64,30 @@ int xh_integrate(double omega, double r_s, double inclination, int S,
                  double else_logT, double* flux, void* workspace,
                  size_t workspace_bytes, int* status_out, void* stream);
 
+/* Replaces xpsi/cellmesh/integrator.pyx:48-667 `integrate(...) -> (status,
+ * flux[N_E, N_P])`, the integrator without azimuthal invariance
+ * (HotRegion.symmetry False), for the blackbody (hot_atm_ext 1): every cell
+ * has its own parameters.  Same arrays as xh_integrate; cellParams [S, S, 2]
+ * with log10 T first per cell; correction: [S, S, 2] per-cell blackbody
+ * elsewhere parameters (log10 T first) whose intensity every cell subtracts
+ * (else_atm_ext 1), or NULL.  9 N_L and 5 N_R + 8 N_L doubles must each fit
+ * in 159 KiB of one block's LDS (N_L up to about 2000).
+ * workspace: xh_integrate_general_workspace_bytes() bytes.
+ * *status_out = 0 success, 1 numerical failure (the reference's ERROR). */
+size_t xh_integrate_general_workspace_bytes(int S, int N_L);
+int xh_integrate_general(double omega, double inclination, int S,
+                         const double* cellArea, const double* radial,
+                         const double* r_s_over_r, const double* theta,
+                         const double* phi, const double* cellParams,
+                         const int* radiates, int N_R, const double* deflection,
+                         const double* cos_alpha, const double* lag,
+                         const double* maxDeflection, const double* cos_gamma,
+                         int N_E, const double* energies, int N_L,
+                         const double* leaves, int N_P, const double* phases,
+                         int image_order_limit, const double* correction,
+                         double* flux, void* workspace, size_t workspace_bytes,
+                         int* status_out, void* stream);
+
 /* Replaces xpsi/cellmesh/integrator_for_time_invariance.pyx:59-98
  * `integrate(...) -> (status, flux[N_E])` for the blackbody elsewhere
  * atmosphere (else_atm_ext 1).  S rings of S cells; cellArea: the common cell

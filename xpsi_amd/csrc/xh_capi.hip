@@ -454,6 +454,80 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     return XH_OK;
 }
 
+extern "C" size_t xh_integrate_general_workspace_bytes(int S, int N_L)
+{
+    const int IO = 8;
+    size_t b = 256;                                  // Member
+    b += 3 * (size_t)S * 8 + (size_t)S * 4;          // theta, logT, logg, ring_ok
+    b = (b + 255) & ~(size_t)255;
+    b += (size_t)S * IO * 4 * N_L * 8;               // geo
+    b += (size_t)S * IO * 4;                         // valid
+    return b + 1024;
+}
+
+extern "C" int xh_integrate_general(double omega, double inclination, int S, const double* cellArea,
+                                    const double* radial, const double* r_s_over_r, const double* theta,
+                                    const double* phi, const double* cellParams, const int* radiates, int N_R,
+                                    const double* deflection, const double* cos_alpha, const double* lag,
+                                    const double* maxDeflection, const double* cos_gamma, int N_E,
+                                    const double* energies, int N_L, const double* leaves, int N_P,
+                                    const double* phases, int image_order_limit, const double* correction,
+                                    double* flux, void* workspace, size_t workspace_bytes, int* status_out,
+                                    void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1)
+        return fail(XH_EINVAL, "xh_integrate_general: bad sizes");
+    // the geometry kernel keeps a few static LDS words beside its dynamic tables
+    const int gen_lds_max = 159 * 1024;
+    if (general_fold_lds(N_L) > (size_t)gen_lds_max || general_geo_lds(N_R, N_L) > (size_t)gen_lds_max)
+        return fail(XH_EINVAL, "xh_integrate_general: too many leaves or rays for one block's LDS");
+    if ((size_t)N_E > 65535u * 32u || (size_t)((N_E + 31) / 32) * ((N_P + 63) / 64) > 65535u)
+        return fail(XH_EINVAL, "xh_integrate_general: too many energies x phases");
+    if (workspace_bytes < xh_integrate_general_workspace_bytes(S, N_L))
+        return fail(XH_EINVAL, "xh_integrate_general: workspace too small");
+    static bool attr = false;
+    if (!attr) {
+        HIPCHK(hipFuncSetAttribute((const void*)&k_general_geometry, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   gen_lds_max));
+        HIPCHK(hipFuncSetAttribute((const void*)&k_general_fold, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   gen_lds_max));
+        attr = true;
+    }
+    const int IO = 8;
+    char* w = (char*)workspace;
+    Member* d_m = (Member*)w; w += 256;
+    double* d_theta = (double*)w; w += (size_t)S * 8;
+    double* d_logT = (double*)w; w += (size_t)S * 8;
+    double* d_logg = (double*)w; w += (size_t)S * 8;
+    int* d_ok = (int*)w; w += (size_t)S * 4;
+    w = (char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+    double* d_geo = (double*)w; w += (size_t)S * IO * 4 * N_L * 8;
+    int* d_valid = (int*)w;
+    Member hm{};
+    hm.Omega = omega; hm.inclination = inclination; hm.active = 1; hm.S = S;
+    HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
+    Layout L{S, N_R, N_L, N_P, N_E, IO, image_order_limit > 0 ? image_order_limit : 0, 1};
+    hipLaunchKernelGGL(k_ring_prep, dim3(1), dim3(256), 0, s, S, theta, cellParams, radiates, d_theta, d_ok, d_logT,
+                       d_logg);
+    HIPCHK(hipGetLastError());
+    RingArgs RA{d_theta, radial, r_s_over_r, cos_gamma, maxDeflection, deflection, cos_alpha, lag, leaves, d_ok,
+                d_geo, d_valid, nullptr};
+    hipLaunchKernelGGL(k_general_geometry, dim3(S), dim3(128), general_geo_lds(N_R, N_L), s, d_m, L, RA);
+    HIPCHK(hipGetLastError());
+    HIPCHK(fill(flux, (size_t)N_E * N_P, 0.0, s));
+    GeneralArgs GA{d_geo, d_valid, cellArea, radiates, phi, cellParams, correction, energies, phases, flux};
+    const int tiles = ((N_E + 31) / 32) * ((N_P + 63) / 64);
+    hipLaunchKernelGGL(k_general_fold, dim3(S, IO, tiles), dim3(256), general_fold_lds(N_L), s, d_m, L, GA);
+    HIPCHK(hipGetLastError());
+    Member back{};
+    HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (back.status == 4) return fail(XH_ERANGE, "xh_integrate_general: more image orders than supported (8)");
+    *status_out = back.status ? 1 : 0;
+    return XH_OK;
+}
+
 extern "C" size_t xh_integrate_time_invariant_workspace_bytes(int S, int N_E)
 {
     return 256 + (size_t)S * N_E * 8 + 256;

@@ -114,6 +114,24 @@ struct FoldArgs {
     const double* atm_S;
 };
 
+// integrator without azimuthal invariance (blackbody, per-cell temperatures)
+struct GeneralArgs {
+    const double* geo;         // [ring][image][4][N_L]: PHASE, Z, ABB, GEOM
+    const int* valid;          // [ring][image]
+    const double* cellArea;    // [S][S]
+    const int* radiates;       // [S][S]
+    const double* phi;         // [S][S]
+    const double* params;      // [S][S][2], log10 T first
+    const double* correction;  // [S][S][2] elsewhere log10 T first, or nullptr
+    const double* energies;    // N_E
+    const double* phases;      // N_P (radians)
+    double* flux;              // [N_E][N_P], zeroed beforehand
+};
+__global__ void k_general_geometry(Member* members, Layout L, RingArgs A);
+__global__ void k_general_fold(Member* members, Layout L, GeneralArgs F);
+inline size_t general_geo_lds(int N_R, int N_L) { return (size_t)(5 * N_R + 8 * N_L) * 8 + N_L + 16; }
+inline size_t general_fold_lds(int N_L) { return (size_t)9 * N_L * 8; }
+
 // wave-autonomous moment fold (precision form, blackbody): one wavefront per
 // (ring block, 64 energies x 64 phases, member); flux zeroed beforehand
 __global__ void k_wave_fold(Member* members, Layout L, FoldArgs F);
