@@ -97,6 +97,27 @@ int or_integrate(double R, double omega, double r_s, double inclination,
                  int atm_ext, const or_atmosphere *atm, int image_order_limit,
                  const double *correction, double *flux);
 
+/* xpsi/cellmesh/integratorIQU_for_azimuthal_invariance.pyx:68-746 with the
+ * options shared by the plain invariant integrator: disk blocking (R_in),
+ * atmospheres 1 and 3, beaming, the per-cell blackbody correction.  flux_Q
+ * and flux_U may be NULL (intensity only).  Returns 0, 1 (the reference's
+ * ERROR) or 2 (an option outside the restatement). */
+int or_integrate_stokes(double omega, double inclination, size_t S_rows, size_t S_cols, const double *cellArea,
+                        const double *radial, const double *r_s_over_r, const double *theta, const double *phi,
+                        const double *cellParams, size_t n_params, const int *radiates, size_t N_R,
+                        const double *deflection, const double *cos_alpha, const double *lag,
+                        const double *maxDeflection, const double *cos_gammaArray, size_t N_E,
+                        const double *energies, size_t N_L, const double *leaves, size_t N_P,
+                        const double *phases, int atm_ext, int beam_opt, int image_order_limit, double R_in,
+                        const double *correction, double *flux_I, double *flux_Q, double *flux_U);
+double or_hot_I(int atm, int beam_opt, double E, double mu, const double *VEC);
+double or_pol_angle(double leaf, double sin_psi, double cos_psi, double sin_alpha, double cos_alpha,
+                    double sin_theta, double cos_theta, double sin_i, double cos_i, double sin_gamma,
+                    double cos_gamma, double Grav_z, double mu, double eta, double beta, double Lorentz,
+                    double cos_xi);
+int or_disk_block(double R_in, double cos_i, double cos_psi, double cos_theta, double r_s_over_r, double radius,
+                  double sin_alpha, double theta_over_pi);
+
 /* xpsi/cellmesh/integrator.pyx:48-667 (blackbody): per-cell parameters */
 int or_integrate_general(double omega, double inclination, size_t S_rows, size_t S_cols,
                          const double *cellArea, const double *radial, const double *r_s_over_r,

@@ -22,6 +22,18 @@
   Z there against the specific-intensity profile here); and rolling a
   temperature map by one cell along a parallel of a pole-centred mesh shifts
   the pulse by that cell's azimuth step.
+- The Stokes integrator and the hot-region options
+  (integratorIQU_for_azimuthal_invariance.pyx:68-746, common_functions.pyx,
+  hot_BB_burst.pyx, hot_wrapper.pyx:115-233): the reference's own checks of
+  the polarisation angle (its non-relativistic limit, xpsi/tests/
+  test_cellmesh.py:38-95) and of disk blocking (a disk inside the star blocks
+  every southern ray, :97-137); with the blackbody it reproduces the plain
+  integrator bit for bit with Q = U = 0; pole-on in flat space the burst
+  atmosphere's and the beamed intensity's fluxes are closed forms (integrals
+  of mu^n over the cap), Q and U cancel, and beam option 3's quadrature
+  converges to option 2; a small spot's Q/I and U/I follow
+  PD(mu) cos 2chi_0, PD(mu) sin 2chi_0 around the rotation; a northern spot
+  is never blocked and a southern one behind a disk inside the star is dark.
 """
 import math
 
@@ -483,3 +495,144 @@ def test_general_integrator_rolled_map_shifts_phase():
         parts.append(fl)
     ref = parts[0] + parts[1]
     assert np.max(np.abs(f - ref)) < 5e-6 * np.max(np.abs(ref))
+
+
+# ---------------------------------------------------------------- Stokes and options
+def _stokes(st, m, params=None, atm=1, beam=0, R_in=1e6, N_L=64, phases=None, stokes=True):
+    energies = np.array([0.5, 1.0, 2.0, 4.0])
+    leaves = np.linspace(0.0, 2.0 * math.pi, N_L)
+    if phases is None:
+        phases = 2.0 * math.pi * np.linspace(0.0, 1.0, N_L)
+    P = m["params"] if params is None else params
+    return O.integrate_stokes(st["Omega"], st["i"], m["cellArea"], m["r"], m["r_s_over_r"], m["theta"],
+                              m["phi"], P, m["radiates"], m["deflection"], m["cos_alpha"], m["lag"],
+                              m["maxDeflection"], m["cos_gamma"], energies, leaves, phases, atm_ext=atm,
+                              beam_opt=beam, R_in=R_in, stokes=stokes)
+
+
+def _member(st, **kw):
+    hr = dict(super_temperature=6.3, phase_shift=0.0, sqrt_num_cells=32, min_sqrt_num_cells=10,
+              max_sqrt_num_cells=100, num_rays=512, num_leaves=64)
+    hr.update(kw)
+    members, err = O.hot_region_embed(st, hr, st["f"])
+    assert err is None
+    return members[0]
+
+
+def test_pol_angle_non_relativistic_limit():
+    rng = np.random.default_rng(1234)
+    for _ in range(10):
+        th = rng.uniform(0.001, math.pi - 0.001)
+        ph = rng.uniform(0.0, 2.0 * math.pi)
+        inc = rng.uniform(0.001, math.pi - 0.001)
+        chi = O.pol_angle(ph, 0.3, math.sqrt(1 - 0.09), 0.4, math.sqrt(1 - 0.16), math.sin(th), math.cos(th),
+                          math.sin(inc), math.cos(inc), 0.0, 1.0, 1.0, 0.5, 1.0, 0.0, 1.0, 0.5)
+        ref = math.atan2(-math.sin(th) * math.sin(ph),
+                         math.sin(inc) * math.cos(th) - math.cos(inc) * math.sin(th) * math.cos(ph))
+        assert abs(chi - ref) < 1e-12
+    assert math.isfinite(O.pol_angle(0.1, 0.2, 0.98, 0.3, 0.95, 0.4, 0.92, 0.5, 0.86, 0.6, 0.8, 1.0, 0.7, 1.0,
+                                     0.2, 1.1, 0.9))
+
+
+def test_disk_blocks_southern_rays_inside_the_star():
+    rng = np.random.default_rng(1234)
+    for _ in range(10):
+        # the reference test's draws, in its order
+        radius = rng.uniform(5.0, 16.0)
+        R_in = rng.uniform(0.1, radius * 0.999)
+        inc = rng.uniform(0.001, math.pi / 2 - 0.001)
+        psi = rng.uniform(0.001, math.pi - 0.001)
+        th = rng.uniform(math.pi / 2 + 0.01, math.pi - 0.01)
+        sin_alpha = rng.uniform(0.1, 0.9)
+        r_s_over_r = rng.uniform(0.01, 0.2)
+        args = (R_in, math.cos(inc), math.cos(psi), math.cos(th), r_s_over_r, radius, sin_alpha)
+        assert O.disk_block(*args, th / math.pi) == 0
+        # the northern hemisphere is never blocked
+        assert O.disk_block(*args, (math.pi - th) / math.pi) == 1
+
+
+def test_stokes_blackbody_is_the_plain_integrator():
+    st = O.spacetime(1.4, 12.0, 1.0, math.cos(1.0), 300.0, "AGM_14")
+    m = _member(st, super_colatitude=1.2, super_radius=0.8)
+    t, I, Q, U = _stokes(st, m)
+    assert t == 0
+    energies = np.array([0.5, 1.0, 2.0, 4.0])
+    leaves = np.linspace(0.0, 2.0 * math.pi, 64)
+    phases = 2.0 * math.pi * np.linspace(0.0, 1.0, 64)
+    t2, ref = O.integrate(st["Omega"], st["r_s"], st["i"], m["cellArea"], m["r"], m["r_s_over_r"], m["theta"],
+                          m["phi"], m["params"], m["radiates"], m["deflection"], m["cos_alpha"], m["lag"],
+                          m["maxDeflection"], m["cos_gamma"], energies, leaves, phases, R=st["R"])
+    assert t2 == 0
+    assert np.array_equal(I, ref)
+    assert not np.any(Q) and not np.any(U)
+
+
+def test_burst_and_beamed_flat_pole_on_closed_forms():
+    st, hr = _flat_cap(48)
+    members, err = O.hot_region_embed(st, hr, st["f"])
+    m = members[0]
+    energies = np.array([0.5, 1.0, 2.0, 4.0])
+    kT = K_B_OVER_KEV * 10.0 ** 6.2
+    base = BB_NORM * energies ** 3 / np.expm1(energies / kT) * 2.0 * math.pi * st["R"] ** 2 / (energies * KEV)
+    rho = 0.5
+    mu1 = math.sin(rho) ** 2 / 2.0            # integral of mu over the cap / (2 pi R^2)
+    mu2 = (1.0 - math.cos(rho) ** 3) / 3.0    # integral of mu^2
+    t, I, Q, U = _stokes(st, m, atm=3, N_L=32)
+    assert t == 0
+    assert np.max(np.abs(I[:, 0] / (base * (0.421 * mu1 + 0.868 * mu2)) - 1.0)) < 3e-5
+    assert np.all(np.abs(I / I[:, :1] - 1.0) < 1e-9)
+    assert np.max(np.abs(Q)) < 1e-8 * np.max(I) and np.max(np.abs(U)) < 1e-8 * np.max(I)
+    P = np.zeros(m["params"].shape[:2] + (7,))
+    P[..., :2] = m["params"]
+    P[..., 2] = 0.3          # a: 1 + a mu (c = 0, so E^c = 1)
+    P[..., 6] = 100
+    t, I1 = _stokes(st, m, params=P, beam=1, N_L=32, stokes=False)
+    assert np.max(np.abs(I1[:, 0] / (base * (mu1 + 0.3 * mu2)) - 1.0)) < 3e-5
+    t, I2 = _stokes(st, m, params=P, beam=2, N_L=32, stokes=False)
+    anorm = 0.5 / (0.5 + 0.3 / 3.0)
+    assert np.max(np.abs(I2[:, 0] / (anorm * base * (mu1 + 0.3 * mu2)) - 1.0)) < 3e-5
+    # energy-dependent terms: 1 + a E^c mu + b E^d mu^2 (E' = E here: no redshift)
+    mu3 = (1.0 - math.cos(rho) ** 4) / 4.0
+    P[..., 3], P[..., 4], P[..., 5] = 0.4, 1.0, 0.5
+    t, I1e = _stokes(st, m, params=P, beam=1, N_L=32, stokes=False)
+    assert np.max(np.abs(I1e[:, 0] / (base * (mu1 + 0.3 * energies * mu2 + 0.4 * energies ** 0.5 * mu3)) - 1.0)) < 3e-5
+    P[..., 3], P[..., 4], P[..., 5] = 0.0, 0.0, 0.0
+    errs = []
+    for n in (10, 100, 1000):
+        P[..., 6] = n
+        t, I3 = _stokes(st, m, params=P, beam=3, N_L=32, stokes=False)
+        errs.append(np.max(np.abs(I3 / I2 - 1.0)))
+    assert errs[2] < 1e-6 and errs[1] < errs[0] / 50.0 and errs[2] < errs[1] / 50.0
+
+
+def test_small_spot_polarisation_follows_pd_and_chi0():
+    st = O.spacetime(1e-7, 12.0, 1.0, math.cos(1.0), 1e-9, "sphere")
+    th0 = 1.1
+    m = _member(st, super_colatitude=th0, super_radius=0.01, sqrt_num_cells=16, max_sqrt_num_cells=64,
+                num_rays=400)
+    ph = 2.0 * math.pi * np.linspace(0.0, 1.0, 64)
+    t, I, Q, U = _stokes(st, m, atm=3, phases=ph)
+    assert t == 0
+    inc = st["i"]
+    cpsi = math.cos(inc) * math.cos(th0) + math.sin(inc) * math.sin(th0) * np.cos(ph)
+    PD = 0.1171 * (cpsi - 1.0) / (1.0 + 3.582 * cpsi)
+    chi0 = np.arctan2(-math.sin(th0) * np.sin(ph),
+                      math.sin(inc) * math.cos(th0) - math.cos(inc) * math.sin(th0) * np.cos(ph))
+    good = cpsi > 0.3
+    assert np.count_nonzero(good) > 20
+    assert np.max(np.abs(Q[1][good] / I[1][good] - PD[good] * np.cos(2 * chi0[good]))) < 3e-5
+    assert np.max(np.abs(U[1][good] / I[1][good] - PD[good] * np.sin(2 * chi0[good]))) < 3e-5
+
+
+def test_disk_blocking_in_the_integrator():
+    st = O.spacetime(1.4, 12.0, 1.0, math.cos(0.5), 300.0, "AGM_14")
+    south = _member(st, super_colatitude=2.2, super_radius=0.4, sqrt_num_cells=24)
+    t, I0 = _stokes(st, south, stokes=False)
+    assert np.max(I0) > 0.0
+    t, Ib = _stokes(st, south, R_in=5e3, stokes=False)
+    assert t == 0 and not np.any(Ib)
+    # more of the southern rays pass as the disk's inner edge recedes
+    sums = [_stokes(st, south, R_in=r, stokes=False)[1].sum() for r in (1.3e4, 1.6e4, 2.0e4)]
+    assert 0.0 < sums[0] < sums[1] <= sums[2] <= I0.sum() * (1 + 1e-12)
+    north = _member(st, super_colatitude=0.8, super_radius=0.5)
+    assert np.array_equal(_stokes(st, north, R_in=5e3, stokes=False)[1], _stokes(st, north, stokes=False)[1])
