@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--fold", default="auto",
                     help="spline fold of the batched form: auto (autotuned per energy group) or kernel/shape, "
                          "e.g. waves/2 or moment/2")
+    ap.add_argument("--wave-chain", type=int, default=-1,
+                    help="intervals per FP32 chain of the wave fold (-1 = library default, 0 = per parallel)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
@@ -98,6 +100,9 @@ def main():
     if args.fold_ec:
         from xpsi_amd import lib
         lib().xh_set_fold_energy_chunk(args.fold_ec)
+    if args.wave_chain >= 0:
+        from xpsi_amd import lib
+        lib().xh_set_wave_fold_chain(args.wave_chain)
     dev = torch.cuda.get_device_name(0)
     stream = torch.cuda.current_stream().cuda_stream
 

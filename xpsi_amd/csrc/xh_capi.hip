@@ -79,7 +79,7 @@ static size_t fold_lds(int N_L, int EC, int S_max)
     return ((size_t)N_L * EC + 4 * (size_t)(N_L - 1) * EC + 5 * (size_t)N_L + 2 * (size_t)S_max) * 8;
 }
 
-static size_t geo_lds(int N_R, int N_L) { return (5 * (size_t)N_R + 5 * (size_t)N_L) * 8 + (N_L / 2 + 2) * 4; }
+static size_t geo_lds(int N_R, int N_L) { return (5 * (size_t)N_R + 6 * (size_t)N_L) * 8 + (N_L / 2 + 2) * 4; }
 
 template <int EC, int K, int ATM>
 static hipError_t launch_fold_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
@@ -238,6 +238,19 @@ extern "C" int xh_set_fold_rings(int rings)
 {
     if (rings < 0) return fail(XH_EINVAL, "parallels per fold block must be >= 0");
     g_fold_rings = rings;
+    return XH_OK;
+}
+
+// intervals per FP32 accumulation chain of the wave fold before the chain's
+// sums are added to the FP64 flux (0 = one chain per parallel).  Longer chains
+// round more; their error is shared by neighbouring bins, so it shows in the
+// likelihood as a small scale error.
+static int g_wave_chain = 8;
+
+extern "C" int xh_set_wave_fold_chain(int intervals)
+{
+    if (intervals < 0) return fail(XH_EINVAL, "wave-fold chain length must be >= 0");
+    g_wave_chain = intervals;
     return XH_OK;
 }
 
@@ -981,6 +994,7 @@ struct xh_pipeline {
     int fold_kind;               // 1 per-point fold, 2 moment form, 3 energy lanes, 4 wave fold
     int fold_rings;              // parallels per fold block (0 = automatic), fixed at creation
     int wave_rings;              // parallels per wave-fold wavefront (0 = automatic)
+    int wave_chain;              // intervals per wave-fold FP32 chain (0 = per parallel)
     int shape;                   // moment form: block shape T (0 = by batch size)
     int lane_nq;                 // energy-lane fold: phases per wave
     bool fp32;                   // precision form
@@ -1097,6 +1111,7 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         p->shape = g_moment_shape;
         p->fold_rings = g_fold_rings;
         p->wave_rings = g_wave_rings;
+        p->wave_chain = g_wave_chain;
         while (p->shape > 1 && moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->shape, c->atm_ext) > kMomentLdsMax)
             p->shape >>= 1;
         const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->shape > 0 ? p->shape : 1, c->atm_ext) <= kMomentLdsMax;
@@ -1534,6 +1549,7 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     FoldArgs FA{p->d_geo, p->d_valid, p->d_cellArea, nullptr, p->d_phi, 0, p->d_logT, p->d_effGrav, p->d_lastq,
                 p->d_atm_axes, p->d_atm_I, {p->atm_N[0], p->atm_N[1], p->atm_N[2], p->atm_N[3]}, p->d_energies,
                 p->d_phases, p->d_flux, rb, nullptr};
+    FA.chain = p->wave_chain;
     if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
     if (p->d_slab && p->fold_kind == 2) {
         hipLaunchKernelGGL(k_table_slab, dim3(8, S, nm), dim3(256), 0, s, (const Member*)p->d_members, L, FA,

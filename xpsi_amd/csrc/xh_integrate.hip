@@ -55,7 +55,8 @@ __global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, 
     double* As = Zs + N_L;              // N_L
     double* Gs = As + N_L;              // N_L
     double* PV = Gs + N_L;              // N_L
-    int* vis = (int*)(PV + N_L);        // N_L/2+1
+    double* aPH = PV + N_L;             // N_L: leaf phases by slot, copied in at their step
+    int* vis = (int*)(aPH + N_L);       // N_L/2+1
     __shared__ int jn_s, term, shown;
     const size_t tb = ro * N_R;
     for (int q = tid; q < N_R; q += nt) {
@@ -111,15 +112,23 @@ __global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, 
                 Zs[kd] = eta * Grav_z;
                 As[kd] = g.mu * eta;
                 Gs[kd] = g.mu * fabs(g.deriv) * Grav_z * eta * eta * eta / superlum;
-                PH[kd] = leaves[kd] + g.phase_lag;
+                aPH[kd] = leaves[kd] + g.phase_lag;
             }
         }
         __syncthreads();
         if (tid == 0) {
-            // invisible-phase padding, in the reference's leaf order (:480-549)
+            // invisible-phase padding, in the reference's leaf order (:480-549);
+            // a visible leaf's phases enter at its own step, so a padding of an
+            // earlier step is overwritten where the reference overwrites it
             int invis = 2, invis_phase = 0;
             for (int k = 0; k < leaf_lim; k++) {
                 if (vis[k]) {
+                    for (int ks = 0; ks < 2; ks++) {
+                        bool take = (0 < k && k < leaf_lim - 1) || (k == 0 && ks == 0) ||
+                                    (k == leaf_lim - 1 && N_L % 2 == 1 && ks == 0) ||
+                                    (k == leaf_lim - 1 && N_L % 2 == 0);
+                        if (take) PH[ks == 0 ? k : N_L - 1 - k] = aPH[ks == 0 ? k : N_L - 1 - k];
+                    }
                     if (k == 0) {
                         PH[N_L - 1] = PH[0] + TWO_PI;
                         Zs[N_L - 1] = Zs[0]; As[N_L - 1] = As[0]; Gs[N_L - 1] = Gs[0];

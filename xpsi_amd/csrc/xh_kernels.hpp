@@ -112,6 +112,9 @@ struct FoldArgs {
     // denominators 1 / prod_{m != k} (x_{b+k} - x_{b+m}), axes one after
     // another; or nullptr (then formed at each query)
     const double* atm_S;
+    // wave fold: intervals per FP32 accumulation chain before its sums go to
+    // the FP64 flux (0 = one chain per parallel)
+    int chain;
 };
 
 // integrator without azimuthal invariance (blackbody, per-cell temperatures)

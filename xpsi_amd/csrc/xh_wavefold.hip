@@ -30,9 +30,12 @@
 //     its lane, and every phase lane evaluates its own cells of the interval.
 // There are no block barriers and no LDS tables of coefficients or moments:
 // LDS holds only the parallel's cell azimuths and areas and the image's leaf
-// geometry.  Each parallel's FP32 sums are scaled in FP64 and added into the
-// flux with FP64 atomics, so parallels of one member may be spread over any
-// number of waves; the flux is zeroed before the launch.
+// geometry.  The FP32 sums of each accumulation chain (a parallel's
+// intervals, cut every F.chain MFMA steps so that the rounding of a long chain
+// does not add up into an error shared by neighbouring bins) are scaled in
+// FP64 and added into the flux with FP64 atomics, so parallels of one member
+// may be spread over any number of waves; the flux is zeroed before the
+// launch.
 //
 // Cell azimuths must increase along a parallel and span less than 2 pi (the
 // equal-area meshes do): then each phase lane's wrapped points are one
@@ -120,7 +123,7 @@ __device__ __forceinline__ int piece_class(float a, float b, float c, float d, f
 
 // grid: x = ring block, y = energy chunk x phase chunk (64 each), z = member;
 // 64 threads; dynamic LDS wave_fold_lds(N_L, S_max)
-__global__ void __launch_bounds__(64, 2) k_wave_fold(Member* members, Layout L, FoldArgs F)
+__global__ void __launch_bounds__(64, 4) k_wave_fold(Member* members, Layout L, FoldArgs F)
 {
     const int mi = blockIdx.z, lane = threadIdx.x;
     if (!members[mi].active || members[mi].status) return;
@@ -182,6 +185,35 @@ __global__ void __launch_bounds__(64, 2) k_wave_fold(Member* members, Layout L, 
         }
         const double inv_kT = 1.0 / (K_B_OVER_KEV * pow(10.0, F.logT[ro]));
         bool any = false;
+        int chain_n = 0;   // MFMA steps in the current FP32 chain
+        // the chain's sums, scaled in FP64 (a_ref, the blackbody norm,
+        // 1 / (E keV)), into the flux; the chain restarts from zero
+        auto flush = [&]() {
+            // the lane index re-read here, so that the flush's addresses are
+            // formed at each flush and not held in registers across the sweep
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int src = 16 * c + 4 * (ln >> 4) + r, ee = e0 + src;
+                    const double sc = a_ref * __shfl(esc, src, 64);
+                    if (ee >= N_E) continue;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const int pq = p0 + 16 * t + (ln & 15);
+                        const float v = acc[c][t][r];
+                        if (pq < N_P && v != 0.f) unsafeAtomicAdd(flux + (size_t)ee * N_P + pq, (double)v * sc);
+                    }
+                }
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[c][t] = v4f{0.f, 0.f, 0.f, 0.f};
+            any = false;
+            chain_n = 0;
+        };
         for (int I = 0; I < L.IO_max; I++) {
             const size_t vix = ro * L.IO_max + I;
             if (!F.valid[vix]) break;
@@ -354,6 +386,8 @@ __global__ void __launch_bounds__(64, 2) k_wave_fold(Member* members, Layout L, 
                             for (int t = 0; t < 4; t++)
                                 acc[c][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(P[c], M[t], acc[c][t], 0, 0, 0);
                         any = true;
+                        // a bounded chain: its rounding stays local to its bins
+                        if (++chain_n == F.chain) flush();
                     }
 
                     // -- sign-changing pieces, point by point with the clipping
@@ -387,27 +421,8 @@ __global__ void __launch_bounds__(64, 2) k_wave_fold(Member* members, Layout L, 
             // a point below its interval: the azimuths were not one sorted turn
             if (__ballot(unsorted) && lane == 0) members[mi].status = 2;
         }
-        if (any) {
-            // the parallel's sums, scaled in FP64: a_ref, the blackbody norm, 1 / (E keV)
-#pragma unroll
-            for (int c = 0; c < 4; c++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int src = 16 * c + 4 * (lane >> 4) + r, ee = e0 + src;
-                    const double sc = a_ref * __shfl(esc, src, 64);
-                    if (ee >= N_E) continue;
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const int pq = p0 + 16 * t + (lane & 15);
-                        const float v = acc[c][t][r];
-                        if (pq < N_P && v != 0.f) unsafeAtomicAdd(flux + (size_t)ee * N_P + pq, (double)v * sc);
-                    }
-                }
-#pragma unroll
-            for (int c = 0; c < 4; c++)
-#pragma unroll
-                for (int t = 0; t < 4; t++) acc[c][t] = v4f{0.f, 0.f, 0.f, 0.f};
-        }
+        // the parallel's last chain
+        if (any) flush();
     }
 }
 

@@ -282,8 +282,11 @@ def autotune_fold(model, params, stream=None, reps=2,
     `params` once untimed and `reps` times with HIP-event stage timing; the
     candidate with the least profile-and-fold time wins.  Its shape is set for
     pipelines created afterwards, and the returned model dict selects its
-    kernel.  Returns (model, timings) with timings {"kernel/shape": fold ms
-    per evaluation, "chosen": "kernel/shape"}."""
+    kernel; every other kernel keeps the best shape measured for it, so a
+    later pipeline that cannot use the winner (the wave fold takes the
+    blackbody precision form only) still gets a tuned shape.  Returns (model,
+    timings) with timings {"kernel/shape": fold ms per evaluation, "chosen":
+    "kernel/shape"}."""
     L = lib()
 
     def select(kernel, shape):
@@ -307,6 +310,10 @@ def autotune_fold(model, params, stream=None, reps=2,
             pl.evaluate(params, stream)
         times["%s/%d" % (kernel, shape)] = pl.stage_ms()["profile_fold"] / reps
         pl.close()
+    for family in ("waves", "moment", "lanes"):
+        own = [k for k in times if k.startswith(family + "/")]
+        if own:
+            select(family, int(min(own, key=times.get).split("/")[1]))
     best = min(times, key=times.get)
     kernel, shape = best.split("/")
     select(kernel, int(shape))
