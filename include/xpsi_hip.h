@@ -64,6 +64,35 @@ int xh_integrate(double omega, double r_s, double inclination, int S,
                  double else_logT, double* flux, void* workspace,
                  size_t workspace_bytes, int* status_out, void* stream);
 
+/* Replaces xpsi/cellmesh/integratorIQU_for_azimuthal_invariance.pyx:68-746
+ * `integrate(...) -> (status, I, Q, U)` (flux_Q, flux_U non-NULL), and serves
+ * integrator_for_azimuthal_invariance.pyx:70-665 for the options xh_integrate
+ * does not take (flux_Q = flux_U = NULL: intensity only): the burst blackbody
+ * atmosphere (atm_ext 3, hot_BB_burst.pyx: I = B (0.421 + 0.868 mu), Q = I *
+ * 0.1171 (mu - 1) / (1 + 3.582 mu)), beaming (beam_opt 1-3,
+ * hot_wrapper.pyx:115-199; 3 with the blackbody only), disk blocking (R_in <
+ * 1e6 m, common_functions.pyx disk_block) and the polarisation angle
+ * (common_functions.pyx compute_pol_ang).  cellParams [S, S, n_params]: log10 T,
+ * log g, and with beaming a, b, c, d, n_mu; correction [S, S, 2] (blackbody
+ * elsewhere log10 T first) or NULL.  Other arrays as xh_integrate.  13 N_L and
+ * 5 N_R + 8 N_L doubles must each fit in 159 KiB of one block's LDS.
+ * workspace: xh_integrate_stokes_workspace_bytes() bytes.
+ * *status_out = 0 success, 1 numerical failure (the reference's ERROR). */
+size_t xh_integrate_stokes_workspace_bytes(int S, int N_R, int N_L);
+int xh_integrate_stokes(double omega, double inclination, int S,
+                        const double* cellArea, const double* radial,
+                        const double* r_s_over_r, const double* theta,
+                        const double* phi, const double* cellParams, int n_params,
+                        const int* radiates, int N_R, const double* deflection,
+                        const double* cos_alpha, const double* lag,
+                        const double* maxDeflection, const double* cos_gamma,
+                        int N_E, const double* energies, int N_L,
+                        const double* leaves, int N_P, const double* phases,
+                        int atm_ext, int beam_opt, int image_order_limit, double R_in,
+                        const double* correction, double* flux_I, double* flux_Q,
+                        double* flux_U, void* workspace, size_t workspace_bytes,
+                        int* status_out, void* stream);
+
 /* Replaces xpsi/cellmesh/integrator.pyx:48-667 `integrate(...) -> (status,
  * flux[N_E, N_P])`, the integrator without azimuthal invariance
  * (HotRegion.symmetry False), for the blackbody (hot_atm_ext 1): every cell

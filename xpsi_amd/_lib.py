@@ -68,6 +68,12 @@ def lib():
     L.xh_integrate.argtypes = ([d, d, d, i32] + [vp] * 7 + [i32] + [vp] * 5 +
                                [i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, i32, i32, d, vp, vp, sz,
                                 C.POINTER(i32), vp])
+    L.xh_integrate_stokes_workspace_bytes.restype = sz
+    L.xh_integrate_stokes_workspace_bytes.argtypes = [i32, i32, i32]
+    L.xh_integrate_stokes.restype = i32
+    L.xh_integrate_stokes.argtypes = ([d, d, i32] + [vp] * 6 + [i32, vp, i32] + [vp] * 5 +
+                                      [i32, vp, i32, vp, i32, vp, i32, i32, i32, d, vp, vp, vp, vp, vp, sz,
+                                       C.POINTER(i32), vp])
     L.xh_integrate_general_workspace_bytes.restype = sz
     L.xh_integrate_general_workspace_bytes.argtypes = [i32, i32]
     L.xh_integrate_general.restype = i32
@@ -204,6 +210,7 @@ def check(rc, what):
 
 EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_integrate_general_workspace_bytes", "xh_integrate_general",
+            "xh_integrate_stokes_workspace_bytes", "xh_integrate_stokes",
             "xh_integrate_time_invariant_workspace_bytes", "xh_integrate_time_invariant",
             "xh_energy_integrator", "xh_fold", "xh_precomputation",
             "xh_marginal_workspace_bytes", "xh_eval_marginal_likelihood", "xh_eval_marginal_likelihood_ex",

@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = ["xh_mesh.hip", "xh_rays.hip", "xh_integrate.hip", "xh_moment.hip", "xh_lanefold.hip", "xh_elsewhere.hip", "xh_signal.hip", "xh_capi.hip",
            "xh_serve.hip", "xh_noalloc.hip", "xh_prior.hip", "xh_atmplugin.hip", "xh_wavefold.hip",
-           "xh_general.hip"]
+           "xh_general.hip", "xh_stokes.hip"]
 LIB = os.path.join(HERE, "libxpsi_hip.so")
 OBJ = os.path.join(HERE, "_build")
 

@@ -4,11 +4,12 @@
 (:657-665): ``(0, flux[N_E, N_P])`` or ``(1, None)``.  It is bound where the
 reference binds its integrator (HotRegion._integrator, HotRegion.py:551-572).
 Supported: the blackbody (hot_atm_ext 1) and four-dimensional table
-(hot_atm_ext 2, `hot_atmosphere` = (logT, logg, mu, logE, buf)) atmospheres
-without beaming or a disk (R_in >= 1e6), with or without the blackbody
-elsewhere correction (else_atm_ext 1) at a temperature uniform over the
-cells (Elsewhere._compute_cellParamVecs); other settings raise
-NotImplementedError.
+(hot_atm_ext 2, `hot_atmosphere` = (logT, logg, mu, logE, buf)) atmospheres,
+with or without the blackbody elsewhere correction (else_atm_ext 1) at a
+temperature uniform over the cells (Elsewhere._compute_cellParamVecs).  The
+burst blackbody (hot_atm_ext 3), beaming (beam_opt 1-3) and disk blocking
+(R_in < 1e6 m) go through the Stokes integrator's kernels (intensity only),
+for the blackbody atmospheres; other settings raise NotImplementedError.
 """
 import ctypes as C
 
@@ -16,6 +17,7 @@ import numpy as np
 
 from .._lib import lib, check
 from .. import _device as D
+from ._stokes import run
 
 
 def integrate(numThreads, R, omega, r_s, inclination, cellArea,
@@ -24,14 +26,22 @@ def integrate(numThreads, R, omega, r_s, inclination, cellArea,
               cos_alpha, lag, maxDeflection, cos_gammaArray, energies, leaves,
               phases, hot_atmosphere, elsewhere_atmosphere, hot_atm_ext,
               else_atm_ext, beam_opt, image_order_limit=None, R_in=1e6):
+    if int(hot_atm_ext) == 3 or int(beam_opt) != 0 or R_in < 1e6:
+        if int(hot_atm_ext) not in (1, 3):
+            raise NotImplementedError("beaming and disk blocking: blackbody atmospheres only")
+        out = run(omega, inclination, cellArea, radialCoords_of_parallels, r_s_over_r, theta, phi,
+                  srcCellParams, CELL_RADIATES, correction_srcCellParams, deflection, cos_alpha, lag,
+                  maxDeflection, cos_gammaArray, energies, leaves, phases, hot_atm_ext, else_atm_ext,
+                  beam_opt, image_order_limit, R_in, stokes=False)
+        return (1, None) if out is None else (0, out[0])
     correct, else_logT = 0, 0.0
     if correction_srcCellParams is not None:
         corr = np.asarray(correction_srcCellParams, dtype=np.float64)
         if int(else_atm_ext) != 1 or not np.all(corr[..., 0] == corr.flat[0]):
             raise NotImplementedError("elsewhere correction: blackbody at one temperature")
         correct, else_logT = 1, float(corr.flat[0])
-    if int(hot_atm_ext) not in (1, 2) or int(beam_opt) != 0 or R_in < 1e6:
-        raise NotImplementedError("blackbody or 4-D table atmospheres without beaming or disk")
+    if int(hot_atm_ext) not in (1, 2):
+        raise NotImplementedError("blackbody, burst blackbody or 4-D table atmospheres")
     S = int(np.shape(cellArea)[0])
     N_R = int(np.shape(deflection)[1])
     N_E, N_L, N_P = len(energies), len(leaves), len(phases)

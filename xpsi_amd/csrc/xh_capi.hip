@@ -79,7 +79,7 @@ static size_t fold_lds(int N_L, int EC, int S_max)
     return ((size_t)N_L * EC + 4 * (size_t)(N_L - 1) * EC + 5 * (size_t)N_L + 2 * (size_t)S_max) * 8;
 }
 
-static size_t geo_lds(int N_R, int N_L) { return (5 * (size_t)N_R + 6 * (size_t)N_L) * 8 + (N_L / 2 + 2) * 4; }
+static size_t geo_lds(int N_R, int N_L) { return (5 * (size_t)N_R + 8 * (size_t)N_L) * 8 + (N_L / 2 + 2) * 4; }
 
 template <int EC, int K, int ATM>
 static hipError_t launch_fold_t(dim3 g, size_t lds, hipStream_t s, const Member* m, Layout L, FoldArgs F)
@@ -433,7 +433,7 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     HIPCHK(hipGetLastError());
     RingArgs RA{d_theta, radial, r_s_over_r, cos_gamma, maxDeflection, deflection, cos_alpha, lag, leaves, d_ok,
                 d_geo, d_valid, d_lastq};
-    hipLaunchKernelGGL(k_geometry, dim3(S, 1, 1), dim3(128), geo_lds(N_R, N_L), s, d_m, L, RA);
+    hipLaunchKernelGGL(k_geometry<0>, dim3(S, 1, 1), dim3(128), geo_lds(N_R, N_L), s, d_m, L, RA);
     HIPCHK(hipGetLastError());
     FoldArgs FA{d_geo, d_valid, cellArea, radiates, phi, S, d_logT, d_logg, d_lastq, atm_axes, atm_I,
                 {0, 0, 0, 0}, energies, phases, flux, S};
@@ -463,6 +463,98 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (back.status == 4) return fail(XH_ERANGE, "xh_integrate: more image orders than supported (8)");
+    *status_out = back.status ? 1 : 0;
+    return XH_OK;
+}
+
+extern "C" size_t xh_integrate_stokes_workspace_bytes(int S, int N_R, int N_L)
+{
+    (void)N_R;
+    const int IO = 8;
+    size_t b = 256;                                  // Member
+    b += 3 * (size_t)S * 8 + (size_t)S * 4;          // theta, logT, logg, ring_ok
+    b = (b + 255) & ~(size_t)255;
+    b += (size_t)S * IO * 5 * N_L * 8;               // geo
+    b += (size_t)S * IO * 2 * N_L * 8;               // cos 2chi, sin 2chi
+    b += 2 * (size_t)S * IO * 4;                     // valid, lastq
+    return b + 1024;
+}
+
+extern "C" int xh_integrate_stokes(double omega, double inclination, int S, const double* cellArea,
+                                   const double* radial, const double* r_s_over_r, const double* theta,
+                                   const double* phi, const double* cellParams, int n_params, const int* radiates,
+                                   int N_R, const double* deflection, const double* cos_alpha, const double* lag,
+                                   const double* maxDeflection, const double* cos_gamma, int N_E,
+                                   const double* energies, int N_L, const double* leaves, int N_P,
+                                   const double* phases, int atm_ext, int beam_opt, int image_order_limit,
+                                   double R_in, const double* correction, double* flux_I, double* flux_Q,
+                                   double* flux_U, void* workspace, size_t workspace_bytes, int* status_out,
+                                   void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1)
+        return fail(XH_EINVAL, "xh_integrate_stokes: bad sizes");
+    if (atm_ext != 1 && atm_ext != 3)
+        return fail(XH_EINVAL, "xh_integrate_stokes: atmosphere extension must be 1 (blackbody) or 3 (burst blackbody)");
+    if (beam_opt < 0 || beam_opt > 3 || (beam_opt == 3 && atm_ext != 1))
+        return fail(XH_EINVAL, "xh_integrate_stokes: beaming option must be 0-3 (3 with the blackbody only)");
+    if (n_params < 2 || (beam_opt != 0 && n_params < 7))
+        return fail(XH_EINVAL, "xh_integrate_stokes: beaming needs 7 parameters per cell (log T, log g, a, b, c, d, n_mu)");
+    if ((flux_Q == nullptr) != (flux_U == nullptr))
+        return fail(XH_EINVAL, "xh_integrate_stokes: Q and U are requested together");
+    const int lds_max = 159 * 1024;
+    if (stokes_fold_lds(N_L) > (size_t)lds_max || geo_lds(N_R, N_L) > (size_t)lds_max)
+        return fail(XH_EINVAL, "xh_integrate_stokes: too many leaves or rays for one block's LDS");
+    const int nt = N_P <= 64 ? 64 : (N_P <= 128 ? 128 : 256);
+    const long tiles = (long)N_E * ((N_P + nt - 1) / nt);
+    if (tiles > 65535) return fail(XH_EINVAL, "xh_integrate_stokes: too many energies x phase tiles");
+    if (workspace_bytes < xh_integrate_stokes_workspace_bytes(S, N_R, N_L))
+        return fail(XH_EINVAL, "xh_integrate_stokes: workspace too small");
+    static bool attr = false;
+    if (!attr) {
+        HIPCHK(hipFuncSetAttribute((const void*)&k_geometry<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+        HIPCHK(hipFuncSetAttribute((const void*)&k_stokes_fold, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+        attr = true;
+    }
+    const int IO = 8;
+    char* w = (char*)workspace;
+    Member* d_m = (Member*)w; w += 256;
+    double* d_theta = (double*)w; w += (size_t)S * 8;
+    double* d_logT = (double*)w; w += (size_t)S * 8;
+    double* d_logg = (double*)w; w += (size_t)S * 8;
+    int* d_ok = (int*)w; w += (size_t)S * 4;
+    w = (char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+    double* d_geo = (double*)w; w += (size_t)S * IO * 5 * N_L * 8;
+    double* d_pol = (double*)w; w += (size_t)S * IO * 2 * N_L * 8;
+    int* d_valid = (int*)w; w += (size_t)S * IO * 4;
+    int* d_lastq = (int*)w;
+    Member hm{};
+    hm.Omega = omega; hm.inclination = inclination; hm.active = 1; hm.S = S;
+    HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
+    Layout L{S, N_R, N_L, N_P, N_E, IO, image_order_limit > 0 ? image_order_limit : 0, atm_ext};
+    hipLaunchKernelGGL(k_ring_prep, dim3(1), dim3(256), 0, s, S, theta, cellParams, radiates, d_theta, d_ok, d_logT,
+                       d_logg);
+    HIPCHK(hipGetLastError());
+    RingArgs RA{d_theta, radial, r_s_over_r, cos_gamma, maxDeflection, deflection, cos_alpha, lag, leaves, d_ok,
+                d_geo, d_valid, d_lastq};
+    RA.disk = R_in < 1.0e6 ? 1 : 0;
+    RA.R_in = R_in;
+    RA.pol = flux_Q ? d_pol : nullptr;
+    hipLaunchKernelGGL(k_geometry<1>, dim3(S, 1, 1), dim3(128), geo_lds(N_R, N_L), s, d_m, L, RA);
+    HIPCHK(hipGetLastError());
+    HIPCHK(fill(flux_I, (size_t)N_E * N_P, 0.0, s));
+    if (flux_Q) {
+        HIPCHK(fill(flux_Q, (size_t)N_E * N_P, 0.0, s));
+        HIPCHK(fill(flux_U, (size_t)N_E * N_P, 0.0, s));
+    }
+    StokesArgs SA{d_geo, RA.pol, d_valid, cellArea, radiates, phi, cellParams, n_params, correction, atm_ext,
+                  beam_opt, energies, phases, flux_I, flux_Q, flux_U};
+    hipLaunchKernelGGL(k_stokes_fold, dim3(S, IO, (unsigned)tiles), dim3(nt), stokes_fold_lds(N_L), s, d_m, L, SA);
+    HIPCHK(hipGetLastError());
+    Member back{};
+    HIPCHK(hipMemcpyAsync(&back, d_m, sizeof(Member), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (back.status == 4) return fail(XH_ERANGE, "xh_integrate_stokes: more image orders than supported (8)");
     *status_out = back.status ? 1 : 0;
     return XH_OK;
 }
@@ -1531,7 +1623,7 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     RingArgs RA{p->d_theta, p->d_r, p->d_u, p->d_cos_gamma, p->d_maxDefl, p->d_defl, p->d_cosa, p->d_lag,
                 p->d_leaves, p->d_ring_ok, p->d_geo, p->d_valid, p->d_lastq};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[2], s));
-    hipLaunchKernelGGL(k_geometry, dim3(S, 1, nm), dim3(128), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
+    hipLaunchKernelGGL(k_geometry<0>, dim3(S, 1, nm), dim3(128), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
     HIPCHK(hipGetLastError());
     if (p->has_else) {
         hipLaunchKernelGGL(k_else_integrate, dim3(Le.S_max, n), dim3(256), else_lds(Le), s, p->d_else, Le,

@@ -25,6 +25,10 @@ namespace xh {
 // then the image orders are taken in turn.  An order contributes only when
 // every lower order has a visible leaf (the reference's image loop), so the
 // walk stops at the first order without one; later orders are marked invalid.
+// OPT = 1 adds the hot-region options of the Stokes integrator (disk
+// blocking, polarisation angles); OPT = 0 is the plain integrator's form,
+// which keeps fewer values live per leaf.
+template <int OPT>
 __global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, Layout L, RingArgs A)
 {
     extern __shared__ double sm[];
@@ -56,7 +60,9 @@ __global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, 
     double* Gs = As + N_L;              // N_L
     double* PV = Gs + N_L;              // N_L
     double* aPH = PV + N_L;             // N_L: leaf phases by slot, copied in at their step
-    int* vis = (int*)(aPH + N_L);       // N_L/2+1
+    double* C2 = aPH + N_L;             // N_L: cos 2chi (Stokes outputs only)
+    double* S2 = C2 + N_L;              // N_L: sin 2chi
+    int* vis = (int*)(S2 + N_L);        // N_L/2+1
     __shared__ int jn_s, term, shown;
     const size_t tb = ro * N_R;
     for (int q = tid; q < N_R; q += nt) {
@@ -86,12 +92,13 @@ __global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, 
     for (int I = 0; I < n_orders; I++) {
         const size_t vix = vbase + I;
         __syncthreads();   // the tables are staged; the previous order's outputs are written
-        for (int k = tid; k < N_L; k += nt) { Gs[k] = 0.0; Zs[k] = 1.0; As[k] = 0.0; PV[k] = -3.0; }
+        for (int k = tid; k < N_L; k += nt) { Gs[k] = 0.0; Zs[k] = 1.0; As[k] = 0.0; PV[k] = -3.0; C2[k] = 0.0; S2[k] = 0.0; }
         if (tid == 0) { term = 0; shown = 0; }
         __syncthreads();
         for (int k = tid; k < leaf_lim; k += nt) {
             LeafGeo g = leaf_geometry(I, leaves[k], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg, maxD,
-                                      D, Ac, Lg, N_R, xalt, yalt, nalt, true);
+                                      D, Ac, Lg, N_R, xalt, yalt, nalt, true, (OPT && A.disk) ? A.R_in : 1.0e6,
+                                      A.u[ro], radius);
             vis[k] = g.calc;
             if (!g.calc) continue;
             for (int ks = 0; ks < 2; ks++) {
@@ -100,9 +107,9 @@ __global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, 
                             (k == leaf_lim - 1 && N_L % 2 == 0);
                 if (!take) continue;
                 int kd = ks == 0 ? k : N_L - 1 - k;
-                double superlum, eta;
+                double superlum, eta, cos_xi = 0.0;
                 if (!eq12(g.psi, 0.0)) {
-                    double cos_xi = g.sin_alpha * sin_i * sin(leaves[kd]) / g.sin_psi;
+                    cos_xi = g.sin_alpha * sin_i * sin(leaves[kd]) / g.sin_psi;
                     superlum = 1.0 + beta * cos_xi;
                     eta = Lorentz / superlum;
                 } else {
@@ -113,6 +120,13 @@ __global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, 
                 As[kd] = g.mu * eta;
                 Gs[kd] = g.mu * fabs(g.deriv) * Grav_z * eta * eta * eta / superlum;
                 aPH[kd] = leaves[kd] + g.phase_lag;
+                if (OPT && A.pol) {
+                    const double chi = pol_angle(leaves[kd], g.sin_psi, g.cos_psi, g.sin_alpha, g.cos_alpha, sin_th,
+                                                 cos_th, sin_i, cos_i, sg, cg, Grav_z, g.mu, eta, beta, Lorentz,
+                                                 cos_xi);
+                    C2[kd] = cos(2 * chi);
+                    S2[kd] = sin(2 * chi);
+                }
             }
         }
         __syncthreads();
@@ -132,6 +146,7 @@ __global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, 
                     if (k == 0) {
                         PH[N_L - 1] = PH[0] + TWO_PI;
                         Zs[N_L - 1] = Zs[0]; As[N_L - 1] = As[0]; Gs[N_L - 1] = Gs[0];
+                        C2[N_L - 1] = C2[0]; S2[N_L - 1] = S2[0];
                     } else if (invis == 2) {
                         double step = leaves[k] / (double)k;
                         for (int q = N_L - k; q < N_L; q++) PH[q] = PH[q - 1] + step;
@@ -187,6 +202,10 @@ __global__ void __launch_bounds__(128) k_geometry(Member* __restrict__ members, 
             out[2 * N_L + q] = As[q];
             out[3 * N_L + q] = Gs[q];
             out[4 * N_L + q] = PV[q];
+        }
+        if (OPT && A.pol) {
+            double* po = A.pol + vix * 2 * N_L;
+            for (int q = tid; q < N_L; q += nt) { po[q] = C2[q]; po[N_L + q] = S2[q]; }
         }
         if (!shown) {
             // no visible leaf at this order: the higher orders do not contribute
@@ -396,5 +415,8 @@ XH_FOLD_INST(64, 4) XH_FOLD_INST(64, 8) XH_FOLD_INST(64, 16)
 XH_FOLD_INST(32, 4) XH_FOLD_INST(32, 8) XH_FOLD_INST(32, 16)
 XH_FOLD_INST(16, 4) XH_FOLD_INST(16, 8) XH_FOLD_INST(16, 16)
 XH_FOLD_INST(8, 4) XH_FOLD_INST(8, 8) XH_FOLD_INST(8, 16)
+
+template __global__ void k_geometry<0>(Member*, Layout, RingArgs);
+template __global__ void k_geometry<1>(Member*, Layout, RingArgs);
 
 }  // namespace xh

@@ -86,6 +86,9 @@ struct RingArgs {
     double* geo;     // [member][ring][image][5][N_L]: PHASE, Z, ABB(mu eta), GEOM, PREV
     int* valid;      // [member][ring][image]
     int* lastq;      // [member][ring][image]: leaf index of the last intensity query
+    int disk;        // a disk blocks rays (R_in below)
+    double R_in;     // the disk's inner edge (m)
+    double* pol;     // [member][ring][image][2][N_L]: cos 2chi, sin 2chi, or nullptr
 };
 
 struct FoldArgs {
@@ -116,6 +119,28 @@ struct FoldArgs {
     // the FP64 flux (0 = one chain per parallel)
     int chain;
 };
+
+// Stokes integrator with azimuthal invariance and the hot-region options
+struct StokesArgs {
+    const double* geo;         // [ring][image][5][N_L] from k_geometry
+    const double* pol;         // [ring][image][2][N_L] cos 2chi, sin 2chi, or nullptr (I only)
+    const int* valid;          // [ring][image]
+    const double* cellArea;    // [S][S]
+    const int* radiates;       // [S][S]
+    const double* phi;         // [S][S]
+    const double* params;      // [S][S][n_params], log10 T first
+    int n_params;
+    const double* correction;  // [S][S][2] elsewhere log10 T first, or nullptr
+    int atm;                   // 1 blackbody, 3 burst blackbody
+    int beam;                  // beaming option 0-3
+    const double* energies;    // N_E
+    const double* phases;      // N_P (radians)
+    double* flux_I;            // [N_E][N_P], zeroed beforehand
+    double* flux_Q;            // same, or nullptr
+    double* flux_U;
+};
+__global__ void k_stokes_fold(Member* members, Layout L, StokesArgs F);
+inline size_t stokes_fold_lds(int N_L) { return (size_t)13 * N_L * 8; }
 
 // integrator without azimuthal invariance (blackbody, per-cell temperatures)
 struct GeneralArgs {
@@ -152,6 +177,7 @@ inline size_t wave_fold_lds(int N_L, int S_max)
 // and shape 1 keeps 4 (the accumulators of a wave stay within 16 doubles).
 __host__ __device__ constexpr int moment_groups(int T) { return T == 4 ? 2 : (T == 2 ? 8 : 4); }
 
+template <int OPT>
 __global__ void k_geometry(Member* members, Layout L, RingArgs A);
 __global__ void k_poisson_bg(MarginalArgs A);
 __global__ void k_star_rate(MarginalArgs A, double* out);
