@@ -371,22 +371,40 @@ int or_integrate(double R, double omega, double r_s, double inclination,
  * (:531-592; the blackbody ignores the interpolated ABB, which only beamed
  * atmospheres read), and every energy's intensity at E / Z is added with the cell's
  * area times GEOM.  Flux [N_E][N_P], divided by E keV at the end (:601-606).
- * Rings are summed in order (one thread). */
-int or_integrate_general(double omega, double inclination, size_t S_rows, size_t S_cols,
-                         const double *cellArea, const double *radial, const double *r_s_over_r,
-                         const double *theta, const double *phi, const double *cellParams,
-                         const int *radiates, size_t N_R, const double *deflection,
-                         const double *cos_alpha, const double *lag, const double *maxDeflection,
-                         const double *cos_gammaArray, size_t N_E, const double *energies, size_t N_L,
-                         const double *leaves, size_t N_P, const double *phases, int image_order_limit,
-                         const double *correction, double *flux)
+ * Rings are summed in order (one thread).  The Stokes form
+ * (xpsi/cellmesh/integratorIQU.pyx:67-717) adds Q and U: the atmosphere's Q
+ * at the interpolated mu eta, rotated by 2 chi, where chi is the polarisation
+ * angle the reference forms inside its cell and phase loop from the leaf
+ * loop's last values (:593-596), hence one angle per parallel and image. */
+int or_integrate_general_ext(double omega, double inclination, size_t S_rows, size_t S_cols,
+                             const double *cellArea, const double *radial, const double *r_s_over_r,
+                             const double *theta, const double *phi, const double *cellParams,
+                             size_t n_params, const int *radiates, size_t N_R, const double *deflection,
+                             const double *cos_alpha, const double *lag, const double *maxDeflection,
+                             const double *cos_gammaArray, size_t N_E, const double *energies, size_t N_L,
+                             const double *leaves, size_t N_P, const double *phases, int image_order_limit,
+                             int atm_ext, int beam_opt, const double *correction, double *flux,
+                             double *flux_Q, double *flux_U)
 {
+    const int pol = flux_Q != NULL && flux_U != NULL;
+    if ((atm_ext != 1 && atm_ext != 3) || beam_opt < 0 || beam_opt > 3 || (beam_opt == 3 && atm_ext != 1) ||
+        (beam_opt != 0 && n_params < 7) || n_params < 2)
+        return 2;
     const double hpi = 0.5 * M_PI, two_pi = 2.0 * M_PI;
     double sin_i = sin(inclination), cos_i = cos(inclination);
     size_t leaf_lim = (N_L % 2 == 0) ? N_L / 2 : (N_L + 1) / 2;
     int image_order = image_order_limit > 0 ? image_order_limit : 0;
     memset(flux, 0, N_E * N_P * sizeof(double));
     double *acc = (double *)calloc(N_E * N_P, sizeof(double));
+    double *accQ = (double *)calloc(N_E * N_P, sizeof(double));
+    double *accU = (double *)calloc(N_E * N_P, sizeof(double));
+    /* the leaf loop's last values, which the Stokes form reads for its
+       polarisation angle (integratorIQU.pyx:593-596): the last leaf written
+       (_kdx, eta, cos_xi), the last leaf (sin psi, cos psi) and the last leaf
+       within the deflection table (sin alpha, cos alpha, mu); they persist
+       from parallel to parallel as the reference's variables do */
+    size_t l_kdx = 0;
+    double l_eta = 0.0, l_cxi = 0.0, l_spsi = 0.0, l_cpsi = 0.0, l_sa = 0.0, l_ca = 0.0, l_mu = 0.0;
     double *PHASE = (double *)calloc(N_L, sizeof(double));
     double *ZZ = (double *)calloc(N_L, sizeof(double));
     double *AB = (double *)calloc(N_L, sizeof(double));
@@ -399,6 +417,7 @@ int or_integrate_general(double omega, double inclination, size_t S_rows, size_t
     double *gb = (double *)malloc(3 * N_L * sizeof(double));   /* Akima, GEOM */
     double *mb = (double *)malloc((N_L + 3) * sizeof(double));
     double *zs = (double *)malloc(4 * N_L * sizeof(double));   /* Steffen, Z */
+    double *as = (double *)malloc(4 * N_L * sizeof(double));   /* Steffen, ABB */
     int terminate = 0;
     for (size_t i = 0; i < S_rows && !terminate; i++) {
         size_t j;
@@ -441,6 +460,8 @@ int or_integrate_general(double omega, double inclination, size_t S_rows, size_t
                     psi = image_deflection(I, acos(cos_psi));
                     sin_psi = sin(psi);
                 }
+                l_spsi = sin_psi;
+                l_cpsi = cos_psi;
                 int calc = 0;
                 double mu = 0.0, sin_alpha = 0.0;
                 if (psi <= maxDeflection[i]) {
@@ -461,6 +482,9 @@ int or_integrate_general(double omega, double inclination, size_t S_rows, size_t
                         else mu = mu - sin_alpha * sin_gamma * cos_delta;
                     }
                     calc = (mu > 0.0) ? 1 : 0;
+                    l_sa = sin_alpha;
+                    l_ca = c_alpha;
+                    l_mu = mu;
                 }
                 if (calc) {
                     if (psi <= hpi && cos_psi >= xalt[0]) {
@@ -487,9 +511,13 @@ int or_integrate_general(double omega, double inclination, size_t S_rows, size_t
                             superlum = 1.0 + beta * cos_xi;
                             eta = Lorentz / superlum;
                         } else {
+                            cos_xi = 0.0;
                             superlum = 1.0;
                             eta = Lorentz;
                         }
+                        l_kdx = kd;
+                        l_eta = eta;
+                        l_cxi = cos_xi;
                         ZZ[kd] = eta * Grav_z;
                         AB[kd] = mu * eta;
                         GE[kd] = mu * fabs(deriv) * Grav_z * eta * eta * eta / superlum;
@@ -565,10 +593,19 @@ int or_integrate_general(double omega, double inclination, size_t S_rows, size_t
             if (terminate) break;
             or_akima_init(PHASE, GE, N_L, 1, gb, gb + N_L, gb + 2 * N_L, mb);
             or_steffen_init(PHASE, ZZ, N_L, zs, zs + N_L, zs + 2 * N_L, zs + 3 * N_L);
+            or_steffen_init(PHASE, AB, N_L, as, as + N_L, as + 2 * N_L, as + 3 * N_L);
+            double c2 = 0.0, s2 = 0.0;
+            if (pol) {
+                const double chi = or_pol_angle(leaves[l_kdx], l_spsi, l_cpsi, l_sa, l_ca, sin_th, cos_th, sin_i,
+                                                cos_i, sin_gamma, cos_gamma, Grav_z, l_mu, l_eta, beta, Lorentz,
+                                                l_cxi);
+                c2 = cos(2 * chi);
+                s2 = sin(2 * chi);
+            }
             for (size_t jj = 0; jj < S_cols && !terminate; jj++) {
                 if (radiates[i * S_cols + jj] != 1) continue;
                 const double shift = phi[i * S_cols + jj], area = cellArea[i * S_cols + jj];
-                const double *VEC = cellParams + (i * S_cols + jj) * 2;
+                const double *VEC = cellParams + (i * S_cols + jj) * n_params;
                 const double *CV = correction ? correction + (i * S_cols + jj) * 2 : NULL;
                 for (size_t k = 0; k < N_P; k++) {
                     double x = phases[k] + shift;
@@ -582,11 +619,18 @@ int or_integrate_general(double omega, double inclination, size_t S_rows, size_t
                     const double G = or_akima_eval(PHASE, GE, N_L, gb, gb + N_L, gb + 2 * N_L, x);
                     if (G > 0.0) {
                         const double Zq = or_steffen_eval(PHASE, N_L, zs, zs + N_L, zs + 2 * N_L, zs + 3 * N_L, x);
+                        const double Aq = or_steffen_eval(PHASE, N_L, as, as + N_L, as + 2 * N_L, as + 3 * N_L, x);
                         for (size_t p = 0; p < N_E; p++) {
                             const double Ep = energies[p] / Zq;
-                            const double IE = bb_I(Ep, VEC) * BB_NORM;
+                            const double IE = (atm_ext == 1 && beam_opt == 0 ? bb_I(Ep, VEC)
+                                                                            : or_hot_I(atm_ext, beam_opt, Ep, Aq, VEC)) * BB_NORM;
                             const double cIE = CV ? bb_I(Ep, CV) * BB_NORM : 0.0;
                             acc[p * N_P + k] += area * (IE - cIE) * G;
+                            if (pol) {
+                                const double QE = or_hot_Q(atm_ext, Ep, Aq, VEC) * BB_NORM;
+                                accQ[p * N_P + k] += area * (QE * c2) * G;
+                                accU[p * N_P + k] += area * (QE * s2) * G;
+                            }
                         }
                     }
                 }
@@ -597,7 +641,28 @@ int or_integrate_general(double omega, double inclination, size_t S_rows, size_t
     for (size_t q = 0; q < N_E * N_P; q++) flux[q] = acc[q];
     for (size_t p = 0; p < N_E; p++)
         for (size_t k = 0; k < N_P; k++) flux[p * N_P + k] /= energies[p] * KEV;
-    free(acc); free(PHASE); free(ZZ); free(AB); free(GE); free(cdefl); free(calt); free(sa); free(sl);
-    free(st); free(gb); free(mb); free(zs);
+    if (pol)
+        for (size_t p = 0; p < N_E; p++)
+            for (size_t k = 0; k < N_P; k++) {
+                flux_Q[p * N_P + k] = accQ[p * N_P + k] / (energies[p] * KEV);
+                flux_U[p * N_P + k] = accU[p * N_P + k] / (energies[p] * KEV);
+            }
+    free(acc); free(accQ); free(accU); free(PHASE); free(ZZ); free(AB); free(GE); free(cdefl); free(calt);
+    free(sa); free(sl); free(st); free(gb); free(mb); free(zs); free(as);
     return terminate;
+}
+
+int or_integrate_general(double omega, double inclination, size_t S_rows, size_t S_cols,
+                         const double *cellArea, const double *radial, const double *r_s_over_r,
+                         const double *theta, const double *phi, const double *cellParams,
+                         const int *radiates, size_t N_R, const double *deflection,
+                         const double *cos_alpha, const double *lag, const double *maxDeflection,
+                         const double *cos_gammaArray, size_t N_E, const double *energies, size_t N_L,
+                         const double *leaves, size_t N_P, const double *phases, int image_order_limit,
+                         const double *correction, double *flux)
+{
+    return or_integrate_general_ext(omega, inclination, S_rows, S_cols, cellArea, radial, r_s_over_r, theta, phi,
+                                    cellParams, 2, radiates, N_R, deflection, cos_alpha, lag, maxDeflection,
+                                    cos_gammaArray, N_E, energies, N_L, leaves, N_P, phases, image_order_limit, 1,
+                                    0, correction, flux, NULL, NULL);
 }

@@ -84,7 +84,7 @@ double or_hot_I(int atm, int beam_opt, double E, double mu, const double *VEC)
 }
 
 /* hot_wrapper.pyx:201-233: Stokes Q of the atmosphere (no beaming) */
-static double hot_Q(int atm, double E, double mu, const double *VEC)
+double or_hot_Q(int atm, double E, double mu, const double *VEC)
 {
     if (atm != 3) return 0.0;
     double I = hot_plain(3, E, mu, VEC);
@@ -268,7 +268,7 @@ int or_integrate_stokes(double omega, double inclination, size_t S_rows, size_t 
                             double cIE = CV ? planck(Ep, CV[0]) * BB_NORM : 0.0;
                             PI_[p * N_L + kd] = (IE * BB_NORM - cIE) * GEOM;
                             if (pol) {
-                                double QE = hot_Q(atm_ext, Ep, ABB, VEC);
+                                double QE = or_hot_Q(atm_ext, Ep, ABB, VEC);
                                 PQ[p * N_L + kd] = (QE * c2 * BB_NORM) * GEOM;
                                 PU[p * N_L + kd] = (QE * s2 * BB_NORM) * GEOM;
                             }

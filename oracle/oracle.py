@@ -95,6 +95,10 @@ def lib():
         L.or_integrate_general.argtypes = ([d, d, sz, sz] + [_dp] * 6 + [_ip, sz] +
                                            [_dp] * 5 + [sz, _dp, sz, _dp, sz, _dp, i,
                                                         C.c_void_p, _dp])
+        L.or_integrate_general_ext.restype = i
+        L.or_integrate_general_ext.argtypes = ([d, d, sz, sz] + [_dp] * 6 + [sz, _ip, sz] + [_dp] * 5 +
+                                               [sz, _dp, sz, _dp, sz, _dp, i, i, i, C.c_void_p, _dp,
+                                                C.c_void_p, C.c_void_p])
         L.or_integrate_stokes.restype = i
         L.or_integrate_stokes.argtypes = ([d, d, sz, sz] + [_dp] * 6 + [sz, _ip, sz] + [_dp] * 5 +
                                           [sz, _dp, sz, _dp, sz, _dp, i, i, i, d, C.c_void_p,
@@ -250,6 +254,33 @@ def integrate_general(omega, inclination, cellArea, radial, r_s_over_r, theta, p
                                    _c(phases), int(image_order_limit or 0),
                                    None if corr is None else corr.ctypes.data, flux)
     return t, flux
+
+
+def integrate_general_stokes(omega, inclination, cellArea, radial, r_s_over_r, theta, phi, cellParams,
+                             radiates, deflection, cos_alpha, lag, maxDeflection, cos_gamma, energies,
+                             leaves, phases, atm_ext=1, beam_opt=0, image_order_limit=None,
+                             correction=None, stokes=True):
+    """integratorIQU.integrate (stokes=True: (status, I, Q, U)) or
+    integrator.integrate with the atmosphere and beaming options (stokes=False:
+    (status, I)); cellParams [S, S, n], correction [S, S, 2] or None."""
+    S0, S1 = cellArea.shape
+    N_R = deflection.shape[1]
+    P = _c(cellParams)
+    fI = np.zeros((len(energies), len(phases)))
+    fQ = np.zeros_like(fI) if stokes else None
+    fU = np.zeros_like(fI) if stokes else None
+    corr = None if correction is None else _c(correction)
+    t = lib().or_integrate_general_ext(omega, inclination, S0, S1, _c(cellArea), _c(radial), _c(r_s_over_r),
+                                       _c(theta), _c(phi), P, P.shape[2], _c(radiates, np.int32), N_R,
+                                       _c(deflection), _c(cos_alpha), _c(lag), _c(maxDeflection),
+                                       _c(cos_gamma), len(energies), _c(energies), len(leaves), _c(leaves),
+                                       len(phases), _c(phases), int(image_order_limit or 0), int(atm_ext),
+                                       int(beam_opt), None if corr is None else corr.ctypes.data, fI,
+                                       None if fQ is None else fQ.ctypes.data,
+                                       None if fU is None else fU.ctypes.data)
+    if t == 2:
+        raise ValueError("option outside the oracle's restatement")
+    return (t, fI, fQ, fU) if stokes else (t, fI)
 
 
 def integrate_stokes(omega, inclination, cellArea, radial, r_s_over_r, theta, phi, cellParams,

@@ -111,6 +111,7 @@ int or_integrate_stokes(double omega, double inclination, size_t S_rows, size_t 
                         const double *phases, int atm_ext, int beam_opt, int image_order_limit, double R_in,
                         const double *correction, double *flux_I, double *flux_Q, double *flux_U);
 double or_hot_I(int atm, int beam_opt, double E, double mu, const double *VEC);
+double or_hot_Q(int atm, double E, double mu, const double *VEC);
 double or_pol_angle(double leaf, double sin_psi, double cos_psi, double sin_alpha, double cos_alpha,
                     double sin_theta, double cos_theta, double sin_i, double cos_i, double sin_gamma,
                     double cos_gamma, double Grav_z, double mu, double eta, double beta, double Lorentz,
@@ -118,6 +119,19 @@ double or_pol_angle(double leaf, double sin_psi, double cos_psi, double sin_alph
 int or_disk_block(double R_in, double cos_i, double cos_psi, double cos_theta, double r_s_over_r, double radius,
                   double sin_alpha, double theta_over_pi);
 
+/* xpsi/cellmesh/integrator.pyx:48-667 and integratorIQU.pyx:67-717: per-cell
+ * parameters [S][S][n_params], atmospheres 1 and 3, beaming, the per-cell
+ * blackbody correction; flux_Q and flux_U may be NULL.  Returns 0, 1 (ERROR)
+ * or 2 (an option outside the restatement). */
+int or_integrate_general_ext(double omega, double inclination, size_t S_rows, size_t S_cols,
+                             const double *cellArea, const double *radial, const double *r_s_over_r,
+                             const double *theta, const double *phi, const double *cellParams,
+                             size_t n_params, const int *radiates, size_t N_R, const double *deflection,
+                             const double *cos_alpha, const double *lag, const double *maxDeflection,
+                             const double *cos_gammaArray, size_t N_E, const double *energies, size_t N_L,
+                             const double *leaves, size_t N_P, const double *phases, int image_order_limit,
+                             int atm_ext, int beam_opt, const double *correction, double *flux,
+                             double *flux_Q, double *flux_U);
 /* xpsi/cellmesh/integrator.pyx:48-667 (blackbody): per-cell parameters */
 int or_integrate_general(double omega, double inclination, size_t S_rows, size_t S_cols,
                          const double *cellArea, const double *radial, const double *r_s_over_r,

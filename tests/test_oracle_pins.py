@@ -34,6 +34,14 @@
   converges to option 2; a small spot's Q/I and U/I follow
   PD(mu) cos 2chi_0, PD(mu) sin 2chi_0 around the rotation; a northern spot
   is never blocked and a southern one behind a disk inside the star is dark.
+- The general integrator's Stokes form and options (integratorIQU.pyx:67-717,
+  integrator.pyx with atmospheres and beaming): with the blackbody it is the
+  plain general integrator bit for bit with Q = U = 0; pole-on the burst and
+  beamed fluxes are the closed forms; the reference forms one polarisation
+  angle per parallel and image (from the leaf loop's last values), so for one
+  radiating parallel and one image U/Q is the same at every energy and phase,
+  and seen pole-on in flat space (mu eta = cos theta on the whole parallel)
+  sqrt(Q^2 + U^2) = |PD(cos theta)| I.
 """
 import math
 
@@ -636,3 +644,75 @@ def test_disk_blocking_in_the_integrator():
     assert 0.0 < sums[0] < sums[1] <= sums[2] <= I0.sum() * (1 + 1e-12)
     north = _member(st, super_colatitude=0.8, super_radius=0.5)
     assert np.array_equal(_stokes(st, north, R_in=5e3, stokes=False)[1], _stokes(st, north, stokes=False)[1])
+
+
+# ------------------------------------------------ general integrator: Stokes and options
+def _gen_stokes(st, m, params=None, atm=1, beam=0, radiates=None, N_L=64, image_order_limit=None,
+                stokes=True):
+    energies = np.array([0.5, 1.0, 2.0, 4.0])
+    leaves = np.linspace(0.0, 2.0 * math.pi, N_L)
+    phases = 2.0 * math.pi * np.linspace(0.0, 1.0, N_L)
+    P = m["params"] if params is None else params
+    R = m["radiates"] if radiates is None else radiates
+    return O.integrate_general_stokes(st["Omega"], st["i"], m["cellArea"], m["r"], m["r_s_over_r"],
+                                      m["theta"], m["phi"], P, R, m["deflection"], m["cos_alpha"], m["lag"],
+                                      m["maxDeflection"], m["cos_gamma"], energies, leaves, phases, atm_ext=atm,
+                                      beam_opt=beam, image_order_limit=image_order_limit, stokes=stokes)
+
+
+def test_general_stokes_blackbody_is_the_general_integrator():
+    st = O.spacetime(1.4, 12.0, 1.0, math.cos(1.0), 300.0, "AGM_14")
+    m = _member(st, super_colatitude=1.2, super_radius=0.8)
+    P = m["params"].copy()
+    P[..., 0] += 0.1 * np.sin(m["phi"])
+    t, I, Q, U = _gen_stokes(st, m, params=P)
+    assert t == 0
+    f, _ = _general(st, dict(super_colatitude=1.2, super_radius=0.8, super_temperature=6.3, phase_shift=0.0,
+                             sqrt_num_cells=32, min_sqrt_num_cells=10, max_sqrt_num_cells=100, num_rays=512,
+                             num_leaves=64), params=lambda mm: P)
+    assert np.array_equal(I, f)
+    assert not np.any(Q) and not np.any(U)
+
+
+def test_general_burst_and_beamed_pole_on_closed_forms():
+    st, hr = _flat_cap(48)
+    members, err = O.hot_region_embed(st, hr, st["f"])
+    m = members[0]
+    energies = np.array([0.5, 1.0, 2.0, 4.0])
+    kT = K_B_OVER_KEV * 10.0 ** 6.2
+    base = BB_NORM * energies ** 3 / np.expm1(energies / kT) * 2.0 * math.pi * st["R"] ** 2 / (energies * KEV)
+    rho = 0.5
+    mu1, mu2 = math.sin(rho) ** 2 / 2.0, (1.0 - math.cos(rho) ** 3) / 3.0
+    t, I, Q, U = _gen_stokes(st, m, atm=3, N_L=32)
+    assert t == 0
+    assert np.max(np.abs(I[:, 0] / (base * (0.421 * mu1 + 0.868 * mu2)) - 1.0)) < 3e-5
+    P = np.zeros(m["params"].shape[:2] + (7,))
+    P[..., :2] = m["params"]
+    P[..., 2], P[..., 6] = 0.3, 100
+    t, I1 = _gen_stokes(st, m, params=P, beam=1, N_L=32, stokes=False)
+    assert np.max(np.abs(I1[:, 0] / (base * (mu1 + 0.3 * mu2)) - 1.0)) < 3e-5
+
+
+def test_general_stokes_one_angle_per_parallel_and_image():
+    # one radiating parallel of an inclined, rotating star: U / Q the same everywhere
+    st = O.spacetime(1.4, 12.0, 1.0, math.cos(1.0), 300.0, "AGM_14")
+    m = _member(st, super_colatitude=1.0, super_radius=0.6)
+    rad = np.zeros_like(m["radiates"])
+    row = int(np.argmax(m["radiates"].sum(axis=1)))
+    rad[row] = m["radiates"][row]
+    t, I, Q, U = _gen_stokes(st, m, atm=3, radiates=rad, image_order_limit=1)
+    assert t == 0
+    big = np.abs(Q) > 1e-3 * np.max(np.abs(Q))
+    ratio = U[big] / Q[big]
+    assert np.max(np.abs(ratio - ratio[0])) < 1e-9 * max(1.0, abs(ratio[0]))
+    # pole-on in flat space: mu eta = cos theta on the parallel, so |P| = |PD| I
+    st0, hr0 = _flat_cap(32)
+    m0 = O.hot_region_embed(st0, hr0, st0["f"])[0][0]
+    rad0 = np.zeros_like(m0["radiates"])
+    r0 = m0["radiates"].shape[0] // 2
+    rad0[r0] = m0["radiates"][r0]
+    t, I0, Q0, U0 = _gen_stokes(st0, m0, atm=3, radiates=rad0, N_L=32, image_order_limit=1)
+    assert t == 0
+    c = math.cos(m0["theta"][r0, 0])
+    PD = 0.1171 * (c - 1.0) / (1.0 + 3.582 * c)
+    assert np.max(np.abs(np.hypot(Q0, U0) / (abs(PD) * I0) - 1.0)) < 1e-6
