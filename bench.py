@@ -33,9 +33,11 @@ KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold", "waves": "k_w
 # factors from a 512 MiB device copy), KiB per launch for the 256-, 128- and
 # 64-energy launches of one step, per autotuned fold (profiles/wave_fold_batched_default.md,
 # profiles/moment_fold_batched_default.md hold the runs).
-# The wave fold's writes are its FP64 atomic additions into the flux.
+# The wave fold's writes are its FP64 atomic additions into the flux (one set
+# per accumulation chain of 8 intervals); its reads include the reloads of its
+# register spills (240 bytes per lane of scratch at four waves per SIMD).
 FOLD_TRAFFIC_KIB = {
-    "waves/1": {256: (113018, 7930018), 128: (102858, 3573981), 64: (110550, 1910209)},
+    "waves/1": {256: (1653820, 31189116), 128: (817597, 13795827), 64: (521537, 7495181)},
     "moment/2": {256: (17123810, 20011418), 128: (9762315, 9300707), 64: (6796957, 5185836)},
 }
 HBM_PEAK_GBS = 8000.0
