@@ -95,26 +95,32 @@ int xh_integrate_stokes(double omega, double inclination, int S,
 
 /* Replaces xpsi/cellmesh/integrator.pyx:48-667 `integrate(...) -> (status,
  * flux[N_E, N_P])`, the integrator without azimuthal invariance
- * (HotRegion.symmetry False), for the blackbody (hot_atm_ext 1): every cell
- * has its own parameters.  Same arrays as xh_integrate; cellParams [S, S, 2]
- * with log10 T first per cell; correction: [S, S, 2] per-cell blackbody
- * elsewhere parameters (log10 T first) whose intensity every cell subtracts
- * (else_atm_ext 1), or NULL.  9 N_L and 5 N_R + 8 N_L doubles must each fit
- * in 159 KiB of one block's LDS (N_L up to about 2000).
+ * (HotRegion.symmetry False), and its Stokes form
+ * xpsi/cellmesh/integratorIQU.pyx:51-717 `-> (status, I, Q, U)` (flux_Q,
+ * flux_U non-NULL): every cell has its own parameters.  Same arrays as
+ * xh_integrate; cellParams [S, S, n_params]: log10 T, log g, and with beaming
+ * a, b, c, d, n_mu; atm_ext 1 (blackbody) or 3 (burst blackbody, with its Q);
+ * beam_opt 0-3 (3 with the blackbody); correction: [S, S, 2] per-cell
+ * blackbody elsewhere parameters (log10 T first) whose intensity every cell
+ * subtracts (else_atm_ext 1), or NULL.  The Stokes form uses one polarisation
+ * angle per parallel and image, formed as the reference forms it from its
+ * leaf loop's last values.  13 N_L and 5 N_R + 8 N_L doubles must each fit in
+ * 159 KiB of one block's LDS (N_L up to about 1500).
  * workspace: xh_integrate_general_workspace_bytes() bytes.
  * *status_out = 0 success, 1 numerical failure (the reference's ERROR). */
 size_t xh_integrate_general_workspace_bytes(int S, int N_L);
 int xh_integrate_general(double omega, double inclination, int S,
                          const double* cellArea, const double* radial,
                          const double* r_s_over_r, const double* theta,
-                         const double* phi, const double* cellParams,
+                         const double* phi, const double* cellParams, int n_params,
                          const int* radiates, int N_R, const double* deflection,
                          const double* cos_alpha, const double* lag,
                          const double* maxDeflection, const double* cos_gamma,
                          int N_E, const double* energies, int N_L,
                          const double* leaves, int N_P, const double* phases,
-                         int image_order_limit, const double* correction,
-                         double* flux, void* workspace, size_t workspace_bytes,
+                         int image_order_limit, int atm_ext, int beam_opt,
+                         const double* correction, double* flux, double* flux_Q,
+                         double* flux_U, void* workspace, size_t workspace_bytes,
                          int* status_out, void* stream);
 
 /* Replaces xpsi/cellmesh/integrator_for_time_invariance.pyx:59-98

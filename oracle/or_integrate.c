@@ -372,7 +372,7 @@ int or_integrate(double R, double omega, double r_s, double inclination,
  * atmospheres read), and every energy's intensity at E / Z is added with the cell's
  * area times GEOM.  Flux [N_E][N_P], divided by E keV at the end (:601-606).
  * Rings are summed in order (one thread).  The Stokes form
- * (xpsi/cellmesh/integratorIQU.pyx:67-717) adds Q and U: the atmosphere's Q
+ * (xpsi/cellmesh/integratorIQU.pyx:51-717) adds Q and U: the atmosphere's Q
  * at the interpolated mu eta, rotated by 2 chi, where chi is the polarisation
  * angle the reference forms inside its cell and phase loop from the leaf
  * loop's last values (:593-596), hence one angle per parallel and image. */

@@ -119,7 +119,7 @@ double or_pol_angle(double leaf, double sin_psi, double cos_psi, double sin_alph
 int or_disk_block(double R_in, double cos_i, double cos_psi, double cos_theta, double r_s_over_r, double radius,
                   double sin_alpha, double theta_over_pi);
 
-/* xpsi/cellmesh/integrator.pyx:48-667 and integratorIQU.pyx:67-717: per-cell
+/* xpsi/cellmesh/integrator.pyx:48-667 and integratorIQU.pyx:51-717: per-cell
  * parameters [S][S][n_params], atmospheres 1 and 3, beaming, the per-cell
  * blackbody correction; flux_Q and flux_U may be NULL.  Returns 0, 1 (ERROR)
  * or 2 (an option outside the restatement). */

@@ -5,7 +5,9 @@ temperature map (up to three image orders); the same with a per-cell
 elsewhere correction; a spot low on the far hemisphere, hidden for part of
 the rotation (the padded stretches of phase, redshift and GEOM); and a ragged
 pole-centred cap with a hot sector (41 leaves, 100 energies, 100 phases:
-part-filled energy and phase tiles)."""
+part-filled energy and phase tiles).  The Stokes form
+(xpsi/cellmesh/integratorIQU.pyx:51-717) with the burst atmosphere, I, Q and
+U, and the beaming options."""
 import math
 import sys
 
@@ -107,3 +109,70 @@ def test_general_ragged_sector():
     P[:, S // 3: S // 3 + 4, 0] = 6.35
     energies = np.logspace(-0.5, 1.0, 100)
     _compare(st, m, energies, 41, 100, P, name="ragged sector")
+
+
+QU_FLOOR = 1.0e-5   # Q and U change sign across the spot: see test_gpu_stokes.py
+
+
+def _compare_stokes(st, m, energies, N_L, N_P, params, atm, beam=0, corr=None, name=""):
+    from xpsi_amd.cellmesh.integratorIQU import integrate
+    leaves = np.linspace(0.0, 2.0 * math.pi, N_L)
+    phases = 2.0 * math.pi * np.linspace(0.0, 1.0, N_P)
+    ref = O.integrate_general_stokes(st["Omega"], st["i"], m["cellArea"], m["r"], m["r_s_over_r"], m["theta"],
+                                     m["phi"], params, m["radiates"], m["deflection"], m["cos_alpha"], m["lag"],
+                                     m["maxDeflection"], m["cos_gamma"], energies, leaves, phases, atm_ext=atm,
+                                     beam_opt=beam, correction=corr)
+    assert ref[0] == 0
+    say("%s: S=%d N_E=%d N_L=%d N_P=%d" % (name, m["S"], len(energies), N_L, N_P))
+    got = integrate(1, st["R"], st["Omega"], st["r_s"], st["i"], m["cellArea"], m["r"], m["r_s_over_r"],
+                    m["theta"], m["phi"], params, m["radiates"], corr, m["deflection"].shape[1],
+                    m["deflection"], m["cos_alpha"], m["lag"], m["maxDeflection"], m["cos_gamma"], energies,
+                    leaves, phases, (), (), (), atm, None if corr is None else 1, beam)
+    assert got[0] == 0
+    for nm, a, b in zip("IQU", got[1:], ref[1:]):
+        err = elem_relerr(a, b) if nm == "I" else elem_relerr(a, b, floor=QU_FLOOR)
+        say("  %s rel err %.3e (max |%s| %.3e)" % (nm, err, nm, np.max(np.abs(b))))
+        assert err < SIG_RTOL
+
+
+def test_general_stokes_burst_temperature_map(ef):
+    st, m, energies = ef
+    _compare_stokes(st, m, energies, 64, 64, _tmap(m, 4), 3, name="general Stokes, burst, T map")
+
+
+def test_general_stokes_burst_correction_hidden():
+    st = O.spacetime(1.6, 13.0, 1.0, math.cos(1.1), 400.0, "AGM_14")
+    hr = dict(super_colatitude=2.3, super_radius=0.5, super_temperature=6.25, phase_shift=0.0,
+              sqrt_num_cells=32, min_sqrt_num_cells=10, max_sqrt_num_cells=64, num_rays=256,
+              num_leaves=64)
+    members, err = O.hot_region_embed(st, hr, st["f"])
+    assert err is None
+    m = members[0]
+    corr = np.ones_like(m["params"])
+    corr[..., 0] = 5.9
+    _compare_stokes(st, m, np.logspace(-0.7, 1.0, 40), 41, 70, _tmap(m, 5), 3, corr=corr,
+                    name="general Stokes, hidden stretches, correction, ragged")
+
+
+@pytest.mark.parametrize("beam", [1, 2, 3])
+def test_general_beaming(ef, beam):
+    from xpsi_amd.cellmesh.integrator import integrate
+    st, m, energies = ef
+    P = np.zeros(m["params"].shape[:2] + (7,))
+    P[..., :2] = _tmap(m, 6)
+    P[..., 2], P[..., 3], P[..., 4], P[..., 5], P[..., 6] = 0.4, -0.3, 0.2, 0.1, 50
+    leaves = np.linspace(0.0, 2.0 * math.pi, 64)
+    phases = 2.0 * math.pi * np.linspace(0.0, 1.0, 64)
+    t, ref = O.integrate_general_stokes(st["Omega"], st["i"], m["cellArea"], m["r"], m["r_s_over_r"], m["theta"],
+                                        m["phi"], P, m["radiates"], m["deflection"], m["cos_alpha"], m["lag"],
+                                        m["maxDeflection"], m["cos_gamma"], energies, leaves, phases,
+                                        beam_opt=beam, stokes=False)
+    assert t == 0
+    status, flux = integrate(1, st["R"], st["Omega"], st["r_s"], st["i"], m["cellArea"], m["r"],
+                             m["r_s_over_r"], m["theta"], m["phi"], P, m["radiates"], None,
+                             m["deflection"].shape[1], m["deflection"], m["cos_alpha"], m["lag"],
+                             m["maxDeflection"], m["cos_gamma"], energies, leaves, phases, (), (), 1, None, beam)
+    assert status == 0
+    err = elem_relerr(flux, ref)
+    say("general, beaming option %d: rel err %.3e" % (beam, err))
+    assert err < SIG_RTOL

@@ -34,7 +34,7 @@
   converges to option 2; a small spot's Q/I and U/I follow
   PD(mu) cos 2chi_0, PD(mu) sin 2chi_0 around the rotation; a northern spot
   is never blocked and a southern one behind a disk inside the star is dark.
-- The general integrator's Stokes form and options (integratorIQU.pyx:67-717,
+- The general integrator's Stokes form and options (integratorIQU.pyx:51-717,
   integrator.pyx with atmospheres and beaming): with the blackbody it is the
   plain general integrator bit for bit with Q = U = 0; pole-on the burst and
   beamed fluxes are the closed forms; the reference forms one polarisation

@@ -77,8 +77,9 @@ def lib():
     L.xh_integrate_general_workspace_bytes.restype = sz
     L.xh_integrate_general_workspace_bytes.argtypes = [i32, i32]
     L.xh_integrate_general.restype = i32
-    L.xh_integrate_general.argtypes = ([d, d, i32] + [vp] * 7 + [i32] + [vp] * 5 +
-                                       [i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, sz, C.POINTER(i32), vp])
+    L.xh_integrate_general.argtypes = ([d, d, i32] + [vp] * 6 + [i32, vp, i32] + [vp] * 5 +
+                                       [i32, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp, vp, vp, vp, sz,
+                                        C.POINTER(i32), vp])
     L.xh_integrate_time_invariant_workspace_bytes.restype = sz
     L.xh_integrate_time_invariant_workspace_bytes.argtypes = [i32, i32]
     L.xh_integrate_time_invariant.restype = i32

@@ -566,23 +566,32 @@ extern "C" size_t xh_integrate_general_workspace_bytes(int S, int N_L)
     b += 3 * (size_t)S * 8 + (size_t)S * 4;          // theta, logT, logg, ring_ok
     b = (b + 255) & ~(size_t)255;
     b += (size_t)S * IO * 4 * N_L * 8;               // geo
+    b += (size_t)S * IO * 2 * 8;                     // polarisation angle per parallel and image
     b += (size_t)S * IO * 4;                         // valid
     return b + 1024;
 }
 
 extern "C" int xh_integrate_general(double omega, double inclination, int S, const double* cellArea,
                                     const double* radial, const double* r_s_over_r, const double* theta,
-                                    const double* phi, const double* cellParams, const int* radiates, int N_R,
-                                    const double* deflection, const double* cos_alpha, const double* lag,
+                                    const double* phi, const double* cellParams, int n_params, const int* radiates,
+                                    int N_R, const double* deflection, const double* cos_alpha, const double* lag,
                                     const double* maxDeflection, const double* cos_gamma, int N_E,
                                     const double* energies, int N_L, const double* leaves, int N_P,
-                                    const double* phases, int image_order_limit, const double* correction,
-                                    double* flux, void* workspace, size_t workspace_bytes, int* status_out,
-                                    void* stream)
+                                    const double* phases, int image_order_limit, int atm_ext, int beam_opt,
+                                    const double* correction, double* flux, double* flux_Q, double* flux_U,
+                                    void* workspace, size_t workspace_bytes, int* status_out, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1)
         return fail(XH_EINVAL, "xh_integrate_general: bad sizes");
+    if (atm_ext != 1 && atm_ext != 3)
+        return fail(XH_EINVAL, "xh_integrate_general: atmosphere extension must be 1 (blackbody) or 3 (burst blackbody)");
+    if (beam_opt < 0 || beam_opt > 3 || (beam_opt == 3 && atm_ext != 1))
+        return fail(XH_EINVAL, "xh_integrate_general: beaming option must be 0-3 (3 with the blackbody only)");
+    if (n_params < 2 || (beam_opt != 0 && n_params < 7))
+        return fail(XH_EINVAL, "xh_integrate_general: beaming needs 7 parameters per cell (log T, log g, a, b, c, d, n_mu)");
+    if ((flux_Q == nullptr) != (flux_U == nullptr))
+        return fail(XH_EINVAL, "xh_integrate_general: Q and U are requested together");
     // the geometry kernel keeps a few static LDS words beside its dynamic tables
     const int gen_lds_max = 159 * 1024;
     if (general_fold_lds(N_L) > (size_t)gen_lds_max || general_geo_lds(N_R, N_L) > (size_t)gen_lds_max)
@@ -608,20 +617,27 @@ extern "C" int xh_integrate_general(double omega, double inclination, int S, con
     int* d_ok = (int*)w; w += (size_t)S * 4;
     w = (char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
     double* d_geo = (double*)w; w += (size_t)S * IO * 4 * N_L * 8;
+    double* d_pol = (double*)w; w += (size_t)S * IO * 2 * 8;
     int* d_valid = (int*)w;
     Member hm{};
     hm.Omega = omega; hm.inclination = inclination; hm.active = 1; hm.S = S;
     HIPCHK(hipMemcpyAsync(d_m, &hm, sizeof(Member), hipMemcpyHostToDevice, s));
-    Layout L{S, N_R, N_L, N_P, N_E, IO, image_order_limit > 0 ? image_order_limit : 0, 1};
+    Layout L{S, N_R, N_L, N_P, N_E, IO, image_order_limit > 0 ? image_order_limit : 0, atm_ext};
     hipLaunchKernelGGL(k_ring_prep, dim3(1), dim3(256), 0, s, S, theta, cellParams, radiates, d_theta, d_ok, d_logT,
                        d_logg);
     HIPCHK(hipGetLastError());
     RingArgs RA{d_theta, radial, r_s_over_r, cos_gamma, maxDeflection, deflection, cos_alpha, lag, leaves, d_ok,
                 d_geo, d_valid, nullptr};
+    RA.pol = flux_Q ? d_pol : nullptr;     // [ring][image][2] here
     hipLaunchKernelGGL(k_general_geometry, dim3(S), dim3(128), general_geo_lds(N_R, N_L), s, d_m, L, RA);
     HIPCHK(hipGetLastError());
     HIPCHK(fill(flux, (size_t)N_E * N_P, 0.0, s));
-    GeneralArgs GA{d_geo, d_valid, cellArea, radiates, phi, cellParams, correction, energies, phases, flux};
+    if (flux_Q) {
+        HIPCHK(fill(flux_Q, (size_t)N_E * N_P, 0.0, s));
+        HIPCHK(fill(flux_U, (size_t)N_E * N_P, 0.0, s));
+    }
+    GeneralArgs GA{d_geo, d_valid, cellArea, radiates, phi, cellParams, correction, energies, phases, flux,
+                   n_params, atm_ext, beam_opt, RA.pol, flux_Q, flux_U};
     const int tiles = ((N_E + 31) / 32) * ((N_P + 63) / 64);
     hipLaunchKernelGGL(k_general_fold, dim3(S, IO, tiles), dim3(256), general_fold_lds(N_L), s, d_m, L, GA);
     HIPCHK(hipGetLastError());

@@ -1,7 +1,8 @@
 // Pulse integration over a hot-region mesh without azimuthal invariance:
-// every cell carries its own parameters (xpsi/cellmesh/integrator.pyx:48-667),
-// blackbody intensity (hot_BB.pyx:54-98), optionally minus a per-cell
-// blackbody elsewhere intensity (the correction, integrator.pyx:580-590).
+// every cell carries its own parameters (xpsi/cellmesh/integrator.pyx:48-667;
+// its Stokes form xpsi/cellmesh/integratorIQU.pyx:51-717), the blackbody or
+// burst blackbody with the beaming options (xh_hot.hpp), optionally minus a
+// per-cell blackbody elsewhere intensity (the correction, integrator.pyx:580-590).
 //
 // k_general_geometry  one block per parallel, image orders in turn: leaf
 //                     geometry in parallel, then the reference's leaf loop in
@@ -10,13 +11,16 @@
 //                     redshift and mu eta and a zero geometric factor.
 //                     Writes PHASE / Z / ABB / GEOM per leaf.
 // k_general_fold      one block per (parallel, image, energy tile x phase
-//                     tile): the image's GEOM (periodic Akima) and Z
+//                     tile): the image's GEOM (periodic Akima), Z and mu eta
 //                     (Steffen) interpolants are built in LDS; each lane
 //                     owns one model phase and eight energies and walks the
 //                     parallel's cells, interpolating GEOM and Z at the
 //                     cell's phase point and adding area * GEOM * I(E / Z) of
-//                     the cell's temperature.  The block's sums, divided by
-//                     E keV, meet in the flux by FP64 atomic additions.
+//                     the cell's temperature (or the burst atmosphere and
+//                     beaming at the interpolated mu eta; the Stokes form adds
+//                     Q and U rotated by the parallel and image's angle).
+//                     The block's sums, divided by E keV, meet in the flux by
+//                     FP64 atomic additions.
 //
 // The bound is FP64 issue: per (cell, phase, energy) one exp and one
 // division (two of each with the correction); the interpolation is shared by
@@ -25,6 +29,7 @@
 #include "xh_types.hpp"
 #include "xh_kernels.hpp"
 #include "xh_leaf.hpp"
+#include "xh_hot.hpp"
 
 namespace xh {
 
@@ -40,7 +45,8 @@ __device__ __forceinline__ bool takes_slot(int k, int ks, int leaf_lim, int N_L)
 // order over all N_L leaves.  A visible leaf of the first half copies its
 // precomputed values (aP/aZ/aA/aG, indexed by destination slot) in at its own
 // step, so a padding written at an earlier step is overwritten exactly where
-// the reference overwrites it.  Returns the final visibility flag (2: the
+// the reference overwrites it.  vis[k] bit 0: leaf k is seen; bit 1: its
+// deflection lies in the table.  Returns the final visibility flag (2: the
 // image was never visible).
 __device__ int leaf_pass(int N_L, int leaf_lim, const double* __restrict__ leaves, const unsigned char* vis,
                          const double* aP, const double* aZ, const double* aA, const double* aG, double* P,
@@ -49,7 +55,7 @@ __device__ int leaf_pass(int N_L, int leaf_lim, const double* __restrict__ leave
     int invis = 2, ip = 0;
 #pragma unroll 1
     for (int k = 0; k < N_L; k++) {
-        if (vis[k]) {
+        if (vis[k] & 1) {
             if (k < leaf_lim) {
                 for (int ks = 0; ks < 2; ks++) {
                     if (!takes_slot(k, ks, leaf_lim, N_L)) continue;
@@ -166,7 +172,7 @@ __global__ void __launch_bounds__(128) k_general_geometry(Member* __restrict__ m
             const bool first = k < leaf_lim;
             LeafGeo g = leaf_geometry(I, leaves[k], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg, maxD, D, Ac,
                                       Lg, N_R, xalt, yalt, nalt, first);
-            vis[k] = (unsigned char)g.calc;
+            vis[k] = (unsigned char)(g.calc | (g.psi <= maxD ? 2 : 0));
             if (!g.calc || !first) continue;
             for (int ks = 0; ks < 2; ks++) {
                 if (!takes_slot(k, ks, leaf_lim, N_L)) continue;
@@ -190,6 +196,39 @@ __global__ void __launch_bounds__(128) k_general_geometry(Member* __restrict__ m
         if (tid == 0) {
             const int invis = leaf_pass(N_L, leaf_lim, leaves, vis, aP, aZ, aA, aG, P, Z, Ab, G);
             shown = invis != 2;
+            if (shown && A.pol) {
+                // the Stokes form's polarisation angle (integratorIQU.pyx:593-596),
+                // formed from the leaf loop's last values: the last leaf written
+                // (eta, cos xi), the last leaf (psi) and the last leaf within the
+                // deflection table (alpha, mu)
+                int k1 = -1, k2 = -1;
+                for (int k = 0; k < N_L; k++) {
+                    if (vis[k] & 2) k1 = k;
+                    if ((vis[k] & 1) && k < leaf_lim) k2 = k;
+                }
+                double c2 = 1.0, s2 = 0.0;
+                if (k1 >= 0 && k2 >= 0) {
+                    const LeafGeo g1 = leaf_geometry(I, leaves[k1], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg,
+                                                     maxD, D, Ac, Lg, N_R, xalt, yalt, nalt, false);
+                    const LeafGeo gN = leaf_geometry(I, leaves[N_L - 1], incl, cos_i, sin_i, cos_th, sin_th, thp, cg,
+                                                     sg, maxD, D, Ac, Lg, N_R, xalt, yalt, nalt, false);
+                    const LeafGeo g2 = leaf_geometry(I, leaves[k2], incl, cos_i, sin_i, cos_th, sin_th, thp, cg, sg,
+                                                     maxD, D, Ac, Lg, N_R, xalt, yalt, nalt, false);
+                    const int kd = takes_slot(k2, 1, leaf_lim, N_L) ? N_L - 1 - k2 : k2;
+                    double cos_xi = 0.0, eta = Lorentz;
+                    if (!eq12(g2.psi, 0.0)) {
+                        cos_xi = g2.sin_alpha * sin_i * sin(leaves[kd]) / g2.sin_psi;
+                        eta = Lorentz / (1.0 + beta * cos_xi);
+                    }
+                    const double chi = pol_angle(leaves[kd], gN.sin_psi, gN.cos_psi, g1.sin_alpha, g1.cos_alpha,
+                                                 sin_th, cos_th, sin_i, cos_i, sg, cg, Grav_z, g1.mu, eta, beta,
+                                                 Lorentz, cos_xi);
+                    c2 = cos(2 * chi);
+                    s2 = sin(2 * chi);
+                }
+                A.pol[vix * 2] = c2;
+                A.pol[vix * 2 + 1] = s2;
+            }
             if (!shown) {
                 A.valid[vix] = 0;
             } else {
@@ -237,33 +276,49 @@ __global__ void __launch_bounds__(256) k_general_fold(Member* __restrict__ membe
     double* Za = ZY + N_L;
     double* Zb = Za + N_L;
     double* Zc = Zb + N_L;
-    const double* geo = F.geo + ((size_t)i * L.IO_max + I) * 4 * N_L;
+    double* AY = Zc + N_L;      // mu eta: read by the burst atmosphere and beaming
+    double* Aa = AY + N_L;
+    double* Ab = Aa + N_L;
+    double* Ac = Ab + N_L;
+    const bool need_abb = F.atm != 1 || F.beam != 0;
+    const size_t vix = (size_t)i * L.IO_max + I;
+    const double* geo = F.geo + vix * 4 * N_L;
     for (int q = tid; q < N_L; q += blockDim.x) {
         PH[q] = geo[q];
         ZY[q] = geo[N_L + q];
+        AY[q] = geo[2 * N_L + q];
         GY[q] = geo[3 * N_L + q];
     }
     __syncthreads();
+    // Steffen coefficients: y = y_i + t (c + t (b + t a))
+    auto steffen = [&](const double* Y, int q, double* a_, double* b_, double* c_) {
+        const double h = PH[q + 1] - PH[q];
+        const double s = (Y[q + 1] - Y[q]) / h;
+        const double y0 = steffen_slope(PH, Y, N_L, q), y1 = steffen_slope(PH, Y, N_L, q + 1);
+        a_[q] = (y0 + y1 - 2.0 * s) / h / h;
+        b_[q] = (3.0 * s - 2.0 * y0 - y1) / h;
+        c_[q] = y0;
+    };
     for (int q = tid; q < N_L - 1; q += blockDim.x) {
         double b, c, d;
         akima_coefs(PH, GY, N_L, true, q, b, c, d);
         Gb[q] = b; Gc[q] = c; Gd[q] = d;
-        const double h = PH[q + 1] - PH[q];
-        const double s = (ZY[q + 1] - ZY[q]) / h;
-        const double y0 = steffen_slope(PH, ZY, N_L, q), y1 = steffen_slope(PH, ZY, N_L, q + 1);
-        Za[q] = (y0 + y1 - 2.0 * s) / h / h;
-        Zb[q] = (3.0 * s - 2.0 * y0 - y1) / h;
-        Zc[q] = y0;
+        steffen(ZY, q, Za, Zb, Zc);
+        if (need_abb) steffen(AY, q, Aa, Ab, Ac);
     }
     __syncthreads();
     const int k = k0 + (tid & 63), g = tid >> 6;
     const bool kin = k < N_P;
-    double E[8], acc[8];
+    const bool pol = F.flux_Q != nullptr;
+    const double c2 = pol ? F.pol[vix * 2] : 0.0, s2 = pol ? F.pol[vix * 2 + 1] : 0.0;
+    double E[8], acc[8], accQ[8], accU[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int p = e0 + g + 4 * q;
         E[q] = p < N_E ? F.energies[p] : 1.0;
         acc[q] = 0.0;
+        accQ[q] = 0.0;
+        accU[q] = 0.0;
     }
     const double x0 = kin ? F.phases[k] : 0.0;
     const double lo = PH[0], hi = PH[N_L - 1];
@@ -280,7 +335,27 @@ __global__ void __launch_bounds__(256) k_general_fold(Member* __restrict__ membe
         if (!(Gx > 0.0) || !kin) continue;
         const double Zx = ZY[iv] + t * (Zc[iv] + t * (Zb[iv] + t * Za[iv]));
         const double wA = F.cellArea[row + j] * Gx;
-        const double kT = K_B_OVER_KEV * pow(10.0, F.params[(row + j) * 2]);
+        const double* VEC = F.params + (row + j) * F.n_params;
+        if (need_abb || pol) {
+            // burst atmosphere, beaming or Stokes: the atmosphere at the
+            // interpolated mu eta
+            const double Ax = AY[iv] + t * (Ac[iv] + t * (Ab[iv] + t * Aa[iv]));
+            const double* CV = F.correction ? F.correction + (row + j) * 2 : nullptr;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const double Ep = E[q] / Zx;
+                const double IE = hot_I(F.atm, F.beam, Ep, Ax, VEC) * BB_NORM;
+                const double cIE = CV ? planck(Ep, CV[0]) * BB_NORM : 0.0;
+                acc[q] += wA * (IE - cIE);
+                if (pol) {
+                    const double QE = hot_Q(F.atm, Ep, Ax, VEC) * BB_NORM;
+                    accQ[q] += wA * (QE * c2);
+                    accU[q] += wA * (QE * s2);
+                }
+            }
+            continue;
+        }
+        const double kT = K_B_OVER_KEV * pow(10.0, VEC[0]);
         if (F.correction) {
             const double kTc = K_B_OVER_KEV * pow(10.0, F.correction[(row + j) * 2]);
 #pragma unroll
@@ -303,7 +378,12 @@ __global__ void __launch_bounds__(256) k_general_fold(Member* __restrict__ membe
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int p = e0 + g + 4 * q;
-        if (p < N_E) unsafeAtomicAdd(&F.flux[(size_t)p * N_P + k], acc[q] / (E[q] * KEV));
+        if (p >= N_E) continue;
+        unsafeAtomicAdd(&F.flux[(size_t)p * N_P + k], acc[q] / (E[q] * KEV));
+        if (pol) {
+            unsafeAtomicAdd(&F.flux_Q[(size_t)p * N_P + k], accQ[q] / (E[q] * KEV));
+            unsafeAtomicAdd(&F.flux_U[(size_t)p * N_P + k], accU[q] / (E[q] * KEV));
+        }
     }
 }
 

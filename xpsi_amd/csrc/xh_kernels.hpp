@@ -149,16 +149,22 @@ struct GeneralArgs {
     const double* cellArea;    // [S][S]
     const int* radiates;       // [S][S]
     const double* phi;         // [S][S]
-    const double* params;      // [S][S][2], log10 T first
+    const double* params;      // [S][S][n_params], log10 T first
     const double* correction;  // [S][S][2] elsewhere log10 T first, or nullptr
     const double* energies;    // N_E
     const double* phases;      // N_P (radians)
     double* flux;              // [N_E][N_P], zeroed beforehand
+    int n_params;
+    int atm;                   // 1 blackbody, 3 burst blackbody
+    int beam;                  // beaming option 0-3
+    const double* pol;         // [ring][image][2] cos 2chi, sin 2chi (Stokes form) or nullptr
+    double* flux_Q;            // [N_E][N_P] or nullptr
+    double* flux_U;
 };
 __global__ void k_general_geometry(Member* members, Layout L, RingArgs A);
 __global__ void k_general_fold(Member* members, Layout L, GeneralArgs F);
 inline size_t general_geo_lds(int N_R, int N_L) { return (size_t)(5 * N_R + 8 * N_L) * 8 + N_L + 16; }
-inline size_t general_fold_lds(int N_L) { return (size_t)9 * N_L * 8; }
+inline size_t general_fold_lds(int N_L) { return (size_t)13 * N_L * 8; }
 
 // wave-autonomous moment fold (precision form, blackbody): one wavefront per
 // (ring block, 64 energies x 64 phases, member); flux zeroed beforehand

@@ -15,6 +15,9 @@ __device__ __forceinline__ double image_deflection(int order, double psi)
     return (double)order * PI + sgn * psi;
 }
 
+// calc: the leaf is seen (mu > 0, not behind a disk).  psi, sin_psi and
+// cos_psi are always set; mu, sin_alpha and cos_alpha when psi lies within the
+// deflection table; deriv and phase_lag when the leaf is seen (with need_all).
 struct LeafGeo {
     int calc;
     double phase_lag, mu, sin_alpha, cos_alpha, deriv, sin_psi, cos_psi, psi;
@@ -80,6 +83,7 @@ __device__ inline LeafGeo leaf_geometry(int I, double leaf, double inclination, 
     g.calc = 0;
     g.psi = psi;
     g.sin_psi = sin_psi;
+    g.cos_psi = cos_psi;
     if (!(psi <= maxD)) return g;
     bool alt = (psi <= HALF_PI && cos_psi >= xalt[0]);
     double ca = alt ? steffen_eval(xalt, yalt, nalt, cos_psi, nullptr) : steffen_eval(D, A, N_R, psi, nullptr);
@@ -92,11 +96,11 @@ __device__ inline LeafGeo leaf_geometry(int I, double leaf, double inclination, 
     }
     g.calc = mu > 0.0 ? 1 : 0;
     if (g.calc && R_in < 1.0e6) g.calc = disk_unblocked(R_in, cos_i, cos_psi, cos_th, u, radius, sa, th_over_pi);
-    if (!g.calc || !need_all) return g;
     g.mu = mu;
     g.sin_alpha = sa;
     g.cos_alpha = ca;
     g.cos_psi = cos_psi;
+    if (!g.calc || !need_all) return g;
     double deriv;
     if (alt) {
         steffen_eval(xalt, yalt, nalt, cos_psi, &deriv);
