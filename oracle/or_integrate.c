@@ -173,7 +173,7 @@ int or_integrate(double R, double omega, double r_s, double inclination,
     double *sl = (double *)malloc(4 * N_R * sizeof(double));   /* psi -> lag */
     double *st = (double *)malloc(4 * N_R * sizeof(double));   /* cos psi -> cos a */
     double *ab = (double *)malloc(3 * N_L * sizeof(double));
-    double *mb = (double *)malloc((N_L + 3) * sizeof(double));
+    double *mb = (double *)malloc((3 * N_L + 3) * sizeof(double));
     double *acc = (double *)calloc(N_E * N_P, sizeof(double));
     n4_state n4;
     if (atm_ext == 2) n4_init(&n4, atm);
@@ -327,7 +327,7 @@ int or_integrate(double R, double omega, double r_s, double inclination,
             if (terminate) break;
             for (size_t p = 0; p < N_E && !terminate; p++) {
                 const double *yp = PROF + p * N_L;
-                or_akima_init(PHASE, yp, N_L, 1, ab, ab + N_L, ab + 2 * N_L, mb);
+                or_phase_init(PHASE, yp, N_L, ab, ab + N_L, ab + 2 * N_L, mb);
                 for (size_t jj = 0; jj < S_cols && !terminate; jj++) {
                     if (radiates[i * S_cols + jj] != 1) continue;
                     double shift = phi[i * S_cols + jj];
@@ -415,7 +415,7 @@ int or_integrate_general_ext(double omega, double inclination, size_t S_rows, si
     double *sl = (double *)malloc(4 * N_R * sizeof(double));
     double *st = (double *)malloc(4 * N_R * sizeof(double));
     double *gb = (double *)malloc(3 * N_L * sizeof(double));   /* Akima, GEOM */
-    double *mb = (double *)malloc((N_L + 3) * sizeof(double));
+    double *mb = (double *)malloc((3 * N_L + 3) * sizeof(double));
     double *zs = (double *)malloc(4 * N_L * sizeof(double));   /* Steffen, Z */
     double *as = (double *)malloc(4 * N_L * sizeof(double));   /* Steffen, ABB */
     int terminate = 0;
@@ -591,7 +591,7 @@ int or_integrate_general_ext(double omega, double inclination, size_t S_rows, si
                 }
             }
             if (terminate) break;
-            or_akima_init(PHASE, GE, N_L, 1, gb, gb + N_L, gb + 2 * N_L, mb);
+            or_phase_init(PHASE, GE, N_L, gb, gb + N_L, gb + 2 * N_L, mb);
             or_steffen_init(PHASE, ZZ, N_L, zs, zs + N_L, zs + 2 * N_L, zs + 3 * N_L);
             or_steffen_init(PHASE, AB, N_L, as, as + N_L, as + 2 * N_L, as + 3 * N_L);
             double c2 = 0.0, s2 = 0.0;

@@ -168,6 +168,98 @@ double or_akima_integ(const double *x, const double *y, size_t n,
     return s;
 }
 
+/* Periodic cubic spline (the unique C2 periodic interpolant; y[n-1] closes
+ * the period): c_i = y''(x_i) / 2 from the cyclic tridiagonal system
+ *   h_{i-1} c_{i-1} + 2 (h_{i-1} + h_i) c_i + h_i c_{i+1}
+ *     = 3 ((y_{i+1} - y_i) / h_i - (y_i - y_{i-1}) / h_{i-1}),  i = 0 .. m-1,
+ * m = n - 1 unknowns with indices taken modulo m, solved by elimination with
+ * the Sherman-Morrison correction for the two corner entries; then per
+ * interval b = s_i - h_i (c_{i+1} + 2 c_i) / 3, d = (c_{i+1} - c_i) / (3 h_i),
+ * y = y_i + t (b + t (c + t d)).  work: 3 n doubles.  Needs n >= 4. */
+void or_cspline_periodic_init(const double *x, const double *y, size_t n, double *b, double *c, double *d,
+                              double *work)
+{
+    const size_t m = n - 1;
+    double *dg = work, *rhs = work + n, *z = work + 2 * n;
+    /* diagonal, right-hand side; off-diagonal e_i = h_i couples i and i + 1 */
+    for (size_t i = 0; i < m; i++) {
+        const size_t im = (i + m - 1) % m;
+        const double hm = x[im + 1] - x[im], hi = x[i + 1] - x[i];
+        dg[i] = 2.0 * (hm + hi);
+        rhs[i] = 3.0 * ((y[i + 1] - y[i]) / hi - (y[im + 1] - y[im]) / hm);
+    }
+    /* Sherman-Morrison: A = T + u v^T with u = (g, 0.., e_last), v = (1, 0.., e_last / g),
+       where T has diag[0] - g and diag[m-1] - e_last^2 / g */
+    const double e_last = x[m] - x[m - 1];   /* couples m-1 and 0 */
+    const double g = -dg[0];
+    double *T = b;                            /* modified diagonal */
+    for (size_t i = 0; i < m; i++) T[i] = dg[i];
+    T[0] -= g;
+    T[m - 1] -= e_last * e_last / g;
+    /* solve T yv = rhs and T zv = u by the Thomas algorithm (symmetric, e_i = h_i) */
+    double *cp = d;                           /* forward-eliminated super-diagonal */
+    double *yv = c;
+    for (size_t i = 0; i < m; i++) z[i] = 0.0;
+    z[0] = g;
+    z[m - 1] = e_last;
+    {
+        double den = T[0];
+        cp[0] = (x[1] - x[0]) / den;
+        yv[0] = rhs[0] / den;
+        z[0] = z[0] / den;
+        for (size_t i = 1; i < m; i++) {
+            const double em = x[i] - x[i - 1];   /* e_{i-1} */
+            den = T[i] - em * cp[i - 1];
+            cp[i] = (i + 1 < m) ? (x[i + 1] - x[i]) / den : 0.0;
+            yv[i] = (rhs[i] - em * yv[i - 1]) / den;
+            z[i] = (z[i] - em * z[i - 1]) / den;
+        }
+        for (size_t i = m - 1; i-- > 0;) {
+            yv[i] -= cp[i] * yv[i + 1];
+            z[i] -= cp[i] * z[i + 1];
+        }
+    }
+    const double fac = (yv[0] + yv[m - 1] * e_last / g) / (1.0 + z[0] + z[m - 1] * e_last / g);
+    for (size_t i = 0; i < m; i++) c[i] = yv[i] - fac * z[i];
+    c[m] = c[0];
+    for (size_t i = 0; i < m; i++) {
+        const double h = x[i + 1] - x[i];
+        b[i] = (y[i + 1] - y[i]) / h - h * (c[i + 1] + 2.0 * c[i]) / 3.0;
+        d[i] = (c[i + 1] - c[i]) / (3.0 * h);
+    }
+}
+
+/* The phase interpolant, a process-wide choice as in the reference
+ * (xpsi/tools/core.pyx:21-116): 0 periodic Akima (default), 1 Steffen,
+ * 2 periodic cubic spline.  All three give y = y_i + t (b + t (c + t d)),
+ * so or_akima_eval and or_akima_integ evaluate and integrate any of them. */
+static int g_phase_kind = 0;
+
+int or_set_phase_interpolant(int kind)
+{
+    if (kind < 0 || kind > 2) return -1;
+    g_phase_kind = kind;
+    return 0;
+}
+
+int or_get_phase_interpolant(void) { return g_phase_kind; }
+
+/* work: 3 n + 3 doubles */
+void or_phase_init(const double *x, const double *y, size_t n, double *b, double *c, double *d, double *work)
+{
+    if (g_phase_kind == 1) {
+        /* Steffen's a, b, c (cubic, quadratic, slope) as d, c, b of the shared form */
+        double *dd = work;
+        or_steffen_init(x, y, n, d, c, b, dd);
+        return;
+    }
+    if (g_phase_kind == 2) {
+        or_cspline_periodic_init(x, y, n, b, c, d, work);
+        return;
+    }
+    or_akima_init(x, y, n, 1, b, c, d, work);
+}
+
 /* ---------------- QAG with the 61-point Gauss-Kronrod rule ---------------- */
 static const double XGK61[31] = {
   0.999484410050490637571325895705810819,

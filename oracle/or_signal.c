@@ -22,7 +22,7 @@ static double unit_draw(void)
     return (double)(lcg_state >> 11) / 9007199254740992.0;
 }
 
-/* energy_integrator.pyx:27-114 (phase interpolant = Akima periodic) */
+/* energy_integrator.pyx:27-114 (the phase interpolant, periodic Akima by default) */
 void or_energy_integrator(size_t N_E, size_t N_P, const double *signal,
                           const double *logE, size_t N_edges,
                           const double *logEdges, double *out)
@@ -31,12 +31,12 @@ void or_energy_integrator(size_t N_E, size_t N_P, const double *signal,
     double maxE = logE[N_E - 1];
     double *y = (double *)malloc(N_E * sizeof(double));
     double *c = (double *)malloc(3 * N_E * sizeof(double));
-    double *m = (double *)malloc((N_E + 3) * sizeof(double));
+    double *m = (double *)malloc((3 * N_E + 3) * sizeof(double));
     memset(out, 0, N_in * N_P * sizeof(double));
     for (size_t i = 0; i < N_P; i++) {
         for (size_t j = 0; j < N_E; j++)
             y[j] = pow(10.0, logE[j]) * signal[j * N_P + i] * log(10.0);
-        or_akima_init(logE, y, N_E, 1, c, c + N_E, c + 2 * N_E, m);
+        or_phase_init(logE, y, N_E, c, c + N_E, c + 2 * N_E, m);
         for (size_t j = 0; j < N_in; j++) {
             double up = logEdges[j + 1] > maxE ? maxE : logEdges[j + 1];
             out[j * N_P + i] = or_akima_integ(logE, y, N_E, c, c + N_E, c + 2 * N_E,
@@ -181,7 +181,7 @@ static int channel_marginal(margs *a, double precomp, double *LL,
 /* compute_expected_counts.pyx:66-197: channel i's expected count rate in
    each phase bin (shifted, wrapped, negative bin integrals dropped unless
    allowed, then clipped at zero), added into st[N_b] (zeroed by the caller);
-   cb [3 (max N_P + 1)] and mb [max N_P + 4] are scratch */
+   cb [3 (max N_P + 1)] and mb [3 max N_P + 4] are scratch */
 static void channel_rate(size_t i, size_t N_b, const double *phases, size_t n_comp,
                          const double *const *comps, const size_t *comp_N_P,
                          const double *const *comp_phases, const double *shifts,
@@ -193,7 +193,7 @@ static void channel_rate(size_t i, size_t N_b, const double *phases, size_t n_co
         const double *pulse = comps[c] + i * NP;
         if (NP > 1) {
             const double *ph = comp_phases[c];
-            or_akima_init(ph, pulse, NP, 1, cb, cb + NP, cb + 2 * NP, mb);
+            or_phase_init(ph, pulse, NP, cb, cb + NP, cb + 2 * NP, mb);
             for (size_t j = 0; j < N_b; j++) {
                 double pa = phases[j] + shifts[c], pb = phases[j + 1] + shifts[c];
                 if (eq12(pb - pa, 1.0)) { pa = 0.0; pb = 1.0; }
@@ -233,7 +233,7 @@ int or_eval_marginal_likelihood(double T_exp, size_t N_b, const double *phases,
     size_t maxP = 0;
     for (size_t c = 0; c < n_comp; c++) if (comp_N_P[c] > maxP) maxP = comp_N_P[c];
     double *cb = (double *)malloc(3 * (maxP + 1) * sizeof(double));
-    double *mb = (double *)malloc((maxP + 4) * sizeof(double));
+    double *mb = (double *)malloc((3 * maxP + 4) * sizeof(double));
     memset(STAR, 0, N_ch * N_b * sizeof(double));
     memset(B_ML, 0, N_ch * sizeof(double));
     memset(B_ML_supp, 0, N_ch * sizeof(double));
@@ -284,7 +284,7 @@ int or_poisson_likelihood_given_background(double T_exp, size_t N_b, const doubl
     size_t maxP = 0;
     for (size_t c = 0; c < n_comp; c++) if (comp_N_P[c] > maxP) maxP = comp_N_P[c];
     double *cb = (double *)malloc(3 * (maxP + 1) * sizeof(double));
-    double *mb = (double *)malloc((maxP + 4) * sizeof(double));
+    double *mb = (double *)malloc((3 * maxP + 4) * sizeof(double));
     memset(EXPEC, 0, N_ch * N_b * sizeof(double));
     double LL = 0.0;
     for (size_t i = 0; i < N_ch; i++) {

@@ -158,7 +158,7 @@ int or_integrate_stokes(double omega, double inclination, size_t S_rows, size_t 
     double *ai = (double *)malloc(3 * N_L * sizeof(double));
     double *aq = (double *)malloc(3 * N_L * sizeof(double));
     double *au = (double *)malloc(3 * N_L * sizeof(double));
-    double *mb = (double *)malloc((N_L + 3) * sizeof(double));
+    double *mb = (double *)malloc((3 * N_L + 3) * sizeof(double));
     int terminate = 0;
     for (size_t i = 0; i < S_rows && !terminate; i++) {
         size_t J = 0, j;
@@ -321,10 +321,10 @@ int or_integrate_stokes(double omega, double inclination, size_t S_rows, size_t 
             if (terminate) break;
             for (size_t p = 0; p < N_E && !terminate; p++) {
                 const double *yi = PI_ + p * N_L, *yq = PQ + p * N_L, *yu = PU + p * N_L;
-                or_akima_init(PHASE, yi, N_L, 1, ai, ai + N_L, ai + 2 * N_L, mb);
+                or_phase_init(PHASE, yi, N_L, ai, ai + N_L, ai + 2 * N_L, mb);
                 if (pol) {
-                    or_akima_init(PHASE, yq, N_L, 1, aq, aq + N_L, aq + 2 * N_L, mb);
-                    or_akima_init(PHASE, yu, N_L, 1, au, au + N_L, au + 2 * N_L, mb);
+                    or_phase_init(PHASE, yq, N_L, aq, aq + N_L, aq + 2 * N_L, mb);
+                    or_phase_init(PHASE, yu, N_L, au, au + N_L, au + 2 * N_L, mb);
                 }
                 for (size_t jj = 0; jj < S_cols && !terminate; jj++) {
                     if (radiates[i * S_cols + jj] != 1) continue;

@@ -78,6 +78,11 @@ def lib():
         L.or_akima_init.argtypes = [_dp, _dp, sz, i, _dp, _dp, _dp, _dp]
         L.or_akima_eval.restype = d
         L.or_akima_eval.argtypes = [_dp, _dp, sz, _dp, _dp, _dp, d]
+        L.or_cspline_periodic_init.argtypes = [_dp, _dp, sz, _dp, _dp, _dp, _dp]
+        L.or_set_phase_interpolant.restype = i
+        L.or_set_phase_interpolant.argtypes = [i]
+        L.or_get_phase_interpolant.restype = i
+        L.or_phase_init.argtypes = [_dp, _dp, sz, _dp, _dp, _dp, _dp]
         L.or_akima_integ.restype = d
         L.or_akima_integ.argtypes = [_dp, _dp, sz, _dp, _dp, _dp, d, d]
         L.or_effective_gravity.restype = d
@@ -156,6 +161,31 @@ def akima(x, y, periodic=True):
     n = len(x)
     co = [np.zeros(n) for _ in range(3)]
     lib().or_akima_init(x, y, n, int(periodic), *co, np.zeros(n + 3))
+    return x, y, co
+
+
+PHASE_INTERPOLANTS = {"Akima": 0, "Steffen": 1, "Cubic": 2}
+
+
+def set_phase_interpolant(name):
+    """xpsi.tools.core.set_phase_interpolant: "Akima" (periodic, default),
+    "Steffen" or "Cubic" (periodic cubic spline), for every later call."""
+    if lib().or_set_phase_interpolant(PHASE_INTERPOLANTS[name]) != 0:
+        raise ValueError(name)
+
+
+def get_phase_interpolant():
+    k = lib().or_get_phase_interpolant()
+    return [n for n, v in PHASE_INTERPOLANTS.items() if v == k][0]
+
+
+def phase_spline(x, y):
+    """Coefficients (x, y, [b, c, d]) of the current phase interpolant, for
+    akima_eval and akima_integ (the shared piece form)."""
+    x, y = _c(x), _c(y)
+    n = len(x)
+    co = [np.zeros(n) for _ in range(3)]
+    lib().or_phase_init(x, y, n, *co, np.zeros(3 * n + 3))
     return x, y, co
 
 

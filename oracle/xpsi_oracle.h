@@ -34,6 +34,14 @@ void or_akima_init(const double *x, const double *y, size_t n, int periodic,
 double or_akima_eval(const double *x, const double *y, size_t n,
                      const double *b, const double *c, const double *d,
                      double xq);
+void or_cspline_periodic_init(const double *x, const double *y, size_t n, double *b, double *c, double *d,
+                              double *work);
+/* the process-wide phase interpolant (xpsi/tools/core.pyx:21-116): 0 periodic
+ * Akima, 1 Steffen, 2 periodic cubic spline; or_phase_init fills the shared
+ * y_i + t (b + t (c + t d)) form (work: 3 n + 3 doubles) */
+int or_set_phase_interpolant(int kind);
+int or_get_phase_interpolant(void);
+void or_phase_init(const double *x, const double *y, size_t n, double *b, double *c, double *d, double *work);
 double or_akima_integ(const double *x, const double *y, size_t n,
                       const double *b, const double *c, const double *d,
                       double lo, double hi);

@@ -147,3 +147,77 @@ def test_precomputation_is_minus_sum_lnfact():
     counts = rng.integers(0, 200, size=(7, 9)).astype(np.int32)
     pre = O.precomputation(counts)
     assert np.allclose(pre, -gammaln(counts + 1.0).sum(axis=1), rtol=1e-13)
+
+
+# ------------------------------------------ the phase interpolant choice
+# xpsi/tools/core.pyx:21-116: "Akima" (periodic, default), "Steffen", "Cubic"
+# (periodic cubic spline).  The periodic cubic spline is the unique C2
+# periodic interpolant, so scipy's CubicSpline(bc_type="periodic") pins it.
+@pytest.fixture
+def phase_interpolant():
+    yield O.set_phase_interpolant
+    O.set_phase_interpolant("Akima")
+
+
+def _periodic_data(seed, n):
+    rng = np.random.default_rng(seed)
+    x = np.sort(np.concatenate([[0.0, 1.0], rng.uniform(0.0, 1.0, n - 2)]))
+    y = np.cos(2 * np.pi * x) + 0.3 * np.sin(6 * np.pi * x) + rng.normal(0.0, 0.05, n)
+    y[-1] = y[0]
+    return x, y
+
+
+def test_periodic_cubic_spline_matches_scipy(phase_interpolant):
+    from scipy.interpolate import CubicSpline
+    for seed, n in ((1, 5), (2, 17), (3, 64)):
+        x, y = _periodic_data(seed, n)
+        ref = CubicSpline(x, y, bc_type="periodic")
+        phase_interpolant("Cubic")
+        sp = O.phase_spline(x, y)
+        xq = np.random.default_rng(seed + 10).uniform(0.0, 1.0, 200)
+        got = np.array([O.akima_eval(sp, q) for q in xq])
+        assert np.max(np.abs(got - ref(xq))) < 1e-12 * np.max(np.abs(y))
+        for a, b in ((0.1, 0.7), (0.0, 1.0), (0.33, 0.34)):
+            assert abs(O.akima_integ(sp, a, b) - ref.integrate(a, b)) < 1e-12 * np.max(np.abs(y))
+
+
+def test_steffen_as_phase_interpolant(phase_interpolant):
+    x, y = _periodic_data(4, 33)
+    st = O.steffen(x, y)
+    phase_interpolant("Steffen")
+    sp = O.phase_spline(x, y)
+    for q in np.linspace(0.0, 1.0, 101):
+        assert abs(O.akima_eval(sp, q) - O.steffen_eval(st, q)) < 1e-14 * np.max(np.abs(y))
+    # the integral of the same cubic pieces, against quadrature
+    val, _ = si.quad(lambda q: O.steffen_eval(st, q), 0.2, 0.9, points=list(x[(x > 0.2) & (x < 0.9)]),
+                     limit=200, epsabs=1e-14, epsrel=1e-13)
+    assert abs(O.akima_integ(sp, 0.2, 0.9) - val) < 1e-12
+
+
+def test_phase_interpolants_in_the_integrator(phase_interpolant):
+    # every interpolant converges to the same pulse with the leaf count, and
+    # the choice reaches the integrator (different at a finite leaf count)
+    st = O.spacetime(1.4, 12.0, 1.0, math.cos(1.0), 300.0, "AGM_14")
+    hr = dict(super_colatitude=1.2, super_radius=0.8, super_temperature=6.3, phase_shift=0.0,
+              sqrt_num_cells=24, min_sqrt_num_cells=10, max_sqrt_num_cells=100, num_rays=512, num_leaves=64)
+    members, err = O.hot_region_embed(st, hr, st["f"])
+    m = members[0]
+    energies = np.array([0.5, 2.0])
+    phases = 2.0 * math.pi * np.linspace(0.0, 1.0, 32)
+
+    def flux(N_L):
+        leaves = np.linspace(0.0, 2.0 * math.pi, N_L)
+        t, f = O.integrate(st["Omega"], st["r_s"], st["i"], m["cellArea"], m["r"], m["r_s_over_r"], m["theta"],
+                           m["phi"], m["params"], m["radiates"], m["deflection"], m["cos_alpha"], m["lag"],
+                           m["maxDeflection"], m["cos_gamma"], energies, leaves, phases, R=st["R"])
+        assert t == 0
+        return f
+
+    ref = flux(512)
+    out = {}
+    for name in ("Akima", "Steffen", "Cubic"):
+        phase_interpolant(name)
+        out[name] = [np.max(np.abs(flux(n) - ref)) / np.max(ref) for n in (48, 96)]
+    for name, (e48, e96) in out.items():
+        assert e96 < e48 and e96 < 1e-3, (name, e48, e96)
+    assert out["Steffen"][0] != out["Akima"][0] and out["Cubic"][0] != out["Akima"][0]
