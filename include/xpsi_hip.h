@@ -511,6 +511,15 @@ int xh_set_wave_fold_rings(int rings);
  * Applies to pipelines created afterwards. */
 int xh_set_wave_fold_chain(int intervals);
 
+/* Replaces xpsi/tools/core.pyx:21-116 set_phase_interpolant /
+ * get_phase_interpolant: the process-wide phase interpolant of the
+ * integrators' profiles, the energy integral and the likelihoods' phase-bin
+ * integrals: 0 periodic Akima (default), 1 Steffen, 2 periodic cubic spline.
+ * Read at each drop-in call; xh_integrate (whose folds are periodic Akima)
+ * and xh_pipeline_create refuse a choice other than 0. */
+int xh_set_phase_interpolant(int kind);
+int xh_get_phase_interpolant(void);
+
 /* Tuning: phases per wave of the energy-lane fold (8 or 16).  Applies to
  * pipelines created afterwards and to xh_integrate. */
 int xh_set_lane_phases(int nq);

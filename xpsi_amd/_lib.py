@@ -176,6 +176,10 @@ def lib():
     L.xh_set_wave_fold_rings.argtypes = [i32]
     L.xh_set_wave_fold_chain.restype = i32
     L.xh_set_wave_fold_chain.argtypes = [i32]
+    L.xh_set_phase_interpolant.restype = i32
+    L.xh_set_phase_interpolant.argtypes = [i32]
+    L.xh_get_phase_interpolant.restype = i32
+    L.xh_get_phase_interpolant.argtypes = []
     L.xh_set_lane_phases.restype = i32
     L.xh_set_lane_phases.argtypes = [i32]
     L.xh_pipeline_use_graphs.restype = i32
@@ -227,6 +231,6 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",
             "xh_pipeline_members_per_request", "xh_pipeline_work",
-            "xh_set_fold_energy_chunk", "xh_set_fold_kernel", "xh_set_moment_shape", "xh_set_fold_rings", "xh_set_wave_fold_rings", "xh_set_wave_fold_chain", "xh_set_lane_phases", "xh_pipeline_use_graphs", "xh_server_create",
+            "xh_set_fold_energy_chunk", "xh_set_fold_kernel", "xh_set_moment_shape", "xh_set_fold_rings", "xh_set_wave_fold_rings", "xh_set_wave_fold_chain", "xh_set_phase_interpolant", "xh_get_phase_interpolant", "xh_set_lane_phases", "xh_pipeline_use_graphs", "xh_server_create",
             "xh_server_submit", "xh_server_poll", "xh_server_stats", "xh_server_destroy",
             "xh_server_set_policy", "xh_server_footprint", "xh_server_warm", "xh_pipeline_capture"]

@@ -241,6 +241,56 @@ extern "C" int xh_set_fold_rings(int rings)
     return XH_OK;
 }
 
+// The phase interpolant (xpsi/tools/core.pyx:21-116), a process-wide choice
+// as in the reference: 0 periodic Akima (default), 1 Steffen, 2 periodic
+// cubic spline.  The drop-ins read it at each call; the batched pipeline's
+// folds are periodic Akima and its creation refuses another choice.
+static int g_phase_interp = 0;
+
+extern "C" int xh_set_phase_interpolant(int kind)
+{
+    if (kind < 0 || kind > 2)
+        return fail(XH_EINVAL, "phase interpolant must be 0 (Akima, periodic), 1 (Steffen) or 2 (cubic, periodic)");
+    g_phase_interp = kind;
+    return XH_OK;
+}
+
+extern "C" int xh_get_phase_interpolant(void) { return g_phase_interp; }
+
+// dynamic LDS of the periodic cubic spline's tables for a marginal-family
+// launch: 3 x (the components' phase counts) + one component's work
+static int cspline_tab_lds(const MarginalArgs& A, hipStream_t s, size_t* out)
+{
+    *out = 0;
+    if (A.interp != 2) return XH_OK;
+    int P_tot = A.n_comp * A.N_P, max_np = A.N_P;
+    if (A.comp_np) {
+        int h[32];
+        HIPCHK(hipMemcpyAsync(h, A.comp_np, (size_t)A.n_comp * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        P_tot = 0;
+        max_np = 0;
+        for (int c = 0; c < A.n_comp; c++) {
+            if (h[c] > 1 && h[c] < 4) return fail(XH_EINVAL, "the periodic cubic spline needs 4 or more phases");
+            P_tot += h[c];
+            max_np = std::max(max_np, h[c]);
+        }
+    } else if (A.N_P > 1 && A.N_P < 4) {
+        return fail(XH_EINVAL, "the periodic cubic spline needs 4 or more phases");
+    }
+    *out = (3 * (size_t)P_tot + (size_t)max_np) * 8;
+    static bool attr = false;
+    if (!attr) {
+        HIPCHK(hipFuncSetAttribute((const void*)&k_marginal, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+        HIPCHK(hipFuncSetAttribute((const void*)&k_poisson_bg, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+        HIPCHK(hipFuncSetAttribute((const void*)&k_star_rate, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+        attr = true;
+    }
+    if (*out + 2 * (size_t)A.N_b * 8 > 159 * 1024)
+        return fail(XH_EINVAL, "the periodic cubic spline's tables do not fit one block's LDS");
+    return XH_OK;
+}
+
 // intervals per FP32 accumulation chain of the wave fold before the chain's
 // sums are added to the FP64 flux (0 = one chain per parallel).  Longer chains
 // round more; their error is shared by neighbouring bins, so it shows in the
@@ -391,6 +441,9 @@ extern "C" int xh_integrate(double omega, double r_s, double inclination, int S,
     if (hot_atm_ext == 2 && (!atm_axes || !atm_N || !atm_I))
         return fail(XH_EINVAL, "xh_integrate: the 4-D table atmosphere needs its axes and values");
     if (S < 2 || N_R < 3 || N_L < 5 || N_E < 1 || N_P < 1) return fail(XH_EINVAL, "xh_integrate: bad sizes");
+    if (g_phase_interp != 0)
+        return fail(XH_EINVAL, "xh_integrate folds with the periodic Akima interpolant; xh_integrate_stokes takes the "
+                               "other phase interpolants (blackbody atmospheres)");
     const int EC = pick_ec(N_L);
     const bool lane_fits = lane_lds(N_L, S, 8) <= 160 * 1024;
     const bool moment_fits = moment_lds(N_L, S, 8, 1, hot_atm_ext) <= kMomentLdsMax;
@@ -549,6 +602,7 @@ extern "C" int xh_integrate_stokes(double omega, double inclination, int S, cons
     }
     StokesArgs SA{d_geo, RA.pol, d_valid, cellArea, radiates, phi, cellParams, n_params, correction, atm_ext,
                   beam_opt, energies, phases, flux_I, flux_Q, flux_U};
+    SA.interp = g_phase_interp;
     hipLaunchKernelGGL(k_stokes_fold, dim3(S, IO, (unsigned)tiles), dim3(nt), stokes_fold_lds(N_L), s, d_m, L, SA);
     HIPCHK(hipGetLastError());
     Member back{};
@@ -638,6 +692,7 @@ extern "C" int xh_integrate_general(double omega, double inclination, int S, con
     }
     GeneralArgs GA{d_geo, d_valid, cellArea, radiates, phi, cellParams, correction, energies, phases, flux,
                    n_params, atm_ext, beam_opt, RA.pol, flux_Q, flux_U};
+    GA.interp = g_phase_interp;
     const int tiles = ((N_E + 31) / 32) * ((N_P + 63) / 64);
     hipLaunchKernelGGL(k_general_fold, dim3(S, IO, tiles), dim3(256), general_fold_lds(N_L), s, d_m, L, GA);
     HIPCHK(hipGetLastError());
@@ -862,6 +917,21 @@ extern "C" int xh_energy_integrator(int N_E, int N_P, const double* signal, cons
     if (N_E > 8192) return fail(XH_EINVAL, "xh_energy_integrator: at most 8192 energies (one LDS row)");
     // no member table: the kernel integrates the signal itself (d^2 = 1)
     EnergyArgs A{signal, logE, logEdges, nullptr, nullptr, out, N_E, N_P, N_edges, 1};
+    A.interp = g_phase_interp;
+    if (A.interp == 2) {
+        // the periodic cubic spline: one block per phase column solves it in LDS
+        const size_t lds = (size_t)5 * N_E * 8;
+        if (lds > 159 * 1024) return fail(XH_EINVAL, "xh_energy_integrator: at most 4070 energies with the cubic spline");
+        static bool attr = false;
+        if (!attr) {
+            HIPCHK(hipFuncSetAttribute((const void*)&k_energy_integrate_cs, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       159 * 1024));
+            attr = true;
+        }
+        hipLaunchKernelGGL(k_energy_integrate_cs, dim3(N_P), dim3(256), lds, s, A);
+        HIPCHK(hipGetLastError());
+        return XH_OK;
+    }
     int n = (N_edges - 1) * N_P;
     hipLaunchKernelGGL(k_energy_integrate, dim3((n + 255) / 256, 1), dim3(256), (size_t)N_E * 8, s,
                        (const Member*)nullptr, A);
@@ -916,10 +986,16 @@ static int marginal_impl(double T, int N_b, const double* phases, int N_ch, cons
     A.exposure = T; A.epsabs = epsabs; A.epsrel = epsrel; A.epsilon = epsilon; A.sigmas = sigmas;
     A.llzero = llzero; A.slim = slim; A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp;
     A.allow_negative = allow; A.seed = next_seed(); A.seed_ptr = nullptr;
+    A.interp = g_phase_interp;
+    size_t tab = 0;
+    {
+        const int rc = cspline_tab_lds(A, s, &tab);
+        if (rc != XH_OK) return rc;
+    }
     HIPCHK(fill(STAR, (size_t)N_ch * N_b, 0.0, s));
     HIPCHK(fill(B_ML, N_ch, 0.0, s));
     HIPCHK(fill(B_ML_supp, N_ch, 0.0, s));
-    hipLaunchKernelGGL(k_marginal, dim3(N_ch, 1), dim3(64), 2 * N_b * 8, s, A);
+    hipLaunchKernelGGL(k_marginal, dim3(N_ch, 1), dim3(64), 2 * N_b * 8 + tab, s, A);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_lnl_reduce, dim3(1), dim3(64), 0, s, A, (const Member*)nullptr, 0, d_lnl);
     HIPCHK(hipGetLastError());
@@ -989,7 +1065,13 @@ static int poisson_impl(double T, int N_b, const double* phases, int N_ch, const
     A.precomp = precomp; A.background = background; A.ll = d_ll; A.status = d_st; A.STAR = EXPEC;
     A.exposure = T; A.llzero = -1.0e90; A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp;
     A.allow_negative = allow; A.seed = next_seed(); A.seed_ptr = nullptr;
-    hipLaunchKernelGGL(k_poisson_bg, dim3(N_ch, 1), dim3(64), 2 * (size_t)N_b * 8, s, A);
+    A.interp = g_phase_interp;
+    size_t tab = 0;
+    {
+        const int rc = cspline_tab_lds(A, s, &tab);
+        if (rc != XH_OK) return rc;
+    }
+    hipLaunchKernelGGL(k_poisson_bg, dim3(N_ch, 1), dim3(64), 2 * (size_t)N_b * 8 + tab, s, A);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_poisson_reduce, dim3(1), dim3(64), 0, s, A, d_lnl);
     HIPCHK(hipGetLastError());
@@ -1042,7 +1124,13 @@ static int synthesise(int mode, double T, double n_star, int N_b, const double* 
     MarginalArgs A{};
     A.signals = comps; A.comp_phases = comp_phases; A.shifts = shifts; A.phases = phases;
     A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp; A.allow_negative = allow_negative ? ~0u : 0u;
-    hipLaunchKernelGGL(k_star_rate, dim3(N_ch), dim3(64), 0, s, A, rate);
+    A.interp = g_phase_interp;
+    size_t tab = 0;
+    {
+        const int rc = cspline_tab_lds(A, s, &tab);
+        if (rc != XH_OK) return rc;
+    }
+    hipLaunchKernelGGL(k_star_rate, dim3(N_ch), dim3(64), tab, s, A, rate);
     HIPCHK(hipGetLastError());
     const int n = N_ch * N_b;
     hipLaunchKernelGGL(k_synth_totals, dim3(1), dim3(64), 0, s, (const double*)rate, background, n,
@@ -1183,6 +1271,10 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
     if (!c || c->n_regions < 1 || c->n_regions > 2 || c->max_batch < 1 || c->N_E < 5 || c->N_E > 8192 || c->num_leaves < 5 ||
         c->num_rays < 3 || c->N_in < 1 || c->N_ch < 1 || c->N_b < 1) {
         fail(XH_EINVAL, "xh_pipeline_create: bad configuration");
+        return nullptr;
+    }
+    if (g_phase_interp != 0) {
+        fail(XH_EINVAL, "xh_pipeline_create: the batched pipeline uses the periodic Akima phase interpolant");
         return nullptr;
     }
     xh_pipeline* p = new xh_pipeline();   // value-initialised: every field zero

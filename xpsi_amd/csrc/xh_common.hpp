@@ -211,6 +211,108 @@ __device__ inline double akima_integ(const double* x, const double* y, int n, bo
     return s;
 }
 
+// ---- the phase interpolant (xpsi/tools/core.pyx:21-116), a process-wide
+// choice: 0 periodic Akima (default), 1 Steffen, 2 periodic cubic spline.
+// Each gives y = y_i + t (b + t (c + t d)) on interval i, so the evaluation
+// and the piece integrals are shared.
+
+// Steffen piece i from the slopes at its two nodes
+__device__ inline void steffen_coefs(const double* x, const double* y, int n, int i, double& b, double& c,
+                                     double& d)
+{
+    const double h = x[i + 1] - x[i];
+    const double s = (y[i + 1] - y[i]) / h;
+    const double y0 = steffen_slope(x, y, n, i), y1 = steffen_slope(x, y, n, i + 1);
+    b = y0;
+    c = (3.0 * s - 2.0 * y0 - y1) / h;
+    d = (y0 + y1 - 2.0 * s) / h / h;
+}
+
+// Periodic cubic spline, one thread: the unique C2 periodic interpolant
+// (y[n-1] closes the period).  c_i = y''(x_i) / 2 solves the cyclic
+// tridiagonal system h_{i-1} c_{i-1} + 2 (h_{i-1} + h_i) c_i + h_i c_{i+1} =
+// 3 (s_i - s_{i-1}) over m = n - 1 unknowns (indices modulo m), by the Thomas
+// algorithm with the Sherman-Morrison correction for the corner entries;
+// then b_i = s_i - h_i (c_{i+1} + 2 c_i) / 3 and d_i = (c_{i+1} - c_i) / (3 h_i).
+// b, c, d: n entries each (c[n-1] = c[0]); z: n of work.  Needs n >= 4.
+__device__ inline void cspline_periodic(const double* x, const double* y, int n, double* b, double* c, double* d,
+                                        double* z)
+{
+    const int m = n - 1;
+    const double e_last = x[m] - x[m - 1];
+    double dg0 = 0.0;
+    for (int i = 0; i < m; i++) {
+        const int im = (i + m - 1) % m;
+        const double hm = x[im + 1] - x[im], hi = x[i + 1] - x[i];
+        const double dg = 2.0 * (hm + hi);
+        if (i == 0) dg0 = dg;
+        b[i] = dg;                                              // diagonal
+        c[i] = 3.0 * ((y[i + 1] - y[i]) / hi - (y[im + 1] - y[im]) / hm);   // right-hand side
+        z[i] = 0.0;
+    }
+    const double g = -dg0;
+    b[0] -= g;
+    b[m - 1] -= e_last * e_last / g;
+    z[0] = g;
+    z[m - 1] = e_last;
+    double den = b[0];
+    d[0] = (x[1] - x[0]) / den;
+    c[0] = c[0] / den;
+    z[0] = z[0] / den;
+    for (int i = 1; i < m; i++) {
+        const double em = x[i] - x[i - 1];
+        den = b[i] - em * d[i - 1];
+        d[i] = (i + 1 < m) ? (x[i + 1] - x[i]) / den : 0.0;
+        c[i] = (c[i] - em * c[i - 1]) / den;
+        z[i] = (z[i] - em * z[i - 1]) / den;
+    }
+    for (int i = m - 2; i >= 0; i--) {
+        c[i] -= d[i] * c[i + 1];
+        z[i] -= d[i] * z[i + 1];
+    }
+    const double fac = (c[0] + c[m - 1] * e_last / g) / (1.0 + z[0] + z[m - 1] * e_last / g);
+    for (int i = 0; i < m; i++) c[i] = c[i] - fac * z[i];
+    c[m] = c[0];
+    for (int i = 0; i < m; i++) {
+        const double h = x[i + 1] - x[i];
+        b[i] = (y[i + 1] - y[i]) / h - h * (c[i + 1] + 2.0 * c[i]) / 3.0;
+        d[i] = (c[i + 1] - c[i]) / (3.0 * h);
+    }
+}
+
+// piece i of the chosen interpolant; kind 2 reads the tables cspline_periodic wrote
+__device__ inline void phase_coefs(int kind, const double* x, const double* y, int n, int i, const double* B,
+                                   const double* C, const double* D, double& b, double& c, double& d)
+{
+    if (kind == 2) { b = B[i]; c = C[i]; d = D[i]; }
+    else if (kind == 1) steffen_coefs(x, y, n, i, b, c, d);
+    else akima_coefs(x, y, n, true, i, b, c, d);
+}
+
+// exact integral of the chosen interpolant over [lo, hi] (NaN outside range)
+__device__ inline double phase_integ(int kind, const double* x, const double* y, int n, const double* B,
+                                     const double* C, const double* D, double lo, double hi)
+{
+    if (lo > hi || lo < x[0] || hi > x[n - 1]) return __builtin_nan("");
+    if (lo == hi) return 0.0;
+    int ia = interval_of(x, n, lo), ib = interval_of(x, n, hi);
+    double s = 0.0;
+    for (int i = ia; i <= ib; i++) {
+        double xl = x[i], xh = x[i + 1], h = xh - xl;
+        if (h == 0.0) return 0.0;
+        double b, c, d;
+        phase_coefs(kind, x, y, n, i, B, C, D, b, c, d);
+        if (i == ia || i == ib) {
+            double a1 = (i == ia) ? lo : xl;
+            double a2 = (i == ib) ? hi : xh;
+            s += akima_piece_integral(y[i], b, c, d, xl, a1, a2);
+        } else {
+            s += h * (y[i] + h * (0.5 * b + h * (c / 3.0 + 0.25 * d * h)));
+        }
+    }
+    return s;
+}
+
 // 61-point Gauss-Kronrod nodes (positive half, descending; index 30 = centre)
 // and weights; the 30-point Gauss weights belong to the odd Kronrod indices.
 // Computed from the Stieltjes polynomial of P_30 in 80-digit arithmetic.

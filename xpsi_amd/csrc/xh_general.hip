@@ -299,10 +299,15 @@ __global__ void __launch_bounds__(256) k_general_fold(Member* __restrict__ membe
         b_[q] = (3.0 * s - 2.0 * y0 - y1) / h;
         c_[q] = y0;
     };
+    // GEOM's phase interpolant (periodic Akima, Steffen, or the periodic cubic
+    // spline solved by one thread); Z and mu eta are Steffen throughout
+    if (F.interp == 2 && tid == 0) cspline_periodic(PH, GY, N_L, Gb, Gc, Gd, Ac + N_L);
     for (int q = tid; q < N_L - 1; q += blockDim.x) {
-        double b, c, d;
-        akima_coefs(PH, GY, N_L, true, q, b, c, d);
-        Gb[q] = b; Gc[q] = c; Gd[q] = d;
+        if (F.interp != 2) {
+            double b, c, d;
+            phase_coefs(F.interp, PH, GY, N_L, q, nullptr, nullptr, nullptr, b, c, d);
+            Gb[q] = b; Gc[q] = c; Gd[q] = d;
+        }
         steffen(ZY, q, Za, Zb, Zc);
         if (need_abb) steffen(AY, q, Aa, Ab, Ac);
     }

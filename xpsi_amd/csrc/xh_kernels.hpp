@@ -12,6 +12,7 @@ struct EnergyArgs {
     const int* member_index;    // [region][members_per_region], -1 = none
     double* out;                // [region][N_in][N_P]
     int N_E, N_P, N_edges, members_per_region;
+    int interp;                 // phase interpolant (0 periodic Akima, 1 Steffen, 2 periodic cubic spline)
 };
 
 struct FoldGemmArgs {
@@ -40,6 +41,7 @@ struct MarginalArgs {
     double exposure, epsabs, epsrel, epsilon, sigmas, llzero, slim;
     int N_ch, N_b, N_P, n_comp;
     unsigned allow_negative;    // bit c: component c keeps negative bin integrals
+    int interp;                 // phase interpolant: 0 periodic Akima, 1 Steffen, 2 periodic cubic spline
     // draw of the llzero-class values: *seed_ptr when set (device memory the
     // host refreshes before each launch, so replayed graphs draw afresh), else seed
     unsigned long long seed;
@@ -64,6 +66,8 @@ __global__ void k_rays(const Member* members, Layout L, const double* rr, const 
 __global__ void k_rays_fix(Member* members, Layout L, const double* rr, double* maxAlpha,
                            double* defl, double* lag, double* maxDefl, int calibrate, const double* uu);
 __global__ void k_energy_integrate(const Member* members, EnergyArgs A);
+// the periodic cubic spline's energy integral (drop-in): block per phase column
+__global__ void k_energy_integrate_cs(EnergyArgs A);
 __global__ void k_fold_f64(FoldGemmArgs A);
 __global__ void k_fold_f32(FoldGemmArgs A, const float* Mf);
 template <int NT>
@@ -138,9 +142,10 @@ struct StokesArgs {
     double* flux_I;            // [N_E][N_P], zeroed beforehand
     double* flux_Q;            // same, or nullptr
     double* flux_U;
+    int interp;                // phase interpolant of the profiles
 };
 __global__ void k_stokes_fold(Member* members, Layout L, StokesArgs F);
-inline size_t stokes_fold_lds(int N_L) { return (size_t)13 * N_L * 8; }
+inline size_t stokes_fold_lds(int N_L) { return (size_t)16 * N_L * 8; }
 
 // integrator without azimuthal invariance (blackbody, per-cell temperatures)
 struct GeneralArgs {
@@ -160,11 +165,12 @@ struct GeneralArgs {
     const double* pol;         // [ring][image][2] cos 2chi, sin 2chi (Stokes form) or nullptr
     double* flux_Q;            // [N_E][N_P] or nullptr
     double* flux_U;
+    int interp;                // phase interpolant of GEOM
 };
 __global__ void k_general_geometry(Member* members, Layout L, RingArgs A);
 __global__ void k_general_fold(Member* members, Layout L, GeneralArgs F);
 inline size_t general_geo_lds(int N_R, int N_L) { return (size_t)(5 * N_R + 8 * N_L) * 8 + N_L + 16; }
-inline size_t general_fold_lds(int N_L) { return (size_t)13 * N_L * 8; }
+inline size_t general_fold_lds(int N_L) { return (size_t)14 * N_L * 8; }
 
 // wave-autonomous moment fold (precision form, blackbody): one wavefront per
 // (ring block, 64 energies x 64 phases, member); flux zeroed beforehand

@@ -60,7 +60,23 @@ __global__ void __launch_bounds__(256) k_energy_integrate(const Member* __restri
                 };
                 double xl = A.logE[i], xh = A.logE[i + 1], h = xh - xl;
                 double b, c, d;
-                akima_piece(sec(i - 2), sec(i - 1), sec(i), sec(i + 1), sec(i + 2), h, b, c, d);
+                if (A.interp == 1) {
+                    // Steffen: the slopes at the interval's two nodes (one-sided at the ends)
+                    auto slope = [&](int kk) {
+                        if (kk == 0) return sec(0);
+                        if (kk == N_E - 1) return sec(N_E - 2);
+                        const double hi_ = A.logE[kk + 1] - A.logE[kk], hm = A.logE[kk] - A.logE[kk - 1];
+                        const double si = sec(kk), sm = sec(kk - 1);
+                        const double pp = (sm * hi_ + si * hm) / (hm + hi_);
+                        return (sgn1(sm) + sgn1(si)) * fmin(fabs(sm), fmin(fabs(si), 0.5 * fabs(pp)));
+                    };
+                    const double s_ = sec(i), y0 = slope(i), y1 = slope(i + 1);
+                    b = y0;
+                    c = (3.0 * s_ - 2.0 * y0 - y1) / h;
+                    d = (y0 + y1 - 2.0 * s_) / h / h;
+                } else {
+                    akima_piece(sec(i - 2), sec(i - 1), sec(i), sec(i + 1), sec(i + 2), h, b, c, d);
+                }
                 double yi = yv(i);
                 if (i == ia || i == ib) {
                     double a1 = (i == ia) ? lo : xl;
@@ -75,6 +91,36 @@ __global__ void __launch_bounds__(256) k_energy_integrate(const Member* __restri
     }
     if (A.atten) total *= A.atten[j];
     A.out[(size_t)reg * N_in * N_P + q] = total;
+}
+
+// The energy integral with the periodic cubic spline (the drop-in; a signal
+// with no member table): one block per phase column k, thread 0 solving the
+// column's spline into LDS, then one thread per input bin (the integral of
+// the spline over the bin, upper edge clipped at the highest energy).
+__global__ void __launch_bounds__(256) k_energy_integrate_cs(EnergyArgs A)
+{
+    extern __shared__ double wE[];
+    const int k = blockIdx.x, N_E = A.N_E, N_P = A.N_P, N_in = A.N_edges - 1;
+    double* Y = wE;
+    double* B = Y + N_E;
+    double* C = B + N_E;
+    double* D = C + N_E;
+    double* Z = D + N_E;
+    for (int e = threadIdx.x; e < N_E; e += blockDim.x)
+        Y[e] = pow(10.0, A.logE[e]) * A.flux[(size_t)e * N_P + k] * log(10.0);
+    __syncthreads();
+    if (threadIdx.x == 0) cspline_periodic(A.logE, Y, N_E, B, C, D, Z);
+    __syncthreads();
+    const double maxE = A.logE[N_E - 1];
+    for (int j = threadIdx.x; j < N_in; j += blockDim.x) {
+        double v = 0.0;
+        if (!(j > 0 && A.logEdges[j] > maxE)) {
+            const double lo = A.logEdges[j], hi = A.logEdges[j + 1] > maxE ? maxE : A.logEdges[j + 1];
+            v = phase_integ(2, A.logE, Y, N_E, B, C, D, lo, hi);
+        }
+        if (A.atten) v *= A.atten[j];
+        A.out[(size_t)j * N_P + k] = v;
+    }
 }
 
 // folded[region][c][k] = sum_j M[c][j] * in[region][j][k] on the matrix
@@ -233,7 +279,7 @@ struct MargFn {
 // component's phase shift (wrapped; a full cycle taken whole), negative bin
 // integrals dropped unless allowed, the total clipped at zero
 // (xpsi/tools/compute_expected_counts.pyx:66-197).
-__device__ double bin_rate(const MarginalArgs& A, int rq, int ch, int j)
+__device__ double bin_rate(const MarginalArgs& A, int rq, int ch, int j, const double* tabs)
 {
     // the request's signals: components one after another, each [N_ch][its phases]
     int P_tot = A.n_comp * A.N_P;
@@ -254,13 +300,18 @@ __device__ double bin_rate(const MarginalArgs& A, int rq, int ch, int j)
             double pa = A.phases[j] + shift, pb = A.phases[j + 1] + shift;
             if (eq12(pb - pa, 1.0)) { pa = 0.0; pb = 1.0; }
             else { pa -= floor(pa); pb -= floor(pb); }
+            // the periodic cubic spline's tables of this component (kind 2)
+            const double* TB = tabs ? tabs + 3 * off : nullptr;
+            const double* TC = tabs ? TB + np : nullptr;
+            const double* TD = tabs ? TC + np : nullptr;
+            const int kind = A.interp;
             if (pa < pb) {
-                double v = akima_integ(cph, pulse, np, true, pa, pb);
+                double v = phase_integ(kind, cph, pulse, np, TB, TC, TD, pa, pb);
                 if (v > 0.0 || neg_ok) st += v;
             } else {
-                double v = akima_integ(cph, pulse, np, true, pa, 1.0);
+                double v = phase_integ(kind, cph, pulse, np, TB, TC, TD, pa, 1.0);
                 if (v > 0.0 || neg_ok) st += v;
-                v = akima_integ(cph, pulse, np, true, 0.0, pb);
+                v = phase_integ(kind, cph, pulse, np, TB, TC, TD, 0.0, pb);
                 if (v > 0.0 || neg_ok) st += v;
             }
         } else if (j == 0) {
@@ -269,6 +320,35 @@ __device__ double bin_rate(const MarginalArgs& A, int rq, int ch, int j)
         off += np;
     }
     return st < 0.0 ? 0.0 : st;
+}
+
+// The periodic cubic spline needs each component's whole row solved first:
+// lane 0 writes the (b, c, d) tables of request rq's channel ch, component
+// after component, into tabs (3 x the components' phase counts, then one
+// component's count of work).  Returns tabs, or nullptr for the local
+// interpolants.  The caller synchronises the block (one wavefront).
+__device__ const double* channel_tables(const MarginalArgs& A, int rq, int ch, double* tabs)
+{
+    if (A.interp != 2) return nullptr;
+    if (threadIdx.x == 0) {
+        int P_tot = A.n_comp * A.N_P;
+        if (A.comp_np) {
+            P_tot = 0;
+            for (int c = 0; c < A.n_comp; c++) P_tot += A.comp_np[c];
+        }
+        const double* sig = A.signals + (size_t)rq * A.N_ch * P_tot;
+        double* z = tabs + 3 * P_tot;
+        int off = 0;
+        for (int c = 0; c < A.n_comp; c++) {
+            const int np = A.comp_np ? A.comp_np[c] : A.N_P;
+            const double* pulse = sig + (size_t)A.N_ch * off + (size_t)ch * np;
+            const double* cph = A.comp_phases + (A.comp_np ? off : 0);
+            double* TB = tabs + 3 * off;
+            if (np >= 4) cspline_periodic(cph, pulse, np, TB, TB + np, TB + 2 * np, z);
+            off += np;
+        }
+    }
+    return tabs;
 }
 
 // wave per (channel, request)
@@ -281,9 +361,11 @@ __global__ void __launch_bounds__(64) k_marginal(MarginalArgs A)
     double* data = star + N_b;      // N_b
     const double n = (double)N_b, T = A.exposure, SCALE = T / n;
     const double* counts = A.counts + (size_t)ch * N_b;
+    const double* tabs = channel_tables(A, rq, ch, data + N_b);
+    __syncthreads();
     // expected count rate from each region, phase-binned (shifted, wrapped)
     for (int j = lane; j < N_b; j += 64) {
-        double st = bin_rate(A, rq, ch, j);
+        double st = bin_rate(A, rq, ch, j, tabs);
         if (A.background) st += A.background[(size_t)ch * N_b + j];
         st *= n;
         star[j] = st;
@@ -481,9 +563,11 @@ __global__ void __launch_bounds__(64) k_poisson_bg(MarginalArgs A)
     double* tm = ex + N_b;          // N_b terms d log(ex) - ex
     const double* counts = A.counts + (size_t)ch * N_b;
     const size_t ro = (size_t)rq * A.N_ch + ch;
+    const double* tabs = channel_tables(A, rq, ch, tm + N_b);
+    __syncthreads();
     int bad = 0;
     for (int j = lane; j < N_b; j += 64) {
-        const double e = (bin_rate(A, rq, ch, j) + A.background[(size_t)ch * N_b + j]) * A.exposure;
+        const double e = (bin_rate(A, rq, ch, j, tabs) + A.background[(size_t)ch * N_b + j]) * A.exposure;
         const double d = counts[j];
         ex[j] = e;
         if (A.STAR) A.STAR[ro * N_b + j] = e;
@@ -587,7 +671,10 @@ __device__ double poisson_draw(double mu, SplitMix& r)
 __global__ void __launch_bounds__(64) k_star_rate(MarginalArgs A, double* __restrict__ out)
 {
     const int ch = blockIdx.x;
-    for (int j = threadIdx.x; j < A.N_b; j += 64) out[(size_t)ch * A.N_b + j] = bin_rate(A, 0, ch, j);
+    extern __shared__ double sm[];
+    const double* tabs = channel_tables(A, 0, ch, sm);
+    __syncthreads();
+    for (int j = threadIdx.x; j < A.N_b; j += 64) out[(size_t)ch * A.N_b + j] = bin_rate(A, 0, ch, j, tabs);
 }
 
 // ordered totals (the reference's sequential loops): scal[0] background sum,

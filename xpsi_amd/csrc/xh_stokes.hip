@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) k_stokes_fold(Member* __restrict__ member
     double* Y = PH + N_L;         // [3][N_L] profiles I, Q, U
     double* Bc = Y + 3 * N_L;     // [3][N_L] Akima b, c, d
     double* Cc = Bc + 3 * N_L;
-    double* Dc = Cc + 3 * N_L;
+    double* Dc = Cc + 3 * N_L;     // then [3][N_L] of work for the cubic spline
     __shared__ int J_s;
     const size_t row = (size_t)i * S;
     if (tid == 0) {
@@ -77,13 +77,21 @@ __global__ void __launch_bounds__(256) k_stokes_fold(Member* __restrict__ member
         Y[2 * N_L + q] = vU;
     }
     __syncthreads();
-    for (int w = tid; w < nst * NI; w += nt) {
-        const int s = w / NI, q = w % NI;
-        double b, c, d;
-        akima_coefs(PH, Y + s * N_L, N_L, true, q, b, c, d);
-        Bc[s * N_L + q] = b;
-        Cc[s * N_L + q] = c;
-        Dc[s * N_L + q] = d;
+    // the profiles' phase interpolant (periodic Akima, Steffen, or the
+    // periodic cubic spline, whose system one thread per profile solves)
+    if (F.interp == 2) {
+        double* Zw = Dc + 3 * N_L;   // [3][N_L] work
+        if (tid < nst) cspline_periodic(PH, Y + tid * N_L, N_L, Bc + tid * N_L, Cc + tid * N_L, Dc + tid * N_L,
+                                        Zw + tid * N_L);
+    } else {
+        for (int w = tid; w < nst * NI; w += nt) {
+            const int s = w / NI, q = w % NI;
+            double b, c, d;
+            phase_coefs(F.interp, PH, Y + s * N_L, N_L, q, nullptr, nullptr, nullptr, b, c, d);
+            Bc[s * N_L + q] = b;
+            Cc[s * N_L + q] = c;
+            Dc[s * N_L + q] = d;
+        }
     }
     __syncthreads();
     if (k >= N_P) return;
