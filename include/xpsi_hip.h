@@ -378,6 +378,9 @@ typedef struct xh_region_config {
     int cede;                 /* ceding ring present */
     int is_antiphased;
     int sqrt_num_cells, min_sqrt_num_cells, max_sqrt_num_cells;
+    int omit;                 /* a circular region cut out of the superseding one
+                                 (HotRegion omit=True; ignored with a ceding ring,
+                                 as the reference ignores it) */
 } xh_region_config;
 
 typedef struct xh_model_config {
@@ -437,6 +440,8 @@ typedef struct xh_request {
     double region[2][8];
     int sqrt_num_cells[2];    /* 0 = the configured value */
     double elsewhere_temperature; /* log10 K, used when the model has elsewhere */
+    double omit[2][3];        /* omit_colatitude, omit_radius, omit_azimuth (radians)
+                                 of regions configured with omit */
 } xh_request;
 
 typedef struct xh_pipeline xh_pipeline;

@@ -223,3 +223,26 @@ def test_integrate_dropin_num4d():
         err = relerr(flux, ref)
         say("integrate drop-in Num4D member %d rel err %.3e" % (k, err))
         assert err < SIG_RTOL
+
+
+def test_pipeline_omitted_region():
+    # HotRegion omit=True (xpsi/HotRegion.py:30-48, mesh_tools.pyx:892-910): a
+    # circular region cut out of the superseding spot, the cells' azimuths
+    # shifted by -omit_azimuth; checked against the oracle, and the hole must
+    # take flux away
+    from xpsi_amd import configs
+    m = configs.config1()
+    hr = dict(m["hot_regions"][0], omit=True, omit_colatitude=0.85, omit_radius=0.15, omit_azimuth=0.1)
+    model = dict(m, hot_regions=[hr])
+    _region_parity(model, "config1 with an omitted region")
+    from xpsi_amd.pipeline import LikelihoodPipeline, params_from_model
+    pl = LikelihoodPipeline(m, max_batch=1)
+    pl.evaluate([params_from_model(m)])
+    full = pl.fetch("folded", 0, 0, pl.N_ch * pl.N_P)[:pl.N_ch * pl.N_P]
+    pl.close()
+    pl = LikelihoodPipeline(model, max_batch=1)
+    pl.evaluate([params_from_model(model)])
+    cut = pl.fetch("folded", 0, 0, pl.N_ch * pl.N_P)[:pl.N_ch * pl.N_P]
+    pl.close()
+    say("omitted region: folded signal %.4f of the full spot's" % (cut.sum() / full.sum()))
+    assert 0.5 < cut.sum() / full.sum() < 0.95

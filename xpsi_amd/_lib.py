@@ -16,7 +16,7 @@ d, i32, sz, vp = C.c_double, C.c_int, C.c_size_t, C.c_void_p
 
 class RegionConfig(C.Structure):
     _fields_ = [("cede", i32), ("is_antiphased", i32), ("sqrt_num_cells", i32),
-                ("min_sqrt_num_cells", i32), ("max_sqrt_num_cells", i32)]
+                ("min_sqrt_num_cells", i32), ("max_sqrt_num_cells", i32), ("omit", i32)]
 
 
 class ModelConfig(C.Structure):
@@ -41,7 +41,7 @@ class ModelConfig(C.Structure):
 class Request(C.Structure):
     _fields_ = [("mass", d), ("radius", d), ("distance", d), ("cos_inclination", d),
                 ("region", (d * 8) * 2), ("sqrt_num_cells", i32 * 2),
-                ("elsewhere_temperature", d)]
+                ("elsewhere_temperature", d), ("omit", (d * 3) * 2)]
 
 
 class XHError(RuntimeError):

@@ -1569,6 +1569,11 @@ static void describe(const xh_pipeline* p, int rq, const xh_request& r)
         const double shift = q[0], sc = q[1], sr = q[2], sT = q[3];
         const bool cede = c.region[g].cede != 0;
         const double cc = cede ? q[4] : sc, cr = cede ? q[5] : 0.0, ca = cede ? q[6] : 0.0, cT = q[7];
+        // an omitted region cuts the superseding member; with a ceding ring
+        // the reference ignores the omission (HotRegion.py:44)
+        const bool omit = c.region[g].omit != 0 && !cede;
+        const double oc = omit ? r.omit[g][0] : sc, orad = omit ? r.omit[g][1] : 0.0,
+                     oa = omit ? r.omit[g][2] : 0.0;
         const int gi = rq * p->nreg + g;
         const int m0 = rq * p->mpr + p->moff[g], m1 = m0 + 1;
         const int nmem = p->moff[g + 1] - p->moff[g];
@@ -1580,8 +1585,8 @@ static void describe(const xh_pipeline* p, int rq, const xh_request& r)
             m.antiphased = c.region[g].is_antiphased;
             if (s == 0) {
                 m.active = 1;
-                m.cedeR = sr; m.cedeColat = sc; m.superR = 0.0; m.superColat = sc; m.superAzi = 0.0;
-                m.phi_offset = -0.0;
+                m.cedeR = sr; m.cedeColat = sc; m.superR = orad; m.superColat = oc; m.superAzi = oa;
+                m.phi_offset = -oa;
                 m.polar = (sc - sr < 0.0) || (sc + sr > M_PI);
                 m.logT = sT;
             } else {
@@ -1596,7 +1601,7 @@ static void describe(const xh_pipeline* p, int rq, const xh_request& r)
         RegionAlloc& A = p->h_regions[gi];
         int snc = r.sqrt_num_cells[g] > 0 ? r.sqrt_num_cells[g] : c.region[g].sqrt_num_cells;
         A.superR = sr; A.cedeR = cr; A.superColat = sc; A.cedeColat = cc; A.superAzi = -ca;
-        A.holeColat = sc; A.holeR = 0.0; A.holeAzi = 0.0;
+        A.holeColat = oc; A.holeR = orad; A.holeAzi = oa;
         A.R_eq = R; A.Omega = Omega; A.zeta = zeta; A.epsilon = eps; A.shape = c.star_shape;
         A.num_cells = snc * snc; A.min_sqrt = c.region[g].min_sqrt_num_cells;
         A.max_sqrt = c.region[g].max_sqrt_num_cells;

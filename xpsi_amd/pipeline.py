@@ -11,7 +11,7 @@ flow).
 Model description (dict):
   spacetime   frequency, star_shape ('AGM_14' | 'sphere')
   hot_regions list of dicts: settings sqrt_num_cells, min/max_sqrt_num_cells,
-              num_rays, num_leaves, num_phases, is_antiphased, cede,
+              num_rays, num_leaves, num_phases, is_antiphased, cede, omit,
               image_order_limit, atm_ext, plus default parameter values
   instrument  matrix [N_ch, N_in], energy_edges [N_in + 1] (keV)
   data        counts [N_ch, N_b], phases [N_b + 1], exposure_time
@@ -26,7 +26,9 @@ Model description (dict):
 A parameter vector is a dict: mass, radius, distance, cos_inclination and
 `hot_regions`: list of dicts phase_shift, super_colatitude, super_radius,
 super_temperature, cede_colatitude, cede_radius, cede_azimuth,
-cede_temperature; optional per-region sqrt_num_cells; elsewhere_temperature
+cede_temperature, and omit_colatitude, omit_radius, omit_azimuth for a region
+configured with omit (HotRegion omit=True: a circular region cut out of the
+superseding one); optional per-region sqrt_num_cells; elsewhere_temperature
 when the model has an elsewhere component.
 """
 import ctypes as C
@@ -38,6 +40,7 @@ from ._lib import lib, check, ModelConfig, RegionConfig, Request, XHError
 
 _REGION_KEYS = ("phase_shift", "super_colatitude", "super_radius", "super_temperature",
                 "cede_colatitude", "cede_radius", "cede_azimuth", "cede_temperature")
+_OMIT_KEYS = ("omit_colatitude", "omit_radius", "omit_azimuth")
 
 
 def fill_request(r, p):
@@ -51,6 +54,9 @@ def fill_request(r, p):
                 v = hp["super_colatitude"] if key == "cede_colatitude" else 0.0
             r.region[g][q] = float(v)
         r.sqrt_num_cells[g] = int(hp.get("sqrt_num_cells", 0))
+        r.omit[g][0] = float(hp.get("omit_colatitude", hp["super_colatitude"]))
+        r.omit[g][1] = float(hp.get("omit_radius", 0.0))
+        r.omit[g][2] = float(hp.get("omit_azimuth", 0.0))
     r.elsewhere_temperature = float(p.get("elsewhere_temperature", 0.0))
 
 
@@ -104,7 +110,8 @@ def model_config(model, max_batch=1, max_sqrt_cells=0):
                                      int(bool(h.get("is_antiphased", False))),
                                      int(h.get("sqrt_num_cells", 32)),
                                      int(h.get("min_sqrt_num_cells", 10)),
-                                     int(h.get("max_sqrt_num_cells", 80)))
+                                     int(h.get("max_sqrt_num_cells", 80)),
+                                     int(bool(h.get("omit", False))))
     cfg.num_rays = int(h0.get("num_rays", 200))
     cfg.num_leaves = int(h0.get("num_leaves", 64))
     cfg.num_phases = int(h0.get("num_phases") or cfg.num_leaves)
@@ -264,7 +271,7 @@ def params_from_model(model):
     sp = model["spacetime"]
     return dict(mass=sp["mass"], radius=sp["radius"], distance=sp["distance"],
                 cos_inclination=sp["cos_inclination"],
-                hot_regions=[{k: h.get(k) for k in _REGION_KEYS + ("sqrt_num_cells",)
+                hot_regions=[{k: h.get(k) for k in _REGION_KEYS + _OMIT_KEYS + ("sqrt_num_cells",)
                               if h.get(k) is not None} for h in model["hot_regions"]],
                 **({"elsewhere_temperature": model["elsewhere"]["elsewhere_temperature"]}
                    if model.get("elsewhere") is not None else {}))
