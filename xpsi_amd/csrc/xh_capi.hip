@@ -773,7 +773,8 @@ extern "C" int xh_compute_rays(int N_u, double r_s, const double* r_s_over_R, do
     HIPCHK(hipGetLastError());
     HIPCHK(fill_int(d_count, 2, 0, s));
     Layout L{N_u, N_R, 0, 0, 0, 8, 0, 1};
-    hipLaunchKernelGGL(k_rays_lane, dim3((N_R - 1 + 63) / 64, N_u, 1), dim3(64), 0, s, (const Member*)d_m, L,
+    hipLaunchKernelGGL(k_rays_lane, dim3((unsigned)(((size_t)N_u * (N_R - 1) + 63) / 64), 1, 1), dim3(64), 0, s,
+                       (const Member*)d_m, L,
                        (const double*)d_rr, (const double*)maxAlpha, cos_alpha, deflection, lag, d_work, d_count,
                        r_s_over_R);
     HIPCHK(hipGetLastError());
@@ -1710,7 +1711,8 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     if (p->timing) HIPCHK(hipEventRecord(p->ev[1], s));
     // rays: lane-per-ray first pass, then the listed rays with full QAG
     HIPCHK(fill_int(p->d_raycount, 2, 0, s));
-    hipLaunchKernelGGL(k_rays_lane, dim3((L.N_R - 1 + 63) / 64, S, nm), dim3(64), 0, s, p->d_members, L, p->d_r,
+    hipLaunchKernelGGL(k_rays_lane, dim3((unsigned)(((size_t)S * (L.N_R - 1) + 63) / 64), 1, nm), dim3(64), 0, s,
+                       p->d_members, L, p->d_r,
                        p->d_maxAlpha, p->d_cosa, p->d_defl, p->d_lag, p->d_raywork, p->d_raycount,
                        (const double*)nullptr);
     HIPCHK(hipGetLastError());
@@ -1721,7 +1723,8 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
                        p->d_defl, p->d_lag, p->d_maxDefl, 1, (const double*)nullptr);
     HIPCHK(hipGetLastError());
     if (p->has_else) {
-        hipLaunchKernelGGL(k_rays_lane, dim3((Le.N_R - 1 + 63) / 64, Le.S_max, n), dim3(64), 0, s, p->d_else, Le,
+        hipLaunchKernelGGL(k_rays_lane, dim3((unsigned)(((size_t)Le.S_max * (Le.N_R - 1) + 63) / 64), 1, n), dim3(64), 0,
+                           s, p->d_else, Le,
                            p->d_else_r, p->d_else_maxA, p->d_else_cosa, p->d_else_defl, p->d_else_lag,
                            p->d_else_raywork, p->d_raycount + 1, (const double*)nullptr);
         HIPCHK(hipGetLastError());

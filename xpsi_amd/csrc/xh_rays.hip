@@ -329,10 +329,13 @@ __global__ void __launch_bounds__(64) k_rays_lane(const Member* __restrict__ mem
                                                   int* __restrict__ work, int* __restrict__ nwork,
                                                   const double* __restrict__ uu)
 {
-    const int mi = blockIdx.z, i = blockIdx.y;
-    const int j = blockIdx.x * 64 + threadIdx.x + 1;
+    // the member's (parallel, ray) pairs packed into full wavefronts: lane q
+    // takes ray j = q mod (N_R - 1) + 1 of parallel i = q / (N_R - 1)
+    const int mi = blockIdx.z;
+    const int q = blockIdx.x * 64 + threadIdx.x, nr = L.N_R - 1;
+    const int i = q / nr, j = q - i * nr + 1;
     const Member m = members[mi];
-    if (!m.active || m.status || i >= m.S || j >= L.N_R) return;
+    if (!m.active || m.status || i >= m.S) return;
     const size_t ro = (size_t)mi * L.S_max + i;
     const double u = uu ? uu[ro] : m.r_s / rr[ro];
     const double maxA = clamped_max_alpha(maxAlpha_in[ro], u);
