@@ -1739,7 +1739,7 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     RingArgs RA{p->d_theta, p->d_r, p->d_u, p->d_cos_gamma, p->d_maxDefl, p->d_defl, p->d_cosa, p->d_lag,
                 p->d_leaves, p->d_ring_ok, p->d_geo, p->d_valid, p->d_lastq};
     if (p->timing) HIPCHK(hipEventRecord(p->ev[2], s));
-    hipLaunchKernelGGL(k_geometry<0>, dim3(S, 1, nm), dim3(128), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
+    hipLaunchKernelGGL(k_geometry<0>, dim3(S, 1, nm), dim3(64), geo_lds(L.N_R, L.N_L), s, p->d_members, L, RA);
     HIPCHK(hipGetLastError());
     if (p->has_else) {
         hipLaunchKernelGGL(k_else_integrate, dim3(Le.S_max, n), dim3(256), else_lds(Le), s, p->d_else, Le,
