@@ -66,6 +66,16 @@ __device__ __forceinline__ double phi_of(double th, double TH, double psi)
     return acos(cp);
 }
 
+// phi_of with the sines and cosines already formed (cos psi, cos TH, sin TH
+// and those of th): the same operations on the same values
+__device__ __forceinline__ double phi_of_cs(double cth, double sth, double cTH, double sTH, double cpsi)
+{
+    double cp = cpsi - cTH * cth;
+    cp /= sTH * sth;
+    if (!(-1.0 <= cp && cp <= 1.0)) return -1.0;
+    return acos(cp);
+}
+
 __device__ __forceinline__ double span_in(double a, double b, double LB, double UB)
 {
     int ac = (LB <= a && a <= UB), bc = (LB <= b && b <= UB);
@@ -79,19 +89,27 @@ __device__ __forceinline__ double span_in(double a, double b, double LB, double 
 }
 
 // azimuthal extent of (ceding region minus superseding region) at colatitude
-// th, clipped to the cell [pa, pb] when `cell` is set
+// th, clipped to the cell [pa, pb] when `cell` is set.  prep() forms the
+// circles' sines and cosines once per integral instead of at every node.
 struct RegionFn {
     double eps, zeta, colat, cedeR, superR, superAzi, superColat, pa, pb, R_eq, Omega;
     int shape, lorentz, cell;
+    double cCo = 0.0, sCo = 0.0, cCR = 0.0, cSu = 0.0, sSu = 0.0, cSR = 0.0;
+    __device__ void prep()
+    {
+        cCo = cos(colat); sCo = sin(colat); cCR = cos(cedeR);
+        cSu = cos(superColat); sSu = sin(superColat); cSR = cos(superR);
+    }
     __device__ double span(double L, double U) const { return cell ? span_in(pa, pb, L, U) : U - L; }
     __device__ double operator()(double th) const
     {
         if (eq12(th, 0.0)) return 0.0;
         double mu = cos(th);
+        const double sth = sin(th);
         double r = radius_norm(mu, eps, zeta, shape);
         double f = f_theta(mu, r, eps, zeta, shape);
         double aLB, aUB, cLB, cUB, dphi = 0.0;
-        aUB = phi_of(th, colat, cedeR);
+        aUB = phi_of_cs(mu, sth, cCo, sCo, cCR);
         if (eq12(aUB, -1.0)) {
             if (th + colat < cedeR) { aUB = PI; aLB = -PI; }
             else return 0.0;
@@ -99,7 +117,7 @@ struct RegionFn {
         if (fabs(th - superColat) > superR) {
             cLB = cUB = 0.0;
         } else {
-            cUB = phi_of(th, superColat, superR);
+            cUB = phi_of_cs(mu, sth, cSu, sSu, cSR);
             if (eq12(cUB, -1.0)) return 0.0;
             cLB = -cUB;
             cUB += superAzi;
@@ -123,10 +141,10 @@ struct RegionFn {
         double w = 1.0;
         if (!cell && lorentz) {
             double rsr = 2.0 * zeta / r;
-            double beta = R_eq * r * Omega * sin(th) / (C_SI * sqrt(1.0 - rsr));
+            double beta = R_eq * r * Omega * sth / (C_SI * sqrt(1.0 - rsr));
             w = 1.0 / sqrt(1.0 - beta * beta);
         }
-        return w * dphi * r * r * sqrt(1.0 + f * f) * sin(th);
+        return w * dphi * r * r * sqrt(1.0 + f * f) * sth;
     }
 };
 
@@ -144,6 +162,7 @@ __device__ double wave_spot(double ta, double tb, double R_eq, double eps, doubl
 {
     RegionFn f{eps, zeta, colat, cedeR, superR, superAzi, superColat, 0.0, 0.0, R_eq, Omega,
                shape, 0, 0};
+    f.prep();
     double r, e;
     wave_qag61(f, ta, tb, 0.0, MESH_EPSREL, QAG_LIMIT, r, e);
     return R_eq * R_eq * r;
@@ -251,6 +270,7 @@ __device__ __noinline__ double wave_cell(double ta, double tb, double pa, double
                             double superAzi, double superColat)
 {
     RegionFn f{eps, zeta, colat, cedeR, superR, superAzi, superColat, pa, pb, R_eq, 0.0, shape, 0, 1};
+    f.prep();
     // candidate breakpoints, sorted
     double bp[16];
     int nbp = 0;
