@@ -273,8 +273,29 @@ __device__ __forceinline__ bool lane_ray_pair(const FB& fb, double a, double b, 
            first_pass_stands(r[1], e[1], ra[1], lo[1], hi[1], 1.0e-12);
 }
 
+// Both integrands of one ray, outgoing (rays.pyx:61-70) or ingoing (:72-88)
+// by a lane's own flag: the same operations as OutBoth and InBoth, so lanes of
+// either kind share one quadrature sweep
+struct AnyBoth {
+    double sas, k;
+    bool out;
+    __device__ void both(double x, double& d, double& l) const
+    {
+        const double w = 1.0 - x * x;
+        const double qi = 2.0 - x * x - w * w * k;
+        const double q = out ? 1.0 - sas + x * x * sas * qi : qi;
+        const double ri = rsqrt_nr(q);
+        const double xr = x * ri;
+        d = out ? xr : ri;
+        l = (out ? xr : ri * ri) * rcp_nr(out ? 1.0 + q * ri : x + ri);
+    }
+};
+
 // wave_ray's branches with every quadrature settled by its first pass;
-// returns false when some quadrature needs QAG's bisection
+// returns false when some quadrature needs QAG's bisection.  The three
+// branches are one sweep with per-lane integrand and limits (a second sweep
+// for the rays beyond 90 degrees), so a wavefront whose rays straddle a
+// branch boundary does not run each branch's sweep in turn.
 __device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, double& lag)
 {
     const double b_ph = 3.0 * sqrt(3.0) / 2.0;
@@ -282,32 +303,39 @@ __device__ bool lane_ray(double cos_alpha, double r_s, double u, double& defl, d
     double sa = sqrt(sas);
     double alpha = acos(cos_alpha);
     double b = sa / (u * sqrt(1.0 - u));
+    // branch 0: b <= b_ph; 1: alpha <= pi/2; 2: beyond
+    const int br = b <= b_ph ? 0 : (alpha <= PI / 2.0 ? 1 : 2);
+    double rc = 1.0 / u, wR = 0.0;
+    if (br == 1 && eq12(sa, 1.0)) {
+        rc = 1.0 / u;
+        wR = 0.0;
+    } else if (br != 0) {
+        rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
+        wR = sqrt(1.0 - rc * u);
+    }
+    if (isnan(wR)) wR = 0.0;
+    const AnyBoth fb{sas, br == 0 ? 1.0 / (1.0 / u - 1.0) : 1.0 / (rc - 1.0), br == 0};
+    const int npass = br == 2 ? 2 : 1;
+    double D1 = 0.0, L1 = 0.0, D0 = 0.0, L0 = 0.0;
+    bool ok = true;
+#pragma unroll 1
+    for (int ps = 0; ps < npass; ps++) {
+        const double a = (br == 1 || ps == 1) ? wR : 0.0;
+        double dd, ll;
+        const bool st = lane_ray_pair(fb, a, 1.0, dd, ll);
+        ok = ok && st;
+        if (ps == 0) { D1 = dd; L1 = ll; }
+        else { D0 = dd; L0 = ll; }
+    }
+    if (!ok) return false;
     double d0, l0;
-    if (b <= b_ph) {
-        OutBoth fb{sas, 1.0 / (1.0 / u - 1.0)};
-        if (!lane_ray_pair(fb, 0.0, 1.0, d0, l0)) return false;
-        d0 *= 2.0 * b * u;
-        l0 *= 2.0 * b * b * u * r_s;
-    } else if (alpha <= PI / 2.0) {
-        double rc, wR;
-        if (eq12(sa, 1.0)) { rc = 1.0 / u; wR = 0.0; }
-        else {
-            rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
-            wR = sqrt(1.0 - rc * u);
-        }
-        if (isnan(wR)) wR = 0.0;
-        InBoth fb{1.0 / (rc - 1.0)};
-        if (!lane_ray_pair(fb, wR, 1.0, d0, l0)) return false;
-        d0 *= 2.0 * b / rc;
-        l0 *= 2.0 * b * b * r_s / rc;
+    if (br == 0) {
+        d0 = D1 * (2.0 * b * u);
+        l0 = L1 * (2.0 * b * b * u * r_s);
+    } else if (br == 1) {
+        d0 = D1 * (2.0 * b / rc);
+        l0 = L1 * (2.0 * b * b * r_s / rc);
     } else {
-        double rc = 2.0 * b * cos((atan(sqrt(4.0 * b * b / 27.0 - 1.0)) - PI) / 3.0) / sqrt(3.0);
-        double wR = sqrt(1.0 - rc * u);
-        if (isnan(wR)) wR = 0.0;
-        InBoth fb{1.0 / (rc - 1.0)};
-        double D1, D0, L1, L0;
-        if (!lane_ray_pair(fb, 0.0, 1.0, D1, L1)) return false;
-        if (!lane_ray_pair(fb, wR, 1.0, D0, L0)) return false;
         d0 = 2.0 * b * (2.0 * D1 - D0) / rc;
         l0 = 2.0 * b * b * r_s * (2.0 * L1 - L0) / rc;
         double cor = 1.0 / u - rc + log((1.0 / u - 1.0) / (rc - 1.0));
