@@ -189,24 +189,46 @@ __global__ void __launch_bounds__(64, 4) k_wave_fold(Member* members, Layout L, 
         // the chain's sums, scaled in FP64 (a_ref, the blackbody norm,
         // 1 / (E keV)), into the flux; the chain restarts from zero
         auto flush = [&]() {
-            // the lane index re-read here, so that the flush's addresses are
-            // formed at each flush and not held in registers across the sweep
-            int ln = lane;
+            // the lane index and the flux row length re-read here, so that the
+            // flush's addresses are formed at each flush and not held in
+            // registers across the sweep
+            int ln = lane, np = N_P;
             asm volatile("" : "+v"(ln));
+            asm volatile("" : "+s"(np));
+            if (e0 + 64 <= N_E && p0 + 64 <= np) {
+                // a full tile: each lane's 64 targets sit at one byte offset
+                // from 64 wave-uniform rows, with no bounds to test
+                const unsigned boff = (unsigned)((4 * (ln >> 4) * np + (ln & 15)) * 8);
+                double* tile = flux + (size_t)e0 * np + p0;
 #pragma unroll
-            for (int c = 0; c < 4; c++)
+                for (int c = 0; c < 4; c++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int src = 16 * c + 4 * (ln >> 4) + r, ee = e0 + src;
-                    const double sc = a_ref * __shfl(esc, src, 64);
-                    if (ee >= N_E) continue;
+                    for (int r = 0; r < 4; r++) {
+                        const double sc = a_ref * __shfl(esc, 16 * c + 4 * (ln >> 4) + r, 64);
 #pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const int pq = p0 + 16 * t + (ln & 15);
-                        const float v = acc[c][t][r];
-                        if (pq < N_P && v != 0.f) unsafeAtomicAdd(flux + (size_t)ee * N_P + pq, (double)v * sc);
+                        for (int t = 0; t < 4; t++) {
+                            double* row = tile + (size_t)((16 * c + r) * np + 16 * t);
+                            const float v = acc[c][t][r];
+                            // most of a short chain's sums are zero: no atomic for those
+                            if (v != 0.f) unsafeAtomicAdd((double*)((char*)row + boff), (double)v * sc);
+                        }
                     }
-                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int src = 16 * c + 4 * (ln >> 4) + r, ee = e0 + src;
+                        const double sc = a_ref * __shfl(esc, src, 64);
+                        if (ee >= N_E) continue;
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            const int pq = p0 + 16 * t + (ln & 15);
+                            const float v = acc[c][t][r];
+                            if (pq < np && v != 0.f) unsafeAtomicAdd(flux + (size_t)ee * np + pq, (double)v * sc);
+                        }
+                    }
+            }
 #pragma unroll
             for (int c = 0; c < 4; c++)
 #pragma unroll
