@@ -527,9 +527,14 @@ __global__ void __launch_bounds__(256) k_ring_bounds(const Member* __restrict__ 
     double cell = (m.polar ? TWO_PI : 2.0 * bphi) * total[mi];
     cell /= m.numCell;
     const double eta = cell / dphi;
-    const double* at = area_tab + (size_t)mi * table_n;
-    __shared__ double ct[1024];
-    for (int k = tid; k < table_n; k += 256) ct[k] = lin(hi, lo, table_n, k);
+    // the member's area table and its colatitudes in LDS: the Steffen
+    // inversions' interval searches read them about ten times in a row
+    __shared__ double ct[1024], at[1024];
+    const double* atg = area_tab + (size_t)mi * table_n;
+    for (int k = tid; k < table_n; k += 256) {
+        ct[k] = lin(hi, lo, table_n, k);
+        at[k] = atg[k];
+    }
     __syncthreads();
     if (i >= S) return;
     const size_t ro = (size_t)mi * L.S_max;
@@ -799,10 +804,14 @@ __global__ void __launch_bounds__(64) k_closed_rings(const Member* __restrict__ 
     if (!m.active || m.status) return;
     const int S = m.S, ncc = S / 2;
     if (i >= ncc) return;
-    __shared__ double ct[1024];
-    for (int k = lane; k < table_n; k += 64) ct[k] = lin(HALF_PI, 0.0, table_n, k);
+    // the area table and its colatitudes in LDS for the interval searches
+    __shared__ double ct[1024], at[1024];
+    const double* atg = area_tab + (size_t)mi * table_n;
+    for (int k = lane; k < table_n; k += 64) {
+        ct[k] = lin(HALF_PI, 0.0, table_n, k);
+        at[k] = atg[k];
+    }
     __syncthreads();
-    const double* at = area_tab + (size_t)mi * table_n;
     double cell = TWO_PI * total[mi] / ((double)S * (double)S);
     cell *= 2.0;
     const double eta = cell * (double)S / TWO_PI;
