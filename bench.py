@@ -37,7 +37,7 @@ KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold", "waves": "k_w
 # per accumulation chain of 8 intervals); its reads include the reloads of its
 # register spills (240 bytes per lane of scratch at four waves per SIMD).
 FOLD_TRAFFIC_KIB = {
-    "waves/1": {256: (1653820, 31189116), 128: (817597, 13795827), 64: (521537, 7495181)},
+    "waves/1": {256: (1692562, 31260897), 128: (834653, 13825472), 64: (527687, 7511708)},
     "moment/2": {256: (17123810, 20011418), 128: (9762315, 9300707), 64: (6796957, 5185836)},
 }
 HBM_PEAK_GBS = 8000.0
