@@ -850,7 +850,7 @@ static hipError_t launch_mesh_cells(const Member* m, const double* tab, const do
     hipLaunchKernelGGL(k_mesh_classify, g, dim3(256), 0, s, m, tot, L, (const double*)ringlim, phi, cellArea, blist,
                        bcount);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const unsigned nb = (unsigned)std::max(1, std::min(2048, nm * S / 8));
+    const unsigned nb = (unsigned)std::max(1, std::min(3072, nm * S / 8));   // three waves per SIMD
     hipLaunchKernelGGL(k_mesh_cells, dim3(nb), dim3(256), 0, s, m, tot, L, (const double*)ringlim, cellArea,
                        (const int2*)blist, (const int*)bcount);
     return hipGetLastError();

@@ -692,7 +692,7 @@ __global__ void __launch_bounds__(256) k_mesh_classify(const Member* __restrict_
 }
 
 // wavefronts over the boundary-cell list (grid-stride): each cell's area
-__global__ void __launch_bounds__(256) k_mesh_cells(const Member* __restrict__ members, const double* __restrict__ total,
+__global__ void __launch_bounds__(256, 3) k_mesh_cells(const Member* __restrict__ members, const double* __restrict__ total,
                                                     Layout L, const double* __restrict__ ringlim,
                                                     double* __restrict__ cellArea, const int2* __restrict__ blist,
                                                     const int* __restrict__ bcount)
