@@ -264,15 +264,16 @@ __device__ double wave_pieces(const Fn& f, double lo, double hi, const double* b
     return total;
 }
 
-// mesh_tools.pyx:429-725 (wave-cooperative)
+// mesh_tools.pyx:429-725 (wave-cooperative).  bp: 16 doubles of LDS owned by
+// the calling wavefront (its lanes hold the same breakpoints; in LDS the
+// sort and the piece loop index them without scratch memory)
 __device__ __noinline__ double wave_cell(double ta, double tb, double pa, double pb, double R_eq, double eps,
                             double zeta, int shape, double colat, double cedeR, double superR,
-                            double superAzi, double superColat)
+                            double superAzi, double superColat, double* bp)
 {
     RegionFn f{eps, zeta, colat, cedeR, superR, superAzi, superColat, pa, pb, R_eq, 0.0, shape, 0, 1};
     f.prep();
     // candidate breakpoints, sorted
-    double bp[16];
     int nbp = 0;
     auto add = [&](double t) { if (t > ta && t < tb && nbp < 16) bp[nbp++] = t; };
     double t2[2];
@@ -700,6 +701,8 @@ __global__ void __launch_bounds__(256, 3) k_mesh_cells(const Member* __restrict_
     const int lane = threadIdx.x & 63;
     const int nw = gridDim.x * 4;
     const int n = *bcount;
+    __shared__ double bps[4][16];   // each wavefront's breakpoint list
+    double* bp = bps[threadIdx.x >> 6];
     for (int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < n; q += nw) {
         const int2 e = blist[q];
         const int mi = e.x, i = e.y >> 16, j = e.y & 0xffff;
@@ -712,7 +715,7 @@ __global__ void __launch_bounds__(256, 3) k_mesh_cells(const Member* __restrict_
         const double first_left = m.polar ? -PI : -bphi;
         const double left = left_edge(first_left, dphi, j), right = left + dphi;
         const double v = wave_cell(rl[0], rl[1], left, right, m.R_eq, m.epsilon, m.zeta, m.shape, colat, m.cedeR,
-                                   m.superR, m.superAzi, sColat);
+                                   m.superR, m.superAzi, sColat, bp);
         const bool mirror = eq12(sColat, colat) && eq12(m.superAzi, 0.0);
         if (lane == 0) {
             double* ca = cellArea + (size_t)mi * L.S_max * L.S_max + (size_t)i * S;
