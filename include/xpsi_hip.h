@@ -307,6 +307,45 @@ int xh_poisson_likelihood_given_background_ex(double exposure_time, int N_b, con
                                               double* lnl_out, double* EXPEC, void* workspace,
                                               size_t workspace_bytes, void* stream);
 
+/* Replaces xpsi/likelihoods/_gaussian_likelihood_given_background_IQU.pyx:14-91
+ * `gaussian_likelihood_given_background(exposure_time, phases, counts, errors,
+ * components, component_phases, phase_shifts, background, allow_negative) ->
+ * (LOGLIKE, EXPEC)`: EXPEC [N_ch, N_b] = (phase-binned star count rate +
+ * background count rate) x exposure_time, as the Poisson statistic forms it
+ * (compute_expected_counts.pyx:199-315), and LOGLIKE = -sum over channels and
+ * bins of (EXPEC - counts)^2 / (2 s^2) + ln(2 pi s^2) / 2 with s = errors
+ * [N_ch, N_b], summed in the reference's order.  Device inputs as for
+ * xh_poisson_likelihood_given_background; *lnl_out to host memory; EXPEC may
+ * be NULL.  workspace of xh_gaussian_workspace_bytes(N_ch, N_b). */
+size_t xh_gaussian_workspace_bytes(int N_ch, int N_b);
+int xh_gaussian_likelihood_given_background(double exposure_time, int N_b, const double* phases, int N_ch,
+                                            const double* counts, const double* errors, int n_comp,
+                                            const double* components, int N_P, const double* comp_phases,
+                                            const double* shifts, const double* background, int allow_negative,
+                                            double* lnl_out, double* EXPEC, void* workspace,
+                                            size_t workspace_bytes, void* stream);
+
+/* xh_gaussian_likelihood_given_background with components on their own phase
+ * grids and allow_negative per component, laid out as for
+ * xh_eval_marginal_likelihood_ex. */
+int xh_gaussian_likelihood_given_background_ex(double exposure_time, int N_b, const double* phases, int N_ch,
+                                               const double* counts, const double* errors, int n_comp,
+                                               const double* components, const int* comp_num_phases,
+                                               const double* comp_phases, const double* shifts,
+                                               const double* background, const int* allow_negative,
+                                               double* lnl_out, double* EXPEC, void* workspace,
+                                               size_t workspace_bytes, void* stream);
+
+/* Replaces xpsi/likelihoods/_gaussian_likelihood_QnUn.pyx:14-44
+ * `gaussian_likelihood_QnUn(phases, counts, errors, star) -> (LOGLIKE, star)`:
+ * counts, errors [N_rows, N_b] and one model row star [N_b] (device memory;
+ * the phase edges are not used, as in the reference); LOGLIKE = -sum over rows
+ * and bins of (star_j - counts_ij)^2 / (2 s_ij^2) + ln(2 pi s_ij^2) / 2 to
+ * host memory.  workspace of xh_gaussian_workspace_bytes(N_rows, N_b). */
+int xh_gaussian_likelihood_QnUn(int N_rows, int N_b, const double* counts, const double* errors,
+                                const double* star, double* lnl_out, void* workspace, size_t workspace_bytes,
+                                void* stream);
+
 /* Replace xpsi/tools/synthesise.pyx:40-148 `synthesise_exposure(...) ->
  * (EXPEC, SYNTHETIC, SCALE_BACKGROUND)` and :152-276
  * `synthesise_given_total_count_number(...) -> (EXPEC, SYNTHETIC, SCALE_STAR,

@@ -312,3 +312,58 @@ int or_poisson_likelihood_given_background(double T_exp, size_t N_b, const doubl
     *lnl = LL;
     return 0;
 }
+
+/* xpsi/likelihoods/_gaussian_likelihood_given_background_IQU.pyx:14-91 with
+   compute_expected_counts.pyx:199-315: EXPEC as for the Poisson statistic;
+   with s2 = errors^2 per channel and bin, lnL is the running
+   LOGLIKE -= (EXPEC - d)^2 / (2 s2) + log(2 pi s2) / 2, channel by channel,
+   bin by bin. */
+int or_gaussian_likelihood_given_background(double T_exp, size_t N_b, const double *phases,
+                                            size_t N_ch, const double *counts, const double *errors,
+                                            size_t n_comp, const double *const *comps,
+                                            const size_t *comp_N_P, const double *const *comp_phases,
+                                            const double *shifts, const double *background,
+                                            const unsigned char *allow_neg, double *lnl, double *EXPEC)
+{
+    size_t maxP = 0;
+    for (size_t c = 0; c < n_comp; c++) if (comp_N_P[c] > maxP) maxP = comp_N_P[c];
+    double *cb = (double *)malloc(3 * (maxP + 1) * sizeof(double));
+    double *mb = (double *)malloc((3 * maxP + 4) * sizeof(double));
+    memset(EXPEC, 0, N_ch * N_b * sizeof(double));
+    for (size_t i = 0; i < N_ch; i++) {
+        double *ex = EXPEC + i * N_b;
+        channel_rate(i, N_b, phases, n_comp, comps, comp_N_P, comp_phases, shifts, allow_neg, cb, mb, ex);
+        for (size_t j = 0; j < N_b; j++) {
+            ex[j] += background[i * N_b + j];
+            ex[j] *= T_exp;
+        }
+    }
+    double LL = 0.0;
+    for (size_t i = 0; i < N_ch; i++)
+        for (size_t j = 0; j < N_b; j++) {
+            const double s2 = pow(errors[i * N_b + j], 2.0);
+            const double norm = 0.5 * log(2.0 * M_PI * s2);
+            const double r = EXPEC[i * N_b + j] - counts[i * N_b + j];
+            LL -= r * r / (2.0 * s2) + norm;
+        }
+    free(cb); free(mb);
+    *lnl = LL;
+    return 0;
+}
+
+/* xpsi/likelihoods/_gaussian_likelihood_QnUn.pyx:14-44: counts and errors
+   [N_rows, N_b], star [N_b] (the same model row for every data row);
+   LOGLIKE -= (star_j - d_ij)^2 / (2 s2_ij) + log(2 pi s2_ij) / 2. */
+double or_gaussian_likelihood_QnUn(size_t N_rows, size_t N_b, const double *counts,
+                                   const double *errors, const double *star)
+{
+    double LL = 0.0;
+    for (size_t i = 0; i < N_rows; i++)
+        for (size_t j = 0; j < N_b; j++) {
+            const double s2 = pow(errors[i * N_b + j], 2.0);
+            const double norm = 0.5 * log(2.0 * M_PI * s2);
+            const double r = star[j] - counts[i * N_b + j];
+            LL -= r * r / (2.0 * s2) + norm;
+        }
+    return LL;
+}

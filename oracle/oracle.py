@@ -132,6 +132,12 @@ def lib():
         L.or_poisson_likelihood_given_background.argtypes = (
             [d, sz, _dp, sz, _dp, sz, C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp,
              C.c_void_p, C.c_void_p, C.POINTER(C.c_double), _dp])
+        L.or_gaussian_likelihood_given_background.restype = i
+        L.or_gaussian_likelihood_given_background.argtypes = (
+            [d, sz, _dp, sz, _dp, _dp, sz, C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp,
+             C.c_void_p, C.POINTER(C.c_double), _dp])
+        L.or_gaussian_likelihood_QnUn.restype = d
+        L.or_gaussian_likelihood_QnUn.argtypes = [sz, sz, _dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -485,6 +491,36 @@ def poisson_likelihood_given_background(exposure_time, phases, counts, component
         _c(phase_shifts), _c(background), None if pre is None else pre.ctypes.data,
         an.ctypes.data, C.byref(lnl), EXPEC)
     return lnl.value, EXPEC
+
+
+def gaussian_likelihood_given_background(exposure_time, phases, counts, errors, components,
+                                         component_phases, phase_shifts, background, allow_negative=False):
+    """_gaussian_likelihood_given_background_IQU.pyx:14-91: (lnL, EXPEC)."""
+    comps = [_c(cmp) for cmp in components]
+    cph = [_c(p) for p in component_phases]
+    nc = len(comps)
+    N_ch, N_b = counts.shape
+    cp = (C.c_void_p * nc)(*[cmp.ctypes.data for cmp in comps])
+    pp = (C.c_void_p * nc)(*[p.ctypes.data for p in cph])
+    nps = (C.c_size_t * nc)(*[cmp.shape[1] for cmp in comps])
+    if isinstance(allow_negative, bool):
+        an = np.full(nc, int(allow_negative), dtype=np.uint8)
+    else:
+        an = np.asarray(allow_negative, dtype=np.uint8)
+    EXPEC = np.zeros((N_ch, N_b))
+    lnl = C.c_double(0.0)
+    lib().or_gaussian_likelihood_given_background(
+        float(exposure_time), N_b, _c(phases), N_ch, _c(counts), _c(errors), nc, cp, nps, pp,
+        _c(phase_shifts), _c(background), an.ctypes.data, C.byref(lnl), EXPEC)
+    return lnl.value, EXPEC
+
+
+def gaussian_likelihood_QnUn(phases, counts, errors, star):
+    """_gaussian_likelihood_QnUn.pyx:14-44: (lnL, star)."""
+    counts = _c(counts)
+    N_rows, N_b = counts.shape
+    star = _c(star)
+    return float(lib().or_gaussian_likelihood_QnUn(N_rows, N_b, counts, _c(errors), star)), star
 
 
 # ---------------------------------------------------------- model pipeline

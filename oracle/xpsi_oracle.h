@@ -201,4 +201,21 @@ int or_poisson_likelihood_given_background(double exposure_time, size_t N_b,
                                            const double *precomp,
                                            const unsigned char *allow_negative,
                                            double *lnl, double *EXPEC);
+/* _gaussian_likelihood_given_background_IQU.pyx: lnL and EXPEC [N_ch, N_b];
+   errors [N_ch, N_b] per channel and bin */
+int or_gaussian_likelihood_given_background(double exposure_time, size_t N_b,
+                                            const double *phases, size_t N_ch,
+                                            const double *counts, const double *errors,
+                                            size_t n_comp,
+                                            const double *const *components,
+                                            const size_t *comp_N_P,
+                                            const double *const *component_phases,
+                                            const double *phase_shifts,
+                                            const double *background,
+                                            const unsigned char *allow_negative,
+                                            double *lnl, double *EXPEC);
+/* _gaussian_likelihood_QnUn.pyx: lnL of counts/errors [N_rows, N_b] against
+   one model row star [N_b] */
+double or_gaussian_likelihood_QnUn(size_t N_rows, size_t N_b, const double *counts,
+                                   const double *errors, const double *star);
 #endif

@@ -321,11 +321,10 @@ def test_marginal_likelihood_brute_force(bounded):
     assert abs(lnl0 - ref) > 1e-3
 
 
-def test_poisson_likelihood_given_background_brute_force():
-    # expected counts = exposure x (phase-bin integral of the shifted pulse +
-    # background rate); lnL = sum d log(lambda) - lambda - ln d!
-    from scipy import integrate as si
-    from scipy.special import gammaln
+def _two_component_case():
+    """Two pulses on one phase grid with their shifts, and each channel's
+    phase-bin integrals of the shifted Akima interpolants by Gauss-Legendre on
+    every cubic piece (exact) between the knots."""
     rng = np.random.default_rng(31)
     T, N_b, N_ch, NP = 2000.0, 10, 4, 40
     ph = np.linspace(0.0, 1.0, NP)
@@ -341,8 +340,6 @@ def test_poisson_likelihood_given_background_brute_force():
             f = lambda x: O.akima_eval(sp, x - math.floor(x))
             for j in range(N_b):
                 a, b = edges[j] + sh, edges[j + 1] + sh
-                # 4-point Gauss-Legendre on every cubic piece (exact) between the
-                # knots, shifted into the integration range
                 knots = np.concatenate([ph + k for k in range(math.floor(a), math.floor(b) + 1)])
                 cuts = np.unique(np.concatenate([[a, b], knots[(knots > a) & (knots < b)]]))
                 gx, gw = np.polynomial.legendre.leggauss(4)
@@ -351,6 +348,14 @@ def test_poisson_likelihood_given_background_brute_force():
                     xs = 0.5 * (hi - lo) * gx + 0.5 * (hi + lo)
                     tot += 0.5 * (hi - lo) * sum(w * f(x) for x, w in zip(xs, gw))
                 star[i, j] += tot
+    return rng, T, N_b, N_ch, ph, pulses, second, shifts, edges, star
+
+
+def test_poisson_likelihood_given_background_brute_force():
+    # expected counts = exposure x (phase-bin integral of the shifted pulse +
+    # background rate); lnL = sum d log(lambda) - lambda - ln d!
+    from scipy.special import gammaln
+    rng, T, N_b, N_ch, ph, pulses, second, shifts, edges, star = _two_component_case()
     bg = rng.uniform(0.001, 0.004, (N_ch, N_b))
     lam = T * (star + bg)
     counts = rng.poisson(lam).astype(np.float64)
@@ -376,6 +381,68 @@ def test_poisson_likelihood_given_background_brute_force():
     lz2, _ = O.poisson_likelihood_given_background(T, edges, c2, [zero], [ph], shifts[:1],
                                                    np.zeros_like(bg))
     assert -2e90 <= lz2 <= -2e89
+
+
+def test_gaussian_likelihood_given_background_against_normal_logpdf():
+    # EXPEC as the Poisson statistic's (the brute-force phase-bin integrals);
+    # lnL = sum of the normal log-densities of the counts about EXPEC
+    from scipy.stats import norm
+    rng, T, N_b, N_ch, ph, pulses, second, shifts, edges, star = _two_component_case()
+    bg = rng.uniform(0.001, 0.004, (N_ch, N_b))
+    lam = T * (star + bg)
+    counts = lam + rng.normal(0.0, 1.0, lam.shape) * np.sqrt(lam)
+    errors = np.sqrt(lam) * rng.uniform(0.8, 1.2, lam.shape)
+    lnl, EXPEC = O.gaussian_likelihood_given_background(T, edges, counts, errors, [pulses, second], [ph, ph],
+                                                        shifts, bg)
+    np.testing.assert_allclose(EXPEC, lam, rtol=1e-11)
+    ref = float(np.sum(norm.logpdf(counts, loc=lam, scale=errors)))
+    assert lnl == pytest.approx(ref, rel=1e-12, abs=1e-9)
+    # per-component allow_negative: the second component negated integrates
+    # below zero in every bin; it is dropped without its flag, and with it the
+    # sum is clipped at zero before the background is added
+    first = _part(ph, pulses, shifts[0], edges)
+    minus = -_part(ph, second, shifts[1], edges)
+    l_drop, E_drop = O.gaussian_likelihood_given_background(T, edges, counts, errors, [pulses, -second],
+                                                            [ph, ph], shifts, bg, allow_negative=[False, False])
+    np.testing.assert_allclose(E_drop, T * (first + bg), rtol=1e-11)
+    l_keep, E_keep = O.gaussian_likelihood_given_background(T, edges, counts, errors, [pulses, -second],
+                                                            [ph, ph], shifts, bg, allow_negative=[False, True])
+    np.testing.assert_allclose(E_keep, T * (np.maximum(first + minus, 0.0) + bg), rtol=1e-11)
+    assert l_drop != l_keep
+
+
+def _part(ph, comp, sh, edges):
+    """Each channel's phase-bin integrals of one shifted Akima pulse (4-point
+    Gauss-Legendre on every cubic piece, exact)."""
+    N_ch, N_b = comp.shape[0], len(edges) - 1
+    out = np.zeros((N_ch, N_b))
+    gx, gw = np.polynomial.legendre.leggauss(4)
+    for i in range(N_ch):
+        sp = O.akima(ph, comp[i], True)
+        f = lambda x: O.akima_eval(sp, x - math.floor(x))
+        for j in range(N_b):
+            a, b = edges[j] + sh, edges[j + 1] + sh
+            knots = np.concatenate([ph + k for k in range(math.floor(a), math.floor(b) + 1)])
+            cuts = np.unique(np.concatenate([[a, b], knots[(knots > a) & (knots < b)]]))
+            for lo, hi in zip(cuts[:-1], cuts[1:]):
+                xs = 0.5 * (hi - lo) * gx + 0.5 * (hi + lo)
+                out[i, j] += 0.5 * (hi - lo) * sum(w * f(x) for x, w in zip(xs, gw))
+    return out
+
+
+def test_gaussian_likelihood_QnUn_against_normal_logpdf():
+    # one model row for every data row: sum over rows and bins of the normal
+    # log-density of d_ij about star_j with scale errors_ij
+    from scipy.stats import norm
+    rng = np.random.default_rng(5)
+    N_rows, N_b = 3, 16
+    star = rng.uniform(-0.3, 0.3, N_b)
+    counts = star[None, :] + rng.normal(0.0, 0.05, (N_rows, N_b))
+    errors = rng.uniform(0.03, 0.08, (N_rows, N_b))
+    lnl, out = O.gaussian_likelihood_QnUn(np.linspace(0.0, 1.0, N_b + 1), counts, errors, star)
+    ref = float(np.sum(norm.logpdf(counts, loc=star[None, :], scale=errors)))
+    assert lnl == pytest.approx(ref, rel=1e-12)
+    np.testing.assert_array_equal(out, star)
 
 
 def _general(st, hr, params=None, correction=None, N_L=64, phases=None, invariant=False):

@@ -104,6 +104,16 @@ def lib():
     L.xh_poisson_likelihood_given_background_ex.restype = i32
     L.xh_poisson_likelihood_given_background_ex.argtypes = (
         [d, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, C.POINTER(d), vp, vp, sz, vp])
+    L.xh_gaussian_workspace_bytes.restype = sz
+    L.xh_gaussian_workspace_bytes.argtypes = [i32, i32]
+    L.xh_gaussian_likelihood_given_background.restype = i32
+    L.xh_gaussian_likelihood_given_background.argtypes = (
+        [d, i32, vp, i32, vp, vp, i32, vp, i32, vp, vp, vp, i32, C.POINTER(d), vp, vp, sz, vp])
+    L.xh_gaussian_likelihood_given_background_ex.restype = i32
+    L.xh_gaussian_likelihood_given_background_ex.argtypes = (
+        [d, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, C.POINTER(d), vp, vp, sz, vp])
+    L.xh_gaussian_likelihood_QnUn.restype = i32
+    L.xh_gaussian_likelihood_QnUn.argtypes = [i32, i32, vp, vp, vp, C.POINTER(d), vp, sz, vp]
     L.xh_compute_rays_workspace_bytes.restype = sz
     L.xh_compute_rays_workspace_bytes.argtypes = [i32, i32]
     L.xh_compute_rays.restype = i32
@@ -220,7 +230,9 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_energy_integrator", "xh_fold", "xh_precomputation",
             "xh_marginal_workspace_bytes", "xh_eval_marginal_likelihood", "xh_eval_marginal_likelihood_ex",
             "xh_eval_marginal_likelihood_noalloc",
-            "xh_poisson_workspace_bytes", "xh_poisson_likelihood_given_background",
+            "xh_poisson_workspace_bytes", "xh_poisson_likelihood_given_background", "xh_gaussian_workspace_bytes",
+            "xh_gaussian_likelihood_given_background", "xh_gaussian_likelihood_given_background_ex",
+            "xh_gaussian_likelihood_QnUn",
             "xh_poisson_likelihood_given_background_ex",
             "xh_compute_rays_workspace_bytes", "xh_compute_rays", "xh_allocate_cells_workspace_bytes",
             "xh_allocate_cells", "xh_cell_mesh_workspace_bytes", "xh_construct_cell_mesh",

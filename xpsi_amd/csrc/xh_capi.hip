@@ -284,6 +284,7 @@ static int cspline_tab_lds(const MarginalArgs& A, hipStream_t s, size_t* out)
         HIPCHK(hipFuncSetAttribute((const void*)&k_marginal, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
         HIPCHK(hipFuncSetAttribute((const void*)&k_poisson_bg, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
         HIPCHK(hipFuncSetAttribute((const void*)&k_star_rate, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+        HIPCHK(hipFuncSetAttribute((const void*)&k_gauss_bg, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
         attr = true;
     }
     if (*out + 2 * (size_t)A.N_b * 8 > 159 * 1024)
@@ -1105,6 +1106,89 @@ extern "C" int xh_poisson_likelihood_given_background_ex(double T, int N_b, cons
     return poisson_impl(T, N_b, phases, N_ch, counts, n_comp, comps, 0, comp_num_phases, comp_phases, shifts,
                         background, precomp, allow_mask(n_comp, allow_negative), lnl_out, EXPEC, workspace,
                         workspace_bytes, stream);
+}
+
+extern "C" size_t xh_gaussian_workspace_bytes(int N_ch, int N_b) { return (size_t)N_ch * N_b * 8 + 64 + 256; }
+
+static int gaussian_impl(double T, int N_b, const double* phases, int N_ch, const double* counts,
+                         const double* errors, int n_comp, const double* comps, int N_P, const int* comp_np,
+                         const double* comp_phases, const double* shifts, const double* background, unsigned allow,
+                         double* lnl_out, double* EXPEC, void* workspace, size_t workspace_bytes, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (N_ch < 1 || N_b < 1 || n_comp < 1 || n_comp > 32 || (!comp_np && N_P < 1))
+        return fail(XH_EINVAL, "xh_gaussian_likelihood_given_background: bad sizes");
+    if (!background || !errors)
+        return fail(XH_EINVAL, "xh_gaussian_likelihood_given_background: the background and errors are required");
+    if (workspace_bytes < xh_gaussian_workspace_bytes(N_ch, N_b))
+        return fail(XH_EINVAL, "xh_gaussian_likelihood_given_background: workspace too small");
+    double* tm = (double*)workspace;
+    double* d_lnl = tm + (size_t)N_ch * N_b;
+    MarginalArgs A{};
+    A.signals = comps; A.comp_phases = comp_phases; A.comp_np = comp_np; A.shifts = shifts; A.phases = phases;
+    A.counts = counts; A.background = background; A.STAR = EXPEC;
+    A.exposure = T; A.N_ch = N_ch; A.N_b = N_b; A.N_P = N_P; A.n_comp = n_comp; A.allow_negative = allow;
+    A.interp = g_phase_interp;
+    size_t tab = 0;
+    {
+        const int rc = cspline_tab_lds(A, s, &tab);
+        if (rc != XH_OK) return rc;
+    }
+    hipLaunchKernelGGL(k_gauss_bg, dim3(N_ch), dim3(64), tab, s, A, errors, tm);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_gauss_reduce, dim3(1), dim3(64), 0, s, (const double*)tm, (size_t)N_ch * N_b, d_lnl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(lnl_out, d_lnl, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return XH_OK;
+}
+
+extern "C" int xh_gaussian_likelihood_given_background(double T, int N_b, const double* phases, int N_ch,
+                                                       const double* counts, const double* errors, int n_comp,
+                                                       const double* comps, int N_P, const double* comp_phases,
+                                                       const double* shifts, const double* background,
+                                                       int allow_negative, double* lnl_out, double* EXPEC,
+                                                       void* workspace, size_t workspace_bytes, void* stream)
+{
+    return gaussian_impl(T, N_b, phases, N_ch, counts, errors, n_comp, comps, N_P, nullptr, comp_phases, shifts,
+                         background, allow_negative ? ~0u : 0u, lnl_out, EXPEC, workspace, workspace_bytes, stream);
+}
+
+extern "C" int xh_gaussian_likelihood_given_background_ex(double T, int N_b, const double* phases, int N_ch,
+                                                          const double* counts, const double* errors, int n_comp,
+                                                          const double* comps, const int* comp_num_phases,
+                                                          const double* comp_phases, const double* shifts,
+                                                          const double* background, const int* allow_negative,
+                                                          double* lnl_out, double* EXPEC, void* workspace,
+                                                          size_t workspace_bytes, void* stream)
+{
+    if (!comp_num_phases)
+        return fail(XH_EINVAL, "xh_gaussian_likelihood_given_background_ex: phases per component required");
+    return gaussian_impl(T, N_b, phases, N_ch, counts, errors, n_comp, comps, 0, comp_num_phases, comp_phases, shifts,
+                         background, allow_mask(n_comp, allow_negative), lnl_out, EXPEC, workspace, workspace_bytes,
+                         stream);
+}
+
+extern "C" int xh_gaussian_likelihood_QnUn(int N_rows, int N_b, const double* counts, const double* errors,
+                                           const double* star, double* lnl_out, void* workspace,
+                                           size_t workspace_bytes, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (N_rows < 1 || N_b < 1) return fail(XH_EINVAL, "xh_gaussian_likelihood_QnUn: bad sizes");
+    if (!counts || !errors || !star) return fail(XH_EINVAL, "xh_gaussian_likelihood_QnUn: null input");
+    if (workspace_bytes < xh_gaussian_workspace_bytes(N_rows, N_b))
+        return fail(XH_EINVAL, "xh_gaussian_likelihood_QnUn: workspace too small");
+    double* tm = (double*)workspace;
+    double* d_lnl = tm + (size_t)N_rows * N_b;
+    const size_t n = (size_t)N_rows * N_b;
+    hipLaunchKernelGGL(k_gauss_qnun, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N_rows, N_b, counts, errors,
+                       star, tm);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_gauss_reduce, dim3(1), dim3(64), 0, s, (const double*)tm, n, d_lnl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(lnl_out, d_lnl, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return XH_OK;
 }
 
 extern "C" size_t xh_synthesise_workspace_bytes(int N_ch, int N_b) { return (size_t)N_ch * N_b * 8 + 256; }

@@ -199,6 +199,10 @@ __global__ void k_synth_draw(int mode, const double* rate, const double* bg, int
 __global__ void k_over(double* out, const double* x, size_t n, double a);
 __global__ void k_spread(double* out, const double* v, int rows, int cols, int by_row);
 __global__ void k_poisson_reduce(MarginalArgs A, double* lnl);
+__global__ void k_gauss_bg(MarginalArgs A, const double* errors, double* tm);
+__global__ void k_gauss_qnun(int n_rows, int N_b, const double* counts, const double* errors, const double* star,
+                             double* tm);
+__global__ void k_gauss_reduce(const double* tm, size_t n, double* lnl);
 // the table atmosphere contracted over (log T, log g) for every parallel
 __global__ void k_table_slab(const Member* members, Layout L, FoldArgs F, double* slab);
 // closed mesh and time-invariant spectrum of the elsewhere component

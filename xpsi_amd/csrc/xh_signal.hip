@@ -619,6 +619,52 @@ __global__ void __launch_bounds__(64) k_poisson_reduce(MarginalArgs A, double* _
 }
 
 
+// Gaussian likelihood given a background
+// (xpsi/likelihoods/_gaussian_likelihood_given_background_IQU.pyx:14-91):
+// wave per channel; each bin's expected counts (the star's plus the
+// background's count rate, times the exposure, as the Poisson statistic forms
+// them) into STAR, and its term (EXPEC - d)^2 / (2 s^2) + ln(2 pi s^2) / 2,
+// s the bin's error, into tm [N_ch][N_b]
+__global__ void __launch_bounds__(64) k_gauss_bg(MarginalArgs A, const double* __restrict__ errors,
+                                                 double* __restrict__ tm)
+{
+    const int ch = blockIdx.x, lane = threadIdx.x;
+    const int N_b = A.N_b;
+    extern __shared__ double sm[];
+    const double* tabs = channel_tables(A, 0, ch, sm);
+    __syncthreads();
+    for (int j = lane; j < N_b; j += 64) {
+        const size_t q = (size_t)ch * N_b + j;
+        const double e = (bin_rate(A, 0, ch, j, tabs) + A.background[q]) * A.exposure;
+        if (A.STAR) A.STAR[q] = e;
+        const double s2 = errors[q] * errors[q], r = e - A.counts[q];
+        tm[q] = r * r / (2.0 * s2) + 0.5 * log(2.0 * PI * s2);
+    }
+}
+
+// xpsi/likelihoods/_gaussian_likelihood_QnUn.pyx:14-44: the term of data row
+// i and bin j against the one model row star [N_b]
+__global__ void __launch_bounds__(256) k_gauss_qnun(int n_rows, int N_b, const double* __restrict__ counts,
+                                                    const double* __restrict__ errors,
+                                                    const double* __restrict__ star, double* __restrict__ tm)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)n_rows * N_b) return;
+    const double s2 = errors[q] * errors[q], r = star[q % N_b] - counts[q];
+    tm[q] = r * r / (2.0 * s2) + 0.5 * log(2.0 * PI * s2);
+}
+
+// the reference's running sum, LOGLIKE -= term, row by row and bin by bin
+__global__ void __launch_bounds__(64) k_gauss_reduce(const double* __restrict__ tm, size_t n,
+                                                     double* __restrict__ lnl)
+{
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (size_t q = 0; q < n; q++) s -= tm[q];
+    *lnl = s;
+}
+
+
 // ---------------------------------------------------------------- synthesis
 // Poisson realisations of the expected counts
 // (xpsi/tools/synthesise.pyx:40-276).  Random numbers: splitmix64 (Steele,
