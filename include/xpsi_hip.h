@@ -346,6 +346,17 @@ int xh_gaussian_likelihood_QnUn(int N_rows, int N_b, const double* counts, const
                                 const double* star, double* lnl_out, void* workspace, size_t workspace_bytes,
                                 void* stream);
 
+/* Replaces xpsi/tools/phase_integrator.pyx:23-121 `phase_integrator(
+ * exposure_time, phases, signal, signal_phases, phase_shift, allow_negative)`:
+ * each row of signal [N_rows, N_P] (device, on the grid signal_phases [N_P],
+ * N_P >= 2) interpolated with the process-wide phase interpolant and
+ * integrated over the intervals phases [N_b + 1] shifted by phase_shift, a
+ * piece kept where positive or allow_negative, times exposure_time, into out
+ * [N_rows, N_b] (device).  Stream-ordered; allocates nothing. */
+int xh_phase_integrator(double exposure_time, int N_b, const double* phases, int N_rows, const double* signal,
+                        int N_P, const double* signal_phases, double phase_shift, int allow_negative, double* out,
+                        void* stream);
+
 /* Replace xpsi/tools/synthesise.pyx:40-148 `synthesise_exposure(...) ->
  * (EXPEC, SYNTHETIC, SCALE_BACKGROUND)` and :152-276
  * `synthesise_given_total_count_number(...) -> (EXPEC, SYNTHETIC, SCALE_STAR,

@@ -367,3 +367,37 @@ double or_gaussian_likelihood_QnUn(size_t N_rows, size_t N_b, const double *coun
         }
     return LL;
 }
+
+/* xpsi/tools/phase_integrator.pyx:23-121: each row's phase interpolant
+   integrated over the shifted intervals (the wrap exact at b - a == 1, not
+   within 1e-12 as the expected counts take it), a piece kept where positive or
+   allowed, the sum not clipped, times the exposure; out [N_rows, N_b]. */
+int or_phase_integrator(double T_exp, size_t N_b, const double *phases, size_t N_rows,
+                        const double *signal, size_t NP, const double *signal_phases,
+                        double shift, int allow_neg, double *out)
+{
+    if (NP < 2) return 1;
+    double *cb = (double *)malloc(3 * (NP + 1) * sizeof(double));
+    double *mb = (double *)malloc((3 * NP + 4) * sizeof(double));
+    for (size_t i = 0; i < N_rows; i++) {
+        const double *pulse = signal + i * NP;
+        or_phase_init(signal_phases, pulse, NP, cb, cb + NP, cb + 2 * NP, mb);
+        for (size_t j = 0; j < N_b; j++) {
+            double a = phases[j] + shift, b = phases[j + 1] + shift, v = 0.0, w;
+            if (b - a == 1.0) { a = 0.0; b = 1.0; }
+            else { a -= floor(a); b -= floor(b); }
+            if (a < b) {
+                w = or_akima_integ(signal_phases, pulse, NP, cb, cb + NP, cb + 2 * NP, a, b);
+                if (w > 0.0 || allow_neg) v = w;
+            } else {
+                w = or_akima_integ(signal_phases, pulse, NP, cb, cb + NP, cb + 2 * NP, a, 1.0);
+                if (w > 0.0 || allow_neg) v = w;
+                w = or_akima_integ(signal_phases, pulse, NP, cb, cb + NP, cb + 2 * NP, 0.0, b);
+                if (w > 0.0 || allow_neg) v += w;
+            }
+            out[i * N_b + j] = v * T_exp;
+        }
+    }
+    free(cb); free(mb);
+    return 0;
+}

@@ -138,6 +138,8 @@ def lib():
              C.c_void_p, C.POINTER(C.c_double), _dp])
         L.or_gaussian_likelihood_QnUn.restype = d
         L.or_gaussian_likelihood_QnUn.argtypes = [sz, sz, _dp, _dp, _dp]
+        L.or_phase_integrator.restype = i
+        L.or_phase_integrator.argtypes = [d, sz, _dp, sz, _dp, sz, _dp, d, i, _dp]
         _LIB = L
     return _LIB
 
@@ -521,6 +523,19 @@ def gaussian_likelihood_QnUn(phases, counts, errors, star):
     N_rows, N_b = counts.shape
     star = _c(star)
     return float(lib().or_gaussian_likelihood_QnUn(N_rows, N_b, counts, _c(errors), star)), star
+
+
+def phase_integrator(exposure_time, phases, signal, signal_phases, phase_shift, allow_negative=0):
+    """phase_integrator.pyx:23-121: [rows, intervals]."""
+    signal = _c(signal)
+    phases = _c(phases)
+    out = np.zeros((signal.shape[0], len(phases) - 1))
+    rc = lib().or_phase_integrator(float(exposure_time), len(phases) - 1, phases, signal.shape[0], signal,
+                                   signal.shape[1], _c(signal_phases), float(phase_shift), int(bool(allow_negative)),
+                                   out)
+    if rc:
+        raise ValueError("phase_integrator needs two or more signal phases")
+    return out
 
 
 # ---------------------------------------------------------- model pipeline

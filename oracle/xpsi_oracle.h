@@ -218,4 +218,8 @@ int or_gaussian_likelihood_given_background(double exposure_time, size_t N_b,
    one model row star [N_b] */
 double or_gaussian_likelihood_QnUn(size_t N_rows, size_t N_b, const double *counts,
                                    const double *errors, const double *star);
+/* phase_integrator.pyx: out [N_rows, N_b]; returns 1 for fewer than two phases */
+int or_phase_integrator(double T_exp, size_t N_b, const double *phases, size_t N_rows,
+                        const double *signal, size_t NP, const double *signal_phases,
+                        double shift, int allow_neg, double *out);
 #endif

@@ -430,6 +430,21 @@ def _part(ph, comp, sh, edges):
     return out
 
 
+def test_phase_integrator_brute_force():
+    # each row's shifted Akima pulse integrated over the phase intervals
+    # (4-point Gauss-Legendre on every cubic piece, exact), times the
+    # exposure; a negative pulse gives zeros without allow_negative and its
+    # negative integrals with it (no clip, unlike the expected counts)
+    rng, T, N_b, N_ch, ph, pulses, second, shifts, edges, star = _two_component_case()
+    for sh in (0.0, 0.37, -0.81):
+        got = O.phase_integrator(T, edges, pulses, ph, sh)
+        np.testing.assert_allclose(got, T * _part(ph, pulses, sh, edges), rtol=1e-12)
+    neg = -pulses
+    np.testing.assert_array_equal(O.phase_integrator(T, edges, neg, ph, 0.2), 0.0)
+    np.testing.assert_allclose(O.phase_integrator(T, edges, neg, ph, 0.2, allow_negative=True),
+                               -T * _part(ph, pulses, 0.2, edges), rtol=1e-12)
+
+
 def test_gaussian_likelihood_QnUn_against_normal_logpdf():
     # one model row for every data row: sum over rows and bins of the normal
     # log-density of d_ij about star_j with scale errors_ij

@@ -114,6 +114,8 @@ def lib():
         [d, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, C.POINTER(d), vp, vp, sz, vp])
     L.xh_gaussian_likelihood_QnUn.restype = i32
     L.xh_gaussian_likelihood_QnUn.argtypes = [i32, i32, vp, vp, vp, C.POINTER(d), vp, sz, vp]
+    L.xh_phase_integrator.restype = i32
+    L.xh_phase_integrator.argtypes = [d, i32, vp, i32, vp, i32, vp, d, i32, vp, vp]
     L.xh_compute_rays_workspace_bytes.restype = sz
     L.xh_compute_rays_workspace_bytes.argtypes = [i32, i32]
     L.xh_compute_rays.restype = i32
@@ -232,7 +234,7 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_eval_marginal_likelihood_noalloc",
             "xh_poisson_workspace_bytes", "xh_poisson_likelihood_given_background", "xh_gaussian_workspace_bytes",
             "xh_gaussian_likelihood_given_background", "xh_gaussian_likelihood_given_background_ex",
-            "xh_gaussian_likelihood_QnUn",
+            "xh_gaussian_likelihood_QnUn", "xh_phase_integrator",
             "xh_poisson_likelihood_given_background_ex",
             "xh_compute_rays_workspace_bytes", "xh_compute_rays", "xh_allocate_cells_workspace_bytes",
             "xh_allocate_cells", "xh_cell_mesh_workspace_bytes", "xh_construct_cell_mesh",

@@ -285,6 +285,8 @@ static int cspline_tab_lds(const MarginalArgs& A, hipStream_t s, size_t* out)
         HIPCHK(hipFuncSetAttribute((const void*)&k_poisson_bg, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
         HIPCHK(hipFuncSetAttribute((const void*)&k_star_rate, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
         HIPCHK(hipFuncSetAttribute((const void*)&k_gauss_bg, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+        HIPCHK(hipFuncSetAttribute((const void*)&k_phase_integrate, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   159 * 1024));
         attr = true;
     }
     if (*out + 2 * (size_t)A.N_b * 8 > 159 * 1024)
@@ -1188,6 +1190,28 @@ extern "C" int xh_gaussian_likelihood_QnUn(int N_rows, int N_b, const double* co
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(lnl_out, d_lnl, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    return XH_OK;
+}
+
+extern "C" int xh_phase_integrator(double exposure_time, int N_b, const double* phases, int N_rows,
+                                   const double* signal, int N_P, const double* signal_phases, double phase_shift,
+                                   int allow_negative, double* out, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (N_rows < 1 || N_b < 1 || N_P < 2) return fail(XH_EINVAL, "xh_phase_integrator: bad sizes (two or more phases)");
+    if (!phases || !signal || !signal_phases || !out) return fail(XH_EINVAL, "xh_phase_integrator: null input");
+    MarginalArgs A{};
+    A.signals = signal; A.comp_phases = signal_phases; A.phases = phases;
+    A.exposure = exposure_time; A.N_ch = N_rows; A.N_b = N_b; A.N_P = N_P; A.n_comp = 1;
+    A.allow_negative = allow_negative ? 1u : 0u;
+    A.interp = g_phase_interp;
+    size_t tab = 0;
+    {
+        const int rc = cspline_tab_lds(A, s, &tab);
+        if (rc != XH_OK) return rc;
+    }
+    hipLaunchKernelGGL(k_phase_integrate, dim3(N_rows), dim3(64), tab, s, A, phase_shift, out);
+    HIPCHK(hipGetLastError());
     return XH_OK;
 }
 

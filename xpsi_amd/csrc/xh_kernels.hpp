@@ -203,6 +203,7 @@ __global__ void k_gauss_bg(MarginalArgs A, const double* errors, double* tm);
 __global__ void k_gauss_qnun(int n_rows, int N_b, const double* counts, const double* errors, const double* star,
                              double* tm);
 __global__ void k_gauss_reduce(const double* tm, size_t n, double* lnl);
+__global__ void k_phase_integrate(MarginalArgs A, double shift, double* out);
 // the table atmosphere contracted over (log T, log g) for every parallel
 __global__ void k_table_slab(const Member* members, Layout L, FoldArgs F, double* slab);
 // closed mesh and time-invariant spectrum of the elsewhere component

@@ -665,6 +665,41 @@ __global__ void __launch_bounds__(64) k_gauss_reduce(const double* __restrict__ 
 }
 
 
+// xpsi/tools/phase_integrator.pyx:23-121: block per row; each row's phase
+// interpolant integrated over the shifted intervals (the wrap exact at
+// b - a == 1), a piece kept where positive or allowed, the sum not clipped,
+// times the exposure, into out [rows][N_b].  A.signals [rows][N_P] on the grid
+// A.comp_phases, one component.
+__global__ void __launch_bounds__(64) k_phase_integrate(MarginalArgs A, double shift, double* __restrict__ out)
+{
+    const int row = blockIdx.x;
+    extern __shared__ double sm[];
+    const double* tabs = channel_tables(A, 0, row, sm);
+    __syncthreads();
+    const int np = A.N_P;
+    const double* pulse = A.signals + (size_t)row * np;
+    const double* TB = tabs;
+    const double* TC = tabs ? TB + np : nullptr;
+    const double* TD = tabs ? TC + np : nullptr;
+    const bool neg_ok = A.allow_negative & 1u;
+    for (int j = threadIdx.x; j < A.N_b; j += 64) {
+        double a = A.phases[j] + shift, b = A.phases[j + 1] + shift, v = 0.0;
+        if (b - a == 1.0) { a = 0.0; b = 1.0; }
+        else { a -= floor(a); b -= floor(b); }
+        if (a < b) {
+            const double w = phase_integ(A.interp, A.comp_phases, pulse, np, TB, TC, TD, a, b);
+            if (w > 0.0 || neg_ok) v = w;
+        } else {
+            const double w1 = phase_integ(A.interp, A.comp_phases, pulse, np, TB, TC, TD, a, 1.0);
+            if (w1 > 0.0 || neg_ok) v = w1;
+            const double w2 = phase_integ(A.interp, A.comp_phases, pulse, np, TB, TC, TD, 0.0, b);
+            if (w2 > 0.0 || neg_ok) v += w2;
+        }
+        out[(size_t)row * A.N_b + j] = v * A.exposure;
+    }
+}
+
+
 // ---------------------------------------------------------------- synthesis
 // Poisson realisations of the expected counts
 // (xpsi/tools/synthesise.pyx:40-276).  Random numbers: splitmix64 (Steele,
