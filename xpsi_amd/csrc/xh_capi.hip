@@ -850,8 +850,8 @@ static hipError_t launch_mesh_cells(const Member* m, const double* tab, const do
     hipLaunchKernelGGL(k_ring_bounds, g, dim3(256), 0, s, m, tab, tot, table_n, L, theta, r, maxAlpha, cos_gamma,
                        effGrav, ringlim);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mesh_classify, g, dim3(256), 0, s, m, tot, L, (const double*)ringlim, phi, cellArea, blist,
-                       bcount);
+    hipLaunchKernelGGL(k_mesh_classify, dim3((unsigned)((S + 7) / 8), (unsigned)nm), dim3(512), 0, s, m, tot, L,
+                       (const double*)ringlim, phi, cellArea, blist, bcount);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const unsigned nb = (unsigned)std::max(1, std::min(3072, nm * S / 8));   // three waves per SIMD
     hipLaunchKernelGGL(k_mesh_cells, dim3(nb), dim3(256), 0, s, m, tot, L, (const double*)ringlim, cellArea,

@@ -587,20 +587,26 @@ __device__ __forceinline__ double left_edge(double first_left, double dphi, int 
     return left;
 }
 
-// one wavefront per parallel, lanes over its cells; blist entries (member,
-// ring << 16 | column)
-__global__ void __launch_bounds__(256) k_mesh_classify(const Member* __restrict__ members,
+// one wavefront per parallel, lanes over its cells, eight parallels of one
+// member per block; blist entries (member, ring << 16 | column).  The
+// boundary cells of a block's round of 64 columns go onto the list with one
+// atomic for the block (the list's counter is shared by every block, and an
+// atomic per wavefront serialised there).
+constexpr int CLASSIFY_WAVES = 8;
+__global__ void __launch_bounds__(64 * CLASSIFY_WAVES) k_mesh_classify(const Member* __restrict__ members,
                                                        const double* __restrict__ total, Layout L,
                                                        const double* __restrict__ ringlim,
                                                        double* __restrict__ phi, double* __restrict__ cellArea,
                                                        int2* __restrict__ blist, int* __restrict__ bcount)
 {
-    const int mi = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
-    const int i = blockIdx.x * 4 + (tid >> 6);
+    const int mi = blockIdx.y, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int i = blockIdx.x * CLASSIFY_WAVES + w;
     const Member m = members[mi];
-    if (!m.active || m.status) return;
+    if (!m.active || m.status) return;   // the whole block
     const int S = m.S;
-    if (i >= S) return;
+    if (blockIdx.x * CLASSIFY_WAVES >= S) return;   // the whole block
+    const bool ring = i < S;                         // this wavefront's parallel exists
+    __shared__ int cnt[CLASSIFY_WAVES], off[CLASSIFY_WAVES], base_s;
     double lo, hi, colat, sColat, dphi, bphi;
     int inv;
     member_lims(m, lo, hi, colat, sColat, inv, dphi, bphi);
@@ -617,7 +623,7 @@ __global__ void __launch_bounds__(256) k_mesh_classify(const Member* __restrict_
             phi[ro + j] = v;
         }
     }
-    const double* rl = ringlim + (ro + i) * 3;
+    const double* rl = ringlim + (ro + (ring ? i : 0)) * 3;   // a row that exists
     const double lower = rl[0], upper = rl[1], cc = rl[2];
     const double sR = m.superR, sAzi = m.superAzi, cR = m.cedeR;
     double s1a = 0, s1b = 0, s2a = 0, s2b = 0, s3a = 0, s3b = 0, s4a = 0, s4b = 0;
@@ -644,11 +650,12 @@ __global__ void __launch_bounds__(256) k_mesh_classify(const Member* __restrict_
     const double cCc = cos(cc), sCc = sin(cc);
     const double cCo = cos(colat), sCo = sin(colat), cSu = cos(sColat), sSu = sin(sColat);
     const double ccR = cos(cR), csR = cos(sR);
-    double* ca = cellArea + co + (size_t)i * S;
+    double* ca = cellArea + co + (size_t)(ring ? i : 0) * S;
+    // j_max is the member's, so every wavefront of the block takes the same rounds
     for (int j0 = 0; j0 < j_max; j0 += 64) {
         const int j = j0 + lane;
         int kind = 0;
-        if (j < j_max) {
+        if (ring && j < j_max) {
             const double left = left_edge(first_left, dphi, j);
             const double right = left + dphi, mid = left + 0.5 * dphi;
             const double cl = cos(left), cr = cos(right), cm = cos(mid);
@@ -681,14 +688,19 @@ __global__ void __launch_bounds__(256) k_mesh_classify(const Member* __restrict_
             ca[j] = v;
             if (mirror) ca[S - 1 - j] = v;
         }
-        // boundary cells onto the list, one atomic per wavefront
+        // boundary cells onto the list: the block's count in one atomic, each
+        // wavefront's entries after those of the wavefronts before it
         const unsigned long long bm = __ballot(kind == 2);
-        if (bm) {
-            int base = 0;
-            if (lane == 0) base = atomicAdd(bcount, __popcll(bm));
-            base = __shfl(base, 0, 64);
-            if (kind == 2) blist[base + __popcll(bm & ((1ull << lane) - 1ull))] = make_int2(mi, (i << 16) | j);
+        if (lane == 0) cnt[w] = __popcll(bm);
+        __syncthreads();
+        if (tid == 0) {
+            int tot = 0;
+            for (int q = 0; q < CLASSIFY_WAVES; q++) { off[q] = tot; tot += cnt[q]; }
+            base_s = tot ? atomicAdd(bcount, tot) : 0;
         }
+        __syncthreads();
+        if (kind == 2) blist[base_s + off[w] + __popcll(bm & ((1ull << lane) - 1ull))] = make_int2(mi, (i << 16) | j);
+        __syncthreads();   // cnt, off and base_s are rewritten by the next round
     }
 }
 
