@@ -27,7 +27,8 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 FP32_MFMA_PEAK_TFLOPS = 157.3
-KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold", "waves": "k_wave_fold"}
+KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold", "waves": "k_wave_fold",
+                "wavesx": "k_wave_fold_shared"}
 # Bytes per fold launch on the default batched workload (4096 prior draws,
 # precision form) from rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0;
 # factors from a 512 MiB device copy), KiB per launch for the 256-, 128- and

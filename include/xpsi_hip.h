@@ -566,6 +566,14 @@ int xh_set_wave_fold_rings(int rings);
  * Applies to pipelines created afterwards. */
 int xh_set_wave_fold_chain(int intervals);
 
+/* Tuning: the wave fold with one block per parallel whose 64-energy chunks
+ * (two to four of them) share the phase lanes' cell walk: the walk's moments
+ * go through LDS in slabs of 16 intervals, and each wavefront folds its own
+ * energies against them (1 = on where it applies, 0 = off, the default).
+ * Applies to pipelines created afterwards.  No counterpart in the reference
+ * (a launch form of this fold). */
+int xh_set_wave_fold_shared(int on);
+
 /* Replaces xpsi/tools/core.pyx:21-116 set_phase_interpolant /
  * get_phase_interpolant: the process-wide phase interpolant of the
  * integrators' profiles, the energy integral and the likelihoods' phase-bin

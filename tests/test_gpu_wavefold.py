@@ -9,7 +9,10 @@ atomics here).  Bars: folded signals within 2e-6 of FP64 element by element
 |lnL - lnL_FP64| < 1e-3, and the llzero class unchanged.  Cases: one region
 (config 1), two regions with a ceding ring, ragged tiles (40 leaves and 40
 phases, 100 energies: part-filled lanes on both sides), the elsewhere
-correction (no clipping), and a batch of prior draws at mixed resolutions."""
+correction (no clipping), and a batch of prior draws at mixed resolutions;
+then the ragged, elsewhere and batch cases again with one block per parallel
+whose energy chunks share the cell walk (k_wave_fold_shared), and that form
+against the one-wavefront form."""
 import sys
 
 import numpy as np
@@ -103,3 +106,46 @@ def test_wave_fold_batched_sample():
         groups.setdefault(d["num_energies"], []).append(k)
     for ne, ks in sorted(groups.items()):
         _compare(configs.config3_model(ne), [cs[k] for k in ks], "batch of %d draws, %d energies" % (len(ks), ne))
+
+
+def _shared(fn):
+    from xpsi_amd import lib
+    from xpsi_amd._lib import check
+    L = lib()
+    check(L.xh_set_wave_fold_shared(1), "xh_set_wave_fold_shared")
+    try:
+        fn()
+    finally:
+        check(L.xh_set_wave_fold_shared(0), "xh_set_wave_fold_shared")
+
+
+def test_wave_fold_shared_ragged():
+    # 100 energies: two chunks, the second part-filled; 40 phases and leaves
+    _shared(test_wave_fold_ragged)
+
+
+def test_wave_fold_shared_elsewhere_correction():
+    _shared(test_wave_fold_elsewhere_correction)
+
+
+def test_wave_fold_shared_batched_sample():
+    _shared(test_wave_fold_batched_sample)
+
+
+def test_wave_fold_shared_matches_one_wavefront_form():
+    """The two forms sum the same chains in the same order per energy: lnL
+    within 1e-9 of each other on a batch at mixed resolutions."""
+    from xpsi_amd import configs
+    cs = sample(12, max_cells=96, max_e=256)
+    ne = max(d["num_energies"] for d in cs)
+    params = [d for d in cs if d["num_energies"] == ne]
+    model = configs.config3_model(ne)
+    say("shared walk vs one wavefront: %d draws, %d energies" % (len(params), ne))
+    l1, _ = _run(model, params, "mixed", "waves")
+    out = {}
+    _shared(lambda: out.setdefault("lx", _run(model, params, "mixed", "waves")[0]))
+    for a, b in zip(l1, out["lx"]):
+        if a <= 0.1 * LLZERO:
+            assert b <= 0.1 * LLZERO
+            continue
+        assert abs(a - b) < 1.0e-9 * max(1.0, abs(a)), (a, b)

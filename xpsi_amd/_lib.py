@@ -188,6 +188,8 @@ def lib():
     L.xh_set_wave_fold_rings.argtypes = [i32]
     L.xh_set_wave_fold_chain.restype = i32
     L.xh_set_wave_fold_chain.argtypes = [i32]
+    L.xh_set_wave_fold_shared.restype = i32
+    L.xh_set_wave_fold_shared.argtypes = [i32]
     L.xh_set_phase_interpolant.restype = i32
     L.xh_set_phase_interpolant.argtypes = [i32]
     L.xh_get_phase_interpolant.restype = i32
@@ -245,6 +247,6 @@ EXPORTED = ["xh_last_error", "xh_integrate_workspace_bytes", "xh_integrate",
             "xh_pipeline_launch", "xh_pipeline_sync", "xh_pipeline_results",
             "xh_pipeline_fetch", "xh_pipeline_set_timing", "xh_pipeline_stage_ms",
             "xh_pipeline_members_per_request", "xh_pipeline_work",
-            "xh_set_fold_energy_chunk", "xh_set_fold_kernel", "xh_set_moment_shape", "xh_set_fold_rings", "xh_set_wave_fold_rings", "xh_set_wave_fold_chain", "xh_set_phase_interpolant", "xh_get_phase_interpolant", "xh_set_lane_phases", "xh_pipeline_use_graphs", "xh_server_create",
+            "xh_set_fold_energy_chunk", "xh_set_fold_kernel", "xh_set_moment_shape", "xh_set_fold_rings", "xh_set_wave_fold_rings", "xh_set_wave_fold_chain", "xh_set_wave_fold_shared", "xh_set_phase_interpolant", "xh_get_phase_interpolant", "xh_set_lane_phases", "xh_pipeline_use_graphs", "xh_server_create",
             "xh_server_submit", "xh_server_poll", "xh_server_stats", "xh_server_destroy",
             "xh_server_set_policy", "xh_server_footprint", "xh_server_warm", "xh_pipeline_capture"]

@@ -310,6 +310,16 @@ extern "C" int xh_set_wave_fold_chain(int intervals)
 // parallels per wavefront of the wave fold (0 = automatic: 1)
 static int g_wave_rings = 0;
 
+// the wave fold with one block per parallel whose energy chunks share the cell
+// walk (k_wave_fold_shared), where the energies need 2 to 4 chunks
+static int g_wave_shared = 0;
+
+extern "C" int xh_set_wave_fold_shared(int on)
+{
+    g_wave_shared = on ? 1 : 0;
+    return XH_OK;
+}
+
 extern "C" int xh_set_wave_fold_rings(int rings)
 {
     if (rings < 0) return fail(XH_EINVAL, "parallels per wave-fold wavefront must be >= 0");
@@ -1300,6 +1310,7 @@ struct xh_pipeline {
     int fold_rings;              // parallels per fold block (0 = automatic), fixed at creation
     int wave_rings;              // parallels per wave-fold wavefront (0 = automatic)
     int wave_chain;              // intervals per wave-fold FP32 chain (0 = per parallel)
+    int wave_shared;             // 1: k_wave_fold_shared where it applies
     int shape;                   // moment form: block shape T (0 = by batch size)
     int lane_nq;                 // energy-lane fold: phases per wave
     bool fp32;                   // precision form
@@ -1421,6 +1432,7 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
         p->fold_rings = g_fold_rings;
         p->wave_rings = g_wave_rings;
         p->wave_chain = g_wave_chain;
+        p->wave_shared = g_wave_shared;
         while (p->shape > 1 && moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->shape, c->atm_ext) > kMomentLdsMax)
             p->shape >>= 1;
         const bool fits = moment_lds(c->num_leaves, smax, p->fp32 ? 4 : 8, p->shape > 0 ? p->shape : 1, c->atm_ext) <= kMomentLdsMax;
@@ -1462,6 +1474,10 @@ extern "C" xh_pipeline* xh_pipeline_create(const xh_model_config* c)
                 fail(XH_EINVAL, "pipeline: the wave fold's LDS exceeds 64 KiB at this leaf count and mesh bound");
                 delete p; return nullptr;
             }
+            // the shared walk for 2 to 4 energy chunks, where its block fits 64 KiB
+            const int ne = c->N_E;
+            if (p->wave_shared && !(ne > 64 && ne <= 256 && wave_fold_shared_lds(c->num_leaves, smax) <= 64 * 1024))
+                p->wave_shared = 0;
         } else if (p->fold_kind == 3) PHIP(launch_lane(p->fp32, p->lane_nq, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
         else if (p->fold_kind == 2) PHIP(launch_moment(p->fp32, 0, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
         else PHIP(launch_fold(p->EC, dim3(0, 1, 1), 0, 0, nullptr, L0, F0));
@@ -1876,8 +1892,13 @@ static int enqueue(xh_pipeline* p, int n, hipStream_t s)
     }
     if (p->fold_kind == 4) {
         const unsigned chunks = (unsigned)(((N_E + 63) / 64) * ((N_P + 63) / 64));
-        hipLaunchKernelGGL(k_wave_fold, dim3((unsigned)((S + rb - 1) / rb), chunks, nm), dim3(64),
-                           wave_fold_lds(L.N_L, L.S_max), s, p->d_members, L, FA);
+        if (p->wave_shared)
+            hipLaunchKernelGGL(k_wave_fold_shared, dim3((unsigned)S, (unsigned)((N_P + 63) / 64), nm),
+                               dim3(64 * ((N_E + 63) / 64)), wave_fold_shared_lds(L.N_L, L.S_max), s, p->d_members,
+                               L, FA);
+        else
+            hipLaunchKernelGGL(k_wave_fold, dim3((unsigned)((S + rb - 1) / rb), chunks, nm), dim3(64),
+                               wave_fold_lds(L.N_L, L.S_max), s, p->d_members, L, FA);
         HIPCHK(hipGetLastError());
     }
     else if (p->fold_kind == 3)

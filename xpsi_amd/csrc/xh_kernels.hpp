@@ -175,9 +175,19 @@ inline size_t general_fold_lds(int N_L) { return (size_t)14 * N_L * 8; }
 // wave-autonomous moment fold (precision form, blackbody): one wavefront per
 // (ring block, 64 energies x 64 phases, member); flux zeroed beforehand
 __global__ void k_wave_fold(Member* members, Layout L, FoldArgs F);
-inline size_t wave_fold_lds(int N_L, int S_max)
+// the wave fold with a parallel's cell walk shared by its energy chunks:
+// grid (S_max, phase chunks, members), 64 x ceil(N_E / 64) <= 256 threads
+__global__ void k_wave_fold_shared(Member* members, Layout L, FoldArgs F);
+__host__ __device__ inline size_t wave_fold_lds(int N_L, int S_max)
 {
     return (((size_t)(3 * N_L + S_max) * 8 + (size_t)(S_max + 2 * N_L) * 4 + (size_t)N_L) + 15) & ~(size_t)15;
+}
+
+constexpr int WF_SLAB = 16;   // intervals per slab of k_wave_fold_shared's moments
+inline size_t wave_fold_shared_lds(int N_L, int S_max)
+{
+    return wave_fold_lds(N_L, S_max) + (size_t)WF_SLAB * 4 * 64 * 4 + (size_t)WF_SLAB * 8 +
+           (size_t)(WF_SLAB + 1) * 64 * 4;
 }
 
 

@@ -448,4 +448,351 @@ __global__ void __launch_bounds__(64, 4) k_wave_fold(Member* members, Layout L, 
     }
 }
 
+
+// The wave fold with a parallel's cell walk shared by its energy chunks.
+// One block per (parallel, 64-phase chunk, member) of NCH = ceil(N_E / 64)
+// wavefronts; wavefront w folds energies 64 w .. 64 w + 63.  The intervals
+// are taken in slabs of WF_SLAB (16): the block's wavefronts split a slab's intervals,
+// each walking the phase lanes' cells through its share (starting from a
+// binary search of the rotation order) and writing the lanes' four moments
+// per interval to LDS (already in the MFMA B operand's order), the lanes
+// with cells as a mask, and each lane's first cell per interval; after a
+// barrier every wavefront runs the slab's intervals for its own energies
+// exactly as k_wave_fold does (profile, Akima coefficients and class, MFMA
+// chain with the same flushes, sign-changing pieces point by point).  The
+// walk is done once per parallel instead of once per energy chunk.
+__global__ void __launch_bounds__(256, 4) k_wave_fold_shared(Member* members, Layout L, FoldArgs F)
+{
+    const int mi = blockIdx.z, tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nch = blockDim.x >> 6;
+    // the member's status is read once for the whole block: another block of
+    // the same member may set it meanwhile, and the block's wavefronts must
+    // agree on whether to run (they meet at barriers)
+    __shared__ int go_s;
+    if (tid == 0) go_s = members[mi].active && !members[mi].status;
+    __syncthreads();
+    if (!go_s) return;                                       // the whole block
+    const int S = members[mi].S, correct = members[mi].correct;
+    const int i = blockIdx.x;                                // the parallel
+    if (i >= S) return;                                      // the whole block
+    const double else_logT = members[mi].else_logT;
+    const int N_L = L.N_L, N_P = L.N_P, N_E = L.N_E, NI = N_L - 1;
+    const int p0 = blockIdx.y * 64, e0 = w * 64;
+    const size_t ro = (size_t)mi * L.S_max + i;
+    if (!F.valid[ro * L.IO_max]) return;                     // the whole block
+
+    extern __shared__ __align__(16) double wsm[];
+    double2* KN = (double2*)wsm;
+    double* PH = (double*)(KN + N_L);
+    double* CPHI = PH + N_L;
+    float2* KH = (float2*)(CPHI + L.S_max);
+    float* CAREA = (float*)(KH + N_L);
+    unsigned char* LIVE = (unsigned char*)(CAREA + L.S_max);
+    char* tail = (char*)wsm + wave_fold_lds(N_L, L.S_max);
+    float* MOM = (float*)tail;                                            // [slab][power][64 phases]
+    unsigned long long* MASK = (unsigned long long*)(MOM + WF_SLAB * 4 * 64);   // [slab]
+    int* CF = (int*)(MASK + WF_SLAB);                                     // [slab + 1][64]: first cell
+    __shared__ double amax_w[4];
+
+    const int e = e0 + lane;
+    const bool e_ok = e < N_E;
+    const double E = F.energies[min(e, N_E - 1)];
+    const double esc = BB_NORM / (E * KEV);
+    const double inv_kT_else = 1.0 / (K_B_OVER_KEV * pow(10.0, else_logT));
+    const int p = p0 + lane;
+    const bool p_ok = p < N_P;
+    const double ph = p_ok ? F.phases[p] : 0.0;
+
+    v4f acc[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[c][t] = v4f{0.f, 0.f, 0.f, 0.f};
+    double* flux = F.flux + (size_t)mi * N_E * N_P;
+    const size_t cbase = (size_t)mi * L.S_max * L.S_max;
+
+    // the parallel's cells, by the block
+    double amax = 0.0;
+    for (int j = tid; j < S; j += blockDim.x) {
+        const size_t c = cbase + (size_t)i * S + j;
+        const double a = F.cellArea[c];
+        const bool rad = F.radiates ? (F.radiates[c] == 1) : (a > 0.0);
+        CPHI[j] = F.phi[(size_t)mi * L.S_max * (F.phi_row_stride ? L.S_max : 1) + (size_t)i * F.phi_row_stride + j];
+        if (rad && a > 0.0) amax = fmax(amax, a);
+    }
+    amax = wave_max(amax);
+    if (lane == 0) amax_w[w] = amax;
+    __syncthreads();
+    amax = 0.0;
+    for (int q = 0; q < nch; q++) amax = fmax(amax, amax_w[q]);
+    const double a_ref = amax > 0.0 ? amax : 1.0;
+    for (int j = tid; j < S; j += blockDim.x) {
+        const size_t c = cbase + (size_t)i * S + j;
+        const double a = F.cellArea[c];
+        const bool rad = F.radiates ? (F.radiates[c] == 1) : (a > 0.0);
+        CAREA[j] = rad ? (float)(a / a_ref) : __builtin_nanf("");
+    }
+    const double inv_kT = 1.0 / (K_B_OVER_KEV * pow(10.0, F.logT[ro]));
+    bool any = false;
+    int chain_n = 0;
+    auto flush = [&]() {
+        int ln = lane, np = N_P;
+        asm volatile("" : "+v"(ln));
+        asm volatile("" : "+s"(np));
+        if (e0 + 64 <= N_E && p0 + 64 <= np) {
+            const unsigned boff = (unsigned)((4 * (ln >> 4) * np + (ln & 15)) * 8);
+            double* tile = flux + (size_t)e0 * np + p0;
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const double sc = a_ref * __shfl(esc, 16 * c + 4 * (ln >> 4) + r, 64);
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        double* row = tile + (size_t)((16 * c + r) * np + 16 * t);
+                        const float v = acc[c][t][r];
+                        if (v != 0.f) unsafeAtomicAdd((double*)((char*)row + boff), (double)v * sc);
+                    }
+                }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int src = 16 * c + 4 * (ln >> 4) + r, ee = e0 + src;
+                    const double sc = a_ref * __shfl(esc, src, 64);
+                    if (ee >= N_E) continue;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const int pq = p0 + 16 * t + (ln & 15);
+                        const float v = acc[c][t][r];
+                        if (pq < np && v != 0.f) unsafeAtomicAdd(flux + (size_t)ee * np + pq, (double)v * sc);
+                    }
+                }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[c][t] = v4f{0.f, 0.f, 0.f, 0.f};
+        any = false;
+        chain_n = 0;
+    };
+    for (int I = 0; I < L.IO_max; I++) {
+        const size_t vix = ro * L.IO_max + I;
+        if (!F.valid[vix]) break;                            // the whole block
+        __syncthreads();   // the previous image's reads of the geometry are done
+        const double* g = F.geo + vix * 5 * N_L;
+        const bool copy_last = g[5 * N_L - 1] < -1.0;
+        for (int q = tid; q < N_L; q += blockDim.x) {
+            const int mm = (q == N_L - 1 && copy_last) ? 0 : q;
+            const double geom = g[3 * N_L + mm];
+            const bool vis = geom != 0.0 && g[4 * N_L + mm] >= -1.0;
+            KN[q] = make_double2(vis ? geom : 0.0, 1.0 / g[N_L + mm]);
+            PH[q] = g[q];
+            if (q < NI) {
+                const double wd = g[q + 1] - g[q];
+                KH[q] = make_float2((float)wd, (float)(1.0 / wd));
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < NI; q += blockDim.x) {
+            bool any_vis = false;
+            for (int k = q - 2; k <= q + 2; k++) {
+                const int sk = k < 0 ? k + NI : (k >= NI ? k - NI : k);
+                any_vis = any_vis || KN[sk].x != 0.0 || KN[sk + 1].x != 0.0;
+            }
+            LIVE[q] = any_vis ? 1 : 0;
+        }
+        __syncthreads();
+        const double lo = PH[0], hi = PH[N_L - 1];
+        // phase lane: rotation point and wrap offsets, as k_wave_fold
+        int j0 = 0;
+        if (p_ok && S > 0) {
+            const double x0 = wrap_phase(ph + CPHI[0], lo, hi);
+            int a = 1, b = S;
+            while (a < b) {
+                const int mid = (a + b) >> 1;
+                if (wrap_phase(ph + CPHI[mid], lo, hi) < x0) b = mid;
+                else a = mid + 1;
+            }
+            j0 = a == S ? 0 : a;
+        }
+        auto cell = [&](int k) { int j = j0 + k; return j >= S ? j - S : j; };
+        auto turns = [&](int j) {
+            const double s0 = ph + CPHI[j], x = wrap_phase(s0, lo, hi);
+            return x == s0 ? 0 : (x == s0 - TWO_PI ? -1 : (x == s0 + TWO_PI ? 1 : 2));
+        };
+        int tA = 0, tB = 0;
+        if (p_ok && S > 0) { tA = turns(j0); tB = j0 > 0 ? turns(0) : 0; }
+        const double offA = tA == -1 ? -TWO_PI : (tA == 1 ? TWO_PI : 0.0);
+        const double offB = tB == -1 ? -TWO_PI : (tB == 1 ? TWO_PI : 0.0);
+        const bool slow = tA == 2 || tB == 2;
+        auto xof = [&](int j) -> double {
+            const double s0 = ph + CPHI[j];
+            double x = s0 + (j >= j0 ? offA : offB);
+            if (slow) x = wrap_phase(s0, lo, hi);
+            return x;
+        };
+        // the rotation must be one sorted turn within [lo, hi] (checked once, by wavefront 0)
+        if (w == 0) {
+            bool unsorted = false;
+            if (p_ok) {
+                double prev = lo;
+                for (int k = 0; k < S; k++) {
+                    const double x = xof(cell(k));
+                    unsorted = unsorted || x < prev || x > hi;
+                    prev = x;
+                }
+            }
+            if (__ballot(unsorted) && lane == 0) members[mi].status = 2;
+        }
+        const int nlanes = p_ok ? S : 0;   // cells a phase lane walks
+
+        auto knot = [&](double2 kn) -> float {
+            if (kn.x == 0.0 || !e_ok) return 0.f;
+            const double Ep = E * kn.y;
+            const float Ef = (float)Ep;
+            const float Ep3 = Ef * Ef * Ef;
+            float I_E = Ep3 / (expf((float)(Ep * inv_kT)) - 1.0f);
+            if (correct) I_E -= Ep3 / (expf((float)(Ep * inv_kT_else)) - 1.0f);
+            return I_E * (float)kn.x;
+        };
+        const float yl2 = knot(KN[NI - 2]), yl1 = knot(KN[NI - 1]), yl0 = knot(KN[NI]);
+        float ya = knot(KN[0]), yb = knot(KN[1]), yk = knot(KN[2]);
+        float sm2 = (yl1 - yl2) * KH[NI - 2].y, sm1 = (yl0 - yl1) * KH[NI - 1].y;
+        float s0 = (yb - ya) * KH[0].y, s1 = (yk - yb) * KH[1].y;
+        const float s0_first = s0, s1_first = s1;
+        float w_c = 0.f, al_c = 0.f;
+        bool carry = false;
+        double2 kn_next = KN[min(3, NI)];
+
+        for (int c0 = 0; c0 < NI; c0 += WF_SLAB) {
+            const int nsl = min(WF_SLAB, NI - c0);
+            // -- the walk: this wavefront's share of the slab's intervals
+            const int iv_a = c0 + (w * nsl) / nch, iv_b = c0 + ((w + 1) * nsl) / nch;
+            if (iv_a < iv_b) {
+                // first cell (rotation order) at or past the share's first leaf
+                int cur = 0;
+                if (iv_a > 0) {
+                    int a = 0, b = nlanes;
+                    const double lb0 = PH[iv_a];
+                    while (a < b) {
+                        const int mid = (a + b) >> 1;
+                        if (xof(cell(mid)) < lb0) a = mid + 1;
+                        else b = mid;
+                    }
+                    cur = a;
+                }
+                for (int iv = iv_a; iv < iv_b; iv++) {
+                    const double lb = PH[iv], ub = iv < NI - 1 ? PH[iv + 1] : __builtin_inf();
+                    const bool piece_live = LIVE[iv] != 0;
+                    CF[(iv - c0) * 64 + lane] = cur;
+                    float M0 = 0.f, M1 = 0.f, M2 = 0.f, M3 = 0.f;
+                    int cnt = 0;
+                    while (cur < nlanes) {
+                        const int j = cell(cur);
+                        const double x = xof(j);
+                        if (!(x < ub)) break;
+                        const float an = CAREA[j];
+                        if (piece_live && !isnan(an)) {
+                            const float tq = (float)(x - lb);
+                            float at = an;
+                            M0 += at; at *= tq; M1 += at; at *= tq; M2 += at; at *= tq; M3 += at;
+                            cnt++;
+                        }
+                        cur++;
+                    }
+                    float* mo = MOM + (iv - c0) * 256;
+                    mo[lane] = M0; mo[64 + lane] = M1; mo[128 + lane] = M2; mo[192 + lane] = M3;
+                    const unsigned long long nz = __ballot(cnt > 0);
+                    if (lane == 0) MASK[iv - c0] = nz;
+                }
+                if (iv_b == c0 + nsl) CF[nsl * 64 + lane] = cur;
+            }
+            __syncthreads();
+            // -- the slab's intervals for this wavefront's energies
+            for (int iv = c0; iv < c0 + nsl; iv++) {
+                const int k2 = iv + 2;
+                const double2 kn_now = kn_next;
+                if (k2 + 2 <= NI) kn_next = KN[k2 + 2];
+                float s2, ynew = 0.f;
+                if (k2 < NI) {
+                    ynew = knot(kn_now);
+                    s2 = (ynew - yk) * KH[k2].y;
+                } else s2 = k2 == NI ? s0_first : s1_first;
+                const bool piece_live = LIVE[iv] != 0;
+                const unsigned long long nzB = MASK[iv - c0];
+                const float wn = fabsf(s2 - s1) + fabsf(s0 - sm1);
+                if (piece_live && nzB) {
+                    if (!carry) {
+                        w_c = fabsf(s1 - s0) + fabsf(sm1 - sm2);
+                        al_c = w_c != 0.f ? fabsf(sm1 - sm2) / w_c : 0.f;
+                    }
+                    const float h = KH[iv].x, ih = KH[iv].y;
+                    const float ca = ya;
+                    float cb, cc, cd;
+                    const float aln = wn != 0.f ? fabsf(s0 - sm1) / wn : 0.f;
+                    if (w_c == 0.f) { cb = s0; cc = 0.f; cd = 0.f; }
+                    else {
+                        const float tn = wn == 0.f ? s0 : (1.f - aln) * s0 + aln * s1;
+                        cb = (1.f - al_c) * sm1 + al_c * s0;
+                        cc = (3.f * s0 - 2.f * cb - tn) * ih;
+                        cd = (cb + tn - 2.f * s0) * (ih * ih);
+                    }
+                    w_c = wn;
+                    al_c = aln;
+                    carry = true;
+                    const int cls = correct ? 1 : piece_class(ca, cb, cc, cd, h);
+                    float P[4];
+                    const bool on = cls == 1 && e_ok;
+                    P[0] = on ? ca : 0.f; P[1] = on ? cb : 0.f; P[2] = on ? cc : 0.f; P[3] = on ? cd : 0.f;
+                    const unsigned long long nzA =
+                        __ballot(on && (ca != 0.f || cb != 0.f || cc != 0.f || cd != 0.f));
+                    const unsigned long long mixA = __ballot(cls == 2 && e_ok);
+                    if (nzA) {
+                        row_transpose(P);
+                        // B operands of the four phase tiles, stored in their order
+                        const float* mo = MOM + (iv - c0) * 256 + (lane >> 4) * 64 + (lane & 15);
+                        float Mt[4];
+#pragma unroll
+                        for (int t = 0; t < 4; t++) Mt[t] = mo[16 * t];
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+#pragma unroll
+                            for (int t = 0; t < 4; t++)
+                                acc[c][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(P[c], Mt[t], acc[c][t], 0, 0, 0);
+                        any = true;
+                        if (++chain_n == F.chain) flush();
+                    }
+                    if (mixA) {
+                        const int c_first = CF[(iv - c0) * 64 + lane], c_last = CF[(iv - c0 + 1) * 64 + lane];
+                        const double lb = PH[iv];
+                        unsigned long long mb = mixA;
+                        while (mb) {
+                            const int el = __ffsll((long long)mb) - 1;
+                            mb &= mb - 1ull;
+                            const float a = readlane_f(ca, el), b = readlane_f(cb, el), c = readlane_f(cc, el),
+                                        d = readlane_f(cd, el);
+                            const double sc = a_ref * readlane_d(esc, el);
+                            float sum = 0.f;
+                            for (int k = c_first; k < c_last; k++) {
+                                const int j = cell(k);
+                                const float ar = CAREA[j];
+                                if (isnan(ar)) continue;
+                                const float tq = (float)(xof(j) - lb);
+                                const float v = a + tq * (b + tq * (c + tq * d));
+                                if (v > 0.f) sum += ar * v;
+                            }
+                            if (sum != 0.f && p_ok) unsafeAtomicAdd(flux + (size_t)(e0 + el) * N_P + p, (double)sum * sc);
+                        }
+                    }
+                } else carry = false;
+                ya = yb; yb = yk; yk = ynew;
+                sm2 = sm1; sm1 = s0; s0 = s1; s1 = s2;
+            }
+            __syncthreads();   // MOM, MASK and CF are rewritten by the next slab
+        }
+    }
+    if (any) flush();
+}
+
 }  // namespace xh

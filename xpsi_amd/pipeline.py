@@ -278,14 +278,15 @@ def params_from_model(model):
 
 
 def autotune_fold(model, params, stream=None, reps=2,
-                  candidates=(("waves", 2), ("waves", 1), ("moment", 4), ("moment", 2), ("moment", 1),
-                              ("lanes", 8))):
+                  candidates=(("wavesx", 1), ("waves", 2), ("waves", 1), ("moment", 4), ("moment", 2),
+                              ("moment", 1), ("lanes", 8))):
     """Pick the spline-fold kernel and its block shape for this model and batch.
 
     Candidates are (kernel, shape): the moment form with blocks of `shape`
     16-phase tiles x `shape` resident 16-energy chunks, or the energy-lane
     form with `shape` phases per wave, or the wave-autonomous moment fold
-    with `shape` parallels per wave (precision form, blackbody).  For each, a pipeline evaluates
+    with `shape` parallels per wave (precision form, blackbody; "wavesx": one
+    block per parallel whose energy chunks share the cell walk).  For each, a pipeline evaluates
     `params` once untimed and `reps` times with HIP-event stage timing; the
     candidate with the least profile-and-fold time wins.  Its shape is set for
     pipelines created afterwards, and the returned model dict selects its
@@ -297,18 +298,22 @@ def autotune_fold(model, params, stream=None, reps=2,
     L = lib()
 
     def select(kernel, shape):
-        if kernel == "waves":
+        if kernel in ("waves", "wavesx"):
             check(L.xh_set_wave_fold_rings(int(shape)), "xh_set_wave_fold_rings")
+            check(L.xh_set_wave_fold_shared(int(kernel == "wavesx")), "xh_set_wave_fold_shared")
         elif kernel == "moment":
             check(L.xh_set_moment_shape(int(shape)), "xh_set_moment_shape")
         elif kernel == "lanes":
             check(L.xh_set_lane_phases(int(shape)), "xh_set_lane_phases")
 
     times = {}
+    n_e = int(math.ceil(model["num_energies"]))
     for kernel, shape in candidates:
+        if kernel == "wavesx" and not 64 < n_e <= 256:
+            continue                      # the shared walk needs two to four energy chunks
         select(kernel, shape)
         try:
-            pl = LikelihoodPipeline(dict(model, fold_kernel=kernel), max_batch=len(params))
+            pl = LikelihoodPipeline(dict(model, fold_kernel=kernel.rstrip("x")), max_batch=len(params))
         except XHError:
             continue                      # this kernel does not fit the model
         pl.evaluate(params, stream)
@@ -318,11 +323,12 @@ def autotune_fold(model, params, stream=None, reps=2,
         times["%s/%d" % (kernel, shape)] = pl.stage_ms()["profile_fold"] / reps
         pl.close()
     for family in ("waves", "moment", "lanes"):
-        own = [k for k in times if k.startswith(family + "/")]
+        own = [k for k in times if k.split("/")[0].rstrip("x") == family]
         if own:
-            select(family, int(min(own, key=times.get).split("/")[1]))
+            k_best, s_best = min(own, key=times.get).split("/")
+            select(k_best, int(s_best))
     best = min(times, key=times.get)
     kernel, shape = best.split("/")
     select(kernel, int(shape))
     times["chosen"] = best
-    return dict(model, fold_kernel=kernel), times
+    return dict(model, fold_kernel=kernel.rstrip("x")), times
