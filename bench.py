@@ -39,6 +39,7 @@ KERNEL_NAMES = {"moment": "k_moment_fold", "lanes": "k_lane_fold", "waves": "k_w
 # register spills (240 bytes per lane of scratch at four waves per SIMD).
 FOLD_TRAFFIC_KIB = {
     "waves/1": {256: (1692562, 31260897), 128: (834653, 13825472), 64: (527687, 7511708)},
+    "wavesx/1": {256: (122594, 27052533), 128: (111036, 11906776)},
     "moment/2": {256: (17123810, 20011418), 128: (9762315, 9300707), 64: (6796957, 5185836)},
 }
 HBM_PEAK_GBS = 8000.0
@@ -205,11 +206,10 @@ def main():
     # measured bytes per fold launch (mean of the step's launches), for the
     # default workload and shape only
     traffic = None
-    chosen = {t["chosen"] for t in tuned.values()}
-    if args.n == 4096 and args.precision == "mixed" and len(chosen) == 1:
-        tab = FOLD_TRAFFIC_KIB.get(chosen.pop())
-        if tab is not None and sorted(groups) == sorted(tab):
-            traffic = sum(2.0 * f + w for f, w in tab.values()) * 1024.0 / len(tab)
+    if args.n == 4096 and args.precision == "mixed":
+        per = [FOLD_TRAFFIC_KIB.get(tuned[ne]["chosen"], {}).get(ne) for ne in sorted(groups)]
+        if all(v is not None for v in per):
+            traffic = sum(2.0 * f + w for f, w in per) * 1024.0 / len(per)
     achieved_overlapped = flops / (stage["profile_fold"] * 1e-3) / 1e12
     n_eval = len(acc)
     value = n_eval * args.steps / dt
