@@ -310,7 +310,10 @@ def autotune_fold(model, params, stream=None, reps=2,
     n_e = int(math.ceil(model["num_energies"]))
     for kernel, shape in candidates:
         if kernel == "wavesx" and not 64 < n_e <= 256:
-            continue                      # the shared walk needs two to four energy chunks
+            # the shared walk needs two to four energy chunks: the one-wavefront form instead
+            if ("waves", shape) in candidates:
+                continue
+            kernel = "waves"
         select(kernel, shape)
         try:
             pl = LikelihoodPipeline(dict(model, fold_kernel=kernel.rstrip("x")), max_batch=len(params))
